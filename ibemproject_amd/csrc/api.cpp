@@ -1,0 +1,613 @@
+// C-ABI of the MI355X iBEM elastic-3D engine (include/ibem3d.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ibem3d.h"
+#include "ibem3d_kernels.h"
+
+using namespace ibem;
+
+namespace {
+
+std::string g_last_error;
+
+struct TimeEntry {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct ibem3d_ctx {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  int NN = 0, NE = 0, num = 0, nump = 0;
+  long n = 0;
+  double E0 = 0, nu0 = 0, E1 = 0, nu1 = 0;
+  Mat M;
+  std::vector<double> h_bc, h_tbc;
+  std::vector<int> node_col, slot_col;
+  // device inputs
+  double *d_nodes = nullptr, *d_cen = nullptr, *d_rad = nullptr, *d_pts = nullptr;
+  double *d_qp = nullptr, *d_uval = nullptr, *d_tval = nullptr;
+  int *d_conn = nullptr, *d_node_col = nullptr, *d_slot_col = nullptr;
+  // system
+  double *d_A = nullptr, *d_b = nullptr, *d_x = nullptr;
+  // solver
+  double *d_r = nullptr, *d_c = nullptr, *d_work = nullptr, *d_res = nullptr;
+  double* d_Af = nullptr;  // FP64 factor copy
+  float* d_A32 = nullptr;
+  float* d_v32 = nullptr;
+  LuWork W;
+  unsigned phases = 0;
+  bool solved = false;
+  double rel_res = -1.0;
+  int iters = 0;
+  std::string err;
+  std::vector<TimeEntry> times;
+  std::map<std::string, hipEvent_t> open;
+};
+
+namespace {
+
+int fail(ibem3d_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  g_last_error = msg;
+  return code;
+}
+
+#define HIPC(call)                                                                          \
+  do {                                                                                      \
+    hipError_t _e = (call);                                                                 \
+    if (_e != hipSuccess)                                                                   \
+      return fail(ctx, IBEM3D_EHIP, std::string(#call) + ": " + hipGetErrorString(_e));    \
+  } while (0)
+
+void tick(void* p, const char* name, int end) {
+  ibem3d_ctx* c = (ibem3d_ctx*)p;
+  if (!end) {
+    hipEvent_t a;
+    hipEventCreate(&a);
+    hipEventRecord(a, c->st);
+    c->open[name] = a;
+  } else {
+    auto it = c->open.find(name);
+    if (it == c->open.end()) return;
+    hipEvent_t b;
+    hipEventCreate(&b);
+    hipEventRecord(b, c->st);
+    c->times.push_back({name, it->second, b});
+    c->open.erase(it);
+  }
+}
+
+struct Scope {
+  ibem3d_ctx* c;
+  const char* n;
+  Scope(ibem3d_ctx* c_, const char* n_) : c(c_), n(n_) { tick(c, n, 0); }
+  ~Scope() { tick(c, n, 1); }
+};
+
+template <typename T>
+int dalloc(ibem3d_ctx* ctx, T** p, size_t count) {
+  if (count == 0) count = 1;
+  HIPC(hipMalloc((void**)p, count * sizeof(T)));
+  return 0;
+}
+
+template <typename T>
+int upload(ibem3d_ctx* ctx, T** p, const T* h, size_t count) {
+  if (int rc = dalloc(ctx, p, count)) return rc;
+  if (h && count) HIPC(hipMemcpyAsync(*p, h, count * sizeof(T), hipMemcpyHostToDevice, ctx->st));
+  return 0;
+}
+
+// Column map of the BC fold (BEMBuilderElastic3D::ApplyBC, BEMBuilderElastic3D.cpp:1960-2016):
+// displacement-unknown dofs in dof order, then traction-unknown (node, dof) keys in first-encounter
+// order over (element, local dof); later slots with the same key share the column (Find :1894).
+int build_column_map(ibem3d_ctx* ctx, const ibem3d_problem* p) {
+  const int NN = p->NN, NE = p->NE;
+  ctx->node_col.assign(3 * NN, -1);
+  ctx->slot_col.assign(12 * NE, -1);
+  int t = 0;
+  for (int j = 0; j < 3 * NN; ++j) {
+    const double f = p->bc[2 * j];
+    if (f == 1.0) ctx->node_col[j] = t++;
+  }
+  std::vector<int> key(3 * NN, -1);
+  for (int e = 0; e < NE; ++e)
+    for (int i = 0; i < 12; ++i) {
+      const double f = p->tbc[24 * e + i];
+      const int nd = p->conn[4 * e + i / 3] - 1;
+      if (nd < 0 || nd >= NN) return fail(ctx, IBEM3D_EINVAL, "element node id out of range");
+      const int k = 3 * nd + i % 3;
+      if (f == 1.0) {
+        ctx->slot_col[12 * e + i] = -1;
+      } else if (f == 0.0) {
+        if (key[k] < 0) key[k] = t++;
+        ctx->slot_col[12 * e + i] = key[k];
+      } else {
+        ctx->slot_col[12 * e + i] = -2;  // neither branch of the reference: ignored
+      }
+    }
+  if (t != 3 * NN)
+    return fail(ctx, IBEM3D_EINVAL,
+                "boundary conditions: displacement unknowns + traction keys = " + std::to_string(t) +
+                    " != 3 NN = " + std::to_string(3 * NN));
+  return 0;
+}
+
+bool finite_all(const double* p, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(p[i])) return false;
+  return true;
+}
+
+double dnrm2_host(const std::vector<double>& v) {
+  double s = 0.0;
+  for (double x : v) s += x * x;
+  return std::sqrt(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ibem3d_last_error(const ibem3d_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_last_error.c_str();
+}
+
+long ibem3d_size(const ibem3d_ctx* ctx) { return ctx ? ctx->n : 0; }
+
+ibem3d_ctx* ibem3d_create(const ibem3d_problem* p, int device) {
+  if (!p || p->NN <= 0 || p->NE <= 0 || p->num < 0 || p->nump < 0 || !p->nodes || !p->conn ||
+      !p->bc || !p->tbc || (p->num > 0 && (!p->centres || !p->radii)) ||
+      (p->nump > 0 && !p->points)) {
+    g_last_error = "ibem3d_create: invalid problem description";
+    return nullptr;
+  }
+  if (!finite_all(p->nodes, 3 * (size_t)p->NN) || !finite_all(p->bc, 6 * (size_t)p->NN) ||
+      !finite_all(p->tbc, 24 * (size_t)p->NE) ||
+      (p->num && (!finite_all(p->centres, 3 * (size_t)p->num) || !finite_all(p->radii, p->num))) ||
+      (p->nump && !finite_all(p->points, 3 * (size_t)p->nump)) || !std::isfinite(p->E0) ||
+      !std::isfinite(p->nu0) || !std::isfinite(p->E1) || !std::isfinite(p->nu1)) {
+    g_last_error = "ibem3d_create: non-finite input";
+    return nullptr;
+  }
+  for (int h = 0; h < p->num; ++h)
+    if (!(p->radii[h] > 0)) {
+      g_last_error = "ibem3d_create: sphere radius must be positive";
+      return nullptr;
+    }
+  ibem3d_ctx* ctx = new ibem3d_ctx();
+  auto bail = [&](const std::string& m) -> ibem3d_ctx* {
+    g_last_error = m.empty() ? ctx->err : m;
+    ibem3d_destroy(ctx);
+    return nullptr;
+  };
+  ctx->dev = device;
+  if (hipSetDevice(device) != hipSuccess) return bail("ibem3d_create: hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess)
+    return bail("ibem3d_create: stream creation failed");
+  ctx->NN = p->NN;
+  ctx->NE = p->NE;
+  ctx->num = p->num;
+  ctx->nump = p->nump;
+  ctx->n = 3L * p->NN + 60L * p->num;
+  ctx->E0 = p->E0;
+  ctx->nu0 = p->nu0;
+  ctx->E1 = p->E1;
+  ctx->nu1 = p->nu1;
+  ctx->M = make_mat(p->E0, p->nu0, p->E1, p->nu1);
+  ctx->h_bc.assign(p->bc, p->bc + 6 * (size_t)p->NN);
+  ctx->h_tbc.assign(p->tbc, p->tbc + 24 * (size_t)p->NE);
+  if (build_column_map(ctx, p)) return bail("");
+  std::vector<double> uval(3 * p->NN), tval(12 * (size_t)p->NE);
+  for (int j = 0; j < 3 * p->NN; ++j) uval[j] = p->bc[2 * j + 1];
+  for (int e = 0; e < p->NE; ++e)
+    for (int i = 0; i < 12; ++i) tval[12 * e + i] = p->tbc[24 * e + 12 + i];
+  const long n = ctx->n;
+  if (upload(ctx, &ctx->d_nodes, p->nodes, 3 * (size_t)p->NN) ||
+      upload(ctx, &ctx->d_conn, p->conn, 4 * (size_t)p->NE) ||
+      upload(ctx, &ctx->d_cen, p->centres, 3 * (size_t)p->num) ||
+      upload(ctx, &ctx->d_rad, p->radii, (size_t)p->num) ||
+      upload(ctx, &ctx->d_pts, p->points, 3 * (size_t)p->nump) ||
+      upload(ctx, &ctx->d_uval, uval.data(), uval.size()) ||
+      upload(ctx, &ctx->d_tval, tval.data(), tval.size()) ||
+      upload(ctx, &ctx->d_node_col, ctx->node_col.data(), ctx->node_col.size()) ||
+      upload(ctx, &ctx->d_slot_col, ctx->slot_col.data(), ctx->slot_col.size()) ||
+      dalloc(ctx, &ctx->d_qp, (size_t)p->NE * 64 * kQpStride) ||
+      dalloc(ctx, &ctx->d_A, (size_t)n * n) || dalloc(ctx, &ctx->d_b, (size_t)n) ||
+      dalloc(ctx, &ctx->d_x, (size_t)n))
+    return bail("");
+  // the boundary-column block and b are accumulated by the assembly kernels
+  if (hipMemsetAsync(ctx->d_A, 0, sizeof(double) * (size_t)n * 3 * p->NN, ctx->st) != hipSuccess ||
+      hipMemsetAsync(ctx->d_b, 0, sizeof(double) * n, ctx->st) != hipSuccess)
+    return bail("ibem3d_create: memset failed");
+  launch_qpoints(ctx->d_nodes, ctx->d_conn, p->NE, ctx->d_qp, ctx->st);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->st) != hipSuccess)
+    return bail("ibem3d_create: quadrature setup failed");
+  return ctx;
+}
+
+void ibem3d_destroy(ibem3d_ctx* ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->dev);
+  if (ctx->st) hipStreamSynchronize(ctx->st);
+  ibem3d_reset_timings(ctx);
+  void* ptrs[] = {ctx->d_nodes, ctx->d_cen,      ctx->d_rad,      ctx->d_pts,  ctx->d_qp,
+                  ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
+                  ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
+                  ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32,
+                  ctx->W.d_ipiv, ctx->W.d_part,  ctx->W.d_info};
+  for (void* q : ptrs)
+    if (q) hipFree(q);
+  if (ctx->st) hipStreamDestroy(ctx->st);
+  delete ctx;
+}
+
+#define CTX_CHECK()                                                           \
+  do {                                                                        \
+    if (!ctx) {                                                               \
+      g_last_error = "null context";                                          \
+      return IBEM3D_EINVAL;                                                   \
+    }                                                                         \
+    if (hipSetDevice(ctx->dev) != hipSuccess) return fail(ctx, IBEM3D_EHIP, "hipSetDevice"); \
+  } while (0)
+
+static int after_launch(ibem3d_ctx* ctx, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ctx, IBEM3D_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+int ibem3d_add_bem(ibem3d_ctx* ctx) {
+  CTX_CHECK();
+  {
+    Scope s(ctx, "assemble.bem");
+    launch_bem(ctx->d_qp, ctx->d_conn, ctx->d_nodes, ctx->NN, ctx->NE, ctx->n, ctx->E0, ctx->nu0,
+               ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b,
+               ctx->st);
+  }
+  ctx->phases |= 1u;
+  ctx->solved = false;
+  return after_launch(ctx, "k_bem");
+}
+
+int ibem3d_add_field(ibem3d_ctx* ctx) {
+  CTX_CHECK();
+  if (ctx->num > 0) {
+    Scope s(ctx, "assemble.field");
+    launch_field(ctx->d_nodes, ctx->NN, ctx->d_cen, ctx->d_rad, ctx->num, ctx->n, 3L * ctx->NN,
+                 ctx->M, ctx->d_A, ctx->st);
+  }
+  ctx->phases |= 2u;
+  ctx->solved = false;
+  return after_launch(ctx, "k_field");
+}
+
+int ibem3d_add_incl(ibem3d_ctx* ctx, int order) {
+  CTX_CHECK();
+  if (order < -1 || order > 2) return fail(ctx, IBEM3D_EINVAL, "add_incl: order must be -1, 0, 1, 2");
+  const int mask = order < 0 ? 7 : (1 << order);
+  if (ctx->num > 0) {
+    Scope s(ctx, "assemble.incl");
+    launch_incl(ctx->d_cen, ctx->d_rad, ctx->num, ctx->n, 3L * ctx->NN, ctx->M, mask, ctx->d_A,
+                ctx->st);
+  }
+  ctx->phases |= (unsigned)mask << 2;
+  ctx->solved = false;
+  return after_launch(ctx, "k_incl");
+}
+
+int ibem3d_add_bem_rows(ibem3d_ctx* ctx, int order) {
+  CTX_CHECK();
+  if (order < -1 || order > 2) return fail(ctx, IBEM3D_EINVAL, "add_bem_rows: order must be -1, 0, 1, 2");
+  const int mask = order < 0 ? 7 : (1 << order);
+  if (ctx->num > 0) {
+    Scope s(ctx, "assemble.rows");
+    launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, ctx->n, 3L * ctx->NN, ctx->M,
+                mask, ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b,
+                ctx->st);
+  }
+  ctx->phases |= (unsigned)mask << 5;
+  ctx->solved = false;
+  return after_launch(ctx, "k_rows");
+}
+
+int ibem3d_apply_bc(ibem3d_ctx* ctx) {
+  CTX_CHECK();
+  if (ctx->phases != 0xFFu)
+    return fail(ctx, IBEM3D_ESTATE, "apply_bc: every assembly phase must run first (mask " +
+                                        std::to_string(ctx->phases) + ")");
+  HIPC(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int ibem3d_assemble(ibem3d_ctx* ctx) {
+  int rc;
+  if ((rc = ibem3d_add_bem(ctx))) return rc;
+  if ((rc = ibem3d_add_field(ctx))) return rc;
+  for (int o = 0; o < 3; ++o)
+    if ((rc = ibem3d_add_incl(ctx, o))) return rc;
+  for (int o = 0; o < 3; ++o)
+    if ((rc = ibem3d_add_bem_rows(ctx, o))) return rc;
+  return ibem3d_apply_bc(ctx);
+}
+
+int ibem3d_get_system(ibem3d_ctx* ctx, double* A, double* b) {
+  CTX_CHECK();
+  const long n = ctx->n;
+  if (A) HIPC(hipMemcpyAsync(A, ctx->d_A, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->st));
+  if (b) HIPC(hipMemcpyAsync(b, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+  HIPC(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+// ---- solve ------------------------------------------------------------------------------------
+static int ensure_solver_ws(ibem3d_ctx* ctx) {
+  const long n = ctx->n;
+  if (!ctx->W.d_ipiv) {
+    if (int rc = dalloc(ctx, &ctx->W.d_ipiv, (size_t)n)) return rc;
+    if (int rc = dalloc(ctx, &ctx->W.d_part, 2 * 8192)) return rc;
+    if (int rc = dalloc(ctx, &ctx->W.d_info, 1)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_r, (size_t)n)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_c, (size_t)n)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_work, (size_t)n)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_res, (size_t)n)) return rc;
+  }
+  return 0;
+}
+
+__global__ void k_scale_vec(double* y, const double* x, const double* s, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i] * s[i];
+}
+__global__ void k_scale_to_f32(float* y, const double* x, const double* s, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)(x[i] * s[i]);
+}
+__global__ void k_axpy_scaled_f32(double* x, const float* d, const double* s, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += (double)d[i] * s[i];
+}
+
+static dim3 vgrid(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// relative residual ||b - A x|| / ||b|| with the FP64 system matrix
+static int residual(ibem3d_ctx* ctx, double* rel) {
+  const long n = ctx->n;
+  HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
+  {
+    Scope s(ctx, "solve.residual");
+    gemv_f64(n, n, -1.0, ctx->d_A, n, ctx->d_x, 1.0, ctx->d_res, ctx->st);
+  }
+  std::vector<double> r(n), b(n);
+  HIPC(hipMemcpyAsync(r.data(), ctx->d_res, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+  HIPC(hipMemcpyAsync(b.data(), ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+  HIPC(hipStreamSynchronize(ctx->st));
+  const double nb = dnrm2_host(b);
+  *rel = nb > 0 ? dnrm2_host(r) / nb : dnrm2_host(r);
+  return 0;
+}
+
+int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
+  CTX_CHECK();
+  if (ctx->phases != 0xFFu) return fail(ctx, IBEM3D_ESTATE, "solve: system not assembled");
+  if (int rc = ensure_solver_ws(ctx)) return rc;
+  const long n = ctx->n;
+  ctx->iters = 0;
+  ctx->rel_res = -1.0;
+  if (mode == IBEM3D_SOLVE_FP64_LU) {
+    // keep A for the residual; factor an equilibrated copy R A C
+    if (!ctx->d_Af)
+      if (int rc = dalloc(ctx, &ctx->d_Af, (size_t)n * n)) return rc;
+    HIPC(hipMemcpyAsync(ctx->d_Af, ctx->d_A, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->st));
+    {
+      Scope s(ctx, "solve.equilibrate");
+      equilibrate_f64(ctx->d_Af, n, n, ctx->d_r, ctx->d_c, ctx->st);
+    }
+    {
+      Scope s(ctx, "solve.lu_f64");
+      if (lu_factor_f64(ctx->d_Af, n, n, ctx->W, ctx->st, tick, ctx))
+        return fail(ctx, IBEM3D_EHIP, "cooperative panel launch failed");
+    }
+    int info = 0;
+    HIPC(hipMemcpyAsync(&info, ctx->W.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    HIPC(hipStreamSynchronize(ctx->st));
+    if (info) return fail(ctx, IBEM3D_ESOLVE, "LU: zero pivot at column " + std::to_string(info - 1));
+    {
+      Scope s(ctx, "solve.trsv_f64");
+      hipLaunchKernelGGL(k_scale_vec, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_b, ctx->d_r, n);
+      lu_solve_f64(ctx->d_Af, n, n, ctx->W.d_ipiv, ctx->d_x, ctx->st);
+      hipLaunchKernelGGL(k_scale_vec, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_x, ctx->d_c, n);
+    }
+    if (int rc = after_launch(ctx, "fp64 solve")) return rc;
+    if (int rc = residual(ctx, &ctx->rel_res)) return rc;
+    ctx->solved = true;
+    return 0;
+  }
+  if (mode == IBEM3D_SOLVE_MIXED_FP32) {
+    if (!(tol > 0)) tol = 1e-10;
+    if (!ctx->d_A32) {
+      if (int rc = dalloc(ctx, &ctx->d_A32, (size_t)n * n)) return rc;
+      if (int rc = dalloc(ctx, &ctx->d_v32, (size_t)n)) return rc;
+    }
+    // equilibrate a scratch FP64 copy to obtain R, C, then convert R A C to FP32
+    if (!ctx->d_Af) {
+      // equilibration factors from A itself (scan only), scaled copy goes straight to FP32
+      if (int rc = dalloc(ctx, &ctx->d_Af, (size_t)n * n)) return rc;
+    }
+    HIPC(hipMemcpyAsync(ctx->d_Af, ctx->d_A, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->st));
+    {
+      Scope s(ctx, "solve.equilibrate");
+      equilibrate_f64(ctx->d_Af, n, n, ctx->d_r, ctx->d_c, ctx->st);
+      convert_f64_to_f32(ctx->d_Af, n, n, ctx->d_A32, n, ctx->st);
+    }
+    {
+      Scope s(ctx, "solve.lu_f32");
+      if (lu_factor_f32(ctx->d_A32, n, n, ctx->W, ctx->st, tick, ctx))
+        return fail(ctx, IBEM3D_EHIP, "cooperative panel launch failed");
+    }
+    int info = 0;
+    HIPC(hipMemcpyAsync(&info, ctx->W.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    HIPC(hipStreamSynchronize(ctx->st));
+    if (info) return fail(ctx, IBEM3D_ESOLVE, "LU(fp32): zero pivot at column " + std::to_string(info - 1));
+    // iterative refinement in FP64: x <- x + C LU32^{-1} (R r)
+    HIPC(hipMemsetAsync(ctx->d_x, 0, sizeof(double) * n, ctx->st));
+    HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
+    double rel = 1.0;
+    const int max_it = 50;
+    for (int it = 0; it < max_it; ++it) {
+      {
+        Scope s(ctx, "solve.trsv_f32");
+        hipLaunchKernelGGL(k_scale_to_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_v32, ctx->d_res,
+                           ctx->d_r, n);
+        lu_solve_f32(ctx->d_A32, n, n, ctx->W.d_ipiv, ctx->d_v32, ctx->st);
+        hipLaunchKernelGGL(k_axpy_scaled_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_v32,
+                           ctx->d_c, n);
+      }
+      if (int rc = after_launch(ctx, "mixed refinement")) return rc;
+      if (int rc = residual(ctx, &rel)) return rc;
+      ctx->iters = it + 1;
+      ctx->rel_res = rel;
+      if (!(rel == rel)) return fail(ctx, IBEM3D_ESOLVE, "refinement diverged (NaN)");
+      if (rel <= tol) break;
+    }
+    ctx->solved = true;
+    if (rel > tol)
+      return fail(ctx, IBEM3D_ESOLVE, "refinement stopped at relative residual " + std::to_string(rel));
+    return 0;
+  }
+  return fail(ctx, IBEM3D_EINVAL, "solve: unknown mode");
+}
+
+int ibem3d_solve_info(const ibem3d_ctx* ctx, double* rel, int* iters) {
+  if (!ctx) return IBEM3D_EINVAL;
+  if (rel) *rel = ctx->rel_res;
+  if (iters) *iters = ctx->iters;
+  return 0;
+}
+
+// getUTVector (BEMBuilderElastic3D.cpp:2053-2117) in the 78 layout
+int ibem3d_get_ut(ibem3d_ctx* ctx, double* U, double* T, double* X60) {
+  CTX_CHECK();
+  if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "get_ut: no solution");
+  const long n = ctx->n;
+  std::vector<double> x(n);
+  HIPC(hipMemcpyAsync(x.data(), ctx->d_x, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
+  HIPC(hipStreamSynchronize(ctx->st));
+  if (X60) std::memcpy(X60, x.data(), sizeof(double) * n);
+  const int NN = ctx->NN, NE = ctx->NE;
+  if (U) {
+    for (int i = 0; i < 3 * NN; ++i) U[i] = ctx->node_col[i] >= 0 ? x[ctx->node_col[i]] : ctx->h_bc[2 * i + 1];
+    for (int h = 0; h < ctx->num; ++h) {
+      const double* s = x.data() + 3L * NN + 60L * h;
+      double* d = U + 3L * NN + 78L * h;
+      for (int u = 0; u < 24; ++u) d[u] = s[u];
+      for (int t = 0; t < 6; ++t)
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) d[24 + 9 * t + 3 * k + l] = s[24 + 6 * t + pair_t(k, l)];
+    }
+  }
+  if (T) {
+    long m = 0;
+    for (int e = 0; e < NE; ++e)
+      for (int i = 0; i < 12; ++i) {
+        const int c = ctx->slot_col[12 * e + i];
+        if (c == -1)
+          T[m++] = ctx->h_tbc[24 * e + 12 + i];
+        else if (c >= 0)
+          T[m++] = x[c];
+      }
+  }
+  return 0;
+}
+
+int ibem3d_stress(ibem3d_ctx* ctx, double* s) {
+  CTX_CHECK();
+  if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "stress: no solution");
+  if (ctx->nump == 0) return 0;
+  const int NN = ctx->NN, NE = ctx->NE;
+  std::vector<double> U(3L * NN + 78L * ctx->num), T(12L * NE);
+  if (int rc = ibem3d_get_ut(ctx, U.data(), T.data(), nullptr)) return rc;
+  // T in element-slot order for the kernel: slots whose flag is neither 0 nor 1 carry 0
+  std::vector<double> Tslot(12L * NE, 0.0);
+  {
+    long m = 0;
+    for (int e = 0; e < NE; ++e)
+      for (int i = 0; i < 12; ++i)
+        if (ctx->slot_col[12 * e + i] != -2) Tslot[12 * e + i] = T[m++];
+  }
+  double *dU = nullptr, *dT = nullptr, *ds = nullptr;
+  int rc = 0;
+  if ((rc = upload(ctx, &dU, U.data(), 3L * NN)) || (rc = upload(ctx, &dT, Tslot.data(), Tslot.size())) ||
+      (rc = dalloc(ctx, &ds, 6L * ctx->nump))) {
+    hipFree(dU);
+    hipFree(dT);
+    hipFree(ds);
+    return rc;
+  }
+  {
+    Scope sc(ctx, "post.stress");
+    launch_stress(ctx->d_qp, ctx->d_conn, ctx->d_nodes, NN, NE, ctx->d_cen, ctx->d_rad, ctx->num,
+                  ctx->d_pts, ctx->nump, ctx->M, dU, dT, ctx->d_x + 3L * NN, ds, ctx->st);
+  }
+  rc = after_launch(ctx, "k_stress");
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(s, ds, sizeof(double) * 6 * ctx->nump, hipMemcpyDeviceToHost, ctx->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
+    if (e != hipSuccess) rc = fail(ctx, IBEM3D_EHIP, std::string("stress copy: ") + hipGetErrorString(e));
+  }
+  hipFree(dU);
+  hipFree(dT);
+  hipFree(ds);
+  return rc;
+}
+
+int ibem3d_timings(const ibem3d_ctx* ctx, char* names, int cap, double* ms, int max_entries) {
+  if (!ctx) return IBEM3D_EINVAL;
+  hipStreamSynchronize(ctx->st);
+  // aggregate by name, in first-seen order
+  std::vector<std::string> order;
+  std::map<std::string, double> tot;
+  for (const auto& t : ctx->times) {
+    float v = 0;
+    hipEventElapsedTime(&v, t.a, t.b);
+    if (!tot.count(t.name)) order.push_back(t.name);
+    tot[t.name] += v;
+  }
+  int cnt = 0, pos = 0;
+  for (const auto& nm : order) {
+    if (cnt >= max_entries) break;
+    if (names) {
+      if (pos + (int)nm.size() + 1 > cap) break;
+      std::memcpy(names + pos, nm.c_str(), nm.size() + 1);
+      pos += (int)nm.size() + 1;
+    }
+    if (ms) ms[cnt] = tot[nm];
+    ++cnt;
+  }
+  return cnt;
+}
+
+void ibem3d_reset_timings(ibem3d_ctx* ctx) {
+  if (!ctx) return;
+  hipStreamSynchronize(ctx->st);
+  for (auto& t : ctx->times) {
+    hipEventDestroy(t.a);
+    hipEventDestroy(t.b);
+  }
+  ctx->times.clear();
+  for (auto& kv : ctx->open) hipEventDestroy(kv.second);
+  ctx->open.clear();
+}
+
+}  // extern "C"

@@ -1,0 +1,353 @@
+// Assembly kernels of the elastic-3D sphere system, FP64, gfx950.
+//
+// The system is built directly in its BC-folded, column-major, 60-unknown layout (n = 3NN + 60num):
+//   rows    [0, 3NN)           boundary collocation rows (3 per node)
+//   rows    [3NN, n)           inclusion rows (60 per sphere)
+//   columns [0, 3NN)           displacement-unknown node dofs and traction-unknown keys (column map)
+//   columns [3NN, n)           eigenstrain unknowns (60 per sphere)
+// Every kernel owns the rows it writes, so accumulation is deterministic and needs no atomics:
+//   k_incl   inclusion rows x eigenstrain columns  (integratorElastic3D.cpp:341-682)
+//   k_field  boundary rows  x eigenstrain columns  (integratorElastic3D.cpp:135-340)
+//   k_bem    boundary rows  x boundary columns + b (BEMBuilderElastic3D.cpp:1355-1509, ApplyBC)
+//   k_rows   inclusion rows x boundary columns + b (BEMBuilderElastic3D.cpp:1511-1867, ApplyBC)
+#include <hip/hip_runtime.h>
+
+#include "ibem3d_kernels.h"
+#include "ibem3d_math.h"
+
+namespace ibem {
+
+// ---- element quadrature geometry: 64 Gauss points x (g[3], nv[3], N[4], w*jac) -----------------
+__global__ __launch_bounds__(64) void k_qpoints(const double* __restrict__ nodes,
+                                                const int* __restrict__ conn, int NE,
+                                                double* __restrict__ qp) {
+  const int e = blockIdx.x;
+  const int g = threadIdx.x;
+  if (e >= NE) return;
+  double el[12];
+  for (int q = 0; q < 4; ++q) {
+    const int nd = conn[4 * e + q] - 1;
+    for (int c = 0; c < 3; ++c) el[3 * q + c] = nodes[3 * nd + c];
+  }
+  const int a = g / 8, b = g % 8;  // xi index outer, eta inner (BEMBuilderElastic3D.cpp:1247-1252)
+  QPoint Q;
+  qpoint(el, gauss8_x(a), gauss8_x(b), Q);
+  double* o = qp + ((size_t)e * 64 + g) * kQpStride;
+  o[0] = Q.g[0];
+  o[1] = Q.g[1];
+  o[2] = Q.g[2];
+  o[3] = Q.nv[0];
+  o[4] = Q.nv[1];
+  o[5] = Q.nv[2];
+  o[6] = Q.N[0];
+  o[7] = Q.N[1];
+  o[8] = Q.N[2];
+  o[9] = Q.N[3];
+  o[10] = gauss8_w(a) * gauss8_w(b) * Q.jac;
+}
+
+// ---- K1: inclusion-inclusion blocks -----------------------------------------------------------
+// Workgroup = kInclTile consecutive target spheres (rows) x one source sphere h (columns). The eight
+// potential tables of each pair (1320 doubles) are built cooperatively in LDS; then thread t owns
+// row t of the tile (rows of consecutive targets are contiguous) and writes the 60 columns, one
+// coalesced column segment per iteration.
+__global__ __launch_bounds__(256) void k_incl(const double* __restrict__ cen,
+                                              const double* __restrict__ rad, int num, long n,
+                                              long row0, Mat M, int rowmask,
+                                              double* __restrict__ A) {
+  __shared__ double tab[kInclTile][8 * kTabN];
+  const int h = blockIdx.y;
+  const int s0 = blockIdx.x * kInclTile;
+  const double ch0 = cen[3 * h], ch1 = cen[3 * h + 1], ch2 = cen[3 * h + 2];
+  const double a = rad[h];
+  for (int u = threadIdx.x; u < kInclTile * kTabN; u += blockDim.x) {
+    const int ps = u / kTabN, t = u % kTabN;
+    const int s = s0 + ps;
+    if (s >= num) continue;
+    const double x[3] = {cen[3 * s] - ch0, cen[3 * s + 1] - ch1, cen[3 * s + 2] - ch2};
+    RadPoint P;
+    rad_point(P, x);
+    const bool inside = is_inside(x, a);
+    double C[8][4];
+    pot_coefs(a, inside, C);
+    double v[8];
+    pot_entry(P, inside, C, t, v);
+#pragma unroll
+    for (int f = 0; f < 8; ++f) tab[ps][f * kTabN + t] = v[f];
+  }
+  __syncthreads();
+  const int rl = threadIdx.x;
+  if (rl >= kInclTile * 60) return;
+  const int ps = rl / 60, r = rl % 60;
+  const int s = s0 + ps;
+  if (s >= num) return;
+  const int ord = r < 6 ? 0 : (r < 24 ? 1 : 2);
+  if (!((rowmask >> ord) & 1)) return;
+  const long row = row0 + 60L * s + r;
+  double* Acol = A + (row0 + 60L * h) * n + row;
+  const bool self = (s == h);
+  for (int c = 0; c < 60; ++c) Acol[(long)c * n] = incl_entry(tab[ps], kTabN, M, self, r, c);
+}
+
+// ---- K3: boundary rows x eigenstrain columns ----------------------------------------------------
+// Thread = (node j, component i); row 3j+i; workgroup = 64 nodes x one sphere.
+__global__ __launch_bounds__(192) void k_field(const double* __restrict__ nodes, int NN,
+                                               const double* __restrict__ cen,
+                                               const double* __restrict__ rad, long n, long row0,
+                                               Mat M, double* __restrict__ A) {
+  const int K = blockIdx.y;
+  const int j = blockIdx.x * 64 + threadIdx.x / 3;
+  const int i = threadIdx.x % 3;
+  if (j >= NN) return;
+  const double x[3] = {nodes[3 * j] - cen[3 * K], nodes[3 * j + 1] - cen[3 * K + 1],
+                       nodes[3 * j + 2] - cen[3 * K + 2]};
+  const double a = rad[K];
+  RadPoint P;
+  rad_point(P, x);
+  const bool inside = is_inside(x, a);
+  double C[8][4];
+  pot_coefs(a, inside, C);
+  const long row = 3L * j + i;
+  double* Acol = A + (row0 + 60L * K) * n + row;
+  for (int c = 0; c < 60; ++c) Acol[(long)c * n] = field_entry(P, inside, C, M, i, c);
+}
+
+// ---- K4: boundary rows x boundary columns, b, rigid-body diagonal -------------------------------
+// Thread = (collocation node i, component ii) = one boundary row; loops over every element. Per
+// element it integrates the Kelvin U*, T* row against the four shape functions (8 x 8 Gauss, or
+// 2 x 8 x 8 on collapsed triangles when node i belongs to the element, skipping T for node i's own
+// shape function: BEMBuilderElastic3D.cpp:1169-1353) and folds the 12 + 12 values through the
+// column map: H columns of displacement-unknown dofs and -G columns of traction-unknown keys into A,
+// known values into b (ApplyBC :1960-2016). The diagonal block is minus the row sum of H over all
+// node columns (:1478-1501).
+__global__ __launch_bounds__(192) void k_bem(const double* __restrict__ qp,
+                                             const int* __restrict__ conn,
+                                             const double* __restrict__ nodes, int NN, int NE,
+                                             long n, double E0, double nu0,
+                                             const int* __restrict__ node_col,
+                                             const int* __restrict__ slot_col,
+                                             const double* __restrict__ uval,
+                                             const double* __restrict__ tval,
+                                             double* __restrict__ A, double* __restrict__ b) {
+  __shared__ double sq[64 * kQpStride];
+  const int i = blockIdx.x * 64 + threadIdx.x / 3;
+  const int ii = threadIdx.x % 3;
+  const bool active = i < NN;
+  const int ic = active ? i : 0;
+  const double xi0 = nodes[3 * ic], xi1 = nodes[3 * ic + 1], xi2 = nodes[3 * ic + 2];
+  const long row = 3L * i + ii;
+  double rs[3] = {0.0, 0.0, 0.0};
+  double bacc = 0.0;
+  for (int e = 0; e < NE; ++e) {
+    __syncthreads();
+    for (int u = threadIdx.x; u < 64 * kQpStride; u += blockDim.x)
+      sq[u] = qp[(size_t)e * 64 * kQpStride + u];
+    __syncthreads();
+    if (!active) continue;
+    int sc[4];
+    int lpi = -1;
+    for (int q = 0; q < 4; ++q) {
+      sc[q] = conn[4 * e + q];
+      if (sc[q] == i + 1) lpi = q;
+    }
+    double aU[4][3], aT[4][3];
+    for (int q = 0; q < 4; ++q)
+      for (int c = 0; c < 3; ++c) aU[q][c] = aT[q][c] = 0.0;
+    if (lpi < 0) {
+      for (int g = 0; g < 64; ++g) {
+        const double* Q = sq + g * kQpStride;
+        const double d0 = Q[0] - xi0, d1 = Q[1] - xi1, d2 = Q[2] - xi2;
+        const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        const double dxr[3] = {d0 / r, d1 / r, d2 / r};
+        double up[3], tp[3];
+        kelvin_ut_row(ii, dxr, r, E0, nu0, Q + 3, up, tp);
+        const double w = Q[10];
+        for (int q = 0; q < 4; ++q) {
+          const double nw = Q[6 + q] * w;
+          for (int c = 0; c < 3; ++c) {
+            aU[q][c] += up[c] * nw;
+            aT[q][c] += tp[c] * nw;
+          }
+        }
+      }
+    } else {
+      double el[12];
+      for (int q = 0; q < 4; ++q)
+        for (int c = 0; c < 3; ++c) el[3 * q + c] = nodes[3 * (sc[q] - 1) + c];
+      for (int ntr = 0; ntr < 2; ++ntr)
+        for (int ga = 0; ga < 8; ++ga)
+          for (int gb = 0; gb < 8; ++gb) {
+            double xs, es, jb;
+            duffy(ntr, lpi, gauss8_x(ga), gauss8_x(gb), xs, es, jb);
+            QPoint Q;
+            qpoint(el, xs, es, Q);
+            const double d0 = Q.g[0] - xi0, d1 = Q.g[1] - xi1, d2 = Q.g[2] - xi2;
+            const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            const double dxr[3] = {d0 / r, d1 / r, d2 / r};
+            double up[3], tp[3];
+            kelvin_ut_row(ii, dxr, r, E0, nu0, Q.nv, up, tp);
+            const double w = gauss8_w(ga) * gauss8_w(gb) * Q.jac * jb;
+            for (int q = 0; q < 4; ++q) {
+              const double nw = Q.N[q] * w;
+              for (int c = 0; c < 3; ++c) {
+                aU[q][c] += up[c] * nw;
+                if (q != lpi) aT[q][c] += tp[c] * nw;
+              }
+            }
+          }
+    }
+    for (int q = 0; q < 4; ++q) {
+      const int nd = sc[q] - 1;
+      for (int c = 0; c < 3; ++c) {
+        const int dof = 3 * nd + c;
+        const double hv = aT[q][c];
+        rs[c] += hv;
+        const int col = node_col[dof];
+        if (col >= 0)
+          A[(long)col * n + row] += hv;
+        else
+          bacc -= hv * uval[dof];
+        const int slot = 12 * e + 3 * q + c;
+        const double gv = aU[q][c];
+        const int kc = slot_col[slot];
+        if (kc >= 0)
+          A[(long)kc * n + row] -= gv;
+        else if (kc == -1)
+          bacc += gv * tval[slot];
+      }
+    }
+  }
+  if (!active) return;
+  for (int c = 0; c < 3; ++c) {
+    const int dof = 3 * i + c;
+    const double dv = -rs[c];
+    const int col = node_col[dof];
+    if (col >= 0)
+      A[(long)col * n + row] += dv;
+    else
+      bacc -= dv * uval[dof];
+  }
+  b[row] += bacc;
+}
+
+// ---- K2: inclusion rows x boundary columns, b ---------------------------------------------------
+// Workgroup = one sphere s (60 rows) x all elements; thread = (component n, row r) with
+// r = threadIdx % 64 (rows 60..63 idle), n = threadIdx / 64. Kelvin derivative tables d r
+// (|beta| <= 6) and d r^-1 (|beta| <= 4) at x = Gauss point - centre are built in LDS for
+// kRowsGpb Gauss points at a time; each thread accumulates its strain kernel against the four shape
+// functions and folds -fac Dsu (H, node columns) and fac Dst (G, traction slots) through the column
+// map (fac = 0.5 for second-order rows, BEMBuilderElastic3D.cpp:1831, 1842).
+__global__ __launch_bounds__(192) void k_rows(const double* __restrict__ qp,
+                                              const int* __restrict__ conn,
+                                              const double* __restrict__ cen, int NE, long n,
+                                              long row0, Mat M, int rowmask,
+                                              const int* __restrict__ node_col,
+                                              const int* __restrict__ slot_col,
+                                              const double* __restrict__ uval,
+                                              const double* __restrict__ tval,
+                                              double* __restrict__ A, double* __restrict__ b) {
+  __shared__ double sq[64 * kQpStride];
+  __shared__ double sS[kRowsGpb][84];
+  __shared__ double sT[kRowsGpb][35];
+  __shared__ double sb[3][64];
+  const int s = blockIdx.x;
+  const int nc = threadIdx.x / 64;
+  const int r = threadIdx.x % 64;
+  const Slot R = slot60(r < 60 ? r : 59);
+  const bool active = r < 60 && ((rowmask >> R.order) & 1);
+  const int sg = R.order == 1 ? -1 : 1;
+  const double fac = R.order == 2 ? 0.5 : 1.0;
+  const double c0 = cen[3 * s], c1 = cen[3 * s + 1], c2 = cen[3 * s + 2];
+  const long row = row0 + 60L * s + r;
+  double bacc = 0.0;
+  for (int e = 0; e < NE; ++e) {
+    __syncthreads();
+    for (int u = threadIdx.x; u < 64 * kQpStride; u += blockDim.x)
+      sq[u] = qp[(size_t)e * 64 * kQpStride + u];
+    double aH[4] = {0.0, 0.0, 0.0, 0.0}, aG[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int g0 = 0; g0 < 64; g0 += kRowsGpb) {
+      __syncthreads();
+      for (int u = threadIdx.x; u < kRowsGpb * 84; u += blockDim.x) {
+        const int gi = u / 84, t = u % 84;
+        const double* Q = sq + (g0 + gi) * kQpStride;
+        const double x[3] = {Q[0] - c0, Q[1] - c1, Q[2] - c2};
+        RadPoint P;
+        rad_point(P, x);
+        double sv, tv;
+        kelvin_entry(P, t, sv, tv);
+        sS[gi][t] = sv;
+        if (t < 35) sT[gi][t] = tv;
+      }
+      __syncthreads();
+      if (active) {
+        for (int gi = 0; gi < kRowsGpb; ++gi) {
+          const double* Q = sq + (g0 + gi) * kQpStride;
+          double su, st;
+          strain_kernel_n(sS[gi], sT[gi], M, M.dlam, M.dmu, Q + 3, R.i, R.j, R.e, sg, nc, su, st);
+          const double w = Q[10];
+          for (int q = 0; q < 4; ++q) {
+            const double nw = Q[6 + q] * w;
+            aH[q] += su * nw;
+            aG[q] += st * nw;
+          }
+        }
+      }
+    }
+    if (active) {
+      for (int q = 0; q < 4; ++q) {
+        const int nd = conn[4 * e + q] - 1;
+        const int dof = 3 * nd + nc;
+        const double hv = -fac * aH[q];
+        const int col = node_col[dof];
+        if (col >= 0)
+          A[(long)col * n + row] += hv;
+        else
+          bacc -= hv * uval[dof];
+        const int slot = 12 * e + 3 * q + nc;
+        const double gv = fac * aG[q];
+        const int kc = slot_col[slot];
+        if (kc >= 0)
+          A[(long)kc * n + row] -= gv;
+        else if (kc == -1)
+          bacc += gv * tval[slot];
+      }
+    }
+  }
+  sb[nc][r] = active ? bacc : 0.0;
+  __syncthreads();
+  if (nc == 0 && active) b[row] += sb[0][r] + sb[1][r] + sb[2][r];
+}
+
+// ---- launchers ----------------------------------------------------------------------------------
+void launch_qpoints(const double* nodes, const int* conn, int NE, double* qp, hipStream_t st) {
+  hipLaunchKernelGGL(k_qpoints, dim3(NE), dim3(64), 0, st, nodes, conn, NE, qp);
+}
+
+void launch_incl(const double* cen, const double* rad, int num, long n, long row0, const Mat& M,
+                 int rowmask, double* A, hipStream_t st) {
+  dim3 grid((num + kInclTile - 1) / kInclTile, num);
+  hipLaunchKernelGGL(k_incl, grid, dim3(256), 0, st, cen, rad, num, n, row0, M, rowmask, A);
+}
+
+void launch_field(const double* nodes, int NN, const double* cen, const double* rad, int num, long n,
+                  long row0, const Mat& M, double* A, hipStream_t st) {
+  dim3 grid((NN + 63) / 64, num);
+  hipLaunchKernelGGL(k_field, grid, dim3(192), 0, st, nodes, NN, cen, rad, n, row0, M, A);
+}
+
+void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, int NE, long n,
+                double E0, double nu0, const int* node_col, const int* slot_col, const double* uval,
+                const double* tval, double* A, double* b, hipStream_t st) {
+  dim3 grid((NN + 63) / 64);
+  hipLaunchKernelGGL(k_bem, grid, dim3(192), 0, st, qp, conn, nodes, NN, NE, n, E0, nu0, node_col,
+                     slot_col, uval, tval, A, b);
+}
+
+void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long n,
+                 long row0, const Mat& M, int rowmask, const int* node_col, const int* slot_col,
+                 const double* uval, const double* tval, double* A, double* b, hipStream_t st) {
+  hipLaunchKernelGGL(k_rows, dim3(num), dim3(192), 0, st, qp, conn, cen, NE, n, row0, M, rowmask,
+                     node_col, slot_col, uval, tval, A, b);
+}
+
+}  // namespace ibem

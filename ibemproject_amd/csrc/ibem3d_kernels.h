@@ -1,0 +1,65 @@
+// Kernel launchers shared between the C-ABI layer and the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ibem3d_math.h"
+
+namespace ibem {
+
+constexpr int kQpStride = 11;  // per Gauss point: g[3], nv[3], N[4], w*jac
+constexpr int kInclTile = 4;   // target spheres per k_incl workgroup
+constexpr int kRowsGpb = 8;    // Gauss points per table batch in k_rows
+
+// ---- assembly (assembly.hip) ----
+void launch_qpoints(const double* nodes, const int* conn, int NE, double* qp, hipStream_t st);
+void launch_incl(const double* cen, const double* rad, int num, long n, long row0, const Mat& M,
+                 int rowmask, double* A, hipStream_t st);
+void launch_field(const double* nodes, int NN, const double* cen, const double* rad, int num, long n,
+                  long row0, const Mat& M, double* A, hipStream_t st);
+void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, int NE, long n,
+                double E0, double nu0, const int* node_col, const int* slot_col, const double* uval,
+                const double* tval, double* A, double* b, hipStream_t st);
+void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long n,
+                 long row0, const Mat& M, int rowmask, const int* node_col, const int* slot_col,
+                 const double* uval, const double* tval, double* A, double* b, hipStream_t st);
+
+// ---- postprocess (post.hip) ----
+void launch_stress(const double* qp, const int* conn, const double* nodes, int NN, int NE,
+                   const double* cen, const double* rad, int num, const double* pts, int nump,
+                   const Mat& M, const double* U, const double* T, const double* E60, double* s,
+                   hipStream_t st);
+
+// ---- dense solver (solver.hip) ----
+struct Timer;  // optional per-launch timing hook (api.cpp)
+
+// C[M x N] -= A[M x K] * B[K x N], column-major, FP64 MFMA.
+void gemm_f64_sub(long M, long N, long K, const double* A, long lda, const double* B, long ldb,
+                  double* C, long ldc, hipStream_t st);
+// same in FP32 (FP32 MFMA)
+void gemm_f32_sub(long M, long N, long K, const float* A, long lda, const float* B, long ldb,
+                  float* C, long ldc, hipStream_t st);
+
+// Blocked right-looking LU with partial pivoting, in place; ipiv[k] = row swapped with row k
+// (absolute, 0-based). Returns 0, or a positive column index + 1 when an exact zero pivot appears.
+struct LuWork {
+  int* d_ipiv = nullptr;   // n
+  double* d_part = nullptr;  // panel reduction workspace
+  int* d_info = nullptr;
+  int coop_blocks = 0;
+};
+int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, void (*tick)(void*, const char*, int), void* tctx);
+int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, void (*tick)(void*, const char*, int), void* tctx);
+// Solve with the factors: x (in/out, n) <- A^{-1} x
+void lu_solve_f64(const double* LU, long n, long lda, const int* ipiv, double* x, hipStream_t st);
+void lu_solve_f32(const float* LU, long n, long lda, const int* ipiv, float* x, hipStream_t st);
+
+// row / column equilibration: r_i = 1 / max_j |A_ij|, c_j = 1 / max_i |r_i A_ij|; A <- R A C
+void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStream_t st);
+// y = alpha A x + beta y (FP64, column-major)
+void gemv_f64(long m, long n, double alpha, const double* A, long lda, const double* x, double beta,
+              double* y, hipStream_t st);
+// FP64 -> FP32 copy of a column-major matrix
+void convert_f64_to_f32(const double* A, long n, long lda, float* B, long ldb, hipStream_t st);
+
+}  // namespace ibem

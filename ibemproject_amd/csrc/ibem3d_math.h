@@ -1,0 +1,605 @@
+// Closed-form math of the elastic-3D sphere path, shared by the HIP kernels (device) and the host
+// validation build (host). FP64 throughout.
+//
+// Sphere potentials. For a ball B of radius a centred at 0 and x = field point - centre,
+//   Phi^w(x) = int_B w(y) / |x - y| dy,  Psi^w(x) = int_B w(y) |x - y| dy,  w in {1, y_p, y_p y_q},
+// are the functions whose derivatives the reference hand-expands (integratorElastic3DHelper.cpp:41-750:
+// Phi_0k, Phi_1k, Phi_2k, Psi_0k, Psi_1k, Psi_2k). Every one of them is a derivative of a radial
+// function f(rho), rho = |x|^2:
+//   Phi^1 = F0,  Phi^p = d_p F1,  Phi^pq = d_pq F2b + delta_pq F2a,
+//   Psi^1 = G0,  Psi^p = d_p G1,  Psi^pq = d_pq G2b + delta_pq G2a,
+// with, outside the ball (the reference's lambda > 0 branch), V = 4 pi a^3 / 3,
+//   F0 = V r^-1, F1 = -(V a^2/5) r^-1, F2a = (V a^2/5) r^-1, F2b = (4 pi a^7/105) r^-1,
+//   G0 = V r + (V a^2/5) r^-1, G1 = -(V a^2/5) r - (4 pi a^7/105) r^-1,
+//   G2a = (V a^2/5) r + (4 pi a^7/105) r^-1, G2b = (4 pi a^7/105) r + (4 pi a^9/945) r^-1
+// (multipole expansion of |x-y|^{+-1} against the ball's moments), and inside (lambda = 0) by the
+// polynomials
+//   F0 = -2pi/3 rho, F1 = pi a^2/3 rho - pi/10 rho^2, F2a = -F1, F2b = pi a^2/20 rho^2 - pi/84 rho^3,
+//   G0 = -pi/15 rho^2, G1 = pi a^2/30 rho^2 - pi/210 rho^3, G2a = -G1,
+//   G2b = pi a^2/420 rho^3 - pi/3024 rho^4,
+// exact for first and higher derivatives of Phi and third and higher of Psi, which is all the path
+// uses. (Derived from the reference's V_n / Phi_.1 / Psi_.3 formulas, :169-543, with I_n constant
+// inside; the derivation is ibemproject_amd/derive_potentials.py.)
+//
+// Radial derivatives. d^beta rho^s (beta a multi-index, k = |beta|) is
+//   sum_j f^(k-j)(rho) 2^(k-j) M_j(beta, x),  f = rho^s,
+//   M_j = sum_{j0+j1+j2 = j} prod_c beta_c! / (j_c! 2^j_c (beta_c - 2 j_c)!) x_c^(beta_c - 2 j_c),
+// i.e. every way of pairing j index slots into Kronecker deltas. Tables hold d^beta for |beta| <= 8
+// in degree-major order (tidx).
+#pragma once
+
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define IBEM_HD __host__ __device__ inline
+#else
+#define IBEM_HD inline
+#endif
+
+namespace ibem {
+
+constexpr double kPi = 3.141592653589793;  // GlobalConstant.h:26
+constexpr int kTabOrder = 8;
+constexpr int kTabN = 165;  // entries of degree <= 8 in 3 variables
+
+// pair (i,j), i<=j, in the order 00 01 02 11 12 22 (configElastic3D.cpp:312-318)
+IBEM_HD int pair_i(int t) { return t < 3 ? 0 : (t < 5 ? 1 : 2); }
+IBEM_HD int pair_j(int t) { return t < 3 ? t : (t < 5 ? t - 2 : 2); }
+IBEM_HD int pair_t(int i, int j) {
+  if (i > j) { int s = i; i = j; j = s; }
+  return i == 0 ? j : (i == 1 ? 2 + j : 5);
+}
+// weight of the unknown E_mn, m<=n, standing for both E_mn and E_nm (GlobalConstant.h:28)
+IBEM_HD double mapw(int m, int n) { return m == n ? 1.0 : 2.0; }
+
+// packed index of the multi-index (b0,b1,b2), degree-major, b0 then b1 descending
+IBEM_HD int tidx(int b0, int b1, int b2) {
+  int d = b0 + b1 + b2, e = d - b0;
+  return d * (d + 1) * (d + 2) / 6 + e * (e + 1) / 2 + (e - b1);
+}
+IBEM_HD void tdecode(int t, int& b0, int& b1, int& b2) {
+  int d = 0;
+  while ((d + 1) * (d + 2) * (d + 3) / 6 <= t) ++d;
+  int r = t - d * (d + 1) * (d + 2) / 6;
+  int e = 0;
+  while ((e + 1) * (e + 2) / 2 <= r) ++e;
+  int f = r - e * (e + 1) / 2;
+  b0 = d - e;
+  b1 = e - f;
+  b2 = d - b0 - b1;
+}
+
+// beta! / (j! 2^j (beta-2j)!) for beta <= 8, j <= 4
+IBEM_HD double pair_count(int b, int j) {
+  // rows b = 0..8, cols j = 0..4
+  static constexpr double t[9][5] = {{1, 0, 0, 0, 0},    {1, 0, 0, 0, 0},     {1, 1, 0, 0, 0},
+                          {1, 3, 0, 0, 0},    {1, 6, 3, 0, 0},     {1, 10, 15, 0, 0},
+                          {1, 15, 45, 15, 0}, {1, 21, 105, 105, 0}, {1, 28, 210, 420, 105}};
+  return t[b][j];
+}
+
+// Geometry of one evaluation point for the radial tables.
+struct RadPoint {
+  double x[3];
+  double rho;  // |x|^2
+  double rs;   // sqrt(rho)
+};
+
+IBEM_HD void rad_point(RadPoint& P, const double* x) {
+  P.x[0] = x[0];
+  P.x[1] = x[1];
+  P.x[2] = x[2];
+  P.rho = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  P.rs = sqrt(P.rho);
+}
+
+IBEM_HD double ipow(double x, int e) {
+  double r = 1.0;
+  for (int q = 0; q < e; ++q) r *= x;
+  return r;
+}
+
+// Pair sums M_j(beta, x), j = 0..4, returned in five registers (no private arrays: every index
+// below is resolved by selects, so nothing spills to scratch on the GPU).
+struct PairSums {
+  double m0, m1, m2, m3, m4;
+  IBEM_HD double get(int j) const {
+    return j == 0 ? m0 : (j == 1 ? m1 : (j == 2 ? m2 : (j == 3 ? m3 : m4)));
+  }
+};
+
+IBEM_HD PairSums pair_sums(const RadPoint& P, int b0, int b1, int b2) {
+  PairSums S{0.0, 0.0, 0.0, 0.0, 0.0};
+  const double x0s = P.x[0] * P.x[0], x1s = P.x[1] * P.x[1], x2s = P.x[2] * P.x[2];
+  double p0 = ipow(P.x[0], b0 & 1);
+  for (int j0 = b0 / 2; j0 >= 0; --j0, p0 *= x0s) {
+    const double c0 = pair_count(b0, j0) * p0;
+    double p1 = ipow(P.x[1], b1 & 1);
+    for (int j1 = b1 / 2; j1 >= 0; --j1, p1 *= x1s) {
+      const double c1 = c0 * pair_count(b1, j1) * p1;
+      double p2 = ipow(P.x[2], b2 & 1);
+      for (int j2 = b2 / 2; j2 >= 0; --j2, p2 *= x2s) {
+        const double v = c1 * pair_count(b2, j2) * p2;
+        const int j = j0 + j1 + j2;
+        S.m0 += j == 0 ? v : 0.0;
+        S.m1 += j == 1 ? v : 0.0;
+        S.m2 += j == 2 ? v : 0.0;
+        S.m3 += j == 3 ? v : 0.0;
+        S.m4 += j == 4 ? v : 0.0;
+      }
+    }
+  }
+  return S;
+}
+
+// d^beta of r and 1/r (outside) from the pair sums: out0 = d^beta r, out1 = d^beta r^-1
+IBEM_HD void rad_half(const RadPoint& P, int k, const PairSums& M, double& out0, double& out1) {
+  // f = rho^s; f^(m) = s (s-1) ... (s-m+1) rho^(s-m)
+  const double inv2 = 2.0 / P.rho;
+  double sp = 0.0, sm = 0.0;
+  for (int j = 0; j <= k / 2; ++j) {
+    const int m = k - j;
+    double ffp = 1.0, ffm = 1.0, pw = 1.0;
+    for (int q = 0; q < m; ++q) {
+      ffp *= (0.5 - q);
+      ffm *= (-0.5 - q);
+      pw *= inv2;
+    }
+    const double mj = M.get(j);
+    sp += ffp * pw * mj;
+    sm += ffm * pw * mj;
+  }
+  out0 = sp * P.rs;
+  out1 = sm / P.rs;
+}
+
+// d^beta of rho^s (inside, s = 1..4) from the pair sums
+IBEM_HD double rad_poly1(const RadPoint& P, int k, const PairSums& M, int s) {
+  double acc = 0.0;
+  for (int j = 0; j <= k / 2; ++j) {
+    const int m = k - j;
+    if (m > s) continue;
+    double ff = 1.0;
+    for (int q = 0; q < m; ++q) ff *= (double)(s - q);
+    acc += ff * ipow(P.rho, s - m) * (double)(1 << m) * M.get(j);
+  }
+  return acc;
+}
+
+// basis values at multi-index beta: outside (d r, d r^-1, 0, 0), inside (d rho .. d rho^4)
+IBEM_HD void rad_basis(const RadPoint& P, bool inside, int b0, int b1, int b2, double& B0, double& B1,
+                       double& B2, double& B3) {
+  const int k = b0 + b1 + b2;
+  const PairSums M = pair_sums(P, b0, b1, b2);
+  if (inside) {
+    B0 = rad_poly1(P, k, M, 1);
+    B1 = rad_poly1(P, k, M, 2);
+    B2 = rad_poly1(P, k, M, 3);
+    B3 = rad_poly1(P, k, M, 4);
+  } else {
+    rad_half(P, k, M, B0, B1);
+    B2 = B3 = 0.0;
+  }
+}
+
+// Potential table slots
+enum { kF0 = 0, kF1 = 1, kF2a = 2, kF2b = 3, kG0 = 4, kG1 = 5, kG2a = 6, kG2b = 7 };
+
+// Coefficients of the eight radial functions on their basis:
+//   outside: basis (r, r^-1); inside: basis (rho, rho^2, rho^3, rho^4).
+IBEM_HD void pot_coefs(double a, bool inside, double C[8][4]) {
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 4; ++j) C[i][j] = 0.0;
+  const double a2 = a * a;
+  if (!inside) {
+    const double V = 4.0 * kPi * a * a2 / 3.0;
+    const double c2 = V * a2 / 5.0;                            // V a^2/5
+    const double c7 = 4.0 * kPi * a2 * a2 * a2 * a / 105.0;    // 4 pi a^7/105
+    const double c9 = 4.0 * kPi * a2 * a2 * a2 * a2 * a / 945.0;  // 4 pi a^9/945
+    C[kF0][1] = V;
+    C[kF1][1] = -c2;
+    C[kF2a][1] = c2;
+    C[kF2b][1] = c7;
+    C[kG0][0] = V;
+    C[kG0][1] = c2;
+    C[kG1][0] = -c2;
+    C[kG1][1] = -c7;
+    C[kG2a][0] = c2;
+    C[kG2a][1] = c7;
+    C[kG2b][0] = c7;
+    C[kG2b][1] = c9;
+  } else {
+    C[kF0][0] = -2.0 * kPi / 3.0;
+    C[kF1][0] = kPi * a2 / 3.0;
+    C[kF1][1] = -kPi / 10.0;
+    C[kF2a][0] = -kPi * a2 / 3.0;
+    C[kF2a][1] = kPi / 10.0;
+    C[kF2b][1] = kPi * a2 / 20.0;
+    C[kF2b][2] = -kPi / 84.0;
+    C[kG0][1] = -kPi / 15.0;
+    C[kG1][1] = kPi * a2 / 30.0;
+    C[kG1][2] = -kPi / 210.0;
+    C[kG2a][1] = -kPi * a2 / 30.0;
+    C[kG2a][2] = kPi / 210.0;
+    C[kG2b][2] = kPi * a2 / 420.0;
+    C[kG2b][3] = -kPi / 3024.0;
+  }
+}
+
+// The reference's inside test: lambda = max(|x|^2 - a^2, 0) == 0 (integratorElastic3DHelper.cpp:136)
+IBEM_HD bool is_inside(const double* x, double a) {
+  return !((x[0] * x[0] + x[1] * x[1] + x[2] * x[2] - a * a) > 0);
+}
+
+// All eight potential tables at table index t (one entry of each table).
+IBEM_HD void pot_entry(const RadPoint& P, bool inside, const double C[8][4], int t, double* out8) {
+  int b0, b1, b2;
+  tdecode(t, b0, b1, b2);
+  double B0, B1, B2, B3;
+  rad_basis(P, inside, b0, b1, b2, B0, B1, B2, B3);
+  for (int f = 0; f < 8; ++f) out8[f] = C[f][0] * B0 + C[f][1] * B1 + C[f][2] * B2 + C[f][3] * B3;
+}
+
+// Derivative of Phi^w (psi = 0) or Psi^w (psi = 1) with multi-index (c0,c1,c2) from the tables
+// laid out as tab[f * stride + tidx]. w: 0 -> 1, 1+p -> y_p, 4+pair_t(p,q) -> y_p y_q.
+IBEM_HD double pot(const double* tab, int stride, int psi, int w, int c0, int c1, int c2) {
+  const double* T = tab + (psi ? 4 : 0) * stride;
+  if (w == 0) return T[tidx(c0, c1, c2)];
+  if (w < 4) {
+    const int p = w - 1;
+    return T[stride + tidx(c0 + (p == 0), c1 + (p == 1), c2 + (p == 2))];
+  }
+  const int p = pair_i(w - 4), q = pair_j(w - 4);
+  double v = T[3 * stride + tidx(c0 + (p == 0) + (q == 0), c1 + (p == 1) + (q == 1),
+                                 c2 + (p == 2) + (q == 2))];
+  if (p == q) v += T[2 * stride + tidx(c0, c1, c2)];
+  return v;
+}
+
+// Material constants (configElastic3D.cpp:54, :119; integratorElastic3DHelper.cpp:2376-2389)
+struct Mat {
+  double mu0, nu0, lam0, mu1, nu1, lam1;
+  double dlam, dmu;  // lam0 - lam1, mu0 - mu1
+  double cD;         // 1 / (8 pi (1 - nu0))
+  double cg1, cg2;   // 1/(4 pi mu0), 1/(16 pi mu0 (1 - nu0))
+};
+
+IBEM_HD Mat make_mat(double E0, double nu0, double E1, double nu1) {
+  Mat m;
+  m.mu0 = E0 / (2.0 * (1.0 + nu0));
+  m.nu0 = nu0;
+  m.lam0 = 2.0 * m.mu0 * nu0 / (1.0 - 2.0 * nu0);
+  m.mu1 = E1 / (2.0 * (1.0 + nu1));
+  m.nu1 = nu1;
+  m.lam1 = 2.0 * m.mu1 * nu1 / (1.0 - 2.0 * nu1);
+  m.dlam = m.lam0 - m.lam1;
+  m.dmu = m.mu0 - m.mu1;
+  m.cD = 1.0 / (8.0 * kPi * (1.0 - nu0));
+  m.cg1 = 1.0 / (4.0 * kPi * m.mu0);
+  m.cg2 = 1.0 / (16.0 * kPi * m.mu0 * (1.0 - nu0));
+  return m;
+}
+
+IBEM_HD double C0(const Mat& M, int i, int j, int k, int l) {
+  return M.lam0 * (i == j) * (k == l) + M.mu0 * ((i == k) * (j == l) + (i == l) * (j == k));
+}
+
+// Eshelby-type tensor D(k,l | m,n | w | extra) of the reference (D_40..D_62,
+// integratorElastic3DHelper.cpp:2394-2554) with strain indices (k,l), eigenstrain (m,n), weight w
+// and extra derivative counts e[3]:
+//   cD [ Psi^w,mnkl - 2 nu0 d_mn Phi^w,kl - (1-nu0)(d_kn Phi^w,ml + d_ln Phi^w,mk
+//        + d_km Phi^w,nl + d_lm Phi^w,nk) ]      (every derivative also carries e)
+IBEM_HD double Dten(const double* tab, int stride, const Mat& M, int w, int k, int l, int m, int n,
+                    const int* e) {
+  int c[3] = {e[0], e[1], e[2]};
+  c[m]++;
+  c[n]++;
+  c[k]++;
+  c[l]++;
+  double v = pot(tab, stride, 1, w, c[0], c[1], c[2]);
+  if (m == n) {
+    int d[3] = {e[0], e[1], e[2]};
+    d[k]++;
+    d[l]++;
+    v -= 2.0 * M.nu0 * pot(tab, stride, 0, w, d[0], d[1], d[2]);
+  }
+  double s = 0.0;
+  if (k == n) { int d[3] = {e[0], e[1], e[2]}; d[m]++; d[l]++; s += pot(tab, stride, 0, w, d[0], d[1], d[2]); }
+  if (l == n) { int d[3] = {e[0], e[1], e[2]}; d[m]++; d[k]++; s += pot(tab, stride, 0, w, d[0], d[1], d[2]); }
+  if (k == m) { int d[3] = {e[0], e[1], e[2]}; d[n]++; d[l]++; s += pot(tab, stride, 0, w, d[0], d[1], d[2]); }
+  if (l == m) { int d[3] = {e[0], e[1], e[2]}; d[n]++; d[k]++; s += pot(tab, stride, 0, w, d[0], d[1], d[2]); }
+  v -= (1.0 - M.nu0) * s;
+  return M.cD * v;
+}
+
+// sum_k D(k,k | m,n | w | e)
+IBEM_HD double Dtrace(const double* tab, int stride, const Mat& M, int w, int m, int n, const int* e) {
+  double v = 0.0;
+  for (int k = 0; k < 3; ++k) v += Dten(tab, stride, M, w, k, k, m, n, e);
+  return v;
+}
+
+// Row / column decode of the 60-unknown sphere block:
+//   0..5   E_ij            (t = pair)
+//   6..23  E_ij,k          6 + 3 t + k
+//   24..59 E_ij,kl, k<=l   24 + 6 t + pair(k,l)
+struct Slot {
+  int order;  // 0, 1, 2
+  int i, j;   // strain pair
+  int e[3];   // extra derivative counts (rows) / weight indices (columns)
+  int p, q;   // extra indices (p only for order 1)
+  int w;      // weight id for columns
+};
+IBEM_HD Slot slot60(int r) {
+  Slot s;
+  s.e[0] = s.e[1] = s.e[2] = 0;
+  s.p = s.q = -1;
+  if (r < 6) {
+    s.order = 0;
+    s.i = pair_i(r);
+    s.j = pair_j(r);
+    s.w = 0;
+  } else if (r < 24) {
+    s.order = 1;
+    const int t = (r - 6) / 3;
+    s.i = pair_i(t);
+    s.j = pair_j(t);
+    s.p = (r - 6) % 3;
+    s.e[s.p]++;
+    s.w = 1 + s.p;
+  } else {
+    s.order = 2;
+    const int t = (r - 24) / 6, kl = (r - 24) % 6;
+    s.i = pair_i(t);
+    s.j = pair_j(t);
+    s.p = pair_i(kl);
+    s.q = pair_j(kl);
+    s.e[s.p]++;
+    s.e[s.q]++;
+    s.w = 4 + kl;
+  }
+  return s;
+}
+
+// One entry of the inclusion-inclusion block (target sphere s rows, source sphere h columns) in
+// the 60-unknown layout. The reference builds it in three passes (integratorElastic3D.cpp:341-682):
+//   row order 0: A = map_mn ( -sum_kl (C0-C1)_ijkl D_40/50/60 + d_sh C0_ijmn [col order 0] )
+//   row order 1: A = map_mn ( -sum_kl (C0-C1)_ijkl D_41/51/61 + d_sh C0_ijmn d_pr [col order 1] )
+//   row order 2: A = map_mn ( -0.5 sum_kl (C0-C1)_ijkl D_42/52/62 + d_sh C0_ijmn d_pr d_qz )
+// Merging the 78-layout columns (p,q) and (q,p) of a quadratic unknown doubles the D part for
+// p < q; keeping only rows r <= z of the symmetric second-order rows leaves the identity intact.
+IBEM_HD double incl_entry(const double* tab, int stride, const Mat& M, bool self, int r, int c) {
+  const Slot R = slot60(r);
+  const Slot Cc = slot60(c);
+  const int m = Cc.i, n = Cc.j, i = R.i, j = R.j;
+  const double mp = mapw(m, n);
+  double sum = 2.0 * M.dmu * Dten(tab, stride, M, Cc.w, i, j, m, n, R.e);
+  if (i == j) sum += M.dlam * Dtrace(tab, stride, M, Cc.w, m, n, R.e);
+  const double fac = (R.order == 2 ? 0.5 : 1.0) * ((Cc.order == 2 && Cc.p != Cc.q) ? 2.0 : 1.0);
+  double v = -fac * sum;
+  if (self && R.order == Cc.order && R.p == Cc.p && R.q == Cc.q) v += C0(M, i, j, m, n);
+  return mp * v;
+}
+
+// ---- node x sphere columns (addFieldToBEM, integratorElastic3D.cpp:135-340) -------------------
+// H(3 node + i, sphere column) = map sum_{jj,ll} C0(jj,ll,m,n) IGreen_w(i,jj,ll) (x 2 for merged
+// quadratic columns), IGreen_w(i,j,l) = cg1 (d_ij Phi^w,l + d_il Phi^w,j)/2 - cg2 Psi^w,ijl
+// (integratorElastic3DHelper.cpp:2604-2624). Potentials evaluated directly (no table).
+IBEM_HD double pot_direct(const RadPoint& P, bool inside, const double C[8][4], int psi, int w,
+                          int c0, int c1, int c2) {
+  auto radial = [&](int f, int b0, int b1, int b2) {
+    double B0, B1, B2, B3;
+    rad_basis(P, inside, b0, b1, b2, B0, B1, B2, B3);
+    double r = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q == f) r = C[q][0] * B0 + C[q][1] * B1 + C[q][2] * B2 + C[q][3] * B3;
+    return r;
+  };
+  const int base = psi ? 4 : 0;
+  if (w == 0) return radial(base + 0, c0, c1, c2);
+  if (w < 4) {
+    const int p = w - 1;
+    return radial(base + 1, c0 + (p == 0), c1 + (p == 1), c2 + (p == 2));
+  }
+  const int p = pair_i(w - 4), q = pair_j(w - 4);
+  double v = radial(base + 3, c0 + (p == 0) + (q == 0), c1 + (p == 1) + (q == 1),
+                    c2 + (p == 2) + (q == 2));
+  if (p == q) v += radial(base + 2, c0, c1, c2);
+  return v;
+}
+
+IBEM_HD double field_entry(const RadPoint& P, bool inside, const double C[8][4], const Mat& M,
+                           int i, int col) {
+  const Slot Cc = slot60(col);
+  const int m = Cc.i, n = Cc.j, w = Cc.w;
+  auto phi1 = [&](int a) {
+    int c[3] = {0, 0, 0};
+    c[a]++;
+    return pot_direct(P, inside, C, 0, w, c[0], c[1], c[2]);
+  };
+  auto psi3 = [&](int a, int b, int d) {
+    int c[3] = {0, 0, 0};
+    c[a]++;
+    c[b]++;
+    c[d]++;
+    return pot_direct(P, inside, C, 1, w, c[0], c[1], c[2]);
+  };
+  // sum_{jj,ll} C0(jj,ll,m,n) IG(i,jj,ll) = lam0 d_mn sum_k IG(i,k,k) + mu0 (IG(i,m,n) + IG(i,n,m))
+  double v = 0.0;
+  if (m == n) {
+    double tr = M.cg1 * phi1(i);
+    for (int k = 0; k < 3; ++k) tr -= M.cg2 * psi3(i, k, k);
+    v += M.lam0 * tr;
+  }
+  double s = -2.0 * M.cg2 * psi3(i, m, n);
+  if (i == m) s += M.cg1 * phi1(n);
+  if (i == n) s += M.cg1 * phi1(m);
+  v += M.mu0 * s;
+  const double fac = (Cc.order == 2 && Cc.p != Cc.q) ? 2.0 : 1.0;
+  return mapw(m, n) * fac * v;
+}
+
+// ---- boundary element geometry (BEMElasticHelper3D.cpp) ---------------------------------------
+// 8-point Gauss-Legendre table as printed in BEMBuilderElastic3D.cpp:1203-1222
+IBEM_HD double gauss8_x(int a) {
+  static constexpr double g[8] = {0.960289856497536, -0.960289856497536, 0.796666477413627, -0.796666477413627,
+                       0.525532409916329, -0.525532409916329, 0.183434642495650, -0.183434642495650};
+  return g[a];
+}
+IBEM_HD double gauss8_w(int a) {
+  static constexpr double w[8] = {0.101228536290376, 0.101228536290376, 0.222381034453374, 0.222381034453374,
+                       0.313706645877887, 0.313706645877887, 0.362683783378362, 0.362683783378362};
+  return w[a];
+}
+
+// Geometry of one quadrature point of a 4-node bilinear element: position, unit normal
+// (d/dxi x d/deta), shape values, Jacobian (Serendip_func :191, Normal_Jac :218, Cartesian :264).
+struct QPoint {
+  double g[3], nv[3], N[4], jac;
+};
+IBEM_HD void qpoint(const double* el, double xi, double eta, QPoint& Q) {
+  Q.N[0] = 0.25 * (1 - xi) * (1 - eta);
+  Q.N[1] = 0.25 * (1 + xi) * (1 - eta);
+  Q.N[2] = 0.25 * (1 + xi) * (1 + eta);
+  Q.N[3] = 0.25 * (1 - xi) * (1 + eta);
+  const double dx[4] = {-0.25 * (1 - eta), 0.25 * (1 - eta), 0.25 * (1 + eta), -0.25 * (1 + eta)};
+  const double de[4] = {-0.25 * (1 - xi), -0.25 * (1 + xi), 0.25 * (1 + xi), 0.25 * (1 - xi)};
+  double vx[3] = {0, 0, 0}, ve[3] = {0, 0, 0};
+  for (int c = 0; c < 3; ++c) {
+    Q.g[c] = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      vx[c] += dx[q] * el[3 * q + c];
+      ve[c] += de[q] * el[3 * q + c];
+      Q.g[c] += Q.N[q] * el[3 * q + c];
+    }
+  }
+  double n0 = vx[1] * ve[2] - ve[1] * vx[2];
+  double n1 = vx[2] * ve[0] - vx[0] * ve[2];
+  double n2 = vx[0] * ve[1] - vx[1] * ve[0];
+  Q.jac = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+  Q.nv[0] = n0 / Q.jac;
+  Q.nv[1] = n1 / Q.jac;
+  Q.nv[2] = n2 / Q.jac;
+}
+
+// Collapsed-triangle (Duffy) map of the quad at singular corner lpi, triangle ntr
+// (Trans_Tri :29-100, Triangle_Coord :102-188): returns (xi, eta) and the map's Jacobian.
+IBEM_HD void duffy(int ntr, int lpi, double xb, double eb, double& xi, double& eta, double& jb) {
+  static constexpr int corners[2][4][3] = {{{1, 2, 0}, {2, 3, 1}, {0, 1, 2}, {0, 1, 3}},
+                                {{2, 3, 0}, {3, 0, 1}, {3, 0, 2}, {1, 2, 3}}};
+  static constexpr double XC[4] = {-1, 1, 1, -1}, EC[4] = {-1, -1, 1, 1};
+  const double Nb[3] = {0.25 * (1 + xb) * (1 - eb), 0.25 * (1 + xb) * (1 + eb), 0.5 * (1 - xb)};
+  const double dNx[3] = {0.25 * (1 - eb), 0.25 * (1 + eb), -0.5};
+  const double dNe[3] = {-0.25 * (1 + xb), 0.25 * (1 + xb), 0.0};
+  xi = eta = 0.0;
+  double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
+  for (int q = 0; q < 3; ++q) {
+    const int c = corners[ntr][lpi][q];
+    xi += Nb[q] * XC[c];
+    eta += Nb[q] * EC[c];
+    J00 += dNx[q] * XC[c];
+    J01 += dNx[q] * EC[c];
+    J10 += dNe[q] * XC[c];
+    J11 += dNe[q] * EC[c];
+  }
+  jb = J00 * J11 - J01 * J10;
+}
+
+// Kelvin displacement / traction kernels U*, T* (UK :276-294, TK :297-321)
+IBEM_HD void kelvin_ut(const double* dxr, double r, double E, double nu, const double* nv,
+                       double UP[3][3], double TP[3][3]) {
+  const double G = E / (2 * (1 + nu));
+  const double conr = 1.0 / (16 * kPi * G * (1 - nu)) / r;
+  const double c1 = 3 - 4 * nu, c3 = 1 - 2 * nu;
+  const double Conr = 1.0 / (8 * kPi * (1 - nu)) / (r * r);
+  const double cth = nv[0] * dxr[0] + nv[1] * dxr[1] + nv[2] * dxr[2];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      UP[i][j] = conr * (c1 * (i == j) + dxr[i] * dxr[j]);
+      TP[i][j] = (i == j) ? -Conr * (c3 + 3 * dxr[i] * dxr[i]) * cth
+                          : -Conr * (3 * dxr[i] * dxr[j] * cth - c3 * (nv[j] * dxr[i] - nv[i] * dxr[j]));
+    }
+}
+
+// ---- boundary -> inclusion strain kernels (get_for_stress_BEM{,1,2}, BEMBuilderElastic3D.cpp) ---
+// Kelvin tensor G_ij,beta = d_ij cg1 d^beta r^-1 - cg2 d^(beta+e_i+e_j) r. With the harmonic and
+// biharmonic identities lap r^-1 = 0, lap r = 2 r^-1, the reference's contractions reduce to
+//   TG1[n] = kap T(n), TG[i][j] = kap T(ij), sum_i G_mi,ni = kap T(mn), trace TG = 0,
+// kap = cg1 - 2 cg2, T = d r^-1, S = d r (all with the row's extra derivatives), so
+//   Dsu[kl|n] = sg sum_m nv_m [ 2 dlam mu0 kap d_kl T(mn) + 2 dmu lam0 kap d_mn T(kl)
+//               + dmu mu0 ( cg1 (d_mk T(nl) + d_ml T(nk) + d_nk T(ml) + d_nl T(mk)) - 4 cg2 S(mknl) ) ]
+//   Dst[kl|n] = -sg ( dlam kap d_kl T(n) + dmu ( cg1 (d_nk T(l) + d_nl T(k)) - 2 cg2 S(nkl) ) )
+// with sg = -1 for first-order rows (BEMBuilderElastic3D.cpp:654-659) and +1 otherwise.
+// tabS / tabT: tables of d r and d r^-1 (degree-major, tidx) at x = quadrature point - sphere
+// centre, the reference's x (BEMBuilderElastic3D.cpp:800-803).
+IBEM_HD void strain_kernel_n(const double* tabS, const double* tabT, const Mat& M, double dlam,
+                             double dmu, const double* nv, int k, int l, const int* e, int sg, int n,
+                             double& su_out, double& st_out) {
+  const double kap = M.cg1 - 2.0 * M.cg2;
+  auto T = [&](int a, int b) {
+    int c[3] = {e[0], e[1], e[2]};
+    if (a >= 0) c[a]++;
+    if (b >= 0) c[b]++;
+    return tabT[tidx(c[0], c[1], c[2])];
+  };
+  auto S4 = [&](int a, int b, int d, int f) {
+    int c[3] = {e[0], e[1], e[2]};
+    c[a]++;
+    c[b]++;
+    c[d]++;
+    if (f >= 0) c[f]++;
+    return tabS[tidx(c[0], c[1], c[2])];
+  };
+  double su = 0.0;
+  for (int m = 0; m < 3; ++m) {
+    double t = 0.0;
+    if (k == l) t += 2.0 * dlam * M.mu0 * kap * T(m, n);
+    if (m == n) t += 2.0 * dmu * M.lam0 * kap * T(k, l);
+    double g = -4.0 * M.cg2 * S4(m, k, n, l);
+    if (m == k) g += M.cg1 * T(n, l);
+    if (m == l) g += M.cg1 * T(n, k);
+    if (n == k) g += M.cg1 * T(m, l);
+    if (n == l) g += M.cg1 * T(m, k);
+    t += dmu * M.mu0 * g;
+    su += t * nv[m];
+  }
+  double st = -2.0 * M.cg2 * S4(n, k, l, -1);
+  if (n == k) st += M.cg1 * T(l, -1);
+  if (n == l) st += M.cg1 * T(k, -1);
+  st = dmu * st;
+  if (k == l) st += dlam * kap * T(n, -1);
+  su_out = sg * su;
+  st_out = -sg * st;
+}
+
+IBEM_HD void strain_kernel(const double* tabS, const double* tabT, const Mat& M, double dlam,
+                           double dmu, const double* nv, int k, int l, const int* e, int sg,
+                           double* su3, double* st3) {
+  for (int n = 0; n < 3; ++n) strain_kernel_n(tabS, tabT, M, dlam, dmu, nv, k, l, e, sg, n, su3[n], st3[n]);
+}
+
+// Kelvin kernel row ii of U* and T* at unit direction dxr, distance r (UK/TK, BEMElasticHelper3D.cpp:276-321)
+IBEM_HD void kelvin_ut_row(int ii, const double* dxr, double r, double E, double nu, const double* nv,
+                           double* up, double* tp) {
+  const double G = E / (2 * (1 + nu));
+  const double conr = 1.0 / (16 * kPi * G * (1 - nu)) / r;
+  const double c1 = 3 - 4 * nu, c3 = 1 - 2 * nu;
+  const double Conr = 1.0 / (8 * kPi * (1 - nu)) / (r * r);
+  const double cth = nv[0] * dxr[0] + nv[1] * dxr[1] + nv[2] * dxr[2];
+  for (int j = 0; j < 3; ++j) {
+    up[j] = conr * (c1 * (ii == j) + dxr[ii] * dxr[j]);
+    tp[j] = (ii == j) ? -Conr * (c3 + 3 * dxr[ii] * dxr[ii]) * cth
+                      : -Conr * (3 * dxr[ii] * dxr[j] * cth - c3 * (nv[j] * dxr[ii] - nv[ii] * dxr[j]));
+  }
+}
+
+// One entry of the Kelvin tables: s = d^beta r, tt = d^beta r^-1 at table index t.
+IBEM_HD void kelvin_entry(const RadPoint& P, int t, double& s, double& tt) {
+  int b0, b1, b2;
+  tdecode(t, b0, b1, b2);
+  const PairSums M = pair_sums(P, b0, b1, b2);
+  rad_half(P, b0 + b1 + b2, M, s, tt);
+}
+
+}  // namespace ibem

@@ -1,0 +1,194 @@
+"""ctypes binding of the HIP engine's C ABI (include/ibem3d.h).
+
+The binding loads the in-tree ``libibem3d.so`` and raises if it is missing or if a call fails; there
+is no CPU fallback. Method names follow the reference plugin phases (iBEMSystemService.h:58-99,
+systemSolverService.h:41-45, postProcessorElastic3D.cpp:525).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .formats import Problem
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libibem3d.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+SOLVE_FP64_LU = 0
+SOLVE_MIXED_FP32 = 1
+
+
+class _CProblem(ctypes.Structure):
+    _fields_ = [("NN", ctypes.c_int), ("NE", ctypes.c_int), ("num", ctypes.c_int),
+                ("nump", ctypes.c_int), ("E0", ctypes.c_double), ("nu0", ctypes.c_double),
+                ("E1", ctypes.c_double), ("nu1", ctypes.c_double), ("nodes", _dp), ("conn", _ip),
+                ("bc", _dp), ("tbc", _dp), ("centres", _dp), ("radii", _dp), ("points", _dp)]
+
+
+_LIB = None
+
+EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size", "ibem3d_add_bem",
+           "ibem3d_add_field", "ibem3d_add_incl", "ibem3d_add_bem_rows", "ibem3d_apply_bc",
+           "ibem3d_assemble", "ibem3d_solve", "ibem3d_get_ut", "ibem3d_stress",
+           "ibem3d_get_system", "ibem3d_solve_info", "ibem3d_timings", "ibem3d_reset_timings"]
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH):
+    """Load libibem3d.so (raises when it is not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise EngineError(f"{path} is not built; run __graft_entry__.build() or "
+                          "python ibemproject_amd/build.py")
+    L = ctypes.CDLL(path)
+    L.ibem3d_create.restype = ctypes.c_void_p
+    L.ibem3d_create.argtypes = [ctypes.POINTER(_CProblem), ctypes.c_int]
+    L.ibem3d_destroy.argtypes = [ctypes.c_void_p]
+    L.ibem3d_last_error.restype = ctypes.c_char_p
+    L.ibem3d_last_error.argtypes = [ctypes.c_void_p]
+    L.ibem3d_size.restype = ctypes.c_long
+    L.ibem3d_size.argtypes = [ctypes.c_void_p]
+    for f in ("ibem3d_add_bem", "ibem3d_add_field", "ibem3d_apply_bc", "ibem3d_assemble"):
+        getattr(L, f).argtypes = [ctypes.c_void_p]
+    for f in ("ibem3d_add_incl", "ibem3d_add_bem_rows"):
+        getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.ibem3d_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+    L.ibem3d_get_ut.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+    L.ibem3d_stress.argtypes = [ctypes.c_void_p, _dp]
+    L.ibem3d_get_system.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.ibem3d_solve_info.argtypes = [ctypes.c_void_p, _dp, _ip]
+    L.ibem3d_timings.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _dp, ctypes.c_int]
+    L.ibem3d_reset_timings.argtypes = [ctypes.c_void_p]
+    _LIB = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Engine:
+    """One RVE on one GPU: owns the device system and its solution."""
+
+    def __init__(self, prob: Problem, device: int = 0):
+        self.L = load()
+        self.prob = prob
+        self._keep = dict(
+            nodes=np.ascontiguousarray(prob.nodes, dtype=np.float64),
+            conn=np.ascontiguousarray(prob.conn, dtype=np.int32),
+            bc=np.ascontiguousarray(prob.bc, dtype=np.float64),
+            tbc=np.ascontiguousarray(prob.tbc, dtype=np.float64),
+            centres=np.ascontiguousarray(prob.centres, dtype=np.float64).reshape(-1, 3),
+            radii=np.ascontiguousarray(prob.radii, dtype=np.float64),
+            points=np.ascontiguousarray(prob.points, dtype=np.float64).reshape(-1, 3),
+        )
+        k = self._keep
+        cp = _CProblem(prob.NN, prob.NE, prob.num, prob.nump, prob.E0, prob.nu0, prob.E1,
+                       prob.nu1, _d(k["nodes"]), k["conn"].ctypes.data_as(_ip), _d(k["bc"]),
+                       _d(k["tbc"]), _d(k["centres"]), _d(k["radii"]), _d(k["points"]))
+        self.ctx = self.L.ibem3d_create(ctypes.byref(cp), device)
+        if not self.ctx:
+            raise EngineError(self.L.ibem3d_last_error(None).decode())
+        self.n = int(self.L.ibem3d_size(self.ctx))
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.ibem3d_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc: int, what: str):
+        if rc != 0:
+            raise EngineError(f"{what}: {self.L.ibem3d_last_error(self.ctx).decode()} (code {rc})")
+
+    # ---- reference phases ----
+    def addBEM(self):
+        self._chk(self.L.ibem3d_add_bem(self.ctx), "addBEM")
+
+    def addFieldToBEM(self):
+        self._chk(self.L.ibem3d_add_field(self.ctx), "addFieldToBEM")
+
+    def addFluxEquivalentInclusion(self, order: int = 0):
+        self._chk(self.L.ibem3d_add_incl(self.ctx, order), "addFluxEquivalentInclusion")
+
+    def addFluxEquivalentBEM(self, order: int = 0):
+        self._chk(self.L.ibem3d_add_bem_rows(self.ctx, order), "addFluxEquivalentBEM")
+
+    def ApplyBC(self):
+        self._chk(self.L.ibem3d_apply_bc(self.ctx), "ApplyBC")
+
+    def assemble(self):
+        self._chk(self.L.ibem3d_assemble(self.ctx), "assemble")
+
+    def solve(self, mode: int = SOLVE_FP64_LU, tol: float = 1e-10):
+        self._chk(self.L.ibem3d_solve(self.ctx, mode, tol), "solve")
+        rel = ctypes.c_double()
+        it = ctypes.c_int()
+        self.L.ibem3d_solve_info(self.ctx, ctypes.byref(rel), ctypes.byref(it))
+        return rel.value, it.value
+
+    def getUTVector(self):
+        p = self.prob
+        U = np.zeros(p.n78)
+        T = np.zeros(12 * p.NE)
+        X = np.zeros(self.n)
+        self._chk(self.L.ibem3d_get_ut(self.ctx, _d(U), _d(T), _d(X)), "getUTVector")
+        return U, T, X
+
+    def getFlux(self):
+        s = np.zeros((self.prob.nump, 6))
+        self._chk(self.L.ibem3d_stress(self.ctx, _d(s)), "getFlux")
+        return s
+
+    def system(self):
+        A = np.zeros((self.n, self.n), order="F")
+        b = np.zeros(self.n)
+        self._chk(self.L.ibem3d_get_system(self.ctx, _d(A), _d(b)), "get_system")
+        return A, b
+
+    def rhs(self):
+        b = np.zeros(self.n)
+        self._chk(self.L.ibem3d_get_system(self.ctx, None, _d(b)), "get_system")
+        return b
+
+    def timings(self) -> dict:
+        names = ctypes.create_string_buffer(8192)
+        ms = np.zeros(128)
+        cnt = self.L.ibem3d_timings(self.ctx, names, 8192, _d(ms), 128)
+        out = {}
+        raw = names.raw
+        pos = 0
+        for i in range(cnt):
+            end = raw.index(b"\0", pos)
+            out[raw[pos:end].decode()] = float(ms[i])
+            pos = end + 1
+        return out
+
+    def reset_timings(self):
+        self.L.ibem3d_reset_timings(self.ctx)
+
+
+def run_reference_phases(eng: Engine):
+    """iBEMSystemService::run order (iBEMSystemService.h:58-67) then ApplyBC."""
+    eng.addBEM()
+    eng.addFieldToBEM()
+    for o in range(3):
+        eng.addFluxEquivalentInclusion(o)
+    for o in range(3):
+        eng.addFluxEquivalentBEM(o)
+    eng.ApplyBC()

@@ -1,0 +1,112 @@
+/* ibem3d -- MI355X engine for the iBEM elastic-3D spherical-inclusion solve.
+ *
+ * C ABI. Plain pointers and sizes; every call returns 0 on success and a negative code on error
+ * (message from ibem3d_last_error). No exception crosses the ABI. Caller-owned inputs are copied to
+ * the device at create; outputs are caller-allocated. A context owns its device memory and one HIP
+ * stream; calls on one context are serialised by the caller; several contexts may coexist.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   configElastic3D::Readpartical_info / ReadBEM_info / ReadPostProcess_info
+ *       configElastic3D.cpp:180-344, configElastic3D.h:76-129     -> ibem3d_create
+ *   BEMBuilderElastic3D::addBEM             BEMBuilderElastic3D.cpp:1355  -> ibem3d_add_bem
+ *   integratorElastic3D::addFieldToBEM      integratorElastic3D.cpp:135   -> ibem3d_add_field
+ *   integratorElastic3D::addFluxEquivalent{,FirstOrder,SecondOrder}Inclusion
+ *                                           integratorElastic3D.cpp:341, 455, 568 -> ibem3d_add_incl
+ *   BEMBuilderElastic3D::addFluxEquivalent{,FirstOrder,SecondOrder}BEM
+ *                                           BEMBuilderElastic3D.cpp:1511, 1629, 1743 -> ibem3d_add_bem_rows
+ *   BEMBuilderElastic3D::ApplyBC            BEMBuilderElastic3D.cpp:1920  -> ibem3d_apply_bc
+ *   systemSolverElastic3D::solve            systemSolverElastic3D.h:44-60 -> ibem3d_solve
+ *   BEMBuilderElastic3D::getUTVector        BEMBuilderElastic3D.cpp:2053  -> ibem3d_get_ut
+ *   postProcessorElastic3D::getFlux         postProcessorElastic3D.cpp:525 -> ibem3d_stress
+ *
+ * Unknown layout. The reference keeps 78 unknowns per sphere (6 uniform, 18 linear, 54 quadratic
+ * eigenstrain coefficients, configElastic3D.cpp:308-343). The quadratic block E_ij,kl is symmetric in
+ * (k,l), so the device system keeps 60 = 6 + 18 + 36 (k <= l) per sphere; n = 3 NN + 60 num.
+ * ibem3d_get_ut returns the reference's 78-layout vectors.
+ *
+ * Boundary conditions are folded while assembling: the dense system A (n x n, column-major) is built
+ * directly in the column order ApplyBC produces (displacement-unknown node dofs, then traction-unknown
+ * (node, dof) keys in first-encounter order, then eigenstrain columns), and b is accumulated at the
+ * same time; the reference's HMAT and GMAT are never materialised.
+ */
+#ifndef IBEM3D_H
+#define IBEM3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ibem3d_ctx ibem3d_ctx;
+
+/* One RVE in the reference's input formats (position.txt, BEM.txt, postprocess.txt). */
+typedef struct {
+  int NN, NE, num, nump;   /* nodes, elements, spheres, stress points */
+  double E0, nu0;          /* matrix (BEM.txt lines 2-3) */
+  double E1, nu1;          /* particles (position.txt lines 2-3) */
+  const double* nodes;     /* NN x 3, row-major */
+  const int* conn;         /* NE x 4, row-major, one-based node ids */
+  const double* bc;        /* 3NN x 2, row-major: (flag 1 = displacement unknown / 0 = known, value) */
+  const double* tbc;       /* NE x 24, row-major: 12 flags (1 = traction known, 0 = unknown), 12 values */
+  const double* centres;   /* num x 3, row-major */
+  const double* radii;     /* num */
+  const double* points;    /* nump x 3, row-major (may be NULL when nump == 0) */
+} ibem3d_problem;
+
+enum {
+  IBEM3D_OK = 0,
+  IBEM3D_EINVAL = -1,   /* bad argument or inconsistent boundary conditions */
+  IBEM3D_EHIP = -2,     /* HIP runtime error (device, memory) */
+  IBEM3D_ESTATE = -3,   /* call out of order */
+  IBEM3D_ESOLVE = -4    /* singular system or refinement did not converge */
+};
+
+/* solver modes for ibem3d_solve */
+enum {
+  IBEM3D_SOLVE_FP64_LU = 0,     /* FP64 blocked LU with partial pivoting (FP64 MFMA update) */
+  IBEM3D_SOLVE_MIXED_FP32 = 1   /* FP32 LU (FP32 MFMA update) + FP64 GMRES iterative refinement */
+};
+
+/* Create a context on HIP device `device`: copies the problem to the device, builds the BC column
+ * map, allocates the n x n system. Returns NULL on failure (reason in ibem3d_last_error(NULL)). */
+ibem3d_ctx* ibem3d_create(const ibem3d_problem* prob, int device);
+void ibem3d_destroy(ibem3d_ctx* ctx);
+const char* ibem3d_last_error(const ibem3d_ctx* ctx);
+
+/* System size n = 3 NN + 60 num. */
+long ibem3d_size(const ibem3d_ctx* ctx);
+
+/* ---- assembly phases (same call order as iBEMSystemService::run, iBEMSystemService.h:58-67) ---- */
+int ibem3d_add_bem(ibem3d_ctx* ctx);                 /* boundary rows x boundary columns, b rows */
+int ibem3d_add_field(ibem3d_ctx* ctx);               /* boundary rows x eigenstrain columns */
+int ibem3d_add_incl(ibem3d_ctx* ctx, int order);     /* inclusion rows of order 0/1/2 (or -1: all) */
+int ibem3d_add_bem_rows(ibem3d_ctx* ctx, int order); /* inclusion rows x boundary columns, b rows */
+int ibem3d_apply_bc(ibem3d_ctx* ctx);                /* checks that every phase ran; system ready */
+/* All of the above in order. */
+int ibem3d_assemble(ibem3d_ctx* ctx);
+
+/* Solve A x = b. mode: IBEM3D_SOLVE_*; tol: relative residual target ||b - A x|| / ||b|| for the
+ * refined modes (ignored by FP64_LU). */
+int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol);
+
+/* Solution: U (3NN + 78 num, known displacements reinserted, eigenstrains in the 78 layout),
+ * T (12 NE, known tractions reinserted) -- BEMBuilderElastic3D::getUTVector. X60 (n) may be NULL. */
+int ibem3d_get_ut(ibem3d_ctx* ctx, double* U, double* T, double* X60);
+
+/* Stresses at the problem's points: nump x 6 row-major (s00 s01 s02 s11 s12 s22). */
+int ibem3d_stress(ibem3d_ctx* ctx, double* s);
+
+/* Copy the assembled system (A column-major n x n, b n) to the host; either may be NULL. */
+int ibem3d_get_system(ibem3d_ctx* ctx, double* A, double* b);
+
+/* Refinement statistics of the last solve: relative residual, iterations (0 for FP64_LU). */
+int ibem3d_solve_info(const ibem3d_ctx* ctx, double* rel_residual, int* iterations);
+
+/* Per-kernel timings of the calls since the last reset, measured with HIP events on the context's
+ * stream: names (NUL-separated, up to cap bytes) and milliseconds; returns the entry count. */
+int ibem3d_timings(const ibem3d_ctx* ctx, char* names, int cap, double* ms, int max_entries);
+void ibem3d_reset_timings(ibem3d_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

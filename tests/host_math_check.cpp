@@ -1,0 +1,77 @@
+// Test infrastructure: compiles the kernels' per-entry math (ibem3d_math.h) for the host so the CPU
+// test suite can check it against the oracle without a GPU. Not part of the product library.
+#include "../ibemproject_amd/csrc/ibem3d_math.h"
+
+#include <vector>
+
+using namespace ibem;
+
+extern "C" {
+
+// 60 x 60 inclusion block (column-major) for offset x = target centre - source centre, radius a.
+void hc_incl_block(double E0, double nu0, double E1, double nu1, const double* x, double a,
+                   int self, double* out) {
+  Mat M = make_mat(E0, nu0, E1, nu1);
+  RadPoint P;
+  rad_point(P, x);
+  const bool inside = is_inside(x, a);
+  double C[8][4];
+  pot_coefs(a, inside, C);
+  std::vector<double> tab(8 * kTabN);
+  for (int t = 0; t < kTabN; ++t) {
+    double v[8];
+    pot_entry(P, inside, C, t, v);
+    for (int f = 0; f < 8; ++f) tab[f * kTabN + t] = v[f];
+  }
+  for (int c = 0; c < 60; ++c)
+    for (int r = 0; r < 60; ++r) out[c * 60 + r] = incl_entry(tab.data(), kTabN, M, self != 0, r, c);
+}
+
+// 3 x 60 node block (column-major) for x = node - sphere centre
+void hc_field_block(double E0, double nu0, double E1, double nu1, const double* x, double a,
+                    double* out) {
+  Mat M = make_mat(E0, nu0, E1, nu1);
+  RadPoint P;
+  rad_point(P, x);
+  const bool inside = is_inside(x, a);
+  double C[8][4];
+  pot_coefs(a, inside, C);
+  for (int c = 0; c < 60; ++c)
+    for (int i = 0; i < 3; ++i) out[c * 3 + i] = field_entry(P, inside, C, M, i, c);
+}
+
+// Element x sphere-centre block in the 60-row layout: H part (60 x 12, = -fac DDsu) and G part
+// (60 x 12, = fac DDst), row-major; post != 0 selects the stress (postprocessor) variant with
+// order-0 rows only (6 x 12).
+void hc_bem_rows(double E0, double nu0, double E1, double nu1, const double* el, const double* point,
+                 int post, double* Hout, double* Gout) {
+  Mat M = make_mat(E0, nu0, E1, nu1);
+  const int rows = post ? 6 : 60;
+  for (int q = 0; q < rows * 12; ++q) Hout[q] = Gout[q] = 0.0;
+  std::vector<double> tabS(kTabN), tabT(kTabN);
+  for (int a = 0; a < 8; ++a)
+    for (int b = 0; b < 8; ++b) {
+      QPoint Q;
+      qpoint(el, gauss8_x(a), gauss8_x(b), Q);
+      const double W = gauss8_w(a) * gauss8_w(b) * Q.jac;
+      double x[3] = {Q.g[0] - point[0], Q.g[1] - point[1], Q.g[2] - point[2]};
+      RadPoint P;
+      rad_point(P, x);
+      for (int t = 0; t < tidx(0, 0, 6) + 1; ++t) kelvin_entry(P, t, tabS[t], tabT[t]);
+      for (int r = 0; r < rows; ++r) {
+        Slot R = slot60(r);
+        double su[3], st[3];
+        const double dlam = post ? M.lam0 : M.dlam, dmu = post ? M.mu0 : M.dmu;
+        strain_kernel(tabS.data(), tabT.data(), M, dlam, dmu, Q.nv, R.i, R.j, R.e,
+                      R.order == 1 ? -1 : 1, su, st);
+        const double fac = R.order == 2 ? 0.5 : 1.0;
+        for (int qn = 0; qn < 4; ++qn)
+          for (int n = 0; n < 3; ++n) {
+            Hout[r * 12 + 3 * qn + n] += (post ? 1.0 : -fac) * su[n] * Q.N[qn] * W;
+            Gout[r * 12 + 3 * qn + n] += fac * st[n] * Q.N[qn] * W;
+          }
+      }
+    }
+}
+
+}  // extern "C"

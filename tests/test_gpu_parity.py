@@ -1,0 +1,118 @@
+"""Parity of the HIP engine with the CPU oracle (run on an MI355X: pytest -m gpu).
+
+Every test prints the case it runs before touching the GPU. Tolerances: assembled entries and b to
+1e-12 of the largest entry of their (row block, column block); solutions and stresses to 1e-10
+relative (FP64 path) or 1e-8 (refined mixed-precision path).
+"""
+import numpy as np
+import pytest
+
+from ibemproject_amd.engine import SOLVE_FP64_LU, SOLVE_MIXED_FP32, Engine, run_reference_phases
+from ibemproject_amd.generate import make_rve
+
+pytestmark = pytest.mark.gpu
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def small_problem(kind):
+    if kind == "two":
+        return make_rve(7, 2, 0.10, 0.15, 2, nump=6)
+    if kind == "five":
+        p = make_rve(11, 5, 0.08, 0.14, 3, nump=12, E1=7.0, nu1=0.22)
+        # points at a sphere centre and inside a sphere exercise the interior branch
+        pts = p.points.copy()
+        pts[0] = p.centres[0]
+        pts[1] = p.centres[1] + np.array([0.3, -0.2, 0.1]) * p.radii[1]
+        p.points = pts
+        return p
+    raise KeyError(kind)
+
+
+def oracle_solution(oracle_lib, prob):
+    op = oracle_lib.OracleProblem(prob)
+    H, G = op.assemble78()
+    A78, b78 = op.apply_bc(H, G)
+    A60, b60 = op.reduce60(A78, b78)
+    X60 = oracle_lib.partialpiv_solve(A60, b60)
+    X78 = op.expand78(X60)
+    U, T = op.get_ut(X78)
+    s = op.stress(U, T)
+    return dict(A60=A60, b60=b60, X60=X60, U=U, T=T, s=s)
+
+
+def blocks(prob):
+    nb = 3 * prob.NN
+    edges = [0, nb] + [nb + 60 * (h + 1) for h in range(prob.num)]
+    return [(edges[i], edges[i + 1]) for i in range(len(edges) - 1)]
+
+
+@pytest.fixture(scope="module", params=["two", "five"])
+def solved(request, oracle_lib):
+    prob = small_problem(request.param)
+    case(f"{request.param}: NN={prob.NN} NE={prob.NE} num={prob.num} n60={prob.n60}")
+    ref = oracle_solution(oracle_lib, prob)
+    eng = Engine(prob)
+    run_reference_phases(eng)
+    A, b = eng.system()
+    rel, it = eng.solve(SOLVE_FP64_LU)
+    U, T, X = eng.getUTVector()
+    s = eng.getFlux()
+    yield dict(prob=prob, ref=ref, A=A, b=b, X=X, U=U, T=T, s=s, rel=rel, eng=eng)
+    eng.close()
+
+
+def test_assembly_blocks(solved):
+    prob, ref, A = solved["prob"], solved["ref"], solved["A"]
+    worst = 0.0
+    for (r0, r1) in blocks(prob):
+        for (c0, c1) in blocks(prob):
+            R = ref["A60"][r0:r1, c0:c1]
+            scale = np.abs(R).max()
+            if scale == 0:
+                assert np.abs(A[r0:r1, c0:c1]).max() == 0
+                continue
+            worst = max(worst, np.abs(A[r0:r1, c0:c1] - R).max() / scale)
+    print("assembly worst block relerr", worst)
+    assert worst < 1e-12
+
+
+def test_rhs(solved):
+    prob, ref, b = solved["prob"], solved["ref"], solved["b"]
+    for (r0, r1) in blocks(prob):
+        R = ref["b60"][r0:r1]
+        scale = max(np.abs(R).max(), 1e-300)
+        assert np.abs(b[r0:r1] - R).max() / scale < 1e-12
+
+
+def test_solution(solved):
+    prob, ref = solved["prob"], solved["ref"]
+    assert solved["rel"] < 1e-12
+    for (r0, r1) in blocks(prob):
+        R = ref["X60"][r0:r1]
+        e = np.linalg.norm(solved["X"][r0:r1] - R) / np.linalg.norm(R)
+        assert e < 1e-10, (r0, r1, e)
+    for name in ("U", "T"):
+        R = ref[name]
+        assert np.linalg.norm(solved[name] - R) / np.linalg.norm(R) < 1e-10
+
+
+def test_stress(solved):
+    ref, s = solved["ref"]["s"], solved["s"]
+    e = np.abs(s - ref).max() / np.abs(ref).max()
+    print("stress relerr", e)
+    assert e < 1e-10
+
+
+def test_mixed_fp32_refined(solved):
+    eng, ref = solved["eng"], solved["ref"]
+    case("mixed fp32 + refinement on the same system")
+    rel, it = eng.solve(SOLVE_MIXED_FP32, 1e-10)
+    assert rel <= 1e-10
+    U, T, X = eng.getUTVector()
+    assert np.linalg.norm(X - ref["X60"]) / np.linalg.norm(ref["X60"]) < 1e-8
+    s = eng.getFlux()
+    assert np.abs(s - ref["s"]).max() / np.abs(ref["s"]).max() < 1e-8
+    eng.solve(SOLVE_FP64_LU)
