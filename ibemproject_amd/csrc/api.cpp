@@ -342,6 +342,16 @@ int ibem3d_assemble(ibem3d_ctx* ctx) {
   return ibem3d_apply_bc(ctx);
 }
 
+int ibem3d_reset(ibem3d_ctx* ctx) {
+  CTX_CHECK();
+  const long n = ctx->n;
+  HIPC(hipMemsetAsync(ctx->d_A, 0, sizeof(double) * (size_t)n * 3 * ctx->NN, ctx->st));
+  HIPC(hipMemsetAsync(ctx->d_b, 0, sizeof(double) * n, ctx->st));
+  ctx->phases = 0;
+  ctx->solved = false;
+  return 0;
+}
+
 int ibem3d_get_system(ibem3d_ctx* ctx, double* A, double* b) {
   CTX_CHECK();
   const long n = ctx->n;

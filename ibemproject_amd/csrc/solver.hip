@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "ibem3d_kernels.h"
 
@@ -48,7 +49,7 @@ constexpr int kBM = 128, kBN = 128, kBK = 16, kPad = 16;
 // C[M x N] -= A[M x K] B[K x N]. Workgroup tile 128 x 128, four waves in a 2 x 2 grid, each wave a
 // 64 x 64 block of 4 x 4 MFMA 16x16 tiles. MFMA operand layout (16x16x4): lane l supplies
 // A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator rows per Vec4<T>::row.
-template <typename T>
+template <typename T, int Role>
 __global__ __launch_bounds__(256) void k_gemm_sub(long M, long N, long K, const T* __restrict__ A,
                                                   long lda, const T* __restrict__ B, long ldb,
                                                   T* __restrict__ C, long ldc) {
@@ -117,12 +118,14 @@ __global__ __launch_bounds__(256) void k_gemm_sub(long M, long N, long K, const 
     }
 }
 
-template <typename T>
+// Role 0: panel / TRSM updates; Role 1: the trailing-matrix update of the blocked LU (a separate
+// kernel symbol so profiles separate the dominant launches).
+template <typename T, int Role = 0>
 static void gemm_sub(long M, long N, long K, const T* A, long lda, const T* B, long ldb, T* C,
                      long ldc, hipStream_t st) {
   if (M <= 0 || N <= 0 || K <= 0) return;
   dim3 grid((unsigned)((M + kBM - 1) / kBM), (unsigned)((N + kBN - 1) / kBN));
-  hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
+  hipLaunchKernelGGL((k_gemm_sub<T, Role>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
 }
 
 void gemm_f64_sub(long M, long N, long K, const double* A, long lda, const double* B, long ldb,
@@ -137,6 +140,7 @@ void gemm_f32_sub(long M, long N, long K, const float* A, long lda, const float*
 // ================================================================ panel (cooperative kernel) ====
 constexpr int kPanelW = 32;      // columns factored by one cooperative launch
 constexpr int kPanelThreads = 256;
+constexpr int kPanelBlocksDefault = 32;  // fewer workgroups make each grid barrier cheaper
 
 // Factor A[j0:n, j0:j0+w) with partial pivoting. Workgroup g owns panel rows
 // [j0 + g chunk, j0 + (g+1) chunk). Per column: (a) scale the previous column and update the
@@ -366,6 +370,12 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st,
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_panel<T>, kPanelThreads, 0);
     W.coop_blocks = std::max(1, std::min(occ, 2) * cus);
+    if (const char* ev = getenv("IBEM3D_PANEL_BLOCKS")) {
+      const int v = atoi(ev);
+      if (v > 0) W.coop_blocks = std::min(v, W.coop_blocks);
+    } else {
+      W.coop_blocks = std::min(W.coop_blocks, kPanelBlocksDefault);
+    }
   }
   hipMemsetAsync(W.d_info, 0, sizeof(int), st);
   for (long k = 0; k < n; k += kLuNB) {
@@ -383,8 +393,8 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st,
     if (tick) tick(tctx, "lu.swap_trsm", 1);
     if (k + kb < n) {
       if (tick) tick(tctx, "lu.gemm", 0);
-      gemm_sub<T>(n - k - kb, n - k - kb, kb, A + k * lda + k + kb, lda, A + (k + kb) * lda + k, lda,
-                  A + (k + kb) * lda + k + kb, lda, st);
+      gemm_sub<T, 1>(n - k - kb, n - k - kb, kb, A + k * lda + k + kb, lda, A + (k + kb) * lda + k,
+                     lda, A + (k + kb) * lda + k + kb, lda, st);
       if (tick) tick(tctx, "lu.gemm", 1);
     }
   }

@@ -34,7 +34,7 @@ _LIB = None
 
 EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size", "ibem3d_add_bem",
            "ibem3d_add_field", "ibem3d_add_incl", "ibem3d_add_bem_rows", "ibem3d_apply_bc",
-           "ibem3d_assemble", "ibem3d_solve", "ibem3d_get_ut", "ibem3d_stress",
+           "ibem3d_assemble", "ibem3d_reset", "ibem3d_solve", "ibem3d_get_ut", "ibem3d_stress",
            "ibem3d_get_system", "ibem3d_solve_info", "ibem3d_timings", "ibem3d_reset_timings"]
 
 
@@ -58,7 +58,8 @@ def load(path: str = LIB_PATH):
     L.ibem3d_last_error.argtypes = [ctypes.c_void_p]
     L.ibem3d_size.restype = ctypes.c_long
     L.ibem3d_size.argtypes = [ctypes.c_void_p]
-    for f in ("ibem3d_add_bem", "ibem3d_add_field", "ibem3d_apply_bc", "ibem3d_assemble"):
+    for f in ("ibem3d_add_bem", "ibem3d_add_field", "ibem3d_apply_bc", "ibem3d_assemble",
+              "ibem3d_reset"):
         getattr(L, f).argtypes = [ctypes.c_void_p]
     for f in ("ibem3d_add_incl", "ibem3d_add_bem_rows"):
         getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -134,6 +135,9 @@ class Engine:
 
     def assemble(self):
         self._chk(self.L.ibem3d_assemble(self.ctx), "assemble")
+
+    def reset(self):
+        self._chk(self.L.ibem3d_reset(self.ctx), "reset")
 
     def solve(self, mode: int = SOLVE_FP64_LU, tol: float = 1e-10):
         self._chk(self.L.ibem3d_solve(self.ctx, mode, tol), "solve")

@@ -83,6 +83,9 @@ int ibem3d_add_bem_rows(ibem3d_ctx* ctx, int order); /* inclusion rows x boundar
 int ibem3d_apply_bc(ibem3d_ctx* ctx);                /* checks that every phase ran; system ready */
 /* All of the above in order. */
 int ibem3d_assemble(ibem3d_ctx* ctx);
+/* Clear the assembled system so the phases can run again on the same inputs (no counterpart in
+ * the reference, which re-reads its inputs for every run). */
+int ibem3d_reset(ibem3d_ctx* ctx);
 
 /* Solve A x = b. mode: IBEM3D_SOLVE_*; tol: relative residual target ||b - A x|| / ||b|| for the
  * refined modes (ignored by FP64_LU). */
