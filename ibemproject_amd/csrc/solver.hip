@@ -44,18 +44,47 @@ struct Vec4<float> {
   static __device__ inline int row(int lk, int v) { return 4 * lk + v; }
 };
 
-constexpr int kBM = 128, kBN = 128, kBK = 16, kPad = 16;
+constexpr int kBM = 128, kBN = 128;
 
-// C[M x N] -= A[M x K] B[K x N]. Workgroup tile 128 x 128, four waves in a 2 x 2 grid, each wave a
-// 64 x 64 block of 4 x 4 MFMA 16x16 tiles. MFMA operand layout (16x16x4): lane l supplies
-// A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator rows per Vec4<T>::row.
+template <typename T>
+struct GemmCfg;
+#ifndef IBEM_GEMM_BK_F64
+#define IBEM_GEMM_BK_F64 16
+#endif
+#ifndef IBEM_GEMM_BK_F32
+#define IBEM_GEMM_BK_F32 32
+#endif
+#ifndef IBEM_GEMM_MINB
+#define IBEM_GEMM_MINB 2
+#endif
+template <>
+struct GemmCfg<double> {
+  static constexpr int BK = IBEM_GEMM_BK_F64;
+};
+template <>
+struct GemmCfg<float> {
+  static constexpr int BK = IBEM_GEMM_BK_F32;
+};
+
+// C[M x N] -= A[M x K] B[K x N], column-major. Workgroup tile 128 x 128 (256 threads, four waves
+// in a 2 x 2 grid, each wave a 64 x 64 block of 4 x 4 MFMA 16x16 tiles). K advances in tiles of BK:
+// the next tile is fetched into registers while the MFMAs consume the current one, and LDS is
+// double-buffered so each K tile costs one barrier. LDS layouts: A as [k][m] (row stride 144,
+// conflict-free fragment reads for the two half-waves), B as [n][k] (stride BK + 1).
+// MFMA operand layout (16x16x4): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16];
+// accumulator rows per Vec4<T>::row.
 template <typename T, int Role>
-__global__ __launch_bounds__(256) void k_gemm_sub(long M, long N, long K, const T* __restrict__ A,
+__global__ __launch_bounds__(256, IBEM_GEMM_MINB) void k_gemm_sub(long M, long N, long K, const T* __restrict__ A,
                                                   long lda, const T* __restrict__ B, long ldb,
                                                   T* __restrict__ C, long ldc) {
   typedef typename Vec4<T>::type V4;
-  __shared__ T sA[kBK][kBM + kPad];
-  __shared__ T sB[kBK][kBN + kPad];
+  constexpr int BK = GemmCfg<T>::BK;
+  constexpr int LDA_S = kBM + 16;  // [k][m]
+  constexpr int LDB_S = BK + 1;    // [n][k]
+  constexpr int EA = kBM * BK / 256;  // A elements per thread per tile
+  constexpr int EB = kBN * BK / 256;
+  __shared__ T sA[2][BK * LDA_S];
+  __shared__ T sB[2][kBN * LDB_S];
   const long m0 = (long)blockIdx.x * kBM, n0 = (long)blockIdx.y * kBN;
   const int tid = threadIdx.x;
   const int wave = tid / 64, lane = tid % 64;
@@ -66,42 +95,61 @@ __global__ __launch_bounds__(256) void k_gemm_sub(long M, long N, long K, const 
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = V4{0, 0, 0, 0};
-  // global -> LDS mapping: A: thread -> (k = tid / 16, 8 rows at (tid % 16) * 8);
-  //                         B: thread -> (n = tid / 2, 8 k at (tid % 2) * 8)
-  const int ak = tid / 16, am = (tid % 16) * 8;
-  const int bn = tid / 2, bk = (tid % 2) * 8;
-  for (long k0 = 0; k0 < K; k0 += kBK) {
-    {
-      const long kk = k0 + ak;
-      const T* src = A + kk * lda + m0 + am;
+  T ra[EA], rb[EB];
+  // per-thread fixed coordinates: A element e is (m = tid % 128, k = tid / 128 + (256 / 128) e),
+  // B element e is (k = tid % BK, n = tid / BK + (256 / BK) e)
+  const int am = tid % kBM, ak = tid / kBM;
+  const int bk = tid % BK, bn = tid / BK;
+  const bool a_in = (m0 + am) < M;
+  const T* pa = A + (long)ak * lda + m0 + am;
+  const T* pb = B + (n0 + bn) * ldb + bk;
+  auto fetch = [&](long k0) {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const long m = m0 + am + e;
-        sA[ak][am + e] = (kk < K && m < M) ? src[e] : T(0);
-      }
+    for (int e = 0; e < EA; ++e) {
+      const long gk = k0 + ak + (256 / kBM) * e;
+      ra[e] = (a_in && gk < K) ? pa[(k0 + (256 / kBM) * e) * lda] : T(0);
     }
-    {
-      const long nn = n0 + bn;
-      const T* src = B + nn * ldb + k0 + bk;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const long kk = k0 + bk + e;
-        sB[bk + e][bn] = (nn < N && kk < K) ? src[e] : T(0);
-      }
+    for (int e = 0; e < EB; ++e) {
+      const long gn = n0 + bn + (256 / BK) * e;
+      const long gk = k0 + bk;
+      rb[e] = (gn < N && gk < K) ? pb[(long)(256 / BK) * e * ldb + k0] : T(0);
     }
-    __syncthreads();
+  };
+  auto stash = [&](int buf) {
 #pragma unroll
-    for (int kk = 0; kk < kBK; kk += 4) {
+    for (int e = 0; e < EA; ++e) {
+      const int idx = tid + 256 * e;
+      sA[buf][(idx / kBM) * LDA_S + idx % kBM] = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      const int idx = tid + 256 * e;
+      sB[buf][(idx / BK) * LDB_S + idx % BK] = rb[e];
+    }
+  };
+  const long ntiles = (K + BK - 1) / BK;
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (long t = 0; t < ntiles; ++t) {
+    const int buf = (int)(t & 1);
+    if (t + 1 < ntiles) fetch((t + 1) * BK);
+    const T* a_s = sA[buf];
+    const T* b_s = sB[buf];
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
       T a[4], b[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = sA[kk + lk][wm + i * 16 + lr];
+      for (int i = 0; i < 4; ++i) a[i] = a_s[(kk + lk) * LDA_S + wm + i * 16 + lr];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = sB[kk + lk][wn + j * 16 + lr];
+      for (int j = 0; j < 4; ++j) b[j] = b_s[(wn + j * 16 + lr) * LDB_S + kk + lk];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
     }
+    if (t + 1 < ntiles) stash(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
