@@ -87,6 +87,7 @@ def cpu_baseline(prob, workload: str):
         try:
             rc = L.ref_run(din.encode(), dout.encode(), 0, 1)
         finally:
+            ctypes.CDLL(None).fflush(None)
             os.dup2(saved, 1)
             os.close(devnull)
             os.close(saved)
