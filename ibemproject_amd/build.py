@@ -16,6 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libibem3d.so")
 SOURCES = ["assembly.hip", "solver.hip", "post.hip", "api.cpp"]
+HOST_SOURCES = ["io.cpp"]           # plain C++ (no HIP), compiled with the host compiler
+DRIVER = os.path.join(HERE, "ibem3d_run")
 ARCH = os.environ.get("IBEM3D_ARCH", "gfx950")
 
 
@@ -47,14 +49,30 @@ def build(verbose: bool = False) -> str:
             raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
         return obj
 
+    def compile_host(src):
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stderr[-6000:]}")
+        return obj
+
     with ThreadPoolExecutor(max_workers=4) as ex:
-        objs = list(ex.map(compile_one, SOURCES))
+        objs = list(ex.map(compile_one, SOURCES)) + list(ex.map(compile_host, HOST_SOURCES))
     tmp = LIB + ".tmp"
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    # the native driver links the library from its own directory
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", DRIVER + ".tmp",
+           os.path.join(CSRC, "ibem3d_run.cpp"), "-L" + HERE, "-libem3d",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver link failed:\n{r.stderr[-4000:]}")
+    os.replace(DRIVER + ".tmp", DRIVER)
     return LIB
 
 

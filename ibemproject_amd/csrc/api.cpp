@@ -246,7 +246,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
                   ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32,
-                  ctx->W.d_ipiv, ctx->W.d_part,  ctx->W.d_info};
+                  ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (ctx->st) hipStreamDestroy(ctx->st);
@@ -366,7 +366,8 @@ static int ensure_solver_ws(ibem3d_ctx* ctx) {
   const long n = ctx->n;
   if (!ctx->W.d_ipiv) {
     if (int rc = dalloc(ctx, &ctx->W.d_ipiv, (size_t)n)) return rc;
-    if (int rc = dalloc(ctx, &ctx->W.d_part, 2 * 8192)) return rc;
+    if (int rc = dalloc(ctx, &ctx->W.d_ws, lu_workspace_doubles(n))) return rc;
+    HIPC(hipMemsetAsync(ctx->W.d_ws, 0, sizeof(double) * lu_workspace_doubles(n), ctx->st));
     if (int rc = dalloc(ctx, &ctx->W.d_info, 1)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_r, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_c, (size_t)n)) return rc;
@@ -427,7 +428,7 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
     {
       Scope s(ctx, "solve.lu_f64");
       if (lu_factor_f64(ctx->d_Af, n, n, ctx->W, ctx->st, tick, ctx))
-        return fail(ctx, IBEM3D_EHIP, "cooperative panel launch failed");
+        return fail(ctx, IBEM3D_EHIP, "LU launch failed");
     }
     int info = 0;
     HIPC(hipMemcpyAsync(&info, ctx->W.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
@@ -464,7 +465,7 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
     {
       Scope s(ctx, "solve.lu_f32");
       if (lu_factor_f32(ctx->d_A32, n, n, ctx->W, ctx->st, tick, ctx))
-        return fail(ctx, IBEM3D_EHIP, "cooperative panel launch failed");
+        return fail(ctx, IBEM3D_EHIP, "LU launch failed");
     }
     int info = 0;
     HIPC(hipMemcpyAsync(&info, ctx->W.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));

@@ -40,14 +40,15 @@ void gemm_f64_sub(long M, long N, long K, const double* A, long lda, const doubl
 void gemm_f32_sub(long M, long N, long K, const float* A, long lda, const float* B, long ldb,
                   float* C, long ldc, hipStream_t st);
 
-// Blocked right-looking LU with partial pivoting, in place; ipiv[k] = row swapped with row k
-// (absolute, 0-based). Returns 0, or a positive column index + 1 when an exact zero pivot appears.
+// Blocked right-looking LU with row pivoting (tournament pivoting inside 32-column leaves), in
+// place; ipiv[k] = row swapped with row k (absolute, 0-based, applied in order). d_info receives
+// the first column + 1 whose pivot is exactly zero. Returns nonzero on a launch error.
 struct LuWork {
-  int* d_ipiv = nullptr;   // n
-  double* d_part = nullptr;  // panel reduction workspace
+  int* d_ipiv = nullptr;  // n
+  double* d_ws = nullptr; // lu_workspace_doubles(n), zero-filled once at allocation
   int* d_info = nullptr;
-  int coop_blocks = 0;
 };
+size_t lu_workspace_doubles(long n);
 int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, void (*tick)(void*, const char*, int), void* tctx);
 int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, void (*tick)(void*, const char*, int), void* tctx);
 // Solve with the factors: x (in/out, n) <- A^{-1} x

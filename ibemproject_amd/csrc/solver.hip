@@ -1,20 +1,17 @@
 // Dense LU solver of the iBEM system on gfx950 (replaces Eigen FullPivLU, systemSolverElastic3D.h:55).
 //
-// Blocked right-looking LU with partial (row) pivoting, column-major, in place:
-//   for each 256-wide panel: recursive panel factorisation (leaves of 32 columns factored by one
-//   cooperative kernel whose workgroups split the panel rows and meet at grid barriers for the pivot
-//   search and row swap), row interchanges on the other columns, recursive unit-lower TRSM, and the
+// Blocked right-looking LU with row pivoting, column-major, in place:
+//   for each 256-wide panel: recursive panel factorisation (leaves of 32 columns factored with
+//   tournament pivoting in two launches), row interchanges on the other columns, recursive
+//   unit-lower TRSM, and the
 //   trailing-matrix update C -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64, or f32 for mixed precision).
 // The trailing update holds 2n^3/3 of the flops; everything else is memory- or latency-bound.
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdlib>
 
 #include "ibem3d_kernels.h"
-
-namespace cg = cooperative_groups;
 
 namespace ibem {
 
@@ -185,113 +182,295 @@ void gemm_f32_sub(long M, long N, long K, const float* A, long lda, const float*
   gemm_sub<float>(M, N, K, A, lda, B, ldb, C, ldc, st);
 }
 
-// ================================================================ panel (cooperative kernel) ====
-constexpr int kPanelW = 32;      // columns factored by one cooperative launch
-constexpr int kPanelThreads = 256;
-constexpr int kPanelBlocksDefault = 32;  // fewer workgroups make each grid barrier cheaper
+// ======================================================= panel leaf: tournament pivoting (CALU) ==
+// A leaf factors w <= 32 columns A[j0:n, j0:j0+w) in two launches, with no grid barrier:
+//  k_ts_tree  every workgroup takes 256 panel rows (thread = row, the row's w values in registers)
+//             and runs partial-pivoting elimination on them to pick w candidate pivot rows, whose
+//             ORIGINAL values it writes out. The last workgroup of each group of 8 to finish
+//             (atomic counter) merges the group's 8 x w candidates the same way, and so on up a
+//             tree; the last one standing (the root) holds the w pivot rows in order with their
+//             L11\U11 factors. It turns the pivot order into LAPACK-style sequential interchanges
+//             (ipiv) and saves the original contents of the top rows that move down.
+//  k_ts_apply every panel row: the top w rows receive L11\U11; every other row (with its new
+//             content where an interchange moved one there) becomes its row of L21 = A21 U11^-1.
+// Tournament pivoting (communication-avoiding LU, Grigori, Demmel and Xiang 2011) replaces the
+// column-by-column global pivot search; it is backward stable in practice like partial pivoting.
+// Ties are broken by the smaller row index and the tree shape is fixed, so results are
+// deterministic.
+constexpr int kPanelW = 32;   // leaf width
+constexpr int kTsRows = 256;  // rows per workgroup (one per thread) at every tree level
+constexpr int kTsFan = 8;     // candidate sets merged per tree node (8 x 32 = 256 rows)
 
-// Factor A[j0:n, j0:j0+w) with partial pivoting. Workgroup g owns panel rows
-// [j0 + g chunk, j0 + (g+1) chunk). Per column: (a) scale the previous column and update the
-// remaining panel columns of the own rows, then the local |max| of the current column;
-// grid barrier; (b) workgroup 0 reduces the partial maxima, swaps the two rows across the panel
-// width and records the pivot; grid barrier.
+static long ts_nodes(long n) { return 2 * ((n + kTsRows - 1) / kTsRows) + 16; }
+size_t lu_workspace_doubles(long n) {
+  const long nodes = ts_nodes(n);
+  return (size_t)nodes * kPanelW * kPanelW + 2 * kPanelW * kPanelW +
+         ((size_t)nodes * kPanelW + (size_t)nodes + kPanelW + 8) / 2 + 1;
+}
+struct TsWs {
+  double* cand;   // nodes x 32 x 32 candidate rows (original values)
+  double* lu11;   // 32 x 32, root's L11\U11 rows in pivot order
+  double* saved;  // 32 x 32, original contents of the top rows that move down
+  int* cidx;      // nodes x 32 candidate row indices (-1: none)
+  int* cnt;       // nodes arrival counters (zero between launches)
+  int* dest;      // 32 positions receiving a saved row
+  int* ndest;     // 1
+};
+static TsWs ts_ws(double* base, long n) {
+  const long nodes = ts_nodes(n);
+  TsWs w;
+  w.cand = base;
+  w.lu11 = base + (size_t)nodes * kPanelW * kPanelW;
+  w.saved = w.lu11 + kPanelW * kPanelW;
+  w.cidx = (int*)(w.saved + kPanelW * kPanelW);
+  w.cnt = w.cidx + (size_t)nodes * kPanelW;
+  w.dest = w.cnt + nodes;
+  w.ndest = w.dest + kPanelW;
+  return w;
+}
+// Partial-pivoting elimination of the workgroup's rows (thread = row), choosing up to w pivot rows.
+// a: the row's values, updated in place (multipliers left of the pivot column, reduced values right
+// of it); mystep: the step at which this row was chosen, or -1. Returns the number of steps that
+// found a row.
 template <typename T>
-__global__ __launch_bounds__(kPanelThreads) void k_panel(T* __restrict__ A, long lda, long n,
-                                                         long j0, int w, int* __restrict__ ipiv,
-                                                         double* __restrict__ part,
-                                                         int* __restrict__ info) {
-  cg::grid_group grid = cg::this_grid();
-  __shared__ double sv[kPanelThreads];
-  __shared__ long si[kPanelThreads];
-  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
-  const long rows = n - j0;
-  const long chunk = (rows + G - 1) / G;
-  const long rb = j0 + (long)g * chunk;
-  const long re = rb + chunk < n ? rb + chunk : n;
-  const long jend = j0 + w;
-  for (int jj = 0; jj <= w; ++jj) {
-    const long col = j0 + jj;
-    if (jj > 0) {
-      const long pc = col - 1;
-      const T piv = A[pc * lda + pc];
-      const long r0 = rb > pc + 1 ? rb : pc + 1;
-      for (long r = r0 + tid; r < re; r += kPanelThreads) {
-        const T l = A[pc * lda + r] / piv;
-        A[pc * lda + r] = l;
-        for (long c = col; c < jend; ++c) A[c * lda + r] -= l * A[c * lda + pc];
+__device__ __forceinline__ int ts_select(T (&a)[kPanelW], bool valid, int gidx, int w, double* sv, int* si, T* sp,
+                         int& mystep) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  mystep = -1;
+  int nsel = 0;
+#pragma unroll
+  for (int c = 0; c < kPanelW; ++c) {
+    if (c < w) {
+      const int par = c & 1;
+      const bool cand = valid && mystep < 0;
+      double v = cand ? fabs((double)a[c]) : -1.0;
+      int id = cand ? gidx : -1;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(v, o);
+        const int i2 = __shfl_xor(id, o);
+        if (v2 > v || (v2 == v && (unsigned)i2 < (unsigned)id)) {
+          v = v2;
+          id = i2;
+        }
       }
-    }
-    if (jj == w) break;
-    double best = -1.0;
-    long bi = -1;
-    const long r0 = rb > col ? rb : col;
-    for (long r = r0 + tid; r < re; r += kPanelThreads) {
-      const double v = fabs((double)A[col * lda + r]);
-      if (v > best) {
-        best = v;
-        bi = r;
+      if (lane == 0) {
+        sv[par * 4 + wv] = v;
+        si[par * 4 + wv] = id;
       }
-    }
-    sv[tid] = best;
-    si[tid] = bi;
-    __syncthreads();
-    for (int s = kPanelThreads / 2; s > 0; s >>= 1) {
-      if (tid < s) {
-        const double v2 = sv[tid + s];
-        const long i2 = si[tid + s];
-        if (v2 > sv[tid] || (v2 == sv[tid] && i2 >= 0 && (si[tid] < 0 || i2 < si[tid]))) {
-          sv[tid] = v2;
-          si[tid] = i2;
+      __syncthreads();
+      double bv = sv[par * 4];
+      int bi = si[par * 4];
+#pragma unroll
+      for (int q = 1; q < kTsRows / 64; ++q) {
+        const double v2 = sv[par * 4 + q];
+        const int i2 = si[par * 4 + q];
+        if (v2 > bv || (v2 == bv && (unsigned)i2 < (unsigned)bi)) {
+          bv = v2;
+          bi = i2;
+        }
+      }
+      if (bi >= 0) {
+        nsel = c + 1;
+        if (cand && gidx == bi) {
+          mystep = c;
+#pragma unroll
+          for (int k = 0; k < kPanelW; ++k) sp[par * kPanelW + k] = a[k];
         }
       }
       __syncthreads();
-    }
-    if (tid == 0) {
-      part[2 * g] = sv[0];
-      part[2 * g + 1] = (double)si[0];
-    }
-    grid.sync();
-    if (g == 0) {
-      double bv = -1.0;
-      long bidx = -1;
-      for (int q = tid; q < G; q += kPanelThreads) {
-        const double v = part[2 * q];
-        const long ix = (long)part[2 * q + 1];
-        if (ix >= 0 && (v > bv || (v == bv && ix < bidx))) {
-          bv = v;
-          bidx = ix;
-        }
-      }
-      sv[tid] = bv;
-      si[tid] = bidx;
-      __syncthreads();
-      for (int s = kPanelThreads / 2; s > 0; s >>= 1) {
-        if (tid < s) {
-          const double v2 = sv[tid + s];
-          const long i2 = si[tid + s];
-          if (i2 >= 0 && (v2 > sv[tid] || (v2 == sv[tid] && (si[tid] < 0 || i2 < si[tid])))) {
-            sv[tid] = v2;
-            si[tid] = i2;
-          }
-        }
-        __syncthreads();
-      }
-      long p = si[0];
-      if (p < 0 || sv[0] == 0.0) {
-        if (tid == 0 && *info == 0) *info = (int)(col + 1);
-        p = col;
-      }
-      if (tid == 0) ipiv[col] = (int)p;
-      if (p != col) {
-        for (long c = j0 + tid; c < jend; c += kPanelThreads) {
-          const T t = A[c * lda + col];
-          A[c * lda + col] = A[c * lda + p];
-          A[c * lda + p] = t;
-        }
+      if (bi >= 0 && valid && mystep < 0) {
+        const T piv = sp[par * kPanelW + c];
+        const T l = piv != T(0) ? a[c] / piv : T(0);
+        a[c] = l;
+#pragma unroll
+        for (int k = c + 1; k < kPanelW; ++k)
+          if (k < w) a[k] -= l * sp[par * kPanelW + k];
       }
     }
-    grid.sync();
   }
+  return nsel;
+}
+
+// position -> original row held there, over the <= 2w positions the interchanges touch
+__device__ inline int ts_who(const int* pos, const int* who, int np, int p) {
+  for (int q = 0; q < np; ++q)
+    if (pos[q] == p) return who[q];
+  return p;
+}
+__device__ inline void ts_set(int* pos, int* who, int& np, int p, int v) {
+  for (int q = 0; q < np; ++q)
+    if (pos[q] == p) {
+      who[q] = v;
+      return;
+    }
+  pos[np] = p;
+  who[np] = v;
+  ++np;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, long lda, long n,
+                                                     long j0, int w, int P, TsWs ws,
+                                                     int* __restrict__ ipiv, int* __restrict__ info) {
+  __shared__ double sv[8];
+  __shared__ int si[8];
+  __shared__ T sp[2 * kPanelW];
+  __shared__ int s_last;
+  __shared__ int s_win[kPanelW];
+  __shared__ double s_diag[kPanelW];
+  __shared__ int s_pos[2 * kPanelW], s_who[2 * kPanelW];
+  __shared__ int s_nd, s_nsel;
+  __shared__ int s_src[kPanelW];
+  const int t = threadIdx.x;
+  int node = blockIdx.x, count = P;
+  long off = 0;  // first node slot of the current tree level
+  const long r = j0 + (long)node * kTsRows + t;
+  bool valid = r < n;
+  int gidx = valid ? (int)r : -1;
+  T a[kPanelW];
+#pragma unroll
+  for (int c = 0; c < kPanelW; ++c) a[c] = (valid && c < w) ? A[(j0 + c) * lda + r] : T(0);
+  // where this row's original values live: -1 = in A (first level), else a candidate slot
+  long src = -1;
+  int mystep = -1, nsel = 0;
+  for (;;) {
+    nsel = ts_select<T>(a, valid, gidx, w, sv, si, sp, mystep);
+    if (count == 1) break;  // this workgroup holds the root set
+    const long slot = off + node;
+    if (mystep >= 0) {
+      double* dst = ws.cand + (slot * kPanelW + mystep) * kPanelW;
+      for (int c = 0; c < w; ++c)
+        dst[c] = src < 0 ? (double)A[(j0 + c) * lda + gidx] : ws.cand[src * kPanelW + c];
+      ws.cidx[slot * kPanelW + mystep] = gidx;
+    }
+    if (t >= nsel && t < kPanelW) ws.cidx[slot * kPanelW + t] = -1;
+    __threadfence();
+    __syncthreads();
+    const int group = node / kTsFan;
+    const int gsize = min(kTsFan, count - group * kTsFan);
+    const long next_off = off + count;
+    if (t == 0) {
+      const int old = atomicAdd(&ws.cnt[next_off + group], 1);
+      s_last = old == gsize - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (t == 0) ws.cnt[next_off + group] = 0;  // ready for the next leaf (a later launch)
+    const int child = group * kTsFan + t / kPanelW, k = t % kPanelW;
+    valid = (t / kPanelW) < gsize && k < w;
+    gidx = valid ? ws.cidx[(off + child) * kPanelW + k] : -1;
+    valid = valid && gidx >= 0;
+    src = (off + child) * kPanelW + k;
+#pragma unroll
+    for (int c = 0; c < kPanelW; ++c) a[c] = (valid && c < w) ? (T)ws.cand[src * kPanelW + c] : T(0);
+    node = group;
+    off = next_off;
+    count = (count + kTsFan - 1) / kTsFan;
+  }
+  // ---- root: the pivot rows in order with their L11\U11 ----
+  if (mystep >= 0) {
+#pragma unroll
+    for (int c = 0; c < kPanelW; ++c)
+      if (c < w) ws.lu11[mystep * kPanelW + c] = (double)a[c];
+    s_win[mystep] = gidx;
+    double d = 0.0;
+#pragma unroll
+    for (int c = 0; c < kPanelW; ++c)
+      if (c == mystep) d = (double)a[c];
+    s_diag[mystep] = d;
+  }
+  if (t == 0) s_nsel = nsel;
+  __syncthreads();
+  if (t == 0) {
+    const int ns = s_nsel;
+    for (int i = 0; i < w; ++i) {
+      const bool ok = i < ns;
+      if ((!ok || s_diag[i] == 0.0) && *info == 0) *info = (int)(j0 + i + 1);
+    }
+    // sequential interchanges: step i brings original row s_win[i] to position j0 + i
+    int np = 0;
+    for (int i = 0; i < w; ++i) {
+      const int top = (int)(j0 + i);
+      int p = top;
+      if (i < ns) {
+        const int rw = s_win[i];
+        p = rw;
+        for (int q = 0; q < np; ++q)
+          if (s_who[q] == rw) {
+            p = s_pos[q];
+            break;
+          }
+      }
+      ipiv[top] = p;
+      if (p != top) {
+        const int w1 = ts_who(s_pos, s_who, np, top), w2 = ts_who(s_pos, s_who, np, p);
+        ts_set(s_pos, s_who, np, top, w2);
+        ts_set(s_pos, s_who, np, p, w1);
+      }
+    }
+    // positions below the leaf's top rows that now hold a (displaced) top row
+    int nd = 0;
+    for (int q = 0; q < np; ++q)
+      if (s_pos[q] >= j0 + w && s_who[q] != s_pos[q]) {
+        ws.dest[nd] = s_pos[q];
+        s_src[nd] = s_who[q];
+        ++nd;
+      }
+    *ws.ndest = nd;
+    s_nd = nd;
+  }
+  __syncthreads();
+  for (int u = t; u < s_nd * kPanelW; u += kTsRows) {
+    const int e = u / kPanelW, c = u % kPanelW;
+    if (c < w) ws.saved[e * kPanelW + c] = (double)A[(j0 + c) * lda + s_src[e]];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTsRows) void k_ts_apply(T* __restrict__ A, long lda, long n, long j0,
+                                                      int w, TsWs ws) {
+  __shared__ T sU[kPanelW][kPanelW + 1];
+  __shared__ int sd[kPanelW];
+  __shared__ int snd;
+  const int t = threadIdx.x;
+  for (int u = t; u < kPanelW * kPanelW; u += kTsRows) {
+    const int i = u / kPanelW, c = u % kPanelW;
+    sU[i][c] = (i < w && c < w) ? (T)ws.lu11[i * kPanelW + c] : T(0);
+  }
+  if (t == 0) snd = *ws.ndest;
+  if (t < kPanelW) sd[t] = ws.dest[t];
+  __syncthreads();
+  const long r = j0 + (long)blockIdx.x * kTsRows + t;
+  if (r >= n) return;
+  if (r < j0 + w) {
+    const int i = (int)(r - j0);
+#pragma unroll
+    for (int c = 0; c < kPanelW; ++c)
+      if (c < w) A[(j0 + c) * lda + r] = sU[i][c];
+    return;
+  }
+  int e = -1;
+  for (int q = 0; q < snd; ++q)
+    if (sd[q] == r) e = q;
+  T a[kPanelW];
+#pragma unroll
+  for (int c = 0; c < kPanelW; ++c)
+    a[c] = c < w ? (e >= 0 ? (T)ws.saved[e * kPanelW + c] : A[(j0 + c) * lda + r]) : T(0);
+#pragma unroll
+  for (int c = 0; c < kPanelW; ++c) {
+    if (c < w) {
+      T x = a[c];
+#pragma unroll
+      for (int k = 0; k < c; ++k) x -= a[k] * sU[k][c];
+      const T d = sU[c][c];
+      a[c] = d != T(0) ? x / d : x;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kPanelW; ++c)
+    if (c < w) A[(j0 + c) * lda + r] = a[c];
 }
 
 // ============================================================================ row swaps ==========
@@ -377,15 +556,12 @@ struct LuCtx {
 
 template <typename T>
 static void panel_leaf(LuCtx<T>& c, long j0, int w) {
-  void* args[] = {(void*)&c.A, (void*)&c.lda, (void*)&c.n, (void*)&j0, (void*)&w,
-                  (void*)&c.W->d_ipiv, (void*)&c.W->d_part, (void*)&c.W->d_info};
-  long rows = c.n - j0;
-  int G = c.W->coop_blocks;
-  long need = (rows + 63) / 64;  // at least 64 rows per workgroup
-  if (need < G) G = (int)std::max<long>(1, need);
-  hipError_t e = hipLaunchCooperativeKernel((const void*)k_panel<T>, dim3(G), dim3(kPanelThreads),
-                                            args, 0, c.st);
-  if (e != hipSuccess) c.err = 1;
+  const long rows = c.n - j0;
+  const int P = (int)((rows + kTsRows - 1) / kTsRows);
+  const TsWs ws = ts_ws(c.W->d_ws, c.n);
+  hipLaunchKernelGGL(k_ts_tree<T>, dim3(P), dim3(kTsRows), 0, c.st, c.A, c.lda, c.n, j0, w, P, ws,
+                     c.W->d_ipiv, c.W->d_info);
+  hipLaunchKernelGGL(k_ts_apply<T>, dim3(P), dim3(kTsRows), 0, c.st, c.A, c.lda, c.n, j0, w, ws);
 }
 
 template <typename T>
@@ -412,19 +588,6 @@ template <typename T>
 static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st,
                      void (*tick)(void*, const char*, int), void* tctx) {
   LuCtx<T> c{A, n, lda, &W, st, tick, tctx, 0};
-  if (W.coop_blocks <= 0) {
-    int dev = 0, cus = 0, occ = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_panel<T>, kPanelThreads, 0);
-    W.coop_blocks = std::max(1, std::min(occ, 2) * cus);
-    if (const char* ev = getenv("IBEM3D_PANEL_BLOCKS")) {
-      const int v = atoi(ev);
-      if (v > 0) W.coop_blocks = std::min(v, W.coop_blocks);
-    } else {
-      W.coop_blocks = std::min(W.coop_blocks, kPanelBlocksDefault);
-    }
-  }
   hipMemsetAsync(W.d_info, 0, sizeof(int), st);
   for (long k = 0; k < n; k += kLuNB) {
     const long kb = std::min(kLuNB, n - k);

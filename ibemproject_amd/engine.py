@@ -35,7 +35,9 @@ _LIB = None
 EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size", "ibem3d_add_bem",
            "ibem3d_add_field", "ibem3d_add_incl", "ibem3d_add_bem_rows", "ibem3d_apply_bc",
            "ibem3d_assemble", "ibem3d_reset", "ibem3d_solve", "ibem3d_get_ut", "ibem3d_stress",
-           "ibem3d_get_system", "ibem3d_solve_info", "ibem3d_timings", "ibem3d_reset_timings"]
+           "ibem3d_get_system", "ibem3d_solve_info", "ibem3d_timings", "ibem3d_reset_timings",
+           "ibem3d_read_inputs", "ibem3d_inputs_problem", "ibem3d_inputs_title",
+           "ibem3d_free_inputs", "ibem3d_io_error", "ibem3d_write_stress"]
 
 
 class EngineError(RuntimeError):
@@ -70,12 +72,51 @@ def load(path: str = LIB_PATH):
     L.ibem3d_solve_info.argtypes = [ctypes.c_void_p, _dp, _ip]
     L.ibem3d_timings.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _dp, ctypes.c_int]
     L.ibem3d_reset_timings.argtypes = [ctypes.c_void_p]
+    L.ibem3d_read_inputs.restype = ctypes.c_void_p
+    L.ibem3d_read_inputs.argtypes = [ctypes.c_char_p]
+    L.ibem3d_inputs_problem.restype = ctypes.POINTER(_CProblem)
+    L.ibem3d_inputs_problem.argtypes = [ctypes.c_void_p]
+    L.ibem3d_inputs_title.restype = ctypes.c_char_p
+    L.ibem3d_inputs_title.argtypes = [ctypes.c_void_p]
+    L.ibem3d_free_inputs.argtypes = [ctypes.c_void_p]
+    L.ibem3d_io_error.restype = ctypes.c_char_p
+    L.ibem3d_io_error.argtypes = []
+    L.ibem3d_write_stress.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int]
     _LIB = L
     return L
 
 
 def _d(a):
     return a.ctypes.data_as(_dp)
+
+
+def read_inputs_native(directory: str) -> Problem:
+    """Parse a problem directory with the library's own reader (ibem3d_read_inputs)."""
+    L = load()
+    h = L.ibem3d_read_inputs(os.fsencode(directory))
+    if not h:
+        raise EngineError(L.ibem3d_io_error().decode())
+    try:
+        c = L.ibem3d_inputs_problem(h).contents
+        NN, NE, num, nump = c.NN, c.NE, c.num, c.nump
+
+        def arr(ptr, n, dt=np.float64):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True) if n else np.zeros(0, dt)
+        return Problem(L.ibem3d_inputs_title(h).decode(), c.E0, c.nu0, c.E1, c.nu1,
+                       arr(c.nodes, 3 * NN).reshape(NN, 3), arr(c.conn, 4 * NE, np.int32).reshape(NE, 4),
+                       arr(c.bc, 6 * NN).reshape(3 * NN, 2), arr(c.tbc, 24 * NE).reshape(NE, 24),
+                       arr(c.centres, 3 * num).reshape(num, 3), arr(c.radii, num),
+                       arr(c.points, 3 * nump).reshape(nump, 3))
+    finally:
+        L.ibem3d_free_inputs(h)
+
+
+def write_stress_native(path: str, s: np.ndarray, full: bool = False) -> None:
+    """Write stresses with the library's writer (reference format, or %.17g with full=True)."""
+    L = load()
+    a = np.ascontiguousarray(s, dtype=np.float64).reshape(-1, 6)
+    if L.ibem3d_write_stress(os.fsencode(path), _d(a), a.shape[0], int(full)) != 0:
+        raise EngineError(L.ibem3d_io_error().decode())
 
 
 class Engine:

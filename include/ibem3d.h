@@ -109,6 +109,23 @@ int ibem3d_solve_info(const ibem3d_ctx* ctx, double* rel_residual, int* iteratio
 int ibem3d_timings(const ibem3d_ctx* ctx, char* names, int cap, double* ms, int max_entries);
 void ibem3d_reset_timings(ibem3d_ctx* ctx);
 
+/* ---- the reference's text formats (host only; no GPU call) ---- */
+typedef struct ibem3d_inputs ibem3d_inputs;
+/* Read position.txt, BEM.txt and (if present) postprocess.txt from directory `dir`:
+ * configElastic3D::Readparticle_num / Readpartical_info (configElastic3D.cpp:105-125),
+ * ReadBEM_num / ReadBEM_info (configElastic3D.h:85-110, configElastic3D.cpp:32-103),
+ * ReadPostProcess_info (configElastic3D.h:113-126) -- without the dense HMAT / GMAT / A the
+ * reference allocates there (configElastic3D.cpp:95-101). Returns NULL on error (ibem3d_io_error). */
+ibem3d_inputs* ibem3d_read_inputs(const char* dir);
+/* The parsed problem; its pointers stay valid until ibem3d_free_inputs. */
+const ibem3d_problem* ibem3d_inputs_problem(const ibem3d_inputs* in);
+const char* ibem3d_inputs_title(const ibem3d_inputs* in);
+void ibem3d_free_inputs(ibem3d_inputs* in);
+const char* ibem3d_io_error(void);
+/* Write nump x 6 stresses as stress.txt rows: full = 0 is the reference's format (std::scientific,
+ * 6 digits; postProcessorElastic3D.cpp:578-585), full = 1 writes %.17g. */
+int ibem3d_write_stress(const char* path, const double* s, int nump, int full);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,3 +67,49 @@ def test_workload_properties():
     eng.assemble()
     assert np.array_equal(eng.rhs(), b1)
     eng.close()
+
+
+def test_native_driver_on_reference_example(tmp_path):
+    """ibem3d_run (the native driver: read txt inputs -> all phases -> solve -> stress.txt) on the
+    reference's example directory."""
+    import shutil
+    import subprocess
+    for f in ("BEM.txt", "position.txt", "postprocess.txt", "calculationType.txt"):
+        shutil.copy(os.path.join(GOLD, f), tmp_path / f)
+    exe = os.path.join(os.path.dirname(GOLD), "..", "..", "ibemproject_amd", "ibem3d_run")
+    case("native driver on the reference example")
+    r = subprocess.run([exe, str(tmp_path), "--mode", "fp64", "--full", "--timings"],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stderr)
+    assert r.returncode == 0
+    ref = np.load(os.path.join(GOLD, "reference.npz"))
+    truth = read_stress(os.path.join(GOLD, "stress_ground_truth.txt"))[:103]
+    s6 = read_stress(str(tmp_path / "stress.txt"))
+    s17 = read_stress(str(tmp_path / "stress_full.txt"))
+    assert np.abs(s17 - ref["stress"]).max() <= 1e-8 * np.abs(ref["stress"]).max()
+    scale = np.abs(truth).max(axis=1, keepdims=True)
+    assert (np.abs(s6 - truth) / scale).max() < 1e-5
+
+
+def test_reference_services_drive_gpu_plugins(tmp_path):
+    """The reference's own service classes (preprocess -> iBEM system -> system solver ->
+    postprocess, compiled from its headers into oracle/_ref/ref_plugin_run where the reference
+    sources were present at build time) drive the GPU plugins of
+    ibemproject_amd/reference_plugins on the reference example; stress.txt is checked against the
+    shipped ground truth."""
+    import shutil
+    import subprocess
+    exe = os.path.join(os.path.dirname(GOLD), "..", "..", "oracle", "_ref", "ref_plugin_run")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/ref_plugin_run not built (needs the reference headers at build time)")
+    for f in ("BEM.txt", "position.txt", "postprocess.txt", "calculationType.txt"):
+        shutil.copy(os.path.join(GOLD, f), tmp_path / f)
+    case("reference services driving the GPU plugins")
+    r = subprocess.run([os.path.abspath(exe)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300, env=dict(os.environ, IBEM3D_SOLVE_MODE="0"))
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    truth = read_stress(os.path.join(GOLD, "stress_ground_truth.txt"))[:103]
+    s6 = read_stress(str(tmp_path / "stress.txt"))
+    scale = np.abs(truth).max(axis=1, keepdims=True)
+    assert (np.abs(s6 - truth) / scale).max() < 1e-5
