@@ -230,14 +230,19 @@ __global__ __launch_bounds__(192) void k_bem(const double* __restrict__ qp,
   b[row] += bacc;
 }
 
-// ---- K2: inclusion rows x boundary columns, b ---------------------------------------------------
-// Workgroup = one sphere s (60 rows) x all elements; thread = (component n, row r) with
-// r = threadIdx % 64 (rows 60..63 idle), n = threadIdx / 64. Kelvin derivative tables d r
-// (|beta| <= 6) and d r^-1 (|beta| <= 4) at x = Gauss point - centre are built in LDS for
-// kRowsGpb Gauss points at a time; each thread accumulates its strain kernel against the four shape
-// functions and folds -fac Dsu (H, node columns) and fac Dst (G, traction slots) through the column
-// map (fac = 0.5 for second-order rows, BEMBuilderElastic3D.cpp:1831, 1842).
-__global__ __launch_bounds__(192) void k_rows(const double* __restrict__ qp,
+// ---- K2: inclusion rows x boundary columns + b -------------------------------------------------
+// Workgroup = one sphere s (its 60 rows), 256 threads = 4 wavefronts; loops over every element.
+// Per element: (1) the element's 64 Gauss points are staged in LDS; (2) lane = Gauss point and
+// wavefront = entry group build the 64 Kelvin derivative tables at x = point - centre
+// (BEMBuilderElastic3D.cpp:800-803) in LDS (kelvin_group, generated straight-line code);
+// (3) lane = row, wavefront w integrates Gauss points 16w .. 16w+15: the row's strain kernels
+// (row_gp, the contraction of get_for_stress_BEM{,1,2}) times shape function x weight x jacobian,
+// for the three components n and the four nodes, H and G parts; (4) the four partial sums meet in
+// LDS and are folded through the column map as in ApplyBC: -fac * H into displacement-unknown
+// columns or b, fac * G into traction-key columns (negated) or b, fac = 0.5 for second-order rows
+// (BEMBuilderElastic3D.cpp:1593-1842, 1960-2016). Each sphere's rows are written by one workgroup
+// in element order: deterministic, no atomics.
+__global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
                                               const int* __restrict__ conn,
                                               const double* __restrict__ cen, int NE, long n,
                                               long row0, Mat M, int rowmask,
@@ -247,75 +252,92 @@ __global__ __launch_bounds__(192) void k_rows(const double* __restrict__ qp,
                                               const double* __restrict__ tval,
                                               double* __restrict__ A, double* __restrict__ b) {
   __shared__ double sq[64 * kQpStride];
-  __shared__ double sS[kRowsGpb][84];
-  __shared__ double sT[kRowsGpb][35];
-  __shared__ double sb[3][64];
+  __shared__ double stab[64 * kKStride];  // tables; then the wavefront partial sums
+  __shared__ double sb[4][64];
   const int s = blockIdx.x;
-  const int nc = threadIdx.x / 64;
-  const int r = threadIdx.x % 64;
-  const Slot R = slot60(r < 60 ? r : 59);
-  const bool active = r < 60 && ((rowmask >> R.order) & 1);
-  const int sg = R.order == 1 ? -1 : 1;
+  const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+  const Slot R = slot60(lane < 60 ? lane : 59);
+  const bool active = lane < 60 && ((rowmask >> R.order) & 1);
+  const RowTerms RT = row_terms(M, R);
   const double fac = R.order == 2 ? 0.5 : 1.0;
   const double c0 = cen[3 * s], c1 = cen[3 * s + 1], c2 = cen[3 * s + 2];
-  const long row = row0 + 60L * s + r;
+  const long row = row0 + 60L * s + lane;
   double bacc = 0.0;
   for (int e = 0; e < NE; ++e) {
+    for (int u = t; u < 64 * kQpStride; u += 256) sq[u] = qp[(size_t)e * 64 * kQpStride + u];
     __syncthreads();
-    for (int u = threadIdx.x; u < 64 * kQpStride; u += blockDim.x)
-      sq[u] = qp[(size_t)e * 64 * kQpStride + u];
-    double aH[4] = {0.0, 0.0, 0.0, 0.0}, aG[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int g0 = 0; g0 < 64; g0 += kRowsGpb) {
-      __syncthreads();
-      for (int u = threadIdx.x; u < kRowsGpb * 84; u += blockDim.x) {
-        const int gi = u / 84, t = u % 84;
-        const double* Q = sq + (g0 + gi) * kQpStride;
-        const double x[3] = {Q[0] - c0, Q[1] - c1, Q[2] - c2};
-        RadPoint P;
-        rad_point(P, x);
-        double sv, tv;
-        kelvin_entry(P, t, sv, tv);
-        sS[gi][t] = sv;
-        if (t < 35) sT[gi][t] = tv;
+    {
+      const double* Q = sq + lane * kQpStride;
+      const double x[3] = {Q[0] - c0, Q[1] - c1, Q[2] - c2};
+      double y0[7], y1[7], y2[7], PR[7], PT[5];
+      kelvin_point(x, y0, y1, y2, PR, PT);
+      double* tab = stab + lane * kKStride;
+      switch (wv) {
+        case 0: kelvin_group<0>(y0, y1, y2, PR, PT, tab); break;
+        case 1: kelvin_group<1>(y0, y1, y2, PR, PT, tab); break;
+        case 2: kelvin_group<2>(y0, y1, y2, PR, PT, tab); break;
+        default: kelvin_group<3>(y0, y1, y2, PR, PT, tab); break;
       }
-      __syncthreads();
-      if (active) {
-        for (int gi = 0; gi < kRowsGpb; ++gi) {
-          const double* Q = sq + (g0 + gi) * kQpStride;
-          double su, st;
-          strain_kernel_n(sS[gi], sT[gi], M, M.dlam, M.dmu, Q + 3, R.i, R.j, R.e, sg, nc, su, st);
-          const double w = Q[10];
-          for (int q = 0; q < 4; ++q) {
-            const double nw = Q[6 + q] * w;
-            aH[q] += su * nw;
-            aG[q] += st * nw;
+    }
+    __syncthreads();
+    double aH[12], aG[12];  // index n * 4 + q
+#pragma unroll
+    for (int o = 0; o < 12; ++o) aH[o] = aG[o] = 0.0;
+    if (active) {
+      for (int gi = 0; gi < 16; ++gi) {
+        const int g = 16 * wv + gi;
+        const double* Q = sq + g * kQpStride;
+        double su[3], st[3];
+        row_gp(RT, stab + g * kKStride, Q + 3, su, st);
+        const double w = Q[10];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double nw = Q[6 + q] * w;
+#pragma unroll
+          for (int nn = 0; nn < 3; ++nn) {
+            aH[nn * 4 + q] += su[nn] * nw;
+            aG[nn * 4 + q] += st[nn] * nw;
           }
         }
       }
     }
+    __syncthreads();
+#pragma unroll
+    for (int o = 0; o < 12; ++o) {
+      stab[(wv * 24 + o) * 64 + lane] = aH[o];
+      stab[(wv * 24 + 12 + o) * 64 + lane] = aG[o];
+    }
+    __syncthreads();
     if (active) {
-      for (int q = 0; q < 4; ++q) {
-        const int nd = conn[4 * e + q] - 1;
-        const int dof = 3 * nd + nc;
-        const double hv = -fac * aH[q];
-        const int col = node_col[dof];
-        if (col >= 0)
-          A[(long)col * n + row] += hv;
-        else
-          bacc -= hv * uval[dof];
-        const int slot = 12 * e + 3 * q + nc;
-        const double gv = fac * aG[q];
-        const int kc = slot_col[slot];
-        if (kc >= 0)
-          A[(long)kc * n + row] -= gv;
-        else if (kc == -1)
-          bacc += gv * tval[slot];
+      for (int oo = 0; oo < 6; ++oo) {
+        const int o = wv * 6 + oo;
+        const double v = stab[o * 64 + lane] + stab[(24 + o) * 64 + lane] +
+                         stab[(48 + o) * 64 + lane] + stab[(72 + o) * 64 + lane];
+        const int og = o < 12 ? o : o - 12;
+        const int nc = og / 4, q = og % 4;
+        if (o < 12) {
+          const int dof = 3 * (conn[4 * e + q] - 1) + nc;
+          const double hv = -fac * v;
+          const int col = node_col[dof];
+          if (col >= 0)
+            A[(long)col * n + row] += hv;
+          else
+            bacc -= hv * uval[dof];
+        } else {
+          const int slot = 12 * e + 3 * q + nc;
+          const double gv = fac * v;
+          const int kc = slot_col[slot];
+          if (kc >= 0)
+            A[(long)kc * n + row] -= gv;
+          else if (kc == -1)
+            bacc += gv * tval[slot];
+        }
       }
     }
   }
-  sb[nc][r] = active ? bacc : 0.0;
+  sb[wv][lane] = active ? bacc : 0.0;
   __syncthreads();
-  if (nc == 0 && active) b[row] += sb[0][r] + sb[1][r] + sb[2][r];
+  if (wv == 0 && active) b[row] += sb[0][lane] + sb[1][lane] + sb[2][lane] + sb[3][lane];
 }
 
 // ---- launchers ----------------------------------------------------------------------------------
@@ -346,7 +368,7 @@ void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, 
 void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long n,
                  long row0, const Mat& M, int rowmask, const int* node_col, const int* slot_col,
                  const double* uval, const double* tval, double* A, double* b, hipStream_t st) {
-  hipLaunchKernelGGL(k_rows, dim3(num), dim3(192), 0, st, qp, conn, cen, NE, n, row0, M, rowmask,
+  hipLaunchKernelGGL(k_rows, dim3(num), dim3(256), 0, st, qp, conn, cen, NE, n, row0, M, rowmask,
                      node_col, slot_col, uval, tval, A, b);
 }
 

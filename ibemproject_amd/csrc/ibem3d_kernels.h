@@ -9,7 +9,6 @@ namespace ibem {
 
 constexpr int kQpStride = 11;  // per Gauss point: g[3], nv[3], N[4], w*jac
 constexpr int kInclTile = 4;   // target spheres per k_incl workgroup
-constexpr int kRowsGpb = 8;    // Gauss points per table batch in k_rows
 
 // ---- assembly (assembly.hip) ----
 void launch_qpoints(const double* nodes, const int* conn, int NE, double* qp, hipStream_t st);

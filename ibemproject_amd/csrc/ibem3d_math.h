@@ -594,6 +594,127 @@ IBEM_HD void kelvin_ut_row(int ii, const double* dxr, double r, double E, double
   }
 }
 
+// ---- per-Gauss-point Kelvin tables for the boundary->inclusion rows ----------------------------
+// Layout of one point's table (kKStride doubles): d^beta r at tidx(beta) for 3 <= |beta| <= 6,
+// d^beta r^-1 at 84 + tidx(beta) for 1 <= |beta| <= 4 (other slots unused).
+constexpr int kKStride = 119;
+
+#include "kelvin_tables.inc"
+
+// Inputs of kelvin_group at x: y_c[p] = (2 x_c)^p, PR[m] = d^m/drho^m rho^(1/2),
+// PT[m] = d^m/drho^m rho^(-1/2).
+IBEM_HD void kelvin_point(const double* x, double* y0, double* y1, double* y2, double* PR,
+                          double* PT) {
+  const double rho = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  const double inv = 1.0 / rho, r = sqrt(rho);
+  y0[0] = y1[0] = y2[0] = 1.0;
+#pragma unroll
+  for (int p = 1; p <= 6; ++p) {
+    y0[p] = y0[p - 1] * (2.0 * x[0]);
+    y1[p] = y1[p - 1] * (2.0 * x[1]);
+    y2[p] = y2[p - 1] * (2.0 * x[2]);
+  }
+  PR[0] = r;
+#pragma unroll
+  for (int m = 1; m <= 6; ++m) PR[m] = PR[m - 1] * (0.5 - (m - 1)) * inv;
+  PT[0] = 1.0 / r;
+#pragma unroll
+  for (int m = 1; m <= 4; ++m) PT[m] = PT[m - 1] * (-0.5 - (m - 1)) * inv;
+}
+
+// select t[i] (i = 0..2 / 0..5) without an indexed private array
+IBEM_HD double sel3(double t0, double t1, double t2, int i) { return i == 0 ? t0 : (i == 1 ? t1 : t2); }
+IBEM_HD double sel6(const double* t, int i) {
+  return i == 0 ? t[0] : (i == 1 ? t[1] : (i == 2 ? t[2] : (i == 3 ? t[3] : (i == 4 ? t[4] : t[5]))));
+}
+
+// Per-row constants of one inclusion row for the strain kernels at a Gauss point (the same
+// contraction as strain_kernel_n, regrouped so the three components n share their table reads):
+//   V_x = sum_m nv_m T(e+m+x),  W_x = sum_m nv_m S(e+m+x+k+l),
+//   su_n = sg [ A1 V_n + A2 nv_n T(e+k+l) + A3 W_n
+//               + A4 (nv_k T(e+n+l) + nv_l T(e+n+k) + d_nk V_l + d_nl V_k) ]
+//   st_n = -sg [ B1 S(e+n+k+l) + B2 (d_nk T(e+l) + d_nl T(e+k)) + B3 T(e+n) ]
+// with A1 = d_kl 2 dlam mu0 kap, A2 = 2 dmu lam0 kap, A3 = -4 dmu mu0 cg2, A4 = dmu mu0 cg1,
+// B1 = -2 dmu cg2, B2 = dmu cg1, B3 = d_kl dlam kap, kap = cg1 - 2 cg2.
+struct RowTerms {
+  int oT[6];   // T(e + m + n), (m,n) in pair order 00 01 02 11 12 22
+  int oS[6];   // S(e + m + n + k + l)
+  int oSn[3];  // S(e + n + k + l)
+  int oTn[3];  // T(e + n)
+  int k, l;
+  double A1, A2, A3, A4, B1, B2, B3, sg;
+};
+
+IBEM_HD RowTerms row_terms(const Mat& M, const Slot& R) {
+  RowTerms t;
+  t.k = R.i;
+  t.l = R.j;
+  const double kap = M.cg1 - 2.0 * M.cg2;
+  const bool kl = R.i == R.j;
+  t.A1 = kl ? 2.0 * M.dlam * M.mu0 * kap : 0.0;
+  t.A2 = 2.0 * M.dmu * M.lam0 * kap;
+  t.A3 = -4.0 * M.dmu * M.mu0 * M.cg2;
+  t.A4 = M.dmu * M.mu0 * M.cg1;
+  t.B1 = -2.0 * M.dmu * M.cg2;
+  t.B2 = M.dmu * M.cg1;
+  t.B3 = kl ? M.dlam * kap : 0.0;
+  t.sg = R.order == 1 ? -1.0 : 1.0;
+  const int e0 = R.e[0], e1 = R.e[1], e2 = R.e[2];
+  const int k0 = (R.i == 0) + (R.j == 0), k1 = (R.i == 1) + (R.j == 1), k2 = (R.i == 2) + (R.j == 2);
+  for (int u = 0; u < 6; ++u) {
+    const int m = pair_i(u), n = pair_j(u);
+    const int a0 = e0 + (m == 0) + (n == 0), a1 = e1 + (m == 1) + (n == 1), a2 = e2 + (m == 2) + (n == 2);
+    t.oT[u] = 84 + tidx(a0, a1, a2);
+    t.oS[u] = tidx(a0 + k0, a1 + k1, a2 + k2);
+  }
+  for (int n = 0; n < 3; ++n) {
+    const int a0 = e0 + (n == 0), a1 = e1 + (n == 1), a2 = e2 + (n == 2);
+    t.oSn[n] = tidx(a0 + k0, a1 + k1, a2 + k2);
+    t.oTn[n] = 84 + tidx(a0, a1, a2);
+  }
+  return t;
+}
+
+// su[n], st[n] (n = 0..2) of one row at one Gauss point with unit normal nv and table tab
+IBEM_HD void row_gp(const RowTerms& R, const double* tab, const double* nv, double* su, double* st) {
+  double T6[6], S6[6];
+#pragma unroll
+  for (int u = 0; u < 6; ++u) {
+    T6[u] = tab[R.oT[u]];
+    S6[u] = tab[R.oS[u]];
+  }
+  const double Sn0 = tab[R.oSn[0]], Sn1 = tab[R.oSn[1]], Sn2 = tab[R.oSn[2]];
+  const double Tn0 = tab[R.oTn[0]], Tn1 = tab[R.oTn[1]], Tn2 = tab[R.oTn[2]];
+  // symmetric 3x3 from pair order: (0,0)=0 (0,1)=1 (0,2)=2 (1,1)=3 (1,2)=4 (2,2)=5
+  const double V0 = nv[0] * T6[0] + nv[1] * T6[1] + nv[2] * T6[2];
+  const double V1 = nv[0] * T6[1] + nv[1] * T6[3] + nv[2] * T6[4];
+  const double V2 = nv[0] * T6[2] + nv[1] * T6[4] + nv[2] * T6[5];
+  const double W0 = nv[0] * S6[0] + nv[1] * S6[1] + nv[2] * S6[2];
+  const double W1 = nv[0] * S6[1] + nv[1] * S6[3] + nv[2] * S6[4];
+  const double W2 = nv[0] * S6[2] + nv[1] * S6[4] + nv[2] * S6[5];
+  const int k = R.k, l = R.l;
+  const double Tkl = sel6(T6, pair_t(k, l));
+  const double nvk = sel3(nv[0], nv[1], nv[2], k), nvl = sel3(nv[0], nv[1], nv[2], l);
+  const double Vk = sel3(V0, V1, V2, k), Vl = sel3(V0, V1, V2, l);
+  const double Tk = sel3(Tn0, Tn1, Tn2, k), Tl = sel3(Tn0, Tn1, Tn2, l);
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const double Vn = n == 0 ? V0 : (n == 1 ? V1 : V2);
+    const double Wn = n == 0 ? W0 : (n == 1 ? W1 : W2);
+    const double Snn = n == 0 ? Sn0 : (n == 1 ? Sn1 : Sn2);
+    const double Tnn = n == 0 ? Tn0 : (n == 1 ? Tn1 : Tn2);
+    const double Tnl = sel6(T6, pair_t(n, l)), Tnk = sel6(T6, pair_t(n, k));
+    double g = nvk * Tnl + nvl * Tnk;
+    if (n == k) g += Vl;
+    if (n == l) g += Vk;
+    su[n] = R.sg * (R.A1 * Vn + R.A2 * nv[n] * Tkl + R.A3 * Wn + R.A4 * g);
+    double h = R.B1 * Snn + R.B3 * Tnn;
+    if (n == k) h += R.B2 * Tl;
+    if (n == l) h += R.B2 * Tk;
+    st[n] = -R.sg * h;
+  }
+}
+
 // One entry of the Kelvin tables: s = d^beta r, tt = d^beta r^-1 at table index t.
 IBEM_HD void kelvin_entry(const RadPoint& P, int t, double& s, double& tt) {
   int b0, b1, b2;

@@ -31,6 +31,8 @@ def hostmath():
     import ctypes
     path = os.path.join(ROOT, "tests", "libhostmath.so")
     src = os.path.join(ROOT, "tests", "host_math_check.cpp")
-    if not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(src):
+    deps = [src] + [os.path.join(ROOT, "ibemproject_amd", "csrc", f)
+                    for f in ("ibem3d_math.h", "kelvin_tables.inc")]
+    if not os.path.exists(path) or os.path.getmtime(path) < max(os.path.getmtime(d) for d in deps):
         subprocess.run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-o", path, src], check=True)
     return ctypes.CDLL(path)

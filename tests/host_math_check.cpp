@@ -75,3 +75,40 @@ void hc_bem_rows(double E0, double nu0, double E1, double nu1, const double* el,
 }
 
 }  // extern "C"
+
+extern "C" {
+// Per-Gauss-point strain rows two ways at offset x with unit normal nv: the generated tables +
+// row_gp (what k_rows runs) and the per-entry tables + strain_kernel_n. out_new/out_old: 60 x 6
+// row-major (su0 su1 su2 st0 st1 st2); tabs: the 119-entry generated table.
+void hc_row_gp(double E0, double nu0, double E1, double nu1, const double* x, const double* nv,
+               double* out_new, double* out_old, double* tabs) {
+  Mat M = make_mat(E0, nu0, E1, nu1);
+  double y0[7], y1[7], y2[7], PR[7], PT[5];
+  kelvin_point(x, y0, y1, y2, PR, PT);
+  double tab[kKStride];
+  for (int i = 0; i < kKStride; ++i) tab[i] = 0.0;
+  kelvin_group<0>(y0, y1, y2, PR, PT, tab);
+  kelvin_group<1>(y0, y1, y2, PR, PT, tab);
+  kelvin_group<2>(y0, y1, y2, PR, PT, tab);
+  kelvin_group<3>(y0, y1, y2, PR, PT, tab);
+  for (int i = 0; i < kKStride; ++i) tabs[i] = tab[i];
+  RadPoint P;
+  rad_point(P, x);
+  double tS[84], tT[35];
+  for (int t = 0; t < 84; ++t) {
+    double sv, tv;
+    kelvin_entry(P, t, sv, tv);
+    tS[t] = sv;
+    if (t < 35) tT[t] = tv;
+  }
+  for (int r = 0; r < 60; ++r) {
+    const Slot R = slot60(r);
+    const RowTerms RT = row_terms(M, R);
+    row_gp(RT, tab, nv, out_new + 6 * r, out_new + 6 * r + 3);
+    const int sg = R.order == 1 ? -1 : 1;
+    for (int n = 0; n < 3; ++n)
+      strain_kernel_n(tS, tT, M, M.dlam, M.dmu, nv, R.i, R.j, R.e, sg, n, out_old[6 * r + n],
+                      out_old[6 * r + 3 + n]);
+  }
+}
+}

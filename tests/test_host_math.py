@@ -96,3 +96,42 @@ def test_stress_rows(setup, oracle_lib):
         su, st = oracle_lib.integ3_stress(mat, 0, el, pt, post=1)
         assert np.abs(Ho.reshape(6, 12) - su).max() <= 1e-13 * np.abs(su).max()
         assert np.abs(Go.reshape(6, 12) - st).max() <= 1e-13 * np.abs(st).max()
+
+
+def test_gauss_point_row_tables(hostmath):
+    """The generated per-point Kelvin tables and the regrouped row contraction (what k_rows runs)
+    against the per-entry radial-derivative tables and strain_kernel_n, and the tables against
+    central differences of r and 1/r."""
+    import ctypes
+    dp = ctypes.POINTER(ctypes.c_double)
+    f = hostmath.hc_row_gp
+    f.argtypes = [ctypes.c_double] * 4 + [dp] * 5
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        x = rng.normal(size=3) * rng.uniform(0.05, 2.0)
+        nv = rng.normal(size=3)
+        nv /= np.linalg.norm(nv)
+        new = np.zeros((60, 6))
+        old = np.zeros((60, 6))
+        tab = np.zeros(119)
+        f(1.0, 0.3, 10.0, 0.25, x.ctypes.data_as(dp), nv.ctypes.data_as(dp), new.ctypes.data_as(dp),
+          old.ctypes.data_as(dp), tab.ctypes.data_as(dp))
+        assert np.abs(new - old).max() <= 1e-12 * np.abs(old).max()
+    # d^beta r at |beta| = 3 via central differences of d^(beta - e_0) r at |beta - e_0| = 2 ... use
+    # the closed form of the Hessian instead: d_ij r = (delta_ij - x_i x_j / r^2) / r
+    x = np.array([0.3, -0.2, 0.45])
+    tab = np.zeros(119)
+    f(1.0, 0.3, 10.0, 0.25, x.ctypes.data_as(dp), nv.ctypes.data_as(dp), new.ctypes.data_as(dp),
+      old.ctypes.data_as(dp), tab.ctypes.data_as(dp))
+    r = np.linalg.norm(x)
+    # T(beta) for |beta| = 1 is d_i r^-1 = -x_i / r^3 (slots 84 + tidx: (1,0,0)=1, (0,1,0)=2, (0,0,1)=3)
+    assert np.allclose(tab[85:88], -x / r ** 3, rtol=1e-14, atol=0)
+    # third derivatives of r by central differences of the Hessian
+    def hess(y):
+        q = np.linalg.norm(y)
+        return (np.eye(3) - np.outer(y, y) / q ** 2) / q
+    h = 1e-5
+    d3 = (hess(x + [h, 0, 0]) - hess(x - [h, 0, 0])) / (2 * h)   # d_0 d_ij r
+    # tidx of degree 3: (3,0,0)=10, (2,1,0)=11, (2,0,1)=12, (1,2,0)=13, (1,1,1)=14, (1,0,2)=15
+    assert np.allclose([tab[10], tab[11], tab[12], tab[13], tab[14], tab[15]],
+                       [d3[0, 0], d3[0, 1], d3[0, 2], d3[1, 1], d3[1, 2], d3[2, 2]], rtol=1e-7)
