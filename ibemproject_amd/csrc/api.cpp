@@ -246,7 +246,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
                   ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32,
-                  ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info};
+                  ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info, ctx->W.d_perm, ctx->W.d_inv, ctx->W.d_tmp};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (ctx->st) hipStreamDestroy(ctx->st);
@@ -369,6 +369,9 @@ static int ensure_solver_ws(ibem3d_ctx* ctx) {
     if (int rc = dalloc(ctx, &ctx->W.d_ws, lu_workspace_doubles(n))) return rc;
     HIPC(hipMemsetAsync(ctx->W.d_ws, 0, sizeof(double) * lu_workspace_doubles(n), ctx->st));
     if (int rc = dalloc(ctx, &ctx->W.d_info, 1)) return rc;
+    if (int rc = dalloc(ctx, &ctx->W.d_perm, (size_t)n)) return rc;
+    if (int rc = dalloc(ctx, &ctx->W.d_inv, lu_inverse_doubles(n))) return rc;
+    if (int rc = dalloc(ctx, &ctx->W.d_tmp, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_r, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_c, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_work, (size_t)n)) return rc;
@@ -435,9 +438,14 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
     HIPC(hipStreamSynchronize(ctx->st));
     if (info) return fail(ctx, IBEM3D_ESOLVE, "LU: zero pivot at column " + std::to_string(info - 1));
     {
+      Scope s(ctx, "solve.prepare");
+      if (lu_prepare_solve_f64(ctx->d_Af, n, n, ctx->W, ctx->st))
+        return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
+    }
+    {
       Scope s(ctx, "solve.trsv_f64");
       hipLaunchKernelGGL(k_scale_vec, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_b, ctx->d_r, n);
-      lu_solve_f64(ctx->d_Af, n, n, ctx->W.d_ipiv, ctx->d_x, ctx->st);
+      lu_solve_f64(ctx->d_Af, n, n, ctx->W, ctx->d_x, ctx->st);
       hipLaunchKernelGGL(k_scale_vec, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_x, ctx->d_c, n);
     }
     if (int rc = after_launch(ctx, "fp64 solve")) return rc;
@@ -471,6 +479,11 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
     HIPC(hipMemcpyAsync(&info, ctx->W.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
     HIPC(hipStreamSynchronize(ctx->st));
     if (info) return fail(ctx, IBEM3D_ESOLVE, "LU(fp32): zero pivot at column " + std::to_string(info - 1));
+    {
+      Scope s(ctx, "solve.prepare");
+      if (lu_prepare_solve_f32(ctx->d_A32, n, n, ctx->W, ctx->st))
+        return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
+    }
     // iterative refinement in FP64: x <- x + C LU32^{-1} (R r)
     HIPC(hipMemsetAsync(ctx->d_x, 0, sizeof(double) * n, ctx->st));
     HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
@@ -481,7 +494,7 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
         Scope s(ctx, "solve.trsv_f32");
         hipLaunchKernelGGL(k_scale_to_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_v32, ctx->d_res,
                            ctx->d_r, n);
-        lu_solve_f32(ctx->d_A32, n, n, ctx->W.d_ipiv, ctx->d_v32, ctx->st);
+        lu_solve_f32(ctx->d_A32, n, n, ctx->W, ctx->d_v32, ctx->st);
         hipLaunchKernelGGL(k_axpy_scaled_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_v32,
                            ctx->d_c, n);
       }

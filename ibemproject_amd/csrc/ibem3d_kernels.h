@@ -43,16 +43,24 @@ void gemm_f32_sub(long M, long N, long K, const float* A, long lda, const float*
 // place; ipiv[k] = row swapped with row k (absolute, 0-based, applied in order). d_info receives
 // the first column + 1 whose pivot is exactly zero. Returns nonzero on a launch error.
 struct LuWork {
-  int* d_ipiv = nullptr;  // n
-  double* d_ws = nullptr; // lu_workspace_doubles(n), zero-filled once at allocation
+  int* d_ipiv = nullptr;   // n
+  double* d_ws = nullptr;  // lu_workspace_doubles(n), zero-filled once at allocation
   int* d_info = nullptr;
+  int* d_perm = nullptr;   // n: row permutation as a gather list (lu_prepare_solve_*)
+  double* d_inv = nullptr; // lu_inverse_doubles(n): inverses of the diagonal blocks of L and U
+  double* d_tmp = nullptr; // n: solve scratch
 };
 size_t lu_workspace_doubles(long n);
+size_t lu_inverse_doubles(long n);
 int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, void (*tick)(void*, const char*, int), void* tctx);
 int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, void (*tick)(void*, const char*, int), void* tctx);
+// After a factorisation: gather list of the interchanges and diagonal-block inverses (synchronises
+// the stream once). Returns nonzero on a HIP error.
+int lu_prepare_solve_f64(const double* LU, long n, long lda, LuWork& W, hipStream_t st);
+int lu_prepare_solve_f32(const float* LU, long n, long lda, LuWork& W, hipStream_t st);
 // Solve with the factors: x (in/out, n) <- A^{-1} x
-void lu_solve_f64(const double* LU, long n, long lda, const int* ipiv, double* x, hipStream_t st);
-void lu_solve_f32(const float* LU, long n, long lda, const int* ipiv, float* x, hipStream_t st);
+void lu_solve_f64(const double* LU, long n, long lda, const LuWork& W, double* x, hipStream_t st);
+void lu_solve_f32(const float* LU, long n, long lda, const LuWork& W, float* x, hipStream_t st);
 
 // row / column equilibration: r_i = 1 / max_j |A_ij|, c_j = 1 / max_i |r_i A_ij|; A <- R A C
 void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStream_t st);

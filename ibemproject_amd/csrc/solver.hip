@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <vector>
 
 #include "ibem3d_kernels.h"
 
@@ -622,86 +623,228 @@ int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st,
 }
 
 // ======================================================================= triangular solves ======
-constexpr int kTB = 256;
+// After the factorisation: the row permutation as a gather list (built on the host from ipiv) and
+// the inverses of the 128 x 128 diagonal blocks of L and U (k_trinv, one workgroup per block). A
+// solve is then one launch per block column and direction: every workgroup subtracts its rows'
+// share of the just-solved block (a coalesced tall GEMV), and workgroup 0, whose rows are the next
+// diagonal block, finishes that block at once with the precomputed inverse. The factors are read
+// once per solve (n^2 elements), so the solve runs at HBM speed rather than at one workgroup's.
+constexpr int kTB = 128;
 
-template <typename T>
-__global__ void k_apply_pivots(T* x, long n, const int* __restrict__ ipiv) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  for (long i = 0; i < n; ++i) {
-    const long p = ipiv[i];
-    if (p != i) {
-      const T t = x[i];
-      x[i] = x[p];
-      x[p] = t;
-    }
-  }
+size_t lu_inverse_doubles(long n) {
+  const long nb = (n + kTB - 1) / kTB;
+  return (size_t)nb * kTB * kTB * 2;
 }
 
-// x[k0:k0+kb] <- L_kk^{-1} x (unit lower) or U_kk^{-1} x (upper), one workgroup.
+// X = inverse of the diagonal block b of unit-lower L (Upper = false) or of U (Upper = true),
+// stored column-major (X[j * kTB + t] = X(t, j)) at inv + b * kTB * kTB. Thread j = column j.
 template <typename T, bool Upper>
-__global__ __launch_bounds__(kTB) void k_trsv_block(const T* __restrict__ LU, long lda, long k0,
-                                                    int kb, T* __restrict__ x) {
-  __shared__ T sx[kTB];
+__global__ __launch_bounds__(kTB) void k_trinv(const T* __restrict__ LU, long lda, long n,
+                                               T* __restrict__ inv) {
+  __shared__ T X[kTB][kTB + 1];
+  __shared__ T srow[kTB];
+  __shared__ T sdiag;
+  const long k0 = (long)blockIdx.x * kTB;
+  const int kb = (int)min((long)kTB, n - k0);
+  const int j = threadIdx.x;
+  for (int tt = 0; tt < kb; ++tt) {
+    const int t = Upper ? kb - 1 - tt : tt;
+    // row t of the triangle: L(t, 0..t-1) or U(t, t..kb-1)
+    if (!Upper ? (j < t) : (j >= t && j < kb)) srow[j] = LU[(k0 + j) * lda + k0 + t];
+    __syncthreads();
+    if (Upper && j == 0) sdiag = srow[t];
+    __syncthreads();
+    if (j < kb) {
+      T v = T(0);
+      if (!Upper) {
+        if (j < t) {
+          T acc = T(0);
+          for (int k = j; k < t; ++k) acc += srow[k] * X[k][j];
+          v = -acc;
+        } else if (j == t) {
+          v = T(1);
+        }
+      } else {
+        if (j >= t) {
+          T acc = j == t ? T(1) : T(0);
+          for (int k = t + 1; k <= j; ++k) acc -= srow[k] * X[k][j];
+          v = sdiag != T(0) ? acc / sdiag : T(0);
+        }
+      }
+      X[t][j] = v;
+    }
+    __syncthreads();
+  }
+  T* out = inv + (size_t)blockIdx.x * kTB * kTB;
+  for (int u = j; u < kTB * kTB; u += kTB) {
+    const int c = u / kTB, t = u % kTB;
+    out[u] = (c < kb && t < kb) ? X[t][c] : T(0);
+  }
+}
+
+// y <- y[perm] (the factorisation's row interchanges as one gather)
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(T* __restrict__ y, const T* __restrict__ x,
+                                                const int* __restrict__ perm, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[perm[i]];
+}
+
+// x_blk <- inverse block b times y_blk (kb <= kTB entries at y + r0), 128 threads of workgroup 0
+template <typename T>
+__device__ void trsv_apply_inverse(const T* __restrict__ invb, int kb, const T* sy, T* __restrict__ y,
+                                   long r0) {
   const int t = threadIdx.x;
-  if (t < kb) sx[t] = x[k0 + t];
-  __syncthreads();
-  if (!Upper) {
-    for (int i = 0; i < kb; ++i) {
-      if (t > i && t < kb) sx[t] -= LU[(k0 + i) * lda + k0 + t] * sx[i];
-      __syncthreads();
-    }
-  } else {
-    for (int i = kb - 1; i >= 0; --i) {
-      if (t == i) sx[i] /= LU[(k0 + i) * lda + k0 + i];
-      __syncthreads();
-      if (t < i) sx[t] -= LU[(k0 + i) * lda + k0 + t] * sx[i];
-      __syncthreads();
-    }
+  if (t < kb) {
+    T acc = T(0);
+    for (int c = 0; c < kb; ++c) acc += invb[c * kTB + t] * sy[c];
+    y[r0 + t] = acc;
   }
-  if (t < kb) x[k0 + t] = sx[t];
 }
 
-// y[0:m] -= A[0:m, 0:k] x[0:k] (column-major), k <= kTB; one thread per row
+// Forward sweep step. Block k (rows [k0, k0 + kb)) of y is solved; rows below it are reduced by
+// L[:, block k] y_k, and workgroup 0 (rows of block k + 1) then solves block k + 1. With k0 < 0 the
+// launch only solves block 0.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemv_sub_small(long m, int k, const T* __restrict__ A,
-                                                        long lda, const T* __restrict__ x,
-                                                        T* __restrict__ y) {
+__global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ LU, long lda, long n, long k0,
+                                                  int kb, const T* __restrict__ inv,
+                                                  T* __restrict__ y) {
   __shared__ T sx[kTB];
-  for (int j = threadIdx.x; j < k; j += blockDim.x) sx[j] = x[j];
+  __shared__ T part[2][kTB];
+  __shared__ T sy[kTB];
+  const int t = threadIdx.x, h = t / kTB, rl = t % kTB;
+  if (k0 < 0) {
+    const int nb0 = (int)min((long)kTB, n);
+    if (t < nb0) sy[t] = y[t];
+    __syncthreads();
+    trsv_apply_inverse(inv, nb0, sy, y, 0);
+    return;
+  }
+  if (t < kb) sx[t] = y[k0 + t];
   __syncthreads();
-  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= m) return;
-  T acc = 0;
-  for (int j = 0; j < k; ++j) acc += A[(long)j * lda + r] * sx[j];
-  y[r] -= acc;
+  const long r0 = k0 + kb + (long)blockIdx.x * kTB;
+  const long r = r0 + rl;
+  T acc = T(0);
+  if (r < n) {
+    const int c0 = h * (kTB / 2), c1 = min(kb, c0 + kTB / 2);
+    const T* a = LU + (k0 + c0) * lda + r;
+#pragma unroll 8
+    for (int c = c0; c < c1; ++c, a += lda) acc += *a * sx[c];
+  }
+  part[h][rl] = acc;
+  __syncthreads();
+  if (h == 0 && r < n) {
+    const T v = y[r] - (part[0][rl] + part[1][rl]);
+    if (blockIdx.x == 0)
+      sy[rl] = v;
+    else
+      y[r] = v;
+  }
+  if (blockIdx.x == 0) {
+    __syncthreads();
+    const int kn = (int)min((long)kTB, n - r0);
+    trsv_apply_inverse(inv + (size_t)(r0 / kTB) * kTB * kTB, kn, sy, y, r0);
+  }
+}
+
+// Backward sweep step. Block k (rows [k0, k0 + kb)) of y is solved; rows above it are reduced by
+// U[:, block k] y_k, and workgroup 0 (rows of block k - 1) then solves block k - 1. With k0 < 0 the
+// launch only solves the last block (rows [lastk, n)).
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ LU, long lda, long n, long k0,
+                                                  int kb, long lastk, const T* __restrict__ inv,
+                                                  T* __restrict__ y) {
+  __shared__ T sx[kTB];
+  __shared__ T part[2][kTB];
+  __shared__ T sy[kTB];
+  const int t = threadIdx.x, h = t / kTB, rl = t % kTB;
+  if (k0 < 0) {
+    const int nbl = (int)(n - lastk);
+    if (t < nbl) sy[t] = y[lastk + t];
+    __syncthreads();
+    trsv_apply_inverse(inv + (size_t)(lastk / kTB) * kTB * kTB, nbl, sy, y, lastk);
+    return;
+  }
+  if (t < kb) sx[t] = y[k0 + t];
+  __syncthreads();
+  const long r0 = k0 - (long)(blockIdx.x + 1) * kTB;  // >= 0: k0 is a multiple of kTB
+  const long r = r0 + rl;
+  T acc = T(0);
+  {
+    const int c0 = h * (kTB / 2), c1 = min(kb, c0 + kTB / 2);
+    const T* a = LU + (k0 + c0) * lda + r;
+#pragma unroll 8
+    for (int c = c0; c < c1; ++c, a += lda) acc += *a * sx[c];
+  }
+  part[h][rl] = acc;
+  __syncthreads();
+  if (h == 0) {
+    const T v = y[r] - (part[0][rl] + part[1][rl]);
+    if (blockIdx.x == 0)
+      sy[rl] = v;
+    else
+      y[r] = v;
+  }
+  if (blockIdx.x == 0) {
+    __syncthreads();
+    trsv_apply_inverse(inv + (size_t)(r0 / kTB) * kTB * kTB, kTB, sy, y, r0);
+  }
 }
 
 template <typename T>
-static void lu_solve(const T* LU, long n, long lda, const int* ipiv, T* x, hipStream_t st) {
-  hipLaunchKernelGGL(k_apply_pivots<T>, dim3(1), dim3(1), 0, st, x, n, ipiv);
-  for (long k = 0; k < n; k += kTB) {
-    const int kb = (int)std::min<long>(kTB, n - k);
-    hipLaunchKernelGGL((k_trsv_block<T, false>), dim3(1), dim3(kTB), 0, st, LU, lda, k, kb, x);
-    const long m = n - k - kb;
-    if (m > 0)
-      hipLaunchKernelGGL(k_gemv_sub_small<T>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, m,
-                         kb, LU + k * lda + k + kb, lda, x + k, x + k + kb);
-  }
-  const long last = ((n - 1) / kTB) * kTB;
-  for (long k = last; k >= 0; k -= kTB) {
-    const int kb = (int)std::min<long>(kTB, n - k);
-    hipLaunchKernelGGL((k_trsv_block<T, true>), dim3(1), dim3(kTB), 0, st, LU, lda, k, kb, x);
-    if (k > 0)
-      hipLaunchKernelGGL(k_gemv_sub_small<T>, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, st, k,
-                         kb, LU + k * lda, lda, x + k, x);
-  }
+static int lu_prepare(const T* LU, long n, long lda, LuWork& W, hipStream_t st) {
+  const long nb = (n + kTB - 1) / kTB;
+  T* inv = (T*)W.d_inv;
+  hipLaunchKernelGGL((k_trinv<T, false>), dim3((unsigned)nb), dim3(kTB), 0, st, LU, lda, n, inv);
+  hipLaunchKernelGGL((k_trinv<T, true>), dim3((unsigned)nb), dim3(kTB), 0, st, LU, lda, n,
+                     inv + (size_t)nb * kTB * kTB);
+  std::vector<int> ip(n), perm(n);
+  if (hipMemcpyAsync(ip.data(), W.d_ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return -1;
+  for (long i = 0; i < n; ++i) perm[i] = (int)i;
+  for (long i = 0; i < n; ++i) std::swap(perm[i], perm[ip[i]]);
+  if (hipMemcpy(W.d_perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) return -1;
+  return 0;
 }
 
-void lu_solve_f64(const double* LU, long n, long lda, const int* ipiv, double* x, hipStream_t st) {
-  lu_solve<double>(LU, n, lda, ipiv, x, st);
+template <typename T>
+static void lu_solve(const T* LU, long n, long lda, const LuWork& W, T* x, hipStream_t st) {
+  const long nb = (n + kTB - 1) / kTB;
+  const T* invL = (const T*)W.d_inv;
+  const T* invU = invL + (size_t)nb * kTB * kTB;
+  T* y = (T*)W.d_tmp;
+  const dim3 vg((unsigned)((n + 255) / 256));
+  hipLaunchKernelGGL(k_gather<T>, vg, dim3(256), 0, st, y, (const T*)x, (const int*)W.d_perm, n);
+  hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(1), dim3(256), 0, st, LU, lda, n, -1L, 0, invL, y);
+  for (long b = 0; b + 1 < nb; ++b) {
+    const long k0 = b * kTB;
+    const long rows = n - k0 - kTB;
+    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3((unsigned)((rows + kTB - 1) / kTB)), dim3(256), 0, st, LU,
+                       lda, n, k0, kTB, invL, y);
+  }
+  const long lastk = (nb - 1) * kTB;
+  hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(1), dim3(256), 0, st, LU, lda, n, -1L, 0, lastk, invU, y);
+  for (long b = nb - 1; b >= 1; --b) {
+    const long k0 = b * kTB;
+    const int kb = (int)std::min<long>(kTB, n - k0);
+    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3((unsigned)b), dim3(256), 0, st, LU, lda, n, k0, kb, lastk,
+                       invU, y);
+  }
+  hipMemcpyAsync(x, y, sizeof(T) * n, hipMemcpyDeviceToDevice, st);
 }
-void lu_solve_f32(const float* LU, long n, long lda, const int* ipiv, float* x, hipStream_t st) {
-  lu_solve<float>(LU, n, lda, ipiv, x, st);
+
+int lu_prepare_solve_f64(const double* LU, long n, long lda, LuWork& W, hipStream_t st) {
+  return lu_prepare<double>(LU, n, lda, W, st);
+}
+int lu_prepare_solve_f32(const float* LU, long n, long lda, LuWork& W, hipStream_t st) {
+  return lu_prepare<float>(LU, n, lda, W, st);
+}
+void lu_solve_f64(const double* LU, long n, long lda, const LuWork& W, double* x, hipStream_t st) {
+  lu_solve<double>(LU, n, lda, W, x, st);
+}
+void lu_solve_f32(const float* LU, long n, long lda, const LuWork& W, float* x, hipStream_t st) {
+  lu_solve<float>(LU, n, lda, W, x, st);
 }
 
 // ========================================================================= equilibration ========
