@@ -44,6 +44,7 @@ struct ibem3d_ctx {
   double *d_r = nullptr, *d_c = nullptr, *d_work = nullptr, *d_res = nullptr;
   double* d_Af = nullptr;  // FP64 factor copy
   float* d_A32 = nullptr;
+  double* d_f16ws = nullptr;  // split operands of the FP16-MFMA trailing update
   float* d_v32 = nullptr;
   LuWork W;
   unsigned phases = 0;
@@ -245,7 +246,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
   void* ptrs[] = {ctx->d_nodes, ctx->d_cen,      ctx->d_rad,      ctx->d_pts,  ctx->d_qp,
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
-                  ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32,
+                  ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32, ctx->d_f16ws,
                   ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info, ctx->W.d_perm, ctx->W.d_inv, ctx->W.d_tmp};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -453,7 +454,7 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
     ctx->solved = true;
     return 0;
   }
-  if (mode == IBEM3D_SOLVE_MIXED_FP32) {
+  if (mode == IBEM3D_SOLVE_MIXED_FP32 || mode == IBEM3D_SOLVE_MIXED_FP16) {
     if (!(tol > 0)) tol = 1e-10;
     if (!ctx->d_A32) {
       if (int rc = dalloc(ctx, &ctx->d_A32, (size_t)n * n)) return rc;
@@ -472,7 +473,10 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
     }
     {
       Scope s(ctx, "solve.lu_f32");
-      if (lu_factor_f32(ctx->d_A32, n, n, ctx->W, ctx->st, tick, ctx))
+      if (mode == IBEM3D_SOLVE_MIXED_FP16 && !ctx->d_f16ws)
+        if (int rc = dalloc(ctx, &ctx->d_f16ws, f16x3_workspace_doubles(n))) return rc;
+      if (lu_factor_f32(ctx->d_A32, n, n, ctx->W, ctx->st, tick, ctx,
+                        mode == IBEM3D_SOLVE_MIXED_FP16 ? ctx->d_f16ws : nullptr))
         return fail(ctx, IBEM3D_EHIP, "LU launch failed");
     }
     int info = 0;

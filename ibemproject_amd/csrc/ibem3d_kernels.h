@@ -38,6 +38,11 @@ void gemm_f64_sub(long M, long N, long K, const double* A, long lda, const doubl
 // same in FP32 (FP32 MFMA)
 void gemm_f32_sub(long M, long N, long K, const float* A, long lda, const float* B, long ldb,
                   float* C, long ldc, hipStream_t st);
+// same in FP32 arithmetic from three half-precision MFMA products per block (K <= 256); ws holds
+// f16x3_workspace_doubles(max(M, N)) doubles
+size_t f16x3_workspace_doubles(long n);
+void gemm_f32_sub_f16x3(long M, long N, int K, const float* A, long lda, const float* B, long ldb,
+                        float* C, long ldc, double* ws, hipStream_t st);
 
 // Blocked right-looking LU with row pivoting (tournament pivoting inside 32-column leaves), in
 // place; ipiv[k] = row swapped with row k (absolute, 0-based, applied in order). d_info receives
@@ -53,7 +58,9 @@ struct LuWork {
 size_t lu_workspace_doubles(long n);
 size_t lu_inverse_doubles(long n);
 int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, void (*tick)(void*, const char*, int), void* tctx);
-int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, void (*tick)(void*, const char*, int), void* tctx);
+// f16ws != nullptr: the trailing updates run as gemm_f32_sub_f16x3 (workspace of
+// f16x3_workspace_doubles(n) doubles); otherwise on FP32 MFMA
+int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, void (*tick)(void*, const char*, int), void* tctx, double* f16ws);
 // After a factorisation: gather list of the interchanges and diagonal-block inverses (synchronises
 // the stream once). Returns nonzero on a HIP error.
 int lu_prepare_solve_f64(const double* LU, long n, long lda, LuWork& W, hipStream_t st);

@@ -183,6 +183,250 @@ void gemm_f32_sub(long M, long N, long K, const float* A, long lda, const float*
   gemm_sub<float>(M, N, K, A, lda, B, ldb, C, ldc, st);
 }
 
+// ================================================= FP32 update on FP16 MFMA (three-term split) ==
+// C -= A B in FP32 arithmetic accuracy from half-precision products. Each operand row (of A) and
+// column (of B) is scaled by a power of two that puts its largest magnitude in [2^14, 2^15), and
+// split x = hi + lo with hi = fp16(x), lo = fp16(x - hi) (22 significant bits). Then
+//   A B = S_A^-1 (Ahi Bhi + Ahi Blo + Alo Bhi) S_B^-1 + O(2^-22 |A| |B|),
+// with every half x half product exact in the MFMA's FP32 accumulator, and the dropped Alo Blo term
+// below FP32 rounding of C for K = 256. Scaling keeps operands out of the FP16 subnormal range down
+// to 2^-28 of their row / column maximum. Three v_mfma_f32_16x16x32_f16 (2.5 PFLOP/s dense)
+// replace 8 v_mfma_f32_16x16x4f32 (157.3 TFLOP/s) per 16x16x32 block.
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+constexpr int kSplitK = 256;     // operand depth (the LU block size)
+constexpr int kSplitPadK = 256;  // row stride of the split operands, in halves
+
+__device__ inline float pow2_scale(float amax) {
+  // 2^(15 - e) with amax in [2^(e-1), 2^e); 1 for amax == 0
+  if (!(amax > 0.0f)) return 1.0f;
+  int e;
+  frexpf(amax, &e);
+  return ldexpf(1.0f, 15 - e);
+}
+
+__device__ inline void split2(float x, _Float16& hi, _Float16& lo) {
+  hi = (_Float16)x;
+  lo = (_Float16)(x - (float)hi);
+}
+
+// A operand: rows r < M of the column-major M x K block at A (lda). Output row-major [M][K] hi / lo
+// (k contiguous) and the row's inverse scale. Workgroup = 64 rows; LDS transpose.
+__global__ __launch_bounds__(256) void k_split_rows(long M, int K, const float* __restrict__ A,
+                                                    long lda, _Float16* __restrict__ hi,
+                                                    _Float16* __restrict__ lo,
+                                                    float* __restrict__ inv_scale) {
+  __shared__ float tile[kSplitK][65];
+  __shared__ float red[4][64];
+  __shared__ float ssc[64];
+  const long r0 = (long)blockIdx.x * 64;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const long r = r0 + lane;
+  float m = 0.0f;
+  for (int k = w; k < K; k += 4) {
+    const float v = r < M ? A[(long)k * lda + r] : 0.0f;
+    tile[k][lane] = v;
+    m = fmaxf(m, fabsf(v));
+  }
+  red[w][lane] = m;
+  __syncthreads();
+  if (w == 0) {
+    const float mm = fmaxf(fmaxf(red[0][lane], red[1][lane]), fmaxf(red[2][lane], red[3][lane]));
+    const float sc = pow2_scale(mm);
+    ssc[lane] = sc;
+    if (r < M) inv_scale[r] = 1.0f / sc;
+  }
+  __syncthreads();
+  // write: thread handles 8 consecutive k of one row per pass (16-byte stores)
+  for (int u = t; u < 64 * (kSplitPadK / 8); u += 256) {
+    const int rl = u / (kSplitPadK / 8), k8 = (u % (kSplitPadK / 8)) * 8;
+    const long rr = r0 + rl;
+    if (rr >= M) continue;
+    half8 h, l;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float v = (k8 + q < K) ? tile[k8 + q][rl] * ssc[rl] : 0.0f;
+      _Float16 a, b;
+      split2(v, a, b);
+      h[q] = a;
+      l[q] = b;
+    }
+    *(half8*)(hi + rr * kSplitPadK + k8) = h;
+    *(half8*)(lo + rr * kSplitPadK + k8) = l;
+  }
+}
+
+// B operand: columns c < N of the column-major K x N block at B (ldb), output [N][K] hi / lo (k
+// contiguous) and the column's inverse scale. One wavefront per column.
+__global__ __launch_bounds__(256) void k_split_cols(long N, int K, const float* __restrict__ B,
+                                                    long ldb, _Float16* __restrict__ hi,
+                                                    _Float16* __restrict__ lo,
+                                                    float* __restrict__ inv_scale) {
+  const long c = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= N) return;
+  const float* col = B + c * ldb;
+  float v[kSplitPadK / 64];
+  float m = 0.0f;
+#pragma unroll
+  for (int q = 0; q < kSplitPadK / 64; ++q) {
+    const int k = lane * (kSplitPadK / 64) + q;
+    v[q] = k < K ? col[k] : 0.0f;
+    m = fmaxf(m, fabsf(v[q]));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  const float sc = pow2_scale(m);
+  if (lane == 0) inv_scale[c] = 1.0f / sc;
+  typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+  half4 h, l;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    _Float16 a, b;
+    split2(v[q] * sc, a, b);
+    h[q] = a;
+    l[q] = b;
+  }
+  *(half4*)(hi + c * kSplitPadK + lane * 4) = h;
+  *(half4*)(lo + c * kSplitPadK + lane * 4) = l;
+}
+
+// C[M x N] (column-major, ldc) -= diag(ia) (Ah Bh^T + Ah Bl^T + Al Bh^T) diag(ib), with A* [M][K]
+// and B* [N][K] (k contiguous, row stride kSplitPadK). Workgroup tile 128 x 128, four waves in a
+// 2 x 2 grid of 64 x 64 (4 x 4 blocks of v_mfma_f32_16x16x32_f16). K advances 32 at a time: the
+// next chunk is fetched into registers while the MFMAs consume the current one from LDS. LDS rows
+// are padded to 40 halves (80 bytes) so the 16-byte fragment reads of a half-wave hit distinct
+// banks. Fragment layout (16x16x32): lane l supplies A[l % 16][8 (l / 16) .. +7] and
+// B[8 (l / 16) .. +7][l % 16]; accumulator register v of lane l is D[4 (l / 16) + v][l % 16].
+constexpr int kF16Pad = 40;
+__global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
+                                                       const _Float16* __restrict__ Ah,
+                                                       const _Float16* __restrict__ Al,
+                                                       const _Float16* __restrict__ Bh,
+                                                       const _Float16* __restrict__ Bl,
+                                                       const float* __restrict__ ia,
+                                                       const float* __restrict__ ib,
+                                                       float* __restrict__ C, long ldc) {
+  __shared__ _Float16 sAh[kBM * kF16Pad], sAl[kBM * kF16Pad], sBh[kBN * kF16Pad], sBl[kBN * kF16Pad];
+  const long m0 = (long)blockIdx.x * kBM, n0 = (long)blockIdx.y * kBN;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+  v4f acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0, 0, 0, 0};
+  // loader: 128 rows x 4 16-byte parts per operand array = 512 vectors; thread t: vectors t, t + 256
+  half8 rAh[2], rAl[2], rBh[2], rBl[2];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int v = tid + 256 * e, row = v >> 2, part = v & 3;
+      const long ga = m0 + row, gb = n0 + row;
+      const long ko = k0 + part * 8;
+      const half8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+      rAh[e] = ga < M ? *(const half8*)(Ah + ga * kSplitPadK + ko) : z;
+      rAl[e] = ga < M ? *(const half8*)(Al + ga * kSplitPadK + ko) : z;
+      rBh[e] = gb < N ? *(const half8*)(Bh + gb * kSplitPadK + ko) : z;
+      rBl[e] = gb < N ? *(const half8*)(Bl + gb * kSplitPadK + ko) : z;
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int v = tid + 256 * e, row = v >> 2, part = v & 3;
+      *(half8*)(sAh + row * kF16Pad + part * 8) = rAh[e];
+      *(half8*)(sAl + row * kF16Pad + part * 8) = rAl[e];
+      *(half8*)(sBh + row * kF16Pad + part * 8) = rBh[e];
+      *(half8*)(sBl + row * kF16Pad + part * 8) = rBl[e];
+    }
+  };
+  const int nchunks = (K + 31) / 32;
+  fetch(0);
+  for (int ch = 0; ch < nchunks; ++ch) {
+    __syncthreads();
+    stash();
+    __syncthreads();
+    if (ch + 1 < nchunks) fetch((ch + 1) * 32);
+    half8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int off = (wm + i * 16 + lr) * kF16Pad + lk * 8;
+      ah[i] = *(const half8*)(sAh + off);
+      al[i] = *(const half8*)(sAl + off);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int off = (wn + j * 16 + lr) * kF16Pad + lk * 8;
+      bh[j] = *(const half8*)(sBh + off);
+      bl[j] = *(const half8*)(sBl + off);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const long nn = n0 + wn + j * 16 + lr;
+    if (nn >= N) continue;
+    const float sb = ib[nn];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long m = m0 + wm + i * 16 + 4 * lk + v;
+        if (m < M) C[nn * ldc + m] -= acc[i][j][v] * ia[m] * sb;
+      }
+  }
+}
+
+// Workspace in doubles for split operands of up to n rows / columns
+size_t f16x3_workspace_doubles(long n) {
+  return ((size_t)4 * n * kSplitPadK * sizeof(_Float16) + 2 * n * sizeof(float)) / sizeof(double) + 16;
+}
+
+// Split operands in the workspace (capacity cap rows / columns)
+struct F16x3Ws {
+  _Float16 *Ah, *Al, *Bh, *Bl;
+  float *ia, *ib;
+};
+static F16x3Ws f16x3_ws(double* ws, long cap) {
+  F16x3Ws w;
+  w.Ah = (_Float16*)ws;
+  w.Al = w.Ah + (size_t)cap * kSplitPadK;
+  w.Bh = w.Al + (size_t)cap * kSplitPadK;
+  w.Bl = w.Bh + (size_t)cap * kSplitPadK;
+  w.ia = (float*)(w.Bl + (size_t)cap * kSplitPadK);
+  w.ib = w.ia + cap;
+  return w;
+}
+static void f16x3_split(long M, long N, int K, const float* A, long lda, const float* B, long ldb,
+                        const F16x3Ws& w, hipStream_t st) {
+  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((M + 63) / 64)), dim3(256), 0, st, M, K, A, lda,
+                     w.Ah, w.Al, w.ia);
+  hipLaunchKernelGGL(k_split_cols, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, N, K, B, ldb,
+                     w.Bh, w.Bl, w.ib);
+}
+static void f16x3_gemm(long M, long N, int K, const F16x3Ws& w, float* C, long ldc, hipStream_t st) {
+  dim3 grid((unsigned)((M + kBM - 1) / kBM), (unsigned)((N + kBN - 1) / kBN));
+  hipLaunchKernelGGL(k_gemm_f16x3, grid, dim3(256), 0, st, M, N, K, w.Ah, w.Al, w.Bh, w.Bl, w.ia,
+                     w.ib, C, ldc);
+}
+
+// C[M x N] -= A[M x K] B[K x N] (column-major FP32, K <= 256) through the split FP16 products.
+void gemm_f32_sub_f16x3(long M, long N, int K, const float* A, long lda, const float* B, long ldb,
+                        float* C, long ldc, double* ws, hipStream_t st) {
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  const F16x3Ws w = f16x3_ws(ws, M > N ? M : N);
+  f16x3_split(M, N, K, A, lda, B, ldb, w, st);
+  f16x3_gemm(M, N, K, w, C, ldc, st);
+}
+
 // ======================================================= panel leaf: tournament pivoting (CALU) ==
 // A leaf factors w <= 32 columns A[j0:n, j0:j0+w) in two launches, with no grid barrier:
 //  k_ts_tree  every workgroup takes 256 panel rows (thread = row, the row's w values in registers)
@@ -587,7 +831,7 @@ constexpr long kLuNB = 256;
 
 template <typename T>
 static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st,
-                     void (*tick)(void*, const char*, int), void* tctx) {
+                     void (*tick)(void*, const char*, int), void* tctx, double* f16ws = nullptr) {
   LuCtx<T> c{A, n, lda, &W, st, tick, tctx, 0};
   hipMemsetAsync(W.d_info, 0, sizeof(int), st);
   for (long k = 0; k < n; k += kLuNB) {
@@ -604,10 +848,22 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st,
     }
     if (tick) tick(tctx, "lu.swap_trsm", 1);
     if (k + kb < n) {
-      if (tick) tick(tctx, "lu.gemm", 0);
-      gemm_sub<T, 1>(n - k - kb, n - k - kb, kb, A + k * lda + k + kb, lda, A + (k + kb) * lda + k,
-                     lda, A + (k + kb) * lda + k + kb, lda, st);
-      if (tick) tick(tctx, "lu.gemm", 1);
+      const long m = n - k - kb;
+      if (f16ws) {
+        const F16x3Ws w = f16x3_ws(f16ws, n);
+        if (tick) tick(tctx, "lu.split", 0);
+        f16x3_split(m, m, (int)kb, (const float*)(A + k * lda + k + kb), lda,
+                    (const float*)(A + (k + kb) * lda + k), lda, w, st);
+        if (tick) tick(tctx, "lu.split", 1);
+        if (tick) tick(tctx, "lu.gemm", 0);
+        f16x3_gemm(m, m, (int)kb, w, (float*)(A + (k + kb) * lda + k + kb), lda, st);
+        if (tick) tick(tctx, "lu.gemm", 1);
+      } else {
+        if (tick) tick(tctx, "lu.gemm", 0);
+        gemm_sub<T, 1>(m, m, kb, A + k * lda + k + kb, lda, A + (k + kb) * lda + k, lda,
+                       A + (k + kb) * lda + k + kb, lda, st);
+        if (tick) tick(tctx, "lu.gemm", 1);
+      }
     }
   }
   return 0;
@@ -618,8 +874,8 @@ int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st,
   return lu_factor<double>(A, n, lda, W, st, tick, tctx);
 }
 int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st,
-                  void (*tick)(void*, const char*, int), void* tctx) {
-  return lu_factor<float>(A, n, lda, W, st, tick, tctx);
+                  void (*tick)(void*, const char*, int), void* tctx, double* f16ws) {
+  return lu_factor<float>(A, n, lda, W, st, tick, tctx, f16ws);
 }
 
 // ======================================================================= triangular solves ======

@@ -21,6 +21,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 SOLVE_FP64_LU = 0
 SOLVE_MIXED_FP32 = 1
+SOLVE_MIXED_FP16 = 2
 
 
 class _CProblem(ctypes.Structure):

@@ -7,7 +7,7 @@
  *
  * Reference interfaces replaced (paths relative to the reference root):
  *   configElastic3D::Readpartical_info / ReadBEM_info / ReadPostProcess_info
- *       configElastic3D.cpp:180-344, configElastic3D.h:76-129     -> ibem3d_create
+ *       configElastic3D.cpp:32-197, configElastic3D.h:76-129      -> ibem3d_create
  *   BEMBuilderElastic3D::addBEM             BEMBuilderElastic3D.cpp:1355  -> ibem3d_add_bem
  *   integratorElastic3D::addFieldToBEM      integratorElastic3D.cpp:135   -> ibem3d_add_field
  *   integratorElastic3D::addFluxEquivalent{,FirstOrder,SecondOrder}Inclusion
@@ -62,8 +62,10 @@ enum {
 
 /* solver modes for ibem3d_solve */
 enum {
-  IBEM3D_SOLVE_FP64_LU = 0,     /* FP64 blocked LU with partial pivoting (FP64 MFMA update) */
-  IBEM3D_SOLVE_MIXED_FP32 = 1   /* FP32 LU (FP32 MFMA update) + FP64 GMRES iterative refinement */
+  IBEM3D_SOLVE_FP64_LU = 0,     /* FP64 blocked LU with row pivoting (FP64 MFMA update) */
+  IBEM3D_SOLVE_MIXED_FP32 = 1,  /* FP32 LU (FP32 MFMA update) + FP64 iterative refinement */
+  IBEM3D_SOLVE_MIXED_FP16 = 2   /* FP32 LU whose trailing updates run on FP16 MFMA with scaled
+                                   hi/lo split operands (FP32-level accuracy) + FP64 refinement */
 };
 
 /* Create a context on HIP device `device`: copies the problem to the device, builds the BC column

@@ -6,7 +6,8 @@ import os
 import numpy as np
 import pytest
 
-from ibemproject_amd.engine import SOLVE_FP64_LU, SOLVE_MIXED_FP32, Engine, run_reference_phases
+from ibemproject_amd.engine import (SOLVE_FP64_LU, SOLVE_MIXED_FP16, SOLVE_MIXED_FP32, Engine,
+                                    run_reference_phases)
 from ibemproject_amd.formats import read_problem, read_stress
 
 pytestmark = pytest.mark.gpu
@@ -17,7 +18,7 @@ def case(msg):
     print(f"[case] {msg}", flush=True)
 
 
-@pytest.mark.parametrize("mode", [SOLVE_FP64_LU, SOLVE_MIXED_FP32])
+@pytest.mark.parametrize("mode", [SOLVE_FP64_LU, SOLVE_MIXED_FP32, SOLVE_MIXED_FP16])
 def test_reference_example(mode):
     """The fixture's system is nearly singular (condition number ~3e17: the traction unknowns at the
     corners of the clamped face z = 0 are each touched by one element only). Any elimination other
@@ -57,6 +58,11 @@ def test_workload_properties():
     assert rel32 <= 1e-10
     _, _, x32 = eng.getUTVector()
     s32 = eng.getFlux()
+    rel16, it16 = eng.solve(SOLVE_MIXED_FP16, 1e-10)
+    print("refinement iterations: fp32", it, "fp16 split", it16)
+    assert rel16 <= 1e-10
+    _, _, x16 = eng.getUTVector()
+    assert np.linalg.norm(x16 - x32) / np.linalg.norm(x32) < 1e-8
     rel64, _ = eng.solve(SOLVE_FP64_LU)
     assert rel64 <= 1e-10
     _, _, x64 = eng.getUTVector()
