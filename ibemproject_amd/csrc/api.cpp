@@ -44,6 +44,7 @@ struct ibem3d_ctx {
   double *d_r = nullptr, *d_c = nullptr, *d_work = nullptr, *d_res = nullptr;
   double* d_Af = nullptr;  // FP64 factor copy
   float* d_A32 = nullptr;
+  long lda32 = 0;  // leading dimension of d_A32: a multiple of 64 for aligned 16-byte accesses
   double* d_f16ws = nullptr;  // split operands of the FP16-MFMA trailing update
   float* d_v32 = nullptr;
   LuWork W;
@@ -457,7 +458,8 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
   if (mode == IBEM3D_SOLVE_MIXED_FP32 || mode == IBEM3D_SOLVE_MIXED_FP16) {
     if (!(tol > 0)) tol = 1e-10;
     if (!ctx->d_A32) {
-      if (int rc = dalloc(ctx, &ctx->d_A32, (size_t)n * n)) return rc;
+      ctx->lda32 = (n + 63) / 64 * 64;
+      if (int rc = dalloc(ctx, &ctx->d_A32, (size_t)ctx->lda32 * n)) return rc;
       if (int rc = dalloc(ctx, &ctx->d_v32, (size_t)n)) return rc;
     }
     // equilibrate a scratch FP64 copy to obtain R, C, then convert R A C to FP32
@@ -469,13 +471,13 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
     {
       Scope s(ctx, "solve.equilibrate");
       equilibrate_f64(ctx->d_Af, n, n, ctx->d_r, ctx->d_c, ctx->st);
-      convert_f64_to_f32(ctx->d_Af, n, n, ctx->d_A32, n, ctx->st);
+      convert_f64_to_f32(ctx->d_Af, n, n, ctx->d_A32, ctx->lda32, ctx->st);
     }
     {
       Scope s(ctx, "solve.lu_f32");
       if (mode == IBEM3D_SOLVE_MIXED_FP16 && !ctx->d_f16ws)
         if (int rc = dalloc(ctx, &ctx->d_f16ws, f16x3_workspace_doubles(n))) return rc;
-      if (lu_factor_f32(ctx->d_A32, n, n, ctx->W, ctx->st, tick, ctx,
+      if (lu_factor_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->st, tick, ctx,
                         mode == IBEM3D_SOLVE_MIXED_FP16 ? ctx->d_f16ws : nullptr))
         return fail(ctx, IBEM3D_EHIP, "LU launch failed");
     }
@@ -485,7 +487,7 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
     if (info) return fail(ctx, IBEM3D_ESOLVE, "LU(fp32): zero pivot at column " + std::to_string(info - 1));
     {
       Scope s(ctx, "solve.prepare");
-      if (lu_prepare_solve_f32(ctx->d_A32, n, n, ctx->W, ctx->st))
+      if (lu_prepare_solve_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->st))
         return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
     }
     // iterative refinement in FP64: x <- x + C LU32^{-1} (R r)
@@ -498,7 +500,7 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
         Scope s(ctx, "solve.trsv_f32");
         hipLaunchKernelGGL(k_scale_to_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_v32, ctx->d_res,
                            ctx->d_r, n);
-        lu_solve_f32(ctx->d_A32, n, n, ctx->W, ctx->d_v32, ctx->st);
+        lu_solve_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->d_v32, ctx->st);
         hipLaunchKernelGGL(k_axpy_scaled_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_v32,
                            ctx->d_c, n);
       }

@@ -292,12 +292,20 @@ __global__ __launch_bounds__(256) void k_split_cols(long N, int K, const float* 
 
 // C[M x N] (column-major, ldc) -= diag(ia) (Ah Bh^T + Ah Bl^T + Al Bh^T) diag(ib), with A* [M][K]
 // and B* [N][K] (k contiguous, row stride kSplitPadK). Workgroup tile 128 x 128, four waves in a
-// 2 x 2 grid of 64 x 64 (4 x 4 blocks of v_mfma_f32_16x16x32_f16). K advances 32 at a time: the
-// next chunk is fetched into registers while the MFMAs consume the current one from LDS. LDS rows
-// are padded to 40 halves (80 bytes) so the 16-byte fragment reads of a half-wave hit distinct
-// banks. Fragment layout (16x16x32): lane l supplies A[l % 16][8 (l / 16) .. +7] and
+// 2 x 2 grid of 64 x 64 (4 x 4 blocks of v_mfma_f32_16x16x32_f16). K advances BK at a time: the
+// next chunk is fetched into registers while the MFMAs consume the current one from LDS; the three
+// products run as three passes over the 16 accumulators, so consecutive MFMAs are independent.
+// LDS rows are padded by 8 halves so the 16-byte fragment reads of a half-wave hit distinct banks.
+// With EPI = 1 the accumulators go through LDS and C is updated with coalesced 16-byte accesses.
+// Fragment layout (16x16x32): lane l supplies A[l % 16][8 (l / 16) .. +7] and
 // B[8 (l / 16) .. +7][l % 16]; accumulator register v of lane l is D[4 (l / 16) + v][l % 16].
-constexpr int kF16Pad = 40;
+#ifndef IBEM_F16_BK
+#define IBEM_F16_BK 64
+#endif
+#ifndef IBEM_F16_EPI
+#define IBEM_F16_EPI 1
+#endif
+template <int BK, int EPI>
 __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
                                                        const _Float16* __restrict__ Ah,
                                                        const _Float16* __restrict__ Al,
@@ -306,7 +314,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
                                                        const float* __restrict__ ia,
                                                        const float* __restrict__ ib,
                                                        float* __restrict__ C, long ldc) {
-  __shared__ _Float16 sAh[kBM * kF16Pad], sAl[kBM * kF16Pad], sBh[kBN * kF16Pad], sBl[kBN * kF16Pad];
+  constexpr int LS = BK + 8;             // LDS row stride (halves)
+  constexpr int VPR = BK / 8;            // 16-byte vectors per row and array
+  constexpr int NV = kBM * VPR / 256;    // vectors per thread and array
+  constexpr int CS = kBM + 4;            // epilogue tile column stride (floats)
+  constexpr int SMEM = (EPI && kBN * CS * 2 > 4 * kBM * LS) ? kBN * CS * 2 : 4 * kBM * LS;
+  __shared__ __attribute__((aligned(16))) _Float16 smem[SMEM];
+  _Float16* sAh = smem;
+  _Float16* sAl = smem + kBM * LS;
+  _Float16* sBh = smem + 2 * kBM * LS;
+  _Float16* sBl = smem + 3 * kBM * LS;
   const long m0 = (long)blockIdx.x * kBM, n0 = (long)blockIdx.y * kBN;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
@@ -316,12 +333,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0, 0, 0, 0};
-  // loader: 128 rows x 4 16-byte parts per operand array = 512 vectors; thread t: vectors t, t + 256
-  half8 rAh[2], rAl[2], rBh[2], rBl[2];
+  half8 rAh[NV], rAl[NV], rBh[NV], rBl[NV];
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int v = tid + 256 * e, row = v >> 2, part = v & 3;
+    for (int e = 0; e < NV; ++e) {
+      const int v = tid + 256 * e, row = v / VPR, part = v % VPR;
       const long ga = m0 + row, gb = n0 + row;
       const long ko = k0 + part * 8;
       const half8 z = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -333,55 +349,101 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
   };
   auto stash = [&]() {
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int v = tid + 256 * e, row = v >> 2, part = v & 3;
-      *(half8*)(sAh + row * kF16Pad + part * 8) = rAh[e];
-      *(half8*)(sAl + row * kF16Pad + part * 8) = rAl[e];
-      *(half8*)(sBh + row * kF16Pad + part * 8) = rBh[e];
-      *(half8*)(sBl + row * kF16Pad + part * 8) = rBl[e];
+    for (int e = 0; e < NV; ++e) {
+      const int v = tid + 256 * e, row = v / VPR, part = v % VPR;
+      *(half8*)(sAh + row * LS + part * 8) = rAh[e];
+      *(half8*)(sAl + row * LS + part * 8) = rAl[e];
+      *(half8*)(sBh + row * LS + part * 8) = rBh[e];
+      *(half8*)(sBl + row * LS + part * 8) = rBl[e];
     }
   };
-  const int nchunks = (K + 31) / 32;
+  const int nchunks = (K + BK - 1) / BK;
   fetch(0);
   for (int ch = 0; ch < nchunks; ++ch) {
     __syncthreads();
     stash();
     __syncthreads();
-    if (ch + 1 < nchunks) fetch((ch + 1) * 32);
-    half8 ah[4], al[4], bh[4], bl[4];
+    if (ch + 1 < nchunks) fetch((ch + 1) * BK);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int off = (wm + i * 16 + lr) * kF16Pad + lk * 8;
-      ah[i] = *(const half8*)(sAh + off);
-      al[i] = *(const half8*)(sAl + off);
-    }
+    for (int kk = 0; kk < BK; kk += 32) {
+      half8 ah[4], al[4], bh[4], bl[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int off = (wn + j * 16 + lr) * kF16Pad + lk * 8;
-      bh[j] = *(const half8*)(sBh + off);
-      bl[j] = *(const half8*)(sBl + off);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i) {
+        const int off = (wm + i * 16 + lr) * LS + kk + lk * 8;
+        ah[i] = *(const half8*)(sAh + off);
+        al[i] = *(const half8*)(sAl + off);
+      }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        const int off = (wn + j * 16 + lr) * LS + kk + lk * 8;
+        bh[j] = *(const half8*)(sBh + off);
+        bl[j] = *(const half8*)(sBl + off);
       }
-  }
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const long nn = n0 + wn + j * 16 + lr;
-    if (nn >= N) continue;
-    const float sb = ib[nn];
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  if constexpr (EPI == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long nn = n0 + wn + j * 16 + lr;
+      if (nn >= N) continue;
+      const float sb = ib[nn];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const long m = m0 + wm + i * 16 + 4 * lk + v;
+          if (m < M) C[nn * ldc + m] -= acc[i][j][v] * ia[m] * sb;
+        }
+    }
+  } else {
+    // tile 128 x 128 floats, column-major with stride CS, over the operand LDS
+    float* sc = (float*)smem;
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const long m = m0 + wm + i * 16 + 4 * lk + v;
-        if (m < M) C[nn * ldc + m] -= acc[i][j][v] * ia[m] * sb;
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sc[(wn + j * 16 + lr) * CS + wm + i * 16 + 4 * lk + v] = acc[i][j][v];
+    __syncthreads();
+    // thread: 4 consecutive rows of one column per pass; 32 threads per column
+    const bool full = (m0 + kBM <= M) && ((ldc & 3) == 0) && ((((size_t)C) & 15) == 0);
+    for (int u = tid; u < kBN * (kBM / 4); u += 256) {
+      const int cl = u / (kBM / 4), r4 = (u % (kBM / 4)) * 4;
+      const long nn = n0 + cl;
+      if (nn >= N) continue;
+      const float sb = ib[nn];
+      const long m = m0 + r4;
+      float* cp = C + nn * ldc + m;
+      if (full) {
+        float4 c = *(float4*)cp;
+        const float4 a = *(const float4*)(ia + m);
+        c.x -= sc[cl * CS + r4] * a.x * sb;
+        c.y -= sc[cl * CS + r4 + 1] * a.y * sb;
+        c.z -= sc[cl * CS + r4 + 2] * a.z * sb;
+        c.w -= sc[cl * CS + r4 + 3] * a.w * sb;
+        *(float4*)cp = c;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (m + q < M) cp[q] -= sc[cl * CS + r4 + q] * ia[m + q] * sb;
       }
+    }
   }
 }
 
@@ -414,8 +476,8 @@ static void f16x3_split(long M, long N, int K, const float* A, long lda, const f
 }
 static void f16x3_gemm(long M, long N, int K, const F16x3Ws& w, float* C, long ldc, hipStream_t st) {
   dim3 grid((unsigned)((M + kBM - 1) / kBM), (unsigned)((N + kBN - 1) / kBN));
-  hipLaunchKernelGGL(k_gemm_f16x3, grid, dim3(256), 0, st, M, N, K, w.Ah, w.Al, w.Bh, w.Bl, w.ia,
-                     w.ib, C, ldc);
+  hipLaunchKernelGGL((k_gemm_f16x3<IBEM_F16_BK, IBEM_F16_EPI>), grid, dim3(256), 0, st, M, N, K, w.Ah,
+                     w.Al, w.Bh, w.Bl, w.ia, w.ib, C, ldc);
 }
 
 // C[M x N] -= A[M x K] B[K x N] (column-major FP32, K <= 256) through the split FP16 products.
