@@ -535,14 +535,51 @@ static TsWs ts_ws(double* base, long n) {
   w.ndest = w.dest + kPanelW;
   return w;
 }
+// Pivot keys: 64-bit, larger wins. High word: the magnitude's top 32 bits (float bits for FP32
+// values; bits 62..31 of the double for FP64 values, i.e. exponent and 21 mantissa bits, monotonic
+// for non-negative values), low word: ~row, so equal magnitudes prefer the smaller row. 0 = no row.
+__device__ inline unsigned long long ts_key(float v, int row) {
+  return ((unsigned long long)__float_as_uint(fabsf(v)) << 32) | (unsigned)(~row);
+}
+__device__ inline unsigned long long ts_key(double v, int row) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v));
+  return ((b >> 31) << 32) | (unsigned)(~row);
+}
+template <int CTRL>
+__device__ inline unsigned long long dpp_u64(unsigned long long k) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)k, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(k >> 32), CTRL, 0xF, 0xF, false);
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+__device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
+// maximum over the 64 lanes (all lanes active), returned in every lane: within each row of 16 lanes
+// by DPP (quad permutations, half-row and row mirrors), then across the four rows by readlane
+__device__ inline unsigned long long wave_max_u64(unsigned long long k) {
+  k = umax64(k, dpp_u64<0xB1>(k));   // quad_perm [1,0,3,2]
+  k = umax64(k, dpp_u64<0x4E>(k));   // quad_perm [2,3,0,1]
+  k = umax64(k, dpp_u64<0x141>(k));  // row_half_mirror
+  k = umax64(k, dpp_u64<0x140>(k));  // row_mirror
+  unsigned long long r = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)k, 16 * q);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(k >> 32), 16 * q);
+    r = umax64(r, ((unsigned long long)hi << 32) | lo);
+  }
+  return r;
+}
+
 // Partial-pivoting elimination of the workgroup's rows (thread = row), choosing up to w pivot rows.
 // a: the row's values, updated in place (multipliers left of the pivot column, reduced values right
 // of it); mystep: the step at which this row was chosen, or -1. Returns the number of steps that
-// found a row.
+// found a row. Per step: a wave maximum of the pivot keys, whose winning lane also publishes its row
+// in the wave's LDS slot; one barrier; every thread takes the best of the four waves and eliminates
+// with that slot's row. Buffers alternate by step parity, so one barrier per step suffices.
 template <typename T>
-__device__ __forceinline__ int ts_select(T (&a)[kPanelW], bool valid, int gidx, int w, double* sv, int* si, T* sp,
-                         int& mystep) {
+__device__ __forceinline__ int ts_select(T (&a)[kPanelW], bool valid, int gidx, int w,
+                                         unsigned long long* skey, T* sp, int& mystep) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  (void)lane;
   mystep = -1;
   int nsel = 0;
 #pragma unroll
@@ -550,49 +587,39 @@ __device__ __forceinline__ int ts_select(T (&a)[kPanelW], bool valid, int gidx, 
     if (c < w) {
       const int par = c & 1;
       const bool cand = valid && mystep < 0;
-      double v = cand ? fabs((double)a[c]) : -1.0;
-      int id = cand ? gidx : -1;
+      const unsigned long long key = cand ? ts_key(a[c], gidx) : 0ull;
+      const unsigned long long wmax = wave_max_u64(key);
+      if (cand && key == wmax) {
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const double v2 = __shfl_xor(v, o);
-        const int i2 = __shfl_xor(id, o);
-        if (v2 > v || (v2 == v && (unsigned)i2 < (unsigned)id)) {
-          v = v2;
-          id = i2;
-        }
+        for (int k = 0; k < kPanelW; ++k) sp[(par * 4 + wv) * kPanelW + k] = a[k];
       }
-      if (lane == 0) {
-        sv[par * 4 + wv] = v;
-        si[par * 4 + wv] = id;
-      }
+      if ((threadIdx.x & 63) == 0) skey[par * 4 + wv] = wmax;
       __syncthreads();
-      double bv = sv[par * 4];
-      int bi = si[par * 4];
+      unsigned long long best = skey[par * 4];
+      int bw = 0;
 #pragma unroll
       for (int q = 1; q < kTsRows / 64; ++q) {
-        const double v2 = sv[par * 4 + q];
-        const int i2 = si[par * 4 + q];
-        if (v2 > bv || (v2 == bv && (unsigned)i2 < (unsigned)bi)) {
-          bv = v2;
-          bi = i2;
+        const unsigned long long k2 = skey[par * 4 + q];
+        if (k2 > best) {
+          best = k2;
+          bw = q;
         }
       }
-      if (bi >= 0) {
+      if (best != 0ull) {
         nsel = c + 1;
-        if (cand && gidx == bi) {
-          mystep = c;
+        if (cand && key == best) mystep = c;
+        const T* prow = sp + (par * 4 + bw) * kPanelW;
+        // the pivot row into registers first (values past w are zero in every row), then the update
+        T pr[kPanelW];
 #pragma unroll
-          for (int k = 0; k < kPanelW; ++k) sp[par * kPanelW + k] = a[k];
+        for (int k = c; k < kPanelW; ++k) pr[k] = prow[k];
+        if (valid && mystep < 0) {
+          const T piv = pr[c];
+          const T l = piv != T(0) ? a[c] / piv : T(0);
+          a[c] = l;
+#pragma unroll
+          for (int k = c + 1; k < kPanelW; ++k) a[k] -= l * pr[k];
         }
-      }
-      __syncthreads();
-      if (bi >= 0 && valid && mystep < 0) {
-        const T piv = sp[par * kPanelW + c];
-        const T l = piv != T(0) ? a[c] / piv : T(0);
-        a[c] = l;
-#pragma unroll
-        for (int k = c + 1; k < kPanelW; ++k)
-          if (k < w) a[k] -= l * sp[par * kPanelW + k];
       }
     }
   }
@@ -620,9 +647,8 @@ template <typename T>
 __global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, long lda, long n,
                                                      long j0, int w, int P, TsWs ws,
                                                      int* __restrict__ ipiv, int* __restrict__ info) {
-  __shared__ double sv[8];
-  __shared__ int si[8];
-  __shared__ T sp[2 * kPanelW];
+  __shared__ unsigned long long skey[8];
+  __shared__ T sp[2 * 4 * kPanelW];
   __shared__ int s_last;
   __shared__ int s_win[kPanelW];
   __shared__ double s_diag[kPanelW];
@@ -642,18 +668,23 @@ __global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, lo
   long src = -1;
   int mystep = -1, nsel = 0;
   for (;;) {
-    nsel = ts_select<T>(a, valid, gidx, w, sv, si, sp, mystep);
+    nsel = ts_select<T>(a, valid, gidx, w, skey, sp, mystep);
     if (count == 1) break;  // this workgroup holds the root set
     const long slot = off + node;
     if (mystep >= 0) {
+      // the row's original values: all loads first, then the stores (no serialised round trips)
+      double v[kPanelW];
+#pragma unroll
+      for (int c = 0; c < kPanelW; ++c)
+        v[c] = c < w ? (src < 0 ? (double)A[(j0 + c) * lda + gidx] : ws.cand[src * kPanelW + c]) : 0.0;
       double* dst = ws.cand + (slot * kPanelW + mystep) * kPanelW;
-      for (int c = 0; c < w; ++c)
-        dst[c] = src < 0 ? (double)A[(j0 + c) * lda + gidx] : ws.cand[src * kPanelW + c];
+#pragma unroll
+      for (int c = 0; c < kPanelW; ++c) dst[c] = v[c];
       ws.cidx[slot * kPanelW + mystep] = gidx;
     }
     if (t >= nsel && t < kPanelW) ws.cidx[slot * kPanelW + t] = -1;
-    __threadfence();
     __syncthreads();
+    if (t == 0) __threadfence();  // release: the workgroup's candidates, before the arrival count
     const int group = node / kTsFan;
     const int gsize = min(kTsFan, count - group * kTsFan);
     const long next_off = off + count;
@@ -663,8 +694,11 @@ __global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, lo
     }
     __syncthreads();
     if (!s_last) return;
-    __threadfence();
-    if (t == 0) ws.cnt[next_off + group] = 0;  // ready for the next leaf (a later launch)
+    if (t == 0) {
+      __threadfence();  // acquire: the group's candidates
+      ws.cnt[next_off + group] = 0;  // ready for the next leaf (a later launch)
+    }
+    __syncthreads();
     const int child = group * kTsFan + t / kPanelW, k = t % kPanelW;
     valid = (t / kPanelW) < gsize && k < w;
     gidx = valid ? ws.cidx[(off + child) * kPanelW + k] : -1;
@@ -690,43 +724,54 @@ __global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, lo
   }
   if (t == 0) s_nsel = nsel;
   __syncthreads();
-  if (t == 0) {
-    const int ns = s_nsel;
-    for (int i = 0; i < w; ++i) {
-      const bool ok = i < ns;
-      if ((!ok || s_diag[i] == 0.0) && *info == 0) *info = (int)(j0 + i + 1);
-    }
-    // sequential interchanges: step i brings original row s_win[i] to position j0 + i
-    int np = 0;
+  if (t < 64) {
+    // wave 0: lane q keeps slot q of the table (position -> original row) of the <= 2w positions
+    // the interchanges touch; lookups are ballots, so each step costs a few wave instructions
+    const int q = t, ns = s_nsel;
+    const unsigned long long zero_piv =
+        __ballot(q < w && (q >= ns || s_diag[q < kPanelW ? q : 0] == 0.0));
+    if (zero_piv && q == 0 && *info == 0) *info = (int)(j0 + __ffsll((long long)zero_piv));
+    int pos = -1, who = -1, np = 0;
     for (int i = 0; i < w; ++i) {
       const int top = (int)(j0 + i);
       int p = top;
       if (i < ns) {
         const int rw = s_win[i];
-        p = rw;
-        for (int q = 0; q < np; ++q)
-          if (s_who[q] == rw) {
-            p = s_pos[q];
-            break;
-          }
+        const unsigned long long mw = __ballot(q < np && who == rw);
+        p = mw ? __shfl(pos, __ffsll((long long)mw) - 1) : rw;
       }
-      ipiv[top] = p;
+      if (q == 0) ipiv[top] = p;
       if (p != top) {
-        const int w1 = ts_who(s_pos, s_who, np, top), w2 = ts_who(s_pos, s_who, np, p);
-        ts_set(s_pos, s_who, np, top, w2);
-        ts_set(s_pos, s_who, np, p, w1);
+        const unsigned long long mt = __ballot(q < np && pos == top);
+        const unsigned long long mp = __ballot(q < np && pos == p);
+        const int w1 = mt ? __shfl(who, __ffsll((long long)mt) - 1) : top;
+        const int w2 = mp ? __shfl(who, __ffsll((long long)mp) - 1) : p;
+        int lt = mt ? __ffsll((long long)mt) - 1 : np;
+        if (!mt) ++np;
+        int lp = mp ? __ffsll((long long)mp) - 1 : np;
+        if (!mp) ++np;
+        if (q == lt) {
+          pos = top;
+          who = w2;
+        }
+        if (q == lp) {
+          pos = p;
+          who = w1;
+        }
       }
     }
     // positions below the leaf's top rows that now hold a (displaced) top row
-    int nd = 0;
-    for (int q = 0; q < np; ++q)
-      if (s_pos[q] >= j0 + w && s_who[q] != s_pos[q]) {
-        ws.dest[nd] = s_pos[q];
-        s_src[nd] = s_who[q];
-        ++nd;
-      }
-    *ws.ndest = nd;
-    s_nd = nd;
+    const bool moved = q < np && pos >= j0 + w && who != pos;
+    const unsigned long long mm = __ballot(moved);
+    const int e = __popcll(mm & ((1ull << q) - 1ull));
+    if (moved) {
+      ws.dest[e] = pos;
+      s_src[e] = who;
+    }
+    if (q == 0) {
+      *ws.ndest = __popcll(mm);
+      s_nd = __popcll(mm);
+    }
   }
   __syncthreads();
   for (int u = t; u < s_nd * kPanelW; u += kTsRows) {
