@@ -72,19 +72,20 @@ int fail(ibem3d_ctx* c, int code, const std::string& msg) {
       return fail(ctx, IBEM3D_EHIP, std::string(#call) + ": " + hipGetErrorString(_e));    \
   } while (0)
 
-void tick(void* p, const char* name, int end) {
+void tick(void* p, const char* name, int end, hipStream_t s) {
   ibem3d_ctx* c = (ibem3d_ctx*)p;
+  if (!s) s = c->st;
   if (!end) {
     hipEvent_t a;
     hipEventCreate(&a);
-    hipEventRecord(a, c->st);
+    hipEventRecord(a, s);
     c->open[name] = a;
   } else {
     auto it = c->open.find(name);
     if (it == c->open.end()) return;
     hipEvent_t b;
     hipEventCreate(&b);
-    hipEventRecord(b, c->st);
+    hipEventRecord(b, s);
     c->times.push_back({name, it->second, b});
     c->open.erase(it);
   }
@@ -93,8 +94,8 @@ void tick(void* p, const char* name, int end) {
 struct Scope {
   ibem3d_ctx* c;
   const char* n;
-  Scope(ibem3d_ctx* c_, const char* n_) : c(c_), n(n_) { tick(c, n, 0); }
-  ~Scope() { tick(c, n, 1); }
+  Scope(ibem3d_ctx* c_, const char* n_) : c(c_), n(n_) { tick(c, n, 0, nullptr); }
+  ~Scope() { tick(c, n, 1, nullptr); }
 };
 
 template <typename T>
@@ -251,6 +252,9 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info, ctx->W.d_perm, ctx->W.d_inv, ctx->W.d_tmp};
   for (void* q : ptrs)
     if (q) hipFree(q);
+  if (ctx->W.st2) hipStreamDestroy(ctx->W.st2);
+  if (ctx->W.ev_a) hipEventDestroy(ctx->W.ev_a);
+  if (ctx->W.ev_b) hipEventDestroy(ctx->W.ev_b);
   if (ctx->st) hipStreamDestroy(ctx->st);
   delete ctx;
 }

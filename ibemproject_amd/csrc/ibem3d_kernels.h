@@ -47,7 +47,11 @@ void gemm_f32_sub_f16x3(long M, long N, int K, const float* A, long lda, const f
 // Blocked right-looking LU with row pivoting (tournament pivoting inside 32-column leaves), in
 // place; ipiv[k] = row swapped with row k (absolute, 0-based, applied in order). d_info receives
 // the first column + 1 whose pivot is exactly zero. Returns nonzero on a launch error.
+// timing callback: (context, phase name, 0 = begin / 1 = end, stream the phase runs on)
+typedef void (*LuTick)(void*, const char*, int, hipStream_t);
 struct LuWork {
+  hipStream_t st2 = nullptr;             // second stream: look-ahead panels
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
   int* d_ipiv = nullptr;   // n
   double* d_ws = nullptr;  // lu_workspace_doubles(n), zero-filled once at allocation
   int* d_info = nullptr;
@@ -57,10 +61,10 @@ struct LuWork {
 };
 size_t lu_workspace_doubles(long n);
 size_t lu_inverse_doubles(long n);
-int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, void (*tick)(void*, const char*, int), void* tctx);
+int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx);
 // f16ws != nullptr: the trailing updates run as gemm_f32_sub_f16x3 (workspace of
 // f16x3_workspace_doubles(n) doubles); otherwise on FP32 MFMA
-int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, void (*tick)(void*, const char*, int), void* tctx, double* f16ws);
+int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx, double* f16ws);
 // After a factorisation: gather list of the interchanges and diagonal-block inverses (synchronises
 // the stream once). Returns nonzero on a HIP error.
 int lu_prepare_solve_f64(const double* LU, long n, long lda, LuWork& W, hipStream_t st);

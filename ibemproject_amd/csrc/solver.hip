@@ -474,10 +474,12 @@ static void f16x3_split(long M, long N, int K, const float* A, long lda, const f
   hipLaunchKernelGGL(k_split_cols, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, N, K, B, ldb,
                      w.Bh, w.Bl, w.ib);
 }
-static void f16x3_gemm(long M, long N, int K, const F16x3Ws& w, float* C, long ldc, hipStream_t st) {
+// C[M x N] -= split A times the split B columns [c0, c0 + N)
+static void f16x3_gemm(long M, long N, int K, const F16x3Ws& w, float* C, long ldc, hipStream_t st,
+                       long c0 = 0) {
   dim3 grid((unsigned)((M + kBM - 1) / kBM), (unsigned)((N + kBN - 1) / kBN));
   hipLaunchKernelGGL((k_gemm_f16x3<IBEM_F16_BK, IBEM_F16_EPI>), grid, dim3(256), 0, st, M, N, K, w.Ah,
-                     w.Al, w.Bh, w.Bl, w.ia, w.ib, C, ldc);
+                     w.Al, w.Bh + c0 * kSplitPadK, w.Bl + c0 * kSplitPadK, w.ia, w.ib + c0, C, ldc);
 }
 
 // C[M x N] -= A[M x K] B[K x N] (column-major FP32, K <= 256) through the split FP16 products.
@@ -901,7 +903,7 @@ struct LuCtx {
   long n, lda;
   LuWork* W;
   hipStream_t st;
-  void (*tick)(void*, const char*, int);
+  LuTick tick;
   void* tctx;
   int err;
 };
@@ -936,52 +938,79 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
 
 constexpr long kLuNB = 256;
 
+// Trailing update of step k restricted to columns [c0, c0 + nc) of the trailing matrix (relative to
+// column k + kb): C -= L21 U12(:, cols).
 template <typename T>
-static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st,
-                     void (*tick)(void*, const char*, int), void* tctx, double* f16ws = nullptr) {
+static void trailing_update(T* A, long n, long lda, long k, long kb, long c0, long nc,
+                            const F16x3Ws* w, hipStream_t st) {
+  const long m = n - k - kb;
+  if (m <= 0 || nc <= 0) return;
+  T* C = A + (k + kb + c0) * lda + k + kb;
+  if (w)
+    f16x3_gemm(m, nc, (int)kb, *w, (float*)C, lda, st, c0);
+  else
+    gemm_sub<T, 1>(m, nc, kb, A + k * lda + k + kb, lda, A + (k + kb + c0) * lda + k, lda, C, lda, st);
+}
+
+// Blocked right-looking LU with look-ahead: after the row interchanges and U12 of step k, the
+// trailing update first covers the next panel's columns; that panel is then factored on a second
+// stream while the rest of the trailing update runs on the main stream. The two never touch the
+// same columns: the panel works inside columns [k + kb, k + 2 kb), the rest of the update writes
+// columns >= k + 2 kb and reads L21 (columns [k, k + kb)) and U12 (rows [k, k + kb)).
+template <typename T>
+static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
+                     double* f16ws = nullptr) {
+  if (!W.st2) {
+    if (hipStreamCreateWithFlags(&W.st2, hipStreamNonBlocking) != hipSuccess) return -1;
+    if (hipEventCreateWithFlags(&W.ev_a, hipEventDisableTiming) != hipSuccess) return -1;
+    if (hipEventCreateWithFlags(&W.ev_b, hipEventDisableTiming) != hipSuccess) return -1;
+  }
   LuCtx<T> c{A, n, lda, &W, st, tick, tctx, 0};
+  LuCtx<T> c2{A, n, lda, &W, W.st2, tick, tctx, 0};
   hipMemsetAsync(W.d_info, 0, sizeof(int), st);
+  const F16x3Ws wsp = f16ws ? f16x3_ws(f16ws, n) : F16x3Ws{};
+  const F16x3Ws* w = f16ws ? &wsp : nullptr;
+  if (tick) tick(tctx, "lu.panel", 0, st);
+  panel_rec(c, 0, std::min(kLuNB, n));
+  if (tick) tick(tctx, "lu.panel", 1, st);
   for (long k = 0; k < n; k += kLuNB) {
     const long kb = std::min(kLuNB, n - k);
-    if (tick) tick(tctx, "lu.panel", 0);
-    panel_rec(c, k, kb);
-    if (tick) tick(tctx, "lu.panel", 1);
-    if (c.err) return -1;
-    if (tick) tick(tctx, "lu.swap_trsm", 0);
+    if (k > 0) hipStreamWaitEvent(st, W.ev_b, 0);  // panel k, factored on the second stream
+    if (tick) tick(tctx, "lu.swap_trsm", 0, st);
     laswp<T>(A, lda, 0, k, W.d_ipiv, k, k + kb, st);
     if (k + kb < n) {
       laswp<T>(A, lda, k + kb, n - k - kb, W.d_ipiv, k, k + kb, st);
       trsm_lower_unit<T>(A + k * lda + k, lda, kb, A + (k + kb) * lda + k, lda, n - k - kb, st);
     }
-    if (tick) tick(tctx, "lu.swap_trsm", 1);
-    if (k + kb < n) {
-      const long m = n - k - kb;
-      if (f16ws) {
-        const F16x3Ws w = f16x3_ws(f16ws, n);
-        if (tick) tick(tctx, "lu.split", 0);
-        f16x3_split(m, m, (int)kb, (const float*)(A + k * lda + k + kb), lda,
-                    (const float*)(A + (k + kb) * lda + k), lda, w, st);
-        if (tick) tick(tctx, "lu.split", 1);
-        if (tick) tick(tctx, "lu.gemm", 0);
-        f16x3_gemm(m, m, (int)kb, w, (float*)(A + (k + kb) * lda + k + kb), lda, st);
-        if (tick) tick(tctx, "lu.gemm", 1);
-      } else {
-        if (tick) tick(tctx, "lu.gemm", 0);
-        gemm_sub<T, 1>(m, m, kb, A + k * lda + k + kb, lda, A + (k + kb) * lda + k, lda,
-                       A + (k + kb) * lda + k + kb, lda, st);
-        if (tick) tick(tctx, "lu.gemm", 1);
-      }
+    if (tick) tick(tctx, "lu.swap_trsm", 1, st);
+    if (k + kb >= n) break;
+    const long m = n - k - kb;
+    const long kb2 = std::min(kLuNB, m);  // width of the next panel
+    if (w) {
+      if (tick) tick(tctx, "lu.split", 0, st);
+      f16x3_split(m, m, (int)kb, (const float*)(A + k * lda + k + kb), lda,
+                  (const float*)(A + (k + kb) * lda + k), lda, *w, st);
+      if (tick) tick(tctx, "lu.split", 1, st);
     }
+    if (tick) tick(tctx, "lu.gemm", 0, st);
+    trailing_update<T>(A, n, lda, k, kb, 0, kb2, w, st);
+    hipEventRecord(W.ev_a, st);
+    hipStreamWaitEvent(W.st2, W.ev_a, 0);
+    if (tick) tick(tctx, "lu.panel", 0, W.st2);
+    panel_rec(c2, k + kb, kb2);
+    if (tick) tick(tctx, "lu.panel", 1, W.st2);
+    hipEventRecord(W.ev_b, W.st2);
+    trailing_update<T>(A, n, lda, k, kb, kb2, m - kb2, w, st);
+    if (tick) tick(tctx, "lu.gemm", 1, st);
   }
   return 0;
 }
 
-int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st,
-                  void (*tick)(void*, const char*, int), void* tctx) {
+int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx) {
   return lu_factor<double>(A, n, lda, W, st, tick, tctx);
 }
-int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st,
-                  void (*tick)(void*, const char*, int), void* tctx, double* f16ws) {
+int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
+                  double* f16ws) {
   return lu_factor<float>(A, n, lda, W, st, tick, tctx, f16ws);
 }
 
