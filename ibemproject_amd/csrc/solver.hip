@@ -193,8 +193,11 @@ void gemm_f32_sub(long M, long N, long K, const float* A, long lda, const float*
 // to 2^-28 of their row / column maximum. Three v_mfma_f32_16x16x32_f16 (2.5 PFLOP/s dense)
 // replace 8 v_mfma_f32_16x16x4f32 (157.3 TFLOP/s) per 16x16x32 block.
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-constexpr int kSplitK = 256;     // operand depth (the LU block size)
-constexpr int kSplitPadK = 256;  // row stride of the split operands, in halves
+#ifndef IBEM_LU_NB
+#define IBEM_LU_NB 512
+#endif
+constexpr int kSplitK = IBEM_LU_NB;     // operand depth (the LU block size)
+constexpr int kSplitPadK = IBEM_LU_NB;  // row stride of the split operands, in halves
 
 __device__ inline float pow2_scale(float amax) {
   // 2^(15 - e) with amax in [2^(e-1), 2^e); 1 for amax == 0
@@ -210,34 +213,37 @@ __device__ inline void split2(float x, _Float16& hi, _Float16& lo) {
 }
 
 // A operand: rows r < M of the column-major M x K block at A (lda). Output row-major [M][K] hi / lo
-// (k contiguous) and the row's inverse scale. Workgroup = 64 rows; LDS transpose.
+// (k contiguous) and the row's inverse scale. Workgroup = kSplitRW rows; LDS transpose.
+constexpr int kSplitRW = 64 * 256 / kSplitK;  // rows per workgroup (the tile stays at 64 KB)
 __global__ __launch_bounds__(256) void k_split_rows(long M, int K, const float* __restrict__ A,
                                                     long lda, _Float16* __restrict__ hi,
                                                     _Float16* __restrict__ lo,
                                                     float* __restrict__ inv_scale) {
-  __shared__ float tile[kSplitK][65];
-  __shared__ float red[4][64];
-  __shared__ float ssc[64];
-  const long r0 = (long)blockIdx.x * 64;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const long r = r0 + lane;
+  constexpr int NG = 256 / kSplitRW;  // k groups
+  __shared__ float tile[kSplitK][kSplitRW + 1];
+  __shared__ float red[NG][kSplitRW];
+  __shared__ float ssc[kSplitRW];
+  const long r0 = (long)blockIdx.x * kSplitRW;
+  const int t = threadIdx.x, rl0 = t % kSplitRW, g = t / kSplitRW;
+  const long r = r0 + rl0;
   float m = 0.0f;
-  for (int k = w; k < K; k += 4) {
+  for (int k = g; k < K; k += NG) {
     const float v = r < M ? A[(long)k * lda + r] : 0.0f;
-    tile[k][lane] = v;
+    tile[k][rl0] = v;
     m = fmaxf(m, fabsf(v));
   }
-  red[w][lane] = m;
+  red[g][rl0] = m;
   __syncthreads();
-  if (w == 0) {
-    const float mm = fmaxf(fmaxf(red[0][lane], red[1][lane]), fmaxf(red[2][lane], red[3][lane]));
+  if (g == 0) {
+    float mm = red[0][rl0];
+    for (int q = 1; q < NG; ++q) mm = fmaxf(mm, red[q][rl0]);
     const float sc = pow2_scale(mm);
-    ssc[lane] = sc;
+    ssc[rl0] = sc;
     if (r < M) inv_scale[r] = 1.0f / sc;
   }
   __syncthreads();
   // write: thread handles 8 consecutive k of one row per pass (16-byte stores)
-  for (int u = t; u < 64 * (kSplitPadK / 8); u += 256) {
+  for (int u = t; u < kSplitRW * (kSplitPadK / 8); u += 256) {
     const int rl = u / (kSplitPadK / 8), k8 = (u % (kSplitPadK / 8)) * 8;
     const long rr = r0 + rl;
     if (rr >= M) continue;
@@ -261,15 +267,16 @@ __global__ __launch_bounds__(256) void k_split_cols(long N, int K, const float* 
                                                     long ldb, _Float16* __restrict__ hi,
                                                     _Float16* __restrict__ lo,
                                                     float* __restrict__ inv_scale) {
+  constexpr int VPL = kSplitPadK / 64;  // values per lane (a multiple of 4)
   const long c = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (c >= N) return;
   const float* col = B + c * ldb;
-  float v[kSplitPadK / 64];
+  float v[VPL];
   float m = 0.0f;
 #pragma unroll
-  for (int q = 0; q < kSplitPadK / 64; ++q) {
-    const int k = lane * (kSplitPadK / 64) + q;
+  for (int q = 0; q < VPL; ++q) {
+    const int k = lane * VPL + q;
     v[q] = k < K ? col[k] : 0.0f;
     m = fmaxf(m, fabsf(v[q]));
   }
@@ -278,16 +285,19 @@ __global__ __launch_bounds__(256) void k_split_cols(long N, int K, const float* 
   const float sc = pow2_scale(m);
   if (lane == 0) inv_scale[c] = 1.0f / sc;
   typedef _Float16 half4 __attribute__((ext_vector_type(4)));
-  half4 h, l;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    _Float16 a, b;
-    split2(v[q] * sc, a, b);
-    h[q] = a;
-    l[q] = b;
+  for (int q4 = 0; q4 < VPL; q4 += 4) {
+    half4 h, l;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      _Float16 a, b;
+      split2(v[q4 + q] * sc, a, b);
+      h[q] = a;
+      l[q] = b;
+    }
+    *(half4*)(hi + c * kSplitPadK + lane * VPL + q4) = h;
+    *(half4*)(lo + c * kSplitPadK + lane * VPL + q4) = l;
   }
-  *(half4*)(hi + c * kSplitPadK + lane * 4) = h;
-  *(half4*)(lo + c * kSplitPadK + lane * 4) = l;
 }
 
 // C[M x N] (column-major, ldc) -= diag(ia) (Ah Bh^T + Ah Bl^T + Al Bh^T) diag(ib), with A* [M][K]
@@ -469,7 +479,7 @@ static F16x3Ws f16x3_ws(double* ws, long cap) {
 }
 static void f16x3_split(long M, long N, int K, const float* A, long lda, const float* B, long ldb,
                         const F16x3Ws& w, hipStream_t st) {
-  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((M + 63) / 64)), dim3(256), 0, st, M, K, A, lda,
+  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((M + kSplitRW - 1) / kSplitRW)), dim3(256), 0, st, M, K, A, lda,
                      w.Ah, w.Al, w.ia);
   hipLaunchKernelGGL(k_split_cols, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, N, K, B, ldb,
                      w.Bh, w.Bl, w.ib);
@@ -936,7 +946,7 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
   laswp<T>(c.A, lda, j0, w1, c.W->d_ipiv, j0 + w1, j0 + w, c.st);
 }
 
-constexpr long kLuNB = 256;
+constexpr long kLuNB = IBEM_LU_NB;
 
 // Trailing update of step k restricted to columns [c0, c0 + nc) of the trailing matrix (relative to
 // column k + kb): C -= L21 U12(:, cols).
