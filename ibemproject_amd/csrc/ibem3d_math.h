@@ -290,32 +290,29 @@ IBEM_HD double C0(const Mat& M, int i, int j, int k, int l) {
 //   cD [ Psi^w,mnkl - 2 nu0 d_mn Phi^w,kl - (1-nu0)(d_kn Phi^w,ml + d_ln Phi^w,mk
 //        + d_km Phi^w,nl + d_lm Phi^w,nk) ]      (every derivative also carries e)
 IBEM_HD double Dten(const double* tab, int stride, const Mat& M, int w, int k, int l, int m, int n,
-                    const int* e) {
-  int c[3] = {e[0], e[1], e[2]};
-  c[m]++;
-  c[n]++;
-  c[k]++;
-  c[l]++;
-  double v = pot(tab, stride, 1, w, c[0], c[1], c[2]);
-  if (m == n) {
-    int d[3] = {e[0], e[1], e[2]};
-    d[k]++;
-    d[l]++;
-    v -= 2.0 * M.nu0 * pot(tab, stride, 0, w, d[0], d[1], d[2]);
-  }
+                    int e0, int e1, int e2) {
+  // multi-index arithmetic by comparisons and values passed by value: no private array whose
+  // address escapes, so nothing goes to scratch memory
+#define IBEM_C3(a, b) e0 + ((a) == 0) + ((b) == 0), e1 + ((a) == 1) + ((b) == 1), e2 + ((a) == 2) + ((b) == 2)
+  double v = pot(tab, stride, 1, w, e0 + (m == 0) + (n == 0) + (k == 0) + (l == 0),
+                 e1 + (m == 1) + (n == 1) + (k == 1) + (l == 1),
+                 e2 + (m == 2) + (n == 2) + (k == 2) + (l == 2));
+  if (m == n) v -= 2.0 * M.nu0 * pot(tab, stride, 0, w, IBEM_C3(k, l));
   double s = 0.0;
-  if (k == n) { int d[3] = {e[0], e[1], e[2]}; d[m]++; d[l]++; s += pot(tab, stride, 0, w, d[0], d[1], d[2]); }
-  if (l == n) { int d[3] = {e[0], e[1], e[2]}; d[m]++; d[k]++; s += pot(tab, stride, 0, w, d[0], d[1], d[2]); }
-  if (k == m) { int d[3] = {e[0], e[1], e[2]}; d[n]++; d[l]++; s += pot(tab, stride, 0, w, d[0], d[1], d[2]); }
-  if (l == m) { int d[3] = {e[0], e[1], e[2]}; d[n]++; d[k]++; s += pot(tab, stride, 0, w, d[0], d[1], d[2]); }
+  if (k == n) s += pot(tab, stride, 0, w, IBEM_C3(m, l));
+  if (l == n) s += pot(tab, stride, 0, w, IBEM_C3(m, k));
+  if (k == m) s += pot(tab, stride, 0, w, IBEM_C3(n, l));
+  if (l == m) s += pot(tab, stride, 0, w, IBEM_C3(n, k));
+#undef IBEM_C3
   v -= (1.0 - M.nu0) * s;
   return M.cD * v;
 }
 
 // sum_k D(k,k | m,n | w | e)
-IBEM_HD double Dtrace(const double* tab, int stride, const Mat& M, int w, int m, int n, const int* e) {
+IBEM_HD double Dtrace(const double* tab, int stride, const Mat& M, int w, int m, int n, int e0,
+                      int e1, int e2) {
   double v = 0.0;
-  for (int k = 0; k < 3; ++k) v += Dten(tab, stride, M, w, k, k, m, n, e);
+  for (int k = 0; k < 3; ++k) v += Dten(tab, stride, M, w, k, k, m, n, e0, e1, e2);
   return v;
 }
 
@@ -331,8 +328,8 @@ struct Slot {
   int w;      // weight id for columns
 };
 IBEM_HD Slot slot60(int r) {
+  // components set by comparisons (an indexed store into e[] would put the struct in scratch)
   Slot s;
-  s.e[0] = s.e[1] = s.e[2] = 0;
   s.p = s.q = -1;
   if (r < 6) {
     s.order = 0;
@@ -345,7 +342,6 @@ IBEM_HD Slot slot60(int r) {
     s.i = pair_i(t);
     s.j = pair_j(t);
     s.p = (r - 6) % 3;
-    s.e[s.p]++;
     s.w = 1 + s.p;
   } else {
     s.order = 2;
@@ -354,10 +350,11 @@ IBEM_HD Slot slot60(int r) {
     s.j = pair_j(t);
     s.p = pair_i(kl);
     s.q = pair_j(kl);
-    s.e[s.p]++;
-    s.e[s.q]++;
     s.w = 4 + kl;
   }
+  s.e[0] = (s.p == 0) + (s.q == 0);
+  s.e[1] = (s.p == 1) + (s.q == 1);
+  s.e[2] = (s.p == 2) + (s.q == 2);
   return s;
 }
 
@@ -373,8 +370,8 @@ IBEM_HD double incl_entry(const double* tab, int stride, const Mat& M, bool self
   const Slot Cc = slot60(c);
   const int m = Cc.i, n = Cc.j, i = R.i, j = R.j;
   const double mp = mapw(m, n);
-  double sum = 2.0 * M.dmu * Dten(tab, stride, M, Cc.w, i, j, m, n, R.e);
-  if (i == j) sum += M.dlam * Dtrace(tab, stride, M, Cc.w, m, n, R.e);
+  double sum = 2.0 * M.dmu * Dten(tab, stride, M, Cc.w, i, j, m, n, R.e[0], R.e[1], R.e[2]);
+  if (i == j) sum += M.dlam * Dtrace(tab, stride, M, Cc.w, m, n, R.e[0], R.e[1], R.e[2]);
   const double fac = (R.order == 2 ? 0.5 : 1.0) * ((Cc.order == 2 && Cc.p != Cc.q) ? 2.0 : 1.0);
   double v = -fac * sum;
   if (self && R.order == Cc.order && R.p == Cc.p && R.q == Cc.q) v += C0(M, i, j, m, n);
@@ -413,17 +410,10 @@ IBEM_HD double field_entry(const RadPoint& P, bool inside, const double C[8][4],
                            int i, int col) {
   const Slot Cc = slot60(col);
   const int m = Cc.i, n = Cc.j, w = Cc.w;
-  auto phi1 = [&](int a) {
-    int c[3] = {0, 0, 0};
-    c[a]++;
-    return pot_direct(P, inside, C, 0, w, c[0], c[1], c[2]);
-  };
+  auto phi1 = [&](int a) { return pot_direct(P, inside, C, 0, w, a == 0, a == 1, a == 2); };
   auto psi3 = [&](int a, int b, int d) {
-    int c[3] = {0, 0, 0};
-    c[a]++;
-    c[b]++;
-    c[d]++;
-    return pot_direct(P, inside, C, 1, w, c[0], c[1], c[2]);
+    return pot_direct(P, inside, C, 1, w, (a == 0) + (b == 0) + (d == 0), (a == 1) + (b == 1) + (d == 1),
+                      (a == 2) + (b == 2) + (d == 2));
   };
   // sum_{jj,ll} C0(jj,ll,m,n) IG(i,jj,ll) = lam0 d_mn sum_k IG(i,k,k) + mu0 (IG(i,m,n) + IG(i,n,m))
   double v = 0.0;

@@ -84,8 +84,7 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
     if (c < 60) {
       // strain e_ij += D(i,j | m,n | w) map_mn (x2 for merged quadratic unknowns) X_c
       const Slot Cc = slot60(c);
-      const int e0[3] = {0, 0, 0};
-      double d = Dten(tab, kPostTab, M, Cc.w, pair_i(ij), pair_j(ij), Cc.i, Cc.j, e0);
+      double d = Dten(tab, kPostTab, M, Cc.w, pair_i(ij), pair_j(ij), Cc.i, Cc.j, 0, 0, 0);
       const double f = mapw(Cc.i, Cc.j) * ((Cc.order == 2 && Cc.p != Cc.q) ? 2.0 : 1.0);
       ein += d * f * E60[60L * h + c];
     }
