@@ -47,46 +47,74 @@ __global__ __launch_bounds__(64) void k_qpoints(const double* __restrict__ nodes
 }
 
 // ---- K1: inclusion-inclusion blocks -----------------------------------------------------------
-// Workgroup = kInclTile consecutive target spheres (rows) x one source sphere h (columns). The eight
-// potential tables of each pair (1320 doubles) are built cooperatively in LDS; then thread t owns
-// row t of the tile (rows of consecutive targets are contiguous) and writes the 60 columns, one
-// coalesced column segment per iteration.
+// Workgroup = kInclTile consecutive target spheres (rows) x one source sphere h (columns). Per pair:
+// the four radial basis tables (d^beta of r, r^-1 outside; of rho .. rho^4 inside, |beta| <= 8) and
+// the potential coefficients, then the weighted tables Psi^w,g / Phi^w,g for the ten weights, all in
+// LDS; then thread = row of the tile: its table positions once (incl_row), and the 60 columns with
+// the column decode resolved at compile time (incl_entry_w), one coalesced column segment per
+// column (rows of consecutive targets are contiguous).
 __global__ __launch_bounds__(256) void k_incl(const double* __restrict__ cen,
                                               const double* __restrict__ rad, int num, long n,
                                               long row0, Mat M, int rowmask,
                                               double* __restrict__ A) {
-  __shared__ double tab[kInclTile][8 * kTabN];
+  __shared__ double sbasis[kInclTile][4 * kTabN];
+  __shared__ double sPS[kInclTile][kWgt * kPS];
+  __shared__ double sPH[kInclTile][kWgt * kPH];
+  __shared__ double sC[kInclTile][8][4];
   const int h = blockIdx.y;
   const int s0 = blockIdx.x * kInclTile;
   const double ch0 = cen[3 * h], ch1 = cen[3 * h + 1], ch2 = cen[3 * h + 2];
   const double a = rad[h];
-  for (int u = threadIdx.x; u < kInclTile * kTabN; u += blockDim.x) {
-    const int ps = u / kTabN, t = u % kTabN;
+  const int t = threadIdx.x;
+  if (t < kInclTile && s0 + t < num) {
+    const int s = s0 + t;
+    const double x[3] = {cen[3 * s] - ch0, cen[3 * s + 1] - ch1, cen[3 * s + 2] - ch2};
+    double C[8][4];
+    pot_coefs(a, is_inside(x, a), C);
+#pragma unroll
+    for (int f = 0; f < 8; ++f)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) sC[t][f][b] = C[f][b];
+  }
+  for (int u = t; u < kInclTile * kTabN; u += blockDim.x) {
+    const int ps = u / kTabN, tt = u % kTabN;
     const int s = s0 + ps;
     if (s >= num) continue;
     const double x[3] = {cen[3 * s] - ch0, cen[3 * s + 1] - ch1, cen[3 * s + 2] - ch2};
     RadPoint P;
     rad_point(P, x);
-    const bool inside = is_inside(x, a);
-    double C[8][4];
-    pot_coefs(a, inside, C);
-    double v[8];
-    pot_entry(P, inside, C, t, v);
-#pragma unroll
-    for (int f = 0; f < 8; ++f) tab[ps][f * kTabN + t] = v[f];
+    int b0, b1, b2;
+    tdecode(tt, b0, b1, b2);
+    double B0, B1, B2, B3;
+    rad_basis(P, is_inside(x, a), b0, b1, b2, B0, B1, B2, B3);
+    sbasis[ps][tt] = B0;
+    sbasis[ps][kTabN + tt] = B1;
+    sbasis[ps][2 * kTabN + tt] = B2;
+    sbasis[ps][3 * kTabN + tt] = B3;
   }
   __syncthreads();
-  const int rl = threadIdx.x;
-  if (rl >= kInclTile * 60) return;
-  const int ps = rl / 60, r = rl % 60;
+  for (int u = t; u < kInclTile * kInclW; u += blockDim.x) {
+    const int ps = u / kInclW, v = u % kInclW;
+    if (s0 + ps >= num) continue;
+    incl_weighted(sbasis[ps], sC[ps], v, sPS[ps], sPH[ps]);
+  }
+  __syncthreads();
+  const int ps = t / 64, r = t % 64;
   const int s = s0 + ps;
-  if (s >= num) return;
-  const int ord = r < 6 ? 0 : (r < 24 ? 1 : 2);
-  if (!((rowmask >> ord) & 1)) return;
+  if (r >= 60 || s >= num) return;
+  const InclRow R = incl_row(r);
+  if (!((rowmask >> R.order) & 1)) return;
   const long row = row0 + 60L * s + r;
   double* Acol = A + (row0 + 60L * h) * n + row;
   const bool self = (s == h);
-  for (int c = 0; c < 60; ++c) Acol[(long)c * n] = incl_entry(tab[ps], kTabN, M, self, r, c);
+  const double* PS = sPS[ps];
+  const double* PH = sPH[ps];
+  for (int mn = 0; mn < 6; ++mn) {
+    const InclCol K = incl_col(R, mn);
+#pragma unroll
+    for (int w = 0; w < kWgt; ++w)
+      Acol[(long)incl_colidx(mn, w) * n] = incl_entry_mw(R, K, PS, PH, M, self, w);
+  }
 }
 
 // ---- K3: boundary rows x eigenstrain columns ----------------------------------------------------

@@ -512,6 +512,166 @@ IBEM_HD void kelvin_ut(const double* dxr, double r, double E, double nu, const d
     }
 }
 
+// ---- inclusion-inclusion block from per-pair weighted tables -----------------------------------
+// The same entries as incl_entry, regrouped so that a row's table positions are computed once:
+//   PS_w[g] = Psi^w,g  (|g| <= 6)   and   PH_w[g] = Phi^w,g  (|g| <= 4)
+// for the ten weights w, built per pair from the radial basis tables; then, for row (i,j,e) and
+// column (m,n,w),
+//   D(i,j) = cD [ PS_w(e+i+j+m+n) - 2 nu d_mn PH_w(e+i+j)
+//                 - (1-nu) (d_in PH_w(e+m+j) + d_jn PH_w(e+m+i) + d_im PH_w(e+n+j) + d_jm PH_w(e+n+i)) ]
+//   sum_k D(k,k) = cD [ sum_k PS_w(e+m+n+2k) - 2 nu d_mn sum_k PH_w(e+2k) - 4 (1-nu) PH_w(e+m+n) ]
+constexpr int kPS = 84, kPH = 35, kWgt = 10;
+constexpr int kInclW = kWgt * (kPS + kPH);  // weighted entries per pair
+
+// Basis table value of radial function f at multi-index (b0,b1,b2): basis is [4][kTabN].
+IBEM_HD double fbasis(const double* basis, const double (*C)[4], int f, int b0, int b1, int b2) {
+  const int t = tidx(b0, b1, b2);
+  return C[f][0] * basis[t] + C[f][1] * basis[kTabN + t] + C[f][2] * basis[2 * kTabN + t] +
+         C[f][3] * basis[3 * kTabN + t];
+}
+
+// One weighted entry u (0 <= u < kInclW): PS (u < 840) or PH, written to PS / PH.
+IBEM_HD void incl_weighted(const double* basis, const double (*C)[4], int u, double* PS, double* PH) {
+  const bool psi = u < kWgt * kPS;
+  const int v = psi ? u : u - kWgt * kPS;
+  const int per = psi ? kPS : kPH;
+  const int w = v / per, g = v % per;
+  int c0, c1, c2;
+  tdecode(g, c0, c1, c2);
+  const int base = psi ? 4 : 0;
+  double val;
+  if (w == 0) {
+    val = fbasis(basis, C, base, c0, c1, c2);
+  } else if (w < 4) {
+    const int p = w - 1;
+    val = fbasis(basis, C, base + 1, c0 + (p == 0), c1 + (p == 1), c2 + (p == 2));
+  } else {
+    const int p = pair_i(w - 4), q = pair_j(w - 4);
+    val = fbasis(basis, C, base + 3, c0 + (p == 0) + (q == 0), c1 + (p == 1) + (q == 1),
+                 c2 + (p == 2) + (q == 2));
+    if (p == q) val += fbasis(basis, C, base + 2, c0, c1, c2);
+  }
+  if (psi)
+    PS[w * kPS + g] = val;
+  else
+    PH[w * kPH + g] = val;
+}
+
+// Table positions of one row (i, j, e), computed once and reused for all 60 columns.
+struct InclRow {
+  int iS[6];    // e + i + j + mn          (PS), mn in pair order
+  int iT[18];   // e + mn + 2k             (PS), 3 mn + k
+  int iXi[3];   // e + x + i               (PH)
+  int iXj[3];   // e + x + j               (PH)
+  int imn[6];   // e + mn                  (PH)
+  int i2k[3];   // e + 2k                  (PH)
+  int iij;      // e + i + j               (PH)
+  int i, j, order, p, q;
+  double fac;   // 0.5 for second-order rows
+};
+IBEM_HD InclRow incl_row(int r) {
+  const Slot R = slot60(r);
+  InclRow o;
+  o.i = R.i;
+  o.j = R.j;
+  o.order = R.order;
+  o.p = R.p;
+  o.q = R.q;
+  o.fac = R.order == 2 ? 0.5 : 1.0;
+  const int e0 = R.e[0], e1 = R.e[1], e2 = R.e[2];
+  const int i = R.i, j = R.j;
+  const int a0 = e0 + (i == 0) + (j == 0), a1 = e1 + (i == 1) + (j == 1), a2 = e2 + (i == 2) + (j == 2);
+  o.iij = tidx(a0, a1, a2);
+#pragma unroll
+  for (int u = 0; u < 6; ++u) {
+    const int m = pair_i(u), n = pair_j(u);
+    const int d0 = (m == 0) + (n == 0), d1 = (m == 1) + (n == 1), d2 = (m == 2) + (n == 2);
+    o.iS[u] = tidx(a0 + d0, a1 + d1, a2 + d2);
+    o.imn[u] = tidx(e0 + d0, e1 + d1, e2 + d2);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      o.iT[3 * u + k] = tidx(e0 + d0 + 2 * (k == 0), e1 + d1 + 2 * (k == 1), e2 + d2 + 2 * (k == 2));
+  }
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    o.iXi[x] = tidx(e0 + (x == 0) + (i == 0), e1 + (x == 1) + (i == 1), e2 + (x == 2) + (i == 2));
+    o.iXj[x] = tidx(e0 + (x == 0) + (j == 0), e1 + (x == 1) + (j == 1), e2 + (x == 2) + (j == 2));
+    o.i2k[x] = tidx(e0 + 2 * (x == 0), e1 + 2 * (x == 1), e2 + 2 * (x == 2));
+  }
+  return o;
+}
+
+// select a[k] (k < 6 / k < 3) by comparisons, so a runtime k never indexes a private array
+IBEM_HD int sel6i(const int* a, int k) {
+  return k == 0 ? a[0] : (k == 1 ? a[1] : (k == 2 ? a[2] : (k == 3 ? a[3] : (k == 4 ? a[4] : a[5]))));
+}
+IBEM_HD int sel3i(const int* a, int k) { return k == 0 ? a[0] : (k == 1 ? a[1] : a[2]); }
+
+// Table positions of one row for one eigenstrain pair (m,n) = pair mn, shared by its ten weights.
+struct InclCol {
+  int m, n, iS, iT0, iT1, iT2, imn, iXjm, iXim, iXjn, iXin;
+};
+IBEM_HD InclCol incl_col(const InclRow& R, int mn) {
+  InclCol K;
+  K.m = pair_i(mn);
+  K.n = pair_j(mn);
+  K.iS = sel6i(R.iS, mn);
+  K.imn = sel6i(R.imn, mn);
+  K.iT0 = mn == 0 ? R.iT[0] : (mn == 1 ? R.iT[3] : (mn == 2 ? R.iT[6] : (mn == 3 ? R.iT[9] : (mn == 4 ? R.iT[12] : R.iT[15]))));
+  K.iT1 = mn == 0 ? R.iT[1] : (mn == 1 ? R.iT[4] : (mn == 2 ? R.iT[7] : (mn == 3 ? R.iT[10] : (mn == 4 ? R.iT[13] : R.iT[16]))));
+  K.iT2 = mn == 0 ? R.iT[2] : (mn == 1 ? R.iT[5] : (mn == 2 ? R.iT[8] : (mn == 3 ? R.iT[11] : (mn == 4 ? R.iT[14] : R.iT[17]))));
+  K.iXjm = sel3i(R.iXj, K.m);
+  K.iXim = sel3i(R.iXi, K.m);
+  K.iXjn = sel3i(R.iXj, K.n);
+  K.iXin = sel3i(R.iXi, K.n);
+  return K;
+}
+
+// column of the 60-unknown layout for eigenstrain pair mn and weight w
+IBEM_HD int incl_colidx(int mn, int w) {
+  return w == 0 ? mn : (w < 4 ? 6 + 3 * mn + (w - 1) : 24 + 6 * mn + (w - 4));
+}
+
+// Entry (row R, pair mn via K, weight w); equals incl_entry(r, incl_colidx(mn, w)).
+IBEM_HD double incl_entry_mw(const InclRow& R, const InclCol& K, const double* PS, const double* PH,
+                             const Mat& M, bool self, int w) {
+  const int m = K.m, n = K.n, i = R.i, j = R.j;
+  const double* ps = PS + w * kPS;
+  const double* ph = PH + w * kPH;
+  const double nu = M.nu0;
+  double D = ps[K.iS];
+  if (m == n) D -= 2.0 * nu * ph[R.iij];
+  double t = 0.0;
+  if (i == n) t += ph[K.iXjm];
+  if (j == n) t += ph[K.iXim];
+  if (i == m) t += ph[K.iXjn];
+  if (j == m) t += ph[K.iXin];
+  D -= (1.0 - nu) * t;
+  double sum = 2.0 * M.dmu * M.cD * D;
+  if (i == j) {
+    double tr = ps[K.iT0] + ps[K.iT1] + ps[K.iT2];
+    if (m == n) tr -= 2.0 * nu * (ph[R.i2k[0]] + ph[R.i2k[1]] + ph[R.i2k[2]]);
+    tr -= 4.0 * (1.0 - nu) * ph[K.imn];
+    sum += M.dlam * M.cD * tr;
+  }
+  // column slot of weight w: order 0 (w = 0), 1 (w = 1..3, p = w - 1), 2 (w = 4.., p<=q pair w - 4)
+  const int corder = w == 0 ? 0 : (w < 4 ? 1 : 2);
+  const int cp = w == 0 ? -1 : (w < 4 ? w - 1 : pair_i(w - 4));
+  const int cq = w < 4 ? -1 : pair_j(w - 4);
+  const double fac = R.fac * ((corder == 2 && cp != cq) ? 2.0 : 1.0);
+  double v = -fac * sum;
+  if (self && R.order == corder && R.p == cp && R.q == cq) v += C0(M, i, j, m, n);
+  return mapw(m, n) * v;
+}
+
+// Entry (row R, column c); equals incl_entry.
+IBEM_HD double incl_entry_w(const InclRow& R, const double* PS, const double* PH, const Mat& M,
+                            bool self, int c) {
+  const Slot Cc = slot60(c);
+  const InclCol K = incl_col(R, pair_t(Cc.i, Cc.j));
+  return incl_entry_mw(R, K, PS, PH, M, self, Cc.w);
+}
+
 // ---- boundary -> inclusion strain kernels (get_for_stress_BEM{,1,2}, BEMBuilderElastic3D.cpp) ---
 // Kelvin tensor G_ij,beta = d_ij cg1 d^beta r^-1 - cg2 d^(beta+e_i+e_j) r. With the harmonic and
 // biharmonic identities lap r^-1 = 0, lap r = 2 r^-1, the reference's contractions reduce to

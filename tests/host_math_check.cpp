@@ -112,3 +112,32 @@ void hc_row_gp(double E0, double nu0, double E1, double nu1, const double* x, co
   }
 }
 }
+
+extern "C" {
+// The same 60 x 60 inclusion block through the per-pair weighted tables (what k_incl runs).
+void hc_incl_block_w(double E0, double nu0, double E1, double nu1, const double* x, double a,
+                     int self, double* out) {
+  Mat M = make_mat(E0, nu0, E1, nu1);
+  RadPoint P;
+  rad_point(P, x);
+  const bool inside = is_inside(x, a);
+  double C[8][4];
+  pot_coefs(a, inside, C);
+  std::vector<double> basis(4 * kTabN), PS(kWgt * kPS), PH(kWgt * kPH);
+  for (int t = 0; t < kTabN; ++t) {
+    int b0, b1, b2;
+    tdecode(t, b0, b1, b2);
+    double B0, B1, B2, B3;
+    rad_basis(P, inside, b0, b1, b2, B0, B1, B2, B3);
+    basis[t] = B0;
+    basis[kTabN + t] = B1;
+    basis[2 * kTabN + t] = B2;
+    basis[3 * kTabN + t] = B3;
+  }
+  for (int u = 0; u < kInclW; ++u) incl_weighted(basis.data(), C, u, PS.data(), PH.data());
+  for (int r = 0; r < 60; ++r) {
+    const InclRow R = incl_row(r);
+    for (int c = 0; c < 60; ++c) out[c * 60 + r] = incl_entry_w(R, PS.data(), PH.data(), M, self != 0, c);
+  }
+}
+}

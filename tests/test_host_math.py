@@ -135,3 +135,27 @@ def test_gauss_point_row_tables(hostmath):
     # tidx of degree 3: (3,0,0)=10, (2,1,0)=11, (2,0,1)=12, (1,2,0)=13, (1,1,1)=14, (1,0,2)=15
     assert np.allclose([tab[10], tab[11], tab[12], tab[13], tab[14], tab[15]],
                        [d3[0, 0], d3[0, 1], d3[0, 2], d3[1, 1], d3[1, 2], d3[2, 2]], rtol=1e-7)
+
+
+def test_inclusion_block_weighted_tables(hostmath):
+    """k_incl's regrouped evaluation (per-pair weighted tables, per-row table positions) against the
+    per-entry evaluation, outside, inside and at the centre (self block)."""
+    import ctypes
+    dp = ctypes.POINTER(ctypes.c_double)
+    for name in ("hc_incl_block", "hc_incl_block_w"):
+        getattr(hostmath, name).argtypes = [ctypes.c_double] * 4 + [dp, ctypes.c_double, ctypes.c_int, dp]
+    rng = np.random.default_rng(4)
+    cases = [(np.zeros(3), 0.1, 1)]
+    for _ in range(6):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        a = rng.uniform(0.03, 0.1)
+        cases.append((d * a * rng.uniform(1.2, 8.0), a, 0))
+        cases.append((d * a * rng.uniform(0.1, 0.9), a, 0))
+    for x, a, self_ in cases:
+        x = np.ascontiguousarray(x)
+        o1 = np.zeros(3600)
+        o2 = np.zeros(3600)
+        hostmath.hc_incl_block(1.0, 0.3, 10.0, 0.25, x.ctypes.data_as(dp), a, self_, o1.ctypes.data_as(dp))
+        hostmath.hc_incl_block_w(1.0, 0.3, 10.0, 0.25, x.ctypes.data_as(dp), a, self_, o2.ctypes.data_as(dp))
+        assert np.abs(o1 - o2).max() <= 1e-12 * np.abs(o1).max()
