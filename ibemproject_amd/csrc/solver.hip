@@ -315,7 +315,10 @@ __global__ __launch_bounds__(256) void k_split_cols(long N, int K, const float* 
 #ifndef IBEM_F16_EPI
 #define IBEM_F16_EPI 1
 #endif
-template <int BK, int EPI>
+#ifndef IBEM_F16_PF
+#define IBEM_F16_PF 1
+#endif
+template <int BK, int EPI, int PF>
 __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
                                                        const _Float16* __restrict__ Ah,
                                                        const _Float16* __restrict__ Al,
@@ -324,9 +327,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
                                                        const float* __restrict__ ia,
                                                        const float* __restrict__ ib,
                                                        float* __restrict__ C, long ldc) {
+  (void)K;  // operands are zero-padded to kSplitPadK, so every tile runs all NCH chunks
   constexpr int LS = BK + 8;             // LDS row stride (halves)
   constexpr int VPR = BK / 8;            // 16-byte vectors per row and array
   constexpr int NV = kBM * VPR / 256;    // vectors per thread and array
+  constexpr int NCH = kSplitPadK / BK;   // K chunks
   constexpr int CS = kBM + 4;            // epilogue tile column stride (floats)
   constexpr int SMEM = (EPI && kBN * CS * 2 > 4 * kBM * LS) ? kBN * CS * 2 : 4 * kBM * LS;
   __shared__ __attribute__((aligned(16))) _Float16 smem[SMEM];
@@ -343,37 +348,39 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0, 0, 0, 0};
-  half8 rAh[NV], rAl[NV], rBh[NV], rBl[NV];
-  auto fetch = [&](int k0) {
+  // register ring: chunk c lives in slot c % PF, fetched PF chunks ahead of its use
+  half8 rAh[PF][NV], rAl[PF][NV], rBh[PF][NV], rBl[PF][NV];
+  auto fetch = [&](int k0, int sl) {
 #pragma unroll
     for (int e = 0; e < NV; ++e) {
       const int v = tid + 256 * e, row = v / VPR, part = v % VPR;
       const long ga = m0 + row, gb = n0 + row;
       const long ko = k0 + part * 8;
       const half8 z = {0, 0, 0, 0, 0, 0, 0, 0};
-      rAh[e] = ga < M ? *(const half8*)(Ah + ga * kSplitPadK + ko) : z;
-      rAl[e] = ga < M ? *(const half8*)(Al + ga * kSplitPadK + ko) : z;
-      rBh[e] = gb < N ? *(const half8*)(Bh + gb * kSplitPadK + ko) : z;
-      rBl[e] = gb < N ? *(const half8*)(Bl + gb * kSplitPadK + ko) : z;
+      rAh[sl][e] = ga < M ? *(const half8*)(Ah + ga * kSplitPadK + ko) : z;
+      rAl[sl][e] = ga < M ? *(const half8*)(Al + ga * kSplitPadK + ko) : z;
+      rBh[sl][e] = gb < N ? *(const half8*)(Bh + gb * kSplitPadK + ko) : z;
+      rBl[sl][e] = gb < N ? *(const half8*)(Bl + gb * kSplitPadK + ko) : z;
     }
   };
-  auto stash = [&]() {
+  auto stash = [&](int sl) {
 #pragma unroll
     for (int e = 0; e < NV; ++e) {
       const int v = tid + 256 * e, row = v / VPR, part = v % VPR;
-      *(half8*)(sAh + row * LS + part * 8) = rAh[e];
-      *(half8*)(sAl + row * LS + part * 8) = rAl[e];
-      *(half8*)(sBh + row * LS + part * 8) = rBh[e];
-      *(half8*)(sBl + row * LS + part * 8) = rBl[e];
+      *(half8*)(sAh + row * LS + part * 8) = rAh[sl][e];
+      *(half8*)(sAl + row * LS + part * 8) = rAl[sl][e];
+      *(half8*)(sBh + row * LS + part * 8) = rBh[sl][e];
+      *(half8*)(sBl + row * LS + part * 8) = rBl[sl][e];
     }
   };
-  const int nchunks = (K + BK - 1) / BK;
-  fetch(0);
-  for (int ch = 0; ch < nchunks; ++ch) {
+#pragma unroll
+  for (int c = 0; c < PF && c < NCH; ++c) fetch(c * BK, c);
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
     __syncthreads();
-    stash();
+    stash(ch % PF);
     __syncthreads();
-    if (ch + 1 < nchunks) fetch((ch + 1) * BK);
+    if (ch + PF < NCH) fetch((ch + PF) * BK, ch % PF);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 32) {
       half8 ah[4], al[4], bh[4], bl[4];
@@ -488,7 +495,7 @@ static void f16x3_split(long M, long N, int K, const float* A, long lda, const f
 static void f16x3_gemm(long M, long N, int K, const F16x3Ws& w, float* C, long ldc, hipStream_t st,
                        long c0 = 0) {
   dim3 grid((unsigned)((M + kBM - 1) / kBM), (unsigned)((N + kBN - 1) / kBN));
-  hipLaunchKernelGGL((k_gemm_f16x3<IBEM_F16_BK, IBEM_F16_EPI>), grid, dim3(256), 0, st, M, N, K, w.Ah,
+  hipLaunchKernelGGL((k_gemm_f16x3<IBEM_F16_BK, IBEM_F16_EPI, IBEM_F16_PF>), grid, dim3(256), 0, st, M, N, K, w.Ah,
                      w.Al, w.Bh + c0 * kSplitPadK, w.Bl + c0 * kSplitPadK, w.ia, w.ib + c0, C, ldc);
 }
 
