@@ -46,6 +46,7 @@ struct ibem3d_ctx {
   float* d_A32 = nullptr;
   long lda32 = 0;  // leading dimension of d_A32: a multiple of 64 for aligned 16-byte accesses
   double* d_f16ws = nullptr;  // split operands of the FP16-MFMA trailing update
+  double* d_eqws = nullptr;   // equilibration partial row maxima
   float* d_v32 = nullptr;
   LuWork W;
   unsigned phases = 0;
@@ -248,7 +249,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
   void* ptrs[] = {ctx->d_nodes, ctx->d_cen,      ctx->d_rad,      ctx->d_pts,  ctx->d_qp,
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
-                  ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32, ctx->d_f16ws,
+                  ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32, ctx->d_f16ws, ctx->d_eqws,
                   ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info, ctx->W.d_perm, ctx->W.d_inv, ctx->W.d_tmp};
   for (void* q : ptrs)
     if (q) hipFree(q);
@@ -378,6 +379,7 @@ static int ensure_solver_ws(ibem3d_ctx* ctx) {
     if (int rc = dalloc(ctx, &ctx->W.d_perm, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->W.d_inv, lu_inverse_doubles(n))) return rc;
     if (int rc = dalloc(ctx, &ctx->W.d_tmp, (size_t)n)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_eqws, equilibrate_workspace_doubles(n))) return rc;
     if (int rc = dalloc(ctx, &ctx->d_r, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_c, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_work, (size_t)n)) return rc;
@@ -429,10 +431,9 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
     // keep A for the residual; factor an equilibrated copy R A C
     if (!ctx->d_Af)
       if (int rc = dalloc(ctx, &ctx->d_Af, (size_t)n * n)) return rc;
-    HIPC(hipMemcpyAsync(ctx->d_Af, ctx->d_A, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->st));
     {
       Scope s(ctx, "solve.equilibrate");
-      equilibrate_f64(ctx->d_Af, n, n, ctx->d_r, ctx->d_c, ctx->st);
+      equilibrate_copy_f64(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_Af, n, ctx->d_eqws, ctx->st);
     }
     {
       Scope s(ctx, "solve.lu_f64");
@@ -466,16 +467,11 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
       if (int rc = dalloc(ctx, &ctx->d_A32, (size_t)ctx->lda32 * n)) return rc;
       if (int rc = dalloc(ctx, &ctx->d_v32, (size_t)n)) return rc;
     }
-    // equilibrate a scratch FP64 copy to obtain R, C, then convert R A C to FP32
-    if (!ctx->d_Af) {
-      // equilibration factors from A itself (scan only), scaled copy goes straight to FP32
-      if (int rc = dalloc(ctx, &ctx->d_Af, (size_t)n * n)) return rc;
-    }
-    HIPC(hipMemcpyAsync(ctx->d_Af, ctx->d_A, sizeof(double) * n * n, hipMemcpyDeviceToDevice, ctx->st));
+    // R A C straight from A into the FP32 factor storage (A itself stays for the residual)
     {
       Scope s(ctx, "solve.equilibrate");
-      equilibrate_f64(ctx->d_Af, n, n, ctx->d_r, ctx->d_c, ctx->st);
-      convert_f64_to_f32(ctx->d_Af, n, n, ctx->d_A32, ctx->lda32, ctx->st);
+      equilibrate_copy_f32(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_A32, ctx->lda32, ctx->d_eqws,
+                           ctx->st);
     }
     {
       Scope s(ctx, "solve.lu_f32");

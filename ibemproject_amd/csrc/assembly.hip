@@ -279,8 +279,12 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
                                               const double* __restrict__ uval,
                                               const double* __restrict__ tval,
                                               double* __restrict__ A, double* __restrict__ b) {
-  __shared__ double sq[64 * kQpStride];
-  __shared__ double stab[64 * kKStride];  // tables; then the wavefront partial sums
+  constexpr int QN = 64 * kQpStride;             // quadrature data of one element
+  constexpr int QPT = (QN + 255) / 256;           // per thread
+  __shared__ double sq[2][QN];                    // current / next element
+  __shared__ double stab[64 * kKStride];          // tables; then the wavefront partial sums
+  __shared__ int scol[2][24];                     // the element's 12 H and 12 G target columns
+  __shared__ double sknown[2][24];                // known displacement / traction values
   __shared__ double sb[4][64];
   const int s = blockIdx.x;
   const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
@@ -291,11 +295,39 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
   const double c0 = cen[3 * s], c1 = cen[3 * s + 1], c2 = cen[3 * s + 2];
   const long row = row0 + 60L * s + lane;
   double bacc = 0.0;
+  // element e's column map (H: node dofs 3 q + n; G: slots 3 q + n) into buffer e & 1
+  auto load_cols = [&](int e) {
+    if (t < 24) {
+      const int par = e & 1;
+      const int k = t < 12 ? t : t - 12, q = k / 3, nc = k % 3;
+      if (t < 12) {
+        const int dof = 3 * (conn[4 * e + q] - 1) + nc;
+        const int col = node_col[dof];
+        scol[par][t] = col;
+        sknown[par][t] = col >= 0 ? 0.0 : uval[dof];
+      } else {
+        const int slot = 12 * e + 3 * q + nc;
+        const int kc = slot_col[slot];
+        scol[par][t] = kc;
+        sknown[par][t] = kc == -1 ? tval[slot] : 0.0;
+      }
+    }
+  };
+  for (int u = t; u < QN; u += 256) sq[0][u] = qp[u];
+  double qn[QPT];
   for (int e = 0; e < NE; ++e) {
-    for (int u = t; u < 64 * kQpStride; u += 256) sq[u] = qp[(size_t)e * 64 * kQpStride + u];
-    __syncthreads();
+    const int cur = e & 1;
+    load_cols(e);
+    if (e + 1 < NE) {
+#pragma unroll
+      for (int k = 0; k < QPT; ++k) {
+        const int u = t + 256 * k;
+        qn[k] = u < QN ? qp[(size_t)(e + 1) * QN + u] : 0.0;
+      }
+    }
+    __syncthreads();  // sq[cur] complete; the previous element's fold is done with stab
     {
-      const double* Q = sq + lane * kQpStride;
+      const double* Q = sq[cur] + lane * kQpStride;
       const double x[3] = {Q[0] - c0, Q[1] - c1, Q[2] - c2};
       double y0[7], y1[7], y2[7], PR[7], PT[5];
       kelvin_point(x, y0, y1, y2, PR, PT);
@@ -314,7 +346,7 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
     if (active) {
       for (int gi = 0; gi < 16; ++gi) {
         const int g = 16 * wv + gi;
-        const double* Q = sq + g * kQpStride;
+        const double* Q = sq[cur] + g * kQpStride;
         double su[3], st[3];
         row_gp(RT, stab + g * kKStride, Q + 3, su, st);
         const double w = Q[10];
@@ -329,6 +361,13 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
         }
       }
     }
+    if (e + 1 < NE) {
+#pragma unroll
+      for (int k = 0; k < QPT; ++k) {
+        const int u = t + 256 * k;
+        if (u < QN) sq[cur ^ 1][u] = qn[k];
+      }
+    }
     __syncthreads();
 #pragma unroll
     for (int o = 0; o < 12; ++o) {
@@ -337,28 +376,36 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
     }
     __syncthreads();
     if (active) {
+      // all reads of A first, then the updates (no store/load round trip between outputs)
+      double v[6], old[6];
+      int col[6];
+#pragma unroll
       for (int oo = 0; oo < 6; ++oo) {
         const int o = wv * 6 + oo;
-        const double v = stab[o * 64 + lane] + stab[(24 + o) * 64 + lane] +
-                         stab[(48 + o) * 64 + lane] + stab[(72 + o) * 64 + lane];
+        v[oo] = stab[o * 64 + lane] + stab[(24 + o) * 64 + lane] + stab[(48 + o) * 64 + lane] +
+                stab[(72 + o) * 64 + lane];
         const int og = o < 12 ? o : o - 12;
-        const int nc = og / 4, q = og % 4;
+        const int k = 3 * (og % 4) + og / 4;  // 3 q + n
+        col[oo] = scol[cur][(o < 12 ? 0 : 12) + k];
+        old[oo] = col[oo] >= 0 ? A[(long)col[oo] * n + row] : 0.0;
+      }
+#pragma unroll
+      for (int oo = 0; oo < 6; ++oo) {
+        const int o = wv * 6 + oo;
+        const int og = o < 12 ? o : o - 12;
+        const int k = 3 * (og % 4) + og / 4;
         if (o < 12) {
-          const int dof = 3 * (conn[4 * e + q] - 1) + nc;
-          const double hv = -fac * v;
-          const int col = node_col[dof];
-          if (col >= 0)
-            A[(long)col * n + row] += hv;
+          const double hv = -fac * v[oo];
+          if (col[oo] >= 0)
+            A[(long)col[oo] * n + row] = old[oo] + hv;
           else
-            bacc -= hv * uval[dof];
+            bacc -= hv * sknown[cur][k];
         } else {
-          const int slot = 12 * e + 3 * q + nc;
-          const double gv = fac * v;
-          const int kc = slot_col[slot];
-          if (kc >= 0)
-            A[(long)kc * n + row] -= gv;
-          else if (kc == -1)
-            bacc += gv * tval[slot];
+          const double gv = fac * v[oo];
+          if (col[oo] >= 0)
+            A[(long)col[oo] * n + row] = old[oo] - gv;
+          else
+            bacc += gv * sknown[cur][12 + k];  // zero unless the traction is known
         }
       }
     }

@@ -78,6 +78,13 @@ void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStrea
 // y = alpha A x + beta y (FP64, column-major)
 void gemv_f64(long m, long n, double alpha, const double* A, long lda, const double* x, double beta,
               double* y, hipStream_t st);
+// out = R A C with the same scaling, A left unchanged (FP64 or FP32 output); ws holds
+// equilibrate_workspace_doubles(n) doubles
+size_t equilibrate_workspace_doubles(long n);
+void equilibrate_copy_f64(const double* A, long n, long lda, double* r, double* c, double* out, long ldo,
+                          double* ws, hipStream_t st);
+void equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
+                          double* ws, hipStream_t st);
 // FP64 -> FP32 copy of a column-major matrix
 void convert_f64_to_f32(const double* A, long n, long lda, float* B, long ldb, hipStream_t st);
 

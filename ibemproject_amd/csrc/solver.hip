@@ -1285,6 +1285,76 @@ __global__ __launch_bounds__(256) void k_colscale(double* __restrict__ A, long n
   for (long i = threadIdx.x; i < n; i += blockDim.x) col[i] = col[i] * r[i] * cj;
 }
 
+// Row maxima in two passes: workgroup (256 rows, one of kRowChunks column chunks) -> partial
+// maxima, then r_i = 1 / max. Every column is read by many workgroups at once (coalesced, HBM rate).
+constexpr int kRowChunks = 32;
+__global__ __launch_bounds__(256) void k_rowmax_part(const double* __restrict__ A, long n, long lda,
+                                                     double* __restrict__ part) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long per = (n + kRowChunks - 1) / kRowChunks;
+  const long j0 = blockIdx.y * per, j1 = j0 + per < n ? j0 + per : n;
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  long j = j0;
+  for (; j + 4 <= j1; j += 4) {
+    m0 = fmax(m0, fabs(A[j * lda + i]));
+    m1 = fmax(m1, fabs(A[(j + 1) * lda + i]));
+    m2 = fmax(m2, fabs(A[(j + 2) * lda + i]));
+    m3 = fmax(m3, fabs(A[(j + 3) * lda + i]));
+  }
+  for (; j < j1; ++j) m0 = fmax(m0, fabs(A[j * lda + i]));
+  part[blockIdx.y * n + i] = fmax(fmax(m0, m1), fmax(m2, m3));
+}
+__global__ __launch_bounds__(256) void k_rowmax_done(const double* __restrict__ part, long n,
+                                                     double* __restrict__ r) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double m = 0.0;
+  for (int q = 0; q < kRowChunks; ++q) m = fmax(m, part[q * n + i]);
+  r[i] = m > 0.0 ? 1.0 / m : 1.0;
+}
+// Column j: c_j = 1 / max_i |r_i A_ij|, out[:, j] = R A C (in TO precision); A is only read.
+template <typename TO>
+__global__ __launch_bounds__(256) void k_colscale_copy(const double* __restrict__ A, long n, long lda,
+                                                       const double* __restrict__ r,
+                                                       double* __restrict__ c, TO* __restrict__ out,
+                                                       long ldo) {
+  __shared__ double red[256];
+  const long j = blockIdx.x;
+  const double* col = A + j * lda;
+  double m = 0.0;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(col[i] * r[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  const double cj = red[0] > 0.0 ? 1.0 / red[0] : 1.0;
+  if (threadIdx.x == 0) c[j] = cj;
+  TO* o = out + j * ldo;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) o[i] = (TO)(col[i] * r[i] * cj);
+}
+
+size_t equilibrate_workspace_doubles(long n) { return (size_t)kRowChunks * n; }
+
+template <typename TO>
+static void equilibrate_copy(const double* A, long n, long lda, double* r, double* c, TO* out, long ldo,
+                             double* ws, hipStream_t st) {
+  hipLaunchKernelGGL(k_rowmax_part, dim3((unsigned)((n + 255) / 256), kRowChunks), dim3(256), 0, st, A,
+                     n, lda, ws);
+  hipLaunchKernelGGL(k_rowmax_done, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ws, n, r);
+  hipLaunchKernelGGL(k_colscale_copy<TO>, dim3((unsigned)n), dim3(256), 0, st, A, n, lda, r, c, out, ldo);
+}
+void equilibrate_copy_f64(const double* A, long n, long lda, double* r, double* c, double* out, long ldo,
+                          double* ws, hipStream_t st) {
+  equilibrate_copy<double>(A, n, lda, r, c, out, ldo, ws, st);
+}
+void equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
+                          double* ws, hipStream_t st) {
+  equilibrate_copy<float>(A, n, lda, r, c, out, ldo, ws, st);
+}
+
 void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStream_t st) {
   hipLaunchKernelGGL(k_rowmax, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, n, lda, r);
   hipLaunchKernelGGL(k_colscale, dim3((unsigned)n), dim3(256), 0, st, A, n, lda, r, c);
