@@ -44,6 +44,33 @@ struct Vec4<float> {
 
 constexpr int kBM = 128, kBN = 128;
 
+// Tile order of the split GEMM (1D grid). Workgroups are dispatched to the 8 XCDs round-robin by
+// linear id and each XCD has its own L2; tiles are enumerated in groups of kGroupM M-tiles (M
+// fastest, then N) and every XCD takes one contiguous eighth of that enumeration, so the
+// workgroups resident on an XCD cover a compact block of tiles and share operand rows and columns
+// in its L2 instead of re-reading them from memory.
+#ifndef IBEM_GROUP_M
+#define IBEM_GROUP_M 8
+#endif
+constexpr int kXcds = 8, kGroupM = IBEM_GROUP_M;
+__device__ inline bool tile_of(long L, long mt, long nt, long& tm, long& tn) {
+  const long T = mt * nt;
+  const long per = (T + kXcds - 1) / kXcds;
+  const long t = (L % kXcds) * per + L / kXcds;
+  if (t >= T) return false;
+  const long grp = t / (kGroupM * nt);
+  const long first = grp * kGroupM;
+  const long gsz = (mt - first) < kGroupM ? (mt - first) : kGroupM;
+  const long in = t - grp * kGroupM * nt;
+  tm = first + in % gsz;
+  tn = in / gsz;
+  return true;
+}
+static inline unsigned tile_grid(long M, long N) {
+  const long T = ((M + kBM - 1) / kBM) * ((N + kBN - 1) / kBN);
+  return (unsigned)(((T + kXcds - 1) / kXcds) * kXcds);
+}
+
 template <typename T>
 struct GemmCfg;
 #ifndef IBEM_GEMM_BK_F64
@@ -318,6 +345,9 @@ __global__ __launch_bounds__(256) void k_split_cols(long N, int K, const float* 
 #ifndef IBEM_F16_PF
 #define IBEM_F16_PF 1
 #endif
+#ifndef IBEM_F16_PAD
+#define IBEM_F16_PAD 8
+#endif
 template <int BK, int EPI, int PF>
 __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
                                                        const _Float16* __restrict__ Ah,
@@ -328,7 +358,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
                                                        const float* __restrict__ ib,
                                                        float* __restrict__ C, long ldc) {
   (void)K;  // operands are zero-padded to kSplitPadK, so every tile runs all NCH chunks
-  constexpr int LS = BK + 8;             // LDS row stride (halves)
+  constexpr int LS = BK + IBEM_F16_PAD;  // LDS row stride (halves)
   constexpr int VPR = BK / 8;            // 16-byte vectors per row and array
   constexpr int NV = kBM * VPR / 256;    // vectors per thread and array
   constexpr int NCH = kSplitPadK / BK;   // K chunks
@@ -339,7 +369,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
   _Float16* sAl = smem + kBM * LS;
   _Float16* sBh = smem + 2 * kBM * LS;
   _Float16* sBl = smem + 3 * kBM * LS;
-  const long m0 = (long)blockIdx.x * kBM, n0 = (long)blockIdx.y * kBN;
+  long tm, tn;
+  if (!tile_of(blockIdx.x, (M + kBM - 1) / kBM, (N + kBN - 1) / kBN, tm, tn)) return;
+  const long m0 = tm * kBM, n0 = tn * kBN;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const int lr = lane & 15, lk = lane >> 4;
@@ -494,8 +526,7 @@ static void f16x3_split(long M, long N, int K, const float* A, long lda, const f
 // C[M x N] -= split A times the split B columns [c0, c0 + N)
 static void f16x3_gemm(long M, long N, int K, const F16x3Ws& w, float* C, long ldc, hipStream_t st,
                        long c0 = 0) {
-  dim3 grid((unsigned)((M + kBM - 1) / kBM), (unsigned)((N + kBN - 1) / kBN));
-  hipLaunchKernelGGL((k_gemm_f16x3<IBEM_F16_BK, IBEM_F16_EPI, IBEM_F16_PF>), grid, dim3(256), 0, st, M, N, K, w.Ah,
+  hipLaunchKernelGGL((k_gemm_f16x3<IBEM_F16_BK, IBEM_F16_EPI, IBEM_F16_PF>), dim3(tile_grid(M, N)), dim3(256), 0, st, M, N, K, w.Ah,
                      w.Al, w.Bh + c0 * kSplitPadK, w.Bl + c0 * kSplitPadK, w.ia, w.ib + c0, C, ldc);
 }
 
