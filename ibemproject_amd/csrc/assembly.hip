@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
         const int g = 16 * wv + gi;
         const double* Q = sq[cur] + g * kQpStride;
         double su[3], st[3];
-        row_gp(RT, stab + g * kKStride, Q + 3, su, st);
+        row_gp2(RT, stab + g * kKStride, Q + 3, su, st);
         const double w = Q[10];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {

@@ -791,7 +791,10 @@ struct RowTerms {
   int oS[6];   // S(e + m + n + k + l)
   int oSn[3];  // S(e + n + k + l)
   int oTn[3];  // T(e + n)
+  int oTnk[3], oTnl[3];  // T(e + n + k), T(e + n + l)
+  int oTkl, oTk, oTl;    // T(e + k + l), T(e + k), T(e + l)
   int k, l;
+  double dk[3], dl[3];   // [n == k], [n == l]
   double A1, A2, A3, A4, B1, B2, B3, sg;
 };
 
@@ -821,8 +824,52 @@ IBEM_HD RowTerms row_terms(const Mat& M, const Slot& R) {
     const int a0 = e0 + (n == 0), a1 = e1 + (n == 1), a2 = e2 + (n == 2);
     t.oSn[n] = tidx(a0 + k0, a1 + k1, a2 + k2);
     t.oTn[n] = 84 + tidx(a0, a1, a2);
+    t.oTnk[n] = 84 + tidx(a0 + (R.i == 0), a1 + (R.i == 1), a2 + (R.i == 2));
+    t.oTnl[n] = 84 + tidx(a0 + (R.j == 0), a1 + (R.j == 1), a2 + (R.j == 2));
+    t.dk[n] = n == R.i ? 1.0 : 0.0;
+    t.dl[n] = n == R.j ? 1.0 : 0.0;
   }
+  t.oTkl = 84 + tidx(e0 + k0, e1 + k1, e2 + k2);
+  t.oTk = 84 + tidx(e0 + (R.i == 0), e1 + (R.i == 1), e2 + (R.i == 2));
+  t.oTl = 84 + tidx(e0 + (R.j == 0), e1 + (R.j == 1), e2 + (R.j == 2));
   return t;
+}
+
+// row_gp with every table value read at a precomputed position (no selects): the k- and l-shifted
+// entries come from the table, and the d_nk, d_nl terms are multiplied in. Same values as row_gp.
+IBEM_HD void row_gp2(const RowTerms& R, const double* tab, const double* nv, double* su, double* st) {
+  double T6[6], S6[6], Tnk[3], Tnl[3], Sn[3], Tn[3];
+#pragma unroll
+  for (int u = 0; u < 6; ++u) {
+    T6[u] = tab[R.oT[u]];
+    S6[u] = tab[R.oS[u]];
+  }
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    Tnk[n] = tab[R.oTnk[n]];
+    Tnl[n] = tab[R.oTnl[n]];
+    Sn[n] = tab[R.oSn[n]];
+    Tn[n] = tab[R.oTn[n]];
+  }
+  const double Tkl = tab[R.oTkl], Tk = tab[R.oTk], Tl = tab[R.oTl];
+  const double V0 = nv[0] * T6[0] + nv[1] * T6[1] + nv[2] * T6[2];
+  const double V1 = nv[0] * T6[1] + nv[1] * T6[3] + nv[2] * T6[4];
+  const double V2 = nv[0] * T6[2] + nv[1] * T6[4] + nv[2] * T6[5];
+  const double W0 = nv[0] * S6[0] + nv[1] * S6[1] + nv[2] * S6[2];
+  const double W1 = nv[0] * S6[1] + nv[1] * S6[3] + nv[2] * S6[4];
+  const double W2 = nv[0] * S6[2] + nv[1] * S6[4] + nv[2] * S6[5];
+  // V_k = sum_m nv_m T(e+m+k), V_l likewise; nv_k, nv_l as dot products with the indicators
+  const double Vk = nv[0] * Tnk[0] + nv[1] * Tnk[1] + nv[2] * Tnk[2];
+  const double Vl = nv[0] * Tnl[0] + nv[1] * Tnl[1] + nv[2] * Tnl[2];
+  const double nvk = nv[0] * R.dk[0] + nv[1] * R.dk[1] + nv[2] * R.dk[2];
+  const double nvl = nv[0] * R.dl[0] + nv[1] * R.dl[1] + nv[2] * R.dl[2];
+  const double V[3] = {V0, V1, V2}, W[3] = {W0, W1, W2};
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    const double g = nvk * Tnl[n] + nvl * Tnk[n] + R.dk[n] * Vl + R.dl[n] * Vk;
+    su[n] = R.sg * (R.A1 * V[n] + R.A2 * nv[n] * Tkl + R.A3 * W[n] + R.A4 * g);
+    st[n] = -R.sg * (R.B1 * Sn[n] + R.B3 * Tn[n] + R.B2 * (R.dk[n] * Tl + R.dl[n] * Tk));
+  }
 }
 
 // su[n], st[n] (n = 0..2) of one row at one Gauss point with unit normal nv and table tab
