@@ -250,7 +250,8 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
                   ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32, ctx->d_f16ws, ctx->d_eqws,
-                  ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info, ctx->W.d_perm, ctx->W.d_inv, ctx->W.d_tmp};
+                  ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info, ctx->W.d_perm, ctx->W.d_inv, ctx->W.d_tmp,
+                  ctx->W.d_trinv, ctx->W.d_trsm};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (ctx->W.st2) hipStreamDestroy(ctx->W.st2);
@@ -380,6 +381,8 @@ static int ensure_solver_ws(ibem3d_ctx* ctx) {
     if (int rc = dalloc(ctx, &ctx->W.d_inv, lu_inverse_doubles(n))) return rc;
     if (int rc = dalloc(ctx, &ctx->W.d_tmp, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_eqws, equilibrate_workspace_doubles(n))) return rc;
+    if (int rc = dalloc(ctx, &ctx->W.d_trinv, lu_trinv_doubles())) return rc;
+    if (int rc = dalloc(ctx, &ctx->W.d_trsm, lu_trsm_doubles(n))) return rc;
     if (int rc = dalloc(ctx, &ctx->d_r, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_c, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_work, (size_t)n)) return rc;

@@ -58,9 +58,14 @@ struct LuWork {
   int* d_perm = nullptr;   // n: row permutation as a gather list (lu_prepare_solve_*)
   double* d_inv = nullptr; // lu_inverse_doubles(n): inverses of the diagonal blocks of L and U
   double* d_tmp = nullptr; // n: solve scratch
+  double* d_trinv = nullptr; // lu_trsm_workspace: inverses of one block row's diagonal blocks
+  double* d_trsm = nullptr;  // lu_trsm_workspace: U12 of one step (kb x n)
 };
 size_t lu_workspace_doubles(long n);
 size_t lu_inverse_doubles(long n);
+// doubles of d_trinv and d_trsm for an n x n factorisation
+size_t lu_trinv_doubles();
+size_t lu_trsm_doubles(long n);
 int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx);
 // f16ws != nullptr: the trailing updates run as gemm_f32_sub_f16x3 (workspace of
 // f16x3_workspace_doubles(n) doubles); otherwise on FP32 MFMA

@@ -186,7 +186,12 @@ __global__ __launch_bounds__(256, IBEM_GEMM_MINB) void k_gemm_sub(long M, long N
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const long m = m0 + wm + i * 16 + Vec4<T>::row(lk, v);
-        if (m < M) C[nn * ldc + m] -= acc[i][j][v];
+        if (m < M) {
+          if constexpr (Role == 2)
+            C[nn * ldc + m] = acc[i][j][v];
+          else
+            C[nn * ldc + m] -= acc[i][j][v];
+        }
       }
     }
 }
@@ -1000,81 +1005,7 @@ static void trailing_update(T* A, long n, long lda, long k, long kb, long c0, lo
     gemm_sub<T, 1>(m, nc, kb, A + k * lda + k + kb, lda, A + (k + kb + c0) * lda + k, lda, C, lda, st);
 }
 
-// Blocked right-looking LU with look-ahead: after the row interchanges and U12 of step k, the
-// trailing update first covers the next panel's columns; that panel is then factored on a second
-// stream while the rest of the trailing update runs on the main stream. The two never touch the
-// same columns: the panel works inside columns [k + kb, k + 2 kb), the rest of the update writes
-// columns >= k + 2 kb and reads L21 (columns [k, k + kb)) and U12 (rows [k, k + kb)).
-template <typename T>
-static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
-                     double* f16ws = nullptr) {
-  if (!W.st2) {
-    if (hipStreamCreateWithFlags(&W.st2, hipStreamNonBlocking) != hipSuccess) return -1;
-    if (hipEventCreateWithFlags(&W.ev_a, hipEventDisableTiming) != hipSuccess) return -1;
-    if (hipEventCreateWithFlags(&W.ev_b, hipEventDisableTiming) != hipSuccess) return -1;
-  }
-  LuCtx<T> c{A, n, lda, &W, st, tick, tctx, 0};
-  LuCtx<T> c2{A, n, lda, &W, W.st2, tick, tctx, 0};
-  hipMemsetAsync(W.d_info, 0, sizeof(int), st);
-  const F16x3Ws wsp = f16ws ? f16x3_ws(f16ws, n) : F16x3Ws{};
-  const F16x3Ws* w = f16ws ? &wsp : nullptr;
-  if (tick) tick(tctx, "lu.panel", 0, st);
-  panel_rec(c, 0, std::min(kLuNB, n));
-  if (tick) tick(tctx, "lu.panel", 1, st);
-  for (long k = 0; k < n; k += kLuNB) {
-    const long kb = std::min(kLuNB, n - k);
-    if (k > 0) hipStreamWaitEvent(st, W.ev_b, 0);  // panel k, factored on the second stream
-    if (tick) tick(tctx, "lu.swap_trsm", 0, st);
-    laswp<T>(A, lda, 0, k, W.d_ipiv, k, k + kb, st);
-    if (k + kb < n) {
-      laswp<T>(A, lda, k + kb, n - k - kb, W.d_ipiv, k, k + kb, st);
-      trsm_lower_unit<T>(A + k * lda + k, lda, kb, A + (k + kb) * lda + k, lda, n - k - kb, st);
-    }
-    if (tick) tick(tctx, "lu.swap_trsm", 1, st);
-    if (k + kb >= n) break;
-    const long m = n - k - kb;
-    const long kb2 = std::min(kLuNB, m);  // width of the next panel
-    if (w) {
-      if (tick) tick(tctx, "lu.split", 0, st);
-      f16x3_split(m, m, (int)kb, (const float*)(A + k * lda + k + kb), lda,
-                  (const float*)(A + (k + kb) * lda + k), lda, *w, st);
-      if (tick) tick(tctx, "lu.split", 1, st);
-    }
-    if (tick) tick(tctx, "lu.gemm", 0, st);
-    trailing_update<T>(A, n, lda, k, kb, 0, kb2, w, st);
-    hipEventRecord(W.ev_a, st);
-    hipStreamWaitEvent(W.st2, W.ev_a, 0);
-    if (tick) tick(tctx, "lu.panel", 0, W.st2);
-    panel_rec(c2, k + kb, kb2);
-    if (tick) tick(tctx, "lu.panel", 1, W.st2);
-    hipEventRecord(W.ev_b, W.st2);
-    trailing_update<T>(A, n, lda, k, kb, kb2, m - kb2, w, st);
-    if (tick) tick(tctx, "lu.gemm", 1, st);
-  }
-  return 0;
-}
-
-int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx) {
-  return lu_factor<double>(A, n, lda, W, st, tick, tctx);
-}
-int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
-                  double* f16ws) {
-  return lu_factor<float>(A, n, lda, W, st, tick, tctx, f16ws);
-}
-
-// ======================================================================= triangular solves ======
-// After the factorisation: the row permutation as a gather list (built on the host from ipiv) and
-// the inverses of the 128 x 128 diagonal blocks of L and U (k_trinv, one workgroup per block). A
-// solve is then one launch per block column and direction: every workgroup subtracts its rows'
-// share of the just-solved block (a coalesced tall GEMV), and workgroup 0, whose rows are the next
-// diagonal block, finishes that block at once with the precomputed inverse. The factors are read
-// once per solve (n^2 elements), so the solve runs at HBM speed rather than at one workgroup's.
-constexpr int kTB = 128;
-
-size_t lu_inverse_doubles(long n) {
-  const long nb = (n + kTB - 1) / kTB;
-  return (size_t)nb * kTB * kTB * 2;
-}
+constexpr int kTB = 128;  // diagonal-block size of the triangular inverses
 
 // X = inverse of the diagonal block b of unit-lower L (Upper = false) or of U (Upper = true),
 // stored column-major (X[j * kTB + t] = X(t, j)) at inv + b * kTB * kTB. Thread j = column j.
@@ -1120,6 +1051,106 @@ __global__ __launch_bounds__(kTB) void k_trinv(const T* __restrict__ LU, long ld
     const int c = u / kTB, t = u % kTB;
     out[u] = (c < kb && t < kb) ? X[t][c] : T(0);
   }
+}
+
+// U12 = L11^-1 A12 for the block row of one step (L11 unit lower, kb x kb; A12 kb x N), through
+// the inverses D_i of L11's 128 x 128 diagonal blocks (k_trinv):
+//   U_i = D_i (A_i - sum_{k<i} L_ik U_k),
+// as full-height GEMMs (128-row tiles) instead of a recursion down to 32-column solves and
+// skinny updates. U is assembled in W.d_trsm (kb x N) and copied back over A12.
+template <typename T>
+static void trsm_by_inverse(const T* L, long ldl, long kb, T* B, long ldb, long N, LuWork& W,
+                            hipStream_t st) {
+  if (kb <= 0 || N <= 0) return;
+  const long nbk = (kb + kTB - 1) / kTB;
+  T* D = (T*)W.d_trinv;  // nbk x (128 x 128)
+  T* U = (T*)W.d_trsm;   // kb x N, leading dimension kb
+  hipLaunchKernelGGL((k_trinv<T, false>), dim3((unsigned)nbk), dim3(kTB), 0, st, L, ldl, kb, D);
+  for (long i = 0; i < nbk; ++i) {
+    const long r0 = i * kTB, rb = std::min<long>(kTB, kb - r0);
+    if (i > 0)
+      gemm_sub<T, 0>(rb, N, r0, L + r0, ldl, U, kb, B + r0, ldb, st);  // A_i -= L_i,<i U_<i
+    gemm_sub<T, 2>(rb, N, rb, D + (size_t)i * kTB * kTB, kTB, B + r0, ldb, U + r0, kb, st);
+  }
+  hipMemcpy2DAsync(B, sizeof(T) * ldb, U, sizeof(T) * kb, sizeof(T) * kb, N, hipMemcpyDeviceToDevice, st);
+}
+
+// Blocked right-looking LU with look-ahead: after the row interchanges and U12 of step k, the
+// trailing update first covers the next panel's columns; that panel is then factored on a second
+// stream while the rest of the trailing update runs on the main stream. The two never touch the
+// same columns: the panel works inside columns [k + kb, k + 2 kb), the rest of the update writes
+// columns >= k + 2 kb and reads L21 (columns [k, k + kb)) and U12 (rows [k, k + kb)).
+template <typename T>
+static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
+                     double* f16ws = nullptr) {
+  if (!W.st2) {
+    if (hipStreamCreateWithFlags(&W.st2, hipStreamNonBlocking) != hipSuccess) return -1;
+    if (hipEventCreateWithFlags(&W.ev_a, hipEventDisableTiming) != hipSuccess) return -1;
+    if (hipEventCreateWithFlags(&W.ev_b, hipEventDisableTiming) != hipSuccess) return -1;
+  }
+  LuCtx<T> c{A, n, lda, &W, st, tick, tctx, 0};
+  LuCtx<T> c2{A, n, lda, &W, W.st2, tick, tctx, 0};
+  hipMemsetAsync(W.d_info, 0, sizeof(int), st);
+  const F16x3Ws wsp = f16ws ? f16x3_ws(f16ws, n) : F16x3Ws{};
+  const F16x3Ws* w = f16ws ? &wsp : nullptr;
+  if (tick) tick(tctx, "lu.panel", 0, st);
+  panel_rec(c, 0, std::min(kLuNB, n));
+  if (tick) tick(tctx, "lu.panel", 1, st);
+  for (long k = 0; k < n; k += kLuNB) {
+    const long kb = std::min(kLuNB, n - k);
+    if (k > 0) hipStreamWaitEvent(st, W.ev_b, 0);  // panel k, factored on the second stream
+    if (tick) tick(tctx, "lu.swap_trsm", 0, st);
+    laswp<T>(A, lda, 0, k, W.d_ipiv, k, k + kb, st);
+    if (k + kb < n) {
+      laswp<T>(A, lda, k + kb, n - k - kb, W.d_ipiv, k, k + kb, st);
+      trsm_by_inverse<T>(A + k * lda + k, lda, kb, A + (k + kb) * lda + k, lda, n - k - kb, W, st);
+    }
+    if (tick) tick(tctx, "lu.swap_trsm", 1, st);
+    if (k + kb >= n) break;
+    const long m = n - k - kb;
+    const long kb2 = std::min(kLuNB, m);  // width of the next panel
+    if (w) {
+      if (tick) tick(tctx, "lu.split", 0, st);
+      f16x3_split(m, m, (int)kb, (const float*)(A + k * lda + k + kb), lda,
+                  (const float*)(A + (k + kb) * lda + k), lda, *w, st);
+      if (tick) tick(tctx, "lu.split", 1, st);
+    }
+    if (tick) tick(tctx, "lu.gemm", 0, st);
+    trailing_update<T>(A, n, lda, k, kb, 0, kb2, w, st);
+    hipEventRecord(W.ev_a, st);
+    hipStreamWaitEvent(W.st2, W.ev_a, 0);
+    if (tick) tick(tctx, "lu.panel", 0, W.st2);
+    panel_rec(c2, k + kb, kb2);
+    if (tick) tick(tctx, "lu.panel", 1, W.st2);
+    hipEventRecord(W.ev_b, W.st2);
+    trailing_update<T>(A, n, lda, k, kb, kb2, m - kb2, w, st);
+    if (tick) tick(tctx, "lu.gemm", 1, st);
+  }
+  return 0;
+}
+
+int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx) {
+  return lu_factor<double>(A, n, lda, W, st, tick, tctx);
+}
+int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
+                  double* f16ws) {
+  return lu_factor<float>(A, n, lda, W, st, tick, tctx, f16ws);
+}
+
+// ======================================================================= triangular solves ======
+// After the factorisation: the row permutation as a gather list (built on the host from ipiv) and
+// the inverses of the 128 x 128 diagonal blocks of L and U (k_trinv, one workgroup per block). A
+// solve is then one launch per block column and direction: every workgroup subtracts its rows'
+// share of the just-solved block (a coalesced tall GEMV), and workgroup 0, whose rows are the next
+// diagonal block, finishes that block at once with the precomputed inverse. The factors are read
+// once per solve (n^2 elements), so the solve runs at HBM speed rather than at one workgroup's.
+
+size_t lu_trinv_doubles() { return (size_t)IBEM_LU_NB * kTB; }
+size_t lu_trsm_doubles(long n) { return (size_t)IBEM_LU_NB * n; }
+
+size_t lu_inverse_doubles(long n) {
+  const long nb = (n + kTB - 1) / kTB;
+  return (size_t)nb * kTB * kTB * 2;
 }
 
 // y <- y[perm] (the factorisation's row interchanges as one gather)
