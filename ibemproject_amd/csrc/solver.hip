@@ -1009,28 +1009,46 @@ constexpr int kTB = 128;  // diagonal-block size of the triangular inverses
 
 // X = inverse of the diagonal block b of unit-lower L (Upper = false) or of U (Upper = true),
 // stored column-major (X[j * kTB + t] = X(t, j)) at inv + b * kTB * kTB. Thread j = column j.
+// The 128 x 128 triangle is staged in LDS first when it fits beside X (FP32: 2 x 66 KB); in FP64
+// each step reads its row of the triangle from memory.
 template <typename T, bool Upper>
 __global__ __launch_bounds__(kTB) void k_trinv(const T* __restrict__ LU, long lda, long n,
                                                T* __restrict__ inv) {
+  constexpr bool kStage = sizeof(T) == 4;
   __shared__ T X[kTB][kTB + 1];
+  __shared__ T Ls[kStage ? kTB : 1][kStage ? kTB + 1 : 1];  // Ls[t][k] = triangle (t, k)
   __shared__ T srow[kTB];
   __shared__ T sdiag;
   const long k0 = (long)blockIdx.x * kTB;
   const int kb = (int)min((long)kTB, n - k0);
   const int j = threadIdx.x;
+  if constexpr (kStage) {
+    // column j of the block is contiguous: thread j walks it (rows t), all threads in step
+    for (int c = 0; c < kb; ++c)
+      if (j < kb) Ls[j][c] = LU[(k0 + c) * lda + k0 + j];
+    __syncthreads();
+  }
   for (int tt = 0; tt < kb; ++tt) {
     const int t = Upper ? kb - 1 - tt : tt;
-    // row t of the triangle: L(t, 0..t-1) or U(t, t..kb-1)
-    if (!Upper ? (j < t) : (j >= t && j < kb)) srow[j] = LU[(k0 + j) * lda + k0 + t];
-    __syncthreads();
-    if (Upper && j == 0) sdiag = srow[t];
-    __syncthreads();
+    const T* row;
+    if constexpr (kStage) {
+      row = Ls[t];
+      if (Upper && j == 0) sdiag = Ls[t][t];
+      __syncthreads();
+    } else {
+      // row t of the triangle: L(t, 0..t-1) or U(t, t..kb-1)
+      if (!Upper ? (j < t) : (j >= t && j < kb)) srow[j] = LU[(k0 + j) * lda + k0 + t];
+      __syncthreads();
+      if (Upper && j == 0) sdiag = srow[t];
+      __syncthreads();
+      row = srow;
+    }
     if (j < kb) {
       T v = T(0);
       if (!Upper) {
         if (j < t) {
           T acc = T(0);
-          for (int k = j; k < t; ++k) acc += srow[k] * X[k][j];
+          for (int k = j; k < t; ++k) acc += row[k] * X[k][j];
           v = -acc;
         } else if (j == t) {
           v = T(1);
@@ -1038,7 +1056,7 @@ __global__ __launch_bounds__(kTB) void k_trinv(const T* __restrict__ LU, long ld
       } else {
         if (j >= t) {
           T acc = j == t ? T(1) : T(0);
-          for (int k = t + 1; k <= j; ++k) acc -= srow[k] * X[k][j];
+          for (int k = t + 1; k <= j; ++k) acc -= row[k] * X[k][j];
           v = sdiag != T(0) ? acc / sdiag : T(0);
         }
       }
@@ -1103,7 +1121,10 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
     laswp<T>(A, lda, 0, k, W.d_ipiv, k, k + kb, st);
     if (k + kb < n) {
       laswp<T>(A, lda, k + kb, n - k - kb, W.d_ipiv, k, k + kb, st);
-      trsm_by_inverse<T>(A + k * lda + k, lda, kb, A + (k + kb) * lda + k, lda, n - k - kb, W, st);
+      if (sizeof(T) == 4)
+        trsm_by_inverse<T>(A + k * lda + k, lda, kb, A + (k + kb) * lda + k, lda, n - k - kb, W, st);
+      else
+        trsm_lower_unit<T>(A + k * lda + k, lda, kb, A + (k + kb) * lda + k, lda, n - k - kb, st);
     }
     if (tick) tick(tctx, "lu.swap_trsm", 1, st);
     if (k + kb >= n) break;
