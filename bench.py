@@ -30,7 +30,7 @@ from ibemproject_amd.generate import make_workload  # noqa: E402
 
 MI355X_PEAK = {"f16_mfma_tflops": 2500.0, "f32_mfma_tflops": 157.3, "f64_mfma_tflops": 78.6,
                "hbm_gbs": 8000.0}
-NB = 512  # LU block size (solver.hip IBEM_LU_NB)
+NB = 1024  # LU block size (solver.hip IBEM_LU_NB)
 
 
 def trailing_update_flops(n: int) -> tuple[float, int]:

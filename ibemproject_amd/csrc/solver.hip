@@ -226,7 +226,7 @@ void gemm_f32_sub(long M, long N, long K, const float* A, long lda, const float*
 // replace 8 v_mfma_f32_16x16x4f32 (157.3 TFLOP/s) per 16x16x32 block.
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 #ifndef IBEM_LU_NB
-#define IBEM_LU_NB 512
+#define IBEM_LU_NB 1024
 #endif
 constexpr int kSplitK = IBEM_LU_NB;     // operand depth (the LU block size)
 constexpr int kSplitPadK = IBEM_LU_NB;  // row stride of the split operands, in halves
