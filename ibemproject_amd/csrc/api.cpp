@@ -377,7 +377,7 @@ static int ensure_solver_ws(ibem3d_ctx* ctx) {
     if (int rc = dalloc(ctx, &ctx->W.d_ws, lu_workspace_doubles(n))) return rc;
     HIPC(hipMemsetAsync(ctx->W.d_ws, 0, sizeof(double) * lu_workspace_doubles(n), ctx->st));
     if (int rc = dalloc(ctx, &ctx->W.d_info, 1)) return rc;
-    if (int rc = dalloc(ctx, &ctx->W.d_perm, (size_t)n)) return rc;
+    if (int rc = dalloc(ctx, &ctx->W.d_perm, lu_moves_ints(n))) return rc;
     if (int rc = dalloc(ctx, &ctx->W.d_inv, lu_inverse_doubles(n))) return rc;
     if (int rc = dalloc(ctx, &ctx->W.d_tmp, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_eqws, equilibrate_workspace_doubles(n))) return rc;

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "ibem3d_math.h"
 
 namespace ibem {
@@ -45,8 +47,11 @@ void gemm_f32_sub_f16x3(long M, long N, int K, const float* A, long lda, const f
                         float* C, long ldc, double* ws, hipStream_t st);
 
 // Blocked right-looking LU with row pivoting (tournament pivoting inside 32-column leaves), in
-// place; ipiv[k] = row swapped with row k (absolute, 0-based, applied in order). d_info receives
-// the first column + 1 whose pivot is exactly zero. Returns nonzero on a launch error.
+// place; ipiv[k] = row swapped with row k (absolute, 0-based, applied in order). The block column
+// of L computed at a step keeps the row order of the end of that step: later steps' interchanges
+// are not applied to it (nothing in the factorisation reads it again), and the solve applies each
+// step's interchanges to the vector instead. d_info receives the first column + 1 whose pivot is
+// exactly zero. Returns nonzero on a launch error.
 // timing callback: (context, phase name, 0 = begin / 1 = end, stream the phase runs on)
 typedef void (*LuTick)(void*, const char*, int, hipStream_t);
 struct LuWork {
@@ -55,7 +60,8 @@ struct LuWork {
   int* d_ipiv = nullptr;   // n
   double* d_ws = nullptr;  // lu_workspace_doubles(n), zero-filled once at allocation
   int* d_info = nullptr;
-  int* d_perm = nullptr;   // n: row permutation as a gather list (lu_prepare_solve_*)
+  int* d_perm = nullptr;   // lu_moves_ints(n): each step's interchanges as (dst, src) moves
+  std::vector<int> mv_off; // per step: first move (host; lu_prepare_solve_*), plus the end
   double* d_inv = nullptr; // lu_inverse_doubles(n): inverses of the diagonal blocks of L and U
   double* d_tmp = nullptr; // n: solve scratch
   double* d_trinv = nullptr; // lu_trsm_workspace: inverses of one block row's diagonal blocks
@@ -63,6 +69,7 @@ struct LuWork {
 };
 size_t lu_workspace_doubles(long n);
 size_t lu_inverse_doubles(long n);
+size_t lu_moves_ints(long n);
 // doubles of d_trinv and d_trsm for an n x n factorisation
 size_t lu_trinv_doubles();
 size_t lu_trsm_doubles(long n);
@@ -70,8 +77,8 @@ int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick
 // f16ws != nullptr: the trailing updates run as gemm_f32_sub_f16x3 (workspace of
 // f16x3_workspace_doubles(n) doubles); otherwise on FP32 MFMA
 int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx, double* f16ws);
-// After a factorisation: gather list of the interchanges and diagonal-block inverses (synchronises
-// the stream once). Returns nonzero on a HIP error.
+// After a factorisation: each step's interchanges as a move list and the diagonal-block inverses
+// (synchronises the stream once). Returns nonzero on a HIP error.
 int lu_prepare_solve_f64(const double* LU, long n, long lda, LuWork& W, hipStream_t st);
 int lu_prepare_solve_f32(const float* LU, long n, long lda, LuWork& W, hipStream_t st);
 // Solve with the factors: x (in/out, n) <- A^{-1} x

@@ -557,8 +557,7 @@ void gemm_f32_sub_f16x3(long M, long N, int K, const float* A, long lda, const f
 //             content where an interchange moved one there) becomes its row of L21 = A21 U11^-1.
 // Tournament pivoting (communication-avoiding LU, Grigori, Demmel and Xiang 2011) replaces the
 // column-by-column global pivot search; it is backward stable in practice like partial pivoting.
-// Ties are broken by the smaller row index and the tree shape is fixed, so results are
-// deterministic.
+// Pivot ties follow a fixed rule over a fixed tree shape, so results are deterministic.
 constexpr int kPanelW = 32;   // leaf width
 constexpr int kTsRows = 256;  // rows per workgroup (one per thread) at every tree level
 constexpr int kTsFan = 8;     // candidate sets merged per tree node (8 x 32 = 256 rows)
@@ -590,92 +589,102 @@ static TsWs ts_ws(double* base, long n) {
   w.ndest = w.dest + kPanelW;
   return w;
 }
-// Pivot keys: 64-bit, larger wins. High word: the magnitude's top 32 bits (float bits for FP32
-// values; bits 62..31 of the double for FP64 values, i.e. exponent and 21 mantissa bits, monotonic
-// for non-negative values), low word: ~row, so equal magnitudes prefer the smaller row. 0 = no row.
-__device__ inline unsigned long long ts_key(float v, int row) {
-  return ((unsigned long long)__float_as_uint(fabsf(v)) << 32) | (unsigned)(~row);
+// Pivot keys: 32 bits, larger wins, 0 = no candidate. A candidate's key is its magnitude's bits
+// shifted to leave room for the +1 that keeps a zero value distinguishable from "no row": FP32 bits
+// 30..1, FP64 bits 62..32 (exponent and 20 mantissa bits); both are monotonic in the magnitude.
+// Magnitudes that agree in these bits tie, and ties go to the lowest lane, then the lowest wave:
+// a fixed rule over a fixed tree shape, so results are deterministic.
+__device__ inline unsigned ts_key(float v) { return (__float_as_uint(fabsf(v)) >> 1) + 1u; }
+__device__ inline unsigned ts_key(double v) {
+  return (unsigned)((unsigned long long)__double_as_longlong(fabs(v)) >> 32) + 1u;
 }
-__device__ inline unsigned long long ts_key(double v, int row) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v));
-  return ((b >> 31) << 32) | (unsigned)(~row);
+// The winning lane's row into its wave's LDS slot, one element per store instruction. Written with
+// the compiler's own stores, the row is gathered into aligned 16-byte register tuples, which ties
+// a[] to a tuple and makes every later partial update of it copy the whole array.
+template <typename T>
+__device__ __forceinline__ void ts_store_row(T* dst, const T (&a)[kPanelW]) {
+  const unsigned base = (unsigned)(size_t)dst;
+#pragma unroll
+  for (int k = 0; k < kPanelW; ++k) {
+    if constexpr (sizeof(T) == 4)
+      asm volatile("ds_write_b32 %0, %1 offset:%2" ::"v"(base), "v"(a[k]), "i"(4 * k) : "memory");
+    else
+      asm volatile("ds_write_b64 %0, %1 offset:%2" ::"v"(base), "v"(a[k]), "i"(8 * k) : "memory");
+  }
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 }
 template <int CTRL>
-__device__ inline unsigned long long dpp_u64(unsigned long long k) {
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)k, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(k >> 32), CTRL, 0xF, 0xF, false);
-  return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+__device__ inline unsigned dpp_max_u32(unsigned k) {
+  return max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, CTRL, 0xF, 0xF, false));
 }
-__device__ inline unsigned long long umax64(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
-// maximum over the 64 lanes (all lanes active), returned in every lane: within each row of 16 lanes
-// by DPP (quad permutations, half-row and row mirrors), then across the four rows by readlane
-__device__ inline unsigned long long wave_max_u64(unsigned long long k) {
-  k = umax64(k, dpp_u64<0xB1>(k));   // quad_perm [1,0,3,2]
-  k = umax64(k, dpp_u64<0x4E>(k));   // quad_perm [2,3,0,1]
-  k = umax64(k, dpp_u64<0x141>(k));  // row_half_mirror
-  k = umax64(k, dpp_u64<0x140>(k));  // row_mirror
-  unsigned long long r = 0;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)k, 16 * q);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(k >> 32), 16 * q);
-    r = umax64(r, ((unsigned long long)hi << 32) | lo);
-  }
-  return r;
+// maximum over the 64 lanes (all lanes active), in every lane: within each row of 16 lanes by DPP
+// (quad permutations, half-row and row mirrors), then across the four rows by readlane
+__device__ inline unsigned wave_max_u32(unsigned k) {
+  k = dpp_max_u32<0xB1>(k);   // quad_perm [1,0,3,2]
+  k = dpp_max_u32<0x4E>(k);   // quad_perm [2,3,0,1]
+  k = dpp_max_u32<0x141>(k);  // row_half_mirror
+  k = dpp_max_u32<0x140>(k);  // row_mirror
+  const unsigned k0 = (unsigned)__builtin_amdgcn_readlane((int)k, 0);
+  const unsigned k1 = (unsigned)__builtin_amdgcn_readlane((int)k, 16);
+  const unsigned k2 = (unsigned)__builtin_amdgcn_readlane((int)k, 32);
+  const unsigned k3 = (unsigned)__builtin_amdgcn_readlane((int)k, 48);
+  return max(max(k0, k1), max(k2, k3));
 }
 
 // Partial-pivoting elimination of the workgroup's rows (thread = row), choosing up to w pivot rows.
 // a: the row's values, updated in place (multipliers left of the pivot column, reduced values right
 // of it); mystep: the step at which this row was chosen, or -1. Returns the number of steps that
-// found a row. Per step: a wave maximum of the pivot keys, whose winning lane also publishes its row
-// in the wave's LDS slot; one barrier; every thread takes the best of the four waves and eliminates
-// with that slot's row. Buffers alternate by step parity, so one barrier per step suffices.
+// found a row. Per step: a wave maximum of the pivot keys, whose lowest winning lane publishes its
+// row in the wave's LDS slot; one barrier; every thread takes the best of the four waves and
+// eliminates with that slot's row. Buffers alternate by step parity, so one barrier per step
+// suffices. The update is branch-free: rows that do not take part use a zero multiplier (all values
+// are finite), which leaves them unchanged.
 template <typename T>
-__device__ __forceinline__ int ts_select(T (&a)[kPanelW], bool valid, int gidx, int w,
-                                         unsigned long long* skey, T* sp, int& mystep) {
+__device__ __forceinline__ int ts_select(T (&a)[kPanelW], bool valid, int w, unsigned* skey,
+                                         int* swin, T* sp, int& mystep) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  (void)lane;
   mystep = -1;
   int nsel = 0;
 #pragma unroll
   for (int c = 0; c < kPanelW; ++c) {
-    if (c < w) {
+    {
       const int par = c & 1;
-      const bool cand = valid && mystep < 0;
-      const unsigned long long key = cand ? ts_key(a[c], gidx) : 0ull;
-      const unsigned long long wmax = wave_max_u64(key);
-      if (cand && key == wmax) {
-#pragma unroll
-        for (int k = 0; k < kPanelW; ++k) sp[(par * 4 + wv) * kPanelW + k] = a[k];
+      const bool cand = valid && mystep < 0 && c < w;  // steps past w find no candidate
+      const unsigned key = cand ? ts_key(a[c]) : 0u;
+      const unsigned wmax = wave_max_u32(key);
+      const unsigned long long hit = __ballot(key == wmax);
+      const int wl = __ffsll((long long)hit) - 1;
+      if (lane == wl) ts_store_row<T>(sp + (par * 4 + wv) * kPanelW, a);
+      if (lane == 0) {
+        skey[par * 4 + wv] = wmax;
+        swin[par * 4 + wv] = wv * 64 + wl;
       }
-      if ((threadIdx.x & 63) == 0) skey[par * 4 + wv] = wmax;
       __syncthreads();
-      unsigned long long best = skey[par * 4];
+      unsigned best = skey[par * 4];
       int bw = 0;
 #pragma unroll
       for (int q = 1; q < kTsRows / 64; ++q) {
-        const unsigned long long k2 = skey[par * 4 + q];
+        const unsigned k2 = skey[par * 4 + q];
         if (k2 > best) {
           best = k2;
           bw = q;
         }
       }
-      if (best != 0ull) {
-        nsel = c + 1;
-        if (cand && key == best) mystep = c;
-        const T* prow = sp + (par * 4 + bw) * kPanelW;
-        // the pivot row into registers first (values past w are zero in every row), then the update
-        T pr[kPanelW];
+      // no branch on `found`: a column without candidates gives l = 0 everywhere (the slot read
+      // then holds some row's finite values)
+      const bool found = best != 0u;
+      nsel = found ? c + 1 : nsel;
+      if (found && (int)threadIdx.x == swin[par * 4 + bw]) mystep = c;
+      const T* prow = sp + (par * 4 + bw) * kPanelW;
+      T pr[kPanelW];
 #pragma unroll
-        for (int k = c; k < kPanelW; ++k) pr[k] = prow[k];
-        if (valid && mystep < 0) {
-          const T piv = pr[c];
-          const T l = piv != T(0) ? a[c] / piv : T(0);
-          a[c] = l;
+      for (int k = c; k < kPanelW; ++k) pr[k] = prow[k];
+      const bool act = found && valid && mystep < 0;
+      const T piv = pr[c];
+      const T l = (act && piv != T(0)) ? a[c] / piv : T(0);
+      a[c] = act ? l : a[c];
 #pragma unroll
-          for (int k = c + 1; k < kPanelW; ++k) a[k] -= l * pr[k];
-        }
-      }
+      for (int k = c + 1; k < kPanelW; ++k) a[k] = fma(-l, pr[k], a[k]);
     }
   }
   return nsel;
@@ -702,8 +711,9 @@ template <typename T>
 __global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, long lda, long n,
                                                      long j0, int w, int P, TsWs ws,
                                                      int* __restrict__ ipiv, int* __restrict__ info) {
-  __shared__ unsigned long long skey[8];
-  __shared__ T sp[2 * 4 * kPanelW];
+  __shared__ unsigned skey[8];
+  __shared__ int swin[8];
+  __shared__ __align__(16) T sp[2 * 4 * kPanelW];
   __shared__ int s_last;
   __shared__ int s_win[kPanelW];
   __shared__ double s_diag[kPanelW];
@@ -723,7 +733,7 @@ __global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, lo
   long src = -1;
   int mystep = -1, nsel = 0;
   for (;;) {
-    nsel = ts_select<T>(a, valid, gidx, w, skey, sp, mystep);
+    nsel = ts_select<T>(a, valid, w, skey, swin, sp, mystep);
     if (count == 1) break;  // this workgroup holds the root set
     const long slot = off + node;
     if (mystep >= 0) {
@@ -838,16 +848,24 @@ __global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, lo
 template <typename T>
 __global__ __launch_bounds__(kTsRows) void k_ts_apply(T* __restrict__ A, long lda, long n, long j0,
                                                       int w, TsWs ws) {
-  __shared__ T sU[kPanelW][kPanelW + 1];
+  __shared__ T sLU[kPanelW][kPanelW + 1];            // root rows in pivot order: L11\U11
+  __shared__ __align__(16) T sUt[kPanelW][kPanelW];  // sUt[c][k] = U11(k, c), k < c (else 0)
+  __shared__ T sR[kPanelW];                          // 1 / U11(c, c) (1 where the pivot is zero)
   __shared__ int sd[kPanelW];
   __shared__ int snd;
   const int t = threadIdx.x;
   for (int u = t; u < kPanelW * kPanelW; u += kTsRows) {
     const int i = u / kPanelW, c = u % kPanelW;
-    sU[i][c] = (i < w && c < w) ? (T)ws.lu11[i * kPanelW + c] : T(0);
+    const T v = (i < w && c < w) ? (T)ws.lu11[i * kPanelW + c] : T(0);
+    sLU[i][c] = v;
+    sUt[c][i] = i < c ? v : T(0);
+  }
+  if (t < kPanelW) {
+    const T d = t < w ? (T)ws.lu11[t * kPanelW + t] : T(0);
+    sR[t] = d != T(0) ? T(1) / d : T(1);
+    sd[t] = ws.dest[t];
   }
   if (t == 0) snd = *ws.ndest;
-  if (t < kPanelW) sd[t] = ws.dest[t];
   __syncthreads();
   const long r = j0 + (long)blockIdx.x * kTsRows + t;
   if (r >= n) return;
@@ -855,29 +873,54 @@ __global__ __launch_bounds__(kTsRows) void k_ts_apply(T* __restrict__ A, long ld
     const int i = (int)(r - j0);
 #pragma unroll
     for (int c = 0; c < kPanelW; ++c)
-      if (c < w) A[(j0 + c) * lda + r] = sU[i][c];
+      if (c < w) A[(j0 + c) * lda + r] = sLU[i][c];
     return;
   }
   int e = -1;
   for (int q = 0; q < snd; ++q)
     if (sd[q] == r) e = q;
+  // columns past w read column w - 1 (inside the panel) and are zeroed: no per-column branches
+  T* col = A + j0 * lda + r;
   T a[kPanelW];
+  if (e >= 0) {
 #pragma unroll
-  for (int c = 0; c < kPanelW; ++c)
-    a[c] = c < w ? (e >= 0 ? (T)ws.saved[e * kPanelW + c] : A[(j0 + c) * lda + r]) : T(0);
+    for (int c = 0; c < kPanelW; ++c) {
+      const T v = (T)ws.saved[e * kPanelW + min(c, w - 1)];
+      a[c] = c < w ? v : T(0);
+    }
+  } else {
 #pragma unroll
-  for (int c = 0; c < kPanelW; ++c) {
-    if (c < w) {
-      T x = a[c];
-#pragma unroll
-      for (int k = 0; k < c; ++k) x -= a[k] * sU[k][c];
-      const T d = sU[c][c];
-      a[c] = d != T(0) ? x / d : x;
+    for (int c = 0; c < kPanelW; ++c) {
+      const T v = col[(long)min(c, w - 1) * lda];
+      a[c] = c < w ? v : T(0);
     }
   }
+  // the row of L21 = a U11^-1: l_c = (a_c - sum_{k<c} l_k U11(k, c)) / U11(c, c), with two partial
+  // sums to halve the dependent chain (entries past w stay zero). Column c's LDS row address passes
+  // through an empty asm that consumes l_{c-2}: otherwise every U read is hoisted to the top and
+  // the 496 values spill.
+  typedef __attribute__((address_space(3))) const T LdsT;
 #pragma unroll
-  for (int c = 0; c < kPanelW; ++c)
-    if (c < w) A[(j0 + c) * lda + r] = a[c];
+  for (int c = 0; c < kPanelW; ++c) {
+    LdsT* ut = (LdsT*)&sUt[c][0];
+    if (c >= 2) asm volatile("" : "+v"(ut) : "v"(a[c - 2]));
+    T x0 = a[c], x1 = T(0);
+#pragma unroll
+    for (int k = 0; k + 1 < c; k += 2) {
+      x0 = fma(-a[k], ut[k], x0);
+      x1 = fma(-a[k + 1], ut[k + 1], x1);
+    }
+    if (c & 1) x0 = fma(-a[c - 1], ut[c - 1], x0);
+    a[c] = (x0 + x1) * sR[c];
+  }
+  if (w == kPanelW) {
+#pragma unroll
+    for (int c = 0; c < kPanelW; ++c) col[(long)c * lda] = a[c];
+  } else {
+#pragma unroll
+    for (int c = 0; c < kPanelW; ++c)
+      if (c < w) col[(long)c * lda] = a[c];
+  }
 }
 
 // ============================================================================ row swaps ==========
@@ -1118,7 +1161,8 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
     const long kb = std::min(kLuNB, n - k);
     if (k > 0) hipStreamWaitEvent(st, W.ev_b, 0);  // panel k, factored on the second stream
     if (tick) tick(tctx, "lu.swap_trsm", 0, st);
-    laswp<T>(A, lda, 0, k, W.d_ipiv, k, k + kb, st);
+    // the interchanges of this step go to the trailing columns only; L's earlier block columns
+    // keep their order (the solve applies the interchanges to the vector instead)
     if (k + kb < n) {
       laswp<T>(A, lda, k + kb, n - k - kb, W.d_ipiv, k, k + kb, st);
       if (sizeof(T) == 4)
@@ -1174,14 +1218,6 @@ size_t lu_inverse_doubles(long n) {
   return (size_t)nb * kTB * kTB * 2;
 }
 
-// y <- y[perm] (the factorisation's row interchanges as one gather)
-template <typename T>
-__global__ __launch_bounds__(256) void k_gather(T* __restrict__ y, const T* __restrict__ x,
-                                                const int* __restrict__ perm, long n) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = x[perm[i]];
-}
-
 // x_blk <- inverse block b times y_blk (kb <= kTB entries at y + r0), 128 threads of workgroup 0
 template <typename T>
 __device__ void trsv_apply_inverse(const T* __restrict__ invb, int kb, const T* sy, T* __restrict__ y,
@@ -1195,23 +1231,16 @@ __device__ void trsv_apply_inverse(const T* __restrict__ invb, int kb, const T* 
 }
 
 // Forward sweep step. Block k (rows [k0, k0 + kb)) of y is solved; rows below it are reduced by
-// L[:, block k] y_k, and workgroup 0 (rows of block k + 1) then solves block k + 1. With k0 < 0 the
-// launch only solves block 0.
+// L[:, block k] y_k, and workgroup 0 (rows of block k + 1) then solves block k + 1 unless that block
+// starts an LU step (solve_next = 0: k_trsv_fwd_step does it after the step's interchanges).
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ LU, long lda, long n, long k0,
                                                   int kb, const T* __restrict__ inv,
-                                                  T* __restrict__ y) {
+                                                  T* __restrict__ y, int solve_next) {
   __shared__ T sx[kTB];
   __shared__ T part[2][kTB];
   __shared__ T sy[kTB];
   const int t = threadIdx.x, h = t / kTB, rl = t % kTB;
-  if (k0 < 0) {
-    const int nb0 = (int)min((long)kTB, n);
-    if (t < nb0) sy[t] = y[t];
-    __syncthreads();
-    trsv_apply_inverse(inv, nb0, sy, y, 0);
-    return;
-  }
   if (t < kb) sx[t] = y[k0 + t];
   __syncthreads();
   const long r0 = k0 + kb + (long)blockIdx.x * kTB;
@@ -1225,18 +1254,39 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ LU, long
   }
   part[h][rl] = acc;
   __syncthreads();
+  const bool own = blockIdx.x == 0 && solve_next;
   if (h == 0 && r < n) {
     const T v = y[r] - (part[0][rl] + part[1][rl]);
-    if (blockIdx.x == 0)
+    if (own)
       sy[rl] = v;
     else
       y[r] = v;
   }
-  if (blockIdx.x == 0) {
+  if (own) {
     __syncthreads();
     const int kn = (int)min((long)kTB, n - r0);
     trsv_apply_inverse(inv + (size_t)(r0 / kTB) * kTB * kTB, kn, sy, y, r0);
   }
+}
+
+// Forward sweep at the first block of an LU step: the step's interchanges applied to y as moves
+// y[dst] <- y[src] (all reads, then all writes), then the diagonal block at r0 solved with its
+// inverse. One workgroup; a step has at most 2 IBEM_LU_NB moves.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_fwd_step(const int* __restrict__ mv, int nmv, long r0,
+                                                       long n, const T* __restrict__ inv,
+                                                       T* __restrict__ y) {
+  __shared__ T sv[2 * IBEM_LU_NB];
+  __shared__ T sy[kTB];
+  const int t = threadIdx.x;
+  for (int q = t; q < nmv; q += 256) sv[q] = y[mv[2 * q + 1]];
+  __syncthreads();
+  for (int q = t; q < nmv; q += 256) y[mv[2 * q]] = sv[q];
+  __syncthreads();
+  const int kn = (int)min((long)kTB, n - r0);
+  if (t < kn) sy[t] = y[r0 + t];
+  __syncthreads();
+  trsv_apply_inverse(inv + (size_t)(r0 / kTB) * kTB * kTB, kn, sy, y, r0);
 }
 
 // Backward sweep step. Block k (rows [k0, k0 + kb)) of y is solved; rows above it are reduced by
@@ -1283,20 +1333,56 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ LU, long
   }
 }
 
+size_t lu_moves_ints(long n) { return (size_t)4 * n + 8; }
+
 template <typename T>
 static int lu_prepare(const T* LU, long n, long lda, LuWork& W, hipStream_t st) {
+  static_assert(IBEM_LU_NB % kTB == 0, "an LU step spans whole solve blocks");
   const long nb = (n + kTB - 1) / kTB;
   T* inv = (T*)W.d_inv;
   hipLaunchKernelGGL((k_trinv<T, false>), dim3((unsigned)nb), dim3(kTB), 0, st, LU, lda, n, inv);
   hipLaunchKernelGGL((k_trinv<T, true>), dim3((unsigned)nb), dim3(kTB), 0, st, LU, lda, n,
                      inv + (size_t)nb * kTB * kTB);
-  std::vector<int> ip(n), perm(n);
+  std::vector<int> ip(n);
   if (hipMemcpyAsync(ip.data(), W.d_ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     return -1;
-  for (long i = 0; i < n; ++i) perm[i] = (int)i;
-  for (long i = 0; i < n; ++i) std::swap(perm[i], perm[ip[i]]);
-  if (hipMemcpy(W.d_perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) return -1;
+  // each step's interchanges composed into moves y[dst] <- y[src] over the positions they touch
+  // (loc[x]: the position, at the start of the step, of the entry now at x; identity elsewhere)
+  std::vector<int> loc(n), mv, touched;
+  for (long i = 0; i < n; ++i) loc[i] = (int)i;
+  std::vector<char> mark(n, 0);
+  W.mv_off.clear();
+  for (long k = 0; k < n; k += IBEM_LU_NB) {
+    const long kb = std::min<long>(IBEM_LU_NB, n - k);
+    W.mv_off.push_back((int)(mv.size() / 2));
+    touched.clear();
+    for (long i = k; i < k + kb; ++i) {
+      const long p = ip[i];
+      if (p < i || p >= n) return -1;
+      for (long x : {i, p})
+        if (!mark[x]) {
+          mark[x] = 1;
+          touched.push_back((int)x);
+        }
+      std::swap(loc[i], loc[p]);
+    }
+    for (int x : touched) {
+      if (loc[x] != x) {
+        mv.push_back(x);
+        mv.push_back(loc[x]);
+      }
+    }
+    for (int x : touched) {
+      loc[x] = x;
+      mark[x] = 0;
+    }
+  }
+  W.mv_off.push_back((int)(mv.size() / 2));
+  if (mv.size() > lu_moves_ints(n)) return -1;
+  if (!mv.empty() &&
+      hipMemcpy(W.d_perm, mv.data(), sizeof(int) * mv.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return -1;
   return 0;
 }
 
@@ -1306,14 +1392,21 @@ static void lu_solve(const T* LU, long n, long lda, const LuWork& W, T* x, hipSt
   const T* invL = (const T*)W.d_inv;
   const T* invU = invL + (size_t)nb * kTB * kTB;
   T* y = (T*)W.d_tmp;
-  const dim3 vg((unsigned)((n + 255) / 256));
-  hipLaunchKernelGGL(k_gather<T>, vg, dim3(256), 0, st, y, (const T*)x, (const int*)W.d_perm, n);
-  hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(1), dim3(256), 0, st, LU, lda, n, -1L, 0, invL, y);
-  for (long b = 0; b + 1 < nb; ++b) {
+  hipMemcpyAsync(y, x, sizeof(T) * n, hipMemcpyDeviceToDevice, st);
+  for (long b = 0; b < nb; ++b) {
     const long k0 = b * kTB;
-    const long rows = n - k0 - kTB;
-    hipLaunchKernelGGL(k_trsv_fwd<T>, dim3((unsigned)((rows + kTB - 1) / kTB)), dim3(256), 0, st, LU,
-                       lda, n, k0, kTB, invL, y);
+    if (k0 % IBEM_LU_NB == 0) {  // first block of an LU step: its interchanges, then the block
+      const long j = k0 / IBEM_LU_NB;
+      const int m0 = W.mv_off[j], nm = W.mv_off[j + 1] - m0;
+      hipLaunchKernelGGL(k_trsv_fwd_step<T>, dim3(1), dim3(256), 0, st, (const int*)W.d_perm + 2 * m0,
+                         nm, k0, n, invL, y);
+    }
+    if (b + 1 < nb) {
+      const long rows = n - k0 - kTB;
+      const int solve_next = ((b + 1) * kTB) % IBEM_LU_NB != 0;
+      hipLaunchKernelGGL(k_trsv_fwd<T>, dim3((unsigned)((rows + kTB - 1) / kTB)), dim3(256), 0, st, LU,
+                         lda, n, k0, kTB, invL, y, solve_next);
+    }
   }
   const long lastk = (nb - 1) * kTB;
   hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(1), dim3(256), 0, st, LU, lda, n, -1L, 0, lastk, invU, y);
