@@ -949,6 +949,109 @@ static void laswp(T* A, long lda, long c0, long nc, const int* ipiv, long p0, lo
                      ipiv, p0, p1);
 }
 
+// The same interchanges for a narrow column range (inside a panel), where one thread per column
+// walking them in order is bound by its dependent memory round trips. Every workgroup composes
+// ipiv[p0, p0 + m) into moves A[dst] <- A[src] over the positions they touch (an LDS table: the m
+// top positions directly, positions below them through an open-addressing hash), then moves its
+// columns, kLaswpCG at a time: all reads, a barrier, all writes.
+constexpr int kLaswpH = 2048;  // hash slots: at least twice the positions below the range (<= m)
+constexpr int kLaswpCG = 4;    // columns per read/write round
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp_mv(T* __restrict__ A, long lda, long c0, long nc,
+                                                  const int* __restrict__ ipiv, long p0, int m,
+                                                  int cpb) {
+  static_assert(2 * IBEM_LU_NB <= 8 * 256, "moves per column <= 8 per thread");
+  __shared__ int hkey[kLaswpH];                 // position below the range, or -1
+  __shared__ int val[IBEM_LU_NB + kLaswpH];     // slot -> original position of the entry there
+  __shared__ int slotp[IBEM_LU_NB];             // slot of ipiv[p0 + i]
+  __shared__ int mdst[2 * IBEM_LU_NB], msrc[2 * IBEM_LU_NB];
+  __shared__ int nmv;
+  const int t = threadIdx.x;
+  for (int h = t; h < kLaswpH; h += 256) hkey[h] = -1;
+  for (int i = t; i < m; i += 256) val[i] = (int)(p0 + i);
+  if (t == 0) nmv = 0;
+  __syncthreads();
+  for (int i = t; i < m; i += 256) {
+    const int p = ipiv[p0 + i];
+    int sl = p - (int)p0;
+    if (p >= p0 + m) {
+      unsigned h = ((unsigned)p * 2654435761u) >> 21;  // 11 bits: kLaswpH slots
+      for (;;) {
+        const int old = atomicCAS(&hkey[h], -1, p);
+        if (old == -1 || old == p) break;
+        h = (h + 1) & (kLaswpH - 1);
+      }
+      val[IBEM_LU_NB + h] = p;
+      sl = IBEM_LU_NB + (int)h;
+    }
+    slotp[i] = sl;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int i = 0; i < m; ++i) {
+      const int b = slotp[i];
+      if (b != i) {
+        const int va = val[i], vb = val[b];
+        val[i] = vb;
+        val[b] = va;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < m; i += 256)
+    if (val[i] != p0 + i) {
+      const int q = atomicAdd(&nmv, 1);
+      mdst[q] = (int)(p0 + i);
+      msrc[q] = val[i];
+    }
+  for (int h = t; h < kLaswpH; h += 256) {
+    const int k = hkey[h];
+    if (k >= 0 && val[IBEM_LU_NB + h] != k) {
+      const int q = atomicAdd(&nmv, 1);
+      mdst[q] = k;
+      msrc[q] = val[IBEM_LU_NB + h];
+    }
+  }
+  __syncthreads();
+  const int nm = nmv;
+  const long cb = c0 + (long)blockIdx.x * cpb, ce = min(cb + cpb, c0 + nc);
+  for (long g = cb; g < ce; g += kLaswpCG) {
+    T v[kLaswpCG][8];
+#pragma unroll
+    for (int j = 0; j < kLaswpCG; ++j) {
+      const T* col = A + min(g + j, ce - 1) * lda;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int q = t + 256 * k;
+        v[j][k] = q < nm ? col[msrc[q]] : T(0);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // every read of the group has returned before the barrier
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kLaswpCG; ++j) {
+      if (g + j < ce) {
+        T* col = A + (g + j) * lda;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int q = t + 256 * k;
+          if (q < nm) col[mdst[q]] = v[j][k];
+        }
+      }
+    }
+    __syncthreads();  // the group's writes are done before the next group reads (same rows)
+  }
+}
+
+template <typename T>
+static void laswp_narrow(T* A, long lda, long c0, long nc, const int* ipiv, long p0, long p1,
+                         hipStream_t st) {
+  if (nc <= 0 || p1 <= p0) return;
+  const long cpb = std::max<long>(kLaswpCG, (nc + 63) / 64 / kLaswpCG * kLaswpCG);
+  hipLaunchKernelGGL(k_laswp_mv<T>, dim3((unsigned)((nc + cpb - 1) / cpb)), dim3(256), 0, st, A, lda,
+                     c0, nc, ipiv, p0, (int)(p1 - p0), (int)cpb);
+}
+
 // ============================================================ small unit-lower triangular solve ==
 // B[m x N] <- L^{-1} B, L unit lower m x m (m <= 32); one thread per column of B.
 template <typename T>
@@ -978,9 +1081,85 @@ __global__ __launch_bounds__(256) void k_trsm_small(const T* __restrict__ L, lon
     if (i < m) col[i] = x[i];
 }
 
+// B[m x N] <- L^{-1} B for a unit-lower L with m <= 64 Q, for the few, tall-ish right-hand sides
+// inside a panel (N up to a few hundred): one wavefront per column of B, lane l holding rows
+// l + 64 q in registers. Step i broadcasts x_i from its lane (readlane) and every lane updates its
+// rows below i with column i of L, which the workgroup stages through LDS in chunks of CW columns
+// (only rows below the current 64-row block are staged and updated). Every index is a compile-time
+// constant, so the whole solve is straight-line code.
+
+// a lane's value, uniform across the wavefront (the lane index is uniform)
+template <typename T>
+__device__ __forceinline__ T readlane_t(T v, int lane) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  }
+}
+
+template <typename T, int Q>
+__global__ __launch_bounds__(256) void k_trsm_wave(const T* __restrict__ L, long ldl, int m,
+                                                   T* __restrict__ B, long ldb, long N) {
+  constexpr int CW = sizeof(T) == 4 ? 32 : 16;  // staged columns per chunk (32 KB of LDS at Q = 4)
+  __shared__ T sL[CW][64 * Q];                  // sL[c][r - 64 qb] = L(r, i0 + c) for r > i0 + c
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long col = (long)blockIdx.x * 4 + wv;
+  const bool has = col < N;
+  T* bcol = B + (has ? col : 0) * ldb;
+  T x[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int r = lane + 64 * q;
+    x[q] = (has && r < m) ? bcol[r] : T(0);
+  }
+#pragma unroll
+  for (int qb = 0; qb < Q; ++qb) {
+    if (64 * qb < m) {  // uniform; a condition rather than a break keeps the loop unrolled
+      const int rows = 64 * (Q - qb);  // staged rows [64 qb, 64 Q)
+#pragma unroll
+      for (int h = 0; h < 64 / CW; ++h) {
+        const int i0 = 64 * qb + h * CW;
+        __syncthreads();  // the previous chunk has been used
+#pragma unroll 2
+        for (int u = threadIdx.x; u < CW * rows; u += 256) {
+          const int c = u / rows, rr = u % rows, r = 64 * qb + rr, i = i0 + c;
+          sL[c][rr] = (r > i && r < m) ? L[(long)i * ldl + r] : T(0);
+        }
+        __syncthreads();
+        // steps past m are harmless: x_i = 0 there and the staged column is zero
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+          const T xi = readlane_t(x[qb], h * CW + c);
+#pragma unroll
+          for (int q = qb; q < Q; ++q) x[q] = fma(-sL[c][64 * (q - qb) + lane], xi, x[q]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int r = lane + 64 * q;
+    if (has && r < m) bcol[r] = x[q];
+  }
+}
+
 template <typename T>
 static void trsm_lower_unit(const T* L, long ldl, long m, T* B, long ldb, long N, hipStream_t st) {
   if (m <= 0 || N <= 0) return;
+  if (N <= 4096 && m <= 256) {  // few right-hand sides (inside a panel): one wavefront per column
+    const dim3 g((unsigned)((N + 3) / 4));
+    if (m <= 64)
+      hipLaunchKernelGGL((k_trsm_wave<T, 1>), g, dim3(256), 0, st, L, ldl, (int)m, B, ldb, N);
+    else if (m <= 128)
+      hipLaunchKernelGGL((k_trsm_wave<T, 2>), g, dim3(256), 0, st, L, ldl, (int)m, B, ldb, N);
+    else
+      hipLaunchKernelGGL((k_trsm_wave<T, 4>), g, dim3(256), 0, st, L, ldl, (int)m, B, ldb, N);
+    return;
+  }
   if (m <= 32) {
     hipLaunchKernelGGL(k_trsm_small<T>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, L, ldl,
                        (int)m, B, ldb, N);
@@ -1024,12 +1203,12 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
   const long w1 = (w / 2 + kPanelW - 1) / kPanelW * kPanelW;
   const long lda = c.lda;
   panel_rec(c, j0, w1);
-  laswp<T>(c.A, lda, j0 + w1, w - w1, c.W->d_ipiv, j0, j0 + w1, c.st);
+  laswp_narrow<T>(c.A, lda, j0 + w1, w - w1, c.W->d_ipiv, j0, j0 + w1, c.st);
   trsm_lower_unit<T>(c.A + j0 * lda + j0, lda, w1, c.A + (j0 + w1) * lda + j0, lda, w - w1, c.st);
   gemm_sub<T>(c.n - j0 - w1, w - w1, w1, c.A + j0 * lda + j0 + w1, lda, c.A + (j0 + w1) * lda + j0,
               lda, c.A + (j0 + w1) * lda + j0 + w1, lda, c.st);
   panel_rec(c, j0 + w1, w - w1);
-  laswp<T>(c.A, lda, j0, w1, c.W->d_ipiv, j0 + w1, j0 + w, c.st);
+  laswp_narrow<T>(c.A, lda, j0, w1, c.W->d_ipiv, j0 + w1, j0 + w, c.st);
 }
 
 constexpr long kLuNB = IBEM_LU_NB;
