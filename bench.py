@@ -31,10 +31,16 @@ from ibemproject_amd.generate import make_workload  # noqa: E402
 MI355X_PEAK = {"f16_mfma_tflops": 2500.0, "f32_mfma_tflops": 157.3, "f64_mfma_tflops": 78.6,
                "hbm_gbs": 8000.0}
 NB = 1024  # LU block size (solver.hip IBEM_LU_NB)
+# HBM bytes per k_gemm_f16x3 launch on rve1000_m16 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch,
+# averaged over one LU's launches; the factors make the counters read 1 GiB for a 1 GiB copy), from
+# the counter runs recorded in profiles/k_gemm_f16x3__rve1000_m16.md
+GEMM_TRAFFIC_BYTES_PER_LAUNCH = 1.4276e10
 
 
 def trailing_update_flops(n: int) -> tuple[float, int]:
-    """Algorithmic flops of the blocked LU's trailing-matrix updates and their launch count."""
+    """Algorithmic flops of the blocked LU's trailing-matrix updates (2 m^2 kb per step, m the
+    trailing size) and the number of k_gemm_f16x3 launches: two per step (the next panel's columns,
+    then the rest), one when the next panel is the whole trailing matrix."""
     flops = 0.0
     launches = 0
     for k in range(0, n, NB):
@@ -42,8 +48,40 @@ def trailing_update_flops(n: int) -> tuple[float, int]:
         m = n - k - kb
         if m > 0:
             flops += 2.0 * m * m * kb
-            launches += 1
+            launches += 1 + (1 if m > min(NB, m) else 0)
     return flops, launches
+
+
+def trailing_update_bytes(n: int) -> float:
+    """Algorithmic bytes per k_gemm_f16x3 launch, averaged over one LU: the split operands (hi + lo,
+    2 bytes each) read once, C read and written once in FP32 (4 m kb + 4 kb nc + 8 m nc)."""
+    total, launches = 0.0, 0
+    for k in range(0, n, NB):
+        kb = min(NB, n - k)
+        m = n - k - kb
+        if m <= 0:
+            continue
+        kb2 = min(NB, m)
+        for nc in ([kb2, m - kb2] if m > kb2 else [kb2]):
+            total += 4.0 * m * kb + 4.0 * kb * nc + 8.0 * m * nc
+            launches += 1
+    return total / launches if launches else 0.0
+
+
+def progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def calibrate_copy(gib: int = 1, reps: int = 3) -> None:
+    """A plain device copy of known size (torch's elementwise kernel, gib GiB read and written per
+    launch): the reference reading for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    x = torch.ones((gib << 30) // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for i in range(reps):
+        torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        progress(f"copy {i + 1}/{reps}: {gib} GiB read, {gib} GiB written")
 
 
 def run_step(eng: Engine, mode: int, tol: float = 1e-10):
@@ -136,7 +174,12 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py: this engine solves one RVE on one GPU; --gpus {args.gpus} is not "
               "supported (a multi-GPU run would shard the LU block-cyclically, see DESIGN.md)",
@@ -146,9 +189,11 @@ def main():
     prob = make_workload(args.workload, seed=args.seed)
     eng = Engine(prob)
     n = eng.n
+    progress(f"workload {args.workload}: n = {n}")
     MODE = SOLVE_MIXED_FP16
-    for _ in range(args.warmup):
+    for i in range(args.warmup):
         run_step(eng, MODE)
+        progress(f"warmup step {i + 1}/{args.warmup} done")
     eng.reset_timings()
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
@@ -160,6 +205,7 @@ def main():
         rels.append((rel, it))
     hip.hipDeviceSynchronize()
     t1 = time.perf_counter()
+    progress(f"{args.steps} timed steps done: {(t1 - t0) / args.steps:.3f} s per step")
     tm = eng.timings()
     step_s = (t1 - t0) / args.steps
     flops, launches = trailing_update_flops(n)
@@ -184,9 +230,15 @@ def main():
                 "phase_ms": t_o}
 
     fp64 = None if args.no_fp64 else one_step(SOLVE_FP64_LU, "solve.lu_f64")
+    if fp64:
+        progress(f"FP64 LU step: {fp64['step_s']:.3f} s")
     fp32 = None if args.no_fp64 else one_step(SOLVE_MIXED_FP32, "solve.lu_f32")
+    if fp32:
+        progress(f"FP32-MFMA mixed step: {fp32['step_s']:.3f} s")
 
     base = None if args.no_cpu_baseline else cpu_baseline(prob, args.workload)
+    if base:
+        progress(f"CPU baseline: {base['value']:.1f} s (extrapolated)")
     out = {
         "metric": "RVE solve time at 1e-10 FP64 residual vs plain FP64 LU; assembly/LU TFLOP/s",
         "value": step_s,
@@ -219,12 +271,17 @@ def main():
                      "bound": "mfma", "achieved": achieved,
                      "peak": MI355X_PEAK["f16_mfma_tflops"], "unit": "TFLOP/s",
                      "frac": achieved / MI355X_PEAK["f16_mfma_tflops"] if achieved else None,
-                     "traffic": None, "launches_per_step": launches,
+                     "traffic": GEMM_TRAFFIC_BYTES_PER_LAUNCH if args.workload == "rve1000_m16" else None,
+                     "traffic_unit": "bytes per launch (HBM, counter-measured, averaged over the "
+                                     "step's launches)",
+                     "traffic_algorithmic": trailing_update_bytes(n),
+                     "launches_per_step": launches,
                      "avg_launch_ms": gemm_ms / launches if launches else None,
                      "flops_per_step": flops,
                      "mfma_issued_tflops": 3 * achieved if achieved else None,
-                     "note": "achieved = algorithmic 2 m^2 256 flop per launch / launch time; the "
-                             "kernel issues three fp16 MFMA products per algorithmic one"},
+                     "note": f"achieved = algorithmic trailing-update flops (2 m^2 {NB} per LU "
+                             "step) / the k_gemm_f16x3 time of the step; the kernel issues three "
+                             "fp16 MFMA products per algorithmic one"},
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)

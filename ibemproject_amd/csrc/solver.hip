@@ -1361,14 +1361,18 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
     }
     if (tick) tick(tctx, "lu.gemm", 0, st);
     trailing_update<T>(A, n, lda, k, kb, 0, kb2, w, st);
+    if (tick) tick(tctx, "lu.gemm", 1, st);
     hipEventRecord(W.ev_a, st);
+    // the rest of the update is enqueued before the panel's many small launches, so that the host's
+    // time spent launching them never leaves the main stream waiting
+    if (tick) tick(tctx, "lu.gemm", 0, st);
+    trailing_update<T>(A, n, lda, k, kb, kb2, m - kb2, w, st);
+    if (tick) tick(tctx, "lu.gemm", 1, st);
     hipStreamWaitEvent(W.st2, W.ev_a, 0);
     if (tick) tick(tctx, "lu.panel", 0, W.st2);
     panel_rec(c2, k + kb, kb2);
     if (tick) tick(tctx, "lu.panel", 1, W.st2);
     hipEventRecord(W.ev_b, W.st2);
-    trailing_update<T>(A, n, lda, k, kb, kb2, m - kb2, w, st);
-    if (tick) tick(tctx, "lu.gemm", 1, st);
   }
   return 0;
 }
