@@ -48,6 +48,10 @@ struct ibem3d_ctx {
   double* d_f16ws = nullptr;  // split operands of the FP16-MFMA trailing update
   double* d_eqws = nullptr;   // equilibration partial row maxima
   float* d_v32 = nullptr;
+  // GMRES-IR: Krylov basis V (kGmres + 1 columns), preconditioned directions Z (kGmres columns),
+  // dot-product partial sums, small coefficient vector
+  double *d_gV = nullptr, *d_gZ = nullptr, *d_gpart = nullptr, *d_gh = nullptr;
+  double* d_bemws = nullptr;  // boundary-boundary quadrature values of one round of elements
   LuWork W;
   unsigned phases = 0;
   bool solved = false;
@@ -250,6 +254,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
                   ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32, ctx->d_f16ws, ctx->d_eqws,
+                  ctx->d_gV,    ctx->d_gZ,       ctx->d_gpart,    ctx->d_gh,   ctx->d_bemws,
                   ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info, ctx->W.d_perm, ctx->W.d_inv, ctx->W.d_tmp,
                   ctx->W.d_trinv, ctx->W.d_trsm};
   for (void* q : ptrs)
@@ -280,9 +285,11 @@ int ibem3d_add_bem(ibem3d_ctx* ctx) {
   CTX_CHECK();
   {
     Scope s(ctx, "assemble.bem");
+    if (!ctx->d_bemws)
+      if (int rc = dalloc(ctx, &ctx->d_bemws, bem_workspace_doubles(ctx->NN, ctx->NE))) return rc;
     launch_bem(ctx->d_qp, ctx->d_conn, ctx->d_nodes, ctx->NN, ctx->NE, ctx->n, ctx->E0, ctx->nu0,
                ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b,
-               ctx->st);
+               ctx->d_bemws, ctx->st);
   }
   ctx->phases |= 1u;
   ctx->solved = false;
@@ -406,6 +413,61 @@ __global__ void k_axpy_scaled_f32(double* x, const float* d, const double* s, lo
 
 static dim3 vgrid(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// ---- GMRES-IR building blocks (FP64; reductions in a fixed order, so results are reproducible) ----
+constexpr int kGmres = 20;      // Krylov vectors per cycle
+constexpr int kDotBlocks = 128; // partial sums per dot product
+
+__global__ void k_scale_from_f32(double* y, const float* x, const double* s, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (double)x[i] * s[i];
+}
+__global__ void k_scale_by(double* y, const double* x, double a, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i];
+}
+// part[i * kDotBlocks + b] = block b's share of V_i . w, i < gridDim.y
+__global__ __launch_bounds__(256) void k_dot_part(const double* V, long ldv, const double* w, long n,
+                                                  double* part) {
+  __shared__ double red[256];
+  const double* v = V + (long)blockIdx.y * ldv;
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)kDotBlocks * 256)
+    acc += v[t] * w[t];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.y * kDotBlocks + blockIdx.x] = red[0];
+}
+// y += sign * sum_{i < k} c_i V_i
+__global__ void k_combo(double* y, const double* V, long ldv, int k, const double* c, double sign,
+                        long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double acc = 0.0;
+  for (int i = 0; i < k; ++i) acc += c[i] * V[(long)i * ldv + t];
+  y[t] += sign * acc;
+}
+
+// h[0..k) = V_i . w for the first k columns of V (device partial sums, host total in a fixed order)
+static int dots(ibem3d_ctx* ctx, const double* V, int k, const double* w, double* h) {
+  const long n = ctx->n;
+  hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks, (unsigned)k), dim3(256), 0, ctx->st, V, n, w, n,
+                     ctx->d_gpart);
+  std::vector<double> part((size_t)k * kDotBlocks);
+  HIPC(hipMemcpyAsync(part.data(), ctx->d_gpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                      ctx->st));
+  HIPC(hipStreamSynchronize(ctx->st));
+  for (int i = 0; i < k; ++i) {
+    double acc = 0.0;
+    for (int b = 0; b < kDotBlocks; ++b) acc += part[(size_t)i * kDotBlocks + b];
+    h[i] = acc;
+  }
+  return 0;
+}
+
 // relative residual ||b - A x|| / ||b|| with the FP64 system matrix
 static int residual(ibem3d_ctx* ctx, double* rel) {
   const long n = ctx->n;
@@ -423,8 +485,118 @@ static int residual(ibem3d_ctx* ctx, double* rel) {
   return 0;
 }
 
+// GMRES-IR (Carson and Higham): FP64 GMRES on A d = r, right-preconditioned by the FP32 factors of
+// the equilibrated system, M^-1 v = C LU32^-1 (R v); restarted every kGmres vectors, each cycle ending
+// with the true FP64 residual b - A x. Arnoldi by classical Gram-Schmidt applied twice; the small
+// Hessenberg least-squares problem by Givens rotations on the host. iters counts preconditioner
+// applications (LU32 solves). Each cycle costs one FP64 matrix-vector product more than the same
+// number of classic refinement steps, so the solve uses it when classic refinement stalls.
+// warm: start from the current x, whose residual b - A x is in d_res with relative norm rel0.
+static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0) {
+  const long n = ctx->n;
+  if (!ctx->d_gV) {
+    if (int rc = dalloc(ctx, &ctx->d_gV, (size_t)(kGmres + 1) * n)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_gZ, (size_t)kGmres * n)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_gpart, (size_t)(kGmres + 1) * kDotBlocks)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_gh, (size_t)kGmres + 1)) return rc;
+  }
+  double* V = ctx->d_gV;
+  double* Z = ctx->d_gZ;
+  double* w = ctx->d_work;
+  double bnorm = 0.0;
+  if (int rc = dots(ctx, ctx->d_b, 1, ctx->d_b, &bnorm)) return rc;
+  bnorm = std::sqrt(bnorm);
+  double rel = bnorm > 0 ? 1.0 : 0.0;
+  if (warm) {
+    rel = rel0;
+  } else {
+    HIPC(hipMemsetAsync(ctx->d_x, 0, sizeof(double) * n, ctx->st));
+    HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
+  }
+  double beta = rel * bnorm;
+  const int max_cycles = 10;
+  for (int cyc = 0; cyc < max_cycles && rel > tol; ++cyc) {
+    // V_0 = r / ||r||
+    hipLaunchKernelGGL(k_scale_by, vgrid(n), dim3(256), 0, ctx->st, V, ctx->d_res, 1.0 / beta, n);
+    std::vector<double> H((size_t)(kGmres + 1) * kGmres, 0.0), cs(kGmres), sn(kGmres),
+        g(kGmres + 1, 0.0);
+    g[0] = beta;
+    int k = 0;
+    for (int j = 0; j < kGmres; ++j) {
+      double* vj = V + (size_t)j * n;
+      double* zj = Z + (size_t)j * n;
+      {
+        Scope s(ctx, "solve.trsv_f32");
+        hipLaunchKernelGGL(k_scale_to_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_v32, vj, ctx->d_r, n);
+        lu_solve_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->d_v32, ctx->st);
+        hipLaunchKernelGGL(k_scale_from_f32, vgrid(n), dim3(256), 0, ctx->st, zj, ctx->d_v32, ctx->d_c, n);
+      }
+      {
+        Scope s(ctx, "solve.residual");
+        gemv_f64(n, n, 1.0, ctx->d_A, n, zj, 0.0, w, ctx->st);
+      }
+      if (int rc = after_launch(ctx, "gmres")) return rc;
+      double* Hj = H.data() + (size_t)j * (kGmres + 1);  // column j
+      {
+        Scope s(ctx, "solve.gmres");
+        std::vector<double> h(j + 1);
+        for (int pass = 0; pass < 2; ++pass) {
+          if (int rc = dots(ctx, V, j + 1, w, h.data())) return rc;
+          HIPC(hipMemcpyAsync(ctx->d_gh, h.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice,
+                              ctx->st));
+          hipLaunchKernelGGL(k_combo, vgrid(n), dim3(256), 0, ctx->st, w, V, n, j + 1, ctx->d_gh, -1.0, n);
+          for (int i = 0; i <= j; ++i) Hj[i] += h[i];
+        }
+        double hn = 0.0;
+        if (int rc = dots(ctx, w, 1, w, &hn)) return rc;
+        hn = std::sqrt(hn);
+        Hj[j + 1] = hn;
+        if (hn > 0)
+          hipLaunchKernelGGL(k_scale_by, vgrid(n), dim3(256), 0, ctx->st, V + (size_t)(j + 1) * n, w,
+                             1.0 / hn, n);
+      }
+      // rotate column j into upper-triangular form and update the residual norm estimate
+      for (int i = 0; i < j; ++i) {
+        const double t = cs[i] * Hj[i] + sn[i] * Hj[i + 1];
+        Hj[i + 1] = -sn[i] * Hj[i] + cs[i] * Hj[i + 1];
+        Hj[i] = t;
+      }
+      const double den = std::hypot(Hj[j], Hj[j + 1]);
+      cs[j] = den > 0 ? Hj[j] / den : 1.0;
+      sn[j] = den > 0 ? Hj[j + 1] / den : 0.0;
+      Hj[j] = den;
+      Hj[j + 1] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      k = j + 1;
+      ctx->iters += 1;
+      if (!(std::fabs(g[j + 1]) > 0.5 * tol * bnorm)) break;  // also stops on a breakdown (NaN)
+    }
+    // y = R^-1 g over the k columns; x += Z y
+    std::vector<double> y(k);
+    for (int i = k - 1; i >= 0; --i) {
+      double acc = g[i];
+      for (int c = i + 1; c < k; ++c) acc -= H[(size_t)c * (kGmres + 1) + i] * y[c];
+      const double d = H[(size_t)i * (kGmres + 1) + i];
+      y[i] = d != 0 ? acc / d : 0.0;
+    }
+    HIPC(hipMemcpyAsync(ctx->d_gh, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, ctx->st));
+    hipLaunchKernelGGL(k_combo, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, Z, n, k, ctx->d_gh, 1.0, n);
+    if (int rc = residual(ctx, &rel)) return rc;  // r = b - A x in d_res
+    ctx->rel_res = rel;
+    if (!(rel == rel)) return fail(ctx, IBEM3D_ESOLVE, "refinement diverged (NaN)");
+    beta = rel * bnorm;
+  }
+  ctx->solved = true;
+  if (rel > tol)
+    return fail(ctx, IBEM3D_ESOLVE, "refinement stopped at relative residual " + std::to_string(rel));
+  return 0;
+}
+
 int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
   CTX_CHECK();
+  const bool gmres_first = (mode & IBEM3D_SOLVE_GMRES) != 0;
+  mode &= ~IBEM3D_SOLVE_GMRES;
   if (ctx->phases != 0xFFu) return fail(ctx, IBEM3D_ESTATE, "solve: system not assembled");
   if (int rc = ensure_solver_ws(ctx)) return rc;
   const long n = ctx->n;
@@ -493,11 +665,13 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
       if (lu_prepare_solve_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->st))
         return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
     }
-    // iterative refinement in FP64: x <- x + C LU32^{-1} (R r)
+    if (gmres_first) return gmres_ir(ctx, tol, false, 1.0);
+    // classic refinement x <- x + C LU32^-1 (R r); GMRES-IR takes over from the current x when an
+    // iteration gains less than a factor of 10
     HIPC(hipMemsetAsync(ctx->d_x, 0, sizeof(double) * n, ctx->st));
     HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
     double rel = 1.0;
-    const int max_it = 50;
+    const int max_it = 30;
     for (int it = 0; it < max_it; ++it) {
       {
         Scope s(ctx, "solve.trsv_f32");
@@ -508,16 +682,25 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
                            ctx->d_c, n);
       }
       if (int rc = after_launch(ctx, "mixed refinement")) return rc;
-      if (int rc = residual(ctx, &rel)) return rc;
+      double rel_new = 0.0;
+      if (int rc = residual(ctx, &rel_new)) return rc;
       ctx->iters = it + 1;
-      ctx->rel_res = rel;
-      if (!(rel == rel)) return fail(ctx, IBEM3D_ESOLVE, "refinement diverged (NaN)");
-      if (rel <= tol) break;
+      ctx->rel_res = rel_new;
+      if (!(rel_new == rel_new)) return fail(ctx, IBEM3D_ESOLVE, "refinement diverged (NaN)");
+      if (rel_new <= tol) {
+        ctx->solved = true;
+        return 0;
+      }
+      if (rel_new > 0.1 * rel) {
+        if (rel_new > rel) {  // diverging: restart from zero
+          return gmres_ir(ctx, tol, false, 1.0);
+        }
+        return gmres_ir(ctx, tol, true, rel_new);
+      }
+      rel = rel_new;
     }
     ctx->solved = true;
-    if (rel > tol)
-      return fail(ctx, IBEM3D_ESOLVE, "refinement stopped at relative residual " + std::to_string(rel));
-    return 0;
+    return fail(ctx, IBEM3D_ESOLVE, "refinement stopped at relative residual " + std::to_string(rel));
   }
   return fail(ctx, IBEM3D_EINVAL, "solve: unknown mode");
 }

@@ -141,32 +141,36 @@ __global__ __launch_bounds__(192) void k_field(const double* __restrict__ nodes,
 }
 
 // ---- K4: boundary rows x boundary columns, b, rigid-body diagonal -------------------------------
-// Thread = (collocation node i, component ii) = one boundary row; loops over every element. Per
-// element it integrates the Kelvin U*, T* row against the four shape functions (8 x 8 Gauss, or
-// 2 x 8 x 8 on collapsed triangles when node i belongs to the element, skipping T for node i's own
-// shape function: BEMBuilderElastic3D.cpp:1169-1353) and folds the 12 + 12 values through the
-// column map: H columns of displacement-unknown dofs and -G columns of traction-unknown keys into A,
-// known values into b (ApplyBC :1960-2016). The diagonal block is minus the row sum of H over all
-// node columns (:1478-1501).
-__global__ __launch_bounds__(192) void k_bem(const double* __restrict__ qp,
-                                             const int* __restrict__ conn,
-                                             const double* __restrict__ nodes, int NN, int NE,
-                                             long n, double E0, double nu0,
-                                             const int* __restrict__ node_col,
-                                             const int* __restrict__ slot_col,
-                                             const double* __restrict__ uval,
-                                             const double* __restrict__ tval,
-                                             double* __restrict__ A, double* __restrict__ b) {
+// Row = (collocation node i, component ii). For every element the row's Kelvin U*, T* are integrated
+// against the four shape functions (8 x 8 Gauss, or 2 x 8 x 8 on collapsed triangles when node i
+// belongs to the element, skipping T for node i's own shape function: BEMBuilderElastic3D.cpp:
+// 1169-1353), and the 12 + 12 values are folded through the column map: H columns of
+// displacement-unknown dofs and -G columns of traction-unknown keys into A, known values into b
+// (ApplyBC :1960-2016). The diagonal block is minus the row sum of H over all node columns
+// (:1478-1501). Two kernels per round of kBemChunk elements, so that the quadrature is spread over
+// (node block, element block) workgroups rather than over the 3 NN rows alone:
+//   k_bem_quad  workgroup = 64 nodes x 3 components x kBemEPB elements: the 24 values of every
+//               (row, element) pair into a workspace, [element][value][row];
+//   k_bem_fold  thread = row: the values in element order through the column map (the order of a
+//               single pass over all elements, so results do not depend on the chunking).
+constexpr int kBemEPB = 4;  // elements per quadrature workgroup
+
+__global__ __launch_bounds__(192) void k_bem_quad(const double* __restrict__ qp,
+                                                  const int* __restrict__ conn,
+                                                  const double* __restrict__ nodes, int NN, int e0,
+                                                  int ne, double E0, double nu0,
+                                                  double* __restrict__ W) {
   __shared__ double sq[64 * kQpStride];
+  const long R = 3L * NN;
   const int i = blockIdx.x * 64 + threadIdx.x / 3;
   const int ii = threadIdx.x % 3;
   const bool active = i < NN;
   const int ic = active ? i : 0;
   const double xi0 = nodes[3 * ic], xi1 = nodes[3 * ic + 1], xi2 = nodes[3 * ic + 2];
   const long row = 3L * i + ii;
-  double rs[3] = {0.0, 0.0, 0.0};
-  double bacc = 0.0;
-  for (int e = 0; e < NE; ++e) {
+  const int el0 = blockIdx.y * kBemEPB, el1 = min(el0 + kBemEPB, ne);
+  for (int el = el0; el < el1; ++el) {
+    const int e = e0 + el;
     __syncthreads();
     for (int u = threadIdx.x; u < 64 * kQpStride; u += blockDim.x)
       sq[u] = qp[(size_t)e * 64 * kQpStride + u];
@@ -199,16 +203,16 @@ __global__ __launch_bounds__(192) void k_bem(const double* __restrict__ qp,
         }
       }
     } else {
-      double el[12];
+      double elx[12];
       for (int q = 0; q < 4; ++q)
-        for (int c = 0; c < 3; ++c) el[3 * q + c] = nodes[3 * (sc[q] - 1) + c];
+        for (int c = 0; c < 3; ++c) elx[3 * q + c] = nodes[3 * (sc[q] - 1) + c];
       for (int ntr = 0; ntr < 2; ++ntr)
         for (int ga = 0; ga < 8; ++ga)
           for (int gb = 0; gb < 8; ++gb) {
             double xs, es, jb;
             duffy(ntr, lpi, gauss8_x(ga), gauss8_x(gb), xs, es, jb);
             QPoint Q;
-            qpoint(el, xs, es, Q);
+            qpoint(elx, xs, es, Q);
             const double d0 = Q.g[0] - xi0, d1 = Q.g[1] - xi1, d2 = Q.g[2] - xi2;
             const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
             const double dxr[3] = {d0 / r, d1 / r, d2 / r};
@@ -224,11 +228,45 @@ __global__ __launch_bounds__(192) void k_bem(const double* __restrict__ qp,
             }
           }
     }
+    double* out = W + (size_t)el * 24 * R + row;
+    for (int q = 0; q < 4; ++q)
+      for (int c = 0; c < 3; ++c) {
+        out[(3 * q + c) * R] = aU[q][c];
+        out[(12 + 3 * q + c) * R] = aT[q][c];
+      }
+  }
+}
+
+// rs (3 R) and bacc (R) carry the row sums of H and the known-value terms across rounds
+__global__ __launch_bounds__(256) void k_bem_fold(const int* __restrict__ conn, int NN, int e0, int ne,
+                                                  long n, const int* __restrict__ node_col,
+                                                  const int* __restrict__ slot_col,
+                                                  const double* __restrict__ uval,
+                                                  const double* __restrict__ tval,
+                                                  const double* __restrict__ W,
+                                                  double* __restrict__ carry, int first, int last,
+                                                  double* __restrict__ A, double* __restrict__ b) {
+  const long R = 3L * NN;
+  const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= R) return;
+  double rs[3], bacc;
+  if (first) {
+    rs[0] = rs[1] = rs[2] = 0.0;
+    bacc = 0.0;
+  } else {
+    rs[0] = carry[row];
+    rs[1] = carry[R + row];
+    rs[2] = carry[2 * R + row];
+    bacc = carry[3 * R + row];
+  }
+  for (int el = 0; el < ne; ++el) {
+    const int e = e0 + el;
+    const double* in = W + (size_t)el * 24 * R + row;
     for (int q = 0; q < 4; ++q) {
-      const int nd = sc[q] - 1;
+      const int nd = conn[4 * e + q] - 1;
       for (int c = 0; c < 3; ++c) {
         const int dof = 3 * nd + c;
-        const double hv = aT[q][c];
+        const double hv = in[(12 + 3 * q + c) * R];
         rs[c] += hv;
         const int col = node_col[dof];
         if (col >= 0)
@@ -236,7 +274,7 @@ __global__ __launch_bounds__(192) void k_bem(const double* __restrict__ qp,
         else
           bacc -= hv * uval[dof];
         const int slot = 12 * e + 3 * q + c;
-        const double gv = aU[q][c];
+        const double gv = in[(3 * q + c) * R];
         const int kc = slot_col[slot];
         if (kc >= 0)
           A[(long)kc * n + row] -= gv;
@@ -245,7 +283,14 @@ __global__ __launch_bounds__(192) void k_bem(const double* __restrict__ qp,
       }
     }
   }
-  if (!active) return;
+  if (!last) {
+    carry[row] = rs[0];
+    carry[R + row] = rs[1];
+    carry[2 * R + row] = rs[2];
+    carry[3 * R + row] = bacc;
+    return;
+  }
+  const int i = (int)(row / 3);
   for (int c = 0; c < 3; ++c) {
     const int dof = 3 * i + c;
     const double dv = -rs[c];
@@ -432,12 +477,24 @@ void launch_field(const double* nodes, int NN, const double* cen, const double* 
   hipLaunchKernelGGL(k_field, grid, dim3(192), 0, st, nodes, NN, cen, rad, n, row0, M, A);
 }
 
+size_t bem_workspace_doubles(int NN, int NE) {
+  return ((size_t)std::min(NE, kBemChunk) * 24 + 4) * 3 * (size_t)NN;
+}
+
 void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, int NE, long n,
                 double E0, double nu0, const int* node_col, const int* slot_col, const double* uval,
-                const double* tval, double* A, double* b, hipStream_t st) {
-  dim3 grid((NN + 63) / 64);
-  hipLaunchKernelGGL(k_bem, grid, dim3(192), 0, st, qp, conn, nodes, NN, NE, n, E0, nu0, node_col,
-                     slot_col, uval, tval, A, b);
+                const double* tval, double* A, double* b, double* ws, hipStream_t st) {
+  const long R = 3L * NN;
+  double* W = ws;
+  double* carry = ws + (size_t)std::min(NE, kBemChunk) * 24 * R;
+  for (int e0 = 0; e0 < NE; e0 += kBemChunk) {
+    const int ne = std::min(kBemChunk, NE - e0);
+    hipLaunchKernelGGL(k_bem_quad, dim3((NN + 63) / 64, (ne + kBemEPB - 1) / kBemEPB), dim3(192), 0, st,
+                       qp, conn, nodes, NN, e0, ne, E0, nu0, W);
+    hipLaunchKernelGGL(k_bem_fold, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st, conn, NN, e0, ne,
+                       n, node_col, slot_col, uval, tval, W, carry, (int)(e0 == 0),
+                       (int)(e0 + ne >= NE), A, b);
+  }
 }
 
 void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long n,

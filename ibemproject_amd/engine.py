@@ -22,6 +22,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 SOLVE_FP64_LU = 0
 SOLVE_MIXED_FP32 = 1
 SOLVE_MIXED_FP16 = 2
+SOLVE_GMRES = 0x100  # flag for the mixed modes: GMRES-IR from the start
 
 
 class _CProblem(ctypes.Structure):

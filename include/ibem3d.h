@@ -64,8 +64,12 @@ enum {
 enum {
   IBEM3D_SOLVE_FP64_LU = 0,     /* FP64 blocked LU with row pivoting (FP64 MFMA update) */
   IBEM3D_SOLVE_MIXED_FP32 = 1,  /* FP32 LU (FP32 MFMA update) + FP64 iterative refinement */
-  IBEM3D_SOLVE_MIXED_FP16 = 2   /* FP32 LU whose trailing updates run on FP16 MFMA with scaled
-                                   hi/lo split operands (FP32-level accuracy) + FP64 refinement */
+  IBEM3D_SOLVE_MIXED_FP16 = 2,  /* FP32 LU whose trailing updates run on FP16 MFMA with scaled hi/lo split
+                                   operands (FP32-level accuracy) + FP64 refinement */
+  IBEM3D_SOLVE_GMRES = 0x100    /* flag for the mixed modes: refine by GMRES-IR (FP64 GMRES, right-
+                                   preconditioned by the FP32 factors) from the start. Without it the
+                                   refinement is classic and switches to GMRES-IR when an iteration
+                                   gains less than a factor of 10. */
 };
 
 /* Create a context on HIP device `device`: copies the problem to the device, builds the BC column
