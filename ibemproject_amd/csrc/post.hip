@@ -3,8 +3,8 @@
 //       + sum_e int_e ( Dsu u + Dst t ) dS          (C1 = 0 variant of the strain kernel)
 // One workgroup per point (6 waves). Boundary term: wave v takes elements v, v+6, ...; lane = Gauss
 // point; each lane keeps its own Kelvin tables (d r up to order 4, d r^-1 up to order 2) in its LDS
-// slice. Inclusion term: per sphere the eight potential tables are built in LDS, then thread
-// (ij, c) accumulates D(ij | c) X_c.
+// slice. Inclusion term: four spheres at a time, their eight potential tables built in LDS, then
+// thread (ij, c) accumulates D(ij | c) X_c.
 #include <hip/hip_runtime.h>
 
 #include "ibem3d_kernels.h"
@@ -14,6 +14,7 @@ namespace ibem {
 
 constexpr int kPostThreads = 384;
 constexpr int kPostTab = 84;  // |beta| <= 6 for the D_4x/5x/6x tables
+constexpr int kPostSph = kPostThreads / kPostTab;  // spheres per round of the inclusion term (4)
 
 __global__ __launch_bounds__(kPostThreads) void k_stress(
     const double* __restrict__ qp, const int* __restrict__ conn, int NE,
@@ -24,8 +25,8 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
   // tables and reduction buffers
   __shared__ double smem[kPostThreads * 45];
   double (*lS)[45] = reinterpret_cast<double (*)[45]>(smem);
-  double* tab = smem;                                                    // 8 * kPostTab
-  double (*red)[6] = reinterpret_cast<double (*)[6]>(smem + 8 * kPostTab);  // kPostThreads x 6
+  double* tab = smem;  // kPostSph x 8 x kPostTab
+  double (*red)[6] = reinterpret_cast<double (*)[6]>(smem + kPostSph * 8 * kPostTab);  // kPostThreads x 6
   const int p = blockIdx.x;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
   const double x0 = pts[3 * p], x1 = pts[3 * p + 1], x2 = pts[3 * p + 2];
@@ -61,16 +62,26 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
       }
   }
   // ---- inclusion term (postprocessor :367-522) ----
+  // kPostSph spheres per round: thread (slot, u) builds entry u of the slot's eight potential
+  // tables, then thread (ij, c) accumulates D(ij | c) X_c over the round's spheres in order
   const int ij = tid / 64, c = tid % 64;  // thread -> (strain pair, unknown)
+  const int slot = tid / kPostTab, u = tid % kPostTab;
   double ein = 0.0;
   int inside_h = -1;
   __syncthreads();  // the arena switches from per-lane tables to shared potential tables
-  for (int h = 0; h < num; ++h) {
-    const double xh[3] = {x0 - cen[3 * h], x1 - cen[3 * h + 1], x2 - cen[3 * h + 2]};
-    const double a = rad[h];
-    if (xh[0] * xh[0] + xh[1] * xh[1] + xh[2] * xh[2] - a * a <= 0) inside_h = h;
+  for (int h0 = 0; h0 < num; h0 += kPostSph) {
+    const int nh = min(kPostSph, num - h0);
+    for (int k = 0; k < nh; ++k) {
+      const int h = h0 + k;
+      const double xh[3] = {x0 - cen[3 * h], x1 - cen[3 * h + 1], x2 - cen[3 * h + 2]};
+      const double a = rad[h];
+      if (xh[0] * xh[0] + xh[1] * xh[1] + xh[2] * xh[2] - a * a <= 0) inside_h = h;
+    }
     __syncthreads();
-    for (int u = tid; u < kPostTab; u += kPostThreads) {
+    if (slot < nh) {
+      const int h = h0 + slot;
+      const double xh[3] = {x0 - cen[3 * h], x1 - cen[3 * h + 1], x2 - cen[3 * h + 2]};
+      const double a = rad[h];
       RadPoint P;
       rad_point(P, xh);
       const bool inside = is_inside(xh, a);
@@ -78,15 +89,18 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
       pot_coefs(a, inside, C);
       double v[8];
       pot_entry(P, inside, C, u, v);
-      for (int f = 0; f < 8; ++f) tab[f * kPostTab + u] = v[f];
+      for (int f = 0; f < 8; ++f) tab[(slot * 8 + f) * kPostTab + u] = v[f];
     }
     __syncthreads();
     if (c < 60) {
       // strain e_ij += D(i,j | m,n | w) map_mn (x2 for merged quadratic unknowns) X_c
       const Slot Cc = slot60(c);
-      double d = Dten(tab, kPostTab, M, Cc.w, pair_i(ij), pair_j(ij), Cc.i, Cc.j, 0, 0, 0);
       const double f = mapw(Cc.i, Cc.j) * ((Cc.order == 2 && Cc.p != Cc.q) ? 2.0 : 1.0);
-      ein += d * f * E60[60L * h + c];
+      for (int k = 0; k < nh; ++k) {
+        const double d = Dten(tab + k * 8 * kPostTab, kPostTab, M, Cc.w, pair_i(ij), pair_j(ij), Cc.i,
+                              Cc.j, 0, 0, 0);
+        ein += d * f * E60[60L * (h0 + k) + c];
+      }
     }
   }
   // reduce the inclusion strain over c for each ij (one wave per ij), then the boundary term
