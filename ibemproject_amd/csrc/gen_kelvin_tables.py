@@ -3,6 +3,7 @@
 The boundary->inclusion strain kernels need, at x = Gauss point - sphere centre,
   S(beta) = d^beta r      for 3 <= |beta| <= 6   (74 entries)
   T(beta) = d^beta r^-1   for 1 <= |beta| <= 4   (34 entries)
+and the stress kernel, at x = Gauss point - stress point, S for |beta| = 3, 4 and T for |beta| = 1, 2.
 For f(rho), rho = |x|^2, the Hermite-type expansion gives
   d^beta f(rho) = sum_{j <= beta/2} prod_c beta_c! / (j_c! (beta_c - 2 j_c)!) (2 x_c)^(beta_c - 2 j_c)
                   * f^(|beta| - |j|)(rho),
@@ -95,6 +96,21 @@ def main():
         for i in sorted(groups[g]):
             lines.append(coded[i][0])
         lines.append("}")
+    # the stress kernel's subset in one function: d^beta r for |beta| = 3, 4 at tidx(beta) and
+    # d^beta r^-1 for |beta| = 1, 2 at 35 + tidx(beta)
+    sub = [("S", b) for d in (3, 4) for b in multi(d)] + [("T", b) for d in (1, 2) for b in multi(d)]
+    lines += [
+        "// kelvin_stress(y0, y1, y2, PR, PT, tab): the same expansion for the stress kernel's entries,",
+        "// tab[tidx(beta)] = d^beta r (|beta| = 3, 4) and tab[35 + tidx(beta)] = d^beta r^-1 (|beta| = 1, 2).",
+        "IBEM_HD void kelvin_stress(const double* y0, const double* y1, const double* y2,",
+        "                           const double* PR, const double* PT, double* tab) {",
+    ]
+    for t, b in sub:
+        code = entry_code(t, b)[0]
+        if t == "T":
+            code = code.replace(f"tab[{tidx(*b) + 84}]", f"tab[{tidx(*b) + 35}]")
+        lines.append(code)
+    lines.append("}")
     with open(os.path.join(HERE, "kelvin_tables.inc"), "w") as f:
         f.write("\n".join(lines) + "\n")
     print("groups:", load)

@@ -702,6 +702,7 @@ IBEM_HD void strain_kernel_n(const double* tabS, const double* tabT, const Mat& 
     return tabS[tidx(c[0], c[1], c[2])];
   };
   double su = 0.0;
+#pragma unroll
   for (int m = 0; m < 3; ++m) {
     double t = 0.0;
     if (k == l) t += 2.0 * dlam * M.mu0 * kap * T(m, n);

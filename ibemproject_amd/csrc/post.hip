@@ -2,8 +2,8 @@
 //   s = C0 : ( sum_h D_4x/5x/6x . E_h  - eigenstrain polynomial if the point is inside a sphere )
 //       + sum_e int_e ( Dsu u + Dst t ) dS          (C1 = 0 variant of the strain kernel)
 // One workgroup per point (6 waves). Boundary term: wave v takes elements v, v+6, ...; lane = Gauss
-// point; each lane keeps its own Kelvin tables (d r up to order 4, d r^-1 up to order 2) in its LDS
-// slice. Inclusion term: four spheres at a time, their eight potential tables built in LDS, then
+// point; each lane builds the Kelvin tables it needs (d r of orders 3 and 4, d r^-1 of orders 1
+// and 2) in registers with generated straight-line code. Inclusion term: four spheres at a time, their eight potential tables built in LDS, then
 // thread (ij, c) accumulates D(ij | c) X_c.
 #include <hip/hip_runtime.h>
 
@@ -21,11 +21,9 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
     const double* __restrict__ cen, const double* __restrict__ rad, int num,
     const double* __restrict__ pts, Mat M, const double* __restrict__ U,
     const double* __restrict__ T, const double* __restrict__ E60, double* __restrict__ s) {
-  // one LDS arena (138 KB): per-lane Kelvin tables in the boundary phase, then the potential
-  // tables and reduction buffers
-  __shared__ double smem[kPostThreads * 45];
-  double (*lS)[45] = reinterpret_cast<double (*)[45]>(smem);
-  double* tab = smem;  // kPostSph x 8 x kPostTab
+  // LDS: the inclusion term's potential tables, then the reduction buffers after them
+  __shared__ double smem[kPostSph * 8 * kPostTab + kPostThreads * 6];
+  double* ptab = smem;  // kPostSph x 8 x kPostTab
   double (*red)[6] = reinterpret_cast<double (*)[6]>(smem + kPostSph * 8 * kPostTab);  // kPostThreads x 6
   const int p = blockIdx.x;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
@@ -35,17 +33,15 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
   for (int e = wave; e < NE; e += kPostThreads / 64) {
     const double* Q = qp + ((size_t)e * 64 + lane) * kQpStride;
     const double x[3] = {Q[0] - x0, Q[1] - x1, Q[2] - x2};
-    RadPoint P;
-    rad_point(P, x);
-    for (int t = 0; t < 35; ++t) {
-      double sv, tv;
-      kelvin_entry(P, t, sv, tv);
-      lS[tid][t] = sv;
-      if (t < 10) lS[tid][35 + t] = tv;
-    }
+    // the lane's Kelvin tables in registers (generated straight-line code, kelvin_stress)
+    double y0[7], y1[7], y2[7], PR[7], PT[5], tab[45];
+    kelvin_point(x, y0, y1, y2, PR, PT);
+    kelvin_stress(y0, y1, y2, PR, PT, tab);
     double u[3] = {0, 0, 0}, tr[3] = {0, 0, 0};
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int nd = conn[4 * e + q] - 1;
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
         u[k] += Q[6 + q] * U[3 * nd + k];
         tr[k] += Q[6 + q] * T[12 * e + 3 * q + k];
@@ -53,11 +49,13 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
     }
     const int e0[3] = {0, 0, 0};
     const double w = Q[10];
+    // every table index below is a compile-time constant once these loops unroll
+#pragma unroll
     for (int l = 0; l < 6; ++l)
+#pragma unroll
       for (int n = 0; n < 3; ++n) {
         double su, st;
-        strain_kernel_n(lS[tid], lS[tid] + 35, M, M.lam0, M.mu0, Q + 3, pair_i(l), pair_j(l), e0, 1, n,
-                        su, st);
+        strain_kernel_n(tab, tab + 35, M, M.lam0, M.mu0, Q + 3, pair_i(l), pair_j(l), e0, 1, n, su, st);
         acc[l] += w * (su * u[n] + st * tr[n]);
       }
   }
@@ -68,7 +66,6 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
   const int slot = tid / kPostTab, u = tid % kPostTab;
   double ein = 0.0;
   int inside_h = -1;
-  __syncthreads();  // the arena switches from per-lane tables to shared potential tables
   for (int h0 = 0; h0 < num; h0 += kPostSph) {
     const int nh = min(kPostSph, num - h0);
     for (int k = 0; k < nh; ++k) {
@@ -89,7 +86,7 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
       pot_coefs(a, inside, C);
       double v[8];
       pot_entry(P, inside, C, u, v);
-      for (int f = 0; f < 8; ++f) tab[(slot * 8 + f) * kPostTab + u] = v[f];
+      for (int f = 0; f < 8; ++f) ptab[(slot * 8 + f) * kPostTab + u] = v[f];
     }
     __syncthreads();
     if (c < 60) {
@@ -97,7 +94,7 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
       const Slot Cc = slot60(c);
       const double f = mapw(Cc.i, Cc.j) * ((Cc.order == 2 && Cc.p != Cc.q) ? 2.0 : 1.0);
       for (int k = 0; k < nh; ++k) {
-        const double d = Dten(tab + k * 8 * kPostTab, kPostTab, M, Cc.w, pair_i(ij), pair_j(ij), Cc.i,
+        const double d = Dten(ptab + k * 8 * kPostTab, kPostTab, M, Cc.w, pair_i(ij), pair_j(ij), Cc.i,
                               Cc.j, 0, 0, 0);
         ein += d * f * E60[60L * (h0 + k) + c];
       }
@@ -134,13 +131,13 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
     for (int l = 0; l < 6; ++l) e6[l] = estr[l];
     if (inside_h >= 0) {  // last containing sphere wins (:383-390)
       const double* E = E60 + 60L * inside_h;
-      const double d[3] = {x0 - cen[3 * inside_h], x1 - cen[3 * inside_h + 1], x2 - cen[3 * inside_h + 2]};
+      const double d0 = x0 - cen[3 * inside_h], d1 = x1 - cen[3 * inside_h + 1],
+                   d2 = x2 - cen[3 * inside_h + 2];
       for (int t = 0; t < 6; ++t) {
-        double v = E[t];
-        for (int k = 0; k < 3; ++k) v += E[6 + 3 * t + k] * d[k];
+        double v = E[t] + E[6 + 3 * t] * d0 + E[6 + 3 * t + 1] * d1 + E[6 + 3 * t + 2] * d2;
         for (int kl = 0; kl < 6; ++kl) {
           const int k = pair_i(kl), l = pair_j(kl);
-          v += mapw(k, l) * E[24 + 6 * t + kl] * d[k] * d[l];
+          v += mapw(k, l) * E[24 + 6 * t + kl] * sel3(d0, d1, d2, k) * sel3(d0, d1, d2, l);
         }
         e6[t] -= v;
       }

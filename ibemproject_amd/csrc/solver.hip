@@ -1265,22 +1265,27 @@ __global__ __launch_bounds__(kTB) void k_trinv(const T* __restrict__ LU, long ld
       __syncthreads();
       row = srow;
     }
+    // The sum runs over the whole range on one side of t, the same for every thread (X is zero
+    // outside its triangle: rows computed so far hold zeros there), with eight independent partial
+    // sums so that the LDS reads pipeline instead of forming one dependent chain.
+    const int ka = Upper ? t + 1 : 0, kz = Upper ? kb : t;
+    T p[8] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+    int k = ka;
+    for (; k + 8 <= kz; k += 8) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) p[q] += row[k + q] * X[k + q][j];
+    }
+    for (; k < kz; ++k) p[0] += row[k] * X[k][j];
+    const T acc = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
     if (j < kb) {
       T v = T(0);
       if (!Upper) {
-        if (j < t) {
-          T acc = T(0);
-          for (int k = j; k < t; ++k) acc += row[k] * X[k][j];
+        if (j < t)
           v = -acc;
-        } else if (j == t) {
+        else if (j == t)
           v = T(1);
-        }
       } else {
-        if (j >= t) {
-          T acc = j == t ? T(1) : T(0);
-          for (int k = t + 1; k <= j; ++k) acc -= row[k] * X[k][j];
-          v = sdiag != T(0) ? acc / sdiag : T(0);
-        }
+        if (j >= t) v = sdiag != T(0) ? ((j == t ? T(1) : T(0)) - acc) / sdiag : T(0);
       }
       X[t][j] = v;
     }
