@@ -350,10 +350,11 @@ int ibem3d_assemble(ibem3d_ctx* ctx) {
   int rc;
   if ((rc = ibem3d_add_bem(ctx))) return rc;
   if ((rc = ibem3d_add_field(ctx))) return rc;
-  for (int o = 0; o < 3; ++o)
-    if ((rc = ibem3d_add_incl(ctx, o))) return rc;
-  for (int o = 0; o < 3; ++o)
-    if ((rc = ibem3d_add_bem_rows(ctx, o))) return rc;
+  // all three eigenstrain orders in one pass each: every row belongs to one order, so the entries
+  // are the ones the per-order calls write, and the per-pair tables are built once instead of three
+  // times
+  if ((rc = ibem3d_add_incl(ctx, -1))) return rc;
+  if ((rc = ibem3d_add_bem_rows(ctx, -1))) return rc;
   return ibem3d_apply_bc(ctx);
 }
 

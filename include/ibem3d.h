@@ -87,7 +87,8 @@ int ibem3d_add_field(ibem3d_ctx* ctx);               /* boundary rows x eigenstr
 int ibem3d_add_incl(ibem3d_ctx* ctx, int order);     /* inclusion rows of order 0/1/2 (or -1: all) */
 int ibem3d_add_bem_rows(ibem3d_ctx* ctx, int order); /* inclusion rows x boundary columns, b rows */
 int ibem3d_apply_bc(ibem3d_ctx* ctx);                /* checks that every phase ran; system ready */
-/* All of the above in order. */
+/* All of the above in order, the inclusion and boundary->inclusion phases once each with order -1
+ * (the same entries as the three per-order calls, with the per-pair tables built once). */
 int ibem3d_assemble(ibem3d_ctx* ctx);
 /* Clear the assembled system so the phases can run again on the same inputs (no counterpart in
  * the reference, which re-reads its inputs for every run). */

@@ -80,6 +80,19 @@ def test_assembly_blocks(solved):
     assert worst < 1e-12
 
 
+def test_fused_assembly_matches_phases(solved):
+    """ibem3d_assemble (inclusion and boundary->inclusion phases with order -1) writes exactly the
+    system the reference's per-order phase sequence writes."""
+    prob, A, b = solved["prob"], solved["A"], solved["b"]
+    case(f"fused assembly, num={prob.num}")
+    eng = Engine(prob)
+    eng.assemble()
+    A2, b2 = eng.system()
+    eng.close()
+    assert np.array_equal(A2, A)
+    assert np.array_equal(b2, b)
+
+
 def test_rhs(solved):
     prob, ref, b = solved["prob"], solved["ref"], solved["b"]
     for (r0, r1) in blocks(prob):
