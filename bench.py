@@ -34,7 +34,7 @@ NB = 1024  # LU block size (solver.hip IBEM_LU_NB)
 # HBM bytes per k_gemm_f16x3 launch on rve1000_m16 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch,
 # averaged over one LU's launches; the factors make the counters read 1 GiB for a 1 GiB copy), from
 # the counter runs recorded in profiles/k_gemm_f16x3__rve1000_m16.md
-GEMM_TRAFFIC_BYTES_PER_LAUNCH = 1.4276e10
+GEMM_TRAFFIC_BYTES_PER_LAUNCH = 1.4354e10
 
 
 def trailing_update_flops(n: int) -> tuple[float, int]:
