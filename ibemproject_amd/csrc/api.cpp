@@ -713,6 +713,50 @@ int ibem3d_solve_info(const ibem3d_ctx* ctx, double* rel, int* iters) {
   return 0;
 }
 
+// Effective-property post-step: volume averages of stress and strain from the boundary solution
+// (divergence theorem; perfect bonding, no body force), on the quadrature table of k_qpoints.
+int ibem3d_average_fields(ibem3d_ctx* ctx, double* sigma, double* eps, double* volume) {
+  CTX_CHECK();
+  if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "average_fields: no solution");
+  const long n78 = 3L * ctx->NN + 78L * ctx->num;
+  std::vector<double> U(n78), T(12L * ctx->NE);
+  if (int rc = ibem3d_get_ut(ctx, U.data(), T.data(), nullptr)) return rc;
+  std::vector<double> qp((size_t)ctx->NE * 64 * kQpStride);
+  std::vector<int> conn(4L * ctx->NE);
+  HIPC(hipMemcpyAsync(qp.data(), ctx->d_qp, sizeof(double) * qp.size(), hipMemcpyDeviceToHost, ctx->st));
+  HIPC(hipMemcpyAsync(conn.data(), ctx->d_conn, sizeof(int) * conn.size(), hipMemcpyDeviceToHost, ctx->st));
+  HIPC(hipStreamSynchronize(ctx->st));
+  double S[3][3] = {}, E[3][3] = {}, V = 0.0;
+  for (int e = 0; e < ctx->NE; ++e) {
+    for (int g = 0; g < 64; ++g) {
+      const double* Q = qp.data() + ((size_t)e * 64 + g) * kQpStride;  // g[3], nv[3], N[4], w*jac
+      double t[3] = {0, 0, 0}, u[3] = {0, 0, 0};
+      for (int q = 0; q < 4; ++q) {
+        const int nd = conn[4 * e + q] - 1;
+        for (int i = 0; i < 3; ++i) {
+          t[i] += Q[6 + q] * T[12L * e + 3 * q + i];
+          u[i] += Q[6 + q] * U[3L * nd + i];
+        }
+      }
+      const double w = Q[10];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          S[i][j] += w * t[i] * Q[j];
+          E[i][j] += w * 0.5 * (u[i] * Q[3 + j] + u[j] * Q[3 + i]);
+        }
+      V += w * (Q[0] * Q[3] + Q[1] * Q[4] + Q[2] * Q[5]) / 3.0;
+    }
+  }
+  if (!(V > 0)) return fail(ctx, IBEM3D_EINVAL, "average_fields: non-positive enclosed volume");
+  static const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
+  for (int c = 0; c < 6; ++c) {
+    if (sigma) sigma[c] = 0.5 * (S[pi[c]][pj[c]] + S[pj[c]][pi[c]]) / V;
+    if (eps) eps[c] = E[pi[c]][pj[c]] / V;
+  }
+  if (volume) *volume = V;
+  return 0;
+}
+
 // getUTVector (BEMBuilderElastic3D.cpp:2053-2117) in the 78 layout
 int ibem3d_get_ut(ibem3d_ctx* ctx, double* U, double* T, double* X60) {
   CTX_CHECK();

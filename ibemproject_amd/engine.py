@@ -39,7 +39,7 @@ EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size"
            "ibem3d_assemble", "ibem3d_reset", "ibem3d_solve", "ibem3d_get_ut", "ibem3d_stress",
            "ibem3d_get_system", "ibem3d_solve_info", "ibem3d_timings", "ibem3d_reset_timings",
            "ibem3d_read_inputs", "ibem3d_inputs_problem", "ibem3d_inputs_title",
-           "ibem3d_free_inputs", "ibem3d_io_error", "ibem3d_write_stress"]
+           "ibem3d_free_inputs", "ibem3d_io_error", "ibem3d_write_stress", "ibem3d_average_fields"]
 
 
 class EngineError(RuntimeError):
@@ -84,6 +84,7 @@ def load(path: str = LIB_PATH):
     L.ibem3d_io_error.restype = ctypes.c_char_p
     L.ibem3d_io_error.argtypes = []
     L.ibem3d_write_stress.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int]
+    L.ibem3d_average_fields.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     _LIB = L
     return L
 
@@ -201,6 +202,16 @@ class Engine:
         s = np.zeros((self.prob.nump, 6))
         self._chk(self.L.ibem3d_stress(self.ctx, _d(s)), "getFlux")
         return s
+
+    def average_fields(self):
+        """Volume averages (stress, strain: 6 values each, 00 01 02 11 12 22; enclosed volume) from
+        the boundary solution -- the effective-property post-step (no reference counterpart)."""
+        sig = np.zeros(6)
+        eps = np.zeros(6)
+        vol = ctypes.c_double()
+        self._chk(self.L.ibem3d_average_fields(self.ctx, _d(sig), _d(eps), ctypes.byref(vol)),
+                  "average_fields")
+        return sig, eps, vol.value
 
     def system(self):
         A = np.zeros((self.n, self.n), order="F")

@@ -105,6 +105,13 @@ int ibem3d_get_ut(ibem3d_ctx* ctx, double* U, double* T, double* X60);
 /* Stresses at the problem's points: nump x 6 row-major (s00 s01 s02 s11 s12 s22). */
 int ibem3d_stress(ibem3d_ctx* ctx, double* s);
 
+/* Volume averages over the RVE from the boundary solution (no counterpart in the reference: the
+ * effective-property post-step). With U, T as ibem3d_get_ut returns them, on every element's 8 x 8
+ * Gauss rule: stress <s_ij> = (1/V) int_S t_i x_j dS (symmetrised), strain
+ * <e_ij> = (1/V) int_S (u_i n_j + u_j n_i) / 2 dS, V = (1/3) int_S x . n dS. sigma, eps: 6 values
+ * each in the order 00 01 02 11 12 22; any pointer may be NULL. Host arithmetic in a fixed order. */
+int ibem3d_average_fields(ibem3d_ctx* ctx, double* sigma, double* eps, double* volume);
+
 /* Copy the assembled system (A column-major n x n, b n) to the host; either may be NULL. */
 int ibem3d_get_system(ibem3d_ctx* ctx, double* A, double* b);
 

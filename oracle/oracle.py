@@ -177,6 +177,47 @@ class OracleProblem:
         return s
 
 
+def average_fields(prob, U, T):
+    """Volume averages from a boundary solution, restated from the divergence theorem (no reference
+    counterpart): <s_ij> = (1/V) int t_i x_j dS (symmetrised), <e_ij> = (1/V) int (u_i n_j + u_j n_i)/2
+    dS, V = (1/3) int x.n dS, on each element's 8 x 8 Gauss-Legendre rule with the bilinear quad map
+    of BEMElasticHelper3D.cpp:191-274 (Serendip_func, Normal_Jac, Cartesian). Returns (sigma6,
+    eps6, V) in the order 00 01 02 11 12 22."""
+    xg, wg = np.polynomial.legendre.leggauss(8)
+    S = np.zeros((3, 3))
+    E = np.zeros((3, 3))
+    V = 0.0
+    nodes = np.asarray(prob.nodes, dtype=np.float64).reshape(-1, 3)
+    conn = np.asarray(prob.conn, dtype=np.int64).reshape(-1, 4) - 1
+    U = np.asarray(U)
+    T = np.asarray(T).reshape(-1, 4, 3)
+    for e in range(conn.shape[0]):
+        X = nodes[conn[e]]          # 4 x 3
+        ue = U[: 3 * prob.NN].reshape(-1, 3)[conn[e]]
+        te = T[e]
+        for a, xs in enumerate(xg):
+            for b, et in enumerate(xg):
+                N = 0.25 * np.array([(1 - xs) * (1 - et), (1 + xs) * (1 - et), (1 + xs) * (1 + et),
+                                     (1 - xs) * (1 + et)])
+                dxs = 0.25 * np.array([-(1 - et), (1 - et), (1 + et), -(1 + et)])
+                det = 0.25 * np.array([-(1 - xs), -(1 + xs), (1 + xs), (1 - xs)])
+                vx, ve = dxs @ X, det @ X
+                nv = np.cross(vx, ve)
+                jac = np.linalg.norm(nv)
+                nv = nv / jac
+                x = N @ X
+                w = wg[a] * wg[b] * jac
+                t = N @ te
+                u = N @ ue
+                S += w * np.outer(t, x)
+                E += w * 0.5 * (np.outer(u, nv) + np.outer(nv, u))
+                V += w * (x @ nv) / 3.0
+    S = 0.5 * (S + S.T) / V
+    E = E / V
+    idx = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    return np.array([S[i, j] for i, j in idx]), np.array([E[i, j] for i, j in idx]), V
+
+
 def fullpiv_solve(A, b):
     n = A.shape[0]
     M = np.array(A, dtype=np.float64, order="F", copy=True)

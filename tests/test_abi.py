@@ -28,3 +28,12 @@ def test_invalid_problem_is_rejected_without_gpu():
     lib = engine.load()
     assert not lib.ibem3d_create(None, 0)
     assert b"invalid" in lib.ibem3d_last_error(None)
+
+
+def test_binding_declares_every_signature():
+    """Every function of the binding has its argument types set: ctypes would otherwise pass a
+    context pointer as a 32-bit int."""
+    lib = engine.load()
+    for name in engine.EXPORTS:
+        f = getattr(lib, name)
+        assert f.argtypes is not None, name

@@ -63,6 +63,16 @@ def test_workload_properties():
     assert rel16 <= 1e-10
     _, _, x16 = eng.getUTVector()
     assert np.linalg.norm(x16 - x32) / np.linalg.norm(x32) < 1e-8
+    # effective-property post-step at full size: the loading fixes <s_xx> = 1, <s_yy> = <s_zz> = 0;
+    # the apparent modulus lies between the Reuss and Voigt bounds of the volume fraction
+    sig, eps, V = eng.average_fields()
+    phi = float(np.sum(4.0 / 3.0 * np.pi * prob.radii ** 3)) / V
+    E = sig[0] / eps[0]
+    reuss = 1.0 / (phi / prob.E1 + (1 - phi) / prob.E0)
+    voigt = phi * prob.E1 + (1 - phi) * prob.E0
+    print(f"phi {phi:.4f} E_apparent {E:.5f} Reuss {reuss:.5f} Voigt {voigt:.5f}")
+    assert abs(sig[0] - 1.0) < 1e-10 and abs(sig[3]) < 1e-10 and abs(sig[5]) < 1e-10
+    assert prob.E0 < E and reuss <= E <= voigt
     relg, itg = eng.solve(SOLVE_MIXED_FP16 | SOLVE_GMRES, 1e-10)
     print("GMRES-IR preconditioner applications", itg)
     assert relg <= 1e-10

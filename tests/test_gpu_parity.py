@@ -93,6 +93,21 @@ def test_fused_assembly_matches_phases(solved):
     assert np.array_equal(b2, b)
 
 
+def test_average_fields(solved, oracle_lib):
+    """Effective-property post-step: the engine's volume averages against the oracle's restatement
+    on the same boundary solution."""
+    prob, eng = solved["prob"], solved["eng"]
+    case(f"average fields, num={prob.num}")
+    eng.solve(SOLVE_FP64_LU)
+    U, T, _ = eng.getUTVector()
+    sig, eps, V = eng.average_fields()
+    rs, re_, rV = oracle_lib.average_fields(prob, U, T)
+    assert abs(V - rV) < 1e-12
+    assert np.abs(sig - rs).max() < 1e-12
+    assert np.abs(eps - re_).max() < 1e-12 * np.abs(re_).max()
+    assert abs(sig[0] - 1.0) < 1e-10 and abs(sig[3]) < 1e-10 and abs(sig[5]) < 1e-10
+
+
 def test_rhs(solved):
     prob, ref, b = solved["prob"], solved["ref"], solved["b"]
     for (r0, r1) in blocks(prob):
