@@ -559,8 +559,12 @@ void gemm_f32_sub_f16x3(long M, long N, int K, const float* A, long lda, const f
 // column-by-column global pivot search; it is backward stable in practice like partial pivoting.
 // Pivot ties follow a fixed rule over a fixed tree shape, so results are deterministic.
 constexpr int kPanelW = 32;   // leaf width
-constexpr int kTsRows = 256;  // rows per workgroup (one per thread) at every tree level
-constexpr int kTsFan = 8;     // candidate sets merged per tree node (8 x 32 = 256 rows)
+#ifndef IBEM_TS_ROWS
+#define IBEM_TS_ROWS 256
+#endif
+constexpr int kTsRows = IBEM_TS_ROWS;       // rows per workgroup (one per thread) at every tree level
+constexpr int kTsWaves = kTsRows / 64;      // wavefronts per tree node
+constexpr int kTsFan = kTsRows / kPanelW;   // candidate sets merged per tree node
 
 static long ts_nodes(long n) { return 2 * ((n + kTsRows - 1) / kTsRows) + 16; }
 size_t lu_workspace_doubles(long n) {
@@ -635,7 +639,7 @@ __device__ inline unsigned wave_max_u32(unsigned k) {
 // a: the row's values, updated in place (multipliers left of the pivot column, reduced values right
 // of it); mystep: the step at which this row was chosen, or -1. Returns the number of steps that
 // found a row. Per step: a wave maximum of the pivot keys, whose lowest winning lane publishes its
-// row in the wave's LDS slot; one barrier; every thread takes the best of the four waves and
+// row in the wave's LDS slot; one barrier; every thread takes the best of the node's waves and
 // eliminates with that slot's row. Buffers alternate by step parity, so one barrier per step
 // suffices. The update is branch-free: rows that do not take part use a zero multiplier (all values
 // are finite), which leaves them unchanged.
@@ -654,17 +658,17 @@ __device__ __forceinline__ int ts_select(T (&a)[kPanelW], bool valid, int w, uns
       const unsigned wmax = wave_max_u32(key);
       const unsigned long long hit = __ballot(key == wmax);
       const int wl = __ffsll((long long)hit) - 1;
-      if (lane == wl) ts_store_row<T>(sp + (par * 4 + wv) * kPanelW, a);
+      if (lane == wl) ts_store_row<T>(sp + (par * kTsWaves + wv) * kPanelW, a);
       if (lane == 0) {
-        skey[par * 4 + wv] = wmax;
-        swin[par * 4 + wv] = wv * 64 + wl;
+        skey[par * kTsWaves + wv] = wmax;
+        swin[par * kTsWaves + wv] = wv * 64 + wl;
       }
       __syncthreads();
-      unsigned best = skey[par * 4];
+      unsigned best = skey[par * kTsWaves];
       int bw = 0;
 #pragma unroll
       for (int q = 1; q < kTsRows / 64; ++q) {
-        const unsigned k2 = skey[par * 4 + q];
+        const unsigned k2 = skey[par * kTsWaves + q];
         if (k2 > best) {
           best = k2;
           bw = q;
@@ -674,8 +678,8 @@ __device__ __forceinline__ int ts_select(T (&a)[kPanelW], bool valid, int w, uns
       // then holds some row's finite values)
       const bool found = best != 0u;
       nsel = found ? c + 1 : nsel;
-      if (found && (int)threadIdx.x == swin[par * 4 + bw]) mystep = c;
-      const T* prow = sp + (par * 4 + bw) * kPanelW;
+      if (found && (int)threadIdx.x == swin[par * kTsWaves + bw]) mystep = c;
+      const T* prow = sp + (par * kTsWaves + bw) * kPanelW;
       T pr[kPanelW];
 #pragma unroll
       for (int k = c; k < kPanelW; ++k) pr[k] = prow[k];
@@ -711,9 +715,9 @@ template <typename T>
 __global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, long lda, long n,
                                                      long j0, int w, int P, TsWs ws,
                                                      int* __restrict__ ipiv, int* __restrict__ info) {
-  __shared__ unsigned skey[8];
-  __shared__ int swin[8];
-  __shared__ __align__(16) T sp[2 * 4 * kPanelW];
+  __shared__ unsigned skey[2 * kTsWaves];
+  __shared__ int swin[2 * kTsWaves];
+  __shared__ __align__(16) T sp[2 * kTsWaves * kPanelW];
   __shared__ int s_last;
   __shared__ int s_win[kPanelW];
   __shared__ double s_diag[kPanelW];
