@@ -52,6 +52,7 @@ struct ibem3d_ctx {
   // dot-product partial sums, small coefficient vector
   double *d_gV = nullptr, *d_gZ = nullptr, *d_gpart = nullptr, *d_gh = nullptr;
   double* d_bemws = nullptr;  // boundary-boundary quadrature values of one round of elements
+  int prepared_mode = -1;      // mode whose factors and solve data are in place (-1: none)
   LuWork W;
   unsigned phases = 0;
   bool solved = false;
@@ -597,33 +598,40 @@ static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0) {
 int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
   CTX_CHECK();
   const bool gmres_first = (mode & IBEM3D_SOLVE_GMRES) != 0;
-  mode &= ~IBEM3D_SOLVE_GMRES;
+  const bool reuse = (mode & IBEM3D_SOLVE_REUSE) != 0;
+  mode &= ~(IBEM3D_SOLVE_GMRES | IBEM3D_SOLVE_REUSE);
+  if (reuse && ctx->prepared_mode != mode)
+    return fail(ctx, IBEM3D_ESTATE, "solve: no factors of this mode to reuse");
   if (ctx->phases != 0xFFu) return fail(ctx, IBEM3D_ESTATE, "solve: system not assembled");
   if (int rc = ensure_solver_ws(ctx)) return rc;
   const long n = ctx->n;
   ctx->iters = 0;
   ctx->rel_res = -1.0;
   if (mode == IBEM3D_SOLVE_FP64_LU) {
-    // keep A for the residual; factor an equilibrated copy R A C
-    if (!ctx->d_Af)
-      if (int rc = dalloc(ctx, &ctx->d_Af, (size_t)n * n)) return rc;
-    {
-      Scope s(ctx, "solve.equilibrate");
-      equilibrate_copy_f64(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_Af, n, ctx->d_eqws, ctx->st);
-    }
-    {
-      Scope s(ctx, "solve.lu_f64");
-      if (lu_factor_f64(ctx->d_Af, n, n, ctx->W, ctx->st, tick, ctx))
-        return fail(ctx, IBEM3D_EHIP, "LU launch failed");
-    }
-    int info = 0;
-    HIPC(hipMemcpyAsync(&info, ctx->W.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
-    HIPC(hipStreamSynchronize(ctx->st));
-    if (info) return fail(ctx, IBEM3D_ESOLVE, "LU: zero pivot at column " + std::to_string(info - 1));
-    {
-      Scope s(ctx, "solve.prepare");
-      if (lu_prepare_solve_f64(ctx->d_Af, n, n, ctx->W, ctx->st))
-        return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
+    if (!reuse) {
+      // keep A for the residual; factor an equilibrated copy R A C
+      ctx->prepared_mode = -1;
+      if (!ctx->d_Af)
+        if (int rc = dalloc(ctx, &ctx->d_Af, (size_t)n * n)) return rc;
+      {
+        Scope s(ctx, "solve.equilibrate");
+        equilibrate_copy_f64(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_Af, n, ctx->d_eqws, ctx->st);
+      }
+      {
+        Scope s(ctx, "solve.lu_f64");
+        if (lu_factor_f64(ctx->d_Af, n, n, ctx->W, ctx->st, tick, ctx))
+          return fail(ctx, IBEM3D_EHIP, "LU launch failed");
+      }
+      int info = 0;
+      HIPC(hipMemcpyAsync(&info, ctx->W.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+      HIPC(hipStreamSynchronize(ctx->st));
+      if (info) return fail(ctx, IBEM3D_ESOLVE, "LU: zero pivot at column " + std::to_string(info - 1));
+      {
+        Scope s(ctx, "solve.prepare");
+        if (lu_prepare_solve_f64(ctx->d_Af, n, n, ctx->W, ctx->st))
+          return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
+      }
+      ctx->prepared_mode = mode;
     }
     {
       Scope s(ctx, "solve.trsv_f64");
@@ -643,28 +651,33 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
       if (int rc = dalloc(ctx, &ctx->d_A32, (size_t)ctx->lda32 * n)) return rc;
       if (int rc = dalloc(ctx, &ctx->d_v32, (size_t)n)) return rc;
     }
-    // R A C straight from A into the FP32 factor storage (A itself stays for the residual)
-    {
-      Scope s(ctx, "solve.equilibrate");
-      equilibrate_copy_f32(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_A32, ctx->lda32, ctx->d_eqws,
-                           ctx->st);
-    }
-    {
-      Scope s(ctx, "solve.lu_f32");
-      if (mode == IBEM3D_SOLVE_MIXED_FP16 && !ctx->d_f16ws)
-        if (int rc = dalloc(ctx, &ctx->d_f16ws, f16x3_workspace_doubles(n))) return rc;
-      if (lu_factor_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->st, tick, ctx,
-                        mode == IBEM3D_SOLVE_MIXED_FP16 ? ctx->d_f16ws : nullptr))
-        return fail(ctx, IBEM3D_EHIP, "LU launch failed");
-    }
-    int info = 0;
-    HIPC(hipMemcpyAsync(&info, ctx->W.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
-    HIPC(hipStreamSynchronize(ctx->st));
-    if (info) return fail(ctx, IBEM3D_ESOLVE, "LU(fp32): zero pivot at column " + std::to_string(info - 1));
-    {
-      Scope s(ctx, "solve.prepare");
-      if (lu_prepare_solve_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->st))
-        return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
+    if (!reuse) {
+      ctx->prepared_mode = -1;
+      // R A C straight from A into the FP32 factor storage (A itself stays for the residual)
+      {
+        Scope s(ctx, "solve.equilibrate");
+        equilibrate_copy_f32(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_A32, ctx->lda32, ctx->d_eqws,
+                             ctx->st);
+      }
+      {
+        Scope s(ctx, "solve.lu_f32");
+        if (mode == IBEM3D_SOLVE_MIXED_FP16 && !ctx->d_f16ws)
+          if (int rc = dalloc(ctx, &ctx->d_f16ws, f16x3_workspace_doubles(n))) return rc;
+        if (lu_factor_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->st, tick, ctx,
+                          mode == IBEM3D_SOLVE_MIXED_FP16 ? ctx->d_f16ws : nullptr))
+          return fail(ctx, IBEM3D_EHIP, "LU launch failed");
+      }
+      int info = 0;
+      HIPC(hipMemcpyAsync(&info, ctx->W.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+      HIPC(hipStreamSynchronize(ctx->st));
+      if (info)
+        return fail(ctx, IBEM3D_ESOLVE, "LU(fp32): zero pivot at column " + std::to_string(info - 1));
+      {
+        Scope s(ctx, "solve.prepare");
+        if (lu_prepare_solve_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->st))
+          return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
+      }
+      ctx->prepared_mode = mode;
     }
     if (gmres_first) return gmres_ir(ctx, tol, false, 1.0);
     // classic refinement x <- x + C LU32^-1 (R r); GMRES-IR takes over from the current x when an
@@ -710,6 +723,73 @@ int ibem3d_solve_info(const ibem3d_ctx* ctx, double* rel, int* iters) {
   if (!ctx) return IBEM3D_EINVAL;
   if (rel) *rel = ctx->rel_res;
   if (iters) *iters = ctx->iters;
+  return 0;
+}
+
+// New known boundary values with the same known/unknown pattern: the folded system matrix does not
+// depend on the values, only b does, so the factors stay valid.
+int ibem3d_set_known_values(ibem3d_ctx* ctx, const double* bc_values, const double* tbc_values) {
+  CTX_CHECK();
+  if (!bc_values || !tbc_values) return fail(ctx, IBEM3D_EINVAL, "set_known_values: null values");
+  const int NN = ctx->NN, NE = ctx->NE;
+  std::vector<double> uval(3L * NN), tval(12L * NE);
+  for (long j = 0; j < 3L * NN; ++j) {
+    ctx->h_bc[2 * j + 1] = bc_values[j];
+    uval[j] = bc_values[j];
+  }
+  for (long e = 0; e < NE; ++e)
+    for (int i = 0; i < 12; ++i) {
+      ctx->h_tbc[24 * e + 12 + i] = tbc_values[12 * e + i];
+      tval[12 * e + i] = tbc_values[12 * e + i];
+    }
+  HIPC(hipMemcpyAsync(ctx->d_uval, uval.data(), sizeof(double) * uval.size(), hipMemcpyHostToDevice,
+                      ctx->st));
+  HIPC(hipMemcpyAsync(ctx->d_tval, tval.data(), sizeof(double) * tval.size(), hipMemcpyHostToDevice,
+                      ctx->st));
+  HIPC(hipStreamSynchronize(ctx->st));
+  return ibem3d_reset(ctx);
+}
+
+// enclosed volume (1/3) int x . n dS on the quadrature table (the same sum as ibem3d_average_fields)
+static int enclosed_volume(ibem3d_ctx* ctx, double* V) {
+  std::vector<double> qp((size_t)ctx->NE * 64 * kQpStride);
+  HIPC(hipMemcpyAsync(qp.data(), ctx->d_qp, sizeof(double) * qp.size(), hipMemcpyDeviceToHost, ctx->st));
+  HIPC(hipStreamSynchronize(ctx->st));
+  double v = 0.0;
+  for (size_t g = 0; g < (size_t)ctx->NE * 64; ++g) {
+    const double* Q = qp.data() + g * kQpStride;
+    v += Q[10] * (Q[0] * Q[3] + Q[1] * Q[4] + Q[2] * Q[5]) / 3.0;
+  }
+  *V = v;
+  return 0;
+}
+
+int ibem3d_average_eigenstrain(ibem3d_ctx* ctx, double* eps_star) {
+  CTX_CHECK();
+  if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "average_eigenstrain: no solution");
+  if (!eps_star) return fail(ctx, IBEM3D_EINVAL, "average_eigenstrain: null output");
+  const long n78 = 3L * ctx->NN + 78L * ctx->num;
+  std::vector<double> U(n78);
+  if (int rc = ibem3d_get_ut(ctx, U.data(), nullptr, nullptr)) return rc;
+  std::vector<double> rad(ctx->num);
+  if (ctx->num > 0) {
+    HIPC(hipMemcpyAsync(rad.data(), ctx->d_rad, sizeof(double) * rad.size(), hipMemcpyDeviceToHost,
+                        ctx->st));
+    HIPC(hipStreamSynchronize(ctx->st));
+  }
+  double V = 0.0;
+  if (int rc = enclosed_volume(ctx, &V)) return rc;
+  if (!(V > 0)) return fail(ctx, IBEM3D_EINVAL, "average_eigenstrain: non-positive enclosed volume");
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int h = 0; h < ctx->num; ++h) {
+    const double a = rad[h], Vs = 4.0 / 3.0 * kPi * a * a * a;
+    const double* E = U.data() + 3L * ctx->NN + 78L * h;  // E_t, E_t,k, E_t,kl (78 layout)
+    for (int t = 0; t < 6; ++t) {
+      const double tr = E[24 + 9 * t + 0] + E[24 + 9 * t + 4] + E[24 + 9 * t + 8];
+      acc[t] += Vs * (E[t] + a * a / 5.0 * tr);
+    }
+  }
+  for (int t = 0; t < 6; ++t) eps_star[t] = acc[t] / V;
   return 0;
 }
 

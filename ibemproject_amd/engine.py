@@ -23,6 +23,7 @@ SOLVE_FP64_LU = 0
 SOLVE_MIXED_FP32 = 1
 SOLVE_MIXED_FP16 = 2
 SOLVE_GMRES = 0x100  # flag for the mixed modes: GMRES-IR from the start
+SOLVE_REUSE = 0x200  # flag: reuse the factors of the last factorisation in the same mode
 
 
 class _CProblem(ctypes.Structure):
@@ -39,7 +40,8 @@ EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size"
            "ibem3d_assemble", "ibem3d_reset", "ibem3d_solve", "ibem3d_get_ut", "ibem3d_stress",
            "ibem3d_get_system", "ibem3d_solve_info", "ibem3d_timings", "ibem3d_reset_timings",
            "ibem3d_read_inputs", "ibem3d_inputs_problem", "ibem3d_inputs_title",
-           "ibem3d_free_inputs", "ibem3d_io_error", "ibem3d_write_stress", "ibem3d_average_fields"]
+           "ibem3d_free_inputs", "ibem3d_io_error", "ibem3d_write_stress", "ibem3d_average_fields",
+           "ibem3d_set_known_values", "ibem3d_average_eigenstrain"]
 
 
 class EngineError(RuntimeError):
@@ -85,6 +87,8 @@ def load(path: str = LIB_PATH):
     L.ibem3d_io_error.argtypes = []
     L.ibem3d_write_stress.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int, ctypes.c_int]
     L.ibem3d_average_fields.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+    L.ibem3d_set_known_values.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.ibem3d_average_eigenstrain.argtypes = [ctypes.c_void_p, _dp]
     _LIB = L
     return L
 
@@ -213,6 +217,22 @@ class Engine:
                   "average_fields")
         return sig, eps, vol.value
 
+    def set_known_values(self, bc_values, tbc_values):
+        """New prescribed values (3 NN displacements, 12 NE tractions) with the known/unknown pattern
+        of the problem the engine was created with; clears the assembled system (assemble again,
+        then solve with SOLVE_REUSE to keep the factors)."""
+        bv = np.ascontiguousarray(bc_values, dtype=np.float64).reshape(-1)
+        tv = np.ascontiguousarray(tbc_values, dtype=np.float64).reshape(-1)
+        if bv.size != 3 * self.prob.NN or tv.size != 12 * self.prob.NE:
+            raise ValueError("set_known_values: expected 3 NN and 12 NE values")
+        self._chk(self.L.ibem3d_set_known_values(self.ctx, _d(bv), _d(tv)), "set_known_values")
+
+    def average_eigenstrain(self):
+        """Volume average of the equivalent eigenstrain (6 values, 00 01 02 11 12 22)."""
+        e = np.zeros(6)
+        self._chk(self.L.ibem3d_average_eigenstrain(self.ctx, _d(e)), "average_eigenstrain")
+        return e
+
     def system(self):
         A = np.zeros((self.n, self.n), order="F")
         b = np.zeros(self.n)
@@ -250,3 +270,58 @@ def run_reference_phases(eng: Engine):
     for o in range(3):
         eng.addFluxEquivalentBEM(o)
     eng.ApplyBC()
+
+
+PAIRS = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+
+
+def unit_strain(t: int) -> np.ndarray:
+    """The symmetric unit tensor of pair t (00 01 02 11 12 22): e_ij = e_ji = 1."""
+    i, j = PAIRS[t]
+    e = np.zeros((3, 3))
+    e[i, j] = e[j, i] = 1.0
+    return e
+
+
+def isotropic_stress(E: float, nu: float, eps6) -> np.ndarray:
+    """s_ij = lam d_ij tr(e) + 2 mu e_ij for tensor components in pair order."""
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    eps6 = np.asarray(eps6, dtype=np.float64)
+    tr = eps6[0] + eps6[3] + eps6[5]
+    return np.array([lam * tr * (i == j) + 2 * mu * eps6[t] for t, (i, j) in enumerate(PAIRS)])
+
+
+def effective_stiffness(prob: Problem, mode: int = SOLVE_MIXED_FP16, tol: float = 1e-10,
+                        device: int = 0):
+    """Apparent stiffness of an RVE from six affine displacement loads u = e_t . x on every face
+    (e_t the unit tensor of pair t), one factorisation reused for the other five loads (no reference
+    counterpart). Returns (K_inc, K_bnd, info): 6 x 6 matrices of tensor components (column t = the
+    average stress for e_t, pair order 00 01 02 11 12 22), K_inc from the inclusions,
+    <s> = C0 : (<e> - <e*>), K_bnd from the boundary tractions, <s> = (1/V) int t x dS; info holds
+    the relative residual and refinement iterations of each load."""
+    from .generate import macro_strain_bc
+    nodes = np.asarray(prob.nodes, dtype=np.float64).reshape(-1, 3)
+    conn = np.asarray(prob.conn).reshape(-1, 4)
+    bc0, tbc0 = macro_strain_bc(nodes, conn, unit_strain(0))
+    p0 = Problem(prob.title, prob.E0, prob.nu0, prob.E1, prob.nu1, nodes, conn, bc0, tbc0,
+                 prob.centres, prob.radii, np.zeros((0, 3)))
+    eng = Engine(p0, device)
+    K_inc = np.zeros((6, 6))
+    K_bnd = np.zeros((6, 6))
+    info = []
+    try:
+        for t in range(6):
+            if t > 0:
+                bc, tbc = macro_strain_bc(nodes, conn, unit_strain(t))
+                eng.set_known_values(bc[:, 1], tbc[:, 12:])
+            eng.assemble()
+            rel, it = eng.solve(mode | (SOLVE_REUSE if t > 0 else 0), tol)
+            info.append((rel, it))
+            sig_b, eps_b, _ = eng.average_fields()
+            eps_star = eng.average_eigenstrain()
+            K_inc[:, t] = isotropic_stress(prob.E0, prob.nu0, eps_b - eps_star)
+            K_bnd[:, t] = sig_b
+    finally:
+        eng.close()
+    return K_inc, K_bnd, info

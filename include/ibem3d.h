@@ -66,10 +66,14 @@ enum {
   IBEM3D_SOLVE_MIXED_FP32 = 1,  /* FP32 LU (FP32 MFMA update) + FP64 iterative refinement */
   IBEM3D_SOLVE_MIXED_FP16 = 2,  /* FP32 LU whose trailing updates run on FP16 MFMA with scaled hi/lo split
                                    operands (FP32-level accuracy) + FP64 refinement */
-  IBEM3D_SOLVE_GMRES = 0x100    /* flag for the mixed modes: refine by GMRES-IR (FP64 GMRES, right-
+  IBEM3D_SOLVE_GMRES = 0x100,   /* flag for the mixed modes: refine by GMRES-IR (FP64 GMRES, right-
                                    preconditioned by the FP32 factors) from the start. Without it the
                                    refinement is classic and switches to GMRES-IR when an iteration
                                    gains less than a factor of 10. */
+  IBEM3D_SOLVE_REUSE = 0x200    /* flag: solve with the factors of the last factorisation in the same
+                                   mode (no new factorisation). Valid while the system matrix is the
+                                   one factored: same context, only known values changed
+                                   (ibem3d_set_known_values). */
 };
 
 /* Create a context on HIP device `device`: copies the problem to the device, builds the BC column
@@ -104,6 +108,21 @@ int ibem3d_get_ut(ibem3d_ctx* ctx, double* U, double* T, double* X60);
 
 /* Stresses at the problem's points: nump x 6 row-major (s00 s01 s02 s11 s12 s22). */
 int ibem3d_stress(ibem3d_ctx* ctx, double* s);
+
+/* Replace the known boundary values (no counterpart in the reference, which re-reads its inputs
+ * for every load): bc_values (3 NN) are the prescribed displacements where BEM.txt marks a
+ * displacement known, tbc_values (12 NE) the prescribed tractions where a traction is known; the
+ * flags of which values are known stay those of ibem3d_create, so the system matrix is unchanged
+ * and its factors stay valid (IBEM3D_SOLVE_REUSE). Clears the assembled system: run the assembly
+ * phases again, then solve. Entries at positions that are not known are ignored. */
+int ibem3d_set_known_values(ibem3d_ctx* ctx, const double* bc_values, const double* tbc_values);
+
+/* Volume average of the equivalent eigenstrain (no counterpart in the reference):
+ * <e*_ij> = (1/V) sum_spheres V_s (E_ij + (a_s^2 / 5) sum_k E_ij,kk), the integral of the quadratic
+ * eigenstrain polynomial over each sphere; V as in ibem3d_average_fields. With it the average
+ * stress follows from the inclusions, <s> = C0 : (<e> - <e*>), without the boundary tractions.
+ * eps_star: 6 values in the order 00 01 02 11 12 22. */
+int ibem3d_average_eigenstrain(ibem3d_ctx* ctx, double* eps_star);
 
 /* Volume averages over the RVE from the boundary solution (no counterpart in the reference: the
  * effective-property post-step). With U, T as ibem3d_get_ut returns them, on every element's 8 x 8

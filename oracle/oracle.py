@@ -218,6 +218,21 @@ def average_fields(prob, U, T):
     return np.array([S[i, j] for i, j in idx]), np.array([E[i, j] for i, j in idx]), V
 
 
+def average_eigenstrain(prob, U, V):
+    """(1/V) sum_s V_s (E_ij + a_s^2/5 sum_k E_ij,kk): the integral over each sphere of the quadratic
+    eigenstrain polynomial E_ij + E_ij,k x_k + E_ij,kl x_k x_l (78-layout unknowns,
+    configElastic3D.cpp:160-195), using int x_k dV = 0 and int x_k x_l dV = V_s a^2/5 d_kl."""
+    U = np.asarray(U)
+    acc = np.zeros(6)
+    for h in range(prob.num):
+        a = float(prob.radii[h])
+        Vs = 4.0 / 3.0 * np.pi * a ** 3
+        E = U[3 * prob.NN + 78 * h: 3 * prob.NN + 78 * (h + 1)]
+        quad = E[24:].reshape(6, 3, 3)
+        acc += Vs * (E[:6] + a * a / 5.0 * np.trace(quad, axis1=1, axis2=2))
+    return acc / V
+
+
 def fullpiv_solve(A, b):
     n = A.shape[0]
     M = np.array(A, dtype=np.float64, order="F", copy=True)

@@ -11,6 +11,7 @@ Pinned by what the mechanics fixes, independently of the code under test:
 The GPU tests compare the engine's averages (ibem3d_average_fields) with the oracle's."""
 import numpy as np
 
+from ibemproject_amd.engine import isotropic_stress
 from ibemproject_amd.generate import make_rve
 
 
@@ -63,3 +64,56 @@ def test_average_fields_matches_its_definition_on_linear_fields(oracle_lib):
     assert abs(V - 1.0) < 1e-13
     assert np.abs(sig - np.array([Sm[i, j] for i, j in idx])).max() < 1e-13
     assert np.abs(eps - np.array([Gs[i, j] for i, j in idx])).max() < 1e-13
+
+
+def oracle_stiffness(oracle_lib, prob):
+    """K_inc and K_bnd (see ibemproject_amd.engine.effective_stiffness) from six oracle solves."""
+    from ibemproject_amd.engine import isotropic_stress, unit_strain
+    from ibemproject_amd.formats import Problem
+    from ibemproject_amd.generate import macro_strain_bc
+    nodes = np.asarray(prob.nodes).reshape(-1, 3)
+    conn = np.asarray(prob.conn).reshape(-1, 4)
+    K_inc = np.zeros((6, 6))
+    K_bnd = np.zeros((6, 6))
+    for t in range(6):
+        bc, tbc = macro_strain_bc(nodes, conn, unit_strain(t))
+        p = Problem(prob.title, prob.E0, prob.nu0, prob.E1, prob.nu1, nodes, conn, bc, tbc,
+                    prob.centres, prob.radii, np.zeros((0, 3)))
+        r = oracle_lib.solve_problem(p, layout=78)
+        sig, eps, V = oracle_lib.average_fields(p, r["U"], r["T"])
+        eps_star = oracle_lib.average_eigenstrain(p, r["U"], V)
+        K_inc[:, t] = isotropic_stress(prob.E0, prob.nu0, eps - eps_star)
+        K_bnd[:, t] = sig
+    return K_inc, K_bnd
+
+
+def test_stiffness_homogeneous_limit(oracle_lib):
+    """Inclusions of the matrix material: the apparent stiffness is C0 itself."""
+    prob = make_rve(3, 2, 0.10, 0.15, 3, nump=0, E1=1.0, nu1=0.3)
+    K_inc, K_bnd = oracle_stiffness(oracle_lib, prob)
+    C0 = np.column_stack([isotropic_stress(prob.E0, prob.nu0, np.eye(6)[t] * (1.0)) for t in range(6)])
+    print("K_inc - C0", np.abs(K_inc - C0).max(), "K_bnd - C0", np.abs(K_bnd - C0).max())
+    assert np.abs(K_inc - C0).max() < 1e-10 * np.abs(C0).max()
+
+
+def test_stiffness_bounds(oracle_lib):
+    """Stiffer spheres: the isotropic projection of K_inc (bulk and shear moduli) lies between the
+    Reuss and Voigt bounds of the volume fraction, and K_inc is nearly symmetric."""
+    prob = make_rve(5, 4, 0.12, 0.16, 3, nump=0, E1=10.0, nu1=0.25)
+    K_inc, K_bnd = oracle_stiffness(oracle_lib, prob)
+    phi = float(np.sum(4.0 / 3.0 * np.pi * prob.radii ** 3))
+    def km(E, nu):
+        return E / (3 * (1 - 2 * nu)), E / (2 * (1 + nu))
+    k0, g0 = km(prob.E0, prob.nu0)
+    k1, g1 = km(prob.E1, prob.nu1)
+    # isotropic projection: 3k = mean of the diagonal-block row sums, 2g = mean of the shear diagonal
+    d = [0, 3, 5]
+    k = np.mean([K_inc[i, d].sum() for i in d]) / 3.0
+    g = np.mean([K_inc[t, t] for t in (1, 2, 4)]) / 2.0
+    print(f"phi {phi:.4f} k {k:.5f} in [{1/(phi/k1+(1-phi)/k0):.5f}, {phi*k1+(1-phi)*k0:.5f}]  "
+          f"g {g:.5f} in [{1/(phi/g1+(1-phi)/g0):.5f}, {phi*g1+(1-phi)*g0:.5f}]")
+    print("asymmetry", np.abs(K_inc - K_inc.T).max() / np.abs(K_inc).max(),
+          "K_bnd vs K_inc", np.abs(K_bnd - K_inc).max() / np.abs(K_inc).max())
+    assert 1 / (phi / k1 + (1 - phi) / k0) <= k <= phi * k1 + (1 - phi) * k0
+    assert 1 / (phi / g1 + (1 - phi) / g0) <= g <= phi * g1 + (1 - phi) * g0
+    assert np.abs(K_inc - K_inc.T).max() < 0.02 * np.abs(K_inc).max()
