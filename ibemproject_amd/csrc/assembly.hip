@@ -151,8 +151,9 @@ __global__ __launch_bounds__(192) void k_field(const double* __restrict__ nodes,
 // (node block, element block) workgroups rather than over the 3 NN rows alone:
 //   k_bem_quad  workgroup = 64 nodes x 3 components x kBemEPB elements: the 24 values of every
 //               (row, element) pair into a workspace, [element][value][row];
-//   k_bem_fold  thread = row: the values in element order through the column map (the order of a
-//               single pass over all elements, so results do not depend on the chunking).
+//   k_bem_fold  thread = (row, component): the values in element order through the column map
+//               (the order of a single pass over all elements, so results do not depend on the
+//               chunking); k_bem_finish adds the three components' known-value sums to b.
 constexpr int kBemEPB = 4;  // elements per quadrature workgroup
 
 __global__ __launch_bounds__(192) void k_bem_quad(const double* __restrict__ qp,
@@ -237,7 +238,10 @@ __global__ __launch_bounds__(192) void k_bem_quad(const double* __restrict__ qp,
   }
 }
 
-// rs (3 R) and bacc (R) carry the row sums of H and the known-value terms across rounds
+// Thread = (row, component c): the columns of component c only (node dof 3 nd + c, traction keys
+// of component c), so the three threads of a row never write the same entry. rs and bacc carry the
+// row sum of H and the known-value terms of (row, c) across rounds ([c][R] each); the last round
+// adds the rigid-body diagonal and leaves the three known-value sums for k_bem_finish.
 __global__ __launch_bounds__(256) void k_bem_fold(const int* __restrict__ conn, int NN, int e0, int ne,
                                                   long n, const int* __restrict__ node_col,
                                                   const int* __restrict__ slot_col,
@@ -245,62 +249,55 @@ __global__ __launch_bounds__(256) void k_bem_fold(const int* __restrict__ conn, 
                                                   const double* __restrict__ tval,
                                                   const double* __restrict__ W,
                                                   double* __restrict__ carry, int first, int last,
-                                                  double* __restrict__ A, double* __restrict__ b) {
+                                                  double* __restrict__ A) {
   const long R = 3L * NN;
   const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
   if (row >= R) return;
-  double rs[3], bacc;
-  if (first) {
-    rs[0] = rs[1] = rs[2] = 0.0;
-    bacc = 0.0;
-  } else {
-    rs[0] = carry[row];
-    rs[1] = carry[R + row];
-    rs[2] = carry[2 * R + row];
-    bacc = carry[3 * R + row];
-  }
+  double rs = first ? 0.0 : carry[c * R + row];
+  double bacc = first ? 0.0 : carry[(3 + c) * R + row];
   for (int el = 0; el < ne; ++el) {
     const int e = e0 + el;
     const double* in = W + (size_t)el * 24 * R + row;
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int nd = conn[4 * e + q] - 1;
-      for (int c = 0; c < 3; ++c) {
-        const int dof = 3 * nd + c;
-        const double hv = in[(12 + 3 * q + c) * R];
-        rs[c] += hv;
-        const int col = node_col[dof];
-        if (col >= 0)
-          A[(long)col * n + row] += hv;
-        else
-          bacc -= hv * uval[dof];
-        const int slot = 12 * e + 3 * q + c;
-        const double gv = in[(3 * q + c) * R];
-        const int kc = slot_col[slot];
-        if (kc >= 0)
-          A[(long)kc * n + row] -= gv;
-        else if (kc == -1)
-          bacc += gv * tval[slot];
-      }
+      const int dof = 3 * nd + c;
+      const double hv = in[(12 + 3 * q + c) * R];
+      rs += hv;
+      const int col = node_col[dof];
+      if (col >= 0)
+        A[(long)col * n + row] += hv;
+      else
+        bacc -= hv * uval[dof];
+      const int slot = 12 * e + 3 * q + c;
+      const double gv = in[(3 * q + c) * R];
+      const int kc = slot_col[slot];
+      if (kc >= 0)
+        A[(long)kc * n + row] -= gv;
+      else if (kc == -1)
+        bacc += gv * tval[slot];
     }
   }
-  if (!last) {
-    carry[row] = rs[0];
-    carry[R + row] = rs[1];
-    carry[2 * R + row] = rs[2];
-    carry[3 * R + row] = bacc;
-    return;
-  }
-  const int i = (int)(row / 3);
-  for (int c = 0; c < 3; ++c) {
-    const int dof = 3 * i + c;
-    const double dv = -rs[c];
+  if (last) {
+    const int dof = 3 * (int)(row / 3) + c;
     const int col = node_col[dof];
     if (col >= 0)
-      A[(long)col * n + row] += dv;
+      A[(long)col * n + row] -= rs;
     else
-      bacc -= dv * uval[dof];
+      bacc += rs * uval[dof];
   }
-  b[row] += bacc;
+  carry[c * R + row] = rs;
+  carry[(3 + c) * R + row] = bacc;
+}
+
+// b[row] += the three components' known-value sums, in component order
+__global__ __launch_bounds__(256) void k_bem_finish(int NN, const double* __restrict__ carry,
+                                                    double* __restrict__ b) {
+  const long R = 3L * NN;
+  const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= R) return;
+  b[row] += (carry[3 * R + row] + carry[4 * R + row]) + carry[5 * R + row];
 }
 
 // ---- K2: inclusion rows x boundary columns + b -------------------------------------------------
@@ -478,7 +475,7 @@ void launch_field(const double* nodes, int NN, const double* cen, const double* 
 }
 
 size_t bem_workspace_doubles(int NN, int NE) {
-  return ((size_t)std::min(NE, kBemChunk) * 24 + 4) * 3 * (size_t)NN;
+  return ((size_t)std::min(NE, kBemChunk) * 24 + 6) * 3 * (size_t)NN;
 }
 
 void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, int NE, long n,
@@ -491,10 +488,11 @@ void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, 
     const int ne = std::min(kBemChunk, NE - e0);
     hipLaunchKernelGGL(k_bem_quad, dim3((NN + 63) / 64, (ne + kBemEPB - 1) / kBemEPB), dim3(192), 0, st,
                        qp, conn, nodes, NN, e0, ne, E0, nu0, W);
-    hipLaunchKernelGGL(k_bem_fold, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st, conn, NN, e0, ne,
-                       n, node_col, slot_col, uval, tval, W, carry, (int)(e0 == 0),
-                       (int)(e0 + ne >= NE), A, b);
+    hipLaunchKernelGGL(k_bem_fold, dim3((unsigned)((R + 255) / 256), 3), dim3(256), 0, st, conn, NN, e0,
+                       ne, n, node_col, slot_col, uval, tval, W, carry, (int)(e0 == 0),
+                       (int)(e0 + ne >= NE), A);
   }
+  hipLaunchKernelGGL(k_bem_finish, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st, NN, carry, b);
 }
 
 void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long n,

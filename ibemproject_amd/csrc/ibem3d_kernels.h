@@ -18,7 +18,7 @@ void launch_incl(const double* cen, const double* rad, int num, long n, long row
                  int rowmask, double* A, hipStream_t st);
 void launch_field(const double* nodes, int NN, const double* cen, const double* rad, int num, long n,
                   long row0, const Mat& M, double* A, hipStream_t st);
-// k_bem's workspace: kBemChunk elements x 24 values x 3 NN rows, plus 4 x 3 NN carried sums
+// k_bem's workspace: kBemChunk elements x 24 values x 3 NN rows, plus 6 x 3 NN carried sums
 constexpr int kBemChunk = 256;
 size_t bem_workspace_doubles(int NN, int NE);
 void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, int NE, long n,

@@ -1051,7 +1051,7 @@ template <typename T>
 static void laswp_narrow(T* A, long lda, long c0, long nc, const int* ipiv, long p0, long p1,
                          hipStream_t st) {
   if (nc <= 0 || p1 <= p0) return;
-  const long cpb = std::max<long>(kLaswpCG, (nc + 63) / 64 / kLaswpCG * kLaswpCG);
+  const long cpb = std::max<long>(kLaswpCG, (nc + 255) / 256 / kLaswpCG * kLaswpCG);
   hipLaunchKernelGGL(k_laswp_mv<T>, dim3((unsigned)((nc + cpb - 1) / cpb)), dim3(256), 0, st, A, lda,
                      c0, nc, ipiv, p0, (int)(p1 - p0), (int)cpb);
 }
@@ -1216,6 +1216,7 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
 }
 
 constexpr long kLuNB = IBEM_LU_NB;
+constexpr long kNarrowCols = 4096;  // trailing columns up to which a step uses the latency-bound forms
 
 // Trailing update of step k restricted to columns [c0, c0 + nc) of the trailing matrix (relative to
 // column k + kb): C -= L21 U12(:, cols).
@@ -1352,11 +1353,18 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
     // the interchanges of this step go to the trailing columns only; L's earlier block columns
     // keep their order (the solve applies the interchanges to the vector instead)
     if (k + kb < n) {
-      laswp<T>(A, lda, k + kb, n - k - kb, W.d_ipiv, k, k + kb, st);
-      if (sizeof(T) == 4)
-        trsm_by_inverse<T>(A + k * lda + k, lda, kb, A + (k + kb) * lda + k, lda, n - k - kb, W, st);
+      // few trailing columns (the last steps, on the critical path): the move-list interchanges and
+      // the wavefront TRSM, whose latency does not grow with the 1024 sequential interchanges or
+      // the block-inverse chain; many columns: the bandwidth-bound forms
+      const long nt = n - k - kb;
+      if (nt <= kNarrowCols)
+        laswp_narrow<T>(A, lda, k + kb, nt, W.d_ipiv, k, k + kb, st);
       else
-        trsm_lower_unit<T>(A + k * lda + k, lda, kb, A + (k + kb) * lda + k, lda, n - k - kb, st);
+        laswp<T>(A, lda, k + kb, nt, W.d_ipiv, k, k + kb, st);
+      if (sizeof(T) == 4 && nt > kNarrowCols)
+        trsm_by_inverse<T>(A + k * lda + k, lda, kb, A + (k + kb) * lda + k, lda, nt, W, st);
+      else
+        trsm_lower_unit<T>(A + k * lda + k, lda, kb, A + (k + kb) * lda + k, lda, nt, st);
     }
     if (tick) tick(tctx, "lu.swap_trsm", 1, st);
     if (k + kb >= n) break;
