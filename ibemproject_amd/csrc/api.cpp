@@ -52,6 +52,7 @@ struct ibem3d_ctx {
   // dot-product partial sums, small coefficient vector
   double *d_gV = nullptr, *d_gZ = nullptr, *d_gpart = nullptr, *d_gh = nullptr;
   double* d_bemws = nullptr;  // boundary-boundary quadrature values of one round of elements
+  double* d_gemvws = nullptr; // residual GEMV partial sums
   int prepared_mode = -1;      // mode whose factors and solve data are in place (-1: none)
   LuWork W;
   unsigned phases = 0;
@@ -255,7 +256,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
                   ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32, ctx->d_f16ws, ctx->d_eqws,
-                  ctx->d_gV,    ctx->d_gZ,       ctx->d_gpart,    ctx->d_gh,   ctx->d_bemws,
+                  ctx->d_gV,    ctx->d_gZ,       ctx->d_gpart,    ctx->d_gh,   ctx->d_bemws, ctx->d_gemvws,
                   ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info, ctx->W.d_perm, ctx->W.d_inv, ctx->W.d_tmp,
                   ctx->W.d_trinv, ctx->W.d_trsm};
   for (void* q : ptrs)
@@ -396,6 +397,7 @@ static int ensure_solver_ws(ibem3d_ctx* ctx) {
     if (int rc = dalloc(ctx, &ctx->d_c, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_work, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_res, (size_t)n)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_gemvws, gemv_workspace_doubles(n))) return rc;
   }
   return 0;
 }
@@ -476,7 +478,7 @@ static int residual(ibem3d_ctx* ctx, double* rel) {
   HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
   {
     Scope s(ctx, "solve.residual");
-    gemv_f64(n, n, -1.0, ctx->d_A, n, ctx->d_x, 1.0, ctx->d_res, ctx->st);
+    gemv_f64(n, n, -1.0, ctx->d_A, n, ctx->d_x, 1.0, ctx->d_res, ctx->d_gemvws, ctx->st);
   }
   std::vector<double> r(n), b(n);
   HIPC(hipMemcpyAsync(r.data(), ctx->d_res, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
@@ -535,7 +537,7 @@ static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0) {
       }
       {
         Scope s(ctx, "solve.residual");
-        gemv_f64(n, n, 1.0, ctx->d_A, n, zj, 0.0, w, ctx->st);
+        gemv_f64(n, n, 1.0, ctx->d_A, n, zj, 0.0, w, ctx->d_gemvws, ctx->st);
       }
       if (int rc = after_launch(ctx, "gmres")) return rc;
       double* Hj = H.data() + (size_t)j * (kGmres + 1);  // column j

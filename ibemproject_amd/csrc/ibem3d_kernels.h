@@ -90,9 +90,10 @@ void lu_solve_f32(const float* LU, long n, long lda, const LuWork& W, float* x, 
 
 // row / column equilibration: r_i = 1 / max_j |A_ij|, c_j = 1 / max_i |r_i A_ij|; A <- R A C
 void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStream_t st);
-// y = alpha A x + beta y (FP64, column-major)
+// y = alpha A x + beta y (FP64, column-major); ws holds gemv_workspace_doubles(m) doubles
+size_t gemv_workspace_doubles(long m);
 void gemv_f64(long m, long n, double alpha, const double* A, long lda, const double* x, double beta,
-              double* y, hipStream_t st);
+              double* y, double* ws, hipStream_t st);
 // out = R A C with the same scaling, A left unchanged (FP64 or FP32 output); ws holds
 // equilibrate_workspace_doubles(n) doubles
 size_t equilibrate_workspace_doubles(long n);

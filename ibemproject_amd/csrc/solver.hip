@@ -1737,30 +1737,45 @@ void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStrea
 }
 
 // ==================================================================================== misc =======
-// y = alpha A x + beta y, column chunks of 1024 reduced through a partial buffer per row chunk.
-__global__ __launch_bounds__(256) void k_gemv(long m, long n, double alpha,
-                                              const double* __restrict__ A, long lda,
-                                              const double* __restrict__ x, double beta,
-                                              double* __restrict__ y) {
+// y = alpha A x + beta y. The columns are split into kGemvChunks ranges so that enough loads are in
+// flight to stream A at HBM speed (one row per thread over all n columns keeps too few); the range
+// partial sums are added in a fixed order by a second kernel.
+constexpr int kGemvChunks = 32;
+size_t gemv_workspace_doubles(long m) { return (size_t)kGemvChunks * m; }
+
+__global__ __launch_bounds__(256) void k_gemv_part(long m, long n, const double* __restrict__ A,
+                                                   long lda, const double* __restrict__ x,
+                                                   double* __restrict__ part) {
   const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= m) return;
-  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
-  long j = 0;
-  for (; j + 4 <= n; j += 4) {
-    acc0 += A[j * lda + r] * x[j];
-    acc1 += A[(j + 1) * lda + r] * x[j + 1];
-    acc2 += A[(j + 2) * lda + r] * x[j + 2];
-    acc3 += A[(j + 3) * lda + r] * x[j + 3];
+  const long cw = (n + kGemvChunks - 1) / kGemvChunks;
+  const long j0 = (long)blockIdx.y * cw, j1 = min(n, j0 + cw);
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long j = j0;
+  for (; j + 8 <= j1; j += 8) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] += A[(j + q) * lda + r] * x[j + q];
   }
-  for (; j < n; ++j) acc0 += A[j * lda + r] * x[j];
-  const double v = alpha * ((acc0 + acc1) + (acc2 + acc3));
+  for (; j < j1; ++j) acc[0] += A[j * lda + r] * x[j];
+  part[(long)blockIdx.y * m + r] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+}
+
+__global__ __launch_bounds__(256) void k_gemv_done(long m, double alpha, const double* __restrict__ part,
+                                                   double beta, double* __restrict__ y) {
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  double v = 0.0;
+  for (int c = 0; c < kGemvChunks; ++c) v += part[(long)c * m + r];
+  v *= alpha;
   y[r] = beta == 0.0 ? v : v + beta * y[r];
 }
 
 void gemv_f64(long m, long n, double alpha, const double* A, long lda, const double* x, double beta,
-              double* y, hipStream_t st) {
-  hipLaunchKernelGGL(k_gemv, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, m, n, alpha, A, lda,
-                     x, beta, y);
+              double* y, double* ws, hipStream_t st) {
+  hipLaunchKernelGGL(k_gemv_part, dim3((unsigned)((m + 255) / 256), kGemvChunks), dim3(256), 0, st, m,
+                     n, A, lda, x, ws);
+  hipLaunchKernelGGL(k_gemv_done, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, m, alpha, ws,
+                     beta, y);
 }
 
 __global__ __launch_bounds__(256) void k_f64_to_f32(const double* __restrict__ A, long n, long lda,
