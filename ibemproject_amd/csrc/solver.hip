@@ -1418,118 +1418,192 @@ size_t lu_inverse_doubles(long n) {
   return (size_t)nb * kTB * kTB * 2;
 }
 
-// x_blk <- inverse block b times y_blk (kb <= kTB entries at y + r0), 128 threads of workgroup 0
+// Solve steps of kTS = 2 kTB rows, run by workgroups of kTT = 1024 threads: every launch reduces
+// all rows past the just-solved step with its 256 columns (a coalesced tall GEMV, 4 threads per row),
+// and workgroup 0, whose rows are the next step, finishes that step at once: two kTB-blocks, each
+// through its precomputed inverse, the second after the first's contribution (8 threads per row).
+// Short per-thread chains keep each dependent launch near its memory latency.
+constexpr int kTS = 2 * kTB;
+constexpr int kTT = 1024;
+static_assert(IBEM_LU_NB % kTS == 0, "an LU step spans whole solve steps");
+static_assert(kTT == 4 * kTS && kTT == 8 * kTB, "thread layout of the solve steps");
+
+// Row sums of a rows x cols block (column-major, leading dimension ldm; rows, cols <= kTB) times xs
+// (LDS): thread (g, i), i = t % kTB, g = t / kTB, sums columns [16 g, 16 g + 16) of row i, and the
+// 8 partials are added in a fixed order. Returns row t's sum to threads t < rows. All kTT threads
+// call it (it holds barriers); pp: kTT entries of LDS.
 template <typename T>
-__device__ void trsv_apply_inverse(const T* __restrict__ invb, int kb, const T* sy, T* __restrict__ y,
-                                   long r0) {
-  const int t = threadIdx.x;
-  if (t < kb) {
-    T acc = T(0);
-    for (int c = 0; c < kb; ++c) acc += invb[c * kTB + t] * sy[c];
-    y[r0 + t] = acc;
+__device__ T block_mv(const T* __restrict__ M, long ldm, int rows, int cols, const T* xs, T* pp) {
+  const int t = threadIdx.x, i = t % kTB, g = t / kTB;
+  T acc0 = T(0), acc1 = T(0);
+  if (i < rows) {
+    const T* m = M + (long)(16 * g) * ldm + i;
+    if (16 * g + 16 <= cols) {
+#pragma unroll
+      for (int c = 0; c < 16; c += 2) {
+        acc0 += m[(long)c * ldm] * xs[16 * g + c];
+        acc1 += m[(long)(c + 1) * ldm] * xs[16 * g + c + 1];
+      }
+    } else {
+      for (int c = 0; 16 * g + c < cols; ++c) acc0 += m[(long)c * ldm] * xs[16 * g + c];
+    }
   }
+  pp[t] = acc0 + acc1;
+  __syncthreads();
+  T r = T(0);
+  if (t < rows)
+    r = ((pp[t] + pp[kTB + t]) + (pp[2 * kTB + t] + pp[3 * kTB + t])) +
+        ((pp[4 * kTB + t] + pp[5 * kTB + t]) + (pp[6 * kTB + t] + pp[7 * kTB + t]));
+  __syncthreads();
+  return r;
 }
 
-// Forward sweep step. Block k (rows [k0, k0 + kb)) of y is solved; rows below it are reduced by
-// L[:, block k] y_k, and workgroup 0 (rows of block k + 1) then solves block k + 1 unless that block
-// starts an LU step (solve_next = 0: k_trsv_fwd_step does it after the step's interchanges).
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_fwd(const T* __restrict__ LU, long lda, long n, long k0,
-                                                  int kb, const T* __restrict__ inv,
-                                                  T* __restrict__ y, int solve_next) {
-  __shared__ T sx[kTB];
-  __shared__ T part[2][kTB];
-  __shared__ T sy[kTB];
-  const int t = threadIdx.x, h = t / kTB, rl = t % kTB;
-  if (t < kb) sx[t] = y[k0 + t];
-  __syncthreads();
-  const long r0 = k0 + kb + (long)blockIdx.x * kTB;
-  const long r = r0 + rl;
-  T acc = T(0);
-  if (r < n) {
-    const int c0 = h * (kTB / 2), c1 = min(kb, c0 + kTB / 2);
-    const T* a = LU + (k0 + c0) * lda + r;
-#pragma unroll 8
-    for (int c = c0; c < c1; ++c, a += lda) acc += *a * sx[c];
+// y[r0, r0 + kn) <- the solution of the kn x kn diagonal block at r0 (kn <= kTS) for the reduced
+// right-hand side sy (LDS). Lower (forward sweep, unit L): x0 = D0 sy0, x1 = D1 (sy1 - L10 x0).
+// Upper (backward sweep): x1 = D1 sy1, x0 = D0 (sy0 - U01 x1). D = inverses of the kTB-blocks
+// (inv + block * kTB^2, column-major); sz: kTS + kTB entries of LDS; pp: kTT entries of LDS.
+template <typename T, bool Upper>
+__device__ void trsv_finish(const T* __restrict__ LU, long lda, const T* __restrict__ inv, long r0,
+                            int kn, const T* sy, T* sz, T* pp, T* __restrict__ y) {
+  const int t = threadIdx.x;
+  const int n0 = min(kTB, kn), n1 = kn - n0;
+  const T* D0 = inv + (size_t)(r0 / kTB) * kTB * kTB;
+  const T* D1 = D0 + (size_t)kTB * kTB;
+  // first the block that does not depend on the other: 0 for Lower, 1 for Upper
+  const int fa = Upper && n1 > 0 ? n0 : 0, na = Upper && n1 > 0 ? n1 : n0;
+  const T xa = block_mv(Upper && n1 > 0 ? D1 : D0, kTB, na, na, sy + fa, pp);
+  if (t < na) {
+    sz[fa + t] = xa;
+    y[r0 + fa + t] = xa;
   }
-  part[h][rl] = acc;
+  if (n1 == 0) return;
+  __syncthreads();
+  // the other block, reduced by the first block's solution through the off-diagonal block of LU:
+  // column (r0 + fa + c), row (r0 + fb + t)
+  const int fb = Upper ? 0 : n0, nb = Upper ? n0 : n1;
+  const T w = block_mv(LU + (r0 + fa) * lda + r0 + fb, lda, nb, na, sz + fa, pp);
+  if (t < nb) sz[kTS + t] = sy[fb + t] - w;
+  __syncthreads();
+  const T xb = block_mv(Upper ? D0 : D1, kTB, nb, nb, sz + kTS, pp);
+  if (t < nb) y[r0 + fb + t] = xb;
+}
+
+// Forward sweep. Step [k0, k0 + kTS) of y is solved; the rows past it are reduced by its columns
+// of L, and workgroup 0 (the next step's rows) finishes the next step unless it starts an LU step
+// (solve_next = 0: k_trsv_fwd_step does it after the step's interchanges).
+template <typename T>
+__global__ __launch_bounds__(1024) void k_trsv_fwd(const T* __restrict__ LU, long lda, long n, long k0,
+                                                   const T* __restrict__ inv, T* __restrict__ y,
+                                                   int solve_next) {
+  __shared__ T sx[kTS];
+  __shared__ T sy[kTS];
+  __shared__ T sz[kTS + kTB];
+  __shared__ T pp[kTT];
+  const int t = threadIdx.x, rl = t % kTS, g = t / kTS;
+  if (t < kTS) sx[t] = y[k0 + t];  // k0 + kTS < n
+  __syncthreads();
+  const long r0 = k0 + kTS + (long)blockIdx.x * kTS;
+  const long r = r0 + rl;
+  T p0 = T(0), p1 = T(0), p2 = T(0), p3 = T(0);
+  if (r < n) {
+    const T* a = LU + (k0 + 64 * g) * lda + r;
+    const T* xs = sx + 64 * g;
+#pragma unroll
+    for (int c = 0; c < 64; c += 4) {
+      p0 += a[(long)c * lda] * xs[c];
+      p1 += a[(long)(c + 1) * lda] * xs[c + 1];
+      p2 += a[(long)(c + 2) * lda] * xs[c + 2];
+      p3 += a[(long)(c + 3) * lda] * xs[c + 3];
+    }
+  }
+  pp[t] = (p0 + p1) + (p2 + p3);
   __syncthreads();
   const bool own = blockIdx.x == 0 && solve_next;
-  if (h == 0 && r < n) {
-    const T v = y[r] - (part[0][rl] + part[1][rl]);
+  if (t < kTS && r < n) {
+    const T v = y[r] - ((pp[t] + pp[kTS + t]) + (pp[2 * kTS + t] + pp[3 * kTS + t]));
     if (own)
-      sy[rl] = v;
+      sy[t] = v;
     else
       y[r] = v;
   }
   if (own) {
     __syncthreads();
-    const int kn = (int)min((long)kTB, n - r0);
-    trsv_apply_inverse(inv + (size_t)(r0 / kTB) * kTB * kTB, kn, sy, y, r0);
+    trsv_finish<T, false>(LU, lda, inv, r0, (int)min((long)kTS, n - r0), sy, sz, pp, y);
   }
 }
 
-// Forward sweep at the first block of an LU step: the step's interchanges applied to y as moves
-// y[dst] <- y[src] (all reads, then all writes), then the diagonal block at r0 solved with its
-// inverse. One workgroup; a step has at most 2 IBEM_LU_NB moves.
+// Forward sweep at the first solve step of an LU step: the step's interchanges applied to y as moves
+// y[dst] <- y[src] (all reads, then all writes), then the solve step at r0 finished. One workgroup;
+// an LU step has at most 2 IBEM_LU_NB moves.
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_fwd_step(const int* __restrict__ mv, int nmv, long r0,
-                                                       long n, const T* __restrict__ inv,
-                                                       T* __restrict__ y) {
+__global__ __launch_bounds__(1024) void k_trsv_fwd_step(const int* __restrict__ mv, int nmv, long r0,
+                                                        long n, const T* __restrict__ LU, long lda,
+                                                        const T* __restrict__ inv, T* __restrict__ y) {
   __shared__ T sv[2 * IBEM_LU_NB];
-  __shared__ T sy[kTB];
+  __shared__ T sy[kTS];
+  __shared__ T sz[kTS + kTB];
+  __shared__ T pp[kTT];
   const int t = threadIdx.x;
-  for (int q = t; q < nmv; q += 256) sv[q] = y[mv[2 * q + 1]];
+  for (int q = t; q < nmv; q += kTT) sv[q] = y[mv[2 * q + 1]];
   __syncthreads();
-  for (int q = t; q < nmv; q += 256) y[mv[2 * q]] = sv[q];
+  for (int q = t; q < nmv; q += kTT) y[mv[2 * q]] = sv[q];
   __syncthreads();
-  const int kn = (int)min((long)kTB, n - r0);
+  const int kn = (int)min((long)kTS, n - r0);
   if (t < kn) sy[t] = y[r0 + t];
   __syncthreads();
-  trsv_apply_inverse(inv + (size_t)(r0 / kTB) * kTB * kTB, kn, sy, y, r0);
+  trsv_finish<T, false>(LU, lda, inv, r0, kn, sy, sz, pp, y);
 }
 
-// Backward sweep step. Block k (rows [k0, k0 + kb)) of y is solved; rows above it are reduced by
-// U[:, block k] y_k, and workgroup 0 (rows of block k - 1) then solves block k - 1. With k0 < 0 the
-// launch only solves the last block (rows [lastk, n)).
+// Backward sweep. Step [k0, k0 + kb) of y is solved; the rows above it are reduced by its columns
+// of U, and workgroup 0 (rows [k0 - kTS, k0)) finishes that step. With k0 < 0 the launch only
+// finishes the last step [lastk, n).
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_bwd(const T* __restrict__ LU, long lda, long n, long k0,
-                                                  int kb, long lastk, const T* __restrict__ inv,
-                                                  T* __restrict__ y) {
-  __shared__ T sx[kTB];
-  __shared__ T part[2][kTB];
-  __shared__ T sy[kTB];
-  const int t = threadIdx.x, h = t / kTB, rl = t % kTB;
+__global__ __launch_bounds__(1024) void k_trsv_bwd(const T* __restrict__ LU, long lda, long n, long k0,
+                                                   int kb, long lastk, const T* __restrict__ inv,
+                                                   T* __restrict__ y) {
+  __shared__ T sx[kTS];
+  __shared__ T sy[kTS];
+  __shared__ T sz[kTS + kTB];
+  __shared__ T pp[kTT];
+  const int t = threadIdx.x;
   if (k0 < 0) {
-    const int nbl = (int)(n - lastk);
-    if (t < nbl) sy[t] = y[lastk + t];
+    const int kn = (int)(n - lastk);
+    if (t < kn) sy[t] = y[lastk + t];
     __syncthreads();
-    trsv_apply_inverse(inv + (size_t)(lastk / kTB) * kTB * kTB, nbl, sy, y, lastk);
+    trsv_finish<T, true>(LU, lda, inv, lastk, kn, sy, sz, pp, y);
     return;
   }
-  if (t < kb) sx[t] = y[k0 + t];
+  const int rl = t % kTS, g = t / kTS;
+  if (t < kTS) sx[t] = t < kb ? y[k0 + t] : T(0);
   __syncthreads();
-  const long r0 = k0 - (long)(blockIdx.x + 1) * kTB;  // >= 0: k0 is a multiple of kTB
+  const long r0 = k0 - (long)(blockIdx.x + 1) * kTS;  // >= 0: k0 is a multiple of kTS
   const long r = r0 + rl;
-  T acc = T(0);
+  T p0 = T(0), p1 = T(0), p2 = T(0), p3 = T(0);
   {
-    const int c0 = h * (kTB / 2), c1 = min(kb, c0 + kTB / 2);
-    const T* a = LU + (k0 + c0) * lda + r;
-#pragma unroll 8
-    for (int c = c0; c < c1; ++c, a += lda) acc += *a * sx[c];
+    const T* a = LU + (k0 + 64 * g) * lda + r;
+    const T* xs = sx + 64 * g;
+    if (kb == kTS) {
+#pragma unroll
+      for (int c = 0; c < 64; c += 4) {
+        p0 += a[(long)c * lda] * xs[c];
+        p1 += a[(long)(c + 1) * lda] * xs[c + 1];
+        p2 += a[(long)(c + 2) * lda] * xs[c + 2];
+        p3 += a[(long)(c + 3) * lda] * xs[c + 3];
+      }
+    } else {  // the last step: only its kb < kTS columns exist
+      for (int c = 0; 64 * g + c < kb && c < 64; ++c) p0 += a[(long)c * lda] * xs[c];
+    }
   }
-  part[h][rl] = acc;
+  pp[t] = (p0 + p1) + (p2 + p3);
   __syncthreads();
-  if (h == 0) {
-    const T v = y[r] - (part[0][rl] + part[1][rl]);
-    if (blockIdx.x == 0)
-      sy[rl] = v;
-    else
-      y[r] = v;
-  }
+  T v = T(0);
+  if (t < kTS) v = y[r] - ((pp[t] + pp[kTS + t]) + (pp[2 * kTS + t] + pp[3 * kTS + t]));
   if (blockIdx.x == 0) {
+    if (t < kTS) sy[t] = v;
     __syncthreads();
-    trsv_apply_inverse(inv + (size_t)(r0 / kTB) * kTB * kTB, kTB, sy, y, r0);
+    trsv_finish<T, true>(LU, lda, inv, r0, kTS, sy, sz, pp, y);
+  } else if (t < kTS) {
+    y[r] = v;
   }
 }
 
@@ -1537,7 +1611,6 @@ size_t lu_moves_ints(long n) { return (size_t)4 * n + 8; }
 
 template <typename T>
 static int lu_prepare(const T* LU, long n, long lda, LuWork& W, hipStream_t st) {
-  static_assert(IBEM_LU_NB % kTB == 0, "an LU step spans whole solve blocks");
   const long nb = (n + kTB - 1) / kTB;
   T* inv = (T*)W.d_inv;
   hipLaunchKernelGGL((k_trinv<T, false>), dim3((unsigned)nb), dim3(kTB), 0, st, LU, lda, n, inv);
@@ -1589,31 +1662,32 @@ static int lu_prepare(const T* LU, long n, long lda, LuWork& W, hipStream_t st) 
 template <typename T>
 static void lu_solve(const T* LU, long n, long lda, const LuWork& W, T* x, hipStream_t st) {
   const long nb = (n + kTB - 1) / kTB;
+  const long ns = (n + kTS - 1) / kTS;
   const T* invL = (const T*)W.d_inv;
   const T* invU = invL + (size_t)nb * kTB * kTB;
   T* y = (T*)W.d_tmp;
   hipMemcpyAsync(y, x, sizeof(T) * n, hipMemcpyDeviceToDevice, st);
-  for (long b = 0; b < nb; ++b) {
-    const long k0 = b * kTB;
-    if (k0 % IBEM_LU_NB == 0) {  // first block of an LU step: its interchanges, then the block
+  for (long s = 0; s < ns; ++s) {
+    const long k0 = s * kTS;
+    if (k0 % IBEM_LU_NB == 0) {  // first solve step of an LU step: its interchanges, then the step
       const long j = k0 / IBEM_LU_NB;
       const int m0 = W.mv_off[j], nm = W.mv_off[j + 1] - m0;
-      hipLaunchKernelGGL(k_trsv_fwd_step<T>, dim3(1), dim3(256), 0, st, (const int*)W.d_perm + 2 * m0,
-                         nm, k0, n, invL, y);
+      hipLaunchKernelGGL(k_trsv_fwd_step<T>, dim3(1), dim3(kTT), 0, st, (const int*)W.d_perm + 2 * m0,
+                         nm, k0, n, LU, lda, invL, y);
     }
-    if (b + 1 < nb) {
-      const long rows = n - k0 - kTB;
-      const int solve_next = ((b + 1) * kTB) % IBEM_LU_NB != 0;
-      hipLaunchKernelGGL(k_trsv_fwd<T>, dim3((unsigned)((rows + kTB - 1) / kTB)), dim3(256), 0, st, LU,
-                         lda, n, k0, kTB, invL, y, solve_next);
+    if (s + 1 < ns) {
+      const long rows = n - k0 - kTS;
+      const int solve_next = ((s + 1) * kTS) % IBEM_LU_NB != 0;
+      hipLaunchKernelGGL(k_trsv_fwd<T>, dim3((unsigned)((rows + kTS - 1) / kTS)), dim3(kTT), 0, st, LU,
+                         lda, n, k0, invL, y, solve_next);
     }
   }
-  const long lastk = (nb - 1) * kTB;
-  hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(1), dim3(256), 0, st, LU, lda, n, -1L, 0, lastk, invU, y);
-  for (long b = nb - 1; b >= 1; --b) {
-    const long k0 = b * kTB;
-    const int kb = (int)std::min<long>(kTB, n - k0);
-    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3((unsigned)b), dim3(256), 0, st, LU, lda, n, k0, kb, lastk,
+  const long lastk = (ns - 1) * kTS;
+  hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(1), dim3(kTT), 0, st, LU, lda, n, -1L, 0, lastk, invU, y);
+  for (long s = ns - 1; s >= 1; --s) {
+    const long k0 = s * kTS;
+    const int kb = (int)std::min<long>(kTS, n - k0);
+    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3((unsigned)s), dim3(kTT), 0, st, LU, lda, n, k0, kb, lastk,
                        invU, y);
   }
   hipMemcpyAsync(x, y, sizeof(T) * n, hipMemcpyDeviceToDevice, st);
