@@ -501,6 +501,187 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
   }
 }
 
+// Large trailing updates: the same product with workgroup tile 256 x 256, eight waves in a 2 x 4
+// grid of 128 x 64 (8 x 4 blocks of v_mfma_f32_16x16x32_f16), one workgroup per CU. Operands move
+// from memory to LDS by direct loads (global_load_lds_dwordx4: no register staging, no LDS store
+// instructions), double-buffered in stages of 32 k (4 arrays x 256 rows x 32 halves x 2 stages =
+// 128 KB of LDS). Per MAC this reads a third less LDS than the 128 x 128 tile and half the operand
+// bytes from L2. A direct load writes lane l's 16 bytes at LDS offset 16 l of the instruction's
+// 1 KB block, so the lane -> (row, chunk) assignment sets the layout: 4 lanes per row (64
+// contiguous bytes from memory), logical chunk c of row r at physical chunk c ^ ((r / 4) % 4), so
+// the 16 rows a half-wave fragment read touches fall in distinct banks. Rows past M (N) load the
+// last valid row instead of zeros; they only feed accumulators that are never stored. Every
+// accumulator sees the three products in the same order and K steps as k_gemm_f16x3, so both
+// kernels give identical results.
+typedef __attribute__((address_space(3))) void lds_void_t;
+constexpr int kBigT = 256;                // tile rows and columns
+constexpr int kBigK = 32;                 // k per LDS stage
+constexpr int kBigArr = kBigT * kBigK;    // halves per operand array and stage
+constexpr int kBigWaves = 8;
+constexpr int kBigQ = 16 / kBigWaves;     // 16-row blocks each wave loads per array and stage
+__global__ __launch_bounds__(64 * kBigWaves, 1) void k_gemm_f16x3_256(
+    long M, long N, const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
+    const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl, const float* __restrict__ ia,
+    const float* __restrict__ ib, float* __restrict__ C, long ldc) {
+  __shared__ __attribute__((aligned(16))) _Float16 smem[2 * 4 * kBigArr];
+  const long mt = (M + kBigT - 1) / kBigT, nt = (N + kBigT - 1) / kBigT;
+  // persistent: workgroup b takes tiles b, b + G, b + 2G, ... (G = grid size, a multiple of the
+  // XCD count, so every tile of a workgroup maps to its own XCD's block of tiles)
+  // tile_of maps L to XCD L % 8's contiguous range only for L below the padded tile count
+  const long Lend = ((mt * nt + kXcds - 1) / kXcds) * kXcds;
+  long L = blockIdx.x, tm, tn;
+  if (L >= Lend || !tile_of(L, mt, nt, tm, tn)) return;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = (wave & 1) * 128, wn = (wave >> 1) * 64;
+  const int lr = lane & 15, lk = lane >> 4;
+  const char* ah0 = (const char*)Ah;
+  const char* al0 = (const char*)Al;
+  const char* bh0 = (const char*)Bh;
+  const char* bl0 = (const char*)Bl;
+  // byte offsets (from the array base, at k = 0) of the 16 bytes this lane loads per block q
+  unsigned offA[kBigQ], offB[kBigQ];
+  auto offsets = [&](long m0, long n0) {
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) {
+      const int row = wave * (kBigQ * 16) + q * 16 + (lane >> 2);
+      const int lc = (lane & 3) ^ ((row >> 2) & 3);
+      const long ga = m0 + row < M ? m0 + row : M - 1, gb = n0 + row < N ? n0 + row : N - 1;
+      offA[q] = (unsigned)(ga * kSplitPadK + lc * 8) * 2u;
+      offB[q] = (unsigned)(gb * kSplitPadK + lc * 8) * 2u;
+    }
+  };
+  auto issue = [&](int st, int k0) {
+    const unsigned kb = (unsigned)k0 * 2u;
+    _Float16* base = smem + st * 4 * kBigArr + wave * kBigQ * 512;
+#pragma unroll
+    for (int q = 0; q < kBigQ; ++q) {
+      __builtin_amdgcn_global_load_lds((const void*)(ah0 + (offA[q] + kb)),
+                                       (lds_void_t*)(base + 0 * kBigArr + q * 512), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(al0 + (offA[q] + kb)),
+                                       (lds_void_t*)(base + 1 * kBigArr + q * 512), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(bh0 + (offB[q] + kb)),
+                                       (lds_void_t*)(base + 2 * kBigArr + q * 512), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(bl0 + (offB[q] + kb)),
+                                       (lds_void_t*)(base + 3 * kBigArr + q * 512), 16, 0, 0);
+    }
+  };
+  const int off = lr * kBigK + ((lk ^ (lr >> 2)) * 8);  // this lane's fragment within a 16-row block
+  constexpr int NCH = kSplitPadK / kBigK;
+  // s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima): this wave's direct loads landed
+  constexpr int kWaitVm0 = 0x0F70;
+  const bool al16 = ((ldc & 3) == 0) && ((((size_t)C) & 15) == 0);
+  offsets(tm * kBigT, tn * kBigT);
+  issue(0, 0);
+  for (;;) {
+    const long m0 = tm * kBigT, n0 = tn * kBigT;
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    __syncthreads();
+    v4f acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0, 0, 0, 0};
+    for (int kc = 0; kc < NCH; ++kc) {
+      if (kc + 1 < NCH) issue((kc + 1) & 1, (kc + 1) * kBigK);  // the other stage, freed by the barrier
+      const _Float16* S = smem + (kc & 1) * 4 * kBigArr;
+      half8 bh[4], bl[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        bh[j] = *(const half8*)(S + 2 * kBigArr + (wn + 16 * j) * kBigK + off);
+        bl[j] = *(const half8*)(S + 3 * kBigArr + (wn + 16 * j) * kBigK + off);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const half8 ah = *(const half8*)(S + (wm + 16 * i) * kBigK + off);
+        const half8 al = *(const half8*)(S + kBigArr + (wm + 16 * i) * kBigK + off);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[j], acc[i][j], 0, 0, 0);
+      }
+      __builtin_amdgcn_s_waitcnt(kWaitVm0);
+      __syncthreads();
+    }
+    // both stages are free (barrier above): the next tile's first stage loads under this epilogue
+    L += gridDim.x;
+    long tm2 = 0, tn2 = 0;
+    const bool more = L < Lend && tile_of(L, mt, nt, tm2, tn2);
+    if (more) {
+      offsets(tm2 * kBigT, tn2 * kBigT);
+      issue(0, 0);
+    }
+    // accumulator register v of lane l in block (i, j): row m0 + wm + 16 i + 4 (l / 16) + v, column
+    // n0 + wn + 16 j + l % 16 -- four consecutive rows of one column, one 16-byte access
+    if (al16 && m0 + kBigT <= M && n0 + kBigT <= N) {
+      // whole tile: the next column block's C loads are in flight while the current one is updated
+      float4 a4[8];
+      float sbv[4];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) a4[i] = *(const float4*)(ia + m0 + wm + 16 * i + 4 * lk);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sbv[j] = ib[n0 + wn + 16 * j + lr];
+      float* cb = C + (n0 + wn + lr) * ldc + m0 + wm + 4 * lk;
+      float4 cur[8], nxt[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) cur[i] = *(const float4*)(cb + 16 * i);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j + 1 < 4) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) nxt[i] = *(const float4*)(cb + (long)(16 * (j + 1)) * ldc + 16 * i);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          float4 c = cur[i];
+          c.x -= acc[i][j][0] * a4[i].x * sbv[j];
+          c.y -= acc[i][j][1] * a4[i].y * sbv[j];
+          c.z -= acc[i][j][2] * a4[i].z * sbv[j];
+          c.w -= acc[i][j][3] * a4[i].w * sbv[j];
+          *(float4*)(cb + (long)(16 * j) * ldc + 16 * i) = c;
+        }
+        if (j + 1 < 4) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) cur[i] = nxt[i];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const long nn = n0 + wn + 16 * j + lr;
+        if (nn >= N) continue;
+        const float sb = ib[nn];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const long m = m0 + wm + 16 * i + 4 * lk;
+          float* cp = C + nn * ldc + m;
+          if (al16 && m + 3 < M) {
+            float4 c = *(float4*)cp;
+            const float4 a = *(const float4*)(ia + m);
+            c.x -= acc[i][j][0] * a.x * sb;
+            c.y -= acc[i][j][1] * a.y * sb;
+            c.z -= acc[i][j][2] * a.z * sb;
+            c.w -= acc[i][j][3] * a.w * sb;
+            *(float4*)cp = c;
+          } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              if (m + v < M) cp[v] -= acc[i][j][v] * ia[m + v] * sb;
+          }
+        }
+      }
+    }
+    if (!more) break;
+    // the offsets are recomputed rather than kept live through the epilogue (register pressure)
+    asm volatile("" : "+s"(tm2), "+s"(tn2));
+    tm = tm2;
+    tn = tn2;
+    offsets(tm * kBigT, tn * kBigT);
+  }
+}
+
 // Workspace in doubles for split operands of up to n rows / columns
 size_t f16x3_workspace_doubles(long n) {
   return ((size_t)4 * n * kSplitPadK * sizeof(_Float16) + 2 * n * sizeof(float)) / sizeof(double) + 16;
@@ -521,6 +702,18 @@ static F16x3Ws f16x3_ws(double* ws, long cap) {
   w.ib = w.ia + cap;
   return w;
 }
+// rows of A (M x K block) into w.Ah / w.Al / w.ia; columns of B (K x N block) into the split-B
+// slots [c0, c0 + N)
+static void f16x3_split_rows(long M, int K, const float* A, long lda, const F16x3Ws& w, hipStream_t st) {
+  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((M + kSplitRW - 1) / kSplitRW)), dim3(256), 0, st, M, K, A, lda,
+                     w.Ah, w.Al, w.ia);
+}
+static void f16x3_split_cols(long N, int K, const float* B, long ldb, const F16x3Ws& w, long c0,
+                             hipStream_t st) {
+  if (N <= 0) return;
+  hipLaunchKernelGGL(k_split_cols, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, N, K, B, ldb,
+                     w.Bh + c0 * kSplitPadK, w.Bl + c0 * kSplitPadK, w.ib + c0);
+}
 static void f16x3_split(long M, long N, int K, const float* A, long lda, const float* B, long ldb,
                         const F16x3Ws& w, hipStream_t st) {
   hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((M + kSplitRW - 1) / kSplitRW)), dim3(256), 0, st, M, K, A, lda,
@@ -529,8 +722,37 @@ static void f16x3_split(long M, long N, int K, const float* A, long lda, const f
                      w.Bh, w.Bl, w.ib);
 }
 // C[M x N] -= split A times the split B columns [c0, c0 + N)
+static int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+      n = 256;
+    cus = n;
+  }
+  return cus;
+}
+// Workgroups of the persistent 256 x 256 kernel: one per CU the stream may use (cus, 0 = all)
+static unsigned big_gemm_grid(long tiles, int cus) {
+  long g = (long)((cus > 0 ? cus : device_cus()) / kXcds) * kXcds;
+  const long padded = ((tiles + kXcds - 1) / kXcds) * kXcds;
+  if (g < kXcds) g = kXcds;
+  return (unsigned)(g < padded ? g : padded);
+}
+#ifndef IBEM_BIG_MIN_TILES
+#define IBEM_BIG_MIN_TILES 512
+#endif
 static void f16x3_gemm(long M, long N, int K, const F16x3Ws& w, float* C, long ldc, hipStream_t st,
-                       long c0 = 0) {
+                       long c0 = 0, int cus = 0) {
+  // 256 x 256 tiles once there are at least two per CU (one workgroup per CU), else 128 x 128
+  const long big = ((M + kBigT - 1) / kBigT) * ((N + kBigT - 1) / kBigT);
+  if (big >= IBEM_BIG_MIN_TILES) {  // operands are zero-padded to kSplitPadK: K is implied
+    hipLaunchKernelGGL(k_gemm_f16x3_256, dim3(big_gemm_grid(big, cus)),
+                       dim3(64 * kBigWaves), 0, st, M, N, w.Ah, w.Al, w.Bh + c0 * kSplitPadK,
+                       w.Bl + c0 * kSplitPadK, w.ia, w.ib + c0, C, ldc);
+    return;
+  }
   hipLaunchKernelGGL((k_gemm_f16x3<IBEM_F16_BK, IBEM_F16_EPI, IBEM_F16_PF>), dim3(tile_grid(M, N)), dim3(256), 0, st, M, N, K, w.Ah,
                      w.Al, w.Bh + c0 * kSplitPadK, w.Bl + c0 * kSplitPadK, w.ia, w.ib + c0, C, ldc);
 }
@@ -1216,18 +1438,22 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
 }
 
 constexpr long kLuNB = IBEM_LU_NB;
+#ifndef IBEM_PANEL_CUS
+#define IBEM_PANEL_CUS 32
+#endif
+constexpr int kPanelCus = IBEM_PANEL_CUS;  // 4 per XCD
 constexpr long kNarrowCols = 4096;  // trailing columns up to which a step uses the latency-bound forms
 
 // Trailing update of step k restricted to columns [c0, c0 + nc) of the trailing matrix (relative to
 // column k + kb): C -= L21 U12(:, cols).
 template <typename T>
 static void trailing_update(T* A, long n, long lda, long k, long kb, long c0, long nc,
-                            const F16x3Ws* w, hipStream_t st) {
+                            const F16x3Ws* w, hipStream_t st, int cus = 0) {
   const long m = n - k - kb;
   if (m <= 0 || nc <= 0) return;
   T* C = A + (k + kb + c0) * lda + k + kb;
   if (w)
-    f16x3_gemm(m, nc, (int)kb, *w, (float*)C, lda, st, c0);
+    f16x3_gemm(m, nc, (int)kb, *w, (float*)C, lda, st, c0, cus);
   else
     gemm_sub<T, 1>(m, nc, kb, A + k * lda + k + kb, lda, A + (k + kb + c0) * lda + k, lda, C, lda, st);
 }
@@ -1349,42 +1575,53 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
   for (long k = 0; k < n; k += kLuNB) {
     const long kb = std::min(kLuNB, n - k);
     if (k > 0) hipStreamWaitEvent(st, W.ev_b, 0);  // panel k, factored on the second stream
-    if (tick) tick(tctx, "lu.swap_trsm", 0, st);
-    // the interchanges of this step go to the trailing columns only; L's earlier block columns
-    // keep their order (the solve applies the interchanges to the vector instead)
-    if (k + kb < n) {
-      // few trailing columns (the last steps, on the critical path): the move-list interchanges and
-      // the wavefront TRSM, whose latency does not grow with the 1024 sequential interchanges or
-      // the block-inverse chain; many columns: the bandwidth-bound forms
-      const long nt = n - k - kb;
-      if (nt <= kNarrowCols)
-        laswp_narrow<T>(A, lda, k + kb, nt, W.d_ipiv, k, k + kb, st);
-      else
-        laswp<T>(A, lda, k + kb, nt, W.d_ipiv, k, k + kb, st);
-      if (sizeof(T) == 4 && nt > kNarrowCols)
-        trsm_by_inverse<T>(A + k * lda + k, lda, kb, A + (k + kb) * lda + k, lda, nt, W, st);
-      else
-        trsm_lower_unit<T>(A + k * lda + k, lda, kb, A + (k + kb) * lda + k, lda, nt, st);
-    }
-    if (tick) tick(tctx, "lu.swap_trsm", 1, st);
     if (k + kb >= n) break;
-    const long m = n - k - kb;
+    const long m = n - k - kb;            // trailing rows and columns
     const long kb2 = std::min(kLuNB, m);  // width of the next panel
+    const T* L11 = A + k * lda + k;
+    // (1) the critical path: the next panel's columns get this step's interchanges, their U12 rows
+    // and their trailing update, in the latency-bound forms. The interchanges go to the trailing
+    // columns only; L's earlier block columns keep their order (the solve applies the
+    // interchanges to the vector instead).
+    if (tick) tick(tctx, "lu.swap_trsm", 0, st);
+    laswp_narrow<T>(A, lda, k + kb, kb2, W.d_ipiv, k, k + kb, st);
+    trsm_lower_unit<T>(L11, lda, kb, A + (k + kb) * lda + k, lda, kb2, st);
+    if (tick) tick(tctx, "lu.swap_trsm", 1, st);
     if (w) {
       if (tick) tick(tctx, "lu.split", 0, st);
-      f16x3_split(m, m, (int)kb, (const float*)(A + k * lda + k + kb), lda,
-                  (const float*)(A + (k + kb) * lda + k), lda, *w, st);
+      f16x3_split_rows(m, (int)kb, (const float*)(A + k * lda + k + kb), lda, *w, st);
+      f16x3_split_cols(kb2, (int)kb, (const float*)(A + (k + kb) * lda + k), lda, *w, 0, st);
       if (tick) tick(tctx, "lu.split", 1, st);
     }
     if (tick) tick(tctx, "lu.gemm", 0, st);
     trailing_update<T>(A, n, lda, k, kb, 0, kb2, w, st);
     if (tick) tick(tctx, "lu.gemm", 1, st);
     hipEventRecord(W.ev_a, st);
-    // the rest of the update is enqueued before the panel's many small launches, so that the host's
-    // time spent launching them never leaves the main stream waiting
-    if (tick) tick(tctx, "lu.gemm", 0, st);
-    trailing_update<T>(A, n, lda, k, kb, kb2, m - kb2, w, st);
-    if (tick) tick(tctx, "lu.gemm", 1, st);
+    // (2) the remaining columns, in the bandwidth-bound forms, while the panel runs; enqueued
+    // before the panel's many small launches, so that the host's time spent launching them never
+    // leaves the main stream waiting. The bulk update leaves kPanelCus CUs idle for the panel.
+    const long nr = m - kb2;
+    if (nr > 0) {
+      T* B2 = A + (k + kb + kb2) * lda + k;
+      if (tick) tick(tctx, "lu.swap_trsm_rest", 0, st);
+      if (nr <= kNarrowCols)
+        laswp_narrow<T>(A, lda, k + kb + kb2, nr, W.d_ipiv, k, k + kb, st);
+      else
+        laswp<T>(A, lda, k + kb + kb2, nr, W.d_ipiv, k, k + kb, st);
+      if (sizeof(T) == 4 && nr > kNarrowCols)
+        trsm_by_inverse<T>(L11, lda, kb, B2, lda, nr, W, st);
+      else
+        trsm_lower_unit<T>(L11, lda, kb, B2, lda, nr, st);
+      if (tick) tick(tctx, "lu.swap_trsm_rest", 1, st);
+      if (w) {
+        if (tick) tick(tctx, "lu.split", 0, st);
+        f16x3_split_cols(nr, (int)kb, (const float*)B2, lda, *w, kb2, st);
+        if (tick) tick(tctx, "lu.split", 1, st);
+      }
+      if (tick) tick(tctx, "lu.gemm", 0, st);
+      trailing_update<T>(A, n, lda, k, kb, kb2, nr, w, st, device_cus() - kPanelCus);
+      if (tick) tick(tctx, "lu.gemm", 1, st);
+    }
     hipStreamWaitEvent(W.st2, W.ev_a, 0);
     if (tick) tick(tctx, "lu.panel", 0, W.st2);
     panel_rec(c2, k + kb, kb2);
