@@ -1438,10 +1438,17 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
 }
 
 constexpr long kLuNB = IBEM_LU_NB;
-#ifndef IBEM_PANEL_CUS
-#define IBEM_PANEL_CUS 32
-#endif
-constexpr int kPanelCus = IBEM_PANEL_CUS;  // 4 per XCD
+// CUs the bulk trailing update leaves idle for the concurrent panel, by trailing size m. The panel
+// (latency-bound, cost ~ a + b m) and the bulk update (~ m^2) overlap; under a full-GPU update the
+// panel's own kernels (tree, internal FP32 updates) run 1.5-3x slower than alone. Early steps are
+// update-bound and keep most CUs on the update; later steps are panel-bound and give it more.
+// Breakpoints from a fit of measured panel / update times on a 64,614-unknown system.
+static int panel_cus(long m) {
+  if (m > 41000) return 32;
+  if (m > 29000) return 64;
+  if (m > 20000) return 96;
+  return 128;
+}
 constexpr long kNarrowCols = 4096;  // trailing columns up to which a step uses the latency-bound forms
 
 // Trailing update of step k restricted to columns [c0, c0 + nc) of the trailing matrix (relative to
@@ -1599,7 +1606,7 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
     hipEventRecord(W.ev_a, st);
     // (2) the remaining columns, in the bandwidth-bound forms, while the panel runs; enqueued
     // before the panel's many small launches, so that the host's time spent launching them never
-    // leaves the main stream waiting. The bulk update leaves kPanelCus CUs idle for the panel.
+    // leaves the main stream waiting. The bulk update leaves panel_cus(m) CUs idle for the panel.
     const long nr = m - kb2;
     if (nr > 0) {
       T* B2 = A + (k + kb + kb2) * lda + k;
@@ -1619,7 +1626,7 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
         if (tick) tick(tctx, "lu.split", 1, st);
       }
       if (tick) tick(tctx, "lu.gemm", 0, st);
-      trailing_update<T>(A, n, lda, k, kb, kb2, nr, w, st, device_cus() - kPanelCus);
+      trailing_update<T>(A, n, lda, k, kb, kb2, nr, w, st, device_cus() - panel_cus(m));
       if (tick) tick(tctx, "lu.gemm", 1, st);
     }
     hipStreamWaitEvent(W.st2, W.ev_a, 0);
