@@ -824,20 +824,20 @@ __device__ inline unsigned ts_key(float v) { return (__float_as_uint(fabsf(v)) >
 __device__ inline unsigned ts_key(double v) {
   return (unsigned)((unsigned long long)__double_as_longlong(fabs(v)) >> 32) + 1u;
 }
-// The winning lane's row into its wave's LDS slot, one element per store instruction. Written with
-// the compiler's own stores, the row is gathered into aligned 16-byte register tuples, which ties
-// a[] to a tuple and makes every later partial update of it copy the whole array.
-template <typename T>
-__device__ __forceinline__ void ts_store_row(T* dst, const T (&a)[kPanelW]) {
+// Columns [c, kPanelW) of this thread's row into its LDS slot, one element per store instruction.
+// Written with the compiler's own stores, the row is gathered into aligned 16-byte register tuples,
+// which ties a[] to a tuple and makes every later partial update of it copy the whole array. The
+// caller waits for the stores (lgkmcnt) before its barrier.
+template <typename T, int C>
+__device__ __forceinline__ void ts_store_tail(T* dst, const T (&a)[kPanelW]) {
   const unsigned base = (unsigned)(size_t)dst;
 #pragma unroll
-  for (int k = 0; k < kPanelW; ++k) {
+  for (int k = C; k < kPanelW; ++k) {
     if constexpr (sizeof(T) == 4)
       asm volatile("ds_write_b32 %0, %1 offset:%2" ::"v"(base), "v"(a[k]), "i"(4 * k) : "memory");
     else
       asm volatile("ds_write_b64 %0, %1 offset:%2" ::"v"(base), "v"(a[k]), "i"(8 * k) : "memory");
   }
-  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 }
 template <int CTRL>
 __device__ inline unsigned dpp_max_u32(unsigned k) {
@@ -860,59 +860,68 @@ __device__ inline unsigned wave_max_u32(unsigned k) {
 // Partial-pivoting elimination of the workgroup's rows (thread = row), choosing up to w pivot rows.
 // a: the row's values, updated in place (multipliers left of the pivot column, reduced values right
 // of it); mystep: the step at which this row was chosen, or -1. Returns the number of steps that
-// found a row. Per step: a wave maximum of the pivot keys, whose lowest winning lane publishes its
-// row in the wave's LDS slot; one barrier; every thread takes the best of the node's waves and
-// eliminates with that slot's row. Buffers alternate by step parity, so one barrier per step
-// suffices. The update is branch-free: rows that do not take part use a zero multiplier (all values
-// are finite), which leaves them unchanged.
+// found a row. Per step: every unchosen row stores the columns the step can need in its own LDS
+// slot (independent of the search, so the stores overlap it); a wave maximum of the pivot keys,
+// packed with the lowest winning row index into one 64-bit word per wave; one barrier; every
+// thread takes the largest of the four words (one LDS round trip) and eliminates with the winner's
+// slot. The words alternate by step parity, so one barrier per step suffices. The update is
+// branch-free: rows that do not take part use a zero multiplier (all values are finite), which
+// leaves them unchanged.
+constexpr int kTsStride = kPanelW + 1;  // LDS row slot stride (elements): consecutive rows in distinct banks
+template <typename T, int C>
+__device__ __forceinline__ void ts_step(T (&a)[kPanelW], bool valid, int w, unsigned long long* skey,
+                                        T* sp, int& mystep, int& nsel) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  constexpr int par = C & 1;
+  const bool cand = valid && mystep < 0 && C < w;  // steps past w find no candidate
+  // one slot per row: a chosen row stops writing, so the slot a step reads after its barrier (the
+  // winner's) is never overwritten while other threads still read it
+  if (valid && mystep < 0) ts_store_tail<T, C>(sp + t * kTsStride, a);
+  const unsigned key = cand ? ts_key(a[C]) : 0u;
+  const unsigned wmax = wave_max_u32(key);
+  const unsigned long long hit = __ballot(key == wmax);
+  const int wl = __ffsll((long long)hit) - 1;
+  // larger key wins; equal keys go to the lower row index (lowest lane, then lowest wave)
+  if (lane == 0) skey[par * kTsWaves + wv] = ((unsigned long long)wmax << 32) | (unsigned)(kTsRows - 1 - (wv * 64 + wl));
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __syncthreads();
+  unsigned long long best = skey[par * kTsWaves];
+#pragma unroll
+  for (int q = 1; q < kTsWaves; ++q) {
+    const unsigned long long k2 = skey[par * kTsWaves + q];
+    best = k2 > best ? k2 : best;
+  }
+  // no branch on `found`: a column without candidates gives l = 0 everywhere (the slot read then
+  // holds some row's finite values)
+  const bool found = (best >> 32) != 0ull;
+  const int win = kTsRows - 1 - (int)(best & 0xffffffffull);
+  nsel = found ? C + 1 : nsel;
+  if (found && t == win) mystep = C;
+  const T* prow = sp + win * kTsStride;
+  T pr[kPanelW];
+#pragma unroll
+  for (int k = C; k < kPanelW; ++k) pr[k] = prow[k];
+  const bool act = found && valid && mystep < 0;
+  const T piv = pr[C];
+  const T l = (act && piv != T(0)) ? a[C] / piv : T(0);
+  a[C] = act ? l : a[C];
+#pragma unroll
+  for (int k = C + 1; k < kPanelW; ++k) a[k] = fma(-l, pr[k], a[k]);
+}
+template <typename T, int C = 0>
+__device__ __forceinline__ void ts_steps(T (&a)[kPanelW], bool valid, int w, unsigned long long* skey,
+                                         T* sp, int& mystep, int& nsel) {
+  if constexpr (C < kPanelW) {
+    ts_step<T, C>(a, valid, w, skey, sp, mystep, nsel);
+    ts_steps<T, C + 1>(a, valid, w, skey, sp, mystep, nsel);
+  }
+}
 template <typename T>
-__device__ __forceinline__ int ts_select(T (&a)[kPanelW], bool valid, int w, unsigned* skey,
-                                         int* swin, T* sp, int& mystep) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+__device__ __forceinline__ int ts_select(T (&a)[kPanelW], bool valid, int w, unsigned long long* skey,
+                                         T* sp, int& mystep) {
   mystep = -1;
   int nsel = 0;
-#pragma unroll
-  for (int c = 0; c < kPanelW; ++c) {
-    {
-      const int par = c & 1;
-      const bool cand = valid && mystep < 0 && c < w;  // steps past w find no candidate
-      const unsigned key = cand ? ts_key(a[c]) : 0u;
-      const unsigned wmax = wave_max_u32(key);
-      const unsigned long long hit = __ballot(key == wmax);
-      const int wl = __ffsll((long long)hit) - 1;
-      if (lane == wl) ts_store_row<T>(sp + (par * kTsWaves + wv) * kPanelW, a);
-      if (lane == 0) {
-        skey[par * kTsWaves + wv] = wmax;
-        swin[par * kTsWaves + wv] = wv * 64 + wl;
-      }
-      __syncthreads();
-      unsigned best = skey[par * kTsWaves];
-      int bw = 0;
-#pragma unroll
-      for (int q = 1; q < kTsRows / 64; ++q) {
-        const unsigned k2 = skey[par * kTsWaves + q];
-        if (k2 > best) {
-          best = k2;
-          bw = q;
-        }
-      }
-      // no branch on `found`: a column without candidates gives l = 0 everywhere (the slot read
-      // then holds some row's finite values)
-      const bool found = best != 0u;
-      nsel = found ? c + 1 : nsel;
-      if (found && (int)threadIdx.x == swin[par * kTsWaves + bw]) mystep = c;
-      const T* prow = sp + (par * kTsWaves + bw) * kPanelW;
-      T pr[kPanelW];
-#pragma unroll
-      for (int k = c; k < kPanelW; ++k) pr[k] = prow[k];
-      const bool act = found && valid && mystep < 0;
-      const T piv = pr[c];
-      const T l = (act && piv != T(0)) ? a[c] / piv : T(0);
-      a[c] = act ? l : a[c];
-#pragma unroll
-      for (int k = c + 1; k < kPanelW; ++k) a[k] = fma(-l, pr[k], a[k]);
-    }
-  }
+  ts_steps<T>(a, valid, w, skey, sp, mystep, nsel);
   return nsel;
 }
 
@@ -937,9 +946,8 @@ template <typename T>
 __global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, long lda, long n,
                                                      long j0, int w, int P, TsWs ws,
                                                      int* __restrict__ ipiv, int* __restrict__ info) {
-  __shared__ unsigned skey[2 * kTsWaves];
-  __shared__ int swin[2 * kTsWaves];
-  __shared__ __align__(16) T sp[2 * kTsWaves * kPanelW];
+  __shared__ __align__(16) unsigned long long skey[2 * kTsWaves];
+  __shared__ __align__(16) T sp[kTsRows * kTsStride];
   __shared__ int s_last;
   __shared__ int s_win[kPanelW];
   __shared__ double s_diag[kPanelW];
@@ -959,7 +967,7 @@ __global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, lo
   long src = -1;
   int mystep = -1, nsel = 0;
   for (;;) {
-    nsel = ts_select<T>(a, valid, w, skey, swin, sp, mystep);
+    nsel = ts_select<T>(a, valid, w, skey, sp, mystep);
     if (count == 1) break;  // this workgroup holds the root set
     const long slot = off + node;
     if (mystep >= 0) {
