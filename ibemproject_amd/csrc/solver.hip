@@ -874,15 +874,15 @@ __device__ __forceinline__ void ts_step(T (&a)[kPanelW], bool valid, int w, unsi
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   constexpr int par = C & 1;
   const bool cand = valid && mystep < 0 && C < w;  // steps past w find no candidate
-  // one slot per row: a chosen row stops writing, so the slot a step reads after its barrier (the
-  // winner's) is never overwritten while other threads still read it
-  if (valid && mystep < 0) ts_store_tail<T, C>(sp + t * kTsStride, a);
   const unsigned key = cand ? ts_key(a[C]) : 0u;
   const unsigned wmax = wave_max_u32(key);
   const unsigned long long hit = __ballot(key == wmax);
   const int wl = __ffsll((long long)hit) - 1;
   // larger key wins; equal keys go to the lower row index (lowest lane, then lowest wave)
   if (lane == 0) skey[par * kTsWaves + wv] = ((unsigned long long)wmax << 32) | (unsigned)(kTsRows - 1 - (wv * 64 + wl));
+  // one slot per row: a chosen row stops writing, so the slot a step reads after its barrier (the
+  // winner's) is never overwritten while other threads still read it
+  if (valid && mystep < 0) ts_store_tail<T, C>(sp + t * kTsStride, a);
   asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
   __syncthreads();
   unsigned long long best = skey[par * kTsWaves];
