@@ -31,16 +31,18 @@ from ibemproject_amd.generate import make_workload  # noqa: E402
 MI355X_PEAK = {"f16_mfma_tflops": 2500.0, "f32_mfma_tflops": 157.3, "f64_mfma_tflops": 78.6,
                "hbm_gbs": 8000.0}
 NB = 1024  # LU block size (solver.hip IBEM_LU_NB)
-# HBM bytes per k_gemm_f16x3 launch on rve1000_m16 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch,
-# averaged over one LU's launches; the factors make the counters read 1 GiB for a 1 GiB copy), from
-# the counter runs recorded in profiles/k_gemm_f16x3__rve1000_m16.md
-GEMM_TRAFFIC_BYTES_PER_LAUNCH = 1.4354e10
+# HBM bytes per trailing-update launch on rve1000_m16 (FETCH_SIZE x 2.667 + WRITE_SIZE x 1.0 per
+# dispatch, averaged over one LU's 125 launches of k_gemm_f16x3_256 / k_gemm_f16x3; the factors make
+# the counters read 1 GiB for a 1 GiB copy), from the counter runs recorded in
+# profiles/k_gemm_f16x3_256__rve1000_m16.md
+GEMM_TRAFFIC_BYTES_PER_LAUNCH = 1.6204e10
 
 
 def trailing_update_flops(n: int) -> tuple[float, int]:
     """Algorithmic flops of the blocked LU's trailing-matrix updates (2 m^2 kb per step, m the
-    trailing size) and the number of k_gemm_f16x3 launches: two per step (the next panel's columns,
-    then the rest), one when the next panel is the whole trailing matrix."""
+    trailing size) and the number of split-GEMM launches (k_gemm_f16x3_256, or k_gemm_f16x3 for
+    small updates): two per step (the next panel's columns, then the rest), one when the next panel
+    is the whole trailing matrix."""
     flops = 0.0
     launches = 0
     for k in range(0, n, NB):
@@ -53,7 +55,7 @@ def trailing_update_flops(n: int) -> tuple[float, int]:
 
 
 def trailing_update_bytes(n: int) -> float:
-    """Algorithmic bytes per k_gemm_f16x3 launch, averaged over one LU: the split operands (hi + lo,
+    """Algorithmic bytes per split-GEMM launch, averaged over one LU: the split operands (hi + lo,
     2 bytes each) read once, C read and written once in FP32 (4 m kb + 4 kb nc + 8 m nc)."""
     total, launches = 0.0, 0
     for k in range(0, n, NB):
@@ -266,8 +268,9 @@ def main():
         "fp64_lu_step": fp64,
         "fp32_mixed_step": fp32,
         "speedup_vs_fp64_lu": (fp64["step_s"] / step_s) if fp64 else None,
-        "roofline": {"kernel": "k_gemm_f16x3 (LU trailing update, v_mfma_f32_16x16x32_f16, three "
-                               "products per block)",
+        "roofline": {"kernel": "k_gemm_f16x3_256 (LU trailing update: persistent 256x256 tiles, "
+                               "direct global-to-LDS loads, v_mfma_f32_16x16x32_f16, three products "
+                               "per block; k_gemm_f16x3 128x128 for updates under 512 tiles)",
                      "bound": "mfma", "achieved": achieved,
                      "peak": MI355X_PEAK["f16_mfma_tflops"], "unit": "TFLOP/s",
                      "frac": achieved / MI355X_PEAK["f16_mfma_tflops"] if achieved else None,
@@ -280,8 +283,9 @@ def main():
                      "flops_per_step": flops,
                      "mfma_issued_tflops": 3 * achieved if achieved else None,
                      "note": f"achieved = algorithmic trailing-update flops (2 m^2 {NB} per LU "
-                             "step) / the k_gemm_f16x3 time of the step; the kernel issues three "
-                             "fp16 MFMA products per algorithmic one"},
+                             "step) / the split-GEMM time of the step (both kernels, HIP events "
+                             "around each launch); the kernels issue three fp16 MFMA products per "
+                             "algorithmic one"},
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)
