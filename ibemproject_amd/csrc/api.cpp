@@ -664,7 +664,7 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
       {
         Scope s(ctx, "solve.lu_f32");
         if (mode == IBEM3D_SOLVE_MIXED_FP16 && !ctx->d_f16ws)
-          if (int rc = dalloc(ctx, &ctx->d_f16ws, f16x3_workspace_doubles(n))) return rc;
+          if (int rc = dalloc(ctx, &ctx->d_f16ws, lu_f16_workspace_doubles(n))) return rc;
         if (lu_factor_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->st, tick, ctx,
                           mode == IBEM3D_SOLVE_MIXED_FP16 ? ctx->d_f16ws : nullptr))
           return fail(ctx, IBEM3D_EHIP, "LU launch failed");

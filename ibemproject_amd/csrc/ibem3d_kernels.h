@@ -77,8 +77,9 @@ size_t lu_moves_ints(long n);
 size_t lu_trinv_doubles();
 size_t lu_trsm_doubles(long n);
 int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx);
-// f16ws != nullptr: the trailing updates run as gemm_f32_sub_f16x3 (workspace of
-// f16x3_workspace_doubles(n) doubles); otherwise on FP32 MFMA
+// f16ws != nullptr: the trailing updates and the panels' deep updates run on split FP16 products
+// (workspace of lu_f16_workspace_doubles(n) doubles); otherwise on FP32 MFMA
+size_t lu_f16_workspace_doubles(long n);
 int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx, double* f16ws);
 // After a factorisation: each step's interchanges as a move list and the diagonal-block inverses
 // (synchronises the stream once). Returns nonzero on a HIP error.

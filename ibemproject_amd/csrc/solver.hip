@@ -245,9 +245,11 @@ __device__ inline void split2(float x, _Float16& hi, _Float16& lo) {
 }
 
 // A operand: rows r < M of the column-major M x K block at A (lda). Output row-major [M][K] hi / lo
-// (k contiguous) and the row's inverse scale. Workgroup = kSplitRW rows; LDS transpose.
+// (k contiguous, row stride kSplitPadK) and the row's inverse scale; k in [K, kpad) written as zeros,
+// k >= kpad not written (kpad: a multiple of 64, the depth the GEMM reads). Workgroup = kSplitRW
+// rows; LDS transpose.
 constexpr int kSplitRW = 64 * 256 / kSplitK;  // rows per workgroup (the tile stays at 64 KB)
-__global__ __launch_bounds__(256) void k_split_rows(long M, int K, const float* __restrict__ A,
+__global__ __launch_bounds__(256) void k_split_rows(long M, int K, int kpad, const float* __restrict__ A,
                                                     long lda, _Float16* __restrict__ hi,
                                                     _Float16* __restrict__ lo,
                                                     float* __restrict__ inv_scale) {
@@ -275,8 +277,9 @@ __global__ __launch_bounds__(256) void k_split_rows(long M, int K, const float* 
   }
   __syncthreads();
   // write: thread handles 8 consecutive k of one row per pass (16-byte stores)
-  for (int u = t; u < kSplitRW * (kSplitPadK / 8); u += 256) {
-    const int rl = u / (kSplitPadK / 8), k8 = (u % (kSplitPadK / 8)) * 8;
+  const int kp8 = kpad / 8;
+  for (int u = t; u < kSplitRW * kp8; u += 256) {
+    const int rl = u / kp8, k8 = (u % kp8) * 8;
     const long rr = r0 + rl;
     if (rr >= M) continue;
     half8 h, l;
@@ -294,8 +297,9 @@ __global__ __launch_bounds__(256) void k_split_rows(long M, int K, const float* 
 }
 
 // B operand: columns c < N of the column-major K x N block at B (ldb), output [N][K] hi / lo (k
-// contiguous) and the column's inverse scale. One wavefront per column.
-__global__ __launch_bounds__(256) void k_split_cols(long N, int K, const float* __restrict__ B,
+// contiguous, row stride kSplitPadK; written for k < kpad) and the column's inverse scale. One
+// wavefront per column.
+__global__ __launch_bounds__(256) void k_split_cols(long N, int K, int kpad, const float* __restrict__ B,
                                                     long ldb, _Float16* __restrict__ hi,
                                                     _Float16* __restrict__ lo,
                                                     float* __restrict__ inv_scale) {
@@ -317,6 +321,7 @@ __global__ __launch_bounds__(256) void k_split_cols(long N, int K, const float* 
   const float sc = pow2_scale(m);
   if (lane == 0) inv_scale[c] = 1.0f / sc;
   typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+  if (lane * VPL >= kpad) return;
 #pragma unroll
   for (int q4 = 0; q4 < VPL; q4 += 4) {
     half4 h, l;
@@ -353,7 +358,7 @@ __global__ __launch_bounds__(256) void k_split_cols(long N, int K, const float* 
 #ifndef IBEM_F16_PAD
 #define IBEM_F16_PAD 8
 #endif
-template <int BK, int EPI, int PF>
+template <int BK, int EPI, int PF, int NCH = kSplitPadK / BK>
 __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
                                                        const _Float16* __restrict__ Ah,
                                                        const _Float16* __restrict__ Al,
@@ -362,11 +367,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
                                                        const float* __restrict__ ia,
                                                        const float* __restrict__ ib,
                                                        float* __restrict__ C, long ldc) {
-  (void)K;  // operands are zero-padded to kSplitPadK, so every tile runs all NCH chunks
+  (void)K;  // operands are zero-padded to NCH * BK, so every tile runs all NCH chunks
   constexpr int LS = BK + IBEM_F16_PAD;  // LDS row stride (halves)
   constexpr int VPR = BK / 8;            // 16-byte vectors per row and array
   constexpr int NV = kBM * VPR / 256;    // vectors per thread and array
-  constexpr int NCH = kSplitPadK / BK;   // K chunks
   constexpr int CS = kBM + 4;            // epilogue tile column stride (floats)
   constexpr int SMEM = (EPI && kBN * CS * 2 > 4 * kBM * LS) ? kBN * CS * 2 : 4 * kBM * LS;
   __shared__ __attribute__((aligned(16))) _Float16 smem[SMEM];
@@ -686,6 +690,8 @@ __global__ __launch_bounds__(64 * kBigWaves, 1) void k_gemm_f16x3_256(
 size_t f16x3_workspace_doubles(long n) {
   return ((size_t)4 * n * kSplitPadK * sizeof(_Float16) + 2 * n * sizeof(float)) / sizeof(double) + 16;
 }
+// The FP16-mode LU's workspace: the bulk trailing updates' split operands and the panel's
+size_t lu_f16_workspace_doubles(long n) { return 2 * f16x3_workspace_doubles(n); }
 
 // Split operands in the workspace (capacity cap rows / columns)
 struct F16x3Ws {
@@ -704,22 +710,21 @@ static F16x3Ws f16x3_ws(double* ws, long cap) {
 }
 // rows of A (M x K block) into w.Ah / w.Al / w.ia; columns of B (K x N block) into the split-B
 // slots [c0, c0 + N)
-static void f16x3_split_rows(long M, int K, const float* A, long lda, const F16x3Ws& w, hipStream_t st) {
-  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((M + kSplitRW - 1) / kSplitRW)), dim3(256), 0, st, M, K, A, lda,
-                     w.Ah, w.Al, w.ia);
+static void f16x3_split_rows(long M, int K, const float* A, long lda, const F16x3Ws& w, hipStream_t st,
+                             int kpad = kSplitPadK) {
+  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((M + kSplitRW - 1) / kSplitRW)), dim3(256), 0, st, M, K, kpad,
+                     A, lda, w.Ah, w.Al, w.ia);
 }
 static void f16x3_split_cols(long N, int K, const float* B, long ldb, const F16x3Ws& w, long c0,
-                             hipStream_t st) {
+                             hipStream_t st, int kpad = kSplitPadK) {
   if (N <= 0) return;
-  hipLaunchKernelGGL(k_split_cols, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, N, K, B, ldb,
+  hipLaunchKernelGGL(k_split_cols, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, N, K, kpad, B, ldb,
                      w.Bh + c0 * kSplitPadK, w.Bl + c0 * kSplitPadK, w.ib + c0);
 }
 static void f16x3_split(long M, long N, int K, const float* A, long lda, const float* B, long ldb,
                         const F16x3Ws& w, hipStream_t st) {
-  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((M + kSplitRW - 1) / kSplitRW)), dim3(256), 0, st, M, K, A, lda,
-                     w.Ah, w.Al, w.ia);
-  hipLaunchKernelGGL(k_split_cols, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, N, K, B, ldb,
-                     w.Bh, w.Bl, w.ib);
+  f16x3_split_rows(M, K, A, lda, w, st);
+  f16x3_split_cols(N, K, B, ldb, w, 0, st);
 }
 // C[M x N] -= split A times the split B columns [c0, c0 + N)
 static int device_cus() {
@@ -757,7 +762,17 @@ static void f16x3_gemm(long M, long N, int K, const F16x3Ws& w, float* C, long l
                      w.Al, w.Bh + c0 * kSplitPadK, w.Bl + c0 * kSplitPadK, w.ia, w.ib + c0, C, ldc);
 }
 
-// C[M x N] -= A[M x K] B[K x N] (column-major FP32, K <= 256) through the split FP16 products.
+// C[M x N] -= split A times split B for a depth K <= 512 (operands split with kpad = K rounded up to
+// 64): the 128 x 128 kernel reads only that depth
+static void f16x3_gemm_short(long M, long N, int K, const F16x3Ws& w, float* C, long ldc, hipStream_t st) {
+  const dim3 g(tile_grid(M, N)), b(256);
+  if (K <= 256)
+    hipLaunchKernelGGL((k_gemm_f16x3<64, 1, 1, 4>), g, b, 0, st, M, N, K, w.Ah, w.Al, w.Bh, w.Bl, w.ia, w.ib, C, ldc);
+  else
+    hipLaunchKernelGGL((k_gemm_f16x3<64, 1, 1, 8>), g, b, 0, st, M, N, K, w.Ah, w.Al, w.Bh, w.Bl, w.ia, w.ib, C, ldc);
+}
+
+// C[M x N] -= A[M x K] B[K x N] (column-major FP32, K <= 1024) through the split FP16 products.
 void gemm_f32_sub_f16x3(long M, long N, int K, const float* A, long lda, const float* B, long ldb,
                         float* C, long ldc, double* ws, hipStream_t st) {
   if (M <= 0 || N <= 0 || K <= 0) return;
@@ -1415,6 +1430,7 @@ struct LuCtx {
   LuTick tick;
   void* tctx;
   int err;
+  const F16x3Ws* pw;  // split operands of the panel's own updates (FP16 mode), else null
 };
 
 template <typename T>
@@ -1427,6 +1443,8 @@ static void panel_leaf(LuCtx<T>& c, long j0, int w) {
   hipLaunchKernelGGL(k_ts_apply<T>, dim3(P), dim3(kTsRows), 0, c.st, c.A, c.lda, c.n, j0, w, ws);
 }
 
+constexpr long kPanelSplitK = 256;      // panel updates at least this deep use the split products
+constexpr long kPanelSplitRows = 2048;  // ... when at least this tall
 template <typename T>
 static void panel_rec(LuCtx<T>& c, long j0, long w) {
   if (c.err) return;
@@ -1439,8 +1457,18 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
   panel_rec(c, j0, w1);
   laswp_narrow<T>(c.A, lda, j0 + w1, w - w1, c.W->d_ipiv, j0, j0 + w1, c.st);
   trsm_lower_unit<T>(c.A + j0 * lda + j0, lda, w1, c.A + (j0 + w1) * lda + j0, lda, w - w1, c.st);
-  gemm_sub<T>(c.n - j0 - w1, w - w1, w1, c.A + j0 * lda + j0 + w1, lda, c.A + (j0 + w1) * lda + j0,
-              lda, c.A + (j0 + w1) * lda + j0 + w1, lda, c.st);
+  const long mr = c.n - j0 - w1;
+  if (sizeof(T) == 4 && c.pw && w1 >= kPanelSplitK && mr >= kPanelSplitRows) {
+    // the deep updates of the panel (three quarters of its flops) on the split FP16 products: the
+    // panel gets few CUs while the bulk update runs, and these do 5 x the work per CU of FP32 MFMA
+    const int kp = (int)((w1 + 63) / 64 * 64);
+    f16x3_split_rows(mr, (int)w1, (const float*)(c.A + j0 * lda + j0 + w1), lda, *c.pw, c.st, kp);
+    f16x3_split_cols(w - w1, (int)w1, (const float*)(c.A + (j0 + w1) * lda + j0), lda, *c.pw, 0, c.st, kp);
+    f16x3_gemm_short(mr, w - w1, (int)w1, *c.pw, (float*)(c.A + (j0 + w1) * lda + j0 + w1), lda, c.st);
+  } else {
+    gemm_sub<T>(mr, w - w1, w1, c.A + j0 * lda + j0 + w1, lda, c.A + (j0 + w1) * lda + j0, lda,
+                c.A + (j0 + w1) * lda + j0 + w1, lda, c.st);
+  }
   panel_rec(c, j0 + w1, w - w1);
   laswp_narrow<T>(c.A, lda, j0, w1, c.W->d_ipiv, j0 + w1, j0 + w, c.st);
 }
@@ -1579,8 +1607,11 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
     if (hipEventCreateWithFlags(&W.ev_a, hipEventDisableTiming) != hipSuccess) return -1;
     if (hipEventCreateWithFlags(&W.ev_b, hipEventDisableTiming) != hipSuccess) return -1;
   }
-  LuCtx<T> c{A, n, lda, &W, st, tick, tctx, 0};
-  LuCtx<T> c2{A, n, lda, &W, W.st2, tick, tctx, 0};
+  // f16ws holds two split workspaces: the bulk trailing updates' and the panel's own
+  const F16x3Ws wspanel = f16ws ? f16x3_ws(f16ws + f16x3_workspace_doubles(n), n) : F16x3Ws{};
+  const F16x3Ws* pw = f16ws ? &wspanel : nullptr;
+  LuCtx<T> c{A, n, lda, &W, st, tick, tctx, 0, pw};
+  LuCtx<T> c2{A, n, lda, &W, W.st2, tick, tctx, 0, pw};
   hipMemsetAsync(W.d_info, 0, sizeof(int), st);
   const F16x3Ws wsp = f16ws ? f16x3_ws(f16ws, n) : F16x3Ws{};
   const F16x3Ws* w = f16ws ? &wsp : nullptr;
