@@ -206,6 +206,49 @@ static void gemm_sub(long M, long N, long K, const T* A, long lda, const T* B, l
   hipLaunchKernelGGL((k_gemm_sub<T, Role>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
 }
 
+// C[M x N] -= A[M x K] B[K x N] for the panel's narrowest updates (K = 32 or 64, N <= 64, M tall):
+// one thread per row of C holds its row of A in registers, B is staged in LDS (read as broadcasts),
+// four columns of C per pass. Light workgroups (no MFMA tiles) so that they start quickly on the
+// few CUs the panel gets while the bulk update runs.
+constexpr int kSkinnyMax = 64;
+template <typename T, int K>
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 2) void k_gemm_skinny(
+    long M, int N, const T* __restrict__ A, long lda, const T* __restrict__ B, long ldb,
+    T* __restrict__ C, long ldc) {
+  __shared__ __align__(16) T sB[kSkinnyMax][K];  // sB[j][k] = B(k, j)
+  const int t = threadIdx.x;
+  for (int u = t; u < K * N; u += 256) {
+    const int k = u % K, j = u / K;
+    sB[j][k] = B[(long)j * ldb + k];
+  }
+  __syncthreads();
+  const long r = (long)blockIdx.x * 256 + t;
+  if (r >= M) return;
+  T a[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) a[k] = A[(long)k * lda + r];
+#pragma unroll 1
+  for (int j = 0; j < N; ++j) {
+    const T* b = sB[j];
+    T p[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int k = 0; k < K; k += 4)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) p[q] = fma(a[k + q], b[k + q], p[q]);
+    C[(long)j * ldc + r] -= (p[0] + p[1]) + (p[2] + p[3]);
+  }
+}
+template <typename T>
+static void gemm_skinny(long M, long N, long K, const T* A, long lda, const T* B, long ldb, T* C,
+                        long ldc, hipStream_t st) {
+  if (M <= 0 || N <= 0) return;
+  const dim3 g((unsigned)((M + 255) / 256));
+  if (K == 32)
+    hipLaunchKernelGGL((k_gemm_skinny<T, 32>), g, dim3(256), 0, st, M, (int)N, A, lda, B, ldb, C, ldc);
+  else
+    hipLaunchKernelGGL((k_gemm_skinny<T, 64>), g, dim3(256), 0, st, M, (int)N, A, lda, B, ldb, C, ldc);
+}
+
 void gemm_f64_sub(long M, long N, long K, const double* A, long lda, const double* B, long ldb,
                   double* C, long ldc, hipStream_t st) {
   gemm_sub<double>(M, N, K, A, lda, B, ldb, C, ldc, st);
@@ -1465,6 +1508,9 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
     f16x3_split_rows(mr, (int)w1, (const float*)(c.A + j0 * lda + j0 + w1), lda, *c.pw, c.st, kp);
     f16x3_split_cols(w - w1, (int)w1, (const float*)(c.A + (j0 + w1) * lda + j0), lda, *c.pw, 0, c.st, kp);
     f16x3_gemm_short(mr, w - w1, (int)w1, *c.pw, (float*)(c.A + (j0 + w1) * lda + j0 + w1), lda, c.st);
+  } else if ((w1 == 32 || w1 == 64) && w - w1 <= kSkinnyMax) {
+    gemm_skinny<T>(mr, w - w1, w1, c.A + j0 * lda + j0 + w1, lda, c.A + (j0 + w1) * lda + j0, lda,
+                   c.A + (j0 + w1) * lda + j0 + w1, lda, c.st);
   } else {
     gemm_sub<T>(mr, w - w1, w1, c.A + j0 * lda + j0 + w1, lda, c.A + (j0 + w1) * lda + j0, lda,
                 c.A + (j0 + w1) * lda + j0 + w1, lda, c.st);
