@@ -1216,33 +1216,9 @@ __global__ __launch_bounds__(kTsRows) void k_ts_apply(T* __restrict__ A, long ld
 }
 
 // ============================================================================ row swaps ==========
-// Apply the interchanges ipiv[p0..p1) in order to columns [c0, c0 + nc).
-template <typename T>
-__global__ __launch_bounds__(256) void k_laswp(T* __restrict__ A, long lda, long c0, long nc,
-                                               const int* __restrict__ ipiv, long p0, long p1) {
-  const long c = c0 + (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= c0 + nc) return;
-  T* col = A + c * lda;
-  for (long i = p0; i < p1; ++i) {
-    const long p = ipiv[i];
-    if (p != i) {
-      const T t = col[i];
-      col[i] = col[p];
-      col[p] = t;
-    }
-  }
-}
-
-template <typename T>
-static void laswp(T* A, long lda, long c0, long nc, const int* ipiv, long p0, long p1,
-                  hipStream_t st) {
-  if (nc <= 0 || p1 <= p0) return;
-  hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, A, lda, c0, nc,
-                     ipiv, p0, p1);
-}
-
-// The same interchanges for a narrow column range (inside a panel), where one thread per column
-// walking them in order is bound by its dependent memory round trips. Every workgroup composes
+// Apply the interchanges ipiv[p0..p1) in order to columns [c0, c0 + nc). One thread per column
+// walking them in order would be bound by its dependent memory round trips (a 1,024-interchange
+// step took ~0.9 ms per call that way on wide trailing matrices). Every workgroup composes
 // ipiv[p0, p0 + m) into moves A[dst] <- A[src] over the positions they touch (an LDS table: the m
 // top positions directly, positions below them through an open-addressing hash), then moves its
 // columns, kLaswpCG at a time: all reads, a barrier, all writes.
@@ -1336,7 +1312,7 @@ __global__ __launch_bounds__(256) void k_laswp_mv(T* __restrict__ A, long lda, l
 }
 
 template <typename T>
-static void laswp_narrow(T* A, long lda, long c0, long nc, const int* ipiv, long p0, long p1,
+static void laswp(T* A, long lda, long c0, long nc, const int* ipiv, long p0, long p1,
                          hipStream_t st) {
   if (nc <= 0 || p1 <= p0) return;
   const long cpb = std::max<long>(kLaswpCG, (nc + 255) / 256 / kLaswpCG * kLaswpCG);
@@ -1498,7 +1474,7 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
   const long w1 = (w / 2 + kPanelW - 1) / kPanelW * kPanelW;
   const long lda = c.lda;
   panel_rec(c, j0, w1);
-  laswp_narrow<T>(c.A, lda, j0 + w1, w - w1, c.W->d_ipiv, j0, j0 + w1, c.st);
+  laswp<T>(c.A, lda, j0 + w1, w - w1, c.W->d_ipiv, j0, j0 + w1, c.st);
   trsm_lower_unit<T>(c.A + j0 * lda + j0, lda, w1, c.A + (j0 + w1) * lda + j0, lda, w - w1, c.st);
   const long mr = c.n - j0 - w1;
   if (sizeof(T) == 4 && c.pw && w1 >= kPanelSplitK && mr >= kPanelSplitRows) {
@@ -1516,7 +1492,7 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
                 c.A + (j0 + w1) * lda + j0 + w1, lda, c.st);
   }
   panel_rec(c, j0 + w1, w - w1);
-  laswp_narrow<T>(c.A, lda, j0, w1, c.W->d_ipiv, j0 + w1, j0 + w, c.st);
+  laswp<T>(c.A, lda, j0, w1, c.W->d_ipiv, j0 + w1, j0 + w, c.st);
 }
 
 constexpr long kLuNB = IBEM_LU_NB;
@@ -1525,13 +1501,18 @@ constexpr long kLuNB = IBEM_LU_NB;
 // panel's own kernels (tree, internal FP32 updates) run 1.5-3x slower than alone. Early steps are
 // update-bound and keep most CUs on the update; later steps are panel-bound and give it more.
 // Breakpoints from a fit of measured panel / update times on a 64,614-unknown system.
+#ifndef IBEM_PANEL_M1
+#define IBEM_PANEL_M1 41000
+#define IBEM_PANEL_M2 29000
+#define IBEM_PANEL_M3 20000
+#endif
 static int panel_cus(long m) {
-  if (m > 41000) return 32;
-  if (m > 29000) return 64;
-  if (m > 20000) return 96;
+  if (m > IBEM_PANEL_M1) return 32;
+  if (m > IBEM_PANEL_M2) return 64;
+  if (m > IBEM_PANEL_M3) return 96;
   return 128;
 }
-constexpr long kNarrowCols = 4096;  // trailing columns up to which a step uses the latency-bound forms
+constexpr long kNarrowCols = 4096;  // columns up to which U12 uses the wavefront TRSM, not the block inverses
 
 // Trailing update of step k restricted to columns [c0, c0 + nc) of the trailing matrix (relative to
 // column k + kb): C -= L21 U12(:, cols).
@@ -1676,7 +1657,7 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
     // columns only; L's earlier block columns keep their order (the solve applies the
     // interchanges to the vector instead).
     if (tick) tick(tctx, "lu.swap_trsm", 0, st);
-    laswp_narrow<T>(A, lda, k + kb, kb2, W.d_ipiv, k, k + kb, st);
+    laswp<T>(A, lda, k + kb, kb2, W.d_ipiv, k, k + kb, st);
     trsm_lower_unit<T>(L11, lda, kb, A + (k + kb) * lda + k, lda, kb2, st);
     if (tick) tick(tctx, "lu.swap_trsm", 1, st);
     if (w) {
@@ -1689,17 +1670,14 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
     trailing_update<T>(A, n, lda, k, kb, 0, kb2, w, st);
     if (tick) tick(tctx, "lu.gemm", 1, st);
     hipEventRecord(W.ev_a, st);
-    // (2) the remaining columns, in the bandwidth-bound forms, while the panel runs; enqueued
+    // (2) the remaining columns (U12 through the block inverses when wide) while the panel runs; enqueued
     // before the panel's many small launches, so that the host's time spent launching them never
     // leaves the main stream waiting. The bulk update leaves panel_cus(m) CUs idle for the panel.
     const long nr = m - kb2;
     if (nr > 0) {
       T* B2 = A + (k + kb + kb2) * lda + k;
       if (tick) tick(tctx, "lu.swap_trsm_rest", 0, st);
-      if (nr <= kNarrowCols)
-        laswp_narrow<T>(A, lda, k + kb + kb2, nr, W.d_ipiv, k, k + kb, st);
-      else
-        laswp<T>(A, lda, k + kb + kb2, nr, W.d_ipiv, k, k + kb, st);
+      laswp<T>(A, lda, k + kb + kb2, nr, W.d_ipiv, k, k + kb, st);
       if (sizeof(T) == 4 && nr > kNarrowCols)
         trsm_by_inverse<T>(L11, lda, kb, B2, lda, nr, W, st);
       else
