@@ -290,52 +290,55 @@ __device__ inline void split2(float x, _Float16& hi, _Float16& lo) {
 // A operand: rows r < M of the column-major M x K block at A (lda). Output row-major [M][K] hi / lo
 // (k contiguous, row stride kSplitPadK) and the row's inverse scale; k in [K, kpad) written as zeros,
 // k >= kpad not written (kpad: a multiple of 64, the depth the GEMM reads). Workgroup = kSplitRW
-// rows; LDS transpose.
-constexpr int kSplitRW = 64 * 256 / kSplitK;  // rows per workgroup (the tile stays at 64 KB)
+// rows, one row per lane of each wavefront so that every column read is 256 contiguous bytes: a
+// pass for the row maxima, then the transpose and split in chunks of kSplitChunk columns through LDS
+// (the second read of A mostly hits L2).
+constexpr int kSplitRW = 64;      // rows per workgroup
+constexpr int kSplitChunk = 128;  // k per LDS round
 __global__ __launch_bounds__(256) void k_split_rows(long M, int K, int kpad, const float* __restrict__ A,
                                                     long lda, _Float16* __restrict__ hi,
                                                     _Float16* __restrict__ lo,
                                                     float* __restrict__ inv_scale) {
-  constexpr int NG = 256 / kSplitRW;  // k groups
-  __shared__ float tile[kSplitK][kSplitRW + 1];
-  __shared__ float red[NG][kSplitRW];
+  __shared__ float tile[kSplitRW][kSplitChunk + 1];
+  __shared__ float red[4][kSplitRW];
   __shared__ float ssc[kSplitRW];
   const long r0 = (long)blockIdx.x * kSplitRW;
-  const int t = threadIdx.x, rl0 = t % kSplitRW, g = t / kSplitRW;
-  const long r = r0 + rl0;
+  const int t = threadIdx.x, rl = t & 63, g = t >> 6;
+  const long r = r0 + rl;
   float m = 0.0f;
-  for (int k = g; k < K; k += NG) {
-    const float v = r < M ? A[(long)k * lda + r] : 0.0f;
-    tile[k][rl0] = v;
-    m = fmaxf(m, fabsf(v));
-  }
-  red[g][rl0] = m;
+  if (r < M)
+    for (int k = g; k < K; k += 4) m = fmaxf(m, fabsf(A[(long)k * lda + r]));
+  red[g][rl] = m;
   __syncthreads();
   if (g == 0) {
-    float mm = red[0][rl0];
-    for (int q = 1; q < NG; ++q) mm = fmaxf(mm, red[q][rl0]);
+    const float mm = fmaxf(fmaxf(red[0][rl], red[1][rl]), fmaxf(red[2][rl], red[3][rl]));
     const float sc = pow2_scale(mm);
-    ssc[rl0] = sc;
+    ssc[rl] = sc;
     if (r < M) inv_scale[r] = 1.0f / sc;
   }
-  __syncthreads();
-  // write: thread handles 8 consecutive k of one row per pass (16-byte stores)
-  const int kp8 = kpad / 8;
-  for (int u = t; u < kSplitRW * kp8; u += 256) {
-    const int rl = u / kp8, k8 = (u % kp8) * 8;
-    const long rr = r0 + rl;
-    if (rr >= M) continue;
-    half8 h, l;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const float v = (k8 + q < K) ? tile[k8 + q][rl] * ssc[rl] : 0.0f;
-      _Float16 a, b;
-      split2(v, a, b);
-      h[q] = a;
-      l[q] = b;
+  for (int k0 = 0; k0 < kpad; k0 += kSplitChunk) {
+    __syncthreads();  // ssc ready / the previous chunk has been written out
+    for (int kk = g; kk < kSplitChunk; kk += 4) {
+      const int k = k0 + kk;
+      tile[rl][kk] = (r < M && k < K) ? A[(long)k * lda + r] : 0.0f;
     }
-    *(half8*)(hi + rr * kSplitPadK + k8) = h;
-    *(half8*)(lo + rr * kSplitPadK + k8) = l;
+    __syncthreads();
+    // 16-byte stores: 8 consecutive k of one row per pass
+    for (int u = t; u < kSplitRW * (kSplitChunk / 8); u += 256) {
+      const int rr = u / (kSplitChunk / 8), k8 = (u % (kSplitChunk / 8)) * 8;
+      const long row = r0 + rr;
+      if (row >= M || k0 + k8 >= kpad) continue;
+      half8 h, l;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        _Float16 x, y;
+        split2(tile[rr][k8 + q] * ssc[rr], x, y);
+        h[q] = x;
+        l[q] = y;
+      }
+      *(half8*)(hi + row * kSplitPadK + k0 + k8) = h;
+      *(half8*)(lo + row * kSplitPadK + k0 + k8) = l;
+    }
   }
 }
 
