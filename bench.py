@@ -156,6 +156,7 @@ def cpu_baseline(prob, workload: str):
            + (ph["bem0"] + ph["bem1"] + ph["bem2"]) * (prob.NE * num) / (ne_s * ns)
            + t_lu * (n78 / nlu) ** 3)
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    fair = fair_cpu_lu(prob.n60)
     return {"value": est, "unit": "s per RVE solve (extrapolated)", "cores": cores,
             "kind": "reference",
             "sample": (f"reference assembly phases (OpenMP {cores} threads) on {ns} of the {workload} "
@@ -164,7 +165,24 @@ def cpu_baseline(prob, workload: str):
                        f"element-sphere) to the full RVE; Eigen FullPivLU (1 thread) timed at "
                        f"n={nlu} ({t_lu:.2f} s) and scaled by (n78/{nlu})^3 with n78={n78}; "
                        f"extrapolated, not run end to end"),
-            "phases_s": ph, "fullpiv_s_at_n": [nlu, t_lu]}
+            "phases_s": ph, "fullpiv_s_at_n": [nlu, t_lu], "fair_lu": fair}
+
+
+def fair_cpu_lu(n60: int, nlu: int = 6000) -> dict:
+    """The "fair CPU" LU beside the reference's unthreaded full-pivot one: LAPACK's blocked
+    partial-pivot getrf (scipy, all host threads) at n = nlu, scaled by (n60 / nlu)^3 to the
+    60-unknown system the GPU factors."""
+    from scipy.linalg import lu_factor
+    rng = np.random.default_rng(1)
+    A = np.asfortranarray(rng.standard_normal((nlu, nlu)) + nlu * np.eye(nlu))
+    lu_factor(A[:500, :500].copy(order="F"))  # load the threads
+    t0 = time.perf_counter()
+    lu_factor(A, overwrite_a=True, check_finite=False)
+    t = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"n": nlu, "s": t, "gflops": 2.0 * nlu ** 3 / 3 / t / 1e9, "threads": threads,
+            "s_at_n60": t * (n60 / nlu) ** 3, "n60": n60,
+            "note": "LAPACK dgetrf (blocked, partial pivoting) through scipy, extrapolated by n^3"}
 
 
 def main():
