@@ -1227,20 +1227,20 @@ __global__ __launch_bounds__(kTsRows) void k_ts_apply(T* __restrict__ A, long ld
 // columns, kLaswpCG at a time: all reads, a barrier, all writes.
 constexpr int kLaswpH = 2048;  // hash slots: at least twice the positions below the range (<= m)
 constexpr int kLaswpCG = 4;    // columns per read/write round
-template <typename T>
-__global__ __launch_bounds__(256) void k_laswp_mv(T* __restrict__ A, long lda, long c0, long nc,
-                                                  const int* __restrict__ ipiv, long p0, int m,
-                                                  int cpb) {
-  static_assert(2 * IBEM_LU_NB <= 8 * 256, "moves per column <= 8 per thread");
-  __shared__ int hkey[kLaswpH];                 // position below the range, or -1
-  __shared__ int val[IBEM_LU_NB + kLaswpH];     // slot -> original position of the entry there
-  __shared__ int slotp[IBEM_LU_NB];             // slot of ipiv[p0 + i]
-  __shared__ int mdst[2 * IBEM_LU_NB], msrc[2 * IBEM_LU_NB];
-  __shared__ int nmv;
+// Move list of ipiv[p0, p0 + m) (shared by the workgroup): returns the number of moves
+// A[mdst[q]] <- A[msrc[q]]. Ends with a barrier.
+struct LaswpLds {
+  int hkey[kLaswpH];              // position below the range, or -1
+  int val[IBEM_LU_NB + kLaswpH];  // slot -> original position of the entry there
+  int slotp[IBEM_LU_NB];          // slot of ipiv[p0 + i]
+  int mdst[2 * IBEM_LU_NB], msrc[2 * IBEM_LU_NB];
+  int nmv;
+};
+__device__ int laswp_moves(const int* __restrict__ ipiv, long p0, int m, LaswpLds& S) {
   const int t = threadIdx.x;
-  for (int h = t; h < kLaswpH; h += 256) hkey[h] = -1;
-  for (int i = t; i < m; i += 256) val[i] = (int)(p0 + i);
-  if (t == 0) nmv = 0;
+  for (int h = t; h < kLaswpH; h += 256) S.hkey[h] = -1;
+  for (int i = t; i < m; i += 256) S.val[i] = (int)(p0 + i);
+  if (t == 0) S.nmv = 0;
   __syncthreads();
   for (int i = t; i < m; i += 256) {
     const int p = ipiv[p0 + i];
@@ -1248,70 +1248,84 @@ __global__ __launch_bounds__(256) void k_laswp_mv(T* __restrict__ A, long lda, l
     if (p >= p0 + m) {
       unsigned h = ((unsigned)p * 2654435761u) >> 21;  // 11 bits: kLaswpH slots
       for (;;) {
-        const int old = atomicCAS(&hkey[h], -1, p);
+        const int old = atomicCAS(&S.hkey[h], -1, p);
         if (old == -1 || old == p) break;
         h = (h + 1) & (kLaswpH - 1);
       }
-      val[IBEM_LU_NB + h] = p;
+      S.val[IBEM_LU_NB + h] = p;
       sl = IBEM_LU_NB + (int)h;
     }
-    slotp[i] = sl;
+    S.slotp[i] = sl;
   }
   __syncthreads();
   if (t == 0) {
     for (int i = 0; i < m; ++i) {
-      const int b = slotp[i];
+      const int b = S.slotp[i];
       if (b != i) {
-        const int va = val[i], vb = val[b];
-        val[i] = vb;
-        val[b] = va;
+        const int va = S.val[i], vb = S.val[b];
+        S.val[i] = vb;
+        S.val[b] = va;
       }
     }
   }
   __syncthreads();
   for (int i = t; i < m; i += 256)
-    if (val[i] != p0 + i) {
-      const int q = atomicAdd(&nmv, 1);
-      mdst[q] = (int)(p0 + i);
-      msrc[q] = val[i];
+    if (S.val[i] != p0 + i) {
+      const int q = atomicAdd(&S.nmv, 1);
+      S.mdst[q] = (int)(p0 + i);
+      S.msrc[q] = S.val[i];
     }
   for (int h = t; h < kLaswpH; h += 256) {
-    const int k = hkey[h];
-    if (k >= 0 && val[IBEM_LU_NB + h] != k) {
-      const int q = atomicAdd(&nmv, 1);
-      mdst[q] = k;
-      msrc[q] = val[IBEM_LU_NB + h];
+    const int k = S.hkey[h];
+    if (k >= 0 && S.val[IBEM_LU_NB + h] != k) {
+      const int q = atomicAdd(&S.nmv, 1);
+      S.mdst[q] = k;
+      S.msrc[q] = S.val[IBEM_LU_NB + h];
     }
   }
   __syncthreads();
-  const int nm = nmv;
-  const long cb = c0 + (long)blockIdx.x * cpb, ce = min(cb + cpb, c0 + nc);
-  for (long g = cb; g < ce; g += kLaswpCG) {
-    T v[kLaswpCG][8];
+  return S.nmv;
+}
+// The moves applied to columns [g, min(g + kLaswpCG, ce)): all reads, a barrier, all writes, a
+// barrier (the next group reads the same rows).
+template <typename T>
+__device__ __forceinline__ void laswp_apply_group(T* __restrict__ A, long lda, long g, long ce, int nm,
+                                                  const LaswpLds& S) {
+  const int t = threadIdx.x;
+  T v[kLaswpCG][8];
 #pragma unroll
-    for (int j = 0; j < kLaswpCG; ++j) {
-      const T* col = A + min(g + j, ce - 1) * lda;
+  for (int j = 0; j < kLaswpCG; ++j) {
+    const T* col = A + min(g + j, ce - 1) * lda;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int q = t + 256 * k;
+      v[j][k] = q < nm ? col[S.msrc[q]] : T(0);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // every read of the group has returned before the barrier
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kLaswpCG; ++j) {
+    if (g + j < ce) {
+      T* col = A + (g + j) * lda;
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const int q = t + 256 * k;
-        v[j][k] = q < nm ? col[msrc[q]] : T(0);
+        if (q < nm) col[S.mdst[q]] = v[j][k];
       }
     }
-    __builtin_amdgcn_s_waitcnt(0);  // every read of the group has returned before the barrier
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kLaswpCG; ++j) {
-      if (g + j < ce) {
-        T* col = A + (g + j) * lda;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const int q = t + 256 * k;
-          if (q < nm) col[mdst[q]] = v[j][k];
-        }
-      }
-    }
-    __syncthreads();  // the group's writes are done before the next group reads (same rows)
   }
+  __syncthreads();
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp_mv(T* __restrict__ A, long lda, long c0, long nc,
+                                                  const int* __restrict__ ipiv, long p0, int m,
+                                                  int cpb) {
+  static_assert(2 * IBEM_LU_NB <= 8 * 256, "moves per column <= 8 per thread");
+  __shared__ LaswpLds S;
+  const int nm = laswp_moves(ipiv, p0, m, S);
+  const long cb = c0 + (long)blockIdx.x * cpb, ce = min(cb + cpb, c0 + nc);
+  for (long g = cb; g < ce; g += kLaswpCG) laswp_apply_group(A, lda, g, ce, nm, S);
 }
 
 template <typename T>
@@ -1372,13 +1386,12 @@ __device__ __forceinline__ T readlane_t(T v, int lane) {
   }
 }
 
-template <typename T, int Q>
-__global__ __launch_bounds__(256) void k_trsm_wave(const T* __restrict__ L, long ldl, int m,
-                                                   T* __restrict__ B, long ldb, long N) {
-  constexpr int CW = sizeof(T) == 4 ? 32 : 16;  // staged columns per chunk (32 KB of LDS at Q = 4)
-  __shared__ T sL[CW][64 * Q];                  // sL[c][r - 64 qb] = L(r, i0 + c) for r > i0 + c
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const long col = (long)blockIdx.x * 4 + wv;
+// Column col (one per wavefront) of B, rows [0, m): x <- L^-1 x with unit-lower L (m <= 64 Q).
+// All 256 threads call it (it stages L through sL with barriers).
+template <typename T, int Q, int CW>
+__device__ __forceinline__ void trsm_wave_cols(const T* __restrict__ L, long ldl, int m, T* __restrict__ B,
+                                               long ldb, long N, long col, T (*sL)[64 * Q]) {
+  const int lane = threadIdx.x & 63;
   const bool has = col < N;
   T* bcol = B + (has ? col : 0) * ldb;
   T x[Q];
@@ -1417,6 +1430,32 @@ __global__ __launch_bounds__(256) void k_trsm_wave(const T* __restrict__ L, long
     if (has && r < m) bcol[r] = x[q];
   }
 }
+template <typename T>
+constexpr int trsm_cw() { return sizeof(T) == 4 ? 32 : 16; }  // staged columns per chunk (32 KB at Q = 4)
+template <typename T, int Q>
+__global__ __launch_bounds__(256) void k_trsm_wave(const T* __restrict__ L, long ldl, int m,
+                                                   T* __restrict__ B, long ldb, long N) {
+  constexpr int CW = trsm_cw<T>();
+  __shared__ T sL[CW][64 * Q];  // sL[c][r - 64 qb] = L(r, i0 + c) for r > i0 + c
+  trsm_wave_cols<T, Q, CW>(L, ldl, m, B, ldb, N, (long)blockIdx.x * 4 + (threadIdx.x >> 6), sL);
+}
+// Inside a panel: the m interchanges ipiv[p0, p0 + m) applied to the N columns at B - p0 rows (of
+// A), then their top m rows solved with the unit-lower L: the interchange and TRSM launches of a
+// recursion node in one. Workgroup = 4 columns, moved together, then one wavefront per column
+// solves it (reading back only what this workgroup wrote).
+template <typename T, int Q>
+__global__ __launch_bounds__(256) void k_laswp_trsm(T* __restrict__ A, long lda, long c0, long N,
+                                                    const int* __restrict__ ipiv, long p0, int m,
+                                                    const T* __restrict__ L) {
+  constexpr int CW = trsm_cw<T>();
+  __shared__ LaswpLds S;
+  __shared__ T sL[CW][64 * Q];
+  const int nm = laswp_moves(ipiv, p0, m, S);
+  const long g = c0 + (long)blockIdx.x * kLaswpCG;
+  laswp_apply_group(A, lda, g, c0 + N, nm, S);
+  trsm_wave_cols<T, Q, CW>(L, lda, m, A + c0 * lda + p0, lda, N, (long)blockIdx.x * 4 + (threadIdx.x >> 6), sL);
+}
+static_assert(kLaswpCG == 4, "the fused interchange + TRSM kernel moves one column per wavefront");
 
 template <typename T>
 static void trsm_lower_unit(const T* L, long ldl, long m, T* B, long ldb, long N, hipStream_t st) {
@@ -1477,8 +1516,19 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
   const long w1 = (w / 2 + kPanelW - 1) / kPanelW * kPanelW;
   const long lda = c.lda;
   panel_rec(c, j0, w1);
-  laswp<T>(c.A, lda, j0 + w1, w - w1, c.W->d_ipiv, j0, j0 + w1, c.st);
-  trsm_lower_unit<T>(c.A + j0 * lda + j0, lda, w1, c.A + (j0 + w1) * lda + j0, lda, w - w1, c.st);
+  if (w1 <= 256) {  // the node's interchanges and U12 in one launch
+    const dim3 g((unsigned)((w - w1 + 3) / 4));
+    const T* L11 = c.A + j0 * lda + j0;
+    if (w1 <= 64)
+      hipLaunchKernelGGL((k_laswp_trsm<T, 1>), g, dim3(256), 0, c.st, c.A, lda, j0 + w1, w - w1, c.W->d_ipiv, j0, (int)w1, L11);
+    else if (w1 <= 128)
+      hipLaunchKernelGGL((k_laswp_trsm<T, 2>), g, dim3(256), 0, c.st, c.A, lda, j0 + w1, w - w1, c.W->d_ipiv, j0, (int)w1, L11);
+    else
+      hipLaunchKernelGGL((k_laswp_trsm<T, 4>), g, dim3(256), 0, c.st, c.A, lda, j0 + w1, w - w1, c.W->d_ipiv, j0, (int)w1, L11);
+  } else {
+    laswp<T>(c.A, lda, j0 + w1, w - w1, c.W->d_ipiv, j0, j0 + w1, c.st);
+    trsm_lower_unit<T>(c.A + j0 * lda + j0, lda, w1, c.A + (j0 + w1) * lda + j0, lda, w - w1, c.st);
+  }
   const long mr = c.n - j0 - w1;
   if (sizeof(T) == 4 && c.pw && w1 >= kPanelSplitK && mr >= kPanelSplitRows) {
     // the deep updates of the panel (three quarters of its flops) on the split FP16 products: the
