@@ -118,26 +118,32 @@ __global__ __launch_bounds__(256) void k_incl(const double* __restrict__ cen,
 }
 
 // ---- K3: boundary rows x eigenstrain columns ----------------------------------------------------
-// Thread = (node j, component i); row 3j+i; workgroup = 64 nodes x one sphere.
+// Thread = (node j, component i); row 3j+i; workgroup = 64 nodes x one sphere. The node's three
+// threads first fill its table of radial values (field_tab_fill, 55 multi-indices) in LDS; then
+// each writes its row's 60 columns from the table.
 __global__ __launch_bounds__(192) void k_field(const double* __restrict__ nodes, int NN,
                                                const double* __restrict__ cen,
                                                const double* __restrict__ rad, long n, long row0,
                                                Mat M, double* __restrict__ A) {
+  __shared__ double stab[64][kFieldTab];
   const int K = blockIdx.y;
-  const int j = blockIdx.x * 64 + threadIdx.x / 3;
-  const int i = threadIdx.x % 3;
-  if (j >= NN) return;
-  const double x[3] = {nodes[3 * j] - cen[3 * K], nodes[3 * j + 1] - cen[3 * K + 1],
-                       nodes[3 * j + 2] - cen[3 * K + 2]};
+  const int jl = threadIdx.x / 3, i = threadIdx.x % 3;
+  const int j = blockIdx.x * 64 + jl;
+  const int jj = j < NN ? j : NN - 1;  // rows past NN fill a table nobody reads
+  const double x[3] = {nodes[3 * jj] - cen[3 * K], nodes[3 * jj + 1] - cen[3 * K + 1],
+                       nodes[3 * jj + 2] - cen[3 * K + 2]};
   const double a = rad[K];
   RadPoint P;
   rad_point(P, x);
   const bool inside = is_inside(x, a);
   double C[8][4];
   pot_coefs(a, inside, C);
+  for (int u = i; u < kFieldMi; u += 3) field_tab_fill(P, inside, C, u, stab[jl]);
+  __syncthreads();
+  if (j >= NN) return;
   const long row = 3L * j + i;
   double* Acol = A + (row0 + 60L * K) * n + row;
-  for (int c = 0; c < 60; ++c) Acol[(long)c * n] = field_entry(P, inside, C, M, i, c);
+  for (int c = 0; c < 60; ++c) Acol[(long)c * n] = field_entry_tab(stab[jl], M, i, c);
 }
 
 // ---- K4: boundary rows x boundary columns, b, rigid-body diagonal -------------------------------

@@ -430,6 +430,82 @@ IBEM_HD double field_entry(const RadPoint& P, bool inside, const double C[8][4],
   return mapw(m, n) * fac * v;
 }
 
+// The same block from a per-(node, sphere) table of the radial values it uses. field_entry asks
+// pot_direct for radial(f, b) = sum_k C[f][k] B_k(b) at 78 distinct (function, multi-index) pairs
+// (f0, f2 at |b| = 1; f1 at 2; f3, f4, f6 at 3; f5 at 4; f7 at 5) and recomputes each many times
+// (~1,400 basis evaluations per block against 55 distinct multi-indices). The table holds each
+// value once, computed by the same expression; field_entry_tab combines them as field_entry does.
+constexpr int kFieldTab = 78;
+constexpr int kFieldMi = 55;  // multi-indices of orders 1..5
+IBEM_HD int field_tab_order(int f) {
+  return f == 0 || f == 2 ? 1 : f == 1 ? 2 : f == 5 ? 4 : f == 7 ? 5 : 3;
+}
+IBEM_HD int field_tab_off(int f) {
+  return f == 0 ? 0 : f == 1 ? 3 : f == 2 ? 9 : f == 3 ? 12 : f == 4 ? 22 : f == 5 ? 32 : f == 6 ? 47 : 57;
+}
+// position of multi-index b among those of its order: b1 + b2 = k, index k (k + 1) / 2 + b2
+IBEM_HD int mi_index(int b0, int b1, int b2) {
+  (void)b0;
+  const int k = b1 + b2;
+  return k * (k + 1) / 2 + b2;
+}
+// multi-index number u (orders 1..5, in order) -> order and b
+IBEM_HD void mi_decode(int u, int& o, int& b0, int& b1, int& b2) {
+  o = u < 3 ? 1 : u < 9 ? 2 : u < 19 ? 3 : u < 34 ? 4 : 5;
+  const int idx = u - (o == 1 ? 0 : o == 2 ? 3 : o == 3 ? 9 : o == 4 ? 19 : 34);
+  int k = 0;
+  while ((k + 1) * (k + 2) / 2 <= idx) ++k;
+  b2 = idx - k * (k + 1) / 2;
+  b1 = k - b2;
+  b0 = o - k;
+}
+// table entries of multi-index number u
+IBEM_HD void field_tab_fill(const RadPoint& P, bool inside, const double C[8][4], int u, double* tab) {
+  int o, b0, b1, b2;
+  mi_decode(u, o, b0, b1, b2);
+  double B0, B1, B2, B3;
+  rad_basis(P, inside, b0, b1, b2, B0, B1, B2, B3);
+  const int idx = mi_index(b0, b1, b2);
+#pragma unroll
+  for (int f = 0; f < 8; ++f)
+    if (field_tab_order(f) == o)
+      tab[field_tab_off(f) + idx] = C[f][0] * B0 + C[f][1] * B1 + C[f][2] * B2 + C[f][3] * B3;
+}
+IBEM_HD double pot_tab(const double* tab, int psi, int w, int c0, int c1, int c2) {
+  auto T = [&](int f, int b0, int b1, int b2) { return tab[field_tab_off(f) + mi_index(b0, b1, b2)]; };
+  const int base = psi ? 4 : 0;
+  if (w == 0) return T(base + 0, c0, c1, c2);
+  if (w < 4) {
+    const int p = w - 1;
+    return T(base + 1, c0 + (p == 0), c1 + (p == 1), c2 + (p == 2));
+  }
+  const int p = pair_i(w - 4), q = pair_j(w - 4);
+  double v = T(base + 3, c0 + (p == 0) + (q == 0), c1 + (p == 1) + (q == 1), c2 + (p == 2) + (q == 2));
+  if (p == q) v += T(base + 2, c0, c1, c2);
+  return v;
+}
+IBEM_HD double field_entry_tab(const double* tab, const Mat& M, int i, int col) {
+  const Slot Cc = slot60(col);
+  const int m = Cc.i, n = Cc.j, w = Cc.w;
+  auto phi1 = [&](int a) { return pot_tab(tab, 0, w, a == 0, a == 1, a == 2); };
+  auto psi3 = [&](int a, int b, int d) {
+    return pot_tab(tab, 1, w, (a == 0) + (b == 0) + (d == 0), (a == 1) + (b == 1) + (d == 1),
+                   (a == 2) + (b == 2) + (d == 2));
+  };
+  double v = 0.0;
+  if (m == n) {
+    double tr = M.cg1 * phi1(i);
+    for (int k = 0; k < 3; ++k) tr -= M.cg2 * psi3(i, k, k);
+    v += M.lam0 * tr;
+  }
+  double s = -2.0 * M.cg2 * psi3(i, m, n);
+  if (i == m) s += M.cg1 * phi1(n);
+  if (i == n) s += M.cg1 * phi1(m);
+  v += M.mu0 * s;
+  const double fac = (Cc.order == 2 && Cc.p != Cc.q) ? 2.0 : 1.0;
+  return mapw(m, n) * fac * v;
+}
+
 // ---- boundary element geometry (BEMElasticHelper3D.cpp) ---------------------------------------
 // 8-point Gauss-Legendre table as printed in BEMBuilderElastic3D.cpp:1203-1222
 IBEM_HD double gauss8_x(int a) {

@@ -41,6 +41,21 @@ void hc_field_block(double E0, double nu0, double E1, double nu1, const double* 
     for (int i = 0; i < 3; ++i) out[c * 3 + i] = field_entry(P, inside, C, M, i, c);
 }
 
+// The same block through the per-(node, sphere) radial table the GPU kernel uses
+void hc_field_block_tab(double E0, double nu0, double E1, double nu1, const double* x, double a,
+                        double* out) {
+  Mat M = make_mat(E0, nu0, E1, nu1);
+  RadPoint P;
+  rad_point(P, x);
+  const bool inside = is_inside(x, a);
+  double C[8][4];
+  pot_coefs(a, inside, C);
+  double tab[kFieldTab];
+  for (int u = 0; u < kFieldMi; ++u) field_tab_fill(P, inside, C, u, tab);
+  for (int c = 0; c < 60; ++c)
+    for (int i = 0; i < 3; ++i) out[c * 3 + i] = field_entry_tab(tab, M, i, c);
+}
+
 // Element x sphere-centre block in the 60-row layout: H part (60 x 12, = -fac DDsu) and G part
 // (60 x 12, = fac DDst), row-major; post != 0 selects the stress (postprocessor) variant with
 // order-0 rows only (6 x 12).

@@ -17,6 +17,7 @@ def setup(hostmath, oracle_lib):
     H = hostmath
     H.hc_incl_block.argtypes = [ctypes.c_double] * 4 + [dp, ctypes.c_double, ctypes.c_int, dp]
     H.hc_field_block.argtypes = [ctypes.c_double] * 4 + [dp, ctypes.c_double, dp]
+    H.hc_field_block_tab.argtypes = [ctypes.c_double] * 4 + [dp, ctypes.c_double, dp]
     H.hc_bem_rows.argtypes = [ctypes.c_double] * 4 + [dp, dp, ctypes.c_int, dp, dp]
     p = make_rve(11, 3, 0.1, 0.16, 2, nump=0, E1=7.0, nu1=0.22)
     op = oracle_lib.OracleProblem(p)
@@ -57,6 +58,21 @@ def test_node_sphere_columns(setup):
                              out.ctypes.data_as(dp))
             ref = A60[3 * j:3 * j + 3, 3 * NN + 60 * K:3 * NN + 60 * K + 60]
             assert np.abs(out.reshape(3, 60, order="F") - ref).max() <= 1e-14 * scale
+
+
+def test_node_sphere_table_form(setup):
+    """The GPU kernel's table form of the node-sphere block (each radial value computed once per
+    pair) equals the direct evaluation bit for bit, at nodes outside and inside spheres."""
+    H, p, _ = setup
+    rng = np.random.default_rng(3)
+    xs = [p.nodes[j] - p.centres[K] for K in range(p.num) for j in range(0, p.NN, 7)]
+    xs += [rng.uniform(-0.5, 0.5, 3) * p.radii[0] for _ in range(8)]  # points inside sphere 0
+    for x in xs:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        a, b = np.zeros(180), np.zeros(180)
+        H.hc_field_block(p.E0, p.nu0, p.E1, p.nu1, x.ctypes.data_as(dp), p.radii[0], a.ctypes.data_as(dp))
+        H.hc_field_block_tab(p.E0, p.nu0, p.E1, p.nu1, x.ctypes.data_as(dp), p.radii[0], b.ctypes.data_as(dp))
+        assert np.array_equal(a, b)
 
 
 def test_strain_rows(setup, oracle_lib):
