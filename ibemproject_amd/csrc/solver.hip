@@ -2124,26 +2124,33 @@ __global__ __launch_bounds__(256) void k_rowmax_done(const double* __restrict__ 
   r[i] = m > 0.0 ? 1.0 / m : 1.0;
 }
 // Column j: c_j = 1 / max_i |r_i A_ij|, out[:, j] = R A C (in TO precision); A is only read.
+// One workgroup per column, kColThreads threads (one workgroup per CU): the column is read for its
+// maximum, then again for the scaled copy; with few columns in flight the second read finds them in
+// the device's last-level cache.
+#ifndef IBEM_COL_THREADS
+#define IBEM_COL_THREADS 1024
+#endif
+constexpr int kColThreads = IBEM_COL_THREADS;
 template <typename TO>
-__global__ __launch_bounds__(256) void k_colscale_copy(const double* __restrict__ A, long n, long lda,
-                                                       const double* __restrict__ r,
-                                                       double* __restrict__ c, TO* __restrict__ out,
-                                                       long ldo) {
-  __shared__ double red[256];
+__global__ __launch_bounds__(kColThreads) void k_colscale_copy(const double* __restrict__ A, long n, long lda,
+                                                               const double* __restrict__ r,
+                                                               double* __restrict__ c, TO* __restrict__ out,
+                                                               long ldo) {
+  __shared__ double red[kColThreads / 64];
   const long j = blockIdx.x;
   const double* col = A + j * lda;
   double m = 0.0;
-  for (long i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(col[i] * r[i]));
-  red[threadIdx.x] = m;
+  for (long i = threadIdx.x; i < n; i += kColThreads) m = fmax(m, fabs(col[i] * r[i]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  const double cj = red[0] > 0.0 ? 1.0 / red[0] : 1.0;
+  double mm = red[0];
+#pragma unroll
+  for (int q = 1; q < kColThreads / 64; ++q) mm = fmax(mm, red[q]);
+  const double cj = mm > 0.0 ? 1.0 / mm : 1.0;
   if (threadIdx.x == 0) c[j] = cj;
   TO* o = out + j * ldo;
-  for (long i = threadIdx.x; i < n; i += blockDim.x) o[i] = (TO)(col[i] * r[i] * cj);
+  for (long i = threadIdx.x; i < n; i += kColThreads) o[i] = (TO)(col[i] * r[i] * cj);
 }
 
 size_t equilibrate_workspace_doubles(long n) { return (size_t)kRowChunks * n; }
@@ -2154,7 +2161,7 @@ static void equilibrate_copy(const double* A, long n, long lda, double* r, doubl
   hipLaunchKernelGGL(k_rowmax_part, dim3((unsigned)((n + 255) / 256), kRowChunks), dim3(256), 0, st, A,
                      n, lda, ws);
   hipLaunchKernelGGL(k_rowmax_done, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ws, n, r);
-  hipLaunchKernelGGL(k_colscale_copy<TO>, dim3((unsigned)n), dim3(256), 0, st, A, n, lda, r, c, out, ldo);
+  hipLaunchKernelGGL(k_colscale_copy<TO>, dim3((unsigned)n), dim3(kColThreads), 0, st, A, n, lda, r, c, out, ldo);
 }
 void equilibrate_copy_f64(const double* A, long n, long lda, double* r, double* c, double* out, long ldo,
                           double* ws, hipStream_t st) {
