@@ -772,17 +772,17 @@ static void f16x3_split(long M, long N, int K, const float* A, long lda, const f
   f16x3_split_rows(M, K, A, lda, w, st);
   f16x3_split_cols(N, K, B, ldb, w, 0, st);
 }
-// C[M x N] -= split A times the split B columns [c0, c0 + N)
+// CUs of the current device (cached per device; 256 if the query fails)
 static int device_cus() {
-  static int cus = 0;
-  if (cus == 0) {
-    int dev = 0, n = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
-      n = 256;
-    cus = n;
-  }
-  return cus;
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) dev = 0;
+  int* slot = dev < 64 ? &cus[dev] : nullptr;
+  if (slot && *slot > 0) return *slot;
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+  if (slot) *slot = n;
+  return n;
 }
 // Workgroups of the persistent 256 x 256 kernel: one per CU the stream may use (cus, 0 = all)
 static unsigned big_gemm_grid(long tiles, int cus) {
@@ -794,6 +794,8 @@ static unsigned big_gemm_grid(long tiles, int cus) {
 #ifndef IBEM_BIG_MIN_TILES
 #define IBEM_BIG_MIN_TILES 512
 #endif
+// C[M x N] -= split A times the split B columns [c0, c0 + N); the persistent 256 x 256 kernel runs
+// on at most cus CUs (0: all)
 static void f16x3_gemm(long M, long N, int K, const F16x3Ws& w, float* C, long ldc, hipStream_t st,
                        long c0 = 0, int cus = 0) {
   // 256 x 256 tiles once there are at least two per CU (one workgroup per CU), else 128 x 128
