@@ -32,3 +32,28 @@ def test_lu_sizes_through_rve(num, m, oracle_lib):
     assert rel < 1e-11
     assert e < 1e-10
     eng.close()
+
+
+@pytest.mark.parametrize("mode_name", ["SOLVE_MIXED_FP16", "SOLVE_MIXED_FP32"])
+def test_mixed_lu_mid_size(mode_name):
+    """A system large enough for the bulk trailing updates to take the 256 x 256 split-product
+    kernel (with partial edge tiles) and for the panels' deep updates to take the split path,
+    refined to 1e-10 and compared with LAPACK's solution."""
+    from ibemproject_amd import engine as E
+    from ibemproject_amd.generate import make_rve
+    prob = make_rve(321, 150, 0.035, 0.05, 8, nump=0)
+    case(f"mixed mid size: {mode_name} num=150 n60={prob.n60}")
+    eng = E.Engine(prob)
+    eng.assemble()
+    A, b = eng.system()
+    rel, it = eng.solve(getattr(E, mode_name), 1e-10)
+    _, _, X = eng.getUTVector()
+    x_ref = np.linalg.solve(A, b)
+    e = np.linalg.norm(X - x_ref) / np.linalg.norm(x_ref)
+    r = np.linalg.norm(b - A @ X) / np.linalg.norm(b)
+    print("n", prob.n60, "iterations", it, "engine residual", rel, "residual", r, "err vs LAPACK", e)
+    assert prob.n60 > 9000
+    assert rel <= 1e-10 and r <= 1e-10
+    assert it <= 8
+    assert e < 1e-8
+    eng.close()
