@@ -656,17 +656,18 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
     if (!reuse) {
       ctx->prepared_mode = -1;
       // R A C straight from A into the FP32 factor storage (A itself stays for the residual)
+      if (mode == IBEM3D_SOLVE_MIXED_FP16 && !ctx->d_f16ws)
+        if (int rc = dalloc(ctx, &ctx->d_f16ws, lu_f16_workspace_doubles(n))) return rc;
+      // columns past the first panel are copied on the LU's second stream while that panel factors
       {
         Scope s(ctx, "solve.equilibrate");
         equilibrate_copy_f32(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_A32, ctx->lda32, ctx->d_eqws,
-                             ctx->st);
+                             ctx->st, &ctx->W);
       }
       {
         Scope s(ctx, "solve.lu_f32");
-        if (mode == IBEM3D_SOLVE_MIXED_FP16 && !ctx->d_f16ws)
-          if (int rc = dalloc(ctx, &ctx->d_f16ws, lu_f16_workspace_doubles(n))) return rc;
         if (lu_factor_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->st, tick, ctx,
-                          mode == IBEM3D_SOLVE_MIXED_FP16 ? ctx->d_f16ws : nullptr))
+                          mode == IBEM3D_SOLVE_MIXED_FP16 ? ctx->d_f16ws : nullptr, true))
           return fail(ctx, IBEM3D_EHIP, "LU launch failed");
       }
       int info = 0;

@@ -60,6 +60,7 @@ typedef void (*LuTick)(void*, const char*, int, hipStream_t);
 struct LuWork {
   hipStream_t st2 = nullptr;             // second stream: look-ahead panels
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  hipEvent_t ev_r = nullptr, ev_cols = nullptr;  // equilibration: row scales ready / last columns copied
   int* d_ipiv = nullptr;   // n
   double* d_ws = nullptr;  // lu_workspace_doubles(n), zero-filled once at allocation
   int* d_info = nullptr;
@@ -80,7 +81,12 @@ int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick
 // f16ws != nullptr: the trailing updates and the panels' deep updates run on split FP16 products
 // (workspace of lu_f16_workspace_doubles(n) doubles); otherwise on FP32 MFMA
 size_t lu_f16_workspace_doubles(long n);
-int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx, double* f16ws);
+// cols_pending: columns past the first panel are still being copied (ev_cols, see
+// equilibrate_copy_f32); the factorisation waits for them after its first panel
+int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx, double* f16ws,
+                  bool cols_pending = false);
+// the LU's second stream and events (created once)
+int lu_streams(LuWork& W);
 // After a factorisation: each step's interchanges as a move list and the diagonal-block inverses
 // (synchronises the stream once). Returns nonzero on a HIP error.
 int lu_prepare_solve_f64(const double* LU, long n, long lda, LuWork& W, hipStream_t st);
@@ -96,12 +102,14 @@ size_t gemv_workspace_doubles(long m);
 void gemv_f64(long m, long n, double alpha, const double* A, long lda, const double* x, double beta,
               double* y, double* ws, hipStream_t st);
 // out = R A C with the same scaling, A left unchanged (FP64 or FP32 output); ws holds
-// equilibrate_workspace_doubles(n) doubles
+// equilibrate_workspace_doubles(n) doubles. With W, the FP32 copy writes its first LU panel's columns
+// on st and the rest on W's second stream, recording W.ev_cols: the next lu_factor_f32 on W must
+// then be called with cols_pending.
 size_t equilibrate_workspace_doubles(long n);
 void equilibrate_copy_f64(const double* A, long n, long lda, double* r, double* c, double* out, long ldo,
                           double* ws, hipStream_t st);
 void equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
-                          double* ws, hipStream_t st);
+                          double* ws, hipStream_t st, LuWork* W = nullptr);
 // FP64 -> FP32 copy of a column-major matrix
 void convert_f64_to_f32(const double* A, long n, long lda, float* B, long ldb, hipStream_t st);
 

@@ -1681,14 +1681,21 @@ static void trsm_by_inverse(const T* L, long ldl, long kb, T* B, long ldb, long 
 // stream while the rest of the trailing update runs on the main stream. The two never touch the
 // same columns: the panel works inside columns [k + kb, k + 2 kb), the rest of the update writes
 // columns >= k + 2 kb and reads L21 (columns [k, k + kb)) and U12 (rows [k, k + kb)).
-template <typename T>
-static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
-                     double* f16ws = nullptr) {
+int lu_streams(LuWork& W) {
   if (!W.st2) {
     if (hipStreamCreateWithFlags(&W.st2, hipStreamNonBlocking) != hipSuccess) return -1;
     if (hipEventCreateWithFlags(&W.ev_a, hipEventDisableTiming) != hipSuccess) return -1;
     if (hipEventCreateWithFlags(&W.ev_b, hipEventDisableTiming) != hipSuccess) return -1;
+    if (hipEventCreateWithFlags(&W.ev_r, hipEventDisableTiming) != hipSuccess) return -1;
+    if (hipEventCreateWithFlags(&W.ev_cols, hipEventDisableTiming) != hipSuccess) return -1;
   }
+  return 0;
+}
+
+template <typename T>
+static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
+                     double* f16ws = nullptr, bool cols_pending = false) {
+  if (lu_streams(W)) return -1;
   // f16ws holds two split workspaces: the bulk trailing updates' and the panel's own
   const F16x3Ws wspanel = f16ws ? f16x3_ws(f16ws + f16x3_workspace_doubles(n), n) : F16x3Ws{};
   const F16x3Ws* pw = f16ws ? &wspanel : nullptr;
@@ -1700,6 +1707,7 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
   if (tick) tick(tctx, "lu.panel", 0, st);
   panel_rec(c, 0, std::min(kLuNB, n));
   if (tick) tick(tctx, "lu.panel", 1, st);
+  if (cols_pending) hipStreamWaitEvent(st, W.ev_cols, 0);  // the columns the first panel did not need
   for (long k = 0; k < n; k += kLuNB) {
     const long kb = std::min(kLuNB, n - k);
     if (k > 0) hipStreamWaitEvent(st, W.ev_b, 0);  // panel k, factored on the second stream
@@ -1760,8 +1768,8 @@ int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick
   return lu_factor<double>(A, n, lda, W, st, tick, tctx);
 }
 int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
-                  double* f16ws) {
-  return lu_factor<float>(A, n, lda, W, st, tick, tctx, f16ws);
+                  double* f16ws, bool cols_pending) {
+  return lu_factor<float>(A, n, lda, W, st, tick, tctx, f16ws, cols_pending && n > kLuNB);
 }
 
 // ======================================================================= triangular solves ======
@@ -2137,9 +2145,9 @@ template <typename TO>
 __global__ __launch_bounds__(kColThreads) void k_colscale_copy(const double* __restrict__ A, long n, long lda,
                                                                const double* __restrict__ r,
                                                                double* __restrict__ c, TO* __restrict__ out,
-                                                               long ldo) {
+                                                               long ldo, long j0) {
   __shared__ double red[kColThreads / 64];
-  const long j = blockIdx.x;
+  const long j = j0 + blockIdx.x;
   const double* col = A + j * lda;
   double m = 0.0;
   for (long i = threadIdx.x; i < n; i += kColThreads) m = fmax(m, fabs(col[i] * r[i]));
@@ -2157,21 +2165,36 @@ __global__ __launch_bounds__(kColThreads) void k_colscale_copy(const double* __r
 
 size_t equilibrate_workspace_doubles(long n) { return (size_t)kRowChunks * n; }
 
+// Row maxima, then the scaled copy. With st_rest, only columns [0, cols_first) are copied on st; the
+// rest are copied on st_rest, whose completion ev_rest records.
 template <typename TO>
 static void equilibrate_copy(const double* A, long n, long lda, double* r, double* c, TO* out, long ldo,
-                             double* ws, hipStream_t st) {
+                             double* ws, hipStream_t st, hipStream_t st_rest = nullptr,
+                             hipEvent_t ev_r = nullptr, hipEvent_t ev_rest = nullptr, long cols_first = 0) {
   hipLaunchKernelGGL(k_rowmax_part, dim3((unsigned)((n + 255) / 256), kRowChunks), dim3(256), 0, st, A,
                      n, lda, ws);
   hipLaunchKernelGGL(k_rowmax_done, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ws, n, r);
-  hipLaunchKernelGGL(k_colscale_copy<TO>, dim3((unsigned)n), dim3(kColThreads), 0, st, A, n, lda, r, c, out, ldo);
+  const long first = st_rest && cols_first > 0 && cols_first < n ? cols_first : n;
+  hipLaunchKernelGGL(k_colscale_copy<TO>, dim3((unsigned)first), dim3(kColThreads), 0, st, A, n, lda, r, c,
+                     out, ldo, 0L);
+  if (first < n) {
+    hipEventRecord(ev_r, st);  // the row scales, and ws free again
+    hipStreamWaitEvent(st_rest, ev_r, 0);
+    hipLaunchKernelGGL(k_colscale_copy<TO>, dim3((unsigned)(n - first)), dim3(kColThreads), 0, st_rest, A, n,
+                       lda, r, c, out, ldo, first);
+    hipEventRecord(ev_rest, st_rest);
+  }
 }
 void equilibrate_copy_f64(const double* A, long n, long lda, double* r, double* c, double* out, long ldo,
                           double* ws, hipStream_t st) {
   equilibrate_copy<double>(A, n, lda, r, c, out, ldo, ws, st);
 }
 void equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
-                          double* ws, hipStream_t st) {
-  equilibrate_copy<float>(A, n, lda, r, c, out, ldo, ws, st);
+                          double* ws, hipStream_t st, LuWork* W) {
+  if (W && lu_streams(*W) == 0)  // the columns past the first panel are copied beside it
+    equilibrate_copy<float>(A, n, lda, r, c, out, ldo, ws, st, W->st2, W->ev_r, W->ev_cols, kLuNB);
+  else
+    equilibrate_copy<float>(A, n, lda, r, c, out, ldo, ws, st);
 }
 
 void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStream_t st) {
