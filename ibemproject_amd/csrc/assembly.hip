@@ -52,14 +52,21 @@ __global__ __launch_bounds__(64) void k_qpoints(const double* __restrict__ nodes
 // the potential coefficients, then the weighted tables Psi^w,g / Phi^w,g for the ten weights, all in
 // LDS; then thread = row of the tile: its table positions once (incl_row), and the 60 columns with
 // the column decode resolved at compile time (incl_entry_w), one coalesced column segment per
-// column (rows of consecutive targets are contiguous).
-__global__ __launch_bounds__(256) void k_incl(const double* __restrict__ cen,
-                                              const double* __restrict__ rad, int num, long n,
-                                              long row0, Mat M, int rowmask,
-                                              double* __restrict__ A) {
-  __shared__ double sbasis[kInclTile][4 * kTabN];
-  __shared__ double sPS[kInclTile][kWgt * kPS];
-  __shared__ double sPH[kInclTile][kWgt * kPH];
+// column (rows of consecutive targets are contiguous). The weighted tables replace the basis tables
+// in the same LDS (each thread holds its kInclPer weighted values in registers across a barrier), so
+// that 38 KB per workgroup lets four workgroups share a CU.
+constexpr int kInclThreads = 256;
+constexpr int kInclU = kInclTile * kInclW;                              // weighted values per workgroup
+constexpr int kInclPer = (kInclU + kInclThreads - 1) / kInclThreads;  // per thread
+static_assert(4 * kTabN <= kInclW, "a pair's basis tables fit in its weighted-table slot");
+static_assert(kInclTile * 64 <= kInclThreads, "one wave per target sphere in the entry phase");
+
+__global__ __launch_bounds__(kInclThreads) void k_incl(const double* __restrict__ cen,
+                                                       const double* __restrict__ rad, int num, long n,
+                                                       long row0, Mat M, int rowmask,
+                                                       double* __restrict__ A) {
+  // pair ps: basis tables [4][kTabN] at sU + ps kInclW, later its weighted tables [PS | PH] there
+  __shared__ double sU[kInclU];
   __shared__ double sC[kInclTile][8][4];
   const int h = blockIdx.y;
   const int s0 = blockIdx.x * kInclTile;
@@ -76,7 +83,7 @@ __global__ __launch_bounds__(256) void k_incl(const double* __restrict__ cen,
 #pragma unroll
       for (int b = 0; b < 4; ++b) sC[t][f][b] = C[f][b];
   }
-  for (int u = t; u < kInclTile * kTabN; u += blockDim.x) {
+  for (int u = t; u < kInclTile * kTabN; u += kInclThreads) {
     const int ps = u / kTabN, tt = u % kTabN;
     const int s = s0 + ps;
     if (s >= num) continue;
@@ -87,16 +94,25 @@ __global__ __launch_bounds__(256) void k_incl(const double* __restrict__ cen,
     tdecode(tt, b0, b1, b2);
     double B0, B1, B2, B3;
     rad_basis(P, is_inside(x, a), b0, b1, b2, B0, B1, B2, B3);
-    sbasis[ps][tt] = B0;
-    sbasis[ps][kTabN + tt] = B1;
-    sbasis[ps][2 * kTabN + tt] = B2;
-    sbasis[ps][3 * kTabN + tt] = B3;
+    double* basis = sU + ps * kInclW;
+    basis[tt] = B0;
+    basis[kTabN + tt] = B1;
+    basis[2 * kTabN + tt] = B2;
+    basis[3 * kTabN + tt] = B3;
   }
   __syncthreads();
-  for (int u = t; u < kInclTile * kInclW; u += blockDim.x) {
+  double wv[kInclPer];
+#pragma unroll
+  for (int k = 0; k < kInclPer; ++k) {
+    const int u = t + kInclThreads * k;
     const int ps = u / kInclW, v = u % kInclW;
-    if (s0 + ps >= num) continue;
-    incl_weighted(sbasis[ps], sC[ps], v, sPS[ps], sPH[ps]);
+    wv[k] = (u < kInclU && s0 + ps < num) ? incl_weighted_val(sU + ps * kInclW, sC[ps], v) : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kInclPer; ++k) {
+    const int u = t + kInclThreads * k;
+    if (u < kInclU) sU[u] = wv[k];
   }
   __syncthreads();
   const int ps = t / 64, r = t % 64;
@@ -107,8 +123,8 @@ __global__ __launch_bounds__(256) void k_incl(const double* __restrict__ cen,
   const long row = row0 + 60L * s + r;
   double* Acol = A + (row0 + 60L * h) * n + row;
   const bool self = (s == h);
-  const double* PS = sPS[ps];
-  const double* PH = sPH[ps];
+  const double* PS = sU + ps * kInclW;
+  const double* PH = PS + kWgt * kPS;
   for (int mn = 0; mn < 6; ++mn) {
     const InclCol K = incl_col(R, mn);
 #pragma unroll
@@ -471,7 +487,7 @@ void launch_qpoints(const double* nodes, const int* conn, int NE, double* qp, hi
 void launch_incl(const double* cen, const double* rad, int num, long n, long row0, const Mat& M,
                  int rowmask, double* A, hipStream_t st) {
   dim3 grid((num + kInclTile - 1) / kInclTile, num);
-  hipLaunchKernelGGL(k_incl, grid, dim3(256), 0, st, cen, rad, num, n, row0, M, rowmask, A);
+  hipLaunchKernelGGL(k_incl, grid, dim3(kInclThreads), 0, st, cen, rad, num, n, row0, M, rowmask, A);
 }
 
 void launch_field(const double* nodes, int NN, const double* cen, const double* rad, int num, long n,

@@ -606,8 +606,8 @@ IBEM_HD double fbasis(const double* basis, const double (*C)[4], int f, int b0, 
          C[f][3] * basis[3 * kTabN + t];
 }
 
-// One weighted entry u (0 <= u < kInclW): PS (u < 840) or PH, written to PS / PH.
-IBEM_HD void incl_weighted(const double* basis, const double (*C)[4], int u, double* PS, double* PH) {
+// One weighted entry u (0 <= u < kInclW): PS[u] for u < kWgt kPS, else PH[u - kWgt kPS].
+IBEM_HD double incl_weighted_val(const double* basis, const double (*C)[4], int u) {
   const bool psi = u < kWgt * kPS;
   const int v = psi ? u : u - kWgt * kPS;
   const int per = psi ? kPS : kPH;
@@ -627,10 +627,14 @@ IBEM_HD void incl_weighted(const double* basis, const double (*C)[4], int u, dou
                  c2 + (p == 2) + (q == 2));
     if (p == q) val += fbasis(basis, C, base + 2, c0, c1, c2);
   }
-  if (psi)
-    PS[w * kPS + g] = val;
+  return val;
+}
+IBEM_HD void incl_weighted(const double* basis, const double (*C)[4], int u, double* PS, double* PH) {
+  const double val = incl_weighted_val(basis, C, u);
+  if (u < kWgt * kPS)
+    PS[u] = val;
   else
-    PH[w * kPH + g] = val;
+    PH[u - kWgt * kPS] = val;
 }
 
 // Table positions of one row (i, j, e), computed once and reused for all 60 columns.
