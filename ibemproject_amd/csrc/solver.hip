@@ -1224,65 +1224,106 @@ __global__ __launch_bounds__(kTsRows) void k_ts_apply(T* __restrict__ A, long ld
 // Apply the interchanges ipiv[p0..p1) in order to columns [c0, c0 + nc). One thread per column
 // walking them in order would be bound by its dependent memory round trips (a 1,024-interchange
 // step took ~0.9 ms per call that way on wide trailing matrices). Every workgroup composes
-// ipiv[p0, p0 + m) into moves A[dst] <- A[src] over the positions they touch (an LDS table: the m
-// top positions directly, positions below them through an open-addressing hash), then moves its
+// ipiv[p0, p0 + m) into moves A[dst] <- A[src] over the positions they touch, then moves its
 // columns, kLaswpCG at a time: all reads, a barrier, all writes.
-constexpr int kLaswpH = 2048;  // hash slots: at least twice the positions below the range (<= m)
+//
+// The composition runs in parallel, from ipiv[p0 + i] >= p0 + i (step i swaps row p0 + i with a row
+// at or below it, so row p0 + i is final after step i). With c(j) the original row whose entry sits
+// at row p0 + j just before step j:
+//   c(j) = c(prev(j)) for prev(j) = the last step j' < j with ipiv[p0 + j'] = p0 + j, else p0 + j,
+// so c(j) = p0 + root(j) along the prev chain (pointer jumping, log2 m rounds). Then
+//   row p0 + i ends with c(i) if ipiv[p0 + i] = p0 + i, else with what row q = ipiv[p0 + i] held
+//   just before step i: c(j) for the previous step j < i with the same pivot row q, else q itself;
+//   a row q >= p0 + m ends with c(j) of the last step j with pivot row q.
+// "Previous step with the same pivot row" comes from one bitonic sort of (pivot row, step) keys.
+// About 70 barriers at m = 1024, where composing the interchanges one after another in LDS was a
+// chain of 1024 dependent read-write rounds (~60 us per workgroup).
 constexpr int kLaswpCG = 4;    // columns per read/write round
-// Move list of ipiv[p0, p0 + m) (shared by the workgroup): returns the number of moves
-// A[mdst[q]] <- A[msrc[q]]. Ends with a barrier.
+// Move list of ipiv[p0, p0 + m), m <= IBEM_LU_NB (shared by the workgroup): returns the number of
+// moves A[mdst[q]] <- A[msrc[q]]. Ends with a barrier.
 struct LaswpLds {
-  int hkey[kLaswpH];              // position below the range, or -1
-  int val[IBEM_LU_NB + kLaswpH];  // slot -> original position of the entry there
-  int slotp[IBEM_LU_NB];          // slot of ipiv[p0 + i]
+  unsigned long long key[IBEM_LU_NB];  // (pivot row << 32 | step), sorted
+  int prv[IBEM_LU_NB];                 // prev(j), then root(j) by pointer jumping
+  int pocc[IBEM_LU_NB];                // previous step with the same pivot row, or -1
   int mdst[2 * IBEM_LU_NB], msrc[2 * IBEM_LU_NB];
   int nmv;
 };
+constexpr int kLaswpPer = IBEM_LU_NB / 256;  // steps per thread
+static_assert(IBEM_LU_NB % 256 == 0 && (IBEM_LU_NB & (IBEM_LU_NB - 1)) == 0, "sort and step layout");
 __device__ int laswp_moves(const int* __restrict__ ipiv, long p0, int m, LaswpLds& S) {
   const int t = threadIdx.x;
-  for (int h = t; h < kLaswpH; h += 256) S.hkey[h] = -1;
-  for (int i = t; i < m; i += 256) S.val[i] = (int)(p0 + i);
+  const int b0 = (int)p0;
+  int m2 = 1;
+  while (m2 < m) m2 <<= 1;
+  for (int i = t; i < m2; i += 256) {
+    S.key[i] = i < m ? ((unsigned long long)(unsigned)ipiv[p0 + i] << 32) | (unsigned)i : ~0ull;
+    if (i < m) S.prv[i] = -1;
+  }
   if (t == 0) S.nmv = 0;
   __syncthreads();
   for (int i = t; i < m; i += 256) {
-    const int p = ipiv[p0 + i];
-    int sl = p - (int)p0;
-    if (p >= p0 + m) {
-      unsigned h = ((unsigned)p * 2654435761u) >> 21;  // 11 bits: kLaswpH slots
-      for (;;) {
-        const int old = atomicCAS(&S.hkey[h], -1, p);
-        if (old == -1 || old == p) break;
-        h = (h + 1) & (kLaswpH - 1);
-      }
-      S.val[IBEM_LU_NB + h] = p;
-      sl = IBEM_LU_NB + (int)h;
-    }
-    S.slotp[i] = sl;
+    const int p = (int)(S.key[i] >> 32);
+    if (p > b0 + i && p < b0 + m) atomicMax(&S.prv[p - b0], i);
   }
-  __syncthreads();
-  if (t == 0) {
-    for (int i = 0; i < m; ++i) {
-      const int b = S.slotp[i];
-      if (b != i) {
-        const int va = S.val[i], vb = S.val[b];
-        S.val[i] = vb;
-        S.val[b] = va;
+  // ascending bitonic sort of the keys (m2 - m padding keys sort last)
+  for (int k = 2; k <= m2; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      __syncthreads();
+      for (int u = t; u < (m2 >> 1); u += 256) {
+        const int a = ((u & ~(j - 1)) << 1) | (u & (j - 1)), b = a + j;
+        const unsigned long long ka = S.key[a], kb = S.key[b];
+        if ((ka > kb) == ((a & k) == 0)) {
+          S.key[a] = kb;
+          S.key[b] = ka;
+        }
       }
     }
+  __syncthreads();
+  // roots of the prev chains (prev(j) < j): prv[j] <- root(j)
+  for (int j = t; j < m; j += 256)
+    if (S.prv[j] < 0) S.prv[j] = j;
+  for (int span = 1; span < m; span <<= 1) {
+    __syncthreads();
+    int v[kLaswpPer];
+#pragma unroll
+    for (int u = 0; u < kLaswpPer; ++u) {
+      const int j = t + 256 * u;
+      v[u] = j < m ? S.prv[S.prv[j]] : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kLaswpPer; ++u) {
+      const int j = t + 256 * u;
+      if (j < m) S.prv[j] = v[u];
+    }
   }
   __syncthreads();
-  for (int i = t; i < m; i += 256)
-    if (S.val[i] != p0 + i) {
-      const int q = atomicAdd(&S.nmv, 1);
-      S.mdst[q] = (int)(p0 + i);
-      S.msrc[q] = S.val[i];
+  for (int s = t; s < m; s += 256) {
+    const unsigned long long kk = S.key[s];
+    const int q = (int)(kk >> 32), i = (int)(kk & 0xffffffffull);
+    const bool first = s == 0 || (int)(S.key[s - 1] >> 32) != q;
+    const bool last = s == m - 1 || (int)(S.key[s + 1] >> 32) != q;
+    S.pocc[i] = first ? -1 : (int)(S.key[s - 1] & 0xffffffffull);
+    if (last && q >= b0 + m) {  // a row below the range ends with its last step's entry
+      const int w = atomicAdd(&S.nmv, 1);
+      S.mdst[w] = q;
+      S.msrc[w] = b0 + S.prv[i];
     }
-  for (int h = t; h < kLaswpH; h += 256) {
-    const int k = S.hkey[h];
-    if (k >= 0 && S.val[IBEM_LU_NB + h] != k) {
-      const int q = atomicAdd(&S.nmv, 1);
-      S.mdst[q] = k;
-      S.msrc[q] = S.val[IBEM_LU_NB + h];
+  }
+  __syncthreads();
+  for (int i = t; i < m; i += 256) {
+    const int q = ipiv[p0 + i];
+    int src;
+    if (q == b0 + i) {
+      src = b0 + S.prv[i];
+    } else {
+      const int j = S.pocc[i];
+      src = j >= 0 ? b0 + S.prv[j] : q;
+    }
+    if (src != b0 + i) {
+      const int w = atomicAdd(&S.nmv, 1);
+      S.mdst[w] = b0 + i;
+      S.msrc[w] = src;
     }
   }
   __syncthreads();
