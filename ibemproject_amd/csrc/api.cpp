@@ -661,8 +661,9 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
       // columns past the first panel are copied on the LU's second stream while that panel factors
       {
         Scope s(ctx, "solve.equilibrate");
-        equilibrate_copy_f32(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_A32, ctx->lda32, ctx->d_eqws,
-                             ctx->st, &ctx->W);
+        if (equilibrate_copy_f32(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_A32, ctx->lda32, ctx->d_eqws,
+                                 ctx->st, &ctx->W))
+          return fail(ctx, IBEM3D_EHIP, "equilibration launch failed");
       }
       {
         Scope s(ctx, "solve.lu_f32");

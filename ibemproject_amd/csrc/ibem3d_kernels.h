@@ -104,12 +104,12 @@ void gemv_f64(long m, long n, double alpha, const double* A, long lda, const dou
 // out = R A C with the same scaling, A left unchanged (FP64 or FP32 output); ws holds
 // equilibrate_workspace_doubles(n) doubles. With W, the FP32 copy writes its first LU panel's columns
 // on st and the rest on W's second stream, recording W.ev_cols: the next lu_factor_f32 on W must
-// then be called with cols_pending.
+// then be called with cols_pending. Returns nonzero on a HIP error.
 size_t equilibrate_workspace_doubles(long n);
 void equilibrate_copy_f64(const double* A, long n, long lda, double* r, double* c, double* out, long ldo,
                           double* ws, hipStream_t st);
-void equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
-                          double* ws, hipStream_t st, LuWork* W = nullptr);
+int equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
+                         double* ws, hipStream_t st, LuWork* W = nullptr);
 // FP64 -> FP32 copy of a column-major matrix
 void convert_f64_to_f32(const double* A, long n, long lda, float* B, long ldb, hipStream_t st);
 

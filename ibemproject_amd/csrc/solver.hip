@@ -1748,7 +1748,8 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
   if (tick) tick(tctx, "lu.panel", 0, st);
   panel_rec(c, 0, std::min(kLuNB, n));
   if (tick) tick(tctx, "lu.panel", 1, st);
-  if (cols_pending) hipStreamWaitEvent(st, W.ev_cols, 0);  // the columns the first panel did not need
+  // the columns the first panel did not need (copied on the second stream)
+  if (cols_pending && hipStreamWaitEvent(st, W.ev_cols, 0) != hipSuccess) return -1;
   for (long k = 0; k < n; k += kLuNB) {
     const long kb = std::min(kLuNB, n - k);
     if (k > 0) hipStreamWaitEvent(st, W.ev_b, 0);  // panel k, factored on the second stream
@@ -2207,11 +2208,12 @@ __global__ __launch_bounds__(kColThreads) void k_colscale_copy(const double* __r
 size_t equilibrate_workspace_doubles(long n) { return (size_t)kRowChunks * n; }
 
 // Row maxima, then the scaled copy. With st_rest, only columns [0, cols_first) are copied on st; the
-// rest are copied on st_rest, whose completion ev_rest records.
+// rest are copied on st_rest, whose completion ev_rest records. Returns nonzero when the events
+// that order the two streams could not be recorded or waited on.
 template <typename TO>
-static void equilibrate_copy(const double* A, long n, long lda, double* r, double* c, TO* out, long ldo,
-                             double* ws, hipStream_t st, hipStream_t st_rest = nullptr,
-                             hipEvent_t ev_r = nullptr, hipEvent_t ev_rest = nullptr, long cols_first = 0) {
+static int equilibrate_copy(const double* A, long n, long lda, double* r, double* c, TO* out, long ldo,
+                            double* ws, hipStream_t st, hipStream_t st_rest = nullptr,
+                            hipEvent_t ev_r = nullptr, hipEvent_t ev_rest = nullptr, long cols_first = 0) {
   hipLaunchKernelGGL(k_rowmax_part, dim3((unsigned)((n + 255) / 256), kRowChunks), dim3(256), 0, st, A,
                      n, lda, ws);
   hipLaunchKernelGGL(k_rowmax_done, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ws, n, r);
@@ -2219,23 +2221,24 @@ static void equilibrate_copy(const double* A, long n, long lda, double* r, doubl
   hipLaunchKernelGGL(k_colscale_copy<TO>, dim3((unsigned)first), dim3(kColThreads), 0, st, A, n, lda, r, c,
                      out, ldo, 0L);
   if (first < n) {
-    hipEventRecord(ev_r, st);  // the row scales, and ws free again
-    hipStreamWaitEvent(st_rest, ev_r, 0);
+    if (hipEventRecord(ev_r, st) != hipSuccess) return -1;  // the row scales
+    if (hipStreamWaitEvent(st_rest, ev_r, 0) != hipSuccess) return -1;
     hipLaunchKernelGGL(k_colscale_copy<TO>, dim3((unsigned)(n - first)), dim3(kColThreads), 0, st_rest, A, n,
                        lda, r, c, out, ldo, first);
-    hipEventRecord(ev_rest, st_rest);
+    if (hipEventRecord(ev_rest, st_rest) != hipSuccess) return -1;
   }
+  return 0;
 }
 void equilibrate_copy_f64(const double* A, long n, long lda, double* r, double* c, double* out, long ldo,
                           double* ws, hipStream_t st) {
-  equilibrate_copy<double>(A, n, lda, r, c, out, ldo, ws, st);
+  (void)equilibrate_copy<double>(A, n, lda, r, c, out, ldo, ws, st);
 }
-void equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
-                          double* ws, hipStream_t st, LuWork* W) {
-  if (W && lu_streams(*W) == 0)  // the columns past the first panel are copied beside it
-    equilibrate_copy<float>(A, n, lda, r, c, out, ldo, ws, st, W->st2, W->ev_r, W->ev_cols, kLuNB);
-  else
-    equilibrate_copy<float>(A, n, lda, r, c, out, ldo, ws, st);
+int equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
+                         double* ws, hipStream_t st, LuWork* W) {
+  if (!W) return equilibrate_copy<float>(A, n, lda, r, c, out, ldo, ws, st);
+  if (lu_streams(*W)) return -1;
+  // the columns past the first panel are copied beside it
+  return equilibrate_copy<float>(A, n, lda, r, c, out, ldo, ws, st, W->st2, W->ev_r, W->ev_cols, kLuNB);
 }
 
 void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStream_t st) {
