@@ -2253,13 +2253,44 @@ void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStrea
 constexpr int kGemvChunks = 32;
 size_t gemv_workspace_doubles(long m) { return (size_t)kGemvChunks * m; }
 
+// thread = rows 2 u and 2 u + 1 (one 16-byte load per column when lda is even and A 16-byte aligned,
+// else one row per thread); each row keeps eight partial sums over its columns j = q (mod 8) of the
+// chunk, added in a fixed order
 __global__ __launch_bounds__(256) void k_gemv_part(long m, long n, const double* __restrict__ A,
                                                    long lda, const double* __restrict__ x,
                                                    double* __restrict__ part) {
-  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= m) return;
   const long cw = (n + kGemvChunks - 1) / kGemvChunks;
   const long j0 = (long)blockIdx.y * cw, j1 = min(n, j0 + cw);
+  const bool pairs = (lda & 1) == 0 && (((size_t)A) & 15) == 0;
+  if (pairs) {
+    const long r = 2 * ((long)blockIdx.x * blockDim.x + threadIdx.x);
+    if (r >= m) return;
+    if (r + 1 < m) {
+      double a0[8] = {0, 0, 0, 0, 0, 0, 0, 0}, a1[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      long j = j0;
+      for (; j + 8 <= j1; j += 8) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const double2 v = *(const double2*)(A + (j + q) * lda + r);
+          const double xq = x[j + q];
+          a0[q] += v.x * xq;
+          a1[q] += v.y * xq;
+        }
+      }
+      for (; j < j1; ++j) {
+        const double2 v = *(const double2*)(A + j * lda + r);
+        a0[0] += v.x * x[j];
+        a1[0] += v.y * x[j];
+      }
+      part[(long)blockIdx.y * m + r] = ((a0[0] + a0[1]) + (a0[2] + a0[3])) + ((a0[4] + a0[5]) + (a0[6] + a0[7]));
+      part[(long)blockIdx.y * m + r + 1] = ((a1[0] + a1[1]) + (a1[2] + a1[3])) + ((a1[4] + a1[5]) + (a1[6] + a1[7]));
+      return;
+    }
+    // the last row of an odd m: the single-row path below
+  }
+  const long r = pairs ? 2 * ((long)blockIdx.x * blockDim.x + threadIdx.x)
+                       : (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   long j = j0;
   for (; j + 8 <= j1; j += 8) {
@@ -2282,8 +2313,10 @@ __global__ __launch_bounds__(256) void k_gemv_done(long m, double alpha, const d
 
 void gemv_f64(long m, long n, double alpha, const double* A, long lda, const double* x, double beta,
               double* y, double* ws, hipStream_t st) {
-  hipLaunchKernelGGL(k_gemv_part, dim3((unsigned)((m + 255) / 256), kGemvChunks), dim3(256), 0, st, m,
-                     n, A, lda, x, ws);
+  const bool pairs = (lda & 1) == 0 && (((size_t)A) & 15) == 0;
+  const long rows_per_wg = pairs ? 512 : 256;
+  hipLaunchKernelGGL(k_gemv_part, dim3((unsigned)((m + rows_per_wg - 1) / rows_per_wg), kGemvChunks), dim3(256),
+                     0, st, m, n, A, lda, x, ws);
   hipLaunchKernelGGL(k_gemv_done, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, m, alpha, ws,
                      beta, y);
 }
