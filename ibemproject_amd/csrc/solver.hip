@@ -305,9 +305,19 @@ __global__ __launch_bounds__(256) void k_split_rows(long M, int K, int kpad, con
   const long r0 = (long)blockIdx.x * kSplitRW;
   const int t = threadIdx.x, rl = t & 63, g = t >> 6;
   const long r = r0 + rl;
+  // eight loads in flight per thread (one at a time, each pass would wait a full memory latency)
   float m = 0.0f;
-  if (r < M)
-    for (int k = g; k < K; k += 4) m = fmaxf(m, fabsf(A[(long)k * lda + r]));
+  if (r < M) {
+    int k = g;
+    for (; k + 28 < K; k += 32) {
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = A[(long)(k + 4 * q) * lda + r];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) m = fmaxf(m, fabsf(v[q]));
+    }
+    for (; k < K; k += 4) m = fmaxf(m, fabsf(A[(long)k * lda + r]));
+  }
   red[g][rl] = m;
   __syncthreads();
   if (g == 0) {
@@ -318,6 +328,7 @@ __global__ __launch_bounds__(256) void k_split_rows(long M, int K, int kpad, con
   }
   for (int k0 = 0; k0 < kpad; k0 += kSplitChunk) {
     __syncthreads();  // ssc ready / the previous chunk has been written out
+#pragma unroll
     for (int kk = g; kk < kSplitChunk; kk += 4) {
       const int k = k0 + kk;
       tile[rl][kk] = (r < M && k < K) ? A[(long)k * lda + r] : 0.0f;
