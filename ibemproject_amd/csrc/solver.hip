@@ -227,8 +227,27 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 4 : 2) void k_gemm_skinny(
   T a[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) a[k] = A[(long)k * lda + r];
+  // four columns at a time: their C loads are issued together, ahead of the four dot products
+  // (one column at a time, every column would wait a full memory latency for its C)
+  int j = 0;
 #pragma unroll 1
-  for (int j = 0; j < N; ++j) {
+  for (; j + 4 <= N; j += 4) {
+    T cv[4];
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) cv[jj] = C[(long)(j + jj) * ldc + r];
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      const T* b = sB[j + jj];
+      T p[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+      for (int k = 0; k < K; k += 4)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) p[q] = fma(a[k + q], b[k + q], p[q]);
+      C[(long)(j + jj) * ldc + r] = cv[jj] - ((p[0] + p[1]) + (p[2] + p[3]));
+    }
+  }
+#pragma unroll 1
+  for (; j < N; ++j) {
     const T* b = sB[j];
     T p[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
