@@ -1405,7 +1405,8 @@ template <typename T>
 static void laswp(T* A, long lda, long c0, long nc, const int* ipiv, long p0, long p1,
                          hipStream_t st) {
   if (nc <= 0 || p1 <= p0) return;
-  const long cpb = std::max<long>(kLaswpCG, (nc + 255) / 256 / kLaswpCG * kLaswpCG);
+  // ~1024 workgroups on wide ranges (four per CU: each group's read round trip overlaps the others')
+  const long cpb = std::max<long>(kLaswpCG, (nc + 1023) / 1024 / kLaswpCG * kLaswpCG);
   hipLaunchKernelGGL(k_laswp_mv<T>, dim3((unsigned)((nc + cpb - 1) / cpb)), dim3(256), 0, st, A, lda,
                      c0, nc, ipiv, p0, (int)(p1 - p0), (int)cpb);
 }
