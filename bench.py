@@ -204,7 +204,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this engine solves one RVE on one GPU; --gpus {args.gpus} is not "
-              "supported (a multi-GPU run would shard the LU block-cyclically, see DESIGN.md)",
+              "supported (a multi-GPU run would shard the LU block-cyclically by 1024-column panels)",
               file=sys.stderr)
         sys.exit(2)
 
