@@ -62,6 +62,12 @@ struct ibem3d_ctx {
   std::string err;
   std::vector<TimeEntry> times;
   std::map<std::string, hipEvent_t> open;
+  std::vector<hipEvent_t> pool;             // recorded-and-read events, ready for reuse
+  std::vector<std::string> folded_order;    // per-name totals of intervals already folded
+  std::map<std::string, double> folded_ms;
+  long timing_errors = 0;
+  // stress inputs and output (kept between calls)
+  double *d_stU = nullptr, *d_stT = nullptr, *d_sts = nullptr;
 };
 
 namespace {
@@ -79,23 +85,64 @@ int fail(ibem3d_ctx* c, int code, const std::string& msg) {
       return fail(ctx, IBEM3D_EHIP, std::string(#call) + ": " + hipGetErrorString(_e));    \
   } while (0)
 
+// Phase timing with HIP events on the stream each phase runs on. Events come from a per-context
+// pool; once many intervals are pending, the finished ones are folded into per-name totals and
+// their events returned to the pool, so a long-lived context holds a bounded number of events.
+constexpr size_t kFoldAt = 4096;
+
+hipEvent_t take_event(ibem3d_ctx* c);
+void fold_times(ibem3d_ctx* c);
+
 void tick(void* p, const char* name, int end, hipStream_t s) {
   ibem3d_ctx* c = (ibem3d_ctx*)p;
   if (!s) s = c->st;
   if (!end) {
-    hipEvent_t a;
-    hipEventCreate(&a);
-    hipEventRecord(a, s);
+    hipEvent_t a = take_event(c);
+    if (!a || hipEventRecord(a, s) != hipSuccess) {
+      c->timing_errors++;
+      return;
+    }
     c->open[name] = a;
   } else {
     auto it = c->open.find(name);
     if (it == c->open.end()) return;
-    hipEvent_t b;
-    hipEventCreate(&b);
-    hipEventRecord(b, s);
+    hipEvent_t b = take_event(c);
+    if (!b || hipEventRecord(b, s) != hipSuccess) {
+      c->timing_errors++;
+      c->pool.push_back(it->second);
+      c->open.erase(it);
+      return;
+    }
     c->times.push_back({name, it->second, b});
     c->open.erase(it);
+    if (c->times.size() >= kFoldAt) fold_times(c);
   }
+}
+
+hipEvent_t take_event(ibem3d_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// add the finished intervals to the per-name totals and return their events to the pool
+void fold_times(ibem3d_ctx* c) {
+  hipStreamSynchronize(c->st);
+  if (c->W.st2) hipStreamSynchronize(c->W.st2);
+  for (const auto& t : c->times) {
+    float v = 0;
+    if (hipEventElapsedTime(&v, t.a, t.b) != hipSuccess) c->timing_errors++;
+    if (!c->folded_ms.count(t.name)) c->folded_order.push_back(t.name);
+    c->folded_ms[t.name] += v;
+    c->pool.push_back(t.a);
+    c->pool.push_back(t.b);
+  }
+  c->times.clear();
 }
 
 struct Scope {
@@ -252,7 +299,10 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
   hipSetDevice(ctx->dev);
   if (ctx->st) hipStreamSynchronize(ctx->st);
   ibem3d_reset_timings(ctx);
-  void* ptrs[] = {ctx->d_nodes, ctx->d_cen,      ctx->d_rad,      ctx->d_pts,  ctx->d_qp,
+  for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
+  ctx->pool.clear();
+  void* ptrs[] = {ctx->d_stU,   ctx->d_stT,      ctx->d_sts,
+                  ctx->d_nodes, ctx->d_cen,      ctx->d_rad,      ctx->d_pts,  ctx->d_qp,
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
                   ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32, ctx->d_f16ws, ctx->d_eqws,
@@ -283,8 +333,17 @@ static int after_launch(ibem3d_ctx* ctx, const char* what) {
   return 0;
 }
 
+// The boundary phases accumulate into A and b: a phase that already ran since the last reset would
+// count its entries twice, so it is refused.
+static int phase_guard(ibem3d_ctx* ctx, unsigned bits, const char* what) {
+  if (ctx->phases & bits)
+    return fail(ctx, IBEM3D_ESTATE, std::string(what) + ": phase already ran since the last reset");
+  return 0;
+}
+
 int ibem3d_add_bem(ibem3d_ctx* ctx) {
   CTX_CHECK();
+  if (int rc = phase_guard(ctx, 1u, "add_bem")) return rc;
   {
     Scope s(ctx, "assemble.bem");
     if (!ctx->d_bemws)
@@ -300,6 +359,7 @@ int ibem3d_add_bem(ibem3d_ctx* ctx) {
 
 int ibem3d_add_field(ibem3d_ctx* ctx) {
   CTX_CHECK();
+  if (int rc = phase_guard(ctx, 2u, "add_field")) return rc;
   if (ctx->num > 0) {
     Scope s(ctx, "assemble.field");
     launch_field(ctx->d_nodes, ctx->NN, ctx->d_cen, ctx->d_rad, ctx->num, ctx->n, 3L * ctx->NN,
@@ -314,6 +374,7 @@ int ibem3d_add_incl(ibem3d_ctx* ctx, int order) {
   CTX_CHECK();
   if (order < -1 || order > 2) return fail(ctx, IBEM3D_EINVAL, "add_incl: order must be -1, 0, 1, 2");
   const int mask = order < 0 ? 7 : (1 << order);
+  if (int rc = phase_guard(ctx, (unsigned)mask << 2, "add_incl")) return rc;
   if (ctx->num > 0) {
     Scope s(ctx, "assemble.incl");
     launch_incl(ctx->d_cen, ctx->d_rad, ctx->num, ctx->n, 3L * ctx->NN, ctx->M, mask, ctx->d_A,
@@ -328,6 +389,7 @@ int ibem3d_add_bem_rows(ibem3d_ctx* ctx, int order) {
   CTX_CHECK();
   if (order < -1 || order > 2) return fail(ctx, IBEM3D_EINVAL, "add_bem_rows: order must be -1, 0, 1, 2");
   const int mask = order < 0 ? 7 : (1 << order);
+  if (int rc = phase_guard(ctx, (unsigned)mask << 5, "add_bem_rows")) return rc;
   if (ctx->num > 0) {
     Scope s(ctx, "assemble.rows");
     launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, ctx->n, 3L * ctx->NN, ctx->M,
@@ -891,53 +953,39 @@ int ibem3d_stress(ibem3d_ctx* ctx, double* s) {
       for (int i = 0; i < 12; ++i)
         if (ctx->slot_col[12 * e + i] != -2) Tslot[12 * e + i] = T[m++];
   }
-  double *dU = nullptr, *dT = nullptr, *ds = nullptr;
-  int rc = 0;
-  if ((rc = upload(ctx, &dU, U.data(), 3L * NN)) || (rc = upload(ctx, &dT, Tslot.data(), Tslot.size())) ||
-      (rc = dalloc(ctx, &ds, 6L * ctx->nump))) {
-    hipFree(dU);
-    hipFree(dT);
-    hipFree(ds);
-    return rc;
+  // device copies of U's boundary part, T and the output: allocated on the first call, kept
+  if (!ctx->d_stU) {
+    if (int rc = dalloc(ctx, &ctx->d_stU, 3L * NN)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_stT, Tslot.size())) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_sts, 6L * ctx->nump)) return rc;
   }
+  HIPC(hipMemcpyAsync(ctx->d_stU, U.data(), sizeof(double) * 3L * NN, hipMemcpyHostToDevice, ctx->st));
+  HIPC(hipMemcpyAsync(ctx->d_stT, Tslot.data(), sizeof(double) * Tslot.size(), hipMemcpyHostToDevice,
+                      ctx->st));
   {
     Scope sc(ctx, "post.stress");
     launch_stress(ctx->d_qp, ctx->d_conn, ctx->d_nodes, NN, NE, ctx->d_cen, ctx->d_rad, ctx->num,
-                  ctx->d_pts, ctx->nump, ctx->M, dU, dT, ctx->d_x + 3L * NN, ds, ctx->st);
+                  ctx->d_pts, ctx->nump, ctx->M, ctx->d_stU, ctx->d_stT, ctx->d_x + 3L * NN,
+                  ctx->d_sts, ctx->st);
   }
-  rc = after_launch(ctx, "k_stress");
-  if (!rc) {
-    hipError_t e = hipMemcpyAsync(s, ds, sizeof(double) * 6 * ctx->nump, hipMemcpyDeviceToHost, ctx->st);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
-    if (e != hipSuccess) rc = fail(ctx, IBEM3D_EHIP, std::string("stress copy: ") + hipGetErrorString(e));
-  }
-  hipFree(dU);
-  hipFree(dT);
-  hipFree(ds);
-  return rc;
+  if (int rc = after_launch(ctx, "k_stress")) return rc;
+  HIPC(hipMemcpyAsync(s, ctx->d_sts, sizeof(double) * 6 * ctx->nump, hipMemcpyDeviceToHost, ctx->st));
+  HIPC(hipStreamSynchronize(ctx->st));
+  return 0;
 }
 
 int ibem3d_timings(const ibem3d_ctx* ctx, char* names, int cap, double* ms, int max_entries) {
   if (!ctx) return IBEM3D_EINVAL;
-  hipStreamSynchronize(ctx->st);
-  // aggregate by name, in first-seen order
-  std::vector<std::string> order;
-  std::map<std::string, double> tot;
-  for (const auto& t : ctx->times) {
-    float v = 0;
-    hipEventElapsedTime(&v, t.a, t.b);
-    if (!tot.count(t.name)) order.push_back(t.name);
-    tot[t.name] += v;
-  }
+  fold_times(const_cast<ibem3d_ctx*>(ctx));
   int cnt = 0, pos = 0;
-  for (const auto& nm : order) {
+  for (const auto& nm : ctx->folded_order) {
     if (cnt >= max_entries) break;
     if (names) {
       if (pos + (int)nm.size() + 1 > cap) break;
       std::memcpy(names + pos, nm.c_str(), nm.size() + 1);
       pos += (int)nm.size() + 1;
     }
-    if (ms) ms[cnt] = tot[nm];
+    if (ms) ms[cnt] = ctx->folded_ms.at(nm);
     ++cnt;
   }
   return cnt;
@@ -945,14 +993,11 @@ int ibem3d_timings(const ibem3d_ctx* ctx, char* names, int cap, double* ms, int 
 
 void ibem3d_reset_timings(ibem3d_ctx* ctx) {
   if (!ctx) return;
-  hipStreamSynchronize(ctx->st);
-  for (auto& t : ctx->times) {
-    hipEventDestroy(t.a);
-    hipEventDestroy(t.b);
-  }
-  ctx->times.clear();
-  for (auto& kv : ctx->open) hipEventDestroy(kv.second);
+  fold_times(ctx);
+  for (auto& kv : ctx->open) ctx->pool.push_back(kv.second);
   ctx->open.clear();
+  ctx->folded_order.clear();
+  ctx->folded_ms.clear();
 }
 
 }  // extern "C"

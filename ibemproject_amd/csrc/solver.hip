@@ -1,10 +1,10 @@
 // Dense LU solver of the iBEM system on gfx950 (replaces Eigen FullPivLU, systemSolverElastic3D.h:55).
 //
 // Blocked right-looking LU with row pivoting, column-major, in place:
-//   for each 256-wide panel: recursive panel factorisation (leaves of 32 columns factored with
-//   tournament pivoting in two launches), row interchanges on the other columns, recursive
-//   unit-lower TRSM, and the
-//   trailing-matrix update C -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64, or f32 for mixed precision).
+//   for each IBEM_LU_NB-wide (1024) panel: recursive panel factorisation (leaves of 32 columns
+//   factored with tournament pivoting in two launches), row interchanges on the other columns,
+//   unit-lower TRSM, and the trailing-matrix update C -= L21 U12 on MFMA
+//   (v_mfma_f64_16x16x4f64, or FP32 / split or single FP16 / BF16 products for mixed precision).
 // The trailing update holds 2n^3/3 of the flops; everything else is memory- or latency-bound.
 #include <hip/hip_runtime.h>
 
@@ -1075,8 +1075,11 @@ __global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, lo
       ws.cidx[slot * kPanelW + mystep] = gidx;
     }
     if (t >= nsel && t < kPanelW) ws.cidx[slot * kPanelW + t] = -1;
+    // release: every wavefront waits for its own candidate stores and writes them back to the
+    // device-coherent level before the barrier, so thread 0's arrival count below is ordered after
+    // all of them (the per-XCD L2s are not coherent with each other)
+    __threadfence();
     __syncthreads();
-    if (t == 0) __threadfence();  // release: the workgroup's candidates, before the arrival count
     const int group = node / kTsFan;
     const int gsize = min(kTsFan, count - group * kTsFan);
     const long next_off = off + count;
@@ -1086,10 +1089,8 @@ __global__ __launch_bounds__(kTsRows) void k_ts_tree(const T* __restrict__ A, lo
     }
     __syncthreads();
     if (!s_last) return;
-    if (t == 0) {
-      __threadfence();  // acquire: the group's candidates
-      ws.cnt[next_off + group] = 0;  // ready for the next leaf (a later launch)
-    }
+    __threadfence();  // acquire, in every wavefront: the group's candidates
+    if (t == 0) ws.cnt[next_off + group] = 0;  // ready for the next leaf (a later launch)
     __syncthreads();
     const int child = group * kTsFan + t / kPanelW, k = t % kPanelW;
     valid = (t / kPanelW) < gsize && k < w;
