@@ -53,7 +53,13 @@ struct ibem3d_ctx {
   double *d_gV = nullptr, *d_gZ = nullptr, *d_gpart = nullptr, *d_gh = nullptr;
   double* d_bemws = nullptr;  // boundary-boundary quadrature values of one round of elements
   double* d_gemvws = nullptr; // residual GEMV partial sums
-  int prepared_mode = -1;      // mode whose factors and solve data are in place (-1: none)
+  int prepared_mode = -1;      // mode (with IBEM3D_SOLVE_COMPACT) whose factors are in place (-1: none)
+  // compact storage: 16-bit copy of the FP32 factor (leading dimension ldf) and its tile scales
+  unsigned short* d_F16 = nullptr;
+  float* d_F16s = nullptr;
+  long ldf = 0;
+  bool f16_bf = false;         // the copy holds BF16 values (else FP16)
+  bool a_factored = false;     // d_A holds a factor in place of the assembled system
   LuWork W;
   unsigned phases = 0;
   bool solved = false;
@@ -306,6 +312,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
                   ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32, ctx->d_f16ws, ctx->d_eqws,
+                  ctx->d_F16,   ctx->d_F16s,
                   ctx->d_gV,    ctx->d_gZ,       ctx->d_gpart,    ctx->d_gh,   ctx->d_bemws, ctx->d_gemvws,
                   ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info, ctx->W.d_perm, ctx->W.d_inv, ctx->W.d_tmp,
                   ctx->W.d_trinv, ctx->W.d_trsm};
@@ -429,11 +436,30 @@ int ibem3d_reset(ibem3d_ctx* ctx) {
   HIPC(hipMemsetAsync(ctx->d_b, 0, sizeof(double) * n, ctx->st));
   ctx->phases = 0;
   ctx->solved = false;
+  if (ctx->a_factored) {
+    // a factor in A's memory is overwritten by the next assembly
+    if (ctx->prepared_mode == (IBEM3D_SOLVE_FP64_LU | IBEM3D_SOLVE_COMPACT)) ctx->prepared_mode = -1;
+    ctx->a_factored = false;
+  }
+  return 0;
+}
+
+// Assemble the system again over A's memory (compact storage: A was overwritten by a factor), with
+// the solve state kept. The kernels are deterministic, so A and b are the ones assembled before.
+static int reassemble(ibem3d_ctx* ctx) {
+  const bool solved = ctx->solved;
+  const int prepared = ctx->prepared_mode;
+  if (int rc = ibem3d_reset(ctx)) return rc;
+  if (int rc = ibem3d_assemble(ctx)) return rc;
+  ctx->solved = solved;
+  ctx->prepared_mode = prepared == (IBEM3D_SOLVE_FP64_LU | IBEM3D_SOLVE_COMPACT) ? -1 : prepared;
   return 0;
 }
 
 int ibem3d_get_system(ibem3d_ctx* ctx, double* A, double* b) {
   CTX_CHECK();
+  if (ctx->a_factored)
+    return fail(ctx, IBEM3D_ESTATE, "get_system: A holds a factor (compact solve); assemble again");
   const long n = ctx->n;
   if (A) HIPC(hipMemcpyAsync(A, ctx->d_A, sizeof(double) * n * n, hipMemcpyDeviceToHost, ctx->st));
   if (b) HIPC(hipMemcpyAsync(b, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
@@ -480,7 +506,7 @@ __global__ void k_axpy_scaled_f32(double* x, const float* d, const double* s, lo
 static dim3 vgrid(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 // ---- GMRES-IR building blocks (FP64; reductions in a fixed order, so results are reproducible) ----
-constexpr int kGmres = 20;      // Krylov vectors per cycle
+constexpr int kGmres = 40;      // Krylov vectors per cycle
 constexpr int kDotBlocks = 128; // partial sums per dot product
 
 __global__ void k_scale_from_f32(double* y, const float* x, const double* s, long n) {
@@ -532,6 +558,15 @@ static int dots(ibem3d_ctx* ctx, const double* V, int k, const double* w, double
     h[i] = acc;
   }
   return 0;
+}
+
+// v <- (LU)^-1 v with the factors of the prepared mixed mode: the FP32 factor, or its 16-bit copy
+// (compact storage)
+static void precond_solve(ibem3d_ctx* ctx, float* v) {
+  if (ctx->prepared_mode & IBEM3D_SOLVE_COMPACT)
+    lu_solve_16(ctx->d_F16, ctx->n, ctx->ldf, ctx->d_F16s, ctx->f16_bf, ctx->W, v, ctx->st);
+  else
+    lu_solve_f32(ctx->d_A32, ctx->n, ctx->lda32, ctx->W, v, ctx->st);
 }
 
 // relative residual ||b - A x|| / ||b|| with the FP64 system matrix
@@ -594,7 +629,7 @@ static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0) {
       {
         Scope s(ctx, "solve.trsv_f32");
         hipLaunchKernelGGL(k_scale_to_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_v32, vj, ctx->d_r, n);
-        lu_solve_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->d_v32, ctx->st);
+        precond_solve(ctx, ctx->d_v32);
         hipLaunchKernelGGL(k_scale_from_f32, vgrid(n), dim3(256), 0, ctx->st, zj, ctx->d_v32, ctx->d_c, n);
       }
       {
@@ -659,130 +694,224 @@ static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0) {
   return 0;
 }
 
+// Free device memory is checked before the second n x n copy: without room for it the factor takes
+// the place of A (IBEM3D_SOLVE_COMPACT). kReserve covers the workspaces allocated beside it.
+constexpr size_t kReserve = (size_t)4 << 30;
+static bool fits(size_t bytes) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return true;  // hipMalloc decides
+  return bytes + kReserve <= fr;
+}
+
+static int check_info(ibem3d_ctx* ctx, const char* what) {
+  int info = 0;
+  HIPC(hipMemcpyAsync(&info, ctx->W.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  HIPC(hipStreamSynchronize(ctx->st));
+  if (info) return fail(ctx, IBEM3D_ESOLVE, std::string(what) + ": zero pivot at column " + std::to_string(info - 1));
+  return 0;
+}
+
+// classic refinement x <- x + C LU^-1 (R r) from x = 0; GMRES-IR takes over from the current x when
+// an iteration gains less than a factor of 10 (from zero when it diverges)
+static int classic_ir(ibem3d_ctx* ctx, double tol) {
+  const long n = ctx->n;
+  HIPC(hipMemsetAsync(ctx->d_x, 0, sizeof(double) * n, ctx->st));
+  HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
+  double rel = 1.0;
+  const int max_it = 30;
+  for (int it = 0; it < max_it; ++it) {
+    {
+      Scope s(ctx, "solve.trsv_f32");
+      hipLaunchKernelGGL(k_scale_to_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_v32, ctx->d_res,
+                         ctx->d_r, n);
+      precond_solve(ctx, ctx->d_v32);
+      hipLaunchKernelGGL(k_axpy_scaled_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_v32,
+                         ctx->d_c, n);
+    }
+    if (int rc = after_launch(ctx, "mixed refinement")) return rc;
+    double rel_new = 0.0;
+    if (int rc = residual(ctx, &rel_new)) return rc;
+    ctx->iters = it + 1;
+    ctx->rel_res = rel_new;
+    if (!(rel_new == rel_new)) return fail(ctx, IBEM3D_ESOLVE, "refinement diverged (NaN)");
+    if (rel_new <= tol) {
+      ctx->solved = true;
+      return 0;
+    }
+    if (rel_new > 0.1 * rel) {
+      if (rel_new > rel) return gmres_ir(ctx, tol, false, 1.0);  // diverging: restart from zero
+      return gmres_ir(ctx, tol, true, rel_new);
+    }
+    rel = rel_new;
+  }
+  ctx->solved = true;
+  return fail(ctx, IBEM3D_ESOLVE, "refinement stopped at relative residual " + std::to_string(rel));
+}
+
+static int solve_fp64(ibem3d_ctx* ctx, bool reuse, bool compact) {
+  const long n = ctx->n;
+  if (!reuse) {
+    ctx->prepared_mode = -1;
+    double* F = ctx->d_A;
+    if (compact) {
+      // R A C in place; A is gone until the next assembly
+      Scope s(ctx, "solve.equilibrate");
+      equilibrate_in_place_f64(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_eqws, ctx->st);
+      ctx->a_factored = true;
+    } else {
+      // keep A for the residual; factor an equilibrated copy R A C
+      if (!ctx->d_Af)
+        if (int rc = dalloc(ctx, &ctx->d_Af, (size_t)n * n)) return rc;
+      Scope s(ctx, "solve.equilibrate");
+      equilibrate_copy_f64(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_Af, n, ctx->d_eqws, ctx->st);
+      F = ctx->d_Af;
+    }
+    {
+      Scope s(ctx, "solve.lu_f64");
+      if (lu_factor_f64(F, n, n, ctx->W, ctx->st, tick, ctx)) return fail(ctx, IBEM3D_EHIP, "LU launch failed");
+    }
+    if (int rc = check_info(ctx, "LU")) return rc;
+    {
+      Scope s(ctx, "solve.prepare");
+      if (lu_prepare_solve_f64(F, n, n, ctx->W, ctx->st)) return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
+    }
+    ctx->prepared_mode = IBEM3D_SOLVE_FP64_LU | (compact ? IBEM3D_SOLVE_COMPACT : 0);
+  }
+  const double* F = (ctx->prepared_mode & IBEM3D_SOLVE_COMPACT) ? ctx->d_A : ctx->d_Af;
+  {
+    Scope s(ctx, "solve.trsv_f64");
+    hipLaunchKernelGGL(k_scale_vec, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_b, ctx->d_r, n);
+    lu_solve_f64(F, n, n, ctx->W, ctx->d_x, ctx->st);
+    hipLaunchKernelGGL(k_scale_vec, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_x, ctx->d_c, n);
+  }
+  if (int rc = after_launch(ctx, "fp64 solve")) return rc;
+  ctx->solved = true;
+  if (ctx->a_factored) {
+    // compact: A holds the factor; ibem3d_residual assembles it again for the residual
+    HIPC(hipStreamSynchronize(ctx->st));
+    ctx->rel_res = -1.0;
+    return 0;
+  }
+  return residual(ctx, &ctx->rel_res);
+}
+
+static int solve_mixed(ibem3d_ctx* ctx, int mode, double tol, bool gmres_first, bool reuse, bool compact) {
+  const long n = ctx->n;
+  if (!(tol > 0)) tol = 1e-10;
+  const int prod = mode == IBEM3D_SOLVE_MIXED_FP16_1 ? kProdF16
+                   : mode == IBEM3D_SOLVE_MIXED_BF16 ? kProdBF16 : kProdF16x3;
+  const bool split = mode != IBEM3D_SOLVE_MIXED_FP32;
+  ctx->lda32 = (n + 63) / 64 * 64;
+  if (!ctx->d_v32)
+    if (int rc = dalloc(ctx, &ctx->d_v32, (size_t)n)) return rc;
+  if (!reuse) {
+    ctx->prepared_mode = -1;
+    // the split workspace also stages the first columns of the in-place FP32 conversion
+    if ((split || compact) && !ctx->d_f16ws)
+      if (int rc = dalloc(ctx, &ctx->d_f16ws, lu_f16_workspace_doubles(n))) return rc;
+    float* F32 = nullptr;
+    if (compact) {
+      F32 = (float*)ctx->d_A;
+      Scope s(ctx, "solve.equilibrate");
+      const size_t tmp_floats = lu_f16_workspace_doubles(n) * 2;
+      ctx->a_factored = true;
+      if (equilibrate_to_f32_in_place(ctx->d_A, n, ctx->d_r, ctx->d_c, ctx->lda32, (float*)ctx->d_f16ws,
+                                      tmp_floats, ctx->d_eqws, ctx->st))
+        return fail(ctx, IBEM3D_EHIP, "in-place FP32 conversion failed");
+    } else {
+      if (!ctx->d_A32)
+        if (int rc = dalloc(ctx, &ctx->d_A32, (size_t)ctx->lda32 * n)) return rc;
+      F32 = ctx->d_A32;
+      // R A C straight from A into the FP32 factor storage (A itself stays for the residual); columns
+      // past the first panel are copied on the LU's second stream while that panel factors
+      Scope s(ctx, "solve.equilibrate");
+      if (equilibrate_copy_f32(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_A32, ctx->lda32, ctx->d_eqws,
+                               ctx->st, &ctx->W))
+        return fail(ctx, IBEM3D_EHIP, "equilibration launch failed");
+    }
+    {
+      Scope s(ctx, "solve.lu_f32");
+      if (lu_factor_f32(F32, n, ctx->lda32, ctx->W, ctx->st, tick, ctx, split ? ctx->d_f16ws : nullptr,
+                        !compact, prod))
+        return fail(ctx, IBEM3D_EHIP, "LU launch failed");
+    }
+    if (int rc = check_info(ctx, "LU(fp32)")) return rc;
+    {
+      Scope s(ctx, "solve.prepare");
+      if (lu_prepare_solve_f32(F32, n, ctx->lda32, ctx->W, ctx->st))
+        return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
+    }
+    if (compact) {
+      // the factor in 16 bits beside A, then A assembled again over the FP32 factor
+      ctx->ldf = ctx->lda32;
+      if (!ctx->d_F16) {
+        if (int rc = dalloc(ctx, &ctx->d_F16, (size_t)ctx->ldf * n)) return rc;
+        if (int rc = dalloc(ctx, &ctx->d_F16s, factor16_scales(n))) return rc;
+      }
+      ctx->f16_bf = mode == IBEM3D_SOLVE_MIXED_BF16;
+      {
+        Scope s(ctx, "solve.factor16");
+        factor_to_16(F32, n, ctx->lda32, ctx->d_F16, ctx->ldf, ctx->d_F16s, ctx->f16_bf, ctx->st);
+      }
+      if (int rc = after_launch(ctx, "factor16")) return rc;
+      ctx->prepared_mode = mode | IBEM3D_SOLVE_COMPACT;
+      {
+        Scope s(ctx, "solve.reassemble");
+        if (int rc = reassemble(ctx)) return rc;
+      }
+    } else {
+      ctx->prepared_mode = mode;
+    }
+  }
+  // a 16-bit factor leaves a contraction too weak for classic refinement: GMRES-IR from the start
+  if (gmres_first || (ctx->prepared_mode & IBEM3D_SOLVE_COMPACT)) return gmres_ir(ctx, tol, false, 1.0);
+  return classic_ir(ctx, tol);
+}
+
 int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
   CTX_CHECK();
   const bool gmres_first = (mode & IBEM3D_SOLVE_GMRES) != 0;
   const bool reuse = (mode & IBEM3D_SOLVE_REUSE) != 0;
-  mode &= ~(IBEM3D_SOLVE_GMRES | IBEM3D_SOLVE_REUSE);
-  if (reuse && ctx->prepared_mode != mode)
-    return fail(ctx, IBEM3D_ESTATE, "solve: no factors of this mode to reuse");
-  if (ctx->phases != 0xFFu) return fail(ctx, IBEM3D_ESTATE, "solve: system not assembled");
-  if (int rc = ensure_solver_ws(ctx)) return rc;
+  bool compact = (mode & IBEM3D_SOLVE_COMPACT) != 0;
+  mode &= ~(IBEM3D_SOLVE_GMRES | IBEM3D_SOLVE_REUSE | IBEM3D_SOLVE_COMPACT);
+  if (mode < IBEM3D_SOLVE_FP64_LU || mode > IBEM3D_SOLVE_MIXED_BF16)
+    return fail(ctx, IBEM3D_EINVAL, "solve: unknown mode");
   const long n = ctx->n;
+  if (reuse) {
+    if (ctx->prepared_mode < 0 || (ctx->prepared_mode & ~IBEM3D_SOLVE_COMPACT) != mode)
+      return fail(ctx, IBEM3D_ESTATE, "solve: no factors of this mode to reuse");
+    compact = (ctx->prepared_mode & IBEM3D_SOLVE_COMPACT) != 0;
+  } else if (!compact) {
+    // the second n x n copy (FP64 factor, or FP32 factor with its 64-aligned columns)
+    const size_t need = mode == IBEM3D_SOLVE_FP64_LU ? (ctx->d_Af ? 0 : sizeof(double) * (size_t)n * n)
+                                                     : (ctx->d_A32 ? 0 : sizeof(float) * (size_t)((n + 63) / 64 * 64) * n);
+    compact = need > 0 && !fits(need);
+  }
+  if (ctx->phases != 0xFFu) return fail(ctx, IBEM3D_ESTATE, "solve: system not assembled");
+  if (ctx->a_factored && !(reuse && mode == IBEM3D_SOLVE_FP64_LU))
+    return fail(ctx, IBEM3D_ESTATE, "solve: A holds a factor (compact solve); assemble again");
+  if (int rc = ensure_solver_ws(ctx)) return rc;
   ctx->iters = 0;
   ctx->rel_res = -1.0;
-  if (mode == IBEM3D_SOLVE_FP64_LU) {
-    if (!reuse) {
-      // keep A for the residual; factor an equilibrated copy R A C
-      ctx->prepared_mode = -1;
-      if (!ctx->d_Af)
-        if (int rc = dalloc(ctx, &ctx->d_Af, (size_t)n * n)) return rc;
-      {
-        Scope s(ctx, "solve.equilibrate");
-        equilibrate_copy_f64(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_Af, n, ctx->d_eqws, ctx->st);
-      }
-      {
-        Scope s(ctx, "solve.lu_f64");
-        if (lu_factor_f64(ctx->d_Af, n, n, ctx->W, ctx->st, tick, ctx))
-          return fail(ctx, IBEM3D_EHIP, "LU launch failed");
-      }
-      int info = 0;
-      HIPC(hipMemcpyAsync(&info, ctx->W.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
-      HIPC(hipStreamSynchronize(ctx->st));
-      if (info) return fail(ctx, IBEM3D_ESOLVE, "LU: zero pivot at column " + std::to_string(info - 1));
-      {
-        Scope s(ctx, "solve.prepare");
-        if (lu_prepare_solve_f64(ctx->d_Af, n, n, ctx->W, ctx->st))
-          return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
-      }
-      ctx->prepared_mode = mode;
-    }
-    {
-      Scope s(ctx, "solve.trsv_f64");
-      hipLaunchKernelGGL(k_scale_vec, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_b, ctx->d_r, n);
-      lu_solve_f64(ctx->d_Af, n, n, ctx->W, ctx->d_x, ctx->st);
-      hipLaunchKernelGGL(k_scale_vec, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_x, ctx->d_c, n);
-    }
-    if (int rc = after_launch(ctx, "fp64 solve")) return rc;
-    if (int rc = residual(ctx, &ctx->rel_res)) return rc;
-    ctx->solved = true;
-    return 0;
+  if (mode == IBEM3D_SOLVE_FP64_LU) return solve_fp64(ctx, reuse, compact);
+  return solve_mixed(ctx, mode, tol, gmres_first, reuse, compact);
+}
+
+// Relative residual of the current solution against the assembled FP64 system; after a compact
+// FP64 solve (A overwritten by its factor) the system is assembled again first.
+int ibem3d_residual(ibem3d_ctx* ctx, double* rel) {
+  CTX_CHECK();
+  if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "residual: no solution");
+  if (ctx->a_factored) {
+    Scope s(ctx, "solve.reassemble");
+    if (int rc = reassemble(ctx)) return rc;
   }
-  if (mode == IBEM3D_SOLVE_MIXED_FP32 || mode == IBEM3D_SOLVE_MIXED_FP16) {
-    if (!(tol > 0)) tol = 1e-10;
-    if (!ctx->d_A32) {
-      ctx->lda32 = (n + 63) / 64 * 64;
-      if (int rc = dalloc(ctx, &ctx->d_A32, (size_t)ctx->lda32 * n)) return rc;
-      if (int rc = dalloc(ctx, &ctx->d_v32, (size_t)n)) return rc;
-    }
-    if (!reuse) {
-      ctx->prepared_mode = -1;
-      // R A C straight from A into the FP32 factor storage (A itself stays for the residual)
-      if (mode == IBEM3D_SOLVE_MIXED_FP16 && !ctx->d_f16ws)
-        if (int rc = dalloc(ctx, &ctx->d_f16ws, lu_f16_workspace_doubles(n))) return rc;
-      // columns past the first panel are copied on the LU's second stream while that panel factors
-      {
-        Scope s(ctx, "solve.equilibrate");
-        if (equilibrate_copy_f32(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_A32, ctx->lda32, ctx->d_eqws,
-                                 ctx->st, &ctx->W))
-          return fail(ctx, IBEM3D_EHIP, "equilibration launch failed");
-      }
-      {
-        Scope s(ctx, "solve.lu_f32");
-        if (lu_factor_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->st, tick, ctx,
-                          mode == IBEM3D_SOLVE_MIXED_FP16 ? ctx->d_f16ws : nullptr, true))
-          return fail(ctx, IBEM3D_EHIP, "LU launch failed");
-      }
-      int info = 0;
-      HIPC(hipMemcpyAsync(&info, ctx->W.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
-      HIPC(hipStreamSynchronize(ctx->st));
-      if (info)
-        return fail(ctx, IBEM3D_ESOLVE, "LU(fp32): zero pivot at column " + std::to_string(info - 1));
-      {
-        Scope s(ctx, "solve.prepare");
-        if (lu_prepare_solve_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->st))
-          return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
-      }
-      ctx->prepared_mode = mode;
-    }
-    if (gmres_first) return gmres_ir(ctx, tol, false, 1.0);
-    // classic refinement x <- x + C LU32^-1 (R r); GMRES-IR takes over from the current x when an
-    // iteration gains less than a factor of 10
-    HIPC(hipMemsetAsync(ctx->d_x, 0, sizeof(double) * n, ctx->st));
-    HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
-    double rel = 1.0;
-    const int max_it = 30;
-    for (int it = 0; it < max_it; ++it) {
-      {
-        Scope s(ctx, "solve.trsv_f32");
-        hipLaunchKernelGGL(k_scale_to_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_v32, ctx->d_res,
-                           ctx->d_r, n);
-        lu_solve_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->d_v32, ctx->st);
-        hipLaunchKernelGGL(k_axpy_scaled_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_v32,
-                           ctx->d_c, n);
-      }
-      if (int rc = after_launch(ctx, "mixed refinement")) return rc;
-      double rel_new = 0.0;
-      if (int rc = residual(ctx, &rel_new)) return rc;
-      ctx->iters = it + 1;
-      ctx->rel_res = rel_new;
-      if (!(rel_new == rel_new)) return fail(ctx, IBEM3D_ESOLVE, "refinement diverged (NaN)");
-      if (rel_new <= tol) {
-        ctx->solved = true;
-        return 0;
-      }
-      if (rel_new > 0.1 * rel) {
-        if (rel_new > rel) {  // diverging: restart from zero
-          return gmres_ir(ctx, tol, false, 1.0);
-        }
-        return gmres_ir(ctx, tol, true, rel_new);
-      }
-      rel = rel_new;
-    }
-    ctx->solved = true;
-    return fail(ctx, IBEM3D_ESOLVE, "refinement stopped at relative residual " + std::to_string(rel));
-  }
-  return fail(ctx, IBEM3D_EINVAL, "solve: unknown mode");
+  if (int rc = ensure_solver_ws(ctx)) return rc;
+  double r = 0.0;
+  if (int rc = residual(ctx, &r)) return rc;
+  ctx->rel_res = r;
+  if (rel) *rel = r;
+  return 0;
 }
 
 int ibem3d_solve_info(const ibem3d_ctx* ctx, double* rel, int* iters) {

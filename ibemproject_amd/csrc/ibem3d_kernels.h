@@ -43,11 +43,15 @@ void gemm_f64_sub(long M, long N, long K, const double* A, long lda, const doubl
 // same in FP32 (FP32 MFMA)
 void gemm_f32_sub(long M, long N, long K, const float* A, long lda, const float* B, long ldb,
                   float* C, long ldc, hipStream_t st);
-// same in FP32 arithmetic from three half-precision MFMA products per block (K <= 256); ws holds
+// Product types of the split-operand GEMMs: three FP16 MFMA products of hi/lo splits (FP32-level
+// accuracy), one FP16 product, or one BF16 product, each of operands whose rows (A) and columns (B)
+// are scaled by powers of two that bring their largest magnitude into [2^14, 2^15).
+enum { kProdF16x3 = 0, kProdF16 = 1, kProdBF16 = 2 };
+// same in FP32 arithmetic from 16-bit MFMA products (K <= 1024); ws holds
 // f16x3_workspace_doubles(max(M, N)) doubles
 size_t f16x3_workspace_doubles(long n);
 void gemm_f32_sub_f16x3(long M, long N, int K, const float* A, long lda, const float* B, long ldb,
-                        float* C, long ldc, double* ws, hipStream_t st);
+                        float* C, long ldc, double* ws, hipStream_t st, int prod = kProdF16x3);
 
 // Blocked right-looking LU with row pivoting (tournament pivoting inside 32-column leaves), in
 // place; ipiv[k] = row swapped with row k (absolute, 0-based, applied in order). The block column
@@ -78,13 +82,14 @@ size_t lu_moves_ints(long n);
 size_t lu_trinv_doubles();
 size_t lu_trsm_doubles(long n);
 int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx);
-// f16ws != nullptr: the trailing updates and the panels' deep updates run on split FP16 products
-// (workspace of lu_f16_workspace_doubles(n) doubles); otherwise on FP32 MFMA
+// f16ws != nullptr: the trailing updates run on 16-bit MFMA products of type prod (kProd*) and the
+// panels' deep updates on split FP16 products (workspace of lu_f16_workspace_doubles(n) doubles);
+// otherwise both run on FP32 MFMA
 size_t lu_f16_workspace_doubles(long n);
 // cols_pending: columns past the first panel are still being copied (ev_cols, see
 // equilibrate_copy_f32); the factorisation waits for them after its first panel
 int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx, double* f16ws,
-                  bool cols_pending = false);
+                  bool cols_pending = false, int prod = kProdF16x3);
 // the LU's second stream and events (created once)
 int lu_streams(LuWork& W);
 // After a factorisation: each step's interchanges as a move list and the diagonal-block inverses
@@ -110,6 +115,22 @@ void equilibrate_copy_f64(const double* A, long n, long lda, double* r, double* 
                           double* ws, hipStream_t st);
 int equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
                          double* ws, hipStream_t st, LuWork* W = nullptr);
+// Compact storage (the factor takes the place of the FP64 system matrix): A <- R A C in place, or
+// FP64 A (leading dimension n) -> FP32 R A C in the same memory with leading dimension lda32
+// (n <= lda32 < 2n), the first columns staged through tmp (at least two columns of lda32 floats).
+// Return nonzero on a HIP error or too small a tmp.
+void equilibrate_in_place_f64(double* A, long n, long lda, double* r, double* c, double* ws, hipStream_t st);
+int equilibrate_to_f32_in_place(double* A, long n, double* r, double* c, long lda32, float* tmp,
+                                size_t tmp_floats, double* ws, hipStream_t st);
+// 16-bit copy of the FP32 factor: each 128 x 128 tile scaled by a power of two (largest magnitude in
+// [2^14, 2^15)) and stored in FP16 or BF16, column-major with leading dimension ldf; isc holds
+// factor16_scales(n) inverse scales. lu_solve_16 solves with it (FP32 vectors), using the inverses
+// of the diagonal blocks that lu_prepare_solve_f32 computed from the FP32 factor.
+size_t factor16_scales(long n);
+void factor_to_16(const float* LU, long n, long lda, unsigned short* F, long ldf, float* isc, bool bf16,
+                  hipStream_t st);
+void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, bool bf16, const LuWork& W,
+                 float* x, hipStream_t st);
 // FP64 -> FP32 copy of a column-major matrix
 void convert_f64_to_f32(const double* A, long n, long lda, float* B, long ldb, hipStream_t st);
 

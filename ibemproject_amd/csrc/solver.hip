@@ -306,6 +306,28 @@ __device__ inline void split2(float x, _Float16& hi, _Float16& lo) {
   lo = (_Float16)(x - (float)hi);
 }
 
+// Product types of the split-operand GEMMs (kProd*, ibem3d_kernels.h). BF16 operands are stored
+// as their bit patterns in the same 16-bit arrays.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+template <int PROD>
+__device__ inline void split_to(float x, _Float16& hi, _Float16& lo) {
+  if constexpr (PROD == kProdBF16) {
+    hi = __builtin_bit_cast(_Float16, (__bf16)x);
+  } else if constexpr (PROD == kProdF16) {
+    hi = (_Float16)x;
+  } else {
+    split2(x, hi, lo);
+  }
+}
+template <int PROD>
+__device__ inline v4f mma16(half8 a, half8 b, v4f c) {
+  if constexpr (PROD == kProdBF16)
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
 // A operand: rows r < M of the column-major M x K block at A (lda). Output row-major [M][K] hi / lo
 // (k contiguous, row stride kSplitPadK) and the row's inverse scale; k in [K, kpad) written as zeros,
 // k >= kpad not written (kpad: a multiple of 64, the depth the GEMM reads). Workgroup = kSplitRW
@@ -314,6 +336,7 @@ __device__ inline void split2(float x, _Float16& hi, _Float16& lo) {
 // (the second read of A mostly hits L2).
 constexpr int kSplitRW = 64;      // rows per workgroup
 constexpr int kSplitChunk = 128;  // k per LDS round
+template <int PROD>
 __global__ __launch_bounds__(256) void k_split_rows(long M, int K, int kpad, const float* __restrict__ A,
                                                     long lda, _Float16* __restrict__ hi,
                                                     _Float16* __restrict__ lo,
@@ -361,20 +384,21 @@ __global__ __launch_bounds__(256) void k_split_rows(long M, int K, int kpad, con
       half8 h, l;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        _Float16 x, y;
-        split2(tile[rr][k8 + q] * ssc[rr], x, y);
+        _Float16 x, y = 0;
+        split_to<PROD>(tile[rr][k8 + q] * ssc[rr], x, y);
         h[q] = x;
         l[q] = y;
       }
       *(half8*)(hi + row * kSplitPadK + k0 + k8) = h;
-      *(half8*)(lo + row * kSplitPadK + k0 + k8) = l;
+      if constexpr (PROD == kProdF16x3) *(half8*)(lo + row * kSplitPadK + k0 + k8) = l;
     }
   }
 }
 
 // B operand: columns c < N of the column-major K x N block at B (ldb), output [N][K] hi / lo (k
 // contiguous, row stride kSplitPadK; written for k < kpad) and the column's inverse scale. One
-// wavefront per column.
+// wavefront per column. With a single product (PROD != kProdF16x3) only hi is written.
+template <int PROD>
 __global__ __launch_bounds__(256) void k_split_cols(long N, int K, int kpad, const float* __restrict__ B,
                                                     long ldb, _Float16* __restrict__ hi,
                                                     _Float16* __restrict__ lo,
@@ -403,13 +427,13 @@ __global__ __launch_bounds__(256) void k_split_cols(long N, int K, int kpad, con
     half4 h, l;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      _Float16 a, b;
-      split2(v[q4 + q] * sc, a, b);
+      _Float16 a, b = 0;
+      split_to<PROD>(v[q4 + q] * sc, a, b);
       h[q] = a;
       l[q] = b;
     }
     *(half4*)(hi + c * kSplitPadK + lane * VPL + q4) = h;
-    *(half4*)(lo + c * kSplitPadK + lane * VPL + q4) = l;
+    if constexpr (PROD == kProdF16x3) *(half4*)(lo + c * kSplitPadK + lane * VPL + q4) = l;
   }
 }
 
@@ -434,7 +458,7 @@ __global__ __launch_bounds__(256) void k_split_cols(long N, int K, int kpad, con
 #ifndef IBEM_F16_PAD
 #define IBEM_F16_PAD 8
 #endif
-template <int BK, int EPI, int PF, int NCH = kSplitPadK / BK>
+template <int BK, int EPI, int PF, int NCH = kSplitPadK / BK, int PROD = kProdF16x3>
 __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
                                                        const _Float16* __restrict__ Ah,
                                                        const _Float16* __restrict__ Al,
@@ -475,9 +499,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
       const long ko = k0 + part * 8;
       const half8 z = {0, 0, 0, 0, 0, 0, 0, 0};
       rAh[sl][e] = ga < M ? *(const half8*)(Ah + ga * kSplitPadK + ko) : z;
-      rAl[sl][e] = ga < M ? *(const half8*)(Al + ga * kSplitPadK + ko) : z;
       rBh[sl][e] = gb < N ? *(const half8*)(Bh + gb * kSplitPadK + ko) : z;
-      rBl[sl][e] = gb < N ? *(const half8*)(Bl + gb * kSplitPadK + ko) : z;
+      if constexpr (PROD == kProdF16x3) {
+        rAl[sl][e] = ga < M ? *(const half8*)(Al + ga * kSplitPadK + ko) : z;
+        rBl[sl][e] = gb < N ? *(const half8*)(Bl + gb * kSplitPadK + ko) : z;
+      }
     }
   };
   auto stash = [&](int sl) {
@@ -485,9 +511,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
     for (int e = 0; e < NV; ++e) {
       const int v = tid + 256 * e, row = v / VPR, part = v % VPR;
       *(half8*)(sAh + row * LS + part * 8) = rAh[sl][e];
-      *(half8*)(sAl + row * LS + part * 8) = rAl[sl][e];
       *(half8*)(sBh + row * LS + part * 8) = rBh[sl][e];
-      *(half8*)(sBl + row * LS + part * 8) = rBl[sl][e];
+      if constexpr (PROD == kProdF16x3) {
+        *(half8*)(sAl + row * LS + part * 8) = rAl[sl][e];
+        *(half8*)(sBl + row * LS + part * 8) = rBl[sl][e];
+      }
     }
   };
 #pragma unroll
@@ -505,29 +533,30 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
       for (int i = 0; i < 4; ++i) {
         const int off = (wm + i * 16 + lr) * LS + kk + lk * 8;
         ah[i] = *(const half8*)(sAh + off);
-        al[i] = *(const half8*)(sAl + off);
+        if constexpr (PROD == kProdF16x3) al[i] = *(const half8*)(sAl + off);
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int off = (wn + j * 16 + lr) * LS + kk + lk * 8;
         bh[j] = *(const half8*)(sBh + off);
-        bl[j] = *(const half8*)(sBl + off);
+        if constexpr (PROD == kProdF16x3) bl[j] = *(const half8*)(sBl + off);
+      }
+      if constexpr (PROD == kProdF16x3) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) acc[i][j] = mma16<PROD>(ah[i], bh[j], acc[i][j]);
     }
   }
   if constexpr (EPI == 0) {
@@ -599,6 +628,7 @@ constexpr int kBigK = 32;                 // k per LDS stage
 constexpr int kBigArr = kBigT * kBigK;    // halves per operand array and stage
 constexpr int kBigWaves = 8;
 constexpr int kBigQ = 16 / kBigWaves;     // 16-row blocks each wave loads per array and stage
+template <int PROD>
 __global__ __launch_bounds__(64 * kBigWaves, 1) void k_gemm_f16x3_256(
     long M, long N, const _Float16* __restrict__ Ah, const _Float16* __restrict__ Al,
     const _Float16* __restrict__ Bh, const _Float16* __restrict__ Bl, const float* __restrict__ ia,
@@ -638,12 +668,14 @@ __global__ __launch_bounds__(64 * kBigWaves, 1) void k_gemm_f16x3_256(
     for (int q = 0; q < kBigQ; ++q) {
       __builtin_amdgcn_global_load_lds((const void*)(ah0 + (offA[q] + kb)),
                                        (lds_void_t*)(base + 0 * kBigArr + q * 512), 16, 0, 0);
-      __builtin_amdgcn_global_load_lds((const void*)(al0 + (offA[q] + kb)),
-                                       (lds_void_t*)(base + 1 * kBigArr + q * 512), 16, 0, 0);
+      if constexpr (PROD == kProdF16x3)
+        __builtin_amdgcn_global_load_lds((const void*)(al0 + (offA[q] + kb)),
+                                         (lds_void_t*)(base + 1 * kBigArr + q * 512), 16, 0, 0);
       __builtin_amdgcn_global_load_lds((const void*)(bh0 + (offB[q] + kb)),
                                        (lds_void_t*)(base + 2 * kBigArr + q * 512), 16, 0, 0);
-      __builtin_amdgcn_global_load_lds((const void*)(bl0 + (offB[q] + kb)),
-                                       (lds_void_t*)(base + 3 * kBigArr + q * 512), 16, 0, 0);
+      if constexpr (PROD == kProdF16x3)
+        __builtin_amdgcn_global_load_lds((const void*)(bl0 + (offB[q] + kb)),
+                                         (lds_void_t*)(base + 3 * kBigArr + q * 512), 16, 0, 0);
     }
   };
   const int off = lr * kBigK + ((lk ^ (lr >> 2)) * 8);  // this lane's fragment within a 16-row block
@@ -669,18 +701,20 @@ __global__ __launch_bounds__(64 * kBigWaves, 1) void k_gemm_f16x3_256(
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         bh[j] = *(const half8*)(S + 2 * kBigArr + (wn + 16 * j) * kBigK + off);
-        bl[j] = *(const half8*)(S + 3 * kBigArr + (wn + 16 * j) * kBigK + off);
+        if constexpr (PROD == kProdF16x3) bl[j] = *(const half8*)(S + 3 * kBigArr + (wn + 16 * j) * kBigK + off);
       }
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         const half8 ah = *(const half8*)(S + (wm + 16 * i) * kBigK + off);
-        const half8 al = *(const half8*)(S + kBigArr + (wm + 16 * i) * kBigK + off);
+        if constexpr (PROD == kProdF16x3) {
+          const half8 al = *(const half8*)(S + kBigArr + (wm + 16 * i) * kBigK + off);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[j], acc[i][j], 0, 0, 0);
+        }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < 4; ++j) acc[i][j] = mma16<PROD>(ah, bh[j], acc[i][j]);
       }
       __builtin_amdgcn_s_waitcnt(kWaitVm0);
       __syncthreads();
@@ -786,21 +820,33 @@ static F16x3Ws f16x3_ws(double* ws, long cap) {
 }
 // rows of A (M x K block) into w.Ah / w.Al / w.ia; columns of B (K x N block) into the split-B
 // slots [c0, c0 + N)
+// prod: kProdF16x3 (hi and lo), kProdF16 or kProdBF16 (hi only)
 static void f16x3_split_rows(long M, int K, const float* A, long lda, const F16x3Ws& w, hipStream_t st,
-                             int kpad = kSplitPadK) {
-  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((M + kSplitRW - 1) / kSplitRW)), dim3(256), 0, st, M, K, kpad,
-                     A, lda, w.Ah, w.Al, w.ia);
+                             int kpad = kSplitPadK, int prod = kProdF16x3) {
+  const dim3 g((unsigned)((M + kSplitRW - 1) / kSplitRW)), b(256);
+  if (prod == kProdF16)
+    hipLaunchKernelGGL(k_split_rows<kProdF16>, g, b, 0, st, M, K, kpad, A, lda, w.Ah, w.Al, w.ia);
+  else if (prod == kProdBF16)
+    hipLaunchKernelGGL(k_split_rows<kProdBF16>, g, b, 0, st, M, K, kpad, A, lda, w.Ah, w.Al, w.ia);
+  else
+    hipLaunchKernelGGL(k_split_rows<kProdF16x3>, g, b, 0, st, M, K, kpad, A, lda, w.Ah, w.Al, w.ia);
 }
 static void f16x3_split_cols(long N, int K, const float* B, long ldb, const F16x3Ws& w, long c0,
-                             hipStream_t st, int kpad = kSplitPadK) {
+                             hipStream_t st, int kpad = kSplitPadK, int prod = kProdF16x3) {
   if (N <= 0) return;
-  hipLaunchKernelGGL(k_split_cols, dim3((unsigned)((N + 3) / 4)), dim3(256), 0, st, N, K, kpad, B, ldb,
-                     w.Bh + c0 * kSplitPadK, w.Bl + c0 * kSplitPadK, w.ib + c0);
+  const dim3 g((unsigned)((N + 3) / 4)), b(256);
+  _Float16 *h = w.Bh + c0 * kSplitPadK, *l = w.Bl + c0 * kSplitPadK;
+  if (prod == kProdF16)
+    hipLaunchKernelGGL(k_split_cols<kProdF16>, g, b, 0, st, N, K, kpad, B, ldb, h, l, w.ib + c0);
+  else if (prod == kProdBF16)
+    hipLaunchKernelGGL(k_split_cols<kProdBF16>, g, b, 0, st, N, K, kpad, B, ldb, h, l, w.ib + c0);
+  else
+    hipLaunchKernelGGL(k_split_cols<kProdF16x3>, g, b, 0, st, N, K, kpad, B, ldb, h, l, w.ib + c0);
 }
 static void f16x3_split(long M, long N, int K, const float* A, long lda, const float* B, long ldb,
-                        const F16x3Ws& w, hipStream_t st) {
-  f16x3_split_rows(M, K, A, lda, w, st);
-  f16x3_split_cols(N, K, B, ldb, w, 0, st);
+                        const F16x3Ws& w, hipStream_t st, int prod = kProdF16x3) {
+  f16x3_split_rows(M, K, A, lda, w, st, kSplitPadK, prod);
+  f16x3_split_cols(N, K, B, ldb, w, 0, st, kSplitPadK, prod);
 }
 // CUs of the current device (cached per device; 256 if the query fails)
 static int device_cus() {
@@ -826,18 +872,30 @@ static unsigned big_gemm_grid(long tiles, int cus) {
 #endif
 // C[M x N] -= split A times the split B columns [c0, c0 + N); the persistent 256 x 256 kernel runs
 // on at most cus CUs (0: all)
-static void f16x3_gemm(long M, long N, int K, const F16x3Ws& w, float* C, long ldc, hipStream_t st,
-                       long c0 = 0, int cus = 0) {
+template <int PROD>
+static void f16x3_gemm_t(long M, long N, int K, const F16x3Ws& w, float* C, long ldc, hipStream_t st,
+                         long c0, int cus) {
   // 256 x 256 tiles once there are at least two per CU (one workgroup per CU), else 128 x 128
   const long big = ((M + kBigT - 1) / kBigT) * ((N + kBigT - 1) / kBigT);
   if (big >= IBEM_BIG_MIN_TILES) {  // operands are zero-padded to kSplitPadK: K is implied
-    hipLaunchKernelGGL(k_gemm_f16x3_256, dim3(big_gemm_grid(big, cus)),
+    hipLaunchKernelGGL(k_gemm_f16x3_256<PROD>, dim3(big_gemm_grid(big, cus)),
                        dim3(64 * kBigWaves), 0, st, M, N, w.Ah, w.Al, w.Bh + c0 * kSplitPadK,
                        w.Bl + c0 * kSplitPadK, w.ia, w.ib + c0, C, ldc);
     return;
   }
-  hipLaunchKernelGGL((k_gemm_f16x3<IBEM_F16_BK, IBEM_F16_EPI, IBEM_F16_PF>), dim3(tile_grid(M, N)), dim3(256), 0, st, M, N, K, w.Ah,
-                     w.Al, w.Bh + c0 * kSplitPadK, w.Bl + c0 * kSplitPadK, w.ia, w.ib + c0, C, ldc);
+  constexpr int NCH = kSplitPadK / IBEM_F16_BK;
+  hipLaunchKernelGGL((k_gemm_f16x3<IBEM_F16_BK, IBEM_F16_EPI, IBEM_F16_PF, NCH, PROD>), dim3(tile_grid(M, N)),
+                     dim3(256), 0, st, M, N, K, w.Ah, w.Al, w.Bh + c0 * kSplitPadK,
+                     w.Bl + c0 * kSplitPadK, w.ia, w.ib + c0, C, ldc);
+}
+static void f16x3_gemm(long M, long N, int K, const F16x3Ws& w, float* C, long ldc, hipStream_t st,
+                       long c0 = 0, int cus = 0, int prod = kProdF16x3) {
+  if (prod == kProdF16)
+    f16x3_gemm_t<kProdF16>(M, N, K, w, C, ldc, st, c0, cus);
+  else if (prod == kProdBF16)
+    f16x3_gemm_t<kProdBF16>(M, N, K, w, C, ldc, st, c0, cus);
+  else
+    f16x3_gemm_t<kProdF16x3>(M, N, K, w, C, ldc, st, c0, cus);
 }
 
 // C[M x N] -= split A times split B for a depth K <= 512 (operands split with kpad = K rounded up to
@@ -852,11 +910,11 @@ static void f16x3_gemm_short(long M, long N, int K, const F16x3Ws& w, float* C, 
 
 // C[M x N] -= A[M x K] B[K x N] (column-major FP32, K <= 1024) through the split FP16 products.
 void gemm_f32_sub_f16x3(long M, long N, int K, const float* A, long lda, const float* B, long ldb,
-                        float* C, long ldc, double* ws, hipStream_t st) {
+                        float* C, long ldc, double* ws, hipStream_t st, int prod) {
   if (M <= 0 || N <= 0 || K <= 0) return;
   const F16x3Ws w = f16x3_ws(ws, M > N ? M : N);
-  f16x3_split(M, N, K, A, lda, B, ldb, w, st);
-  f16x3_gemm(M, N, K, w, C, ldc, st);
+  f16x3_split(M, N, K, A, lda, B, ldb, w, st, prod);
+  f16x3_gemm(M, N, K, w, C, ldc, st, 0, 0, prod);
 }
 
 // ======================================================= panel leaf: tournament pivoting (CALU) ==
@@ -1646,12 +1704,12 @@ constexpr long kNarrowCols = 4096;  // columns up to which U12 uses the wavefron
 // column k + kb): C -= L21 U12(:, cols).
 template <typename T>
 static void trailing_update(T* A, long n, long lda, long k, long kb, long c0, long nc,
-                            const F16x3Ws* w, hipStream_t st, int cus = 0) {
+                            const F16x3Ws* w, hipStream_t st, int cus = 0, int prod = kProdF16x3) {
   const long m = n - k - kb;
   if (m <= 0 || nc <= 0) return;
   T* C = A + (k + kb + c0) * lda + k + kb;
   if (w)
-    f16x3_gemm(m, nc, (int)kb, *w, (float*)C, lda, st, c0, cus);
+    f16x3_gemm(m, nc, (int)kb, *w, (float*)C, lda, st, c0, cus, prod);
   else
     gemm_sub<T, 1>(m, nc, kb, A + k * lda + k + kb, lda, A + (k + kb + c0) * lda + k, lda, C, lda, st);
 }
@@ -1767,7 +1825,7 @@ int lu_streams(LuWork& W) {
 
 template <typename T>
 static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
-                     double* f16ws = nullptr, bool cols_pending = false) {
+                     double* f16ws = nullptr, bool cols_pending = false, int prod = kProdF16x3) {
   if (lu_streams(W)) return -1;
   // f16ws holds two split workspaces: the bulk trailing updates' and the panel's own
   const F16x3Ws wspanel = f16ws ? f16x3_ws(f16ws + f16x3_workspace_doubles(n), n) : F16x3Ws{};
@@ -1799,12 +1857,12 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
     if (tick) tick(tctx, "lu.swap_trsm", 1, st);
     if (w) {
       if (tick) tick(tctx, "lu.split", 0, st);
-      f16x3_split_rows(m, (int)kb, (const float*)(A + k * lda + k + kb), lda, *w, st);
-      f16x3_split_cols(kb2, (int)kb, (const float*)(A + (k + kb) * lda + k), lda, *w, 0, st);
+      f16x3_split_rows(m, (int)kb, (const float*)(A + k * lda + k + kb), lda, *w, st, kSplitPadK, prod);
+      f16x3_split_cols(kb2, (int)kb, (const float*)(A + (k + kb) * lda + k), lda, *w, 0, st, kSplitPadK, prod);
       if (tick) tick(tctx, "lu.split", 1, st);
     }
     if (tick) tick(tctx, "lu.gemm", 0, st);
-    trailing_update<T>(A, n, lda, k, kb, 0, kb2, w, st);
+    trailing_update<T>(A, n, lda, k, kb, 0, kb2, w, st, 0, prod);
     if (tick) tick(tctx, "lu.gemm", 1, st);
     hipEventRecord(W.ev_a, st);
     // (2) the remaining columns (U12 through the block inverses when wide) while the panel runs; enqueued
@@ -1822,11 +1880,11 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
       if (tick) tick(tctx, "lu.swap_trsm_rest", 1, st);
       if (w) {
         if (tick) tick(tctx, "lu.split", 0, st);
-        f16x3_split_cols(nr, (int)kb, (const float*)B2, lda, *w, kb2, st);
+        f16x3_split_cols(nr, (int)kb, (const float*)B2, lda, *w, kb2, st, kSplitPadK, prod);
         if (tick) tick(tctx, "lu.split", 1, st);
       }
       if (tick) tick(tctx, "lu.gemm", 0, st);
-      trailing_update<T>(A, n, lda, k, kb, kb2, nr, w, st, device_cus() - panel_cus(m));
+      trailing_update<T>(A, n, lda, k, kb, kb2, nr, w, st, device_cus() - panel_cus(m), prod);
       if (tick) tick(tctx, "lu.gemm", 1, st);
     }
     hipStreamWaitEvent(W.st2, W.ev_a, 0);
@@ -1842,8 +1900,8 @@ int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick
   return lu_factor<double>(A, n, lda, W, st, tick, tctx);
 }
 int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
-                  double* f16ws, bool cols_pending) {
-  return lu_factor<float>(A, n, lda, W, st, tick, tctx, f16ws, cols_pending && n > kLuNB);
+                  double* f16ws, bool cols_pending, int prod) {
+  return lu_factor<float>(A, n, lda, W, st, tick, tctx, f16ws, cols_pending && n > kLuNB, prod);
 }
 
 // ======================================================================= triangular solves ======
@@ -1872,24 +1930,58 @@ constexpr int kTT = 1024;
 static_assert(IBEM_LU_NB % kTS == 0, "an LU step spans whole solve steps");
 static_assert(kTT == 4 * kTS && kTT == 8 * kTB, "thread layout of the solve steps");
 
-// Row sums of a rows x cols block (column-major, leading dimension ldm; rows, cols <= kTB) times xs
+// Factor accessors of the solves: entry (r, c) of the stored factor and the scale that multiplies
+// the stored values of its kTB x kTB tile (every sum the solves form runs inside one tile, so the
+// scale is applied once per partial sum).
+template <typename T>
+struct FacPlain {  // the factor as the LU left it (FP32 or FP64)
+  const T* p;
+  long ld;
+  __device__ T at(long r, long c) const { return p[c * ld + r]; }
+  __device__ T scale(long, long) const { return T(1); }
+};
+// 16-bit copy of the FP32 factor: FP16 (BF = false) or BF16 values of the kTB x kTB tile (rows
+// [kTB i, kTB i + kTB), columns [kTB j, ...)) scaled by a power of two, whose inverse is
+// isc[j * ntr + i]
+template <bool BF>
+struct Fac16 {
+  const unsigned short* p;
+  long ld;
+  const float* isc;
+  long ntr;
+  __device__ float at(long r, long c) const {
+    const unsigned short u = p[c * ld + r];
+    if constexpr (BF)
+      return __uint_as_float((unsigned)u << 16);
+    else
+      return (float)__builtin_bit_cast(_Float16, u);
+  }
+  __device__ float scale(long r, long c) const { return isc[(c / kTB) * ntr + r / kTB]; }
+};
+
+// Row sums of a rows x cols block (rows, cols <= kTB) at (r0, c0) of M, a factor accessor, times xs
 // (LDS): thread (g, i), i = t % kTB, g = t / kTB, sums columns [16 g, 16 g + 16) of row i, and the
 // 8 partials are added in a fixed order. Returns row t's sum to threads t < rows. All kTT threads
 // call it (it holds barriers); pp: kTT entries of LDS.
-template <typename T>
-__device__ T block_mv(const T* __restrict__ M, long ldm, int rows, int cols, const T* xs, T* pp) {
+template <typename T, typename FA>
+__device__ T block_mv(const FA& M, long r0, long c0, int rows, int cols, const T* xs, T* pp) {
   const int t = threadIdx.x, i = t % kTB, g = t / kTB;
   T acc0 = T(0), acc1 = T(0);
   if (i < rows) {
-    const T* m = M + (long)(16 * g) * ldm + i;
+    const long r = r0 + i, cb = c0 + 16 * g;
     if (16 * g + 16 <= cols) {
 #pragma unroll
       for (int c = 0; c < 16; c += 2) {
-        acc0 += m[(long)c * ldm] * xs[16 * g + c];
-        acc1 += m[(long)(c + 1) * ldm] * xs[16 * g + c + 1];
+        acc0 += M.at(r, cb + c) * xs[16 * g + c];
+        acc1 += M.at(r, cb + c + 1) * xs[16 * g + c + 1];
       }
     } else {
-      for (int c = 0; 16 * g + c < cols; ++c) acc0 += m[(long)c * ldm] * xs[16 * g + c];
+      for (int c = 0; 16 * g + c < cols; ++c) acc0 += M.at(r, cb + c) * xs[16 * g + c];
+    }
+    if (16 * g < cols) {
+      const T sc = M.scale(r, cb);
+      acc0 *= sc;
+      acc1 *= sc;
     }
   }
   pp[t] = acc0 + acc1;
@@ -1906,8 +1998,8 @@ __device__ T block_mv(const T* __restrict__ M, long ldm, int rows, int cols, con
 // right-hand side sy (LDS). Lower (forward sweep, unit L): x0 = D0 sy0, x1 = D1 (sy1 - L10 x0).
 // Upper (backward sweep): x1 = D1 sy1, x0 = D0 (sy0 - U01 x1). D = inverses of the kTB-blocks
 // (inv + block * kTB^2, column-major); sz: kTS + kTB entries of LDS; pp: kTT entries of LDS.
-template <typename T, bool Upper>
-__device__ void trsv_finish(const T* __restrict__ LU, long lda, const T* __restrict__ inv, long r0,
+template <typename T, bool Upper, typename FA>
+__device__ void trsv_finish(const FA& LU, const T* __restrict__ inv, long r0,
                             int kn, const T* sy, T* sz, T* pp, T* __restrict__ y) {
   const int t = threadIdx.x;
   const int n0 = min(kTB, kn), n1 = kn - n0;
@@ -1915,7 +2007,7 @@ __device__ void trsv_finish(const T* __restrict__ LU, long lda, const T* __restr
   const T* D1 = D0 + (size_t)kTB * kTB;
   // first the block that does not depend on the other: 0 for Lower, 1 for Upper
   const int fa = Upper && n1 > 0 ? n0 : 0, na = Upper && n1 > 0 ? n1 : n0;
-  const T xa = block_mv(Upper && n1 > 0 ? D1 : D0, kTB, na, na, sy + fa, pp);
+  const T xa = block_mv(FacPlain<T>{Upper && n1 > 0 ? D1 : D0, kTB}, 0, 0, na, na, sy + fa, pp);
   if (t < na) {
     sz[fa + t] = xa;
     y[r0 + fa + t] = xa;
@@ -1925,18 +2017,18 @@ __device__ void trsv_finish(const T* __restrict__ LU, long lda, const T* __restr
   // the other block, reduced by the first block's solution through the off-diagonal block of LU:
   // column (r0 + fa + c), row (r0 + fb + t)
   const int fb = Upper ? 0 : n0, nb = Upper ? n0 : n1;
-  const T w = block_mv(LU + (r0 + fa) * lda + r0 + fb, lda, nb, na, sz + fa, pp);
+  const T w = block_mv(LU, r0 + fb, r0 + fa, nb, na, sz + fa, pp);
   if (t < nb) sz[kTS + t] = sy[fb + t] - w;
   __syncthreads();
-  const T xb = block_mv(Upper ? D0 : D1, kTB, nb, nb, sz + kTS, pp);
+  const T xb = block_mv(FacPlain<T>{Upper ? D0 : D1, kTB}, 0, 0, nb, nb, sz + kTS, pp);
   if (t < nb) y[r0 + fb + t] = xb;
 }
 
 // Forward sweep. Step [k0, k0 + kTS) of y is solved; the rows past it are reduced by its columns
 // of L, and workgroup 0 (the next step's rows) finishes the next step unless it starts an LU step
 // (solve_next = 0: k_trsv_fwd_step does it after the step's interchanges).
-template <typename T>
-__global__ __launch_bounds__(1024) void k_trsv_fwd(const T* __restrict__ LU, long lda, long n, long k0,
+template <typename T, typename FA>
+__global__ __launch_bounds__(1024) void k_trsv_fwd(FA LU, long n, long k0,
                                                    const T* __restrict__ inv, T* __restrict__ y,
                                                    int solve_next) {
   __shared__ T sx[kTS];
@@ -1948,19 +2040,20 @@ __global__ __launch_bounds__(1024) void k_trsv_fwd(const T* __restrict__ LU, lon
   __syncthreads();
   const long r0 = k0 + kTS + (long)blockIdx.x * kTS;
   const long r = r0 + rl;
-  T p0 = T(0), p1 = T(0), p2 = T(0), p3 = T(0);
+  T p0 = T(0), p1 = T(0), p2 = T(0), p3 = T(0), sc = T(1);
   if (r < n) {
-    const T* a = LU + (k0 + 64 * g) * lda + r;
+    const long cb = k0 + 64 * g;
     const T* xs = sx + 64 * g;
 #pragma unroll
     for (int c = 0; c < 64; c += 4) {
-      p0 += a[(long)c * lda] * xs[c];
-      p1 += a[(long)(c + 1) * lda] * xs[c + 1];
-      p2 += a[(long)(c + 2) * lda] * xs[c + 2];
-      p3 += a[(long)(c + 3) * lda] * xs[c + 3];
+      p0 += LU.at(r, cb + c) * xs[c];
+      p1 += LU.at(r, cb + c + 1) * xs[c + 1];
+      p2 += LU.at(r, cb + c + 2) * xs[c + 2];
+      p3 += LU.at(r, cb + c + 3) * xs[c + 3];
     }
+    sc = LU.scale(r, cb);
   }
-  pp[t] = (p0 + p1) + (p2 + p3);
+  pp[t] = ((p0 + p1) + (p2 + p3)) * sc;
   __syncthreads();
   const bool own = blockIdx.x == 0 && solve_next;
   if (t < kTS && r < n) {
@@ -1972,16 +2065,16 @@ __global__ __launch_bounds__(1024) void k_trsv_fwd(const T* __restrict__ LU, lon
   }
   if (own) {
     __syncthreads();
-    trsv_finish<T, false>(LU, lda, inv, r0, (int)min((long)kTS, n - r0), sy, sz, pp, y);
+    trsv_finish<T, false>(LU, inv, r0, (int)min((long)kTS, n - r0), sy, sz, pp, y);
   }
 }
 
 // Forward sweep at the first solve step of an LU step: the step's interchanges applied to y as moves
 // y[dst] <- y[src] (all reads, then all writes), then the solve step at r0 finished. One workgroup;
 // an LU step has at most 2 IBEM_LU_NB moves.
-template <typename T>
+template <typename T, typename FA>
 __global__ __launch_bounds__(1024) void k_trsv_fwd_step(const int* __restrict__ mv, int nmv, long r0,
-                                                        long n, const T* __restrict__ LU, long lda,
+                                                        long n, FA LU,
                                                         const T* __restrict__ inv, T* __restrict__ y) {
   __shared__ T sv[2 * IBEM_LU_NB];
   __shared__ T sy[kTS];
@@ -1995,14 +2088,14 @@ __global__ __launch_bounds__(1024) void k_trsv_fwd_step(const int* __restrict__ 
   const int kn = (int)min((long)kTS, n - r0);
   if (t < kn) sy[t] = y[r0 + t];
   __syncthreads();
-  trsv_finish<T, false>(LU, lda, inv, r0, kn, sy, sz, pp, y);
+  trsv_finish<T, false>(LU, inv, r0, kn, sy, sz, pp, y);
 }
 
 // Backward sweep. Step [k0, k0 + kb) of y is solved; the rows above it are reduced by its columns
 // of U, and workgroup 0 (rows [k0 - kTS, k0)) finishes that step. With k0 < 0 the launch only
 // finishes the last step [lastk, n).
-template <typename T>
-__global__ __launch_bounds__(1024) void k_trsv_bwd(const T* __restrict__ LU, long lda, long n, long k0,
+template <typename T, typename FA>
+__global__ __launch_bounds__(1024) void k_trsv_bwd(FA LU, long n, long k0,
                                                    int kb, long lastk, const T* __restrict__ inv,
                                                    T* __restrict__ y) {
   __shared__ T sx[kTS];
@@ -2014,7 +2107,7 @@ __global__ __launch_bounds__(1024) void k_trsv_bwd(const T* __restrict__ LU, lon
     const int kn = (int)(n - lastk);
     if (t < kn) sy[t] = y[lastk + t];
     __syncthreads();
-    trsv_finish<T, true>(LU, lda, inv, lastk, kn, sy, sz, pp, y);
+    trsv_finish<T, true>(LU, inv, lastk, kn, sy, sz, pp, y);
     return;
   }
   const int rl = t % kTS, g = t / kTS;
@@ -2022,30 +2115,31 @@ __global__ __launch_bounds__(1024) void k_trsv_bwd(const T* __restrict__ LU, lon
   __syncthreads();
   const long r0 = k0 - (long)(blockIdx.x + 1) * kTS;  // >= 0: k0 is a multiple of kTS
   const long r = r0 + rl;
-  T p0 = T(0), p1 = T(0), p2 = T(0), p3 = T(0);
+  T p0 = T(0), p1 = T(0), p2 = T(0), p3 = T(0), sc = T(1);
   {
-    const T* a = LU + (k0 + 64 * g) * lda + r;
+    const long cb = k0 + 64 * g;
     const T* xs = sx + 64 * g;
     if (kb == kTS) {
 #pragma unroll
       for (int c = 0; c < 64; c += 4) {
-        p0 += a[(long)c * lda] * xs[c];
-        p1 += a[(long)(c + 1) * lda] * xs[c + 1];
-        p2 += a[(long)(c + 2) * lda] * xs[c + 2];
-        p3 += a[(long)(c + 3) * lda] * xs[c + 3];
+        p0 += LU.at(r, cb + c) * xs[c];
+        p1 += LU.at(r, cb + c + 1) * xs[c + 1];
+        p2 += LU.at(r, cb + c + 2) * xs[c + 2];
+        p3 += LU.at(r, cb + c + 3) * xs[c + 3];
       }
     } else {  // the last step: only its kb < kTS columns exist
-      for (int c = 0; 64 * g + c < kb && c < 64; ++c) p0 += a[(long)c * lda] * xs[c];
+      for (int c = 0; 64 * g + c < kb && c < 64; ++c) p0 += LU.at(r, cb + c) * xs[c];
     }
+    if (64 * g < kb) sc = LU.scale(r, cb);
   }
-  pp[t] = (p0 + p1) + (p2 + p3);
+  pp[t] = ((p0 + p1) + (p2 + p3)) * sc;
   __syncthreads();
   T v = T(0);
   if (t < kTS) v = y[r] - ((pp[t] + pp[kTS + t]) + (pp[2 * kTS + t] + pp[3 * kTS + t]));
   if (blockIdx.x == 0) {
     if (t < kTS) sy[t] = v;
     __syncthreads();
-    trsv_finish<T, true>(LU, lda, inv, r0, kTS, sy, sz, pp, y);
+    trsv_finish<T, true>(LU, inv, r0, kTS, sy, sz, pp, y);
   } else if (t < kTS) {
     y[r] = v;
   }
@@ -2103,8 +2197,8 @@ static int lu_prepare(const T* LU, long n, long lda, LuWork& W, hipStream_t st) 
   return 0;
 }
 
-template <typename T>
-static void lu_solve(const T* LU, long n, long lda, const LuWork& W, T* x, hipStream_t st) {
+template <typename T, typename FA>
+static void lu_solve(const FA& LU, long n, const LuWork& W, T* x, hipStream_t st) {
   const long nb = (n + kTB - 1) / kTB;
   const long ns = (n + kTS - 1) / kTS;
   const T* invL = (const T*)W.d_inv;
@@ -2116,22 +2210,22 @@ static void lu_solve(const T* LU, long n, long lda, const LuWork& W, T* x, hipSt
     if (k0 % IBEM_LU_NB == 0) {  // first solve step of an LU step: its interchanges, then the step
       const long j = k0 / IBEM_LU_NB;
       const int m0 = W.mv_off[j], nm = W.mv_off[j + 1] - m0;
-      hipLaunchKernelGGL(k_trsv_fwd_step<T>, dim3(1), dim3(kTT), 0, st, (const int*)W.d_perm + 2 * m0,
-                         nm, k0, n, LU, lda, invL, y);
+      hipLaunchKernelGGL((k_trsv_fwd_step<T, FA>), dim3(1), dim3(kTT), 0, st, (const int*)W.d_perm + 2 * m0,
+                         nm, k0, n, LU, invL, y);
     }
     if (s + 1 < ns) {
       const long rows = n - k0 - kTS;
       const int solve_next = ((s + 1) * kTS) % IBEM_LU_NB != 0;
-      hipLaunchKernelGGL(k_trsv_fwd<T>, dim3((unsigned)((rows + kTS - 1) / kTS)), dim3(kTT), 0, st, LU,
-                         lda, n, k0, invL, y, solve_next);
+      hipLaunchKernelGGL((k_trsv_fwd<T, FA>), dim3((unsigned)((rows + kTS - 1) / kTS)), dim3(kTT), 0, st, LU,
+                         n, k0, invL, y, solve_next);
     }
   }
   const long lastk = (ns - 1) * kTS;
-  hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(1), dim3(kTT), 0, st, LU, lda, n, -1L, 0, lastk, invU, y);
+  hipLaunchKernelGGL((k_trsv_bwd<T, FA>), dim3(1), dim3(kTT), 0, st, LU, n, -1L, 0, lastk, invU, y);
   for (long s = ns - 1; s >= 1; --s) {
     const long k0 = s * kTS;
     const int kb = (int)std::min<long>(kTS, n - k0);
-    hipLaunchKernelGGL(k_trsv_bwd<T>, dim3((unsigned)s), dim3(kTT), 0, st, LU, lda, n, k0, kb, lastk,
+    hipLaunchKernelGGL((k_trsv_bwd<T, FA>), dim3((unsigned)s), dim3(kTT), 0, st, LU, n, k0, kb, lastk,
                        invU, y);
   }
   hipMemcpyAsync(x, y, sizeof(T) * n, hipMemcpyDeviceToDevice, st);
@@ -2144,10 +2238,67 @@ int lu_prepare_solve_f32(const float* LU, long n, long lda, LuWork& W, hipStream
   return lu_prepare<float>(LU, n, lda, W, st);
 }
 void lu_solve_f64(const double* LU, long n, long lda, const LuWork& W, double* x, hipStream_t st) {
-  lu_solve<double>(LU, n, lda, W, x, st);
+  lu_solve<double>(FacPlain<double>{LU, lda}, n, W, x, st);
 }
 void lu_solve_f32(const float* LU, long n, long lda, const LuWork& W, float* x, hipStream_t st) {
-  lu_solve<float>(LU, n, lda, W, x, st);
+  lu_solve<float>(FacPlain<float>{LU, lda}, n, W, x, st);
+}
+
+// ============================================== 16-bit copy of the factor (compact storage) ======
+// Every kTB x kTB tile of the FP32 factor is scaled by the power of two that brings its largest
+// magnitude into [2^14, 2^15) and stored in FP16 or BF16; the tile's inverse scale goes to isc.
+// One workgroup per tile: the maximum, then the conversion (the second read hits the cache).
+long factor16_tile_rows(long n) { return (n + kTB - 1) / kTB; }
+size_t factor16_scales(long n) { return (size_t)factor16_tile_rows(n) * factor16_tile_rows(n); }
+
+template <bool BF>
+__global__ __launch_bounds__(256) void k_factor16(const float* __restrict__ LU, long n, long lda,
+                                                  unsigned short* __restrict__ F, long ldf,
+                                                  float* __restrict__ isc) {
+  __shared__ float red[4];
+  const long ntr = (n + kTB - 1) / kTB;
+  const long ti = blockIdx.x, tj = blockIdx.y;
+  const long r0 = ti * kTB, c0 = tj * kTB;
+  const int rows = (int)min((long)kTB, n - r0), cols = (int)min((long)kTB, n - c0);
+  const int t = threadIdx.x, rl = t % kTB, cg = t / kTB;  // two columns per pass
+  float m = 0.0f;
+  if (rl < rows)
+    for (int c = cg; c < cols; c += 2) m = fmaxf(m, fabsf(LU[(c0 + c) * lda + r0 + rl]));
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  if ((t & 63) == 0) red[t >> 6] = m;
+  __syncthreads();
+  const float mm = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  const float sc = pow2_scale(mm);
+  if (t == 0) isc[tj * ntr + ti] = 1.0f / sc;
+  if (rl < rows)
+    for (int c = cg; c < cols; c += 2) {
+      const float v = LU[(c0 + c) * lda + r0 + rl] * sc;
+      unsigned short u;
+      if constexpr (BF)
+        u = __builtin_bit_cast(unsigned short, (__bf16)v);
+      else
+        u = __builtin_bit_cast(unsigned short, (_Float16)v);
+      F[(c0 + c) * ldf + r0 + rl] = u;
+    }
+}
+
+void factor_to_16(const float* LU, long n, long lda, unsigned short* F, long ldf, float* isc, bool bf16,
+                  hipStream_t st) {
+  const long ntr = factor16_tile_rows(n);
+  const dim3 g((unsigned)ntr, (unsigned)ntr), b(256);
+  if (bf16)
+    hipLaunchKernelGGL(k_factor16<true>, g, b, 0, st, LU, n, lda, F, ldf, isc);
+  else
+    hipLaunchKernelGGL(k_factor16<false>, g, b, 0, st, LU, n, lda, F, ldf, isc);
+}
+
+void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, bool bf16, const LuWork& W,
+                 float* x, hipStream_t st) {
+  const long ntr = factor16_tile_rows(n);
+  if (bf16)
+    lu_solve<float>(Fac16<true>{F, ldf, isc, ntr}, n, W, x, st);
+  else
+    lu_solve<float>(Fac16<false>{F, ldf, isc, ntr}, n, W, x, st);
 }
 
 // ========================================================================= equilibration ========
@@ -2271,6 +2422,64 @@ int equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c
   if (lu_streams(*W)) return -1;
   // the columns past the first panel are copied beside it
   return equilibrate_copy<float>(A, n, lda, r, c, out, ldo, ws, st, W->st2, W->ev_r, W->ev_cols, kLuNB);
+}
+
+// ---- in place (compact storage: the factor overwrites the FP64 matrix) ----
+// Column j: c_j = 1 / max_i |r_i A_ij|, then A[:, j] <- R A C, each element read and written by the
+// same thread
+__global__ __launch_bounds__(kColThreads) void k_colscale_inplace(double* A, long n, long lda,
+                                                                  const double* __restrict__ r,
+                                                                  double* __restrict__ c) {
+  __shared__ double red[kColThreads / 64];
+  const long j = blockIdx.x;
+  double* col = A + j * lda;
+  double m = 0.0;
+  for (long i = threadIdx.x; i < n; i += kColThreads) m = fmax(m, fabs(col[i] * r[i]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  double mm = red[0];
+#pragma unroll
+  for (int q = 1; q < kColThreads / 64; ++q) mm = fmax(mm, red[q]);
+  const double cj = mm > 0.0 ? 1.0 / mm : 1.0;
+  if (threadIdx.x == 0) c[j] = cj;
+  for (long i = threadIdx.x; i < n; i += kColThreads) col[i] = col[i] * r[i] * cj;
+}
+
+void equilibrate_in_place_f64(double* A, long n, long lda, double* r, double* c, double* ws, hipStream_t st) {
+  hipLaunchKernelGGL(k_rowmax_part, dim3((unsigned)((n + 255) / 256), kRowChunks), dim3(256), 0, st, A,
+                     n, lda, ws);
+  hipLaunchKernelGGL(k_rowmax_done, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ws, n, r);
+  hipLaunchKernelGGL(k_colscale_inplace, dim3((unsigned)n), dim3(kColThreads), 0, st, A, n, lda, r, c);
+}
+
+// FP64 A (n x n, leading dimension n) -> FP32 R A C with leading dimension lda32 (>= n) in the same
+// memory. FP32 column j occupies bytes [4 lda32 j, 4 lda32 (j + 1)), which lie inside FP64 columns
+// < j: columns are converted in increasing blocks [j0, j1) with 4 lda32 j1 <= 8 n j0, so that a
+// launch writes only over FP64 columns that earlier launches have read, and never over the columns
+// it reads itself. The first columns (whose FP32 place overlaps themselves) go through tmp first.
+// Returns nonzero when tmp holds fewer than two columns.
+int equilibrate_to_f32_in_place(double* A, long n, double* r, double* c, long lda32, float* tmp,
+                                size_t tmp_floats, double* ws, hipStream_t st) {
+  if (lda32 < n || lda32 >= 2 * n) return -1;
+  const long S = std::min<long>(n, (long)(tmp_floats / (size_t)lda32));
+  if (S < 2 && S < n) return -1;
+  hipLaunchKernelGGL(k_rowmax_part, dim3((unsigned)((n + 255) / 256), kRowChunks), dim3(256), 0, st, A,
+                     n, n, ws);
+  hipLaunchKernelGGL(k_rowmax_done, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ws, n, r);
+  hipLaunchKernelGGL(k_colscale_copy<float>, dim3((unsigned)S), dim3(kColThreads), 0, st, A, n, n, r, c,
+                     tmp, lda32, 0L);
+  float* out = (float*)A;
+  for (long j0 = S; j0 < n;) {
+    const long j1 = std::min<long>(n, (2 * n * j0) / lda32);  // 4 lda32 j1 <= 8 n j0
+    if (j1 <= j0) return -1;
+    hipLaunchKernelGGL(k_colscale_copy<float>, dim3((unsigned)(j1 - j0)), dim3(kColThreads), 0, st, A, n,
+                       n, r, c, out, lda32, j0);
+    j0 = j1;
+  }
+  if (hipMemcpyAsync(out, tmp, sizeof(float) * (size_t)S * lda32, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return -1;
+  return 0;
 }
 
 void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStream_t st) {

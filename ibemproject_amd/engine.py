@@ -21,9 +21,12 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 SOLVE_FP64_LU = 0
 SOLVE_MIXED_FP32 = 1
-SOLVE_MIXED_FP16 = 2
+SOLVE_MIXED_FP16 = 2     # FP32 LU, trailing updates on three FP16 products of hi/lo splits
+SOLVE_MIXED_FP16_1 = 3   # FP32 LU, trailing updates on one FP16 product (power-of-two scaled operands)
+SOLVE_MIXED_BF16 = 4     # FP32 LU, trailing updates on one BF16 product
 SOLVE_GMRES = 0x100  # flag for the mixed modes: GMRES-IR from the start
 SOLVE_REUSE = 0x200  # flag: reuse the factors of the last factorisation in the same mode
+SOLVE_COMPACT = 0x400  # flag: the factor takes A's place (16-bit factor copy for the mixed modes)
 
 
 class _CProblem(ctypes.Structure):
@@ -41,7 +44,7 @@ EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size"
            "ibem3d_get_system", "ibem3d_solve_info", "ibem3d_timings", "ibem3d_reset_timings",
            "ibem3d_read_inputs", "ibem3d_inputs_problem", "ibem3d_inputs_title",
            "ibem3d_free_inputs", "ibem3d_io_error", "ibem3d_write_stress", "ibem3d_average_fields",
-           "ibem3d_set_known_values", "ibem3d_average_eigenstrain"]
+           "ibem3d_set_known_values", "ibem3d_average_eigenstrain", "ibem3d_residual"]
 
 
 class EngineError(RuntimeError):
@@ -89,6 +92,7 @@ def load(path: str = LIB_PATH):
     L.ibem3d_average_fields.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
     L.ibem3d_set_known_values.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.ibem3d_average_eigenstrain.argtypes = [ctypes.c_void_p, _dp]
+    L.ibem3d_residual.argtypes = [ctypes.c_void_p, _dp]
     _LIB = L
     return L
 
@@ -193,6 +197,13 @@ class Engine:
         it = ctypes.c_int()
         self.L.ibem3d_solve_info(self.ctx, ctypes.byref(rel), ctypes.byref(it))
         return rel.value, it.value
+
+    def residual(self) -> float:
+        """Relative residual ||b - A x|| / ||b|| of the current solution (after a compact FP64 solve
+        the system is assembled again first)."""
+        rel = ctypes.c_double()
+        self._chk(self.L.ibem3d_residual(self.ctx, ctypes.byref(rel)), "residual")
+        return rel.value
 
     def getUTVector(self):
         p = self.prob

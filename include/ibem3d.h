@@ -66,14 +66,28 @@ enum {
   IBEM3D_SOLVE_MIXED_FP32 = 1,  /* FP32 LU (FP32 MFMA update) + FP64 iterative refinement */
   IBEM3D_SOLVE_MIXED_FP16 = 2,  /* FP32 LU whose trailing updates run on FP16 MFMA with scaled hi/lo split
                                    operands (FP32-level accuracy) + FP64 refinement */
+  IBEM3D_SOLVE_MIXED_FP16_1 = 3,/* FP32 LU whose trailing updates are single FP16 MFMA products of
+                                   operands scaled per row of L21 / column of U12 by powers of two
+                                   (FP32 accumulation) + FP64 refinement */
+  IBEM3D_SOLVE_MIXED_BF16 = 4,  /* the same with single BF16 products */
   IBEM3D_SOLVE_GMRES = 0x100,   /* flag for the mixed modes: refine by GMRES-IR (FP64 GMRES, right-
-                                   preconditioned by the FP32 factors) from the start. Without it the
+                                   preconditioned by the LU factors) from the start. Without it the
                                    refinement is classic and switches to GMRES-IR when an iteration
                                    gains less than a factor of 10. */
-  IBEM3D_SOLVE_REUSE = 0x200    /* flag: solve with the factors of the last factorisation in the same
+  IBEM3D_SOLVE_REUSE = 0x200,   /* flag: solve with the factors of the last factorisation in the same
                                    mode (no new factorisation). Valid while the system matrix is the
                                    one factored: same context, only known values changed
                                    (ibem3d_set_known_values). */
+  IBEM3D_SOLVE_COMPACT = 0x400  /* flag: the factorisation takes the place of A instead of a second
+                                   n x n copy (chosen without the flag when free device memory cannot
+                                   hold that copy). FP64_LU: A <- R A C and its LU in place; the
+                                   residual is not computed (ibem3d_residual assembles A again for
+                                   it). Mixed modes: FP32 R A C converted in place, factored, then
+                                   stored as a 16-bit copy (BF16 for MIXED_BF16, else FP16; every
+                                   128 x 128 tile scaled by a power of two), A assembled again over
+                                   the FP32 factor, and FP64 GMRES-IR against A preconditioned by
+                                   the 16-bit factor. Peak memory 8 n^2 + 2 n^2 bytes instead of
+                                   8 n^2 + 4 n^2 (mixed) or 16 n^2 (FP64). */
 };
 
 /* Create a context on HIP device `device`: copies the problem to the device, builds the BC column
@@ -134,7 +148,13 @@ int ibem3d_average_fields(ibem3d_ctx* ctx, double* sigma, double* eps, double* v
 /* Copy the assembled system (A column-major n x n, b n) to the host; either may be NULL. */
 int ibem3d_get_system(ibem3d_ctx* ctx, double* A, double* b);
 
-/* Refinement statistics of the last solve: relative residual, iterations (0 for FP64_LU). */
+/* Relative residual ||b - A x|| / ||b|| of the current solution with the FP64 system matrix. After a
+ * compact FP64 solve (A overwritten by its factor) the system is assembled again first, which ends
+ * the factor's validity (no counterpart in the reference, which reports no residual). */
+int ibem3d_residual(ibem3d_ctx* ctx, double* rel_residual);
+
+/* Refinement statistics of the last solve: relative residual (-1 when not computed: compact FP64
+ * solve), iterations (0 for FP64_LU). */
 int ibem3d_solve_info(const ibem3d_ctx* ctx, double* rel_residual, int* iterations);
 
 /* Per-kernel timings of the calls since the last reset, measured with HIP events on the context's

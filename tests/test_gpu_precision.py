@@ -1,0 +1,153 @@
+"""Factor precisions and storage layouts of the solve (MI355X), against LAPACK on the same assembled
+system: single FP16 and BF16 trailing-update products (per-row / per-column power-of-two scaled
+operands), and compact storage, where the factor takes the place of A (FP64 in place; mixed modes:
+FP32 in place, then a 16-bit per-tile-scaled copy and A assembled again). Every refined result must
+reach a 1e-10 relative residual and agree with LAPACK to 1e-8; the FP64 path to 1e-10. Each test
+prints its error against the FP64 LU solution."""
+import numpy as np
+import pytest
+
+from ibemproject_amd import engine as E
+from ibemproject_amd.generate import make_rve
+
+pytestmark = pytest.mark.gpu
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def mid():
+    """A system past several LU steps whose bulk updates take the 256 x 256 kernel, and whose
+    in-place FP32 conversion needs blocks beyond the staged first columns."""
+    prob = make_rve(321, 150, 0.035, 0.05, 8, nump=40)
+    case(f"mid size num=150 n60={prob.n60}")
+    eng = E.Engine(prob)
+    eng.assemble()
+    A, b = eng.system()
+    eng.solve(E.SOLVE_FP64_LU)
+    _, _, x64 = eng.getUTVector()
+    s64 = eng.getFlux()
+    eng.close()
+    x_ref = np.linalg.solve(A, b)
+    return dict(prob=prob, A=A, b=b, x_ref=x_ref, x64=x64, s64=s64)
+
+
+def check_refined(eng, d, rel, it, label):
+    A, b, x_ref = d["A"], d["b"], d["x_ref"]
+    _, _, X = eng.getUTVector()
+    e = np.linalg.norm(X - x_ref) / np.linalg.norm(x_ref)
+    e64 = np.linalg.norm(X - d["x64"]) / np.linalg.norm(d["x64"])
+    r = np.linalg.norm(b - A @ X) / np.linalg.norm(b)
+    s = eng.getFlux()
+    es = np.abs(s - d["s64"]).max() / np.abs(d["s64"]).max()
+    print(f"{label}: n {len(b)} iterations {it} engine residual {rel:.3e} residual {r:.3e} "
+          f"err vs LAPACK {e:.3e} vs FP64 LU {e64:.3e} stress vs FP64 LU {es:.3e}")
+    assert rel <= 1e-10 and r <= 1.5e-10
+    assert e < 1e-8 and e64 < 1e-8 and es < 1e-8
+    return X
+
+
+@pytest.mark.parametrize("mode_name", ["SOLVE_MIXED_FP16_1", "SOLVE_MIXED_BF16"])
+def test_single_product_modes(mid, mode_name):
+    case(f"single-product LU: {mode_name} n60={mid['prob'].n60}")
+    eng = E.Engine(mid["prob"])
+    eng.assemble()
+    rel, it = eng.solve(getattr(E, mode_name), 1e-10)
+    check_refined(eng, mid, rel, it, mode_name)
+    eng.close()
+
+
+@pytest.mark.parametrize("mode_name", ["SOLVE_MIXED_FP32", "SOLVE_MIXED_FP16", "SOLVE_MIXED_FP16_1",
+                                       "SOLVE_MIXED_BF16"])
+def test_compact_mixed_modes(mid, mode_name):
+    """Compact storage: the FP32 factor overwrites A in place, its 16-bit copy preconditions FP64
+    GMRES against the re-assembled A, which must equal the first assembly bit for bit."""
+    case(f"compact mixed: {mode_name} n60={mid['prob'].n60}")
+    eng = E.Engine(mid["prob"])
+    eng.assemble()
+    rel, it = eng.solve(getattr(E, mode_name) | E.SOLVE_COMPACT, 1e-10)
+    check_refined(eng, mid, rel, it, mode_name + " compact")
+    A2, b2 = eng.system()
+    assert np.array_equal(A2, mid["A"]) and np.array_equal(b2, mid["b"])
+    eng.close()
+
+
+def test_compact_fp64(mid):
+    """FP64 LU in place of A: no residual from the solve; ibem3d_residual assembles A again."""
+    case(f"compact fp64 n60={mid['prob'].n60}")
+    eng = E.Engine(mid["prob"])
+    eng.assemble()
+    rel, _ = eng.solve(E.SOLVE_FP64_LU | E.SOLVE_COMPACT)
+    assert rel == -1.0
+    _, _, X = eng.getUTVector()
+    s = eng.getFlux()
+    with pytest.raises(E.EngineError):
+        eng.system()  # A holds the factor
+    r_engine = eng.residual()
+    A2, b2 = eng.system()
+    assert np.array_equal(A2, mid["A"]) and np.array_equal(b2, mid["b"])
+    e = np.linalg.norm(X - mid["x_ref"]) / np.linalg.norm(mid["x_ref"])
+    e64 = np.linalg.norm(X - mid["x64"]) / np.linalg.norm(mid["x64"])
+    es = np.abs(s - mid["s64"]).max() / np.abs(mid["s64"]).max()
+    print(f"compact fp64: residual {r_engine:.3e} err vs LAPACK {e:.3e} vs FP64 LU (copy) {e64:.3e} "
+          f"stress {es:.3e}")
+    assert r_engine < 1e-11 and e < 1e-10 and es < 1e-10
+    # the factor was overwritten by the assembly: reuse is refused
+    with pytest.raises(E.EngineError):
+        eng.solve(E.SOLVE_FP64_LU | E.SOLVE_REUSE)
+    eng.close()
+
+
+@pytest.mark.parametrize("seed,num,m", [(11, 3, 7), (13, 5, 10)])
+def test_compact_past_first_panel(seed, num, m):
+    """Compact mixed solves of systems just past one LU step (the whole FP32 conversion staged)."""
+    prob = make_rve(seed, num, 0.05, 0.07, m, nump=0)
+    case(f"compact past first panel: num={num} m={m} n60={prob.n60}")
+    eng = E.Engine(prob)
+    eng.assemble()
+    A, b = eng.system()
+    rel, it = eng.solve(E.SOLVE_MIXED_FP16_1 | E.SOLVE_COMPACT, 1e-10)
+    _, _, X = eng.getUTVector()
+    x_ref = np.linalg.solve(A, b)
+    e = np.linalg.norm(X - x_ref) / np.linalg.norm(x_ref)
+    print("n", prob.n60, "iterations", it, "residual", rel, "err vs LAPACK", e)
+    assert rel <= 1e-10 and e < 1e-8
+    eng.close()
+
+
+def test_compact_reuse_new_load(mid):
+    """The 16-bit factor copy stays valid when only the known values change (A is assembled again by
+    the caller): a reused solve matches a fresh one."""
+    from ibemproject_amd.engine import unit_strain
+    from ibemproject_amd.formats import Problem
+    from ibemproject_amd.generate import macro_strain_bc
+    p = mid["prob"]
+    nodes = np.asarray(p.nodes).reshape(-1, 3)
+    conn = np.asarray(p.conn).reshape(-1, 4)
+    bc0, tbc0 = macro_strain_bc(nodes, conn, unit_strain(0))
+    bc1, tbc1 = macro_strain_bc(nodes, conn, unit_strain(4))
+    p0 = Problem(p.title, p.E0, p.nu0, p.E1, p.nu1, nodes, conn, bc0, tbc0, p.centres, p.radii,
+                 np.zeros((0, 3)))
+    p1 = Problem(p.title, p.E0, p.nu0, p.E1, p.nu1, nodes, conn, bc1, tbc1, p.centres, p.radii,
+                 np.zeros((0, 3)))
+    case(f"compact reuse n60={p.n60}")
+    mode = E.SOLVE_MIXED_FP16_1 | E.SOLVE_COMPACT
+    eng = E.Engine(p0)
+    eng.assemble()
+    eng.solve(mode, 1e-10)
+    eng.set_known_values(bc1[:, 1], tbc1[:, 12:])
+    eng.assemble()
+    rel, it = eng.solve(E.SOLVE_MIXED_FP16_1 | E.SOLVE_REUSE, 1e-10)
+    assert rel <= 1e-10
+    _, _, x_reuse = eng.getUTVector()
+    eng.close()
+    eng = E.Engine(p1)
+    eng.assemble()
+    eng.solve(E.SOLVE_FP64_LU)
+    _, _, x_fresh = eng.getUTVector()
+    eng.close()
+    e = np.linalg.norm(x_reuse - x_fresh) / np.linalg.norm(x_fresh)
+    print("reuse iterations", it, "residual", rel, "vs fresh FP64", e)
+    assert e < 1e-8
