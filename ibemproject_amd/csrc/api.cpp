@@ -58,7 +58,6 @@ struct ibem3d_ctx {
   unsigned short* d_F16 = nullptr;
   float* d_F16s = nullptr;
   long ldf = 0;
-  bool f16_bf = false;         // the copy holds BF16 values (else FP16)
   bool a_factored = false;     // d_A holds a factor in place of the assembled system
   LuWork W;
   unsigned phases = 0;
@@ -506,7 +505,7 @@ __global__ void k_axpy_scaled_f32(double* x, const float* d, const double* s, lo
 static dim3 vgrid(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 // ---- GMRES-IR building blocks (FP64; reductions in a fixed order, so results are reproducible) ----
-constexpr int kGmres = 40;      // Krylov vectors per cycle
+constexpr int kGmres = 100;     // Krylov vectors per cycle
 constexpr int kDotBlocks = 128; // partial sums per dot product
 
 __global__ void k_scale_from_f32(double* y, const float* x, const double* s, long n) {
@@ -564,7 +563,7 @@ static int dots(ibem3d_ctx* ctx, const double* V, int k, const double* w, double
 // (compact storage)
 static void precond_solve(ibem3d_ctx* ctx, float* v) {
   if (ctx->prepared_mode & IBEM3D_SOLVE_COMPACT)
-    lu_solve_16(ctx->d_F16, ctx->n, ctx->ldf, ctx->d_F16s, ctx->f16_bf, ctx->W, v, ctx->st);
+    lu_solve_16(ctx->d_F16, ctx->n, ctx->ldf, ctx->d_F16s, ctx->W, v, ctx->st);
   else
     lu_solve_f32(ctx->d_A32, ctx->n, ctx->lda32, ctx->W, v, ctx->st);
 }
@@ -842,16 +841,18 @@ static int solve_mixed(ibem3d_ctx* ctx, int mode, double tol, bool gmres_first, 
         return fail(ctx, IBEM3D_EHIP, "solve preparation failed");
     }
     if (compact) {
-      // the factor in 16 bits beside A, then A assembled again over the FP32 factor
+      // the factor in FP16 (per-tile scaled) beside A, then A assembled again over the FP32 factor;
+      // FP16 storage for every product type: with the tile scaling its 11-bit significand is the
+      // more accurate 16-bit format (BF16 storage left the BF16-product solve of a 160,374-unknown
+      // RVE without GMRES convergence)
       ctx->ldf = ctx->lda32;
       if (!ctx->d_F16) {
         if (int rc = dalloc(ctx, &ctx->d_F16, (size_t)ctx->ldf * n)) return rc;
         if (int rc = dalloc(ctx, &ctx->d_F16s, factor16_scales(n))) return rc;
       }
-      ctx->f16_bf = mode == IBEM3D_SOLVE_MIXED_BF16;
       {
         Scope s(ctx, "solve.factor16");
-        factor_to_16(F32, n, ctx->lda32, ctx->d_F16, ctx->ldf, ctx->d_F16s, ctx->f16_bf, ctx->st);
+        factor_to_16(F32, n, ctx->lda32, ctx->d_F16, ctx->ldf, ctx->d_F16s, ctx->st);
       }
       if (int rc = after_launch(ctx, "factor16")) return rc;
       ctx->prepared_mode = mode | IBEM3D_SOLVE_COMPACT;

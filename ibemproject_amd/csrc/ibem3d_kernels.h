@@ -122,15 +122,14 @@ int equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c
 void equilibrate_in_place_f64(double* A, long n, long lda, double* r, double* c, double* ws, hipStream_t st);
 int equilibrate_to_f32_in_place(double* A, long n, double* r, double* c, long lda32, float* tmp,
                                 size_t tmp_floats, double* ws, hipStream_t st);
-// 16-bit copy of the FP32 factor: each 128 x 128 tile scaled by a power of two (largest magnitude in
-// [2^14, 2^15)) and stored in FP16 or BF16, column-major with leading dimension ldf; isc holds
-// factor16_scales(n) inverse scales. lu_solve_16 solves with it (FP32 vectors), using the inverses
-// of the diagonal blocks that lu_prepare_solve_f32 computed from the FP32 factor.
+// FP16 copy of the FP32 factor: each 128 x 128 tile scaled by a power of two (largest magnitude in
+// [2^14, 2^15)), column-major with leading dimension ldf; isc holds factor16_scales(n) inverse
+// scales. lu_solve_16 solves with it (FP32 vectors), using the inverses of the diagonal blocks that
+// lu_prepare_solve_f32 computed from the FP32 factor.
 size_t factor16_scales(long n);
-void factor_to_16(const float* LU, long n, long lda, unsigned short* F, long ldf, float* isc, bool bf16,
-                  hipStream_t st);
-void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, bool bf16, const LuWork& W,
-                 float* x, hipStream_t st);
+void factor_to_16(const float* LU, long n, long lda, unsigned short* F, long ldf, float* isc, hipStream_t st);
+void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, const LuWork& W, float* x,
+                 hipStream_t st);
 // FP64 -> FP32 copy of a column-major matrix
 void convert_f64_to_f32(const double* A, long n, long lda, float* B, long ldb, hipStream_t st);
 

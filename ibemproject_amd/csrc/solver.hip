@@ -1940,22 +1940,14 @@ struct FacPlain {  // the factor as the LU left it (FP32 or FP64)
   __device__ T at(long r, long c) const { return p[c * ld + r]; }
   __device__ T scale(long, long) const { return T(1); }
 };
-// 16-bit copy of the FP32 factor: FP16 (BF = false) or BF16 values of the kTB x kTB tile (rows
-// [kTB i, kTB i + kTB), columns [kTB j, ...)) scaled by a power of two, whose inverse is
-// isc[j * ntr + i]
-template <bool BF>
+// FP16 copy of the FP32 factor: the values of the kTB x kTB tile (rows [kTB i, kTB i + kTB),
+// columns [kTB j, ...)) scaled by a power of two, whose inverse is isc[j * ntr + i]
 struct Fac16 {
   const unsigned short* p;
   long ld;
   const float* isc;
   long ntr;
-  __device__ float at(long r, long c) const {
-    const unsigned short u = p[c * ld + r];
-    if constexpr (BF)
-      return __uint_as_float((unsigned)u << 16);
-    else
-      return (float)__builtin_bit_cast(_Float16, u);
-  }
+  __device__ float at(long r, long c) const { return (float)__builtin_bit_cast(_Float16, p[c * ld + r]); }
   __device__ float scale(long r, long c) const { return isc[(c / kTB) * ntr + r / kTB]; }
 };
 
@@ -2244,14 +2236,13 @@ void lu_solve_f32(const float* LU, long n, long lda, const LuWork& W, float* x, 
   lu_solve<float>(FacPlain<float>{LU, lda}, n, W, x, st);
 }
 
-// ============================================== 16-bit copy of the factor (compact storage) ======
+// ============================================== FP16 copy of the factor (compact storage) =======
 // Every kTB x kTB tile of the FP32 factor is scaled by the power of two that brings its largest
-// magnitude into [2^14, 2^15) and stored in FP16 or BF16; the tile's inverse scale goes to isc.
-// One workgroup per tile: the maximum, then the conversion (the second read hits the cache).
+// magnitude into [2^14, 2^15) and stored in FP16; the tile's inverse scale goes to isc. One
+// workgroup per tile: the maximum, then the conversion (the second read hits the cache).
 long factor16_tile_rows(long n) { return (n + kTB - 1) / kTB; }
 size_t factor16_scales(long n) { return (size_t)factor16_tile_rows(n) * factor16_tile_rows(n); }
 
-template <bool BF>
 __global__ __launch_bounds__(256) void k_factor16(const float* __restrict__ LU, long n, long lda,
                                                   unsigned short* __restrict__ F, long ldf,
                                                   float* __restrict__ isc) {
@@ -2271,34 +2262,19 @@ __global__ __launch_bounds__(256) void k_factor16(const float* __restrict__ LU, 
   const float sc = pow2_scale(mm);
   if (t == 0) isc[tj * ntr + ti] = 1.0f / sc;
   if (rl < rows)
-    for (int c = cg; c < cols; c += 2) {
-      const float v = LU[(c0 + c) * lda + r0 + rl] * sc;
-      unsigned short u;
-      if constexpr (BF)
-        u = __builtin_bit_cast(unsigned short, (__bf16)v);
-      else
-        u = __builtin_bit_cast(unsigned short, (_Float16)v);
-      F[(c0 + c) * ldf + r0 + rl] = u;
-    }
+    for (int c = cg; c < cols; c += 2)
+      F[(c0 + c) * ldf + r0 + rl] =
+          __builtin_bit_cast(unsigned short, (_Float16)(LU[(c0 + c) * lda + r0 + rl] * sc));
 }
 
-void factor_to_16(const float* LU, long n, long lda, unsigned short* F, long ldf, float* isc, bool bf16,
-                  hipStream_t st) {
+void factor_to_16(const float* LU, long n, long lda, unsigned short* F, long ldf, float* isc, hipStream_t st) {
   const long ntr = factor16_tile_rows(n);
-  const dim3 g((unsigned)ntr, (unsigned)ntr), b(256);
-  if (bf16)
-    hipLaunchKernelGGL(k_factor16<true>, g, b, 0, st, LU, n, lda, F, ldf, isc);
-  else
-    hipLaunchKernelGGL(k_factor16<false>, g, b, 0, st, LU, n, lda, F, ldf, isc);
+  hipLaunchKernelGGL(k_factor16, dim3((unsigned)ntr, (unsigned)ntr), dim3(256), 0, st, LU, n, lda, F, ldf, isc);
 }
 
-void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, bool bf16, const LuWork& W,
-                 float* x, hipStream_t st) {
-  const long ntr = factor16_tile_rows(n);
-  if (bf16)
-    lu_solve<float>(Fac16<true>{F, ldf, isc, ntr}, n, W, x, st);
-  else
-    lu_solve<float>(Fac16<false>{F, ldf, isc, ntr}, n, W, x, st);
+void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, const LuWork& W, float* x,
+                 hipStream_t st) {
+  lu_solve<float>(Fac16{F, ldf, isc, factor16_tile_rows(n)}, n, W, x, st);
 }
 
 // ========================================================================= equilibration ========

@@ -83,8 +83,8 @@ enum {
                                    hold that copy). FP64_LU: A <- R A C and its LU in place; the
                                    residual is not computed (ibem3d_residual assembles A again for
                                    it). Mixed modes: FP32 R A C converted in place, factored, then
-                                   stored as a 16-bit copy (BF16 for MIXED_BF16, else FP16; every
-                                   128 x 128 tile scaled by a power of two), A assembled again over
+                                   stored as an FP16 copy (every 128 x 128 tile scaled by a power
+                                   of two), A assembled again over
                                    the FP32 factor, and FP64 GMRES-IR against A preconditioned by
                                    the 16-bit factor. Peak memory 8 n^2 + 2 n^2 bytes instead of
                                    8 n^2 + 4 n^2 (mixed) or 16 n^2 (FP64). */
