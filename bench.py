@@ -1,15 +1,19 @@
 """Benchmark: one iBEM elastic-3D RVE solve on one MI355X.
 
 A step is one pass of the hot path over one RVE whose inputs are already resident on the device:
-assembly of the dense system (all phases), the solve to a relative residual of 1e-10, and the
+assembly of the dense system (all phases), the solve to a relative FP64 residual of 1e-10, and the
 stresses at the RVE's 1000 points. The timed solve is the mixed-precision path: an FP32 LU whose
-trailing updates run on FP16 MFMA with scaled hi/lo split operands, then FP64 iterative refinement
-against the FP64 system. One step with the plain FP64 LU, and one with the FP32-MFMA LU, are timed
-beside it for the comparison the metric asks for.
+trailing updates run as single FP16 MFMA products of operands scaled per row / column by powers of
+two (FP32 accumulation), then FP64 GMRES-IR against the FP64 system. On the 2,500-sphere RVE the
+FP64 A takes 205.8 GB of the 288 GB, so the solve uses compact storage: the FP32 factor is built in
+A's memory, kept as a per-tile-scaled FP16 copy beside A, and A is assembled again for the FP64
+residual. The plain FP64 LU (in place of A) is timed beside it, with the FP16 split-product (three
+products) LU; every result states its error against the FP64 LU.
 
-Workload (default): 1000 spheres, radii U[0.02, 0.03], surface gap 0.01, wall clearance 0.02, in a
-unit cube meshed with 16 x 16 four-node quads per face; E0 = 1, nu0 = 0.3, E1 = 10, nu1 = 0.25;
-traction 1 on x = 1 with symmetric supports; n = 3 * 1538 + 60 * 1000 = 64,614 unknowns.
+Workload (default, rve2500_m24): 2500 spheres, radii U[0.015, 0.022], surface gap 0.01, wall
+clearance 0.02, in a unit cube meshed with 24 x 24 four-node quads per face; E0 = 1, nu0 = 0.3,
+E1 = 10, nu1 = 0.25; traction 1 on x = 1 with symmetric supports; n = 3 * 3458 + 60 * 2500 =
+160,374 unknowns.
 """
 from __future__ import annotations
 
@@ -25,24 +29,35 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from ibemproject_amd.engine import SOLVE_FP64_LU, SOLVE_MIXED_FP16, SOLVE_MIXED_FP32, Engine  # noqa: E402
+from ibemproject_amd.engine import (SOLVE_FP64_LU, SOLVE_MIXED_BF16, SOLVE_MIXED_FP16,  # noqa: E402
+                                    SOLVE_MIXED_FP16_1, SOLVE_MIXED_FP32, Engine)
 from ibemproject_amd.generate import make_workload  # noqa: E402
 
 MI355X_PEAK = {"f16_mfma_tflops": 2500.0, "f32_mfma_tflops": 157.3, "f64_mfma_tflops": 78.6,
                "hbm_gbs": 8000.0}
 NB = 1024  # LU block size (solver.hip IBEM_LU_NB)
-# HBM bytes per trailing-update launch on rve1000_m16 (FETCH_SIZE x 2.667 + WRITE_SIZE x 1.0 per
-# dispatch, averaged over one LU's 125 launches of k_gemm_f16x3_256 / k_gemm_f16x3; the factors make
-# the counters read 1 GiB for a 1 GiB copy), from the counter runs recorded in
-# profiles/k_gemm_f16x3_256__rve1000_m16.md
-GEMM_TRAFFIC_BYTES_PER_LAUNCH = 1.6204e10
+DEFAULT_WORKLOAD = "rve2500_m24"
+MODES = {"fp16_1": SOLVE_MIXED_FP16_1, "fp16x3": SOLVE_MIXED_FP16, "bf16": SOLVE_MIXED_BF16,
+         "fp32": SOLVE_MIXED_FP32}
+# the trailing-update kernel of each mode (k_gemm_f16x3_256 / k_gemm_f16x3 instantiated for the
+# product type) and its MFMA products per algorithmic multiply-add
+GEMM_KERNEL = {"fp16_1": ("k_gemm_f16x3_256<kProdF16> (LU trailing update: persistent 256x256 tiles, "
+                          "direct global-to-LDS loads, one v_mfma_f32_16x16x32_f16 product of operands "
+                          "scaled per row / column by powers of two; k_gemm_f16x3<..., kProdF16> "
+                          "128x128 for updates under 512 tiles)", 1),
+               "fp16x3": ("k_gemm_f16x3_256<kProdF16x3> (three FP16 products of hi/lo splits)", 3),
+               "bf16": ("k_gemm_f16x3_256<kProdBF16> (one v_mfma_f32_16x16x32_bf16 product)", 1),
+               "fp32": ("k_gemm_sub<float> (FP32 MFMA)", 1)}
+# HBM bytes per trailing-update launch (FETCH_SIZE x f + WRITE_SIZE x 1.0 per dispatch, averaged over
+# one LU's launches, f the factor that makes the counter read 1 GiB for a 1 GiB copy in the same
+# session), from the counter runs recorded under profiles/
+GEMM_TRAFFIC_BYTES_PER_LAUNCH = {("rve1000_m16", "fp16x3"): 1.6204e10}
 
 
 def trailing_update_flops(n: int) -> tuple[float, int]:
     """Algorithmic flops of the blocked LU's trailing-matrix updates (2 m^2 kb per step, m the
-    trailing size) and the number of split-GEMM launches (k_gemm_f16x3_256, or k_gemm_f16x3 for
-    small updates): two per step (the next panel's columns, then the rest), one when the next panel
-    is the whole trailing matrix."""
+    trailing size) and the number of trailing-update GEMM launches: two per step (the next panel's
+    columns, then the rest), one when the next panel is the whole trailing matrix."""
     flops = 0.0
     launches = 0
     for k in range(0, n, NB):
@@ -54,9 +69,10 @@ def trailing_update_flops(n: int) -> tuple[float, int]:
     return flops, launches
 
 
-def trailing_update_bytes(n: int) -> float:
-    """Algorithmic bytes per split-GEMM launch, averaged over one LU: the split operands (hi + lo,
-    2 bytes each) read once, C read and written once in FP32 (4 m kb + 4 kb nc + 8 m nc)."""
+def trailing_update_bytes(n: int, operand_bytes: int) -> float:
+    """Algorithmic bytes per trailing-update launch, averaged over one LU: the 16-bit operands
+    (operand_bytes per element: 2 for one product, 4 for hi + lo) read once, C read and written
+    once in FP32."""
     total, launches = 0.0, 0
     for k in range(0, n, NB):
         kb = min(NB, n - k)
@@ -65,7 +81,7 @@ def trailing_update_bytes(n: int) -> float:
             continue
         kb2 = min(NB, m)
         for nc in ([kb2, m - kb2] if m > kb2 else [kb2]):
-            total += 4.0 * m * kb + 4.0 * kb * nc + 8.0 * m * nc
+            total += operand_bytes * (m * kb + kb * nc) + 8.0 * m * nc
             launches += 1
     return total / launches if launches else 0.0
 
@@ -94,23 +110,32 @@ def run_step(eng: Engine, mode: int, tol: float = 1e-10):
     return rel, it, s
 
 
-def cpu_baseline(prob, workload: str):
-    """Reference CPU path (oracle/_ref/libibemref.so, the reference's own sources compiled here)
-    on a bounded sample of the workload, extrapolated to the full RVE."""
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(prob, workload: str, ns: int = 16, ms: int = 3):
+    """Reference CPU path (oracle/_ref/libibemref.so, the reference's own sources compiled here) on a
+    bounded sample of the workload, extrapolated to the full RVE: the assembly phases on the RVE's
+    first ns spheres with an ms x ms-per-face mesh (each phase scaled by its pair count), Eigen's
+    FullPivLU (the reference's solver, one thread) at three sizes with the exponent fitted."""
     import ctypes
     lib = os.path.join(ROOT, "oracle", "_ref", "libibemref.so")
     if not os.path.exists(lib):
         return None
     from ibemproject_amd.formats import Problem, write_problem
+    from ibemproject_amd.generate import cube_mesh, symmetric_support_bc
     L = ctypes.CDLL(lib)
     L.ref_run.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
     L.ref_fullpiv_solve.restype = ctypes.c_double
     dp = ctypes.POINTER(ctypes.c_double)
     L.ref_fullpiv_solve.argtypes = [ctypes.c_int, dp, dp, dp]
-    # sample: the RVE's first 3 spheres on a 6 x 6-per-face mesh with the same boundary conditions;
-    # per-pair costs do not depend on the mesh density, so each phase is scaled by its pair count
-    from ibemproject_amd.generate import cube_mesh, symmetric_support_bc
-    ns, ms = 3, 6
     nodes, conn, face = cube_mesh(ms)
     bc, tbc = symmetric_support_bc(nodes, conn, face)
     sample = Problem(prob.title, prob.E0, prob.nu0, prob.E1, prob.nu1, nodes, conn, bc, tbc,
@@ -139,33 +164,38 @@ def cpu_baseline(prob, workload: str):
         for ln in open(os.path.join(dout, "times.txt")):
             k, v = ln.split()
             ph[k] = float(v)
-    nlu = 1200
+    wall_asm = time.time() - t0
     rng = np.random.default_rng(0)
-    A = rng.standard_normal((nlu, nlu)) + nlu * np.eye(nlu)
-    A = np.asfortranarray(A)
-    b = rng.standard_normal(nlu)
-    x = np.zeros(nlu)
-    t_lu = L.ref_fullpiv_solve(nlu, A.ctypes.data_as(dp), b.ctypes.data_as(dp), x.ctypes.data_as(dp))
-    wall = time.time() - t0
-    num = prob.num
-    n78 = prob.n78
+    lu_pts = []
+    for nlu in (1024, 1448, 2048):
+        A = np.asfortranarray(rng.standard_normal((nlu, nlu)) + nlu * np.eye(nlu))
+        b = rng.standard_normal(nlu)
+        x = np.zeros(nlu)
+        lu_pts.append((nlu, L.ref_fullpiv_solve(nlu, A.ctypes.data_as(dp), b.ctypes.data_as(dp),
+                                                 x.ctypes.data_as(dp))))
+    expo, lnc = np.polyfit(np.log([p[0] for p in lu_pts]), np.log([p[1] for p in lu_pts]), 1)
+    num, n78 = prob.num, prob.n78
     ne_s, nn_s = sample.NE, sample.NN
-    est = (ph["addBEM"] * (prob.NE * prob.NN) / (ne_s * nn_s)
-           + ph["addFieldToBEM"] * (prob.NN * num) / (nn_s * ns)
-           + (ph["incl0"] + ph["incl1"] + ph["incl2"]) * (num / ns) ** 2
-           + (ph["bem0"] + ph["bem1"] + ph["bem2"]) * (prob.NE * num) / (ne_s * ns)
-           + t_lu * (n78 / nlu) ** 3)
+    t_lu = float(np.exp(lnc) * n78 ** expo)
+    est_asm = (ph["addBEM"] * (prob.NE * prob.NN) / (ne_s * nn_s)
+               + ph["addFieldToBEM"] * (prob.NN * num) / (nn_s * ns)
+               + (ph["incl0"] + ph["incl1"] + ph["incl2"]) * (num / ns) ** 2
+               + (ph["bem0"] + ph["bem1"] + ph["bem2"]) * (prob.NE * num) / (ne_s * ns))
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     fair = fair_cpu_lu(prob.n60)
-    return {"value": est, "unit": "s per RVE solve (extrapolated)", "cores": cores,
-            "kind": "reference",
-            "sample": (f"reference assembly phases (OpenMP {cores} threads) on {ns} of the {workload} "
-                       f"spheres with a {ms}x{ms}-per-face mesh, {wall:.1f} s wall, each phase scaled "
-                       f"by its pair count (element-node, node-sphere, sphere-sphere, "
-                       f"element-sphere) to the full RVE; Eigen FullPivLU (1 thread) timed at "
-                       f"n={nlu} ({t_lu:.2f} s) and scaled by (n78/{nlu})^3 with n78={n78}; "
-                       f"extrapolated, not run end to end"),
-            "phases_s": ph, "fullpiv_s_at_n": [nlu, t_lu], "fair_lu": fair}
+    return {"value": est_asm + t_lu, "unit": "s per RVE solve (extrapolated)", "cores": cores,
+            "cpu_model": cpu_model(), "kind": "reference",
+            "sample": (f"reference assembly phases (OpenMP, {cores} threads) on {ns} of the {workload} "
+                       f"spheres with a {ms}x{ms}-per-face mesh ({wall_asm:.1f} s wall), each phase "
+                       f"scaled by its pair count (element-node, node-sphere, sphere-sphere, "
+                       f"element-sphere) to the full RVE: {est_asm:.0f} s; Eigen FullPivLU (the "
+                       f"reference's solver, 1 thread) timed at n = "
+                       f"{', '.join(str(p[0]) for p in lu_pts)}, fitted t ~ n^{expo:.2f} and "
+                       f"extrapolated to the reference's n78 = {n78} unknowns: {t_lu:.3g} s; not run "
+                       f"end to end"),
+            "phases_s": ph, "assembly_s_extrapolated": est_asm,
+            "fullpiv_lu": {"points": lu_pts, "exponent": expo, "s_at_n78": t_lu, "n78": n78},
+            "fair_lu": fair}
 
 
 def fair_cpu_lu(n60: int, nlu: int = 6000) -> dict:
@@ -187,15 +217,66 @@ def fair_cpu_lu(n60: int, nlu: int = 6000) -> dict:
             "note": "LAPACK dgetrf (blocked, partial pivoting) through scipy, extrapolated by n^3"}
 
 
+def timed(eng: Engine, hip, fn):
+    eng.reset_timings()
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    hip.hipDeviceSynchronize()
+    return out, time.perf_counter() - t0, eng.timings()
+
+
+def side_step(eng: Engine, hip, mode: int, lu_key: str, flops: float, x_ref, s_ref):
+    """One untimed-region step of another mode: its time, LU rate and error against x_ref / s_ref."""
+    (rel, it, s), t, tm = timed(eng, hip, lambda: run_step(eng, mode))
+    if rel < 0:  # compact FP64: A holds the factor; assembled again for the residual (not timed)
+        rel = eng.residual()
+    _, _, x = eng.getUTVector()
+    lu = tm.get(lu_key)
+    g = tm.get("lu.gemm", 0.0)
+    return {"step_s": t, "rel_residual": rel, "refinement_iterations": it, "lu_ms": lu,
+            "lu_tflops": 2.0 * len(x) ** 3 / 3 / (lu * 1e-3) / 1e12 if lu else None,
+            "trailing_gemm_tflops": flops / (g * 1e-3) / 1e12 if g else None,
+            "x_relerr_vs_ref": float(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)),
+            "stress_relerr_vs_ref": float(np.abs(s - s_ref).max() / np.abs(s_ref).max()),
+            "phase_ms": tm}, x, s
+
+
+def contrasts(args, hip):
+    """Particle/matrix stiffness contrasts E1/E0 = 2, 10, 100 on the workload's geometry: the timed
+    mode and the plain FP64 LU on each, with the error of the first against the second."""
+    out = []
+    for ratio in (2.0, 10.0, 100.0):
+        prob = make_workload(args.workload, seed=args.seed, E1=ratio)
+        eng = Engine(prob)
+        (rel, it, s), t, tm = timed(eng, hip, lambda: run_step(eng, MODES[args.mode]))
+        _, _, x = eng.getUTVector()
+        (rel64, _, s64), t64, _ = timed(eng, hip, lambda: run_step(eng, SOLVE_FP64_LU))
+        rel64 = eng.residual()
+        _, _, x64 = eng.getUTVector()
+        eng.close()
+        rec = {"E1_over_E0": ratio, "nu1": prob.nu1, "step_s": t, "rel_residual": rel,
+               "iterations": it, "fp64_step_s": t64, "fp64_rel_residual": rel64,
+               "x_relerr_vs_fp64_lu": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
+               "stress_relerr_vs_fp64_lu": float(np.abs(s - s64).max() / np.abs(s64).max())}
+        progress(f"contrast {ratio:g}: {rec}")
+        out.append(rec)
+    print(json.dumps({"workload": args.workload, "mode": args.mode, "contrasts": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rve1000_m16")
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD)
+    ap.add_argument("--mode", default="fp16_1", choices=sorted(MODES))
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP16 split-product step")
+    ap.add_argument("--contrasts", action="store_true",
+                    help="run the E1/E0 = 2, 10, 100 sweep (timed mode and FP64 LU) and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -207,18 +288,21 @@ def main():
               "supported (a multi-GPU run would shard the LU block-cyclically by 1024-column panels)",
               file=sys.stderr)
         sys.exit(2)
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    if args.contrasts:
+        contrasts(args, hip)
+        return
 
     prob = make_workload(args.workload, seed=args.seed)
     eng = Engine(prob)
     n = eng.n
-    progress(f"workload {args.workload}: n = {n}")
-    MODE = SOLVE_MIXED_FP16
+    progress(f"workload {args.workload}: n = {n}, mode {args.mode}")
+    MODE = MODES[args.mode]
     for i in range(args.warmup):
         run_step(eng, MODE)
         progress(f"warmup step {i + 1}/{args.warmup} done")
     eng.reset_timings()
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
     hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
     rels = []
@@ -227,40 +311,33 @@ def main():
         rels.append((rel, it))
     hip.hipDeviceSynchronize()
     t1 = time.perf_counter()
-    progress(f"{args.steps} timed steps done: {(t1 - t0) / args.steps:.3f} s per step")
-    tm = eng.timings()
     step_s = (t1 - t0) / args.steps
+    progress(f"{args.steps} timed steps done: {step_s:.3f} s per step")
+    tm = eng.timings()
+    _, _, x = eng.getUTVector()
     flops, launches = trailing_update_flops(n)
     gemm_ms = tm.get("lu.gemm", 0.0) / args.steps
     achieved = flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
     asm_ms = sum(v for k, v in tm.items() if k.startswith("assemble.")) / args.steps
     lu_ms = tm.get("solve.lu_f32", 0.0) / args.steps
 
-    def one_step(mode, lu_key):
-        eng.reset_timings()
-        hip.hipDeviceSynchronize()
-        f0 = time.perf_counter()
-        rel_o, it_o, s_o = run_step(eng, mode)
-        hip.hipDeviceSynchronize()
-        f1 = time.perf_counter()
-        t_o = eng.timings()
-        g_o = t_o.get("lu.gemm", 0.0)
-        return {"step_s": f1 - f0, "rel_residual": rel_o, "refinement_iterations": it_o,
-                "lu_ms": t_o.get(lu_key), "lu_tflops": 2.0 * n ** 3 / 3 / (t_o.get(lu_key) * 1e-3) / 1e12,
-                "trailing_gemm_tflops": flops / (g_o * 1e-3) / 1e12 if g_o else None,
-                "stress_maxdiff_vs_timed": float(np.abs(s_o - s).max() / np.abs(s_o).max()),
-                "phase_ms": t_o}
-
-    fp64 = None if args.no_fp64 else one_step(SOLVE_FP64_LU, "solve.lu_f64")
-    if fp64:
+    fp64 = None
+    if not args.no_fp64:
+        fp64, x64, s64 = side_step(eng, hip, SOLVE_FP64_LU, "solve.lu_f64", flops, x, s)
         progress(f"FP64 LU step: {fp64['step_s']:.3f} s")
-    fp32 = None if args.no_fp64 else one_step(SOLVE_MIXED_FP32, "solve.lu_f32")
-    if fp32:
-        progress(f"FP32-MFMA mixed step: {fp32['step_s']:.3f} s")
+    cmp_mode = "fp16x3" if args.mode != "fp16x3" else "fp16_1"
+    other = None
+    if not args.no_compare:
+        ref_x, ref_s = (x64, s64) if fp64 else (x, s)
+        other, _, _ = side_step(eng, hip, MODES[cmp_mode], "solve.lu_f32", flops, ref_x, ref_s)
+        progress(f"{cmp_mode} step: {other['step_s']:.3f} s")
 
     base = None if args.no_cpu_baseline else cpu_baseline(prob, args.workload)
     if base:
-        progress(f"CPU baseline: {base['value']:.1f} s (extrapolated)")
+        progress(f"CPU baseline: {base['value']:.4g} s (extrapolated)")
+    kname, prods = GEMM_KERNEL[args.mode]
+    op_bytes = 4 if args.mode == "fp16x3" else (4 if args.mode == "fp32" else 2)
+    peak = MI355X_PEAK["f32_mfma_tflops"] if args.mode == "fp32" else MI355X_PEAK["f16_mfma_tflops"]
     out = {
         "metric": "RVE solve time at 1e-10 FP64 residual vs plain FP64 LU; assembly/LU TFLOP/s",
         "value": step_s,
@@ -272,40 +349,39 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32 (fp16 mfma split products)",
+        "dtype": ("f64 assembly, residual and solution update; f32 LU with f16 mfma products "
+                  "(one per multiply-add, per-row/column pow2 scaling) and f32 accumulation; "
+                  "f16 factor copy (per-tile pow2 scaling)" if args.mode == "fp16_1" else args.mode),
         "data": "synthetic (seeded RSA spheres, generated mesh)",
         "config": {"workload": args.workload, "spheres": prob.num, "nodes": prob.NN,
                    "elements": prob.NE, "unknowns": n, "points": prob.nump,
-                   "solver": "fp32 LU (trailing updates on fp16 MFMA, scaled hi/lo split) + fp64 "
-                             "iterative refinement to 1e-10",
+                   "solver": f"{args.mode} LU + fp64 GMRES-IR / iterative refinement to 1e-10",
                    "parallelism": "single GPU"},
         "rel_residual": rels[-1][0],
         "refinement_iterations": rels[-1][1],
+        "error_vs_fp64_lu": ({"x_rel": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
+                              "stress_rel": float(np.abs(s - s64).max() / np.abs(s64).max())}
+                             if fp64 else None),
         "assembly_ms": asm_ms,
         "lu_ms": lu_ms,
         "lu_tflops_effective": 2.0 * n ** 3 / 3 / (lu_ms * 1e-3) / 1e12 if lu_ms else None,
         "phase_ms": {k: v / args.steps for k, v in tm.items()},
         "fp64_lu_step": fp64,
-        "fp32_mixed_step": fp32,
+        f"{cmp_mode}_step": other,
         "speedup_vs_fp64_lu": (fp64["step_s"] / step_s) if fp64 else None,
-        "roofline": {"kernel": "k_gemm_f16x3_256 (LU trailing update: persistent 256x256 tiles, "
-                               "direct global-to-LDS loads, v_mfma_f32_16x16x32_f16, three products "
-                               "per block; k_gemm_f16x3 128x128 for updates under 512 tiles)",
-                     "bound": "mfma", "achieved": achieved,
-                     "peak": MI355X_PEAK["f16_mfma_tflops"], "unit": "TFLOP/s",
-                     "frac": achieved / MI355X_PEAK["f16_mfma_tflops"] if achieved else None,
-                     "traffic": GEMM_TRAFFIC_BYTES_PER_LAUNCH if args.workload == "rve1000_m16" else None,
+        "roofline": {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak if achieved else None,
+                     "traffic": GEMM_TRAFFIC_BYTES_PER_LAUNCH.get((args.workload, args.mode)),
                      "traffic_unit": "bytes per launch (HBM, counter-measured, averaged over the "
                                      "step's launches)",
-                     "traffic_algorithmic": trailing_update_bytes(n),
+                     "traffic_algorithmic": trailing_update_bytes(n, op_bytes),
                      "launches_per_step": launches,
                      "avg_launch_ms": gemm_ms / launches if launches else None,
                      "flops_per_step": flops,
-                     "mfma_issued_tflops": 3 * achieved if achieved else None,
+                     "mfma_issued_tflops": prods * achieved if achieved else None,
                      "note": f"achieved = algorithmic trailing-update flops (2 m^2 {NB} per LU "
-                             "step) / the split-GEMM time of the step (both kernels, HIP events "
-                             "around each launch); the kernels issue three fp16 MFMA products per "
-                             "algorithmic one"},
+                             "step) / the trailing-update GEMM time of the step (HIP events around "
+                             "each launch on the stream it runs on)"},
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)
