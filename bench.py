@@ -51,7 +51,7 @@ GEMM_KERNEL = {"fp16_1": ("k_gemm_f16x3_256<kProdF16> (LU trailing update: persi
 # HBM bytes per trailing-update launch (FETCH_SIZE x f + WRITE_SIZE x 1.0 per dispatch, averaged over
 # one LU's launches, f the factor that makes the counter read 1 GiB for a 1 GiB copy in the same
 # session), from the counter runs recorded under profiles/
-GEMM_TRAFFIC_BYTES_PER_LAUNCH = {("rve1000_m16", "fp16x3"): 1.6204e10}
+GEMM_TRAFFIC_BYTES_PER_LAUNCH = {("rve1000_m16", "fp16x3"): 1.6204e10, ("rve1000_m16", "fp16_1"): 9.92e9}
 
 
 def trailing_update_flops(n: int) -> tuple[float, int]:
