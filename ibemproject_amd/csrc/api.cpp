@@ -924,17 +924,19 @@ int ibem3d_solve_info(const ibem3d_ctx* ctx, double* rel, int* iters) {
 
 // New known boundary values with the same known/unknown pattern: the folded system matrix does not
 // depend on the values, only b does, so the factors stay valid.
-int ibem3d_set_known_values(ibem3d_ctx* ctx, const double* bc_values, const double* tbc_values) {
-  CTX_CHECK();
-  if (!bc_values || !tbc_values) return fail(ctx, IBEM3D_EINVAL, "set_known_values: null values");
+static int upload_known_values(ibem3d_ctx* ctx, const double* bc_values, const double* tbc_values) {
+  if (!bc_values || !tbc_values) return fail(ctx, IBEM3D_EINVAL, "known values: null values");
   const int NN = ctx->NN, NE = ctx->NE;
   std::vector<double> uval(3L * NN), tval(12L * NE);
   for (long j = 0; j < 3L * NN; ++j) {
+    if (!std::isfinite(bc_values[j])) return fail(ctx, IBEM3D_EINVAL, "known values: non-finite value");
     ctx->h_bc[2 * j + 1] = bc_values[j];
     uval[j] = bc_values[j];
   }
   for (long e = 0; e < NE; ++e)
     for (int i = 0; i < 12; ++i) {
+      if (!std::isfinite(tbc_values[12 * e + i]))
+        return fail(ctx, IBEM3D_EINVAL, "known values: non-finite value");
       ctx->h_tbc[24 * e + 12 + i] = tbc_values[12 * e + i];
       tval[12 * e + i] = tbc_values[12 * e + i];
     }
@@ -943,7 +945,38 @@ int ibem3d_set_known_values(ibem3d_ctx* ctx, const double* bc_values, const doub
   HIPC(hipMemcpyAsync(ctx->d_tval, tval.data(), sizeof(double) * tval.size(), hipMemcpyHostToDevice,
                       ctx->st));
   HIPC(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int ibem3d_set_known_values(ibem3d_ctx* ctx, const double* bc_values, const double* tbc_values) {
+  CTX_CHECK();
+  if (int rc = upload_known_values(ctx, bc_values, tbc_values)) return rc;
   return ibem3d_reset(ctx);
+}
+
+// b only (BEMBuilderElastic3D::ApplyBC's known-value folds, BEMBuilderElastic3D.cpp:1973-1990, of the
+// boundary and boundary->inclusion phases), in the order of the full assembly; A, or the factor
+// that occupies its memory, is not touched.
+int ibem3d_update_known_values(ibem3d_ctx* ctx, const double* bc_values, const double* tbc_values) {
+  CTX_CHECK();
+  if (ctx->phases != 0xFFu)
+    return fail(ctx, IBEM3D_ESTATE, "update_known_values: the system must have been assembled");
+  if (int rc = upload_known_values(ctx, bc_values, tbc_values)) return rc;
+  HIPC(hipMemsetAsync(ctx->d_b, 0, sizeof(double) * ctx->n, ctx->st));
+  {
+    Scope s(ctx, "assemble.rhs");
+    if (!ctx->d_bemws)
+      if (int rc = dalloc(ctx, &ctx->d_bemws, bem_workspace_doubles(ctx->NN, ctx->NE))) return rc;
+    launch_bem(ctx->d_qp, ctx->d_conn, ctx->d_nodes, ctx->NN, ctx->NE, ctx->n, ctx->E0, ctx->nu0,
+               ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b,
+               ctx->d_bemws, ctx->st, false);
+    if (ctx->num > 0)
+      launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, ctx->n, 3L * ctx->NN, ctx->M, 7,
+                  ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b,
+                  ctx->st, false);
+  }
+  ctx->solved = false;
+  return after_launch(ctx, "update_known_values");
 }
 
 // enclosed volume (1/3) int x . n dS on the quadrature table (the same sum as ibem3d_average_fields)

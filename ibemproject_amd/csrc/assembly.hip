@@ -271,7 +271,7 @@ __global__ __launch_bounds__(256) void k_bem_fold(const int* __restrict__ conn, 
                                                   const double* __restrict__ tval,
                                                   const double* __restrict__ W,
                                                   double* __restrict__ carry, int first, int last,
-                                                  double* __restrict__ A) {
+                                                  double* __restrict__ A, int write_A) {
   const long R = 3L * NN;
   const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int c = blockIdx.y;
@@ -288,26 +288,29 @@ __global__ __launch_bounds__(256) void k_bem_fold(const int* __restrict__ conn, 
       const double hv = in[(12 + 3 * q + c) * R];
       rs += hv;
       const int col = node_col[dof];
-      if (col >= 0)
-        A[(long)col * n + row] += hv;
-      else
+      if (col >= 0) {
+        if (write_A) A[(long)col * n + row] += hv;
+      } else {
         bacc -= hv * uval[dof];
+      }
       const int slot = 12 * e + 3 * q + c;
       const double gv = in[(3 * q + c) * R];
       const int kc = slot_col[slot];
-      if (kc >= 0)
-        A[(long)kc * n + row] -= gv;
-      else if (kc == -1)
+      if (kc >= 0) {
+        if (write_A) A[(long)kc * n + row] -= gv;
+      } else if (kc == -1) {
         bacc += gv * tval[slot];
+      }
     }
   }
   if (last) {
     const int dof = 3 * (int)(row / 3) + c;
     const int col = node_col[dof];
-    if (col >= 0)
-      A[(long)col * n + row] -= rs;
-    else
+    if (col >= 0) {
+      if (write_A) A[(long)col * n + row] -= rs;
+    } else {
       bacc += rs * uval[dof];
+    }
   }
   carry[c * R + row] = rs;
   carry[(3 + c) * R + row] = bacc;
@@ -342,7 +345,8 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
                                               const int* __restrict__ slot_col,
                                               const double* __restrict__ uval,
                                               const double* __restrict__ tval,
-                                              double* __restrict__ A, double* __restrict__ b) {
+                                              double* __restrict__ A, double* __restrict__ b,
+                                              int write_A) {
   constexpr int QN = 64 * kQpStride;             // quadrature data of one element
   constexpr int QPT = (QN + 255) / 256;           // per thread
   __shared__ double sq[2][QN];                    // current / next element
@@ -451,7 +455,7 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
         const int og = o < 12 ? o : o - 12;
         const int k = 3 * (og % 4) + og / 4;  // 3 q + n
         col[oo] = scol[cur][(o < 12 ? 0 : 12) + k];
-        old[oo] = col[oo] >= 0 ? A[(long)col[oo] * n + row] : 0.0;
+        old[oo] = col[oo] >= 0 && write_A ? A[(long)col[oo] * n + row] : 0.0;
       }
 #pragma unroll
       for (int oo = 0; oo < 6; ++oo) {
@@ -460,16 +464,18 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
         const int k = 3 * (og % 4) + og / 4;
         if (o < 12) {
           const double hv = -fac * v[oo];
-          if (col[oo] >= 0)
-            A[(long)col[oo] * n + row] = old[oo] + hv;
-          else
+          if (col[oo] >= 0) {
+            if (write_A) A[(long)col[oo] * n + row] = old[oo] + hv;
+          } else {
             bacc -= hv * sknown[cur][k];
+          }
         } else {
           const double gv = fac * v[oo];
-          if (col[oo] >= 0)
-            A[(long)col[oo] * n + row] = old[oo] - gv;
-          else
+          if (col[oo] >= 0) {
+            if (write_A) A[(long)col[oo] * n + row] = old[oo] - gv;
+          } else {
             bacc += gv * sknown[cur][12 + k];  // zero unless the traction is known
+          }
         }
       }
     }
@@ -502,7 +508,7 @@ size_t bem_workspace_doubles(int NN, int NE) {
 
 void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, int NE, long n,
                 double E0, double nu0, const int* node_col, const int* slot_col, const double* uval,
-                const double* tval, double* A, double* b, double* ws, hipStream_t st) {
+                const double* tval, double* A, double* b, double* ws, hipStream_t st, bool write_A) {
   const long R = 3L * NN;
   double* W = ws;
   double* carry = ws + (size_t)std::min(NE, kBemChunk) * 24 * R;
@@ -512,16 +518,17 @@ void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, 
                        qp, conn, nodes, NN, e0, ne, E0, nu0, W);
     hipLaunchKernelGGL(k_bem_fold, dim3((unsigned)((R + 255) / 256), 3), dim3(256), 0, st, conn, NN, e0,
                        ne, n, node_col, slot_col, uval, tval, W, carry, (int)(e0 == 0),
-                       (int)(e0 + ne >= NE), A);
+                       (int)(e0 + ne >= NE), A, (int)write_A);
   }
   hipLaunchKernelGGL(k_bem_finish, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st, NN, carry, b);
 }
 
 void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long n,
                  long row0, const Mat& M, int rowmask, const int* node_col, const int* slot_col,
-                 const double* uval, const double* tval, double* A, double* b, hipStream_t st) {
+                 const double* uval, const double* tval, double* A, double* b, hipStream_t st,
+                 bool write_A) {
   hipLaunchKernelGGL(k_rows, dim3(num), dim3(256), 0, st, qp, conn, cen, NE, n, row0, M, rowmask,
-                     node_col, slot_col, uval, tval, A, b);
+                     node_col, slot_col, uval, tval, A, b, (int)write_A);
 }
 
 }  // namespace ibem

@@ -23,10 +23,13 @@ constexpr int kBemChunk = 256;
 size_t bem_workspace_doubles(int NN, int NE);
 void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, int NE, long n,
                 double E0, double nu0, const int* node_col, const int* slot_col, const double* uval,
-                const double* tval, double* A, double* b, double* ws, hipStream_t st);
+                const double* tval, double* A, double* b, double* ws, hipStream_t st,
+                bool write_A = true);
 void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long n,
                  long row0, const Mat& M, int rowmask, const int* node_col, const int* slot_col,
-                 const double* uval, const double* tval, double* A, double* b, hipStream_t st);
+                 const double* uval, const double* tval, double* A, double* b, hipStream_t st,
+                 bool write_A = true);
+// (write_A = false: b only, accumulated in the same order as with the matrix entries)
 
 // ---- postprocess (post.hip) ----
 void launch_stress(const double* qp, const int* conn, const double* nodes, int NN, int NE,

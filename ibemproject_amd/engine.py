@@ -44,7 +44,8 @@ EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size"
            "ibem3d_get_system", "ibem3d_solve_info", "ibem3d_timings", "ibem3d_reset_timings",
            "ibem3d_read_inputs", "ibem3d_inputs_problem", "ibem3d_inputs_title",
            "ibem3d_free_inputs", "ibem3d_io_error", "ibem3d_write_stress", "ibem3d_average_fields",
-           "ibem3d_set_known_values", "ibem3d_average_eigenstrain", "ibem3d_residual"]
+           "ibem3d_set_known_values", "ibem3d_average_eigenstrain", "ibem3d_residual",
+           "ibem3d_update_known_values"]
 
 
 class EngineError(RuntimeError):
@@ -93,6 +94,7 @@ def load(path: str = LIB_PATH):
     L.ibem3d_set_known_values.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.ibem3d_average_eigenstrain.argtypes = [ctypes.c_void_p, _dp]
     L.ibem3d_residual.argtypes = [ctypes.c_void_p, _dp]
+    L.ibem3d_update_known_values.argtypes = [ctypes.c_void_p, _dp, _dp]
     _LIB = L
     return L
 
@@ -238,6 +240,15 @@ class Engine:
             raise ValueError("set_known_values: expected 3 NN and 12 NE values")
         self._chk(self.L.ibem3d_set_known_values(self.ctx, _d(bv), _d(tv)), "set_known_values")
 
+    def update_known_values(self, bc_values, tbc_values):
+        """New prescribed values with b assembled again and the matrix (or its in-place factor)
+        kept: solve next with SOLVE_REUSE."""
+        bv = np.ascontiguousarray(bc_values, dtype=np.float64).reshape(-1)
+        tv = np.ascontiguousarray(tbc_values, dtype=np.float64).reshape(-1)
+        if bv.size != 3 * self.prob.NN or tv.size != 12 * self.prob.NE:
+            raise ValueError("update_known_values: expected 3 NN and 12 NE values")
+        self._chk(self.L.ibem3d_update_known_values(self.ctx, _d(bv), _d(tv)), "update_known_values")
+
     def average_eigenstrain(self):
         """Volume average of the equivalent eigenstrain (6 values, 00 01 02 11 12 22)."""
         e = np.zeros(6)
@@ -325,8 +336,9 @@ def effective_stiffness(prob: Problem, mode: int = SOLVE_MIXED_FP16, tol: float 
         for t in range(6):
             if t > 0:
                 bc, tbc = macro_strain_bc(nodes, conn, unit_strain(t))
-                eng.set_known_values(bc[:, 1], tbc[:, 12:])
-            eng.assemble()
+                eng.update_known_values(bc[:, 1], tbc[:, 12:])  # b only; A and its factors kept
+            else:
+                eng.assemble()
             rel, it = eng.solve(mode | (SOLVE_REUSE if t > 0 else 0), tol)
             info.append((rel, it))
             sig_b, eps_b, _ = eng.average_fields()

@@ -131,6 +131,13 @@ int ibem3d_stress(ibem3d_ctx* ctx, double* s);
  * phases again, then solve. Entries at positions that are not known are ignored. */
 int ibem3d_set_known_values(ibem3d_ctx* ctx, const double* bc_values, const double* tbc_values);
 
+/* The same new known values without a new matrix (no counterpart in the reference): b is assembled
+ * again (the known-value folds of ApplyBC, BEMBuilderElastic3D.cpp:1973-1990, from the boundary and
+ * boundary->inclusion phases, in the full assembly's order, so b equals the full assembly's bit for
+ * bit) and A -- or the factor that occupies its memory after a compact FP64 solve -- is left as it
+ * is. Then solve with IBEM3D_SOLVE_REUSE. Requires an assembled system. */
+int ibem3d_update_known_values(ibem3d_ctx* ctx, const double* bc_values, const double* tbc_values);
+
 /* Volume average of the equivalent eigenstrain (no counterpart in the reference):
  * <e*_ij> = (1/V) sum_spheres V_s (E_ij + (a_s^2 / 5) sum_k E_ij,kk), the integral of the quadratic
  * eigenstrain polynomial over each sphere; V as in ibem3d_average_fields. With it the average

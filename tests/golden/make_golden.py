@@ -9,6 +9,10 @@ Runs in the build container only (needs /root/reference and oracle/_ref/libibemr
   rve5/              5 spheres, 3x3 quads per face: inputs + X, U, T, stress
   example/           the reference's examples/elastic3D_spherical inputs and shipped ground truth,
                      plus the reference's X, U, T, stress on them
+  rve64_m8/          the 64-sphere, 8x8-per-face workload (seed 1, 1000 stress points): inputs + X,
+                     U, T, stress (the reference's end-to-end solution at that size)
+  contrast2/,        8 spheres, 3x3 quads per face, particle/matrix contrast E1/E0 = 2 (nu1 0.25)
+  contrast100/       and 100 (nu1 0.35): inputs + X, U, T, stress
 """
 import ctypes
 import os
@@ -24,7 +28,7 @@ sys.path.insert(0, ROOT)
 REF_EXAMPLE = "/root/reference/examples/elastic3D_spherical"
 
 from ibemproject_amd.formats import read_problem, write_problem  # noqa: E402
-from ibemproject_amd.generate import make_rve  # noqa: E402
+from ibemproject_amd.generate import make_rve, make_workload  # noqa: E402
 
 L = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libibemref.so"))
 dp = ctypes.POINTER(ctypes.c_double)
@@ -188,7 +192,7 @@ def example():
 
 
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["closed", "quad", "tiny", "rve5", "example"]
+    what = sys.argv[1:] or ["closed", "quad", "tiny", "rve5", "example", "contrast", "rve64"]
     if "closed" in what:
         closed_forms(os.path.join(HERE, "closed_forms.npz"))
     if "quad" in what:
@@ -204,4 +208,9 @@ if __name__ == "__main__":
         rve_case("rve5", p, False)
     if "example" in what:
         example()
+    if "contrast" in what:
+        rve_case("contrast2", make_rve(23, 8, 0.07, 0.11, 3, nump=40, E1=2.0, nu1=0.25), False)
+        rve_case("contrast100", make_rve(23, 8, 0.07, 0.11, 3, nump=40, E1=100.0, nu1=0.35), False)
+    if "rve64" in what:
+        rve_case("rve64_m8", make_workload("rve64_m8", seed=1), False)
     print("golden vectors written under", HERE)

@@ -1,10 +1,12 @@
 """Effective stiffness on the GPU (pytest -m gpu): six affine displacement loads on one
-factorisation (ibem3d_set_known_values + IBEM3D_SOLVE_REUSE), against the oracle's six solves."""
+factorisation (ibem3d_update_known_values, which assembles b only, + IBEM3D_SOLVE_REUSE), against
+the oracle's six solves; the right-hand-side-only assembly against a full one."""
 import numpy as np
 import pytest
 
-from ibemproject_amd.engine import (SOLVE_FP64_LU, SOLVE_MIXED_FP16, SOLVE_REUSE, Engine,
-                                    effective_stiffness, isotropic_stress, unit_strain)
+from ibemproject_amd.engine import (SOLVE_COMPACT, SOLVE_FP64_LU, SOLVE_MIXED_FP16, SOLVE_MIXED_FP16_1,
+                                    SOLVE_REUSE, Engine, effective_stiffness, isotropic_stress,
+                                    unit_strain)
 from ibemproject_amd.formats import Problem
 from ibemproject_amd.generate import macro_strain_bc, make_rve, make_workload
 from test_effective import oracle_stiffness
@@ -47,6 +49,55 @@ def test_reused_factors_match_a_fresh_solve(mode):
     assert np.linalg.norm(x_reuse - x_fresh) / np.linalg.norm(x_fresh) < 1e-9
 
 
+def test_rhs_only_assembly_matches_full():
+    """ibem3d_update_known_values: b of a new load equals a full assembly's b bit for bit, and A is
+    left as it was."""
+    base = make_rve(5, 4, 0.12, 0.16, 3, nump=0)
+    case(f"rhs-only assembly: num={base.num}")
+    p0, _, _ = with_load(base, 0)
+    p3, bc3, tbc3 = with_load(base, 3)
+    eng = Engine(p0)
+    eng.assemble()
+    A0, _ = eng.system()
+    eng.update_known_values(bc3[:, 1], tbc3[:, 12:])
+    A1, b1 = eng.system()
+    eng.close()
+    eng = Engine(p3)
+    eng.assemble()
+    A3, b3 = eng.system()
+    eng.close()
+    assert np.array_equal(A1, A0) and np.array_equal(A3, A0)
+    assert np.array_equal(b1, b3)
+
+
+@pytest.mark.parametrize("mode", [SOLVE_FP64_LU | SOLVE_COMPACT, SOLVE_MIXED_FP16_1 | SOLVE_COMPACT])
+def test_compact_reuse_with_rhs_update(mode):
+    """A new load on the factor kept in place of A (compact FP64) or in its FP16 copy: the reused
+    solve matches a fresh FP64 solve of the new load."""
+    base = make_rve(5, 4, 0.12, 0.16, 3, nump=0)
+    case(f"compact reuse + rhs update, mode {mode}")
+    p0, _, _ = with_load(base, 0)
+    p3, bc3, tbc3 = with_load(base, 3)
+    eng = Engine(p0)
+    eng.assemble()
+    eng.solve(mode, 1e-10)
+    eng.update_known_values(bc3[:, 1], tbc3[:, 12:])
+    rel, _ = eng.solve((mode & ~SOLVE_COMPACT) | SOLVE_REUSE, 1e-10)
+    _, _, x_reuse = eng.getUTVector()
+    if rel < 0:
+        rel = eng.residual()
+    eng.close()
+    assert rel <= 1e-10
+    eng = Engine(p3)
+    eng.assemble()
+    eng.solve(SOLVE_FP64_LU)
+    _, _, x_fresh = eng.getUTVector()
+    eng.close()
+    e = np.linalg.norm(x_reuse - x_fresh) / np.linalg.norm(x_fresh)
+    print("residual", rel, "vs fresh", e)
+    assert e < (1e-12 if mode & 0xFF == SOLVE_FP64_LU else 1e-8)
+
+
 def test_reuse_without_factors_is_rejected():
     base = make_rve(5, 2, 0.12, 0.16, 2, nump=0)
     case("reuse before any factorisation")
@@ -82,7 +133,18 @@ def test_stiffness_full_size():
     g = np.mean([K[t, t] for t in (1, 2, 4)]) / 2.0
     k0, g0 = prob.E0 / (3 * (1 - 2 * prob.nu0)), prob.E0 / (2 * (1 + prob.nu0))
     k1, g1 = prob.E1 / (3 * (1 - 2 * prob.nu1)), prob.E1 / (2 * (1 + prob.nu1))
-    print(f"phi {phi:.4f} k {k:.5f} g {g:.5f}")
+    # Hashin-Shtrikman bounds (stiffer spheres: the lower bound is the matrix-based one) and the
+    # dilute Eshelby estimate of spheres without interaction
+    k_lo = k0 + phi / (1 / (k1 - k0) + 3 * (1 - phi) / (3 * k0 + 4 * g0))
+    k_hi = k1 + (1 - phi) / (1 / (k0 - k1) + 3 * phi / (3 * k1 + 4 * g1))
+    g_lo = g0 + phi / (1 / (g1 - g0) + 6 * (1 - phi) * (k0 + 2 * g0) / (5 * g0 * (3 * k0 + 4 * g0)))
+    g_hi = g1 + (1 - phi) / (1 / (g0 - g1) + 6 * phi * (k1 + 2 * g1) / (5 * g1 * (3 * k1 + 4 * g1)))
+    beta = 6 * (k0 + 2 * g0) / (5 * (3 * k0 + 4 * g0))
+    k_dil = k0 + phi * (k1 - k0) * (3 * k0 + 4 * g0) / (3 * k1 + 4 * g0)
+    g_dil = g0 + phi * (g1 - g0) * g0 / (g0 + beta * (g1 - g0))
+    print(f"phi {phi:.4f} k {k:.5f} (HS {k_lo:.5f}..{k_hi:.5f}, dilute {k_dil:.5f}) "
+          f"g {g:.5f} (HS {g_lo:.5f}..{g_hi:.5f}, dilute {g_dil:.5f})")
     assert 1 / (phi / k1 + (1 - phi) / k0) <= k <= phi * k1 + (1 - phi) * k0
     assert 1 / (phi / g1 + (1 - phi) / g0) <= g <= phi * g1 + (1 - phi) * g0
+    assert k_lo <= k <= k_hi and g_lo <= g <= g_hi
     assert np.abs(K - K.T).max() < 0.02 * np.abs(K).max()

@@ -146,3 +146,21 @@ def test_mixed_refined(solved, mode):
     s = eng.getFlux()
     assert np.abs(s - ref["s"]).max() / np.abs(ref["s"]).max() < 1e-8
     eng.solve(SOLVE_FP64_LU)
+
+
+def test_repeated_phase_is_refused():
+    """The boundary phases accumulate into A and b: running one twice without a reset would count
+    its entries twice, so the second call is refused (and a reset makes it valid again)."""
+    from ibemproject_amd.engine import EngineError
+    prob = small_problem("two")
+    case("repeated assembly phase")
+    eng = Engine(prob)
+    eng.addBEM()
+    with pytest.raises(EngineError):
+        eng.addBEM()
+    eng.addFluxEquivalentInclusion(0)
+    with pytest.raises(EngineError):
+        eng.addFluxEquivalentInclusion(0)
+    eng.reset()
+    eng.assemble()
+    eng.close()

@@ -155,7 +155,7 @@ def test_tiny_rve_phases(oracle_lib):
     assert np.abs(b - ref["b"]).max() <= 1e-12 * np.abs(ref["b"]).max()
 
 
-@pytest.mark.parametrize("name", ["tiny_rve", "rve5"])
+@pytest.mark.parametrize("name", ["tiny_rve", "rve5", "contrast2", "contrast100"])
 def test_rve_solution_and_stress(oracle_lib, name):
     d = os.path.join(GOLD, name)
     prob = read_problem(d)
