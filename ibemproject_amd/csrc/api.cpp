@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/ibem3d.h"
+#include "batch_internal.h"
 #include "ibem3d_kernels.h"
 
 using namespace ibem;
@@ -73,6 +74,9 @@ struct ibem3d_ctx {
   long timing_errors = 0;
   // stress inputs and output (kept between calls)
   double *d_stU = nullptr, *d_stT = nullptr, *d_sts = nullptr;
+  // host copies for the volume averages
+  std::vector<double> h_qp, h_rad;
+  std::vector<int> h_conn;
 };
 
 namespace {
@@ -290,9 +294,8 @@ ibem3d_ctx* ibem3d_create(const ibem3d_problem* p, int device) {
       dalloc(ctx, &ctx->d_x, (size_t)n))
     return bail("");
   // the boundary-column block and b are accumulated by the assembly kernels
-  if (hipMemsetAsync(ctx->d_A, 0, sizeof(double) * (size_t)n * 3 * p->NN, ctx->st) != hipSuccess ||
-      hipMemsetAsync(ctx->d_b, 0, sizeof(double) * n, ctx->st) != hipSuccess)
-    return bail("ibem3d_create: memset failed");
+  fill_zero(ctx->d_A, sizeof(double) * (size_t)n * 3 * p->NN, ctx->st);
+  fill_zero(ctx->d_b, sizeof(double) * n, ctx->st);
   launch_qpoints(ctx->d_nodes, ctx->d_conn, p->NE, ctx->d_qp, ctx->st);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->st) != hipSuccess)
     return bail("ibem3d_create: quadrature setup failed");
@@ -431,8 +434,9 @@ int ibem3d_assemble(ibem3d_ctx* ctx) {
 int ibem3d_reset(ibem3d_ctx* ctx) {
   CTX_CHECK();
   const long n = ctx->n;
-  HIPC(hipMemsetAsync(ctx->d_A, 0, sizeof(double) * (size_t)n * 3 * ctx->NN, ctx->st));
-  HIPC(hipMemsetAsync(ctx->d_b, 0, sizeof(double) * n, ctx->st));
+  fill_zero(ctx->d_A, sizeof(double) * (size_t)n * 3 * ctx->NN, ctx->st);
+  fill_zero(ctx->d_b, sizeof(double) * n, ctx->st);
+  if (int rc = after_launch(ctx, "reset")) return rc;
   ctx->phases = 0;
   ctx->solved = false;
   if (ctx->a_factored) {
@@ -472,7 +476,7 @@ static int ensure_solver_ws(ibem3d_ctx* ctx) {
   if (!ctx->W.d_ipiv) {
     if (int rc = dalloc(ctx, &ctx->W.d_ipiv, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->W.d_ws, lu_workspace_doubles(n))) return rc;
-    HIPC(hipMemsetAsync(ctx->W.d_ws, 0, sizeof(double) * lu_workspace_doubles(n), ctx->st));
+    fill_zero(ctx->W.d_ws, sizeof(double) * lu_workspace_doubles(n), ctx->st);
     if (int rc = dalloc(ctx, &ctx->W.d_info, 1)) return rc;
     if (int rc = dalloc(ctx, &ctx->W.d_perm, lu_moves_ints(n))) return rc;
     if (int rc = dalloc(ctx, &ctx->W.d_inv, lu_inverse_doubles(n))) return rc;
@@ -610,7 +614,7 @@ static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0) {
   if (warm) {
     rel = rel0;
   } else {
-    HIPC(hipMemsetAsync(ctx->d_x, 0, sizeof(double) * n, ctx->st));
+    fill_zero(ctx->d_x, sizeof(double) * n, ctx->st);
     HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
   }
   double beta = rel * bnorm;
@@ -714,7 +718,7 @@ static int check_info(ibem3d_ctx* ctx, const char* what) {
 // an iteration gains less than a factor of 10 (from zero when it diverges)
 static int classic_ir(ibem3d_ctx* ctx, double tol) {
   const long n = ctx->n;
-  HIPC(hipMemsetAsync(ctx->d_x, 0, sizeof(double) * n, ctx->st));
+  fill_zero(ctx->d_x, sizeof(double) * n, ctx->st);
   HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
   double rel = 1.0;
   const int max_it = 30;
@@ -962,7 +966,7 @@ int ibem3d_update_known_values(ibem3d_ctx* ctx, const double* bc_values, const d
   if (ctx->phases != 0xFFu)
     return fail(ctx, IBEM3D_ESTATE, "update_known_values: the system must have been assembled");
   if (int rc = upload_known_values(ctx, bc_values, tbc_values)) return rc;
-  HIPC(hipMemsetAsync(ctx->d_b, 0, sizeof(double) * ctx->n, ctx->st));
+  fill_zero(ctx->d_b, sizeof(double) * ctx->n, ctx->st);
   {
     Scope s(ctx, "assemble.rhs");
     if (!ctx->d_bemws)
@@ -979,107 +983,31 @@ int ibem3d_update_known_values(ibem3d_ctx* ctx, const double* bc_values, const d
   return after_launch(ctx, "update_known_values");
 }
 
-// enclosed volume (1/3) int x . n dS on the quadrature table (the same sum as ibem3d_average_fields)
-static int enclosed_volume(ibem3d_ctx* ctx, double* V) {
-  std::vector<double> qp((size_t)ctx->NE * 64 * kQpStride);
-  HIPC(hipMemcpyAsync(qp.data(), ctx->d_qp, sizeof(double) * qp.size(), hipMemcpyDeviceToHost, ctx->st));
-  HIPC(hipStreamSynchronize(ctx->st));
-  double v = 0.0;
-  for (size_t g = 0; g < (size_t)ctx->NE * 64; ++g) {
-    const double* Q = qp.data() + g * kQpStride;
-    v += Q[10] * (Q[0] * Q[3] + Q[1] * Q[4] + Q[2] * Q[5]) / 3.0;
-  }
-  *V = v;
-  return 0;
-}
-
-int ibem3d_average_eigenstrain(ibem3d_ctx* ctx, double* eps_star) {
-  CTX_CHECK();
-  if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "average_eigenstrain: no solution");
-  if (!eps_star) return fail(ctx, IBEM3D_EINVAL, "average_eigenstrain: null output");
-  const long n78 = 3L * ctx->NN + 78L * ctx->num;
-  std::vector<double> U(n78);
-  if (int rc = ibem3d_get_ut(ctx, U.data(), nullptr, nullptr)) return rc;
-  std::vector<double> rad(ctx->num);
-  if (ctx->num > 0) {
-    HIPC(hipMemcpyAsync(rad.data(), ctx->d_rad, sizeof(double) * rad.size(), hipMemcpyDeviceToHost,
+// Host copies of the quadrature table, connectivity and radii (downloaded once) for the averages
+static int host_geometry(ibem3d_ctx* ctx) {
+  if (!ctx->h_qp.empty()) return 0;
+  ctx->h_qp.resize((size_t)ctx->NE * 64 * kQpStride);
+  ctx->h_conn.resize(4L * ctx->NE);
+  ctx->h_rad.resize(ctx->num);
+  HIPC(hipMemcpyAsync(ctx->h_qp.data(), ctx->d_qp, sizeof(double) * ctx->h_qp.size(), hipMemcpyDeviceToHost,
+                      ctx->st));
+  HIPC(hipMemcpyAsync(ctx->h_conn.data(), ctx->d_conn, sizeof(int) * ctx->h_conn.size(),
+                      hipMemcpyDeviceToHost, ctx->st));
+  if (ctx->num > 0)
+    HIPC(hipMemcpyAsync(ctx->h_rad.data(), ctx->d_rad, sizeof(double) * ctx->num, hipMemcpyDeviceToHost,
                         ctx->st));
-    HIPC(hipStreamSynchronize(ctx->st));
-  }
-  double V = 0.0;
-  if (int rc = enclosed_volume(ctx, &V)) return rc;
-  if (!(V > 0)) return fail(ctx, IBEM3D_EINVAL, "average_eigenstrain: non-positive enclosed volume");
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int h = 0; h < ctx->num; ++h) {
-    const double a = rad[h], Vs = 4.0 / 3.0 * kPi * a * a * a;
-    const double* E = U.data() + 3L * ctx->NN + 78L * h;  // E_t, E_t,k, E_t,kl (78 layout)
-    for (int t = 0; t < 6; ++t) {
-      const double tr = E[24 + 9 * t + 0] + E[24 + 9 * t + 4] + E[24 + 9 * t + 8];
-      acc[t] += Vs * (E[t] + a * a / 5.0 * tr);
-    }
-  }
-  for (int t = 0; t < 6; ++t) eps_star[t] = acc[t] / V;
+  HIPC(hipStreamSynchronize(ctx->st));
   return 0;
 }
 
-// Effective-property post-step: volume averages of stress and strain from the boundary solution
-// (divergence theorem; perfect bonding, no body force), on the quadrature table of k_qpoints.
-int ibem3d_average_fields(ibem3d_ctx* ctx, double* sigma, double* eps, double* volume) {
-  CTX_CHECK();
-  if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "average_fields: no solution");
-  const long n78 = 3L * ctx->NN + 78L * ctx->num;
-  std::vector<double> U(n78), T(12L * ctx->NE);
-  if (int rc = ibem3d_get_ut(ctx, U.data(), T.data(), nullptr)) return rc;
-  std::vector<double> qp((size_t)ctx->NE * 64 * kQpStride);
-  std::vector<int> conn(4L * ctx->NE);
-  HIPC(hipMemcpyAsync(qp.data(), ctx->d_qp, sizeof(double) * qp.size(), hipMemcpyDeviceToHost, ctx->st));
-  HIPC(hipMemcpyAsync(conn.data(), ctx->d_conn, sizeof(int) * conn.size(), hipMemcpyDeviceToHost, ctx->st));
-  HIPC(hipStreamSynchronize(ctx->st));
-  double S[3][3] = {}, E[3][3] = {}, V = 0.0;
-  for (int e = 0; e < ctx->NE; ++e) {
-    for (int g = 0; g < 64; ++g) {
-      const double* Q = qp.data() + ((size_t)e * 64 + g) * kQpStride;  // g[3], nv[3], N[4], w*jac
-      double t[3] = {0, 0, 0}, u[3] = {0, 0, 0};
-      for (int q = 0; q < 4; ++q) {
-        const int nd = conn[4 * e + q] - 1;
-        for (int i = 0; i < 3; ++i) {
-          t[i] += Q[6 + q] * T[12L * e + 3 * q + i];
-          u[i] += Q[6 + q] * U[3L * nd + i];
-        }
-      }
-      const double w = Q[10];
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-          S[i][j] += w * t[i] * Q[j];
-          E[i][j] += w * 0.5 * (u[i] * Q[3 + j] + u[j] * Q[3 + i]);
-        }
-      V += w * (Q[0] * Q[3] + Q[1] * Q[4] + Q[2] * Q[5]) / 3.0;
-    }
-  }
-  if (!(V > 0)) return fail(ctx, IBEM3D_EINVAL, "average_fields: non-positive enclosed volume");
-  static const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
-  for (int c = 0; c < 6; ++c) {
-    if (sigma) sigma[c] = 0.5 * (S[pi[c]][pj[c]] + S[pj[c]][pi[c]]) / V;
-    if (eps) eps[c] = E[pi[c]][pj[c]] / V;
-  }
-  if (volume) *volume = V;
-  return 0;
-}
-
-// getUTVector (BEMBuilderElastic3D.cpp:2053-2117) in the 78 layout
-int ibem3d_get_ut(ibem3d_ctx* ctx, double* U, double* T, double* X60) {
-  CTX_CHECK();
-  if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "get_ut: no solution");
-  const long n = ctx->n;
-  std::vector<double> x(n);
-  HIPC(hipMemcpyAsync(x.data(), ctx->d_x, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
-  HIPC(hipStreamSynchronize(ctx->st));
-  if (X60) std::memcpy(X60, x.data(), sizeof(double) * n);
+// getUTVector (BEMBuilderElastic3D.cpp:2053-2117) of a host solution x in the 78 layout; U or T may
+// be null
+static void ut_from_x(const ibem3d_ctx* ctx, const double* x, double* U, double* T) {
   const int NN = ctx->NN, NE = ctx->NE;
   if (U) {
     for (int i = 0; i < 3 * NN; ++i) U[i] = ctx->node_col[i] >= 0 ? x[ctx->node_col[i]] : ctx->h_bc[2 * i + 1];
     for (int h = 0; h < ctx->num; ++h) {
-      const double* s = x.data() + 3L * NN + 60L * h;
+      const double* s = x + 3L * NN + 60L * h;
       double* d = U + 3L * NN + 78L * h;
       for (int u = 0; u < 24; ++u) d[u] = s[u];
       for (int t = 0; t < 6; ++t)
@@ -1098,6 +1026,101 @@ int ibem3d_get_ut(ibem3d_ctx* ctx, double* U, double* T, double* X60) {
           T[m++] = x[c];
       }
   }
+}
+
+// Volume averages of stress and strain from the boundary solution (divergence theorem; perfect
+// bonding, no body force) on the quadrature table, and the enclosed volume V = (1/3) int x . n dS.
+static void fields_from_ut(const ibem3d_ctx* ctx, const double* U, const double* T, double* sigma,
+                           double* eps, double* volume) {
+  const double* qp = ctx->h_qp.data();
+  const int* conn = ctx->h_conn.data();
+  double S[3][3] = {}, E[3][3] = {}, V = 0.0;
+  for (int e = 0; e < ctx->NE; ++e) {
+    for (int g = 0; g < 64; ++g) {
+      const double* Q = qp + ((size_t)e * 64 + g) * kQpStride;  // g[3], nv[3], N[4], w*jac
+      double t[3] = {0, 0, 0}, u[3] = {0, 0, 0};
+      for (int q = 0; q < 4; ++q) {
+        const int nd = conn[4 * e + q] - 1;
+        for (int i = 0; i < 3; ++i) {
+          if (T) t[i] += Q[6 + q] * T[12L * e + 3 * q + i];
+          if (U) u[i] += Q[6 + q] * U[3L * nd + i];
+        }
+      }
+      const double w = Q[10];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          S[i][j] += w * t[i] * Q[j];
+          E[i][j] += w * 0.5 * (u[i] * Q[3 + j] + u[j] * Q[3 + i]);
+        }
+      V += w * (Q[0] * Q[3] + Q[1] * Q[4] + Q[2] * Q[5]) / 3.0;
+    }
+  }
+  static const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
+  for (int c = 0; c < 6; ++c) {
+    if (sigma) sigma[c] = 0.5 * (S[pi[c]][pj[c]] + S[pj[c]][pi[c]]) / V;
+    if (eps) eps[c] = E[pi[c]][pj[c]] / V;
+  }
+  if (volume) *volume = V;
+}
+
+// <e*_ij> = (1/V) sum_spheres V_s (E_ij + (a_s^2 / 5) sum_k E_ij,kk) from U (78 layout)
+static void eps_star_from_u(const ibem3d_ctx* ctx, const double* U, double V, double* eps_star) {
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int h = 0; h < ctx->num; ++h) {
+    const double a = ctx->h_rad[h], Vs = 4.0 / 3.0 * kPi * a * a * a;
+    const double* E = U + 3L * ctx->NN + 78L * h;  // E_t, E_t,k, E_t,kl (78 layout)
+    for (int t = 0; t < 6; ++t) {
+      const double tr = E[24 + 9 * t + 0] + E[24 + 9 * t + 4] + E[24 + 9 * t + 8];
+      acc[t] += Vs * (E[t] + a * a / 5.0 * tr);
+    }
+  }
+  for (int t = 0; t < 6; ++t) eps_star[t] = acc[t] / V;
+}
+
+static int host_solution(ibem3d_ctx* ctx, std::vector<double>& x) {
+  x.resize(ctx->n);
+  HIPC(hipMemcpyAsync(x.data(), ctx->d_x, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->st));
+  HIPC(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int ibem3d_average_eigenstrain(ibem3d_ctx* ctx, double* eps_star) {
+  CTX_CHECK();
+  if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "average_eigenstrain: no solution");
+  if (!eps_star) return fail(ctx, IBEM3D_EINVAL, "average_eigenstrain: null output");
+  if (int rc = host_geometry(ctx)) return rc;
+  std::vector<double> x, U(3L * ctx->NN + 78L * ctx->num);
+  if (int rc = host_solution(ctx, x)) return rc;
+  ut_from_x(ctx, x.data(), U.data(), nullptr);
+  double V = 0.0;
+  fields_from_ut(ctx, nullptr, nullptr, nullptr, nullptr, &V);
+  if (!(V > 0)) return fail(ctx, IBEM3D_EINVAL, "average_eigenstrain: non-positive enclosed volume");
+  eps_star_from_u(ctx, U.data(), V, eps_star);
+  return 0;
+}
+
+int ibem3d_average_fields(ibem3d_ctx* ctx, double* sigma, double* eps, double* volume) {
+  CTX_CHECK();
+  if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "average_fields: no solution");
+  if (int rc = host_geometry(ctx)) return rc;
+  std::vector<double> x, U(3L * ctx->NN + 78L * ctx->num), T(12L * ctx->NE);
+  if (int rc = host_solution(ctx, x)) return rc;
+  ut_from_x(ctx, x.data(), U.data(), T.data());
+  double V = 0.0;
+  fields_from_ut(ctx, U.data(), T.data(), sigma, eps, &V);
+  if (!(V > 0)) return fail(ctx, IBEM3D_EINVAL, "average_fields: non-positive enclosed volume");
+  if (volume) *volume = V;
+  return 0;
+}
+
+// getUTVector (BEMBuilderElastic3D.cpp:2053-2117) in the 78 layout
+int ibem3d_get_ut(ibem3d_ctx* ctx, double* U, double* T, double* X60) {
+  CTX_CHECK();
+  if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "get_ut: no solution");
+  std::vector<double> x;
+  if (int rc = host_solution(ctx, x)) return rc;
+  if (X60) std::memcpy(X60, x.data(), sizeof(double) * ctx->n);
+  ut_from_x(ctx, x.data(), U, T);
   return 0;
 }
 
@@ -1164,3 +1187,135 @@ void ibem3d_reset_timings(ibem3d_ctx* ctx) {
 }
 
 }  // extern "C"
+
+// ================================================================ batched solves (batch.cpp) ====
+// relative residual on the device: part[0..B) = block sums of r^2, part[B..2B) = of b^2 (fixed
+// order), then out = sqrt(sum r^2 / sum b^2)
+__global__ __launch_bounds__(256) void k_sumsq_part(const double* __restrict__ v, long n,
+                                                    double* __restrict__ part) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * 256 + threadIdx.x; t < n; t += (long)kDotBlocks * 256) acc += v[t] * v[t];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ void k_rel_finish(const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double r = 0.0, b = 0.0;
+  for (int i = 0; i < kDotBlocks; ++i) {
+    r += part[i];
+    b += part[kDotBlocks + i];
+  }
+  *out = b > 0.0 ? sqrt(r / b) : sqrt(r);
+}
+
+namespace ibem_batch {
+
+hipStream_t stream(ibem3d_ctx* ctx) { return ctx->st; }
+long size(ibem3d_ctx* ctx) { return ctx->n; }
+size_t dot_blocks() { return kDotBlocks; }
+
+int prepare(ibem3d_ctx* ctx, int mode) {
+  CTX_CHECK();
+  if (mode < IBEM3D_SOLVE_MIXED_FP32 || mode > IBEM3D_SOLVE_MIXED_BF16)
+    return fail(ctx, IBEM3D_EINVAL, "batch: mixed modes only");
+  const long n = ctx->n;
+  if (int rc = ensure_solver_ws(ctx)) return rc;
+  ctx->lda32 = (n + 63) / 64 * 64;
+  if (!ctx->d_A32)
+    if (int rc = dalloc(ctx, &ctx->d_A32, (size_t)ctx->lda32 * n)) return rc;
+  if (!ctx->d_v32)
+    if (int rc = dalloc(ctx, &ctx->d_v32, (size_t)n)) return rc;
+  if (mode != IBEM3D_SOLVE_MIXED_FP32 && !ctx->d_f16ws)
+    if (int rc = dalloc(ctx, &ctx->d_f16ws, lu_f16_workspace_doubles(n))) return rc;
+  if (!ctx->d_bemws)
+    if (int rc = dalloc(ctx, &ctx->d_bemws, bem_workspace_doubles(ctx->NN, ctx->NE))) return rc;
+  if (lu_streams(ctx->W)) return fail(ctx, IBEM3D_EHIP, "batch: LU streams");
+  if (int rc = host_geometry(ctx)) return rc;
+  HIPC(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+int enqueue(ibem3d_ctx* ctx, int mode, int K, int nloads, const double* d_uvals, double* d_xs,
+            double* d_rel, double* d_part) {
+  const long n = ctx->n, NN = ctx->NN;
+  hipStream_t st = ctx->st;
+  const int prod = mode == IBEM3D_SOLVE_MIXED_FP16_1 ? kProdF16
+                   : mode == IBEM3D_SOLVE_MIXED_BF16 ? kProdBF16 : kProdF16x3;
+  const bool split = mode != IBEM3D_SOLVE_MIXED_FP32;
+  HIPC(hipMemcpyAsync(ctx->d_uval, d_uvals, sizeof(double) * 3 * NN, hipMemcpyDeviceToDevice, st));
+  // assembly (every phase, all orders at once)
+  fill_zero(ctx->d_A, sizeof(double) * (size_t)n * 3 * NN, st);
+  fill_zero(ctx->d_b, sizeof(double) * n, st);
+  launch_bem(ctx->d_qp, ctx->d_conn, ctx->d_nodes, ctx->NN, ctx->NE, n, ctx->E0, ctx->nu0, ctx->d_node_col,
+             ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, ctx->d_bemws, st);
+  if (ctx->num > 0) {
+    launch_field(ctx->d_nodes, ctx->NN, ctx->d_cen, ctx->d_rad, ctx->num, n, 3 * NN, ctx->M, ctx->d_A, st);
+    launch_incl(ctx->d_cen, ctx->d_rad, ctx->num, n, 3 * NN, ctx->M, 7, ctx->d_A, st);
+    launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, n, 3 * NN, ctx->M, 7, ctx->d_node_col,
+                ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, st);
+  }
+  // mixed LU of R A C
+  if (equilibrate_copy_f32(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_A32, ctx->lda32, ctx->d_eqws, st, &ctx->W))
+    return fail(ctx, IBEM3D_EHIP, "batch: equilibration");
+  if (lu_factor_f32(ctx->d_A32, n, ctx->lda32, ctx->W, st, nullptr, nullptr, split ? ctx->d_f16ws : nullptr, true,
+                    prod))
+    return fail(ctx, IBEM3D_EHIP, "batch: LU");
+  if (lu_prepare_solve_f32(ctx->d_A32, n, ctx->lda32, ctx->W, st)) return fail(ctx, IBEM3D_EHIP, "batch: prepare");
+  for (int t = 0; t < nloads; ++t) {
+    if (t > 0) {
+      HIPC(hipMemcpyAsync(ctx->d_uval, d_uvals + (size_t)t * 3 * NN, sizeof(double) * 3 * NN,
+                          hipMemcpyDeviceToDevice, st));
+      fill_zero(ctx->d_b, sizeof(double) * n, st);
+      launch_bem(ctx->d_qp, ctx->d_conn, ctx->d_nodes, ctx->NN, ctx->NE, n, ctx->E0, ctx->nu0, ctx->d_node_col,
+                 ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, ctx->d_bemws, st, false);
+      if (ctx->num > 0)
+        launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, n, 3 * NN, ctx->M, 7,
+                    ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, st, false);
+    }
+    // K steps of classic refinement from x = 0: x <- x + C LU^-1 R (b - A x)
+    fill_zero(ctx->d_x, sizeof(double) * n, st);
+    HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    for (int it = 0; it < K; ++it) {
+      hipLaunchKernelGGL(k_scale_to_f32, vgrid(n), dim3(256), 0, st, ctx->d_v32, ctx->d_res, ctx->d_r, n);
+      lu_solve_f32(ctx->d_A32, n, ctx->lda32, ctx->W, ctx->d_v32, st);
+      hipLaunchKernelGGL(k_axpy_scaled_f32, vgrid(n), dim3(256), 0, st, ctx->d_x, ctx->d_v32, ctx->d_c, n);
+      HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+      gemv_f64(n, n, -1.0, ctx->d_A, n, ctx->d_x, 1.0, ctx->d_res, ctx->d_gemvws, st);
+    }
+    HIPC(hipMemcpyAsync(d_xs + (size_t)t * n, ctx->d_x, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_sumsq_part, dim3(kDotBlocks), dim3(256), 0, st, ctx->d_res, n, d_part);
+    hipLaunchKernelGGL(k_sumsq_part, dim3(kDotBlocks), dim3(256), 0, st, ctx->d_b, n, d_part + kDotBlocks);
+    hipLaunchKernelGGL(k_rel_finish, dim3(1), dim3(64), 0, st, d_part, d_rel + t);
+  }
+  return after_launch(ctx, "batch enqueue");
+}
+
+void mark_solved(ibem3d_ctx* ctx, int mode) {
+  ctx->phases = 0xFFu;
+  ctx->a_factored = false;
+  ctx->prepared_mode = mode;
+  ctx->solved = true;
+}
+
+int averages(ibem3d_ctx* ctx, const double* x, const double* uvals, double* sigma, double* eps,
+             double* eps_star, double* V) {
+  CTX_CHECK();
+  if (int rc = host_geometry(ctx)) return rc;
+  for (long j = 0; j < 3L * ctx->NN; ++j) ctx->h_bc[2 * j + 1] = uvals[j];
+  std::vector<double> U(3L * ctx->NN + 78L * ctx->num), T(12L * ctx->NE);
+  ut_from_x(ctx, x, U.data(), T.data());
+  double vol = 0.0;
+  fields_from_ut(ctx, U.data(), T.data(), sigma, eps, &vol);
+  if (!(vol > 0)) return fail(ctx, IBEM3D_EINVAL, "averages: non-positive enclosed volume");
+  eps_star_from_u(ctx, U.data(), vol, eps_star);
+  if (V) *V = vol;
+  return 0;
+}
+
+}  // namespace ibem_batch

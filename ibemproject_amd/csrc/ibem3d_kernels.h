@@ -71,8 +71,7 @@ struct LuWork {
   int* d_ipiv = nullptr;   // n
   double* d_ws = nullptr;  // lu_workspace_doubles(n), zero-filled once at allocation
   int* d_info = nullptr;
-  int* d_perm = nullptr;   // lu_moves_ints(n): each step's interchanges as (dst, src) moves
-  std::vector<int> mv_off; // per step: first move (host; lu_prepare_solve_*), plus the end
+  int* d_perm = nullptr;   // lu_moves_ints(n): each step's interchanges as (dst, src) moves + counts
   double* d_inv = nullptr; // lu_inverse_doubles(n): inverses of the diagonal blocks of L and U
   double* d_tmp = nullptr; // n: solve scratch
   double* d_trinv = nullptr; // lu_trsm_workspace: inverses of one block row's diagonal blocks
@@ -95,8 +94,8 @@ int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick 
                   bool cols_pending = false, int prod = kProdF16x3);
 // the LU's second stream and events (created once)
 int lu_streams(LuWork& W);
-// After a factorisation: each step's interchanges as a move list and the diagonal-block inverses
-// (synchronises the stream once). Returns nonzero on a HIP error.
+// After a factorisation: each step's interchanges as a move list and the diagonal-block inverses,
+// all on the device (no synchronisation). Returns nonzero on a launch error.
 int lu_prepare_solve_f64(const double* LU, long n, long lda, LuWork& W, hipStream_t st);
 int lu_prepare_solve_f32(const float* LU, long n, long lda, LuWork& W, hipStream_t st);
 // Solve with the factors: x (in/out, n) <- A^{-1} x
@@ -133,6 +132,8 @@ size_t factor16_scales(long n);
 void factor_to_16(const float* LU, long n, long lda, unsigned short* F, long ldf, float* isc, hipStream_t st);
 void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, const LuWork& W, float* x,
                  hipStream_t st);
+// Zero bytes (a multiple of 4; p 16-byte aligned) with a kernel instead of a memset (capturable)
+void fill_zero(void* p, size_t bytes, hipStream_t st);
 // FP64 -> FP32 copy of a column-major matrix
 void convert_f64_to_f32(const double* A, long n, long lda, float* B, long ldb, hipStream_t st);
 

@@ -1832,7 +1832,7 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
   const F16x3Ws* pw = f16ws ? &wspanel : nullptr;
   LuCtx<T> c{A, n, lda, &W, st, tick, tctx, 0, pw};
   LuCtx<T> c2{A, n, lda, &W, W.st2, tick, tctx, 0, pw};
-  hipMemsetAsync(W.d_info, 0, sizeof(int), st);
+  fill_zero(W.d_info, sizeof(int), st);
   const F16x3Ws wsp = f16ws ? f16x3_ws(f16ws, n) : F16x3Ws{};
   const F16x3Ws* w = f16ws ? &wsp : nullptr;
   if (tick) tick(tctx, "lu.panel", 0, st);
@@ -2065,7 +2065,8 @@ __global__ __launch_bounds__(1024) void k_trsv_fwd(FA LU, long n, long k0,
 // y[dst] <- y[src] (all reads, then all writes), then the solve step at r0 finished. One workgroup;
 // an LU step has at most 2 IBEM_LU_NB moves.
 template <typename T, typename FA>
-__global__ __launch_bounds__(1024) void k_trsv_fwd_step(const int* __restrict__ mv, int nmv, long r0,
+__global__ __launch_bounds__(1024) void k_trsv_fwd_step(const int* __restrict__ mv,
+                                                        const int* __restrict__ nmvp, long r0,
                                                         long n, FA LU,
                                                         const T* __restrict__ inv, T* __restrict__ y) {
   __shared__ T sv[2 * IBEM_LU_NB];
@@ -2073,6 +2074,7 @@ __global__ __launch_bounds__(1024) void k_trsv_fwd_step(const int* __restrict__ 
   __shared__ T sz[kTS + kTB];
   __shared__ T pp[kTT];
   const int t = threadIdx.x;
+  const int nmv = *nmvp;
   for (int q = t; q < nmv; q += kTT) sv[q] = y[mv[2 * q + 1]];
   __syncthreads();
   for (int q = t; q < nmv; q += kTT) y[mv[2 * q]] = sv[q];
@@ -2137,7 +2139,28 @@ __global__ __launch_bounds__(1024) void k_trsv_bwd(FA LU, long n, long k0,
   }
 }
 
-size_t lu_moves_ints(long n) { return (size_t)4 * n + 8; }
+// Each LU step's interchanges composed into a move list on the device (laswp_moves, the composition
+// the interchange kernels use): step j's (dst, src) pairs at perm + j * kMvCap ints, its count at
+// perm[steps * kMvCap + j]. The solve applies them to the vector as all reads, then all writes, so
+// their order (atomic slot claims) does not matter. No host round trip: the preparation can be
+// captured in a graph.
+constexpr long kMvCap = 4 * IBEM_LU_NB;  // ints per step: at most 2 IBEM_LU_NB moves of two ints
+static long lu_steps(long n) { return (n + IBEM_LU_NB - 1) / IBEM_LU_NB; }
+size_t lu_moves_ints(long n) { return (size_t)lu_steps(n) * (kMvCap + 1); }
+
+__global__ __launch_bounds__(256) void k_step_moves(const int* __restrict__ ipiv, long n, long steps,
+                                                    int* __restrict__ perm) {
+  __shared__ LaswpLds S;
+  const long j = blockIdx.x, p0 = j * IBEM_LU_NB;
+  const int m = (int)min((long)IBEM_LU_NB, n - p0);
+  const int nm = laswp_moves(ipiv, p0, m, S);
+  int* out = perm + j * kMvCap;
+  for (int q = threadIdx.x; q < nm; q += 256) {
+    out[2 * q] = S.mdst[q];
+    out[2 * q + 1] = S.msrc[q];
+  }
+  if (threadIdx.x == 0) perm[steps * kMvCap + j] = nm;
+}
 
 template <typename T>
 static int lu_prepare(const T* LU, long n, long lda, LuWork& W, hipStream_t st) {
@@ -2146,47 +2169,9 @@ static int lu_prepare(const T* LU, long n, long lda, LuWork& W, hipStream_t st) 
   hipLaunchKernelGGL((k_trinv<T, false>), dim3((unsigned)nb), dim3(kTB), 0, st, LU, lda, n, inv);
   hipLaunchKernelGGL((k_trinv<T, true>), dim3((unsigned)nb), dim3(kTB), 0, st, LU, lda, n,
                      inv + (size_t)nb * kTB * kTB);
-  std::vector<int> ip(n);
-  if (hipMemcpyAsync(ip.data(), W.d_ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess)
-    return -1;
-  // each step's interchanges composed into moves y[dst] <- y[src] over the positions they touch
-  // (loc[x]: the position, at the start of the step, of the entry now at x; identity elsewhere)
-  std::vector<int> loc(n), mv, touched;
-  for (long i = 0; i < n; ++i) loc[i] = (int)i;
-  std::vector<char> mark(n, 0);
-  W.mv_off.clear();
-  for (long k = 0; k < n; k += IBEM_LU_NB) {
-    const long kb = std::min<long>(IBEM_LU_NB, n - k);
-    W.mv_off.push_back((int)(mv.size() / 2));
-    touched.clear();
-    for (long i = k; i < k + kb; ++i) {
-      const long p = ip[i];
-      if (p < i || p >= n) return -1;
-      for (long x : {i, p})
-        if (!mark[x]) {
-          mark[x] = 1;
-          touched.push_back((int)x);
-        }
-      std::swap(loc[i], loc[p]);
-    }
-    for (int x : touched) {
-      if (loc[x] != x) {
-        mv.push_back(x);
-        mv.push_back(loc[x]);
-      }
-    }
-    for (int x : touched) {
-      loc[x] = x;
-      mark[x] = 0;
-    }
-  }
-  W.mv_off.push_back((int)(mv.size() / 2));
-  if (mv.size() > lu_moves_ints(n)) return -1;
-  if (!mv.empty() &&
-      hipMemcpy(W.d_perm, mv.data(), sizeof(int) * mv.size(), hipMemcpyHostToDevice) != hipSuccess)
-    return -1;
-  return 0;
+  hipLaunchKernelGGL(k_step_moves, dim3((unsigned)lu_steps(n)), dim3(256), 0, st, W.d_ipiv, n, lu_steps(n),
+                     W.d_perm);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <typename T, typename FA>
@@ -2201,9 +2186,9 @@ static void lu_solve(const FA& LU, long n, const LuWork& W, T* x, hipStream_t st
     const long k0 = s * kTS;
     if (k0 % IBEM_LU_NB == 0) {  // first solve step of an LU step: its interchanges, then the step
       const long j = k0 / IBEM_LU_NB;
-      const int m0 = W.mv_off[j], nm = W.mv_off[j + 1] - m0;
-      hipLaunchKernelGGL((k_trsv_fwd_step<T, FA>), dim3(1), dim3(kTT), 0, st, (const int*)W.d_perm + 2 * m0,
-                         nm, k0, n, LU, invL, y);
+      const int* mv = (const int*)W.d_perm;
+      hipLaunchKernelGGL((k_trsv_fwd_step<T, FA>), dim3(1), dim3(kTT), 0, st, mv + j * kMvCap,
+                         mv + lu_steps(n) * kMvCap + j, k0, n, LU, invL, y);
     }
     if (s + 1 < ns) {
       const long rows = n - k0 - kTS;
@@ -2464,6 +2449,22 @@ void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStrea
 }
 
 // ==================================================================================== misc =======
+// Zero fill by a kernel (stream-ordered, and a kernel node when captured in a graph, where memset
+// nodes are avoided): 16-byte stores over the aligned part, then the tail in 4-byte words.
+__global__ __launch_bounds__(256) void k_zero(unsigned* __restrict__ p, size_t words) {
+  const size_t q = words / 4;
+  uint4* p4 = (uint4*)p;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < q; i += (size_t)gridDim.x * 256)
+    p4[i] = make_uint4(0u, 0u, 0u, 0u);
+  const size_t t = 4 * q + (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < words) p[t] = 0u;
+}
+void fill_zero(void* p, size_t bytes, hipStream_t st) {
+  if (!bytes) return;
+  const size_t words = bytes / 4;  // every buffer here holds 4- or 8-byte elements
+  const size_t blocks = std::min<size_t>((words / 4 + 255) / 256 + 1, 65536);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, (unsigned*)p, words);
+}
 // y = alpha A x + beta y. The columns are split into kGemvChunks ranges so that enough loads are in
 // flight to stream A at HBM speed (one row per thread over all n columns keeps too few); the range
 // partial sums are added in a fixed order by a second kernel.

@@ -29,6 +29,13 @@ SOLVE_REUSE = 0x200  # flag: reuse the factors of the last factorisation in the 
 SOLVE_COMPACT = 0x400  # flag: the factor takes A's place (16-bit factor copy for the mixed modes)
 
 
+class BatchOptions(ctypes.Structure):
+    """ibem3d_batch_options: mode, in-graph refinement steps, RVEs per wave, wave memory budget
+    (GB), graph capture on/off."""
+    _fields_ = [("mode", ctypes.c_int), ("refine_steps", ctypes.c_int), ("wave_max", ctypes.c_int),
+                ("mem_budget_gb", ctypes.c_double), ("use_graph", ctypes.c_int)]
+
+
 class _CProblem(ctypes.Structure):
     _fields_ = [("NN", ctypes.c_int), ("NE", ctypes.c_int), ("num", ctypes.c_int),
                 ("nump", ctypes.c_int), ("E0", ctypes.c_double), ("nu0", ctypes.c_double),
@@ -45,7 +52,8 @@ EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size"
            "ibem3d_read_inputs", "ibem3d_inputs_problem", "ibem3d_inputs_title",
            "ibem3d_free_inputs", "ibem3d_io_error", "ibem3d_write_stress", "ibem3d_average_fields",
            "ibem3d_set_known_values", "ibem3d_average_eigenstrain", "ibem3d_residual",
-           "ibem3d_update_known_values"]
+           "ibem3d_update_known_values", "ibem3d_batch_create", "ibem3d_batch_destroy",
+           "ibem3d_batch_last_error", "ibem3d_batch_stiffness"]
 
 
 class EngineError(RuntimeError):
@@ -95,6 +103,13 @@ def load(path: str = LIB_PATH):
     L.ibem3d_average_eigenstrain.argtypes = [ctypes.c_void_p, _dp]
     L.ibem3d_residual.argtypes = [ctypes.c_void_p, _dp]
     L.ibem3d_update_known_values.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.ibem3d_batch_create.restype = ctypes.c_void_p
+    L.ibem3d_batch_create.argtypes = [ctypes.POINTER(_CProblem), ctypes.c_int, ctypes.c_int]
+    L.ibem3d_batch_destroy.argtypes = [ctypes.c_void_p]
+    L.ibem3d_batch_last_error.restype = ctypes.c_char_p
+    L.ibem3d_batch_last_error.argtypes = []
+    L.ibem3d_batch_stiffness.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchOptions), ctypes.c_double,
+                                         _dp, _dp, _dp, _ip, _dp]
     _LIB = L
     return L
 
@@ -132,25 +147,65 @@ def write_stress_native(path: str, s: np.ndarray, full: bool = False) -> None:
         raise EngineError(L.ibem3d_io_error().decode())
 
 
+def _c_problem(prob: Problem):
+    """The ibem3d_problem of prob and the arrays it points into (keep them alive while it is used)."""
+    keep = dict(
+        nodes=np.ascontiguousarray(prob.nodes, dtype=np.float64),
+        conn=np.ascontiguousarray(prob.conn, dtype=np.int32),
+        bc=np.ascontiguousarray(prob.bc, dtype=np.float64),
+        tbc=np.ascontiguousarray(prob.tbc, dtype=np.float64),
+        centres=np.ascontiguousarray(prob.centres, dtype=np.float64).reshape(-1, 3),
+        radii=np.ascontiguousarray(prob.radii, dtype=np.float64),
+        points=np.ascontiguousarray(prob.points, dtype=np.float64).reshape(-1, 3),
+    )
+    k = keep
+    cp = _CProblem(prob.NN, prob.NE, prob.num, prob.nump, prob.E0, prob.nu0, prob.E1, prob.nu1,
+                   _d(k["nodes"]), k["conn"].ctypes.data_as(_ip), _d(k["bc"]), _d(k["tbc"]),
+                   _d(k["centres"]), _d(k["radii"]), _d(k["points"]))
+    return keep, cp
+
+
+def batch_stiffness(problems, mode: int = SOLVE_MIXED_FP16, refine_steps: int = 4, wave_max: int = 16,
+                    mem_budget_gb: float = 200.0, use_graph: bool = True, tol: float = 1e-10,
+                    device: int = 0):
+    """Apparent stiffness of many RVEs (ibem3d_batch_stiffness): problems whose boundary
+    displacements are all prescribed; six unit macro strains per RVE on one factorisation. Returns
+    dict(C (count, 6, 6) from the inclusions, C_bnd (count, 6, 6) from the tractions, rel (count, 6),
+    host_refined (count, 6), times {setup, capture, instantiate, run, host, waves})."""
+    L = load()
+    count = len(problems)
+    keeps, cps = [], (_CProblem * count)()
+    for i, p in enumerate(problems):
+        k, cp = _c_problem(p)
+        keeps.append(k)
+        cps[i] = cp
+    b = L.ibem3d_batch_create(cps, count, device)
+    if not b:
+        raise EngineError(L.ibem3d_batch_last_error().decode())
+    C = np.zeros((count, 6, 6))
+    Cb = np.zeros((count, 6, 6))
+    rel = np.zeros((count, 6))
+    hr = np.zeros((count, 6), dtype=np.int32)
+    tm = np.zeros(6)
+    opt = BatchOptions(mode, refine_steps, wave_max, mem_budget_gb, 1 if use_graph else 0)
+    try:
+        rc = L.ibem3d_batch_stiffness(b, ctypes.byref(opt), tol, _d(C), _d(Cb), _d(rel),
+                                      hr.ctypes.data_as(_ip), _d(tm))
+        if rc != 0:
+            raise EngineError(f"batch_stiffness: {L.ibem3d_batch_last_error().decode()} (code {rc})")
+    finally:
+        L.ibem3d_batch_destroy(b)
+    names = ("setup", "capture", "instantiate", "run", "host", "waves")
+    return dict(C=C, C_bnd=Cb, rel=rel, host_refined=hr, times=dict(zip(names, tm.tolist())))
+
+
 class Engine:
     """One RVE on one GPU: owns the device system and its solution."""
 
     def __init__(self, prob: Problem, device: int = 0):
         self.L = load()
         self.prob = prob
-        self._keep = dict(
-            nodes=np.ascontiguousarray(prob.nodes, dtype=np.float64),
-            conn=np.ascontiguousarray(prob.conn, dtype=np.int32),
-            bc=np.ascontiguousarray(prob.bc, dtype=np.float64),
-            tbc=np.ascontiguousarray(prob.tbc, dtype=np.float64),
-            centres=np.ascontiguousarray(prob.centres, dtype=np.float64).reshape(-1, 3),
-            radii=np.ascontiguousarray(prob.radii, dtype=np.float64),
-            points=np.ascontiguousarray(prob.points, dtype=np.float64).reshape(-1, 3),
-        )
-        k = self._keep
-        cp = _CProblem(prob.NN, prob.NE, prob.num, prob.nump, prob.E0, prob.nu0, prob.E1,
-                       prob.nu1, _d(k["nodes"]), k["conn"].ctypes.data_as(_ip), _d(k["bc"]),
-                       _d(k["tbc"]), _d(k["centres"]), _d(k["radii"]), _d(k["points"]))
+        self._keep, cp = _c_problem(prob)
         self.ctx = self.L.ibem3d_create(ctypes.byref(cp), device)
         if not self.ctx:
             raise EngineError(self.L.ibem3d_last_error(None).decode())

@@ -182,3 +182,87 @@ def make_workload(name: str, seed: int = 1, **over) -> Problem:
     kw = dict(WORKLOADS[name])
     kw.update(over)
     return make_rve(seed, **kw)
+
+
+def unit_strain_tensor(v6) -> np.ndarray:
+    """The symmetric tensor of six components in pair order 00 01 02 11 12 22."""
+    e = np.zeros((3, 3))
+    for (i, j), v in zip(((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)), v6):
+        e[i, j] = e[j, i] = v
+    return e
+
+
+def rsa_spheres_np(rng: np.random.Generator, num: int, rmin: float, rmax: float, gap: float,
+                   wall: float, vf_cap: float | None = None, tries: int = 20000, chunk: int = 256):
+    """Random sequential addition with candidates drawn in chunks (numpy): each sphere takes the
+    first non-overlapping candidate of its chunks; placement stops early when the volume-fraction cap
+    would be exceeded or a sphere finds no place in `tries` candidates (a jammed packing)."""
+    centres = np.zeros((num, 3))
+    radii = np.zeros(num)
+    placed, vol = 0, 0.0
+    while placed < num:
+        r = rng.uniform(rmin, rmax)
+        if vf_cap is not None and vol + 4.0 / 3.0 * math.pi * r ** 3 > vf_cap:
+            break
+        lo, hi = r + wall, 1.0 - r - wall
+        spot = None
+        drawn, size = 0, 8  # chunks double while the candidates keep colliding
+        while drawn < tries:
+            cand = rng.uniform(lo, hi, size=(size, 3))
+            drawn += size
+            if placed == 0:
+                spot = cand[0]
+                break
+            d2 = ((cand[:, None, :] - centres[None, :placed, :]) ** 2).sum(-1)
+            ok = np.all(d2 >= (radii[None, :placed] + r + gap) ** 2, axis=1)
+            if ok.any():
+                spot = cand[int(np.argmax(ok))]
+                break
+            size = min(2 * size, chunk)
+        if spot is None:
+            break
+        centres[placed] = spot
+        radii[placed] = r
+        vol += 4.0 / 3.0 * math.pi * r ** 3
+        placed += 1
+    return centres[:placed].copy(), radii[:placed].copy()
+
+
+def make_batch_rve(seed: int, index: int, *, nmin: int = 8, nmax: int = 256, rmin: float = 0.03,
+                   rmax: float = 0.08, vf_cap: float = 0.3, m: int = 8, gap: float = 0.005,
+                   wall: float = 0.02, E0: float = 1.0, nu0: float = 0.3):
+    """One RVE of the batched workload (seed, index): sphere count drawn uniform in [nmin, nmax]
+    (fewer when the volume-fraction cap or a jammed packing stops the placement), radii uniform in
+    [rmin, rmax], particle modulus log-uniform in [2, 100] x E0 and Poisson ratio uniform in
+    [0.2, 0.35], an m x m mesh per face with the displacement u = eps . x prescribed on every node for
+    a random unit macro strain eps (uniform direction in the six-component space). numpy's PCG64
+    seeded with (seed, index) draws everything, so every machine builds the same RVEs. Returns
+    (Problem, eps as 6 components)."""
+    rng = np.random.default_rng([seed, index])
+    num = int(rng.integers(nmin, nmax + 1))
+    E1 = E0 * math.exp(rng.uniform(math.log(2.0), math.log(100.0)))
+    nu1 = float(rng.uniform(0.2, 0.35))
+    g = rng.standard_normal(6)
+    eps6 = g / np.linalg.norm(g)
+    centres, radii = rsa_spheres_np(rng, num, rmin, rmax, gap, wall, vf_cap=vf_cap)
+    nodes, conn, face = cube_mesh(m)
+    bc, tbc = macro_strain_bc(nodes, conn, unit_strain_tensor(eps6))
+    prob = Problem(f"batch_rve_{seed}_{index}", E0, nu0, E1, nu1, nodes, conn, bc, tbc, centres, radii,
+                   np.zeros((0, 3)))
+    return prob, eps6
+
+
+def _batch_item(args):
+    seed, i, kw = args
+    return make_batch_rve(seed, i, **kw)
+
+
+def make_batch(seed: int, count: int, workers: int = 1, **kw):
+    """count RVEs of the batched workload: [(Problem, eps6)]; with workers > 1 built in that many
+    processes (each RVE depends only on (seed, index), so the result is the same)."""
+    items = [(seed, i, kw) for i in range(count)]
+    if workers <= 1:
+        return [_batch_item(a) for a in items]
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(_batch_item, items, chunksize=8))

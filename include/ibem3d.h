@@ -169,6 +169,32 @@ int ibem3d_solve_info(const ibem3d_ctx* ctx, double* rel_residual, int* iteratio
 int ibem3d_timings(const ibem3d_ctx* ctx, char* names, int cap, double* ms, int max_entries);
 void ibem3d_reset_timings(ibem3d_ctx* ctx);
 
+/* ---- batched RVE stiffness (no counterpart in the reference, which solves one RVE per process,
+ * configElastic3D.cpp:27-30) ---- */
+typedef struct ibem3d_batch ibem3d_batch;
+typedef struct {
+  int mode;             /* IBEM3D_SOLVE_MIXED_* */
+  int refine_steps;     /* classic refinement steps per load inside the graph (>= 1) */
+  int wave_max;         /* RVEs per wave at most */
+  double mem_budget_gb; /* device memory a wave may use (GB); a wave holds at least one RVE */
+  int use_graph;        /* 1: each wave captured as one graph and run by one launch; 0: direct launches */
+} ibem3d_batch_options;
+/* count problems whose boundary displacements are all prescribed (every BEM.txt displacement flag 0,
+ * every traction flag 0): the affine loads of the apparent stiffness. The problems are copied. */
+ibem3d_batch* ibem3d_batch_create(const ibem3d_problem* probs, int count, int device);
+void ibem3d_batch_destroy(ibem3d_batch* b);
+const char* ibem3d_batch_last_error(void);
+/* Apparent stiffness of every RVE from the six unit macro strains u = e_t . x (t in the order 00 01 02
+ * 11 12 22, e_ij = e_ji = 1), one factorisation per RVE: C (count x 36, row-major 6 x 6 per RVE,
+ * entry [c][t] = stress component c under load t) from the inclusions, <s> = C0 : (<e> - <e*>);
+ * C_bnd (may be NULL) from the boundary tractions, <s> = (1/V) int t x dS; rel (count x 6, may be
+ * NULL): each load's final relative FP64 residual; host_refined (count x 6, may be NULL): 1 where the
+ * in-graph refinement had not reached tol and the load was refined further on the host; times (6,
+ * may be NULL): seconds of setup, capture, instantiation, device run, host post-processing, and the
+ * number of waves. */
+int ibem3d_batch_stiffness(ibem3d_batch* b, const ibem3d_batch_options* opt, double tol, double* C,
+                           double* C_bnd, double* rel, int* host_refined, double* times);
+
 /* ---- the reference's text formats (host only; no GPU call) ---- */
 typedef struct ibem3d_inputs ibem3d_inputs;
 /* Read position.txt, BEM.txt and (if present) postprocess.txt from directory `dir`:
