@@ -232,6 +232,15 @@ const char* ibem3d_last_error(const ibem3d_ctx* ctx) {
 
 long ibem3d_size(const ibem3d_ctx* ctx) { return ctx ? ctx->n : 0; }
 
+int ibem3d_set_tuning(int key, long value) {
+  if (set_tuning(key, value)) {
+    g_last_error = "set_tuning: unknown key or value out of range";
+    return IBEM3D_EINVAL;
+  }
+  return 0;
+}
+long ibem3d_get_tuning(int key) { return get_tuning(key); }
+
 ibem3d_ctx* ibem3d_create(const ibem3d_problem* p, int device) {
   if (!p || p->NN <= 0 || p->NE <= 0 || p->num < 0 || p->nump < 0 || !p->nodes || !p->conn ||
       !p->bc || !p->tbc || (p->num > 0 && (!p->centres || !p->radii)) ||

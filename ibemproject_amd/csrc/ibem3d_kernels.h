@@ -132,6 +132,12 @@ size_t factor16_scales(long n);
 void factor_to_16(const float* LU, long n, long lda, unsigned short* F, long ldf, float* isc, hipStream_t st);
 void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, const LuWork& W, float* x,
                  hipStream_t st);
+// Runtime tunables of the LU (process-wide): the 256 x 256 tile count from which trailing updates
+// take the persistent kernel, and the percentage of the panel CU schedule the bulk update leaves
+// to the concurrent panel. set_tuning returns nonzero for an unknown key or a value out of range.
+enum { kTuneBigMinTiles = 1, kTunePanelCuPercent = 2 };
+int set_tuning(int key, long value);
+long get_tuning(int key);
 // Zero bytes (a multiple of 4; p 16-byte aligned) with a kernel instead of a memset (capturable)
 void fill_zero(void* p, size_t bytes, hipStream_t st);
 // FP64 -> FP32 copy of a column-major matrix

@@ -870,6 +870,25 @@ static unsigned big_gemm_grid(long tiles, int cus) {
 #ifndef IBEM_BIG_MIN_TILES
 #define IBEM_BIG_MIN_TILES 512
 #endif
+// Runtime tunables (process-wide; set through ibem3d_set_tuning before the work they shape is
+// launched or captured): the 256 x 256 tile count from which trailing updates take the persistent
+// kernel, and the share (percent) of the panel_cus schedule the bulk update leaves to the panel.
+static long g_big_min_tiles = IBEM_BIG_MIN_TILES;
+static long g_panel_cu_percent = 100;
+int set_tuning(int key, long value) {
+  if (key == kTuneBigMinTiles && value >= 1) {
+    g_big_min_tiles = value;
+    return 0;
+  }
+  if (key == kTunePanelCuPercent && value >= 0 && value <= 400) {
+    g_panel_cu_percent = value;
+    return 0;
+  }
+  return -1;
+}
+long get_tuning(int key) {
+  return key == kTuneBigMinTiles ? g_big_min_tiles : key == kTunePanelCuPercent ? g_panel_cu_percent : -1;
+}
 // C[M x N] -= split A times the split B columns [c0, c0 + N); the persistent 256 x 256 kernel runs
 // on at most cus CUs (0: all)
 template <int PROD>
@@ -877,7 +896,7 @@ static void f16x3_gemm_t(long M, long N, int K, const F16x3Ws& w, float* C, long
                          long c0, int cus) {
   // 256 x 256 tiles once there are at least two per CU (one workgroup per CU), else 128 x 128
   const long big = ((M + kBigT - 1) / kBigT) * ((N + kBigT - 1) / kBigT);
-  if (big >= IBEM_BIG_MIN_TILES) {  // operands are zero-padded to kSplitPadK: K is implied
+  if (big >= g_big_min_tiles) {  // operands are zero-padded to kSplitPadK: K is implied
     hipLaunchKernelGGL(k_gemm_f16x3_256<PROD>, dim3(big_gemm_grid(big, cus)),
                        dim3(64 * kBigWaves), 0, st, M, N, w.Ah, w.Al, w.Bh + c0 * kSplitPadK,
                        w.Bl + c0 * kSplitPadK, w.ia, w.ib + c0, C, ldc);
@@ -1692,11 +1711,16 @@ constexpr long kLuNB = IBEM_LU_NB;
 #define IBEM_PANEL_M2 29000
 #define IBEM_PANEL_M3 20000
 #endif
-static int panel_cus(long m) {
+static int panel_cus_base(long m) {
   if (m > IBEM_PANEL_M1) return 32;
   if (m > IBEM_PANEL_M2) return 64;
   if (m > IBEM_PANEL_M3) return 96;
   return 128;
+}
+// the schedule scaled by the tunable, in whole XCDs (8 CUs), below the device's CU count
+static int panel_cus(long m) {
+  const long c = (panel_cus_base(m) * g_panel_cu_percent / 100 + 4) / 8 * 8;
+  return (int)std::min<long>(c, device_cus() - 8);
 }
 constexpr long kNarrowCols = 4096;  // columns up to which U12 uses the wavefront TRSM, not the block inverses
 

@@ -27,6 +27,18 @@ SOLVE_MIXED_BF16 = 4     # FP32 LU, trailing updates on one BF16 product
 SOLVE_GMRES = 0x100  # flag for the mixed modes: GMRES-IR from the start
 SOLVE_REUSE = 0x200  # flag: reuse the factors of the last factorisation in the same mode
 SOLVE_COMPACT = 0x400  # flag: the factor takes A's place (16-bit factor copy for the mixed modes)
+TUNE_BIG_MIN_TILES = 1
+TUNE_PANEL_CU_PERCENT = 2
+
+
+def set_tuning(key: int, value: int) -> None:
+    L = load()
+    if L.ibem3d_set_tuning(key, value) != 0:
+        raise EngineError(f"set_tuning({key}, {value}): {L.ibem3d_last_error(None).decode()}")
+
+
+def get_tuning(key: int) -> int:
+    return int(load().ibem3d_get_tuning(key))
 
 
 class BatchOptions(ctypes.Structure):
@@ -53,7 +65,8 @@ EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size"
            "ibem3d_free_inputs", "ibem3d_io_error", "ibem3d_write_stress", "ibem3d_average_fields",
            "ibem3d_set_known_values", "ibem3d_average_eigenstrain", "ibem3d_residual",
            "ibem3d_update_known_values", "ibem3d_batch_create", "ibem3d_batch_destroy",
-           "ibem3d_batch_last_error", "ibem3d_batch_stiffness"]
+           "ibem3d_batch_last_error", "ibem3d_batch_stiffness", "ibem3d_set_tuning",
+           "ibem3d_get_tuning"]
 
 
 class EngineError(RuntimeError):
@@ -103,6 +116,9 @@ def load(path: str = LIB_PATH):
     L.ibem3d_average_eigenstrain.argtypes = [ctypes.c_void_p, _dp]
     L.ibem3d_residual.argtypes = [ctypes.c_void_p, _dp]
     L.ibem3d_update_known_values.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.ibem3d_set_tuning.argtypes = [ctypes.c_int, ctypes.c_long]
+    L.ibem3d_get_tuning.restype = ctypes.c_long
+    L.ibem3d_get_tuning.argtypes = [ctypes.c_int]
     L.ibem3d_batch_create.restype = ctypes.c_void_p
     L.ibem3d_batch_create.argtypes = [ctypes.POINTER(_CProblem), ctypes.c_int, ctypes.c_int]
     L.ibem3d_batch_destroy.argtypes = [ctypes.c_void_p]

@@ -96,6 +96,17 @@ ibem3d_ctx* ibem3d_create(const ibem3d_problem* prob, int device);
 void ibem3d_destroy(ibem3d_ctx* ctx);
 const char* ibem3d_last_error(const ibem3d_ctx* ctx);
 
+/* Runtime tunables of the solver, process-wide (no counterpart in the reference); they shape the
+ * launches enqueued or captured after the call. Set by the batch autotuner. */
+enum {
+  IBEM3D_TUNE_BIG_MIN_TILES = 1,     /* 256 x 256 tiles from which a trailing update takes the persistent
+                                        kernel (default 512; below it the 128 x 128 kernel) */
+  IBEM3D_TUNE_PANEL_CU_PERCENT = 2   /* percent of the default schedule of CUs the bulk trailing update
+                                        leaves to the concurrent panel (default 100, 0..400) */
+};
+int ibem3d_set_tuning(int key, long value);
+long ibem3d_get_tuning(int key);  /* -1 for an unknown key */
+
 /* System size n = 3 NN + 60 num. */
 long ibem3d_size(const ibem3d_ctx* ctx);
 
