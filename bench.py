@@ -283,7 +283,7 @@ def batch_leg(args, hip):
     cfg = tuned["best"]
     mode = autotune.MODES[cfg["mode"]]
     kw = dict(refine_steps=cfg["refine_steps"], wave_max=cfg["wave_max"], mem_budget_gb=budget,
-              use_graph=cfg["use_graph"])
+              use_graph=cfg["use_graph"], host_threads=cfg["host_threads"])
     for i in range(args.warmup):
         batch_stiffness(probs, mode, **kw)
         progress(f"warmup pass {i + 1}/{args.warmup} done")
@@ -352,6 +352,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import ctypes
+    if args.workload.startswith("batch"):
+        # the batch's latency-bound small solves overlap across streams: give the process 16
+        # hardware queues (HIP's default is 4) before the runtime starts
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
     hip = ctypes.CDLL("libamdhip64.so")
     if args.contrasts:
         contrasts(args, hip)

@@ -3,9 +3,10 @@
 The batch's schedule has choices whose best values depend on the RVE sizes and the device: the
 LU's product type (FP16 split products, single FP16 products, FP32), the number of classic
 refinement steps run inside the graph (too few send loads to the slower host refinement), the
-number of RVEs per wave (graph branches that run concurrently), graph capture or direct launches,
-and two kernel-level tunables (the tile count from which trailing updates take the 256 x 256
-kernel, and the share of CUs the bulk update leaves to the concurrent panel). `tune` measures
+number of RVEs per wave (graph branches that run concurrently), graph capture or direct launches
+(enqueued from one or several host threads), and two kernel-level tunables (the tile count from
+which trailing updates take the 256 x 256 kernel, and the share of CUs the bulk update leaves to
+the concurrent panel). `tune` measures
 candidate settings on a sample of the batch that spans its size range, one choice at a time (a
 coordinate search: mode, refinement steps, wave size, graph, kernel tunables), keeps a setting only
 when every load converged inside the graph or its time improves, and returns the best with the
@@ -39,13 +40,13 @@ def measure(probs, cfg: dict, mem_budget_gb: float) -> dict:
     t0 = time.perf_counter()
     r = E.batch_stiffness(probs, MODES[cfg["mode"]], refine_steps=cfg["refine_steps"],
                           wave_max=cfg["wave_max"], mem_budget_gb=mem_budget_gb,
-                          use_graph=cfg["use_graph"])
+                          use_graph=cfg["use_graph"], host_threads=cfg["host_threads"])
     wall = time.perf_counter() - t0
     return {"cfg": dict(cfg), "wall_s": wall, "host_refined": int(r["host_refined"].sum()),
             "max_rel": float(r["rel"].max()), "times": r["times"]}
 
 
-DEFAULT = {"mode": "fp16x3", "refine_steps": 4, "wave_max": 16, "use_graph": True,
+DEFAULT = {"mode": "fp16x3", "refine_steps": 4, "wave_max": 16, "use_graph": True, "host_threads": 1,
            "big_min_tiles": 512, "panel_cu_percent": 100}
 
 SPACE = [
@@ -53,6 +54,7 @@ SPACE = [
     ("refine_steps", [2, 3, 4, 5, 6]),
     ("wave_max", [4, 8, 16, 32, 64]),
     ("use_graph", [True, False]),
+    ("host_threads", [1, 4, 8, 16]),
     ("big_min_tiles", [128, 256, 512, 1024]),
     ("panel_cu_percent", [0, 50, 100, 200]),
 ]

@@ -8,8 +8,11 @@
 // (configElastic3D.cpp:27-30).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <thread>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -246,11 +249,29 @@ int ibem3d_batch_stiffness(ibem3d_batch* b, const ibem3d_batch_options* opt, dou
         rc = IBEM3D_EHIP;
       }
     } else {
-      // the same work launched directly, every context on its own streams
-      bool ok = true;
-      for (int w = 0; w < W && ok; ++w)
-        ok = ibem_batch::enqueue(ctx[w], mode, K, 6, d_u[w], d_x[w], d_rel[w], d_part[w]) == 0;
-      for (int w = 0; w < W && ok; ++w) ok = hipStreamSynchronize(ibem_batch::stream(ctx[w])) == hipSuccess;
+      // the same work launched directly, every context on its own streams; with host_threads > 1
+      // the contexts are enqueued from that many host threads (the launch rate of one thread
+      // bounds the small RVEs' latency-bound solves)
+      std::atomic<int> next{0}, bad{0};
+      auto worker = [&]() {
+        if (hipSetDevice(b->dev) != hipSuccess) {
+          bad = 1;
+          return;
+        }
+        for (int w; (w = next++) < W;) {
+          if (ibem_batch::enqueue(ctx[w], mode, K, 6, d_u[w], d_x[w], d_rel[w], d_part[w]) != 0) bad = 1;
+        }
+      };
+      const int nt = std::max(1, std::min(opt->host_threads, W));
+      if (nt == 1) {
+        worker();
+      } else {
+        std::vector<std::thread> th;
+        for (int i = 0; i < nt; ++i) th.emplace_back(worker);
+        for (auto& t : th) t.join();
+      }
+      bool ok = !bad;
+      for (int w = 0; w < W; ++w) ok = hipStreamSynchronize(ibem_batch::stream(ctx[w])) == hipSuccess && ok;
       t_run += now() - tc;
       if (!ok) {
         g_batch_error = "batch: launch failed";

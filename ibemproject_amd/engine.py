@@ -45,7 +45,8 @@ class BatchOptions(ctypes.Structure):
     """ibem3d_batch_options: mode, in-graph refinement steps, RVEs per wave, wave memory budget
     (GB), graph capture on/off."""
     _fields_ = [("mode", ctypes.c_int), ("refine_steps", ctypes.c_int), ("wave_max", ctypes.c_int),
-                ("mem_budget_gb", ctypes.c_double), ("use_graph", ctypes.c_int)]
+                ("mem_budget_gb", ctypes.c_double), ("use_graph", ctypes.c_int),
+                ("host_threads", ctypes.c_int)]
 
 
 class _CProblem(ctypes.Structure):
@@ -183,7 +184,7 @@ def _c_problem(prob: Problem):
 
 def batch_stiffness(problems, mode: int = SOLVE_MIXED_FP16, refine_steps: int = 4, wave_max: int = 16,
                     mem_budget_gb: float = 200.0, use_graph: bool = True, tol: float = 1e-10,
-                    device: int = 0):
+                    device: int = 0, host_threads: int = 1):
     """Apparent stiffness of many RVEs (ibem3d_batch_stiffness): problems whose boundary
     displacements are all prescribed; six unit macro strains per RVE on one factorisation. Returns
     dict(C (count, 6, 6) from the inclusions, C_bnd (count, 6, 6) from the tractions, rel (count, 6),
@@ -203,7 +204,7 @@ def batch_stiffness(problems, mode: int = SOLVE_MIXED_FP16, refine_steps: int = 
     rel = np.zeros((count, 6))
     hr = np.zeros((count, 6), dtype=np.int32)
     tm = np.zeros(6)
-    opt = BatchOptions(mode, refine_steps, wave_max, mem_budget_gb, 1 if use_graph else 0)
+    opt = BatchOptions(mode, refine_steps, wave_max, mem_budget_gb, 1 if use_graph else 0, host_threads)
     try:
         rc = L.ibem3d_batch_stiffness(b, ctypes.byref(opt), tol, _d(C), _d(Cb), _d(rel),
                                       hr.ctypes.data_as(_ip), _d(tm))

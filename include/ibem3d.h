@@ -189,6 +189,7 @@ typedef struct {
   int wave_max;         /* RVEs per wave at most */
   double mem_budget_gb; /* device memory a wave may use (GB); a wave holds at least one RVE */
   int use_graph;        /* 1: each wave captured as one graph and run by one launch; 0: direct launches */
+  int host_threads;     /* direct launches: host threads enqueuing the wave's RVEs (<= 1: one) */
 } ibem3d_batch_options;
 /* count problems whose boundary displacements are all prescribed (every BEM.txt displacement flag 0,
  * every traction flag 0): the affine loads of the apparent stiffness. The problems are copied. */
