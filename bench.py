@@ -29,15 +29,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from ibemproject_amd.engine import (SOLVE_FP64_LU, SOLVE_MIXED_BF16, SOLVE_MIXED_FP16,  # noqa: E402
-                                    SOLVE_MIXED_FP16_1, SOLVE_MIXED_FP32, Engine)
+from ibemproject_amd.engine import (SOLVE_BJ_GMRES, SOLVE_FP64_LU, SOLVE_MIXED_BF16,  # noqa: E402
+                                    SOLVE_MIXED_FP16, SOLVE_MIXED_FP16_1, SOLVE_MIXED_FP32, Engine)
 from ibemproject_amd.generate import make_workload  # noqa: E402
 
 MI355X_PEAK = {"f16_mfma_tflops": 2500.0, "f32_mfma_tflops": 157.3, "f64_mfma_tflops": 78.6,
                "hbm_gbs": 8000.0}
 NB = 1024  # LU block size (solver.hip IBEM_LU_NB)
 DEFAULT_WORKLOAD = "rve2500_m24"
-MODES = {"fp16_1": SOLVE_MIXED_FP16_1, "fp16x3": SOLVE_MIXED_FP16, "bf16": SOLVE_MIXED_BF16,
+MODES = {"bj_gmres": SOLVE_BJ_GMRES, "fp16_1": SOLVE_MIXED_FP16_1, "fp16x3": SOLVE_MIXED_FP16, "bf16": SOLVE_MIXED_BF16,
          "fp32": SOLVE_MIXED_FP32}
 # the trailing-update kernel of each mode (k_gemm_f16x3_256 / k_gemm_f16x3 instantiated for the
 # product type) and its MFMA products per algorithmic multiply-add
@@ -331,7 +331,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD)
-    ap.add_argument("--mode", default="fp16_1", choices=sorted(MODES))
+    ap.add_argument("--mode", default="bj_gmres", choices=sorted(MODES))
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -395,7 +395,7 @@ def main():
     if not args.no_fp64:
         fp64, x64, s64 = side_step(eng, hip, SOLVE_FP64_LU, "solve.lu_f64", flops, x, s)
         progress(f"FP64 LU step: {fp64['step_s']:.3f} s")
-    cmp_mode = "fp16x3" if args.mode != "fp16x3" else "fp16_1"
+    cmp_mode = {"bj_gmres": "fp16_1", "fp16x3": "fp16_1"}.get(args.mode, "fp16x3")
     other = None
     if not args.no_compare:
         ref_x, ref_s = (x64, s64) if fp64 else (x, s)
@@ -405,9 +405,49 @@ def main():
     base = None if args.no_cpu_baseline else cpu_baseline(prob, args.workload)
     if base:
         progress(f"CPU baseline: {base['value']:.4g} s (extrapolated)")
-    kname, prods = GEMM_KERNEL[args.mode]
-    op_bytes = 4 if args.mode == "fp16x3" else (4 if args.mode == "fp32" else 2)
-    peak = MI355X_PEAK["f32_mfma_tflops"] if args.mode == "fp32" else MI355X_PEAK["f16_mfma_tflops"]
+    gemv = None
+    if args.mode == "bj_gmres":
+        # the timed step's dominant kernel: the FP64 matrix-vector product over A (k_gemv_part +
+        # k_gemv_done), one per GMRES iteration and one per restart cycle (its true residual)
+        its = rels[-1][1]
+        gemv_launches = its + max(1, -(-its // 100))
+        gemv_ms = tm.get("solve.residual", 0.0) / args.steps / gemv_launches
+        gbytes = 8.0 * n * n + 8.0 * 33 * n  # A once, x per column chunk, partial sums
+        gemv = {"kernel": "k_gemv_part + k_gemv_done (FP64 A x: 32 column chunks, two rows per thread, "
+                          "16-byte loads; fixed-order partial sums)",
+                "bound": "hbm", "achieved": gbytes / (gemv_ms * 1e-3) / 1e9, "peak": MI355X_PEAK["hbm_gbs"],
+                "unit": "GB/s", "frac": gbytes / (gemv_ms * 1e-3) / 1e9 / MI355X_PEAK["hbm_gbs"],
+                "traffic": None, "traffic_algorithmic": gbytes, "launches_per_step": gemv_launches,
+                "avg_launch_ms": gemv_ms,
+                "note": "achieved = algorithmic bytes per product (A read once: 8 n^2, plus x and the "
+                        "partial sums) / the average product time (HIP events around each product on "
+                        "the context's stream)"}
+    lu_side = other if args.mode == "bj_gmres" else None
+    lu_mode = cmp_mode if args.mode == "bj_gmres" else args.mode
+    lu_gemm_ms = (lu_side["phase_ms"].get("lu.gemm", 0.0) if lu_side else gemm_ms)
+    lu_achieved = flops / (lu_gemm_ms * 1e-3) / 1e12 if lu_gemm_ms else None
+    kname, prods = GEMM_KERNEL[lu_mode]
+    op_bytes = 4 if lu_mode in ("fp16x3", "fp32") else 2
+    peak = MI355X_PEAK["f32_mfma_tflops"] if lu_mode == "fp32" else MI355X_PEAK["f16_mfma_tflops"]
+    lu_roof = {"kernel": kname, "bound": "mfma", "achieved": lu_achieved, "peak": peak,
+               "unit": "TFLOP/s", "frac": lu_achieved / peak if lu_achieved else None,
+               "traffic": GEMM_TRAFFIC_BYTES_PER_LAUNCH.get((args.workload, lu_mode)),
+               "traffic_unit": "bytes per launch (HBM, counter-measured, averaged over the LU's launches)",
+               "traffic_algorithmic": trailing_update_bytes(n, op_bytes),
+               "launches_per_step": launches,
+               "avg_launch_ms": lu_gemm_ms / launches if launches else None,
+               "flops_per_step": flops,
+               "mfma_issued_tflops": prods * lu_achieved if lu_achieved else None,
+               "note": f"{lu_mode} LU step: achieved = algorithmic trailing-update flops (2 m^2 {NB} "
+                       "per LU step) / the trailing-update GEMM time of the step (HIP events around "
+                       "each launch on the stream it runs on)"}
+    solver = ("fp64 GMRES right-preconditioned by block Jacobi (boundary-block LU, inverse 60x60 "
+              "sphere blocks) to 1e-10, mixed LU as fallback" if args.mode == "bj_gmres"
+              else f"{args.mode} LU + fp64 GMRES-IR / iterative refinement to 1e-10")
+    dtype = {"bj_gmres": "f64 (assembly, block-Jacobi preconditioner, GMRES, residual)",
+             "fp16_1": ("f64 assembly, residual and solution update; f32 LU with f16 mfma products "
+                        "(one per multiply-add, per-row/column pow2 scaling) and f32 accumulation; "
+                        "f16 factor copy (per-tile pow2 scaling)")}.get(args.mode, args.mode)
     out = {
         "metric": "RVE solve time at 1e-10 FP64 residual vs plain FP64 LU; assembly/LU TFLOP/s",
         "value": step_s,
@@ -419,39 +459,25 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f64 assembly, residual and solution update; f32 LU with f16 mfma products "
-                  "(one per multiply-add, per-row/column pow2 scaling) and f32 accumulation; "
-                  "f16 factor copy (per-tile pow2 scaling)" if args.mode == "fp16_1" else args.mode),
+        "dtype": dtype,
         "data": "synthetic (seeded RSA spheres, generated mesh)",
         "config": {"workload": args.workload, "spheres": prob.num, "nodes": prob.NN,
                    "elements": prob.NE, "unknowns": n, "points": prob.nump,
-                   "solver": f"{args.mode} LU + fp64 GMRES-IR / iterative refinement to 1e-10",
-                   "parallelism": "single GPU"},
+                   "solver": solver, "parallelism": "single GPU"},
         "rel_residual": rels[-1][0],
-        "refinement_iterations": rels[-1][1],
+        "iterations": rels[-1][1],
         "error_vs_fp64_lu": ({"x_rel": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
                               "stress_rel": float(np.abs(s - s64).max() / np.abs(s64).max())}
                              if fp64 else None),
         "assembly_ms": asm_ms,
-        "lu_ms": lu_ms,
+        "lu_ms": lu_ms or None,
         "lu_tflops_effective": 2.0 * n ** 3 / 3 / (lu_ms * 1e-3) / 1e12 if lu_ms else None,
         "phase_ms": {k: v / args.steps for k, v in tm.items()},
         "fp64_lu_step": fp64,
         f"{cmp_mode}_step": other,
         "speedup_vs_fp64_lu": (fp64["step_s"] / step_s) if fp64 else None,
-        "roofline": {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak if achieved else None,
-                     "traffic": GEMM_TRAFFIC_BYTES_PER_LAUNCH.get((args.workload, args.mode)),
-                     "traffic_unit": "bytes per launch (HBM, counter-measured, averaged over the "
-                                     "step's launches)",
-                     "traffic_algorithmic": trailing_update_bytes(n, op_bytes),
-                     "launches_per_step": launches,
-                     "avg_launch_ms": gemm_ms / launches if launches else None,
-                     "flops_per_step": flops,
-                     "mfma_issued_tflops": prods * achieved if achieved else None,
-                     "note": f"achieved = algorithmic trailing-update flops (2 m^2 {NB} per LU "
-                             "step) / the trailing-update GEMM time of the step (HIP events around "
-                             "each launch on the stream it runs on)"},
+        "roofline": gemv if gemv else lu_roof,
+        "lu_roofline": lu_roof if gemv else None,
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)

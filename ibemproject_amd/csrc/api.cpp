@@ -60,6 +60,15 @@ struct ibem3d_ctx {
   float* d_F16s = nullptr;
   long ldf = 0;
   bool a_factored = false;     // d_A holds a factor in place of the assembled system
+  // block-Jacobi preconditioner (IBEM3D_SOLVE_BJ_GMRES): the equilibrated boundary block R A_bb C
+  // (nb x nb, nb = 3 NN) with its LU workspace, and the inverse of every sphere's 60 x 60 diagonal block
+  double *d_Abb = nullptr, *d_rb = nullptr, *d_cb = nullptr, *d_Binv = nullptr, *d_tb = nullptr;
+  LuWork Wb;
+  // initial guess of the eigenstrain unknowns (60 num) for the block-Jacobi GMRES, if set
+  double* d_x0s = nullptr;
+  bool has_x0 = false;
+  bool bj_fallback = false;    // the last block-Jacobi solve fell back to the mixed LU
+  double warm_rel = -1.0;      // relative residual of the last warm start (-1: none)
   LuWork W;
   unsigned phases = 0;
   bool solved = false;
@@ -323,13 +332,19 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
                   ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32, ctx->d_f16ws, ctx->d_eqws,
-                  ctx->d_F16,   ctx->d_F16s,
+                  ctx->d_F16,   ctx->d_F16s,     ctx->d_Abb,      ctx->d_rb,   ctx->d_cb,
+                  ctx->d_Binv,  ctx->d_tb,       ctx->d_x0s,      ctx->Wb.d_ipiv, ctx->Wb.d_ws,
+                  ctx->Wb.d_info, ctx->Wb.d_perm, ctx->Wb.d_inv,  ctx->Wb.d_tmp, ctx->Wb.d_trinv,
+                  ctx->Wb.d_trsm,
                   ctx->d_gV,    ctx->d_gZ,       ctx->d_gpart,    ctx->d_gh,   ctx->d_bemws, ctx->d_gemvws,
                   ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info, ctx->W.d_perm, ctx->W.d_inv, ctx->W.d_tmp,
                   ctx->W.d_trinv, ctx->W.d_trsm};
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (ctx->W.st2) hipStreamDestroy(ctx->W.st2);
+  if (ctx->Wb.st2) hipStreamDestroy(ctx->Wb.st2);
+  for (hipEvent_t e : {ctx->Wb.ev_a, ctx->Wb.ev_b, ctx->Wb.ev_r, ctx->Wb.ev_cols, ctx->W.ev_r, ctx->W.ev_cols})
+    if (e) hipEventDestroy(e);
   if (ctx->W.ev_a) hipEventDestroy(ctx->W.ev_a);
   if (ctx->W.ev_b) hipEventDestroy(ctx->W.ev_b);
   if (ctx->st) hipStreamDestroy(ctx->st);
@@ -480,19 +495,26 @@ int ibem3d_get_system(ibem3d_ctx* ctx, double* A, double* b) {
 }
 
 // ---- solve ------------------------------------------------------------------------------------
+// the device workspaces of an LU of order n (FP64-sized, so FP32 fits too)
+static int alloc_luwork(ibem3d_ctx* ctx, LuWork& W, long n) {
+  if (W.d_ipiv) return 0;
+  if (int rc = dalloc(ctx, &W.d_ipiv, (size_t)n)) return rc;
+  if (int rc = dalloc(ctx, &W.d_ws, lu_workspace_doubles(n))) return rc;
+  fill_zero(W.d_ws, sizeof(double) * lu_workspace_doubles(n), ctx->st);
+  if (int rc = dalloc(ctx, &W.d_info, 1)) return rc;
+  if (int rc = dalloc(ctx, &W.d_perm, lu_moves_ints(n))) return rc;
+  if (int rc = dalloc(ctx, &W.d_inv, lu_inverse_doubles(n))) return rc;
+  if (int rc = dalloc(ctx, &W.d_tmp, (size_t)n)) return rc;
+  if (int rc = dalloc(ctx, &W.d_trinv, lu_trinv_doubles())) return rc;
+  if (int rc = dalloc(ctx, &W.d_trsm, lu_trsm_doubles(n))) return rc;
+  return 0;
+}
+
 static int ensure_solver_ws(ibem3d_ctx* ctx) {
   const long n = ctx->n;
   if (!ctx->W.d_ipiv) {
-    if (int rc = dalloc(ctx, &ctx->W.d_ipiv, (size_t)n)) return rc;
-    if (int rc = dalloc(ctx, &ctx->W.d_ws, lu_workspace_doubles(n))) return rc;
-    fill_zero(ctx->W.d_ws, sizeof(double) * lu_workspace_doubles(n), ctx->st);
-    if (int rc = dalloc(ctx, &ctx->W.d_info, 1)) return rc;
-    if (int rc = dalloc(ctx, &ctx->W.d_perm, lu_moves_ints(n))) return rc;
-    if (int rc = dalloc(ctx, &ctx->W.d_inv, lu_inverse_doubles(n))) return rc;
-    if (int rc = dalloc(ctx, &ctx->W.d_tmp, (size_t)n)) return rc;
+    if (int rc = alloc_luwork(ctx, ctx->W, n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_eqws, equilibrate_workspace_doubles(n))) return rc;
-    if (int rc = dalloc(ctx, &ctx->W.d_trinv, lu_trinv_doubles())) return rc;
-    if (int rc = dalloc(ctx, &ctx->W.d_trsm, lu_trsm_doubles(n))) return rc;
     if (int rc = dalloc(ctx, &ctx->d_r, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_c, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_work, (size_t)n)) return rc;
@@ -572,6 +594,123 @@ static int dots(ibem3d_ctx* ctx, const double* V, int k, const double* w, double
   return 0;
 }
 
+// ---- block-Jacobi preconditioner -------------------------------------------------------------
+// Inverse of every sphere's 60 x 60 diagonal block of A (rows and columns 3 NN + 60 h ...): Gauss-
+// Jordan elimination with partial pivoting on [B | I] in LDS, one workgroup per sphere. The output is
+// column-major, inv + 3600 h. A zero pivot leaves the block's inverse as NaN (GMRES then stops and
+// the solve falls back to the LU).
+constexpr int kBlk = 60;
+__global__ __launch_bounds__(256) void k_block_inv(const double* __restrict__ A, long n, long row0,
+                                                   double* __restrict__ inv) {
+  __shared__ double M[kBlk][2 * kBlk + 1];
+  __shared__ int piv;
+  const int h = blockIdx.x, t = threadIdx.x;
+  const long base = row0 + (long)kBlk * h;
+  for (int u = t; u < kBlk * kBlk; u += 256) {
+    const int c = u / kBlk, r = u % kBlk;  // column-major reads: consecutive threads, consecutive rows
+    M[r][c] = A[(base + c) * n + base + r];
+    M[r][kBlk + c] = r == c ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < kBlk; ++k) {
+    if (t == 0) {
+      int p = k;
+      double best = fabs(M[k][k]);
+      for (int i = k + 1; i < kBlk; ++i)
+        if (fabs(M[i][k]) > best) {
+          best = fabs(M[i][k]);
+          p = i;
+        }
+      piv = p;
+    }
+    __syncthreads();
+    const int p = piv;
+    if (p != k)
+      for (int c = t; c < 2 * kBlk; c += 256) {
+        const double a = M[k][c];
+        M[k][c] = M[p][c];
+        M[p][c] = a;
+      }
+    __syncthreads();
+    const double d = 1.0 / M[k][k];
+    __syncthreads();
+    for (int c = t; c < 2 * kBlk; c += 256) M[k][c] *= d;
+    __syncthreads();
+    for (int u = t; u < kBlk * 2 * kBlk; u += 256) {
+      const int i = u / (2 * kBlk), c = u % (2 * kBlk);
+      if (i != k && c != k) M[i][c] -= M[i][k] * M[k][c];
+    }
+    __syncthreads();
+    // the column k multipliers M[i][k] are read by the whole pass above: they are zeroed only now
+    for (int i = t; i < kBlk; i += 256)
+      if (i != k) M[i][k] = 0.0;
+    __syncthreads();
+  }
+  double* out = inv + (size_t)kBlk * kBlk * h;
+  for (int u = t; u < kBlk * kBlk; u += 256) {
+    const int c = u / kBlk, r = u % kBlk;
+    out[u] = M[r][kBlk + c];
+  }
+}
+
+// z_h = inv_h v_h for every sphere block h (rows 3 NN + 60 h ...)
+__global__ __launch_bounds__(64) void k_block_mv(const double* __restrict__ inv, const double* __restrict__ v,
+                                                 double* __restrict__ z, long row0) {
+  __shared__ double sv[kBlk];
+  const int h = blockIdx.x, t = threadIdx.x;
+  const long base = row0 + (long)kBlk * h;
+  if (t < kBlk) sv[t] = v[base + t];
+  __syncthreads();
+  if (t >= kBlk) return;
+  const double* B = inv + (size_t)kBlk * kBlk * h;
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int c = 0; c < kBlk; c += 2) {
+    acc0 += B[c * kBlk + t] * sv[c];
+    acc1 += B[(c + 1) * kBlk + t] * sv[c + 1];
+  }
+  z[base + t] = acc0 + acc1;
+}
+
+// z = M^-1 v with M = blockdiag(A_bb, A_11, ..., A_num,num): the boundary block through its
+// equilibrated LU, C_b (R_b A_bb C_b)^-1 R_b, the sphere blocks through their inverses
+static void bj_apply(ibem3d_ctx* ctx, const double* v, double* z) {
+  const long nb = 3L * ctx->NN;
+  hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, ctx->d_tb, v, ctx->d_rb, nb);
+  lu_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->d_tb, ctx->st);
+  hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, z, ctx->d_tb, ctx->d_cb, nb);
+  if (ctx->num > 0)
+    hipLaunchKernelGGL(k_block_mv, dim3(ctx->num), dim3(64), 0, ctx->st, ctx->d_Binv, v, z, nb);
+}
+
+// the preconditioner from the assembled A: boundary block copied, equilibrated and factored; every
+// sphere block inverted
+static int bj_setup(ibem3d_ctx* ctx) {
+  const long n = ctx->n, nb = 3L * ctx->NN;
+  if (!ctx->d_Abb) {
+    if (int rc = dalloc(ctx, &ctx->d_Abb, (size_t)nb * nb)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_rb, (size_t)nb)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_cb, (size_t)nb)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_tb, (size_t)nb)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_Binv, (size_t)kBlk * kBlk * std::max(ctx->num, 1))) return rc;
+    if (int rc = alloc_luwork(ctx, ctx->Wb, nb)) return rc;
+  }
+  {
+    Scope s(ctx, "solve.bj_setup");
+    equilibrate_copy_f64(ctx->d_A, nb, n, ctx->d_rb, ctx->d_cb, ctx->d_Abb, nb, ctx->d_eqws, ctx->st);
+    if (lu_factor_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->st, nullptr, nullptr))
+      return fail(ctx, IBEM3D_EHIP, "boundary block LU launch failed");
+    if (lu_prepare_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->st))
+      return fail(ctx, IBEM3D_EHIP, "boundary block solve preparation failed");
+    if (ctx->num > 0)
+      hipLaunchKernelGGL(k_block_inv, dim3(ctx->num), dim3(256), 0, ctx->st, ctx->d_A, n, nb, ctx->d_Binv);
+  }
+  int info = 0;
+  HIPC(hipMemcpyAsync(&info, ctx->Wb.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+  HIPC(hipStreamSynchronize(ctx->st));
+  if (info) return fail(ctx, IBEM3D_ESOLVE, "boundary block LU: zero pivot");
+  return after_launch(ctx, "block-Jacobi setup");
+}
+
 // v <- (LU)^-1 v with the factors of the prepared mixed mode: the FP32 factor, or its 16-bit copy
 // (compact storage)
 static void precond_solve(ibem3d_ctx* ctx, float* v) {
@@ -605,7 +744,7 @@ static int residual(ibem3d_ctx* ctx, double* rel) {
 // applications (LU32 solves). Each cycle costs one FP64 matrix-vector product more than the same
 // number of classic refinement steps, so the solve uses it when classic refinement stalls.
 // warm: start from the current x, whose residual b - A x is in d_res with relative norm rel0.
-static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0) {
+static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0, int max_cycles = 10) {
   const long n = ctx->n;
   if (!ctx->d_gV) {
     if (int rc = dalloc(ctx, &ctx->d_gV, (size_t)(kGmres + 1) * n)) return rc;
@@ -627,7 +766,6 @@ static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0) {
     HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
   }
   double beta = rel * bnorm;
-  const int max_cycles = 10;
   for (int cyc = 0; cyc < max_cycles && rel > tol; ++cyc) {
     // V_0 = r / ||r||
     hipLaunchKernelGGL(k_scale_by, vgrid(n), dim3(256), 0, ctx->st, V, ctx->d_res, 1.0 / beta, n);
@@ -638,7 +776,10 @@ static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0) {
     for (int j = 0; j < kGmres; ++j) {
       double* vj = V + (size_t)j * n;
       double* zj = Z + (size_t)j * n;
-      {
+      if (ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES) {
+        Scope s(ctx, "solve.precond_bj");
+        bj_apply(ctx, vj, zj);
+      } else {
         Scope s(ctx, "solve.trsv_f32");
         hipLaunchKernelGGL(k_scale_to_f32, vgrid(n), dim3(256), 0, ctx->st, ctx->d_v32, vj, ctx->d_r, n);
         precond_solve(ctx, ctx->d_v32);
@@ -882,20 +1023,68 @@ static int solve_mixed(ibem3d_ctx* ctx, int mode, double tol, bool gmres_first, 
   return classic_ir(ctx, tol);
 }
 
+// FP64 GMRES right-preconditioned by block Jacobi (no n x n factor). With an initial guess of the
+// eigenstrain unknowns, the boundary unknowns start from the boundary block's solve against it,
+// x_b = A_bb^-1 (b_b - A_bs x_s). When GMRES does not reach tol the mixed single-FP16 LU solves the
+// system instead (the fallback, compact storage when the copy does not fit).
+constexpr int kBjCycles = 3;  // GMRES cycles (of kGmres iterations) before the LU fallback
+static int solve_bj(ibem3d_ctx* ctx, double tol, bool reuse) {
+  const long n = ctx->n, nb = 3L * ctx->NN;
+  if (!(tol > 0)) tol = 1e-10;
+  ctx->bj_fallback = false;
+  if (!reuse || ctx->prepared_mode != IBEM3D_SOLVE_BJ_GMRES) {
+    ctx->prepared_mode = -1;
+    if (int rc = bj_setup(ctx)) return rc;
+    ctx->prepared_mode = IBEM3D_SOLVE_BJ_GMRES;
+  }
+  int rc;
+  if (ctx->has_x0 && ctx->num > 0) {
+    Scope s(ctx, "solve.warm_start");
+    // x = [0; x_s], r = b - A x, then x_b = A_bb^-1 r_b
+    fill_zero(ctx->d_x, sizeof(double) * nb, ctx->st);
+    HIPC(hipMemcpyAsync(ctx->d_x + nb, ctx->d_x0s, sizeof(double) * (n - nb), hipMemcpyDeviceToDevice, ctx->st));
+    HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
+    gemv_f64(n, n, -1.0, ctx->d_A, n, ctx->d_x, 1.0, ctx->d_res, ctx->d_gemvws, ctx->st);
+    hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, ctx->d_tb, ctx->d_res, ctx->d_rb, nb);
+    lu_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->d_tb, ctx->st);
+    hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_tb, ctx->d_cb, nb);
+    double rel0 = 0.0;
+    if ((rc = residual(ctx, &rel0))) return rc;
+    ctx->rel_res = rel0;
+    ctx->warm_rel = rel0;
+    rc = rel0 <= tol ? 0 : gmres_ir(ctx, tol, true, rel0, kBjCycles);
+  } else {
+    ctx->warm_rel = -1.0;
+    rc = gmres_ir(ctx, tol, false, 1.0, kBjCycles);
+  }
+  if (rc == IBEM3D_ESOLVE) {
+    // the fallback: the mixed LU (it reports its own refinement steps)
+    ctx->bj_fallback = true;
+    const int bj_iters = ctx->iters;
+    ctx->iters = 0;
+    ctx->solved = false;
+    const bool compact = !fits(sizeof(float) * (size_t)((n + 63) / 64 * 64) * n);
+    rc = solve_mixed(ctx, IBEM3D_SOLVE_MIXED_FP16_1, tol, false, false, compact);
+    ctx->iters += bj_iters;
+  }
+  ctx->solved = rc == 0 || ctx->solved;
+  return rc;
+}
+
 int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
   CTX_CHECK();
   const bool gmres_first = (mode & IBEM3D_SOLVE_GMRES) != 0;
   const bool reuse = (mode & IBEM3D_SOLVE_REUSE) != 0;
   bool compact = (mode & IBEM3D_SOLVE_COMPACT) != 0;
   mode &= ~(IBEM3D_SOLVE_GMRES | IBEM3D_SOLVE_REUSE | IBEM3D_SOLVE_COMPACT);
-  if (mode < IBEM3D_SOLVE_FP64_LU || mode > IBEM3D_SOLVE_MIXED_BF16)
+  if (mode < IBEM3D_SOLVE_FP64_LU || mode > IBEM3D_SOLVE_BJ_GMRES)
     return fail(ctx, IBEM3D_EINVAL, "solve: unknown mode");
   const long n = ctx->n;
   if (reuse) {
     if (ctx->prepared_mode < 0 || (ctx->prepared_mode & ~IBEM3D_SOLVE_COMPACT) != mode)
       return fail(ctx, IBEM3D_ESTATE, "solve: no factors of this mode to reuse");
     compact = (ctx->prepared_mode & IBEM3D_SOLVE_COMPACT) != 0;
-  } else if (!compact) {
+  } else if (!compact && mode != IBEM3D_SOLVE_BJ_GMRES) {
     // the second n x n copy (FP64 factor, or FP32 factor with its 64-aligned columns)
     const size_t need = mode == IBEM3D_SOLVE_FP64_LU ? (ctx->d_Af ? 0 : sizeof(double) * (size_t)n * n)
                                                      : (ctx->d_A32 ? 0 : sizeof(float) * (size_t)((n + 63) / 64 * 64) * n);
@@ -908,8 +1097,29 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
   ctx->iters = 0;
   ctx->rel_res = -1.0;
   if (mode == IBEM3D_SOLVE_FP64_LU) return solve_fp64(ctx, reuse, compact);
+  if (mode == IBEM3D_SOLVE_BJ_GMRES) return solve_bj(ctx, tol, reuse);
   return solve_mixed(ctx, mode, tol, gmres_first, reuse, compact);
 }
+
+// Initial guess of the eigenstrain unknowns (60 num, the device's 60 layout) for the next block-
+// Jacobi GMRES solves; NULL clears it.
+int ibem3d_set_initial_guess(ibem3d_ctx* ctx, const double* x_eig) {
+  CTX_CHECK();
+  if (!x_eig) {
+    ctx->has_x0 = false;
+    return 0;
+  }
+  for (long i = 0; i < 60L * ctx->num; ++i)
+    if (!std::isfinite(x_eig[i])) return fail(ctx, IBEM3D_EINVAL, "set_initial_guess: non-finite value");
+  if (!ctx->d_x0s)
+    if (int rc = dalloc(ctx, &ctx->d_x0s, (size_t)std::max(60L * ctx->num, 1L))) return rc;
+  HIPC(hipMemcpyAsync(ctx->d_x0s, x_eig, sizeof(double) * 60L * ctx->num, hipMemcpyHostToDevice, ctx->st));
+  HIPC(hipStreamSynchronize(ctx->st));
+  ctx->has_x0 = true;
+  return 0;
+}
+
+int ibem3d_solve_fell_back(const ibem3d_ctx* ctx) { return ctx && ctx->bj_fallback ? 1 : 0; }
 
 // Relative residual of the current solution against the assembled FP64 system; after a compact
 // FP64 solve (A overwritten by its factor) the system is assembled again first.

@@ -24,6 +24,7 @@ SOLVE_MIXED_FP32 = 1
 SOLVE_MIXED_FP16 = 2     # FP32 LU, trailing updates on three FP16 products of hi/lo splits
 SOLVE_MIXED_FP16_1 = 3   # FP32 LU, trailing updates on one FP16 product (power-of-two scaled operands)
 SOLVE_MIXED_BF16 = 4     # FP32 LU, trailing updates on one BF16 product
+SOLVE_BJ_GMRES = 5       # FP64 GMRES preconditioned by block Jacobi (no n x n factor), LU fallback
 SOLVE_GMRES = 0x100  # flag for the mixed modes: GMRES-IR from the start
 SOLVE_REUSE = 0x200  # flag: reuse the factors of the last factorisation in the same mode
 SOLVE_COMPACT = 0x400  # flag: the factor takes A's place (16-bit factor copy for the mixed modes)
@@ -67,7 +68,7 @@ EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size"
            "ibem3d_set_known_values", "ibem3d_average_eigenstrain", "ibem3d_residual",
            "ibem3d_update_known_values", "ibem3d_batch_create", "ibem3d_batch_destroy",
            "ibem3d_batch_last_error", "ibem3d_batch_stiffness", "ibem3d_set_tuning",
-           "ibem3d_get_tuning"]
+           "ibem3d_get_tuning", "ibem3d_set_initial_guess", "ibem3d_solve_fell_back"]
 
 
 class EngineError(RuntimeError):
@@ -117,6 +118,8 @@ def load(path: str = LIB_PATH):
     L.ibem3d_average_eigenstrain.argtypes = [ctypes.c_void_p, _dp]
     L.ibem3d_residual.argtypes = [ctypes.c_void_p, _dp]
     L.ibem3d_update_known_values.argtypes = [ctypes.c_void_p, _dp, _dp]
+    L.ibem3d_set_initial_guess.argtypes = [ctypes.c_void_p, _dp]
+    L.ibem3d_solve_fell_back.argtypes = [ctypes.c_void_p]
     L.ibem3d_set_tuning.argtypes = [ctypes.c_int, ctypes.c_long]
     L.ibem3d_get_tuning.restype = ctypes.c_long
     L.ibem3d_get_tuning.argtypes = [ctypes.c_int]
@@ -271,6 +274,19 @@ class Engine:
         it = ctypes.c_int()
         self.L.ibem3d_solve_info(self.ctx, ctypes.byref(rel), ctypes.byref(it))
         return rel.value, it.value
+
+    def set_initial_guess(self, x_eig):
+        """Eigenstrain unknowns (60 num, 60 layout) the block-Jacobi GMRES starts from; None clears."""
+        if x_eig is None:
+            self._chk(self.L.ibem3d_set_initial_guess(self.ctx, None), "set_initial_guess")
+            return
+        v = np.ascontiguousarray(x_eig, dtype=np.float64).reshape(-1)
+        if v.size != 60 * self.prob.num:
+            raise ValueError("set_initial_guess: expected 60 num values")
+        self._chk(self.L.ibem3d_set_initial_guess(self.ctx, _d(v)), "set_initial_guess")
+
+    def fell_back(self) -> bool:
+        return bool(self.L.ibem3d_solve_fell_back(self.ctx))
 
     def residual(self) -> float:
         """Relative residual ||b - A x|| / ||b|| of the current solution (after a compact FP64 solve

@@ -70,6 +70,13 @@ enum {
                                    operands scaled per row of L21 / column of U12 by powers of two
                                    (FP32 accumulation) + FP64 refinement */
   IBEM3D_SOLVE_MIXED_BF16 = 4,  /* the same with single BF16 products */
+  IBEM3D_SOLVE_BJ_GMRES = 5,    /* no n x n factor: FP64 GMRES right-preconditioned by block Jacobi
+                                   (the LU of the equilibrated boundary block, 3 NN x 3 NN, and the
+                                   inverse of every sphere's 60 x 60 diagonal block), from the initial
+                                   guess of ibem3d_set_initial_guess when one is set; when GMRES does
+                                   not reach tol in 300 iterations (three restarts of 100), the
+                                   MIXED_FP16_1 LU solves the system instead
+                                   (ibem3d_solve_fell_back reports it) */
   IBEM3D_SOLVE_GMRES = 0x100,   /* flag for the mixed modes: refine by GMRES-IR (FP64 GMRES, right-
                                    preconditioned by the LU factors) from the start. Without it the
                                    refinement is classic and switches to GMRES-IR when an iteration
@@ -165,6 +172,14 @@ int ibem3d_average_fields(ibem3d_ctx* ctx, double* sigma, double* eps, double* v
 
 /* Copy the assembled system (A column-major n x n, b n) to the host; either may be NULL. */
 int ibem3d_get_system(ibem3d_ctx* ctx, double* A, double* b);
+
+/* Initial guess of the eigenstrain unknowns (60 num, the device's 60 layout: 6 uniform, 18 linear,
+ * 36 quadratic coefficients with k <= l, per sphere) for the IBEM3D_SOLVE_BJ_GMRES solves that follow
+ * -- for instance the learned per-sphere prediction; the boundary unknowns start from the boundary
+ * block's solve against it. NULL clears it. (No counterpart in the reference.) */
+int ibem3d_set_initial_guess(ibem3d_ctx* ctx, const double* x_eig);
+/* 1 if the last IBEM3D_SOLVE_BJ_GMRES solve fell back to the mixed LU, else 0. */
+int ibem3d_solve_fell_back(const ibem3d_ctx* ctx);
 
 /* Relative residual ||b - A x|| / ||b|| of the current solution with the FP64 system matrix. After a
  * compact FP64 solve (A overwritten by its factor) the system is assembled again first, which ends

@@ -151,3 +151,25 @@ def test_compact_reuse_new_load(mid):
     e = np.linalg.norm(x_reuse - x_fresh) / np.linalg.norm(x_fresh)
     print("reuse iterations", it, "residual", rel, "vs fresh FP64", e)
     assert e < 1e-8
+
+
+def test_block_jacobi_gmres(mid):
+    """FP64 GMRES preconditioned by block Jacobi (no n x n factor): 1e-10 residual, the LU
+    solution to 1e-8; started from the exact eigenstrains perturbed by 1 %, it needs fewer
+    iterations than from zero, and the same solution."""
+    case(f"block-Jacobi GMRES n60={mid['prob'].n60}")
+    eng = E.Engine(mid["prob"])
+    eng.assemble()
+    rel, it = eng.solve(E.SOLVE_BJ_GMRES, 1e-10)
+    assert not eng.fell_back()
+    X = check_refined(eng, mid, rel, it, "bj-gmres cold")
+    nb = 3 * mid["prob"].NN
+    rng = np.random.default_rng(3)
+    guess = mid["x64"][nb:] * (1 + 0.01 * rng.standard_normal(X.size - nb))
+    eng.set_initial_guess(guess)
+    rel_w, it_w = eng.solve(E.SOLVE_BJ_GMRES | E.SOLVE_REUSE, 1e-10)
+    check_refined(eng, mid, rel_w, it_w, "bj-gmres warm 1%")
+    print("iterations cold", it, "warm", it_w)
+    assert it_w < it
+    eng.set_initial_guess(None)
+    eng.close()

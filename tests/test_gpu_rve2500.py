@@ -41,6 +41,12 @@ def test_rve2500_contrast(contrast):
     print(f"  fp16 single product, compact: residual {rel:.3e} GMRES iterations {it}")
     assert rel <= 1e-10
     averages_ok(eng, prob)
+    _, _, x1 = eng.getUTVector()
+    rel, it = eng.solve(E.SOLVE_BJ_GMRES, 1e-10)
+    print(f"  block-Jacobi GMRES: residual {rel:.3e} iterations {it} fell back {eng.fell_back()}")
+    assert rel <= 1e-10
+    _, _, xb = eng.getUTVector()
+    assert np.linalg.norm(xb - x1) / np.linalg.norm(x1) < 1e-8
     eng.close()
 
 
@@ -55,6 +61,11 @@ def test_rve2500_against_fp64_lu():
     print(f"  fp16 single: residual {rel1:.3e} iterations {it1}")
     assert rel1 <= 1e-10
     averages_ok(eng, prob)
+    relb, itb = eng.solve(E.SOLVE_BJ_GMRES, 1e-10)
+    _, _, xb = eng.getUTVector()
+    sb = eng.getFlux()
+    print(f"  block-Jacobi GMRES: residual {relb:.3e} iterations {itb} fell back {eng.fell_back()}")
+    assert relb <= 1e-10
     eng.reset()
     eng.assemble()
     rel3, it3 = eng.solve(E.SOLVE_MIXED_FP16, 1e-10)
@@ -71,7 +82,7 @@ def test_rve2500_against_fp64_lu():
     rel64 = eng.residual()
     print(f"  fp64 LU (in place): residual {rel64:.3e}")
     assert rel64 <= 1e-10
-    for name, x, s in (("fp16 single", x1, s1), ("fp16 split", x3, s3)):
+    for name, x, s in (("fp16 single", x1, s1), ("fp16 split", x3, s3), ("block-Jacobi GMRES", xb, sb)):
         ex = np.linalg.norm(x - x64) / np.linalg.norm(x64)
         es = np.abs(s - s64).max() / np.abs(s64).max()
         print(f"  {name} vs fp64 LU: x {ex:.3e} stress {es:.3e}")
