@@ -325,6 +325,90 @@ def batch_leg(args, hip):
     print(json.dumps(out), flush=True)
 
 
+def learned_leg(args, hip):
+    """The learned initial guess: a network trained (by its own HIP kernels) on the eigenstrains of
+    seeded batched RVEs (8-256 spheres, six unit loads each, from the batched solver), then a
+    held-out RVE of the 2,500-sphere geometry (contrast 10, traction load, seed != the training
+    seeds) solved by block-Jacobi GMRES from zero and from the network's guess (BF16 inference), the
+    mixed LU beside them; every solution is compared with the LU's."""
+    from ibemproject_amd import learned as Lr
+    from ibemproject_amd.generate import make_batch
+    t0 = time.perf_counter()
+    probs = [p for p, _ in make_batch(args.seed + 1000, args.train_rves, workers=min(16, os.cpu_count() or 1))]
+    kw = dict(refine_steps=3, wave_max=32, mem_budget_gb=200.0, use_graph=False, host_threads=8)
+    edge, node, tgt, r = Lr.training_set(probs, **kw)
+    t_data = time.perf_counter() - t0
+    progress(f"training set: {len(probs)} RVEs, {len(tgt)} spheres in {t_data:.1f} s")
+    g = Lr.Gnn(seed=3)
+    g.set_data(edge, node, tgt)
+    t1 = time.perf_counter()
+    hist = g.train(epochs=args.train_epochs, batch=256, lr=2e-3)
+    hip.hipDeviceSynchronize()
+    t_train = time.perf_counter() - t1
+    progress(f"trained {args.train_epochs} epochs in {t_train:.1f} s, loss {hist[0]:.4g} -> {hist[-1]:.4g}")
+    prob = make_workload(args.workload if args.workload != "learned" else DEFAULT_WORKLOAD, seed=args.seed + 7)
+    eng = Engine(prob)
+    n = eng.n
+    run_step(eng, SOLVE_BJ_GMRES)  # loads every kernel, allocates the workspaces
+    (rel_c, it_c, s_c), t_cold, tm_c = timed(eng, hip, lambda: run_step(eng, SOLVE_BJ_GMRES))
+    _, _, x_c = eng.getUTVector()
+    t2 = time.perf_counter()
+    _, e1, n1 = Lr.graph(prob)
+    y = g.predict(e1, n1, bf16=True)
+    eps = Lr.macro_strain_of_uniaxial_traction(prob.E0, prob.nu0)
+    xg = Lr.guess(y, prob.radii, eps)
+    t_inf = time.perf_counter() - t2
+    xd = Lr.guess(np.tile(Lr.dilute_response(prob.E0, prob.nu0, prob.E1, prob.nu1), (prob.num, 1)),
+                  prob.radii, eps)
+
+    def warm(x0):
+        eng.set_initial_guess(x0)
+        return run_step(eng, SOLVE_BJ_GMRES)
+    (rel_w, it_w, s_w), t_warm, _ = timed(eng, hip, lambda: warm(xg))
+    r0, alpha = eng.warm_start_info()
+    fb = eng.fell_back()
+    _, _, x_w = eng.getUTVector()
+    (_, it_d, _), _, _ = timed(eng, hip, lambda: warm(xd))
+    r0_d, alpha_d = eng.warm_start_info()
+    # the accuracy a warm start needs on this operator: the LU's own eigenstrains with relative noise
+    nb = 3 * prob.NN
+    rng = np.random.default_rng(5)
+    noisy = {}
+    for lvl in (1e-3, 1e-2):
+        _, itn, _ = warm(x_c[nb:] * (1 + lvl * rng.standard_normal(n - nb)))
+        noisy[str(lvl)] = {"iterations": itn, "start_rel_residual": eng.warm_start_info()[0]}
+    eng.set_initial_guess(None)
+    (rel_l, it_l, s_l), t_lu, _ = timed(eng, hip, lambda: run_step(eng, SOLVE_MIXED_FP16_1))
+    _, _, x_l = eng.getUTVector()
+    eng.close()
+
+    def by_order(x):
+        o = np.tile(Lr.ORDER, prob.num)
+        return [float(np.linalg.norm(x[o == k] - x_l[nb:][o == k]) / np.linalg.norm(x_l[nb:][o == k]))
+                for k in (0, 1, 2)]
+    out = {
+        "metric": "learned initial guess: block-Jacobi GMRES iterations and time to 1e-10, with and "
+                  "without the network, beside the mixed LU",
+        "value": t_warm + t_inf, "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "config": {"workload": prob.title, "unknowns": n, "spheres": prob.num, "contrast": prob.E1 / prob.E0,
+                   "train_rves": len(probs), "train_spheres": int(len(tgt)), "epochs": args.train_epochs},
+        "training": {"data_s": t_data, "train_s": t_train, "loss_first": float(hist[0]),
+                     "loss_last": float(hist[-1]), "max_rel_residual": float(r["rel"].max())},
+        "inference_s": t_inf,
+        "guess_err_vs_lu_by_order": {"network": by_order(xg), "dilute": by_order(xd)},
+        "warm_start": {"alpha": alpha, "start_rel_residual": r0, "dilute_alpha": alpha_d,
+                       "dilute_start_rel_residual": r0_d, "dilute_iterations": it_d},
+        "lu_with_noise": noisy,
+        "cold": {"step_s": t_cold, "iterations": it_c, "rel_residual": rel_c,
+                 "x_relerr_vs_lu": float(np.linalg.norm(x_c - x_l) / np.linalg.norm(x_l))},
+        "warm": {"step_s": t_warm, "iterations": it_w, "rel_residual": rel_w, "fell_back": fb,
+                 "x_relerr_vs_lu": float(np.linalg.norm(x_w - x_l) / np.linalg.norm(x_l)),
+                 "stress_relerr_vs_lu": float(np.abs(s_w - s_l).max() / np.abs(s_l).max())},
+        "mixed_lu": {"step_s": t_lu, "iterations": it_l, "rel_residual": rel_l},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -341,6 +425,10 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration) and exit")
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--learned", action="store_true",
+                    help="train the eigenstrain network and compare its warm start (then exit)")
+    ap.add_argument("--train-rves", type=int, default=2000)
+    ap.add_argument("--train-epochs", type=int, default=200)
     ap.add_argument("--batch-budget-gb", type=float, default=240.0)
     args = ap.parse_args()
     if args.calibrate_copy:
@@ -352,7 +440,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import ctypes
-    if args.workload.startswith("batch"):
+    if args.workload.startswith("batch") or args.learned:
         # the batch's latency-bound small solves overlap across streams: give the process 16
         # hardware queues (HIP's default is 4) before the runtime starts
         os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
@@ -362,6 +450,9 @@ def main():
         return
     if args.workload.startswith("batch"):
         batch_leg(args, hip)
+        return
+    if args.learned:
+        learned_leg(args, hip)
         return
 
     prob = make_workload(args.workload, seed=args.seed)

@@ -15,7 +15,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libibem3d.so")
-SOURCES = ["assembly.hip", "solver.hip", "post.hip", "api.cpp", "batch.cpp"]
+SOURCES = ["assembly.hip", "solver.hip", "post.hip", "api.cpp", "batch.cpp", "learned.hip"]
 HOST_SOURCES = ["io.cpp"]           # plain C++ (no HIP), compiled with the host compiler
 DRIVER = os.path.join(HERE, "ibem3d_run")
 ARCH = os.environ.get("IBEM3D_ARCH", "gfx950")

@@ -69,6 +69,7 @@ struct ibem3d_ctx {
   bool has_x0 = false;
   bool bj_fallback = false;    // the last block-Jacobi solve fell back to the mixed LU
   double warm_rel = -1.0;      // relative residual of the last warm start (-1: none)
+  double warm_alpha = 0.0;     // the scale the last warm start applied to the guess
   LuWork W;
   unsigned phases = 0;
   bool solved = false;
@@ -550,6 +551,11 @@ __global__ void k_scale_from_f32(double* y, const float* x, const double* s, lon
 __global__ void k_scale_by(double* y, const double* x, double a, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = a * x[i];
+}
+// y <- a x + c y
+__global__ void k_axpby(double* y, const double* x, double a, double c, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + c * y[i];
 }
 // part[i * kDotBlocks + b] = block b's share of V_i . w, i < gridDim.y
 __global__ __launch_bounds__(256) void k_dot_part(const double* V, long ldv, const double* w, long n,
@@ -1050,8 +1056,22 @@ static int solve_bj(ibem3d_ctx* ctx, double tol, bool reuse) {
     hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_tb, ctx->d_cb, nb);
     double rel0 = 0.0;
     if ((rc = residual(ctx, &rel0))) return rc;
+    // x0 <- alpha x0 with alpha minimising ||b - alpha A x0|| (A x0 = b - r0): the start is never
+    // worse than x = 0, whatever the quality of the guess
+    double bb = 0, br = 0, rr = 0;
+    if ((rc = dots(ctx, ctx->d_b, 1, ctx->d_b, &bb)) || (rc = dots(ctx, ctx->d_b, 1, ctx->d_res, &br)) ||
+        (rc = dots(ctx, ctx->d_res, 1, ctx->d_res, &rr)))
+      return rc;
+    const double den = bb - 2 * br + rr;
+    const double alpha = den > 0 ? std::min(1.0, std::max(0.0, (bb - br) / den)) : 0.0;
+    hipLaunchKernelGGL(k_scale_by, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_x, alpha, n);
+    hipLaunchKernelGGL(k_axpby, vgrid(n), dim3(256), 0, ctx->st, ctx->d_res, ctx->d_b, 1.0 - alpha, alpha, n);
+    rel0 = bb > 0 ? std::sqrt(std::max(0.0, (1 - alpha) * (1 - alpha) * bb + 2 * alpha * (1 - alpha) * br +
+                                              alpha * alpha * rr) / bb)
+                  : 0.0;
     ctx->rel_res = rel0;
     ctx->warm_rel = rel0;
+    ctx->warm_alpha = alpha;
     rc = rel0 <= tol ? 0 : gmres_ir(ctx, tol, true, rel0, kBjCycles);
   } else {
     ctx->warm_rel = -1.0;
@@ -1120,6 +1140,13 @@ int ibem3d_set_initial_guess(ibem3d_ctx* ctx, const double* x_eig) {
 }
 
 int ibem3d_solve_fell_back(const ibem3d_ctx* ctx) { return ctx && ctx->bj_fallback ? 1 : 0; }
+
+int ibem3d_warm_start_info(const ibem3d_ctx* ctx, double* rel0, double* alpha) {
+  if (!ctx) return IBEM3D_EINVAL;
+  if (rel0) *rel0 = ctx->warm_rel;
+  if (alpha) *alpha = ctx->warm_alpha;
+  return 0;
+}
 
 // Relative residual of the current solution against the assembled FP64 system; after a compact
 // FP64 solve (A overwritten by its factor) the system is assembled again first.

@@ -132,7 +132,7 @@ ibem3d_batch* ibem3d_batch_create(const ibem3d_problem* probs, int count, int de
 void ibem3d_batch_destroy(ibem3d_batch* b) { delete b; }
 
 int ibem3d_batch_stiffness(ibem3d_batch* b, const ibem3d_batch_options* opt, double tol, double* C,
-                           double* C_bnd, double* rel, int* host_refined, double* times) {
+                           double* C_bnd, double* rel, int* host_refined, double* times, double* eig) {
   if (!b || !opt || !C) {
     g_batch_error = "batch_stiffness: null argument";
     return IBEM3D_EINVAL;
@@ -159,6 +159,8 @@ int ibem3d_batch_stiffness(ibem3d_batch* b, const ibem3d_batch_options* opt, dou
   }
   int rc = 0;
   const size_t nblk = ibem_batch::dot_blocks();
+  std::vector<long> eig_off(count + 1, 0);  // each RVE's eigenstrain block: 6 loads x 60 num
+  for (int k = 0; k < count; ++k) eig_off[k + 1] = eig_off[k] + 6L * 60 * b->rves[k].num;
   for (int k0 = 0; k0 < count && rc == 0;) {
     // the wave: consecutive RVEs while they fit the budget (at least one)
     int k1 = k0;
@@ -315,6 +317,8 @@ int ibem3d_batch_stiffness(ibem3d_batch* b, const ibem3d_batch_options* opt, dou
         }
         if (rel) rel[6L * k + t] = rl[t];
         if (host_refined) host_refined[6L * k + t] = refined;
+        if (eig)
+          std::memcpy(eig + eig_off[k] + 60L * r.num * t, x + 3L * r.NN, sizeof(double) * 60 * r.num);
       }
     }
     t_host += now() - th;

@@ -68,7 +68,11 @@ EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size"
            "ibem3d_set_known_values", "ibem3d_average_eigenstrain", "ibem3d_residual",
            "ibem3d_update_known_values", "ibem3d_batch_create", "ibem3d_batch_destroy",
            "ibem3d_batch_last_error", "ibem3d_batch_stiffness", "ibem3d_set_tuning",
-           "ibem3d_get_tuning", "ibem3d_set_initial_guess", "ibem3d_solve_fell_back"]
+           "ibem3d_get_tuning", "ibem3d_set_initial_guess", "ibem3d_solve_fell_back",
+           "ibem3d_warm_start_info",
+           "ibem3d_gnn_last_error", "ibem3d_gnn_param_count", "ibem3d_gnn_graph", "ibem3d_gnn_create",
+           "ibem3d_gnn_destroy", "ibem3d_gnn_get_params", "ibem3d_gnn_set_params", "ibem3d_gnn_set_data",
+           "ibem3d_gnn_loss_grad", "ibem3d_gnn_train", "ibem3d_gnn_predict"]
 
 
 class EngineError(RuntimeError):
@@ -120,6 +124,7 @@ def load(path: str = LIB_PATH):
     L.ibem3d_update_known_values.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.ibem3d_set_initial_guess.argtypes = [ctypes.c_void_p, _dp]
     L.ibem3d_solve_fell_back.argtypes = [ctypes.c_void_p]
+    L.ibem3d_warm_start_info.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.ibem3d_set_tuning.argtypes = [ctypes.c_int, ctypes.c_long]
     L.ibem3d_get_tuning.restype = ctypes.c_long
     L.ibem3d_get_tuning.argtypes = [ctypes.c_int]
@@ -129,7 +134,23 @@ def load(path: str = LIB_PATH):
     L.ibem3d_batch_last_error.restype = ctypes.c_char_p
     L.ibem3d_batch_last_error.argtypes = []
     L.ibem3d_batch_stiffness.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchOptions), ctypes.c_double,
-                                         _dp, _dp, _dp, _ip, _dp]
+                                         _dp, _dp, _dp, _ip, _dp, _dp]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    L.ibem3d_gnn_last_error.restype = ctypes.c_char_p
+    L.ibem3d_gnn_last_error.argtypes = []
+    L.ibem3d_gnn_param_count.restype = ctypes.c_long
+    L.ibem3d_gnn_param_count.argtypes = []
+    L.ibem3d_gnn_graph.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   _ip, _fp, _fp]
+    L.ibem3d_gnn_create.restype = ctypes.c_void_p
+    L.ibem3d_gnn_create.argtypes = [ctypes.c_int, ctypes.c_ulonglong]
+    L.ibem3d_gnn_destroy.argtypes = [ctypes.c_void_p]
+    L.ibem3d_gnn_get_params.argtypes = [ctypes.c_void_p, _fp]
+    L.ibem3d_gnn_set_params.argtypes = [ctypes.c_void_p, _fp]
+    L.ibem3d_gnn_set_data.argtypes = [ctypes.c_void_p, _fp, _fp, _fp, ctypes.c_long]
+    L.ibem3d_gnn_loss_grad.argtypes = [ctypes.c_void_p, _ip, ctypes.c_long, _fp, _fp]
+    L.ibem3d_gnn_train.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int, ctypes.c_long, ctypes.c_float, _fp]
+    L.ibem3d_gnn_predict.argtypes = [ctypes.c_void_p, _fp, _fp, ctypes.c_long, ctypes.c_int, _fp]
     _LIB = L
     return L
 
@@ -187,11 +208,13 @@ def _c_problem(prob: Problem):
 
 def batch_stiffness(problems, mode: int = SOLVE_MIXED_FP16, refine_steps: int = 4, wave_max: int = 16,
                     mem_budget_gb: float = 200.0, use_graph: bool = True, tol: float = 1e-10,
-                    device: int = 0, host_threads: int = 1):
+                    device: int = 0, host_threads: int = 1, eigenstrains: bool = False):
     """Apparent stiffness of many RVEs (ibem3d_batch_stiffness): problems whose boundary
     displacements are all prescribed; six unit macro strains per RVE on one factorisation. Returns
     dict(C (count, 6, 6) from the inclusions, C_bnd (count, 6, 6) from the tractions, rel (count, 6),
-    host_refined (count, 6), times {setup, capture, instantiate, run, host, waves})."""
+    host_refined (count, 6), times {setup, capture, instantiate, run, host, waves}; with eigenstrains:
+    eig, a list of (num, 60, 6) arrays, each sphere's eigenstrain unknowns under the six unit
+    macro strains)."""
     L = load()
     count = len(problems)
     keeps, cps = [], (_CProblem * count)()
@@ -207,16 +230,25 @@ def batch_stiffness(problems, mode: int = SOLVE_MIXED_FP16, refine_steps: int = 
     rel = np.zeros((count, 6))
     hr = np.zeros((count, 6), dtype=np.int32)
     tm = np.zeros(6)
+    eig = np.zeros(sum(360 * p.num for p in problems)) if eigenstrains else None
     opt = BatchOptions(mode, refine_steps, wave_max, mem_budget_gb, 1 if use_graph else 0, host_threads)
     try:
         rc = L.ibem3d_batch_stiffness(b, ctypes.byref(opt), tol, _d(C), _d(Cb), _d(rel),
-                                      hr.ctypes.data_as(_ip), _d(tm))
+                                      hr.ctypes.data_as(_ip), _d(tm), _d(eig) if eig is not None else None)
         if rc != 0:
             raise EngineError(f"batch_stiffness: {L.ibem3d_batch_last_error().decode()} (code {rc})")
     finally:
         L.ibem3d_batch_destroy(b)
     names = ("setup", "capture", "instantiate", "run", "host", "waves")
-    return dict(C=C, C_bnd=Cb, rel=rel, host_refined=hr, times=dict(zip(names, tm.tolist())))
+    out = dict(C=C, C_bnd=Cb, rel=rel, host_refined=hr, times=dict(zip(names, tm.tolist())))
+    if eig is not None:
+        blocks, off = [], 0
+        for p in problems:
+            k = 360 * p.num
+            blocks.append(eig[off:off + k].reshape(6, p.num, 60).transpose(1, 2, 0).copy())
+            off += k
+        out["eig"] = blocks
+    return out
 
 
 class Engine:
@@ -284,6 +316,12 @@ class Engine:
         if v.size != 60 * self.prob.num:
             raise ValueError("set_initial_guess: expected 60 num values")
         self._chk(self.L.ibem3d_set_initial_guess(self.ctx, _d(v)), "set_initial_guess")
+
+    def warm_start_info(self):
+        """(relative residual of the scaled guess, its scale alpha); rel = -1 when none was used."""
+        r, a = ctypes.c_double(), ctypes.c_double()
+        self.L.ibem3d_warm_start_info(self.ctx, ctypes.byref(r), ctypes.byref(a))
+        return r.value, a.value
 
     def fell_back(self) -> bool:
         return bool(self.L.ibem3d_solve_fell_back(self.ctx))

@@ -180,6 +180,10 @@ int ibem3d_get_system(ibem3d_ctx* ctx, double* A, double* b);
 int ibem3d_set_initial_guess(ibem3d_ctx* ctx, const double* x_eig);
 /* 1 if the last IBEM3D_SOLVE_BJ_GMRES solve fell back to the mixed LU, else 0. */
 int ibem3d_solve_fell_back(const ibem3d_ctx* ctx);
+/* The last block-Jacobi solve's start from the initial guess: relative residual of the scaled guess
+ * alpha x0 (alpha in [0, 1] minimises ||b - alpha A x0||, so the start is never worse than zero);
+ * rel0 = -1 when no guess was used. */
+int ibem3d_warm_start_info(const ibem3d_ctx* ctx, double* rel0, double* alpha);
 
 /* Relative residual ||b - A x|| / ||b|| of the current solution with the FP64 system matrix. After a
  * compact FP64 solve (A overwritten by its factor) the system is assembled again first, which ends
@@ -218,9 +222,37 @@ const char* ibem3d_batch_last_error(void);
  * NULL): each load's final relative FP64 residual; host_refined (count x 6, may be NULL): 1 where the
  * in-graph refinement had not reached tol and the load was refined further on the host; times (6,
  * may be NULL): seconds of setup, capture, instantiation, device run, host post-processing, and the
- * number of waves. */
+ * number of waves; eig (may be NULL): every RVE's eigenstrain solution, RVE after RVE, 6 loads x
+ * 60 num values each (the device's 60 layout per sphere). */
 int ibem3d_batch_stiffness(ibem3d_batch* b, const ibem3d_batch_options* opt, double tol, double* C,
-                           double* C_bnd, double* rel, int* host_refined, double* times);
+                           double* C_bnd, double* rel, int* host_refined, double* times, double* eig);
+
+/* ---- learned per-sphere eigenstrain predictor (no counterpart in the reference): a message-passing
+ * MLP over each sphere's 16 nearest neighbours whose 360 outputs are the sphere's 60 eigenstrain
+ * coefficients (coefficient c of order o times r^o) under each of the six unit macro strains; trained
+ * and evaluated by its own HIP kernels (FP32 training, BF16-MFMA or FP32 inference). The prediction
+ * contracted with the macro strain is an initial guess for IBEM3D_SOLVE_BJ_GMRES. ---- */
+typedef struct ibem3d_gnn ibem3d_gnn;
+const char* ibem3d_gnn_last_error(void);
+long ibem3d_gnn_param_count(void);
+/* Graph of one RVE on the device: nbr (num x 16; the 16 nearest other spheres by exact integer
+ * distance of centres quantised to 2^-28, ties to the smaller index, -1 padding), edge features
+ * (num x 16 x 8 floats) and node features (num x 4 floats). */
+int ibem3d_gnn_graph(const double* centres, const double* radii, int num, double E1_over_E0, double nu1,
+                     int device, int* nbr, float* edge, float* node);
+ibem3d_gnn* ibem3d_gnn_create(int device, unsigned long long seed);
+void ibem3d_gnn_destroy(ibem3d_gnn* g);
+int ibem3d_gnn_get_params(ibem3d_gnn* g, float* p);
+int ibem3d_gnn_set_params(ibem3d_gnn* g, const float* p);
+/* Training set on the device: S spheres' edges (S x 16 x 8), nodes (S x 4), targets (S x 360). */
+int ibem3d_gnn_set_data(ibem3d_gnn* g, const float* edge, const float* node, const float* target, long S);
+/* Mean squared error and its gradient (param_count floats, may be NULL) over dataset rows idx[0..B). */
+int ibem3d_gnn_loss_grad(ibem3d_gnn* g, const int* idx, long B, float* loss, float* grad);
+/* Adam (lr) over epochs passes, each a permutation of the S rows (perms: epochs x S), minibatches of
+ * B rows; loss_hist (epochs, may be NULL): mean minibatch loss per epoch. */
+int ibem3d_gnn_train(ibem3d_gnn* g, const int* perms, int epochs, long B, float lr, float* loss_hist);
+/* y (S x 360) for S spheres; bf16 = 1: dense layers on BF16 MFMA with FP32 accumulation. */
+int ibem3d_gnn_predict(ibem3d_gnn* g, const float* edge, const float* node, long S, int bf16, float* y);
 
 /* ---- the reference's text formats (host only; no GPU call) ---- */
 typedef struct ibem3d_inputs ibem3d_inputs;
