@@ -169,6 +169,20 @@ def test_rve_solution_and_stress(oracle_lib, name):
         assert np.abs(s - ref["stress"]).max() <= 1e-10 * np.abs(ref["stress"]).max()
 
 
+@pytest.mark.skipif(not os.environ.get("IBEM3D_SLOW_TESTS"),
+                    reason="about 150 s on one core; set IBEM3D_SLOW_TESTS=1 to run")
+def test_rve64_workload_solution(oracle_lib):
+    """The 64-sphere, 8x8-per-face workload (n = 4998, 1000 stress points): the oracle against the
+    reference's end-to-end solution (measured: U 1.7e-11, T 8.9e-11, stresses 1.3e-11)."""
+    d = os.path.join(GOLD, "rve64_m8")
+    prob = read_problem(d)
+    ref = np.load(os.path.join(d, "reference.npz"))
+    res = oracle_lib.solve_problem(prob, layout=60)
+    for k in ("U", "T"):
+        assert np.linalg.norm(res[k] - ref[k]) / np.linalg.norm(ref[k]) < 1e-10, k
+    assert np.abs(res["stress"] - ref["stress"]).max() <= 1e-10 * np.abs(ref["stress"]).max()
+
+
 def test_reduced_system_is_exact(oracle_lib):
     """The reference's 78-unknown solution satisfies the 60-unknown system (E_ij,kl = E_ij,lk)."""
     d = os.path.join(GOLD, "tiny_rve")
