@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -360,15 +361,58 @@ void dense_fwd(hipStream_t st, const float* X, long M, int K, int ldx, const flo
   }
 }
 
-// forward over B spheres whose edges / nodes are at e, f; output in G->y
-void forward(ibem3d_gnn* G, const float* e, const float* f, long B, bool bf16) {
-  const float* P = G->p;
+// forward over B spheres whose edges / nodes are at e, f, with the activations in z1 (B x 16 x kH),
+// in2 (B x kIn2), z2, z3 (B x kH) and the output in y (B x kOut); every activation element is written
+// before it is read
+void forward_into(hipStream_t st, const float* P, const float* e, const float* f, long B, bool bf16, float* z1,
+                  float* in2, float* z2, float* z3, float* y) {
   const long E = B * kNN;
-  dense_fwd(G->st, e, E, kFe, kFeS, P + kOffW1, P + kOffB1, kH, G->z1, true, bf16);
-  hipLaunchKernelGGL(k_aggregate, dim3((unsigned)B), dim3(128), 0, G->st, G->z1, e, f, B, G->in2);
-  dense_fwd(G->st, G->in2, B, kIn2, kIn2, P + kOffW2, P + kOffB2, kH, G->z2, true, bf16);
-  dense_fwd(G->st, G->z2, B, kH, kH, P + kOffW3, P + kOffB3, kH, G->z3, true, bf16);
-  dense_fwd(G->st, G->z3, B, kH, kH, P + kOffW4, P + kOffB4, kOut, G->y, false, bf16);
+  dense_fwd(st, e, E, kFe, kFeS, P + kOffW1, P + kOffB1, kH, z1, true, bf16);
+  hipLaunchKernelGGL(k_aggregate, dim3((unsigned)B), dim3(128), 0, st, z1, e, f, B, in2);
+  dense_fwd(st, in2, B, kIn2, kIn2, P + kOffW2, P + kOffB2, kH, z2, true, bf16);
+  dense_fwd(st, z2, B, kH, kH, P + kOffW3, P + kOffB3, kH, z3, true, bf16);
+  dense_fwd(st, z3, B, kH, kH, P + kOffW4, P + kOffB4, kOut, y, false, bf16);
+}
+
+void forward(ibem3d_gnn* G, const float* e, const float* f, long B, bool bf16) {
+  forward_into(G->st, G->p, e, f, B, bf16, G->z1, G->in2, G->z2, G->z3, G->y);
+}
+
+// ---- the prediction that allocates nothing: its activations live in a file-scope device array ----
+// 16 MiB per device, held until the process ends; every call on a device shares it (calls that overlap
+// on one device race). Spheres pass through it in chunks of kScratchSpheres.
+constexpr size_t kScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr long kActAlign = 64;  // floats: every activation region starts on a 256-byte boundary
+constexpr long align_up(long v) { return (v + kActAlign - 1) / kActAlign * kActAlign; }
+constexpr long kScratchSpheres = 1664;
+constexpr long kScrZ1 = 0, kScrIn2 = kScrZ1 + align_up(kScratchSpheres * kNN * kH),
+               kScrZ2 = kScrIn2 + align_up(kScratchSpheres * kIn2), kScrZ3 = kScrZ2 + align_up(kScratchSpheres * kH),
+               kScrEnd = kScrZ3 + align_up(kScratchSpheres * kH);
+static_assert(kScrEnd * (long)sizeof(float) <= (long)kScratchBytes, "activations exceed the scratch array");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch[kMaxDevices] = {};  // the array's device address on each device (nullptr: not looked up)
+
+// with `dev` current: the device address of `scratch` on dev, looked up once per device
+int scratch_on(int dev, float** out) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    g_gnn_error = "scratch: device index out of range";
+    return IBEM3D_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      g_gnn_error = std::string("scratch: hipGetSymbolAddress: ") + hipGetErrorString(e);
+      return IBEM3D_EHIP;
+    }
+    g_scratch[dev] = static_cast<float*>(ptr);
+  }
+  *out = g_scratch[dev];
+  return 0;
 }
 
 void weight_grad(ibem3d_gnn* G, const float* X, int ldx, const float* dY, long M, int K, int N, float* gw, float* gb) {
@@ -497,6 +541,11 @@ ibem3d_gnn* ibem3d_gnn_create(int device, unsigned long long seed) {
   if (hipMemcpy(G->am, z.data(), sizeof(float) * kParams, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(G->av, z.data(), sizeof(float) * kParams, hipMemcpyHostToDevice) != hipSuccess) {
     g_gnn_error = "gnn_create: device buffers";
+    ibem3d_gnn_destroy(G);
+    return nullptr;
+  }
+  float* scr = nullptr;  // the no-allocation prediction's array, looked up here with the device current
+  if (scratch_on(device, &scr)) {
     ibem3d_gnn_destroy(G);
     return nullptr;
   }
@@ -670,6 +719,27 @@ int ibem3d_gnn_predict(ibem3d_gnn* G, const float* edge, const float* node, long
     GCHK(hipMemcpyAsync(y + s0 * kOut, G->y, sizeof(float) * c * kOut, hipMemcpyDeviceToHost, G->st));
   }
   GCHK(hipStreamSynchronize(G->st));
+  return 0;
+}
+
+// ibem3d_gnn_predict on device buffers with no allocation: the activations in the device's scratch
+// array, the launches enqueued on `stream` (NULL: g's stream) and not waited for
+int ibem3d_gnn_predict_noalloc(ibem3d_gnn* G, const float* d_edge, const float* d_node, long S, int bf16,
+                               float* d_y, void* stream) {
+  if (!G || !d_edge || !d_node || !d_y || S <= 0) {
+    g_gnn_error = "gnn_predict_noalloc: invalid arguments";
+    return IBEM3D_EINVAL;
+  }
+  GCHK(hipSetDevice(G->dev));
+  float* scr = nullptr;
+  if (int rc = scratch_on(G->dev, &scr)) return rc;
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : G->st;
+  for (long s0 = 0; s0 < S; s0 += kScratchSpheres) {
+    const long c = std::min(kScratchSpheres, S - s0);
+    forward_into(st, G->p, d_edge + s0 * kNN * kFeS, d_node + s0 * kFnS, c, bf16 != 0, scr + kScrZ1,
+                 scr + kScrIn2, scr + kScrZ2, scr + kScrZ3, d_y + s0 * kOut);
+  }
+  GCHK(hipGetLastError());
   return 0;
 }
 

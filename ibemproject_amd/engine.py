@@ -72,7 +72,7 @@ EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size"
            "ibem3d_warm_start_info",
            "ibem3d_gnn_last_error", "ibem3d_gnn_param_count", "ibem3d_gnn_graph", "ibem3d_gnn_create",
            "ibem3d_gnn_destroy", "ibem3d_gnn_get_params", "ibem3d_gnn_set_params", "ibem3d_gnn_set_data",
-           "ibem3d_gnn_loss_grad", "ibem3d_gnn_train", "ibem3d_gnn_predict"]
+           "ibem3d_gnn_loss_grad", "ibem3d_gnn_train", "ibem3d_gnn_predict", "ibem3d_gnn_predict_noalloc"]
 
 
 class EngineError(RuntimeError):
@@ -151,6 +151,8 @@ def load(path: str = LIB_PATH):
     L.ibem3d_gnn_loss_grad.argtypes = [ctypes.c_void_p, _ip, ctypes.c_long, _fp, _fp]
     L.ibem3d_gnn_train.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int, ctypes.c_long, ctypes.c_float, _fp]
     L.ibem3d_gnn_predict.argtypes = [ctypes.c_void_p, _fp, _fp, ctypes.c_long, ctypes.c_int, _fp]
+    L.ibem3d_gnn_predict_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
+                                             ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     _LIB = L
     return L
 

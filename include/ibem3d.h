@@ -253,6 +253,16 @@ int ibem3d_gnn_loss_grad(ibem3d_gnn* g, const int* idx, long B, float* loss, flo
 int ibem3d_gnn_train(ibem3d_gnn* g, const int* perms, int epochs, long B, float lr, float* loss_hist);
 /* y (S x 360) for S spheres; bf16 = 1: dense layers on BF16 MFMA with FP32 accumulation. */
 int ibem3d_gnn_predict(ibem3d_gnn* g, const float* edge, const float* node, long S, int bf16, float* y);
+/* No counterpart in the reference: ibem3d_gnn_predict for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops). d_edge (S x 16 x 8), d_node (S x 4) and d_y
+ * (S x 360) are device pointers on g's device; the call allocates nothing: the activations live in a
+ * 16 MiB file-scope device array (one per device, held until the process ends) that the spheres pass
+ * through in chunks of 1664, and the kernels are enqueued on `stream` (a hipStream_t; NULL: g's own
+ * stream) without waiting. Every call on a device shares that array, so calls that overlap on one
+ * device race; no part of it needs clearing (every element is written before it is read). Results
+ * equal ibem3d_gnn_predict's bit for bit. */
+int ibem3d_gnn_predict_noalloc(ibem3d_gnn* g, const float* d_edge, const float* d_node, long S, int bf16,
+                               float* d_y, void* stream);
 
 /* ---- the reference's text formats (host only; no GPU call) ---- */
 typedef struct ibem3d_inputs ibem3d_inputs;
