@@ -18,8 +18,10 @@
 //   G0 = -pi/15 rho^2, G1 = pi a^2/30 rho^2 - pi/210 rho^3, G2a = -G1,
 //   G2b = pi a^2/420 rho^3 - pi/3024 rho^4,
 // exact for first and higher derivatives of Phi and third and higher of Psi, which is all the path
-// uses. (Derived from the reference's V_n / Phi_.1 / Psi_.3 formulas, :169-543, with I_n constant
-// inside; the derivation is ibemproject_amd/derive_potentials.py.)
+// uses. (The same functions as the reference's V_n / Phi_.1 / Psi_.3 formulas, :169-543, with I_n
+// constant inside. Derivation: both kernels expanded in Legendre polynomials of the angle between x
+// and y, the angles integrated in closed form, the radius shell by shell split at |y| = |x|;
+// tests/test_potentials.py carries it out in sympy and checks pot_coefs against it.)
 //
 // Radial derivatives. d^beta rho^s (beta a multi-index, k = |beta|) is
 //   sum_j f^(k-j)(rho) 2^(k-j) M_j(beta, x),  f = rho^s,

@@ -162,3 +162,13 @@ void hc_incl_block_w(double E0, double nu0, double E1, double nu1, const double*
   }
 }
 }
+
+extern "C" {
+// The eight radial functions' coefficients (pot_coefs) for radius a, inside or outside: C[8][4].
+void hc_pot_coefs(double a, int inside, double* out) {
+  double C[8][4];
+  pot_coefs(a, inside != 0, C);
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 4; ++j) out[4 * i + j] = C[i][j];
+}
+}
