@@ -20,11 +20,10 @@ def case(msg):
 
 @pytest.mark.parametrize("mode", [SOLVE_FP64_LU, SOLVE_MIXED_FP32, SOLVE_MIXED_FP16])
 def test_reference_example(mode):
-    """The fixture's system is nearly singular (condition number ~3e17: the traction unknowns at the
-    corners of the clamped face z = 0 are each touched by one element only). Any elimination other
-    than the reference's own complete pivoting moves the solution along those near-null directions by
-    ~1e-8 of its norm, so U and T are held to 1e-6 and the residual and the stresses -- which those
-    directions do not reach -- to 1e-10 and 1e-8."""
+    """U, T and the stresses against the reference's own solution: to 1e-10 on the FP64 path, 1e-8
+    on the refined mixed-precision paths (measured: U 6e-15, T 1.2e-13, stresses 1.1e-14 on every
+    path). The system's condition number is ~3e17 in its raw column scaling (E0 = 1e6 separates the
+    displacement and traction columns), which the relative residual check does not depend on."""
     prob = read_problem(GOLD)
     ref = np.load(os.path.join(GOLD, "reference.npz"))
     truth = read_stress(os.path.join(GOLD, "stress_ground_truth.txt"))[:prob.nump]
@@ -36,11 +35,13 @@ def test_reference_example(mode):
     U, T, X = eng.getUTVector()
     s = eng.getFlux()
     eng.close()
-    print("residual", rel, "iterations", it)
+    eu = np.linalg.norm(U - ref["U"]) / np.linalg.norm(ref["U"])
+    et = np.linalg.norm(T - ref["T"]) / np.linalg.norm(ref["T"])
+    es = np.abs(s - ref["stress"]).max() / np.abs(ref["stress"]).max()
+    print(f"residual {rel:.2e} iterations {it}; vs the reference: U {eu:.2e} T {et:.2e} stress {es:.2e}")
     assert np.linalg.norm(A @ X - b) <= 1e-10 * np.linalg.norm(b)
-    for got, want in ((U, ref["U"]), (T, ref["T"])):
-        assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-6
-    assert np.abs(s - ref["stress"]).max() <= 1e-8 * np.abs(ref["stress"]).max()
+    tol = 1e-10 if mode == SOLVE_FP64_LU else 1e-8
+    assert eu < tol and et < tol and es < tol
     scale = np.abs(truth).max(axis=1, keepdims=True)
     assert (np.abs(s - truth) / scale).max() < 1e-5
 

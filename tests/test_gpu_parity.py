@@ -1,6 +1,8 @@
 """Parity of the HIP engine with the CPU oracle (run on an MI355X: pytest -m gpu).
 
-Every test prints the case it runs before touching the GPU. Tolerances: assembled entries and b to
+Cases: 2 spheres on 2x2 quads per face, 5 spheres on 3x3, and 24 spheres on 7x7 (294 elements, past
+one 256-element chunk of the boundary fold). Every test prints the case it runs before touching the
+GPU. Tolerances: assembled entries and b to
 1e-12 of the largest entry of their (row block, column block); solutions and stresses to 1e-10
 relative (FP64 path) or 1e-8 (refined mixed-precision path).
 """
@@ -29,6 +31,14 @@ def small_problem(kind):
         pts[1] = p.centres[1] + np.array([0.3, -0.2, 0.1]) * p.radii[1]
         p.points = pts
         return p
+    if kind == "many":
+        # 294 boundary elements: the boundary fold runs two rounds of 256-element chunks; 24 spheres
+        # give the inclusion kernel six 4-sphere target tiles per source sphere; contrast 30
+        p = make_rve(13, 24, 0.05, 0.08, 7, nump=40, E1=30.0, nu1=0.3)
+        pts = p.points.copy()
+        pts[0] = p.centres[3]
+        p.points = pts
+        return p
     raise KeyError(kind)
 
 
@@ -50,7 +60,7 @@ def blocks(prob):
     return [(edges[i], edges[i + 1]) for i in range(len(edges) - 1)]
 
 
-@pytest.fixture(scope="module", params=["two", "five"])
+@pytest.fixture(scope="module", params=["two", "five", "many"])
 def solved(request, oracle_lib):
     prob = small_problem(request.param)
     case(f"{request.param}: NN={prob.NN} NE={prob.NE} num={prob.num} n60={prob.n60}")
