@@ -29,15 +29,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from ibemproject_amd.engine import (SOLVE_BJ_GMRES, SOLVE_FP64_LU, SOLVE_MIXED_BF16,  # noqa: E402
-                                    SOLVE_MIXED_FP16, SOLVE_MIXED_FP16_1, SOLVE_MIXED_FP32, Engine)
+from ibemproject_amd.engine import (SOLVE_BJ_GMRES, SOLVE_BJ_GMRES16, SOLVE_FP64_LU,  # noqa: E402
+                                    SOLVE_MIXED_BF16, SOLVE_MIXED_FP16, SOLVE_MIXED_FP16_1, SOLVE_MIXED_FP32,
+                                    Engine)
 from ibemproject_amd.generate import make_workload  # noqa: E402
 
 MI355X_PEAK = {"f16_mfma_tflops": 2500.0, "f32_mfma_tflops": 157.3, "f64_mfma_tflops": 78.6,
                "hbm_gbs": 8000.0}
 NB = 1024  # LU block size (solver.hip IBEM_LU_NB)
 DEFAULT_WORKLOAD = "rve2500_m24"
-MODES = {"bj_gmres": SOLVE_BJ_GMRES, "fp16_1": SOLVE_MIXED_FP16_1, "fp16x3": SOLVE_MIXED_FP16, "bf16": SOLVE_MIXED_BF16,
+MODES = {"bj_gmres16": SOLVE_BJ_GMRES16, "bj_gmres": SOLVE_BJ_GMRES, "fp16_1": SOLVE_MIXED_FP16_1, "fp16x3": SOLVE_MIXED_FP16, "bf16": SOLVE_MIXED_BF16,
          "fp32": SOLVE_MIXED_FP32}
 # the trailing-update kernel of each mode (k_gemm_f16x3_256 / k_gemm_f16x3 instantiated for the
 # product type) and its MFMA products per algorithmic multiply-add
@@ -317,6 +318,12 @@ def batch_leg(args, hip):
         "times": res["times"], "max_rel_residual": float(res["rel"].max()),
         "host_refined_loads": int(res["host_refined"].sum()),
         "lu_tflops_effective": lu_flops / wall / 1e12,
+        # whole-batch rate: the algorithmic 2n^3/3 of every RVE's LU over the batch's wall time (all
+        # kernels of the batch, assembly and refinement included), against the LU's product peak
+        "roofline": {"bound": "mfma", "achieved": lu_flops / wall / 1e12,
+                     "peak": 157.3 if cfg["mode"] == "fp32" else 2500.0, "unit": "TFLOP/s",
+                     "frac": lu_flops / wall / 1e12 / (157.3 if cfg["mode"] == "fp32" else 2500.0),
+                     "traffic": None, "scope": "whole batch, all kernels"},
         "stiffness": {"C11_mean": float(C[:, 0, 0].mean()), "C11_min": float(C[:, 0, 0].min()),
                       "C11_max": float(C[:, 0, 0].max()),
                       "directional_modulus_mean": float(energy.mean()),
@@ -476,6 +483,7 @@ def main():
     progress(f"{args.steps} timed steps done: {step_s:.3f} s per step")
     tm = eng.timings()
     _, _, x = eng.getUTVector()
+    inner_its = eng.inner_iterations() if args.mode == "bj_gmres16" else 0  # before the side steps
     flops, launches = trailing_update_flops(n)
     gemm_ms = tm.get("lu.gemm", 0.0) / args.steps
     achieved = flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
@@ -486,7 +494,8 @@ def main():
     if not args.no_fp64:
         fp64, x64, s64 = side_step(eng, hip, SOLVE_FP64_LU, "solve.lu_f64", flops, x, s)
         progress(f"FP64 LU step: {fp64['step_s']:.3f} s")
-    cmp_mode = {"bj_gmres": "fp16_1", "fp16x3": "fp16_1"}.get(args.mode, "fp16x3")
+    bj = args.mode in ("bj_gmres", "bj_gmres16")
+    cmp_mode = "fp16_1" if bj or args.mode == "fp16x3" else "fp16x3"
     other = None
     if not args.no_compare:
         ref_x, ref_s = (x64, s64) if fp64 else (x, s)
@@ -497,24 +506,39 @@ def main():
     if base:
         progress(f"CPU baseline: {base['value']:.4g} s (extrapolated)")
     gemv = None
-    if args.mode == "bj_gmres":
-        # the timed step's dominant kernel: the FP64 matrix-vector product over A (k_gemv_part +
-        # k_gemv_done), one per GMRES iteration and one per restart cycle (its true residual)
+    if bj:
+        # the timed step's dominant kernel: a matrix-vector product over A. FP64 (k_gemv_part +
+        # k_gemv_done): one per outer GMRES iteration and one per restart cycle (its true residual);
+        # FP16 copy (k_gemv16_part + k_gemv_done_n, bj_gmres16): one per inner iteration
         its = rels[-1][1]
-        gemv_launches = its + max(1, -(-its // 100))
-        gemv_ms = tm.get("solve.residual", 0.0) / args.steps / gemv_launches
-        gbytes = 8.0 * n * n + 8.0 * 33 * n  # A once, x per column chunk, partial sums
-        gemv = {"kernel": "k_gemv_part + k_gemv_done (FP64 A x: 32 column chunks, two rows per thread, "
-                          "16-byte loads; fixed-order partial sums)",
-                "bound": "hbm", "achieved": gbytes / (gemv_ms * 1e-3) / 1e9, "peak": MI355X_PEAK["hbm_gbs"],
-                "unit": "GB/s", "frac": gbytes / (gemv_ms * 1e-3) / 1e9 / MI355X_PEAK["hbm_gbs"],
-                "traffic": None, "traffic_algorithmic": gbytes, "launches_per_step": gemv_launches,
-                "avg_launch_ms": gemv_ms,
-                "note": "achieved = algorithmic bytes per product (A read once: 8 n^2, plus x and the "
+        inner = inner_its
+        g64_launches = its + max(1, -(-its // 100))
+        g64_ms = tm.get("solve.residual", 0.0) / args.steps / g64_launches
+        g64_bytes = 8.0 * n * n + 8.0 * 33 * n  # A once, x per column chunk, partial sums
+        g16_ms = tm.get("solve.gemv_m16", 0.0) / args.steps / inner if inner > 0 else 0.0
+        ld16 = (n + 255) // 256 * 256
+        g16_bytes = 2.0 * ld16 * n + 4.0 * (ld16 // 256) * ((n + 63) // 64) + 8.0 * 33 * n
+        use16 = inner > 0 and g16_ms * inner >= g64_ms * g64_launches
+        gms, gbytes, glaunch = (g16_ms, g16_bytes, inner) if use16 else (g64_ms, g64_bytes, g64_launches)
+        gemv = {"kernel": ("k_gemv16_part + k_gemv_done_n (FP16 copy of A, power-of-two scale per 256x64 "
+                           "tile: eight rows per thread, 16-byte loads, FP32 sums per tile, FP64 across)"
+                           if use16 else
+                           "k_gemv_part + k_gemv_done (FP64 A x: 32 column chunks, two rows per thread, "
+                           "16-byte loads; fixed-order partial sums)"),
+                "bound": "hbm", "achieved": gbytes / (gms * 1e-3) / 1e9, "peak": MI355X_PEAK["hbm_gbs"],
+                "unit": "GB/s", "frac": gbytes / (gms * 1e-3) / 1e9 / MI355X_PEAK["hbm_gbs"],
+                "traffic": None, "traffic_algorithmic": gbytes, "launches_per_step": glaunch,
+                "avg_launch_ms": gms,
+                "fp64_product": {"avg_launch_ms": g64_ms, "launches_per_step": g64_launches,
+                                 "achieved_gbs": g64_bytes / (g64_ms * 1e-3) / 1e9 if g64_ms else None},
+                "fp16_product": ({"avg_launch_ms": g16_ms, "launches_per_step": inner,
+                                  "achieved_gbs": g16_bytes / (g16_ms * 1e-3) / 1e9}
+                                 if inner > 0 else None),
+                "note": "achieved = algorithmic bytes per product (the matrix read once, plus x and the "
                         "partial sums) / the average product time (HIP events around each product on "
                         "the context's stream)"}
-    lu_side = other if args.mode == "bj_gmres" else None
-    lu_mode = cmp_mode if args.mode == "bj_gmres" else args.mode
+    lu_side = other if bj else None
+    lu_mode = cmp_mode if bj else args.mode
     lu_gemm_ms = (lu_side["phase_ms"].get("lu.gemm", 0.0) if lu_side else gemm_ms)
     lu_achieved = flops / (lu_gemm_ms * 1e-3) / 1e12 if lu_gemm_ms else None
     kname, prods = GEMM_KERNEL[lu_mode]
@@ -532,10 +556,15 @@ def main():
                "note": f"{lu_mode} LU step: achieved = algorithmic trailing-update flops (2 m^2 {NB} "
                        "per LU step) / the trailing-update GEMM time of the step (HIP events around "
                        "each launch on the stream it runs on)"}
-    solver = ("fp64 GMRES right-preconditioned by block Jacobi (boundary-block LU, inverse 60x60 "
-              "sphere blocks) to 1e-10, mixed LU as fallback" if args.mode == "bj_gmres"
-              else f"{args.mode} LU + fp64 GMRES-IR / iterative refinement to 1e-10")
+    solver = {"bj_gmres": "fp64 GMRES right-preconditioned by block Jacobi (boundary-block LU, inverse "
+                          "60x60 sphere blocks) to 1e-10, mixed LU as fallback",
+              "bj_gmres16": "fp64 flexible GMRES (fp64 residuals and products) whose directions come from "
+                            "inner block-Jacobi GMRES on a per-tile-scaled fp16 copy of A, to 1e-10; mixed "
+                            "LU as fallback"}.get(args.mode, f"{args.mode} LU + fp64 GMRES-IR / iterative "
+                                                             "refinement to 1e-10")
     dtype = {"bj_gmres": "f64 (assembly, block-Jacobi preconditioner, GMRES, residual)",
+             "bj_gmres16": ("f64 assembly, preconditioner, outer GMRES and residual; inner products with an "
+                            "f16 copy of A (per-tile pow2 scaling, f32 sums)"),
              "fp16_1": ("f64 assembly, residual and solution update; f32 LU with f16 mfma products "
                         "(one per multiply-add, per-row/column pow2 scaling) and f32 accumulation; "
                         "f16 factor copy (per-tile pow2 scaling)")}.get(args.mode, args.mode)
@@ -557,6 +586,7 @@ def main():
                    "solver": solver, "parallelism": "single GPU"},
         "rel_residual": rels[-1][0],
         "iterations": rels[-1][1],
+        "inner_iterations": inner_its if args.mode == "bj_gmres16" else None,
         "error_vs_fp64_lu": ({"x_rel": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
                               "stress_rel": float(np.abs(s - s64).max() / np.abs(s64).max())}
                              if fp64 else None),

@@ -64,6 +64,14 @@ struct ibem3d_ctx {
   // (nb x nb, nb = 3 NN) with its LU workspace, and the inverse of every sphere's 60 x 60 diagonal block
   double *d_Abb = nullptr, *d_rb = nullptr, *d_cb = nullptr, *d_Binv = nullptr, *d_tb = nullptr;
   LuWork Wb;
+  // IBEM3D_SOLVE_BJ_GMRES16: the 16-bit copy of A (leading dimension ld16) with its tile scales, and
+  // the inner GMRES's basis, directions and product vector
+  unsigned short* d_A16 = nullptr;
+  float* d_A16s = nullptr;
+  long ld16 = 0;
+  double *d_iV = nullptr, *d_iZ = nullptr, *d_iw = nullptr;
+  long inner_iters = 0;        // inner iterations of the last solve
+  bool m16_skipped = false;    // the last BJ_GMRES16 solve ran as BJ_GMRES (no room for the copy)
   // initial guess of the eigenstrain unknowns (60 num) for the block-Jacobi GMRES, if set
   double* d_x0s = nullptr;
   bool has_x0 = false;
@@ -335,6 +343,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32, ctx->d_f16ws, ctx->d_eqws,
                   ctx->d_F16,   ctx->d_F16s,     ctx->d_Abb,      ctx->d_rb,   ctx->d_cb,
                   ctx->d_Binv,  ctx->d_tb,       ctx->d_x0s,      ctx->Wb.d_ipiv, ctx->Wb.d_ws,
+                  ctx->d_A16,   ctx->d_A16s,     ctx->d_iV,       ctx->d_iZ,   ctx->d_iw,
                   ctx->Wb.d_info, ctx->Wb.d_perm, ctx->Wb.d_inv,  ctx->Wb.d_tmp, ctx->Wb.d_trinv,
                   ctx->Wb.d_trsm,
                   ctx->d_gV,    ctx->d_gZ,       ctx->d_gpart,    ctx->d_gh,   ctx->d_bemws, ctx->d_gemvws,
@@ -726,6 +735,84 @@ static void precond_solve(ibem3d_ctx* ctx, float* v) {
     lu_solve_f32(ctx->d_A32, ctx->n, ctx->lda32, ctx->W, v, ctx->st);
 }
 
+// z ~ A^-1 v for the outer GMRES of IBEM3D_SOLVE_BJ_GMRES16: GMRES on A16 z = v from z = 0, right-
+// preconditioned by block Jacobi, CGS2 Arnoldi, at most kTuneInnerIters iterations, stopping at the
+// relative residual 10^-kTuneInnerTolExp (the 16-bit matrix holds A to about 2^-12, so a tighter
+// inner solve buys the outer iteration nothing)
+static int inner_m16(ibem3d_ctx* ctx, const double* v, double* z) {
+  const long n = ctx->n;
+  const int kmax = (int)get_tuning(kTuneInnerIters);
+  const double tol = std::pow(10.0, -(double)get_tuning(kTuneInnerTolExp));
+  double* V = ctx->d_iV;
+  double* Z = ctx->d_iZ;
+  double* w = ctx->d_iw;
+  double beta = 0.0;
+  if (int rc = dots(ctx, v, 1, v, &beta)) return rc;
+  beta = std::sqrt(beta);
+  fill_zero(z, sizeof(double) * n, ctx->st);
+  if (!(beta > 0)) return 0;
+  hipLaunchKernelGGL(k_scale_by, vgrid(n), dim3(256), 0, ctx->st, V, v, 1.0 / beta, n);
+  std::vector<double> H((size_t)(kmax + 1) * kmax, 0.0), cs(kmax), sn(kmax), g(kmax + 1, 0.0);
+  g[0] = beta;
+  int k = 0;
+  for (int j = 0; j < kmax; ++j) {
+    double* vj = V + (size_t)j * n;
+    double* zj = Z + (size_t)j * n;
+    bj_apply(ctx, vj, zj);
+    {
+      Scope s(ctx, "solve.gemv_m16");
+      gemv_m16(n, 1.0, ctx->d_A16, ctx->ld16, ctx->d_A16s, zj, 0.0, w, ctx->d_gemvws, ctx->st);
+    }
+    double* Hj = H.data() + (size_t)j * (kmax + 1);
+    std::vector<double> h(j + 1);
+    for (int pass = 0; pass < 2; ++pass) {
+      if (int rc = dots(ctx, V, j + 1, w, h.data())) return rc;
+      HIPC(hipMemcpyAsync(ctx->d_gh, h.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, ctx->st));
+      hipLaunchKernelGGL(k_combo, vgrid(n), dim3(256), 0, ctx->st, w, V, n, j + 1, ctx->d_gh, -1.0, n);
+      for (int i = 0; i <= j; ++i) Hj[i] += h[i];
+    }
+    double hn = 0.0;
+    if (int rc = dots(ctx, w, 1, w, &hn)) return rc;
+    hn = std::sqrt(hn);
+    Hj[j + 1] = hn;
+    if (hn > 0 && j + 1 < kmax)
+      hipLaunchKernelGGL(k_scale_by, vgrid(n), dim3(256), 0, ctx->st, V + (size_t)(j + 1) * n, w, 1.0 / hn, n);
+    for (int i = 0; i < j; ++i) {
+      const double t = cs[i] * Hj[i] + sn[i] * Hj[i + 1];
+      Hj[i + 1] = -sn[i] * Hj[i] + cs[i] * Hj[i + 1];
+      Hj[i] = t;
+    }
+    const double den = std::hypot(Hj[j], Hj[j + 1]);
+    cs[j] = den > 0 ? Hj[j] / den : 1.0;
+    sn[j] = den > 0 ? Hj[j + 1] / den : 0.0;
+    Hj[j] = den;
+    Hj[j + 1] = 0.0;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    k = j + 1;
+    ctx->inner_iters += 1;
+    if (!(std::fabs(g[j + 1]) > tol * beta)) break;  // also stops on a breakdown (NaN)
+  }
+  std::vector<double> y(k);
+  for (int i = k - 1; i >= 0; --i) {
+    double acc = g[i];
+    for (int c = i + 1; c < k; ++c) acc -= H[(size_t)c * (kmax + 1) + i] * y[c];
+    const double d = H[(size_t)i * (kmax + 1) + i];
+    y[i] = d != 0 ? acc / d : 0.0;
+  }
+  HIPC(hipMemcpyAsync(ctx->d_gh, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, ctx->st));
+  hipLaunchKernelGGL(k_combo, vgrid(n), dim3(256), 0, ctx->st, z, Z, n, k, ctx->d_gh, 1.0, n);
+  return after_launch(ctx, "inner GMRES");
+}
+
+static void free_m16(ibem3d_ctx* ctx) {
+  for (void* q : {(void*)ctx->d_A16, (void*)ctx->d_A16s, (void*)ctx->d_iV, (void*)ctx->d_iZ, (void*)ctx->d_iw})
+    if (q) hipFree(q);
+  ctx->d_A16 = nullptr;
+  ctx->d_A16s = nullptr;
+  ctx->d_iV = ctx->d_iZ = ctx->d_iw = nullptr;
+}
+
 // relative residual ||b - A x|| / ||b|| with the FP64 system matrix
 static int residual(ibem3d_ctx* ctx, double* rel) {
   const long n = ctx->n;
@@ -782,7 +869,10 @@ static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0, int max
     for (int j = 0; j < kGmres; ++j) {
       double* vj = V + (size_t)j * n;
       double* zj = Z + (size_t)j * n;
-      if (ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES) {
+      if (ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES16 && !ctx->m16_skipped) {
+        Scope s(ctx, "solve.inner_m16");
+        if (int rc = inner_m16(ctx, vj, zj)) return rc;
+      } else if (ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES || ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES16) {
         Scope s(ctx, "solve.precond_bj");
         bj_apply(ctx, vj, zj);
       } else {
@@ -1034,14 +1124,39 @@ static int solve_mixed(ibem3d_ctx* ctx, int mode, double tol, bool gmres_first, 
 // x_b = A_bb^-1 (b_b - A_bs x_s). When GMRES does not reach tol the mixed single-FP16 LU solves the
 // system instead (the fallback, compact storage when the copy does not fit).
 constexpr int kBjCycles = 3;  // GMRES cycles (of kGmres iterations) before the LU fallback
-static int solve_bj(ibem3d_ctx* ctx, double tol, bool reuse) {
+// mode: IBEM3D_SOLVE_BJ_GMRES, or IBEM3D_SOLVE_BJ_GMRES16 (the 16-bit copy of A built here when it fits)
+static int solve_bj(ibem3d_ctx* ctx, double tol, bool reuse, int mode) {
   const long n = ctx->n, nb = 3L * ctx->NN;
   if (!(tol > 0)) tol = 1e-10;
   ctx->bj_fallback = false;
-  if (!reuse || ctx->prepared_mode != IBEM3D_SOLVE_BJ_GMRES) {
+  if (!reuse || ctx->prepared_mode != mode) {
     ctx->prepared_mode = -1;
     if (int rc = bj_setup(ctx)) return rc;
-    ctx->prepared_mode = IBEM3D_SOLVE_BJ_GMRES;
+    ctx->m16_skipped = false;
+    if (mode == IBEM3D_SOLVE_BJ_GMRES16) {
+      const long ld16 = m16_ld(n);
+      const size_t need = ctx->d_A16 ? 0
+                                     : 2 * (size_t)ld16 * n + 4 * m16_scale_count(n) +
+                                           8 * (size_t)(2 * kInnerItersMax + 2) * n;
+      if (need == 0 || fits(need)) {
+        if (!ctx->d_A16) {
+          if (int rc = dalloc(ctx, &ctx->d_A16, (size_t)ld16 * n)) return rc;
+          if (int rc = dalloc(ctx, &ctx->d_A16s, m16_scale_count(n))) return rc;
+          if (int rc = dalloc(ctx, &ctx->d_iV, (size_t)(kInnerItersMax + 1) * n)) return rc;
+          if (int rc = dalloc(ctx, &ctx->d_iZ, (size_t)kInnerItersMax * n)) return rc;
+          if (int rc = dalloc(ctx, &ctx->d_iw, (size_t)n)) return rc;
+          ctx->ld16 = ld16;
+        }
+        {
+          Scope s(ctx, "solve.m16_build");
+          m16_build(ctx->d_A, n, n, ctx->d_A16, ctx->ld16, ctx->d_A16s, ctx->st);
+        }
+        if (int rc = after_launch(ctx, "16-bit copy of A")) return rc;
+      } else {
+        ctx->m16_skipped = true;
+      }
+    }
+    ctx->prepared_mode = mode;
   }
   int rc;
   if (ctx->has_x0 && ctx->num > 0) {
@@ -1083,6 +1198,7 @@ static int solve_bj(ibem3d_ctx* ctx, double tol, bool reuse) {
     const int bj_iters = ctx->iters;
     ctx->iters = 0;
     ctx->solved = false;
+    free_m16(ctx);  // room for the LU
     const bool compact = !fits(sizeof(float) * (size_t)((n + 63) / 64 * 64) * n);
     rc = solve_mixed(ctx, IBEM3D_SOLVE_MIXED_FP16_1, tol, false, false, compact);
     ctx->iters += bj_iters;
@@ -1097,14 +1213,15 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
   const bool reuse = (mode & IBEM3D_SOLVE_REUSE) != 0;
   bool compact = (mode & IBEM3D_SOLVE_COMPACT) != 0;
   mode &= ~(IBEM3D_SOLVE_GMRES | IBEM3D_SOLVE_REUSE | IBEM3D_SOLVE_COMPACT);
-  if (mode < IBEM3D_SOLVE_FP64_LU || mode > IBEM3D_SOLVE_BJ_GMRES)
+  if (mode < IBEM3D_SOLVE_FP64_LU || mode > IBEM3D_SOLVE_BJ_GMRES16)
     return fail(ctx, IBEM3D_EINVAL, "solve: unknown mode");
+  const bool bj = mode == IBEM3D_SOLVE_BJ_GMRES || mode == IBEM3D_SOLVE_BJ_GMRES16;
   const long n = ctx->n;
   if (reuse) {
     if (ctx->prepared_mode < 0 || (ctx->prepared_mode & ~IBEM3D_SOLVE_COMPACT) != mode)
       return fail(ctx, IBEM3D_ESTATE, "solve: no factors of this mode to reuse");
     compact = (ctx->prepared_mode & IBEM3D_SOLVE_COMPACT) != 0;
-  } else if (!compact && mode != IBEM3D_SOLVE_BJ_GMRES) {
+  } else if (!compact && !bj) {
     // the second n x n copy (FP64 factor, or FP32 factor with its 64-aligned columns)
     const size_t need = mode == IBEM3D_SOLVE_FP64_LU ? (ctx->d_Af ? 0 : sizeof(double) * (size_t)n * n)
                                                      : (ctx->d_A32 ? 0 : sizeof(float) * (size_t)((n + 63) / 64 * 64) * n);
@@ -1115,9 +1232,15 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
     return fail(ctx, IBEM3D_ESTATE, "solve: A holds a factor (compact solve); assemble again");
   if (int rc = ensure_solver_ws(ctx)) return rc;
   ctx->iters = 0;
+  ctx->inner_iters = 0;
   ctx->rel_res = -1.0;
+  if (mode != IBEM3D_SOLVE_BJ_GMRES16 && ctx->d_A16) {
+    // the 16-bit copy of A gives its memory back to the other modes
+    if (ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES16) ctx->prepared_mode = -1;
+    free_m16(ctx);
+  }
   if (mode == IBEM3D_SOLVE_FP64_LU) return solve_fp64(ctx, reuse, compact);
-  if (mode == IBEM3D_SOLVE_BJ_GMRES) return solve_bj(ctx, tol, reuse);
+  if (bj) return solve_bj(ctx, tol, reuse, mode);
   return solve_mixed(ctx, mode, tol, gmres_first, reuse, compact);
 }
 
@@ -1140,6 +1263,11 @@ int ibem3d_set_initial_guess(ibem3d_ctx* ctx, const double* x_eig) {
 }
 
 int ibem3d_solve_fell_back(const ibem3d_ctx* ctx) { return ctx && ctx->bj_fallback ? 1 : 0; }
+
+long ibem3d_inner_iterations(const ibem3d_ctx* ctx) {
+  if (!ctx) return -1;
+  return ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES16 && ctx->m16_skipped ? -1 : ctx->inner_iters;
+}
 
 int ibem3d_warm_start_info(const ibem3d_ctx* ctx, double* rel0, double* alpha) {
   if (!ctx) return IBEM3D_EINVAL;

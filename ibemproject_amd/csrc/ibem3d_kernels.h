@@ -108,6 +108,14 @@ void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStrea
 size_t gemv_workspace_doubles(long m);
 void gemv_f64(long m, long n, double alpha, const double* A, long lda, const double* x, double beta,
               double* y, double* ws, hipStream_t st);
+// 16-bit copy of the n x n FP64 A (per 256 x 64 tile power-of-two scaled FP16, leading dimension
+// m16_ld(n), m16_scale_count(n) float scales) and y = alpha A16 x + beta y with it (FP32 sums within a
+// tile, FP64 across; ws as for gemv_f64)
+long m16_ld(long n);
+size_t m16_scale_count(long n);
+void m16_build(const double* A, long n, long lda, unsigned short* A16, long ld16, float* scale, hipStream_t st);
+void gemv_m16(long n, double alpha, const unsigned short* A16, long ld16, const float* scale, const double* x,
+              double beta, double* y, double* ws, hipStream_t st);
 // out = R A C with the same scaling, A left unchanged (FP64 or FP32 output); ws holds
 // equilibrate_workspace_doubles(n) doubles. With W, the FP32 copy writes its first LU panel's columns
 // on st and the rest on W's second stream, recording W.ev_cols: the next lu_factor_f32 on W must
@@ -135,7 +143,10 @@ void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, co
 // Runtime tunables of the LU (process-wide): the 256 x 256 tile count from which trailing updates
 // take the persistent kernel, and the percentage of the panel CU schedule the bulk update leaves
 // to the concurrent panel. set_tuning returns nonzero for an unknown key or a value out of range.
-enum { kTuneBigMinTiles = 1, kTunePanelCuPercent = 2 };
+// kTuneInnerIters / kTuneInnerTolExp: the inner GMRES of IBEM3D_SOLVE_BJ_GMRES16 (iterations per outer
+// direction, at most kInnerItersMax; stopping tolerance 10^-exp).
+enum { kTuneBigMinTiles = 1, kTunePanelCuPercent = 2, kTuneInnerIters = 3, kTuneInnerTolExp = 4 };
+constexpr int kInnerItersMax = 16;
 int set_tuning(int key, long value);
 long get_tuning(int key);
 // Zero bytes (a multiple of 4; p 16-byte aligned) with a kernel instead of a memset (capturable)

@@ -875,6 +875,8 @@ static unsigned big_gemm_grid(long tiles, int cus) {
 // kernel, and the share (percent) of the panel_cus schedule the bulk update leaves to the panel.
 static long g_big_min_tiles = IBEM_BIG_MIN_TILES;
 static long g_panel_cu_percent = 100;
+static long g_inner_iters = 6;     // inner GMRES iterations on the 16-bit A per outer direction
+static long g_inner_tol_exp = 3;   // inner GMRES stops at relative residual 10^-exp
 int set_tuning(int key, long value) {
   if (key == kTuneBigMinTiles && value >= 1) {
     g_big_min_tiles = value;
@@ -884,10 +886,24 @@ int set_tuning(int key, long value) {
     g_panel_cu_percent = value;
     return 0;
   }
+  if (key == kTuneInnerIters && value >= 1 && value <= kInnerItersMax) {
+    g_inner_iters = value;
+    return 0;
+  }
+  if (key == kTuneInnerTolExp && value >= 1 && value <= 8) {
+    g_inner_tol_exp = value;
+    return 0;
+  }
   return -1;
 }
 long get_tuning(int key) {
-  return key == kTuneBigMinTiles ? g_big_min_tiles : key == kTunePanelCuPercent ? g_panel_cu_percent : -1;
+  switch (key) {
+    case kTuneBigMinTiles: return g_big_min_tiles;
+    case kTunePanelCuPercent: return g_panel_cu_percent;
+    case kTuneInnerIters: return g_inner_iters;
+    case kTuneInnerTolExp: return g_inner_tol_exp;
+    default: return -1;
+  }
 }
 // C[M x N] -= split A times the split B columns [c0, c0 + N); the persistent 256 x 256 kernel runs
 // on at most cus CUs (0: all)
@@ -2553,6 +2569,17 @@ __global__ __launch_bounds__(256) void k_gemv_done(long m, double alpha, const d
   y[r] = beta == 0.0 ? v : v + beta * y[r];
 }
 
+__global__ __launch_bounds__(256) void k_gemv_done_n(long m, int chunks, double alpha,
+                                                     const double* __restrict__ part, double beta,
+                                                     double* __restrict__ y) {
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  double v = 0.0;
+  for (int c = 0; c < chunks; ++c) v += part[(long)c * m + r];
+  v *= alpha;
+  y[r] = beta == 0.0 ? v : v + beta * y[r];
+}
+
 void gemv_f64(long m, long n, double alpha, const double* A, long lda, const double* x, double beta,
               double* y, double* ws, hipStream_t st) {
   const bool pairs = (lda & 1) == 0 && (((size_t)A) & 15) == 0;
@@ -2573,6 +2600,102 @@ __global__ __launch_bounds__(256) void k_f64_to_f32(const double* __restrict__ A
 void convert_f64_to_f32(const double* A, long n, long lda, float* B, long ldb, hipStream_t st) {
   hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, st, A,
                      n, lda, B, ldb);
+}
+
+// ---- 16-bit copy of A for the inner matrix-vector products of block-Jacobi GMRES -------------------
+// A16[j * ld16 + i] = fp16(A[j * lda + i] 2^-e), one power-of-two exponent e per tile of kM16R rows x
+// kM16C columns, chosen so the tile's largest magnitude lands in [2^13, 2^14) (an FP16 significand of
+// 11 bits for everything within 2^27 of it); scale[ct * row_tiles + rt] = 2^e. Rows m <= i < ld16 hold
+// zeros, so the product reads whole 16-byte row groups.
+constexpr int kM16R = 256, kM16C = 64;
+long m16_ld(long n) { return (n + kM16R - 1) / kM16R * kM16R; }
+size_t m16_scale_count(long n) { return (size_t)(m16_ld(n) / kM16R) * (size_t)((n + kM16C - 1) / kM16C); }
+
+__global__ __launch_bounds__(256) void k_m16_build(const double* __restrict__ A, long m, long n, long lda,
+                                                   _Float16* __restrict__ A16, long ld16,
+                                                   float* __restrict__ scale) {
+  __shared__ double wmax[4];
+  const long rt = blockIdx.x, ct = blockIdx.y;
+  const long i = rt * kM16R + threadIdx.x;
+  const long j0 = ct * kM16C;
+  const int nc = (int)min((long)kM16C, n - j0);
+  double v[kM16C];
+  double mx = 0.0;
+#pragma unroll
+  for (int c = 0; c < kM16C; ++c) {
+    v[c] = (c < nc && i < m) ? A[(j0 + c) * lda + i] : 0.0;
+    mx = fmax(mx, fabs(v[c]));
+  }
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  mx = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+  int ex = 0;
+  if (mx > 0) frexp(mx, &ex);  // mx = f 2^ex, f in [0.5, 1)
+  const int e = mx > 0 ? ex - 14 : 0;
+  const double down = ldexp(1.0, -e);
+  if (threadIdx.x == 0) scale[ct * gridDim.x + rt] = ldexpf(1.0f, e);
+#pragma unroll
+  for (int c = 0; c < kM16C; ++c)
+    if (c < nc) A16[(j0 + c) * ld16 + i] = (_Float16)(float)(v[c] * down);
+}
+
+void m16_build(const double* A, long n, long lda, unsigned short* A16, long ld16, float* scale, hipStream_t st) {
+  hipLaunchKernelGGL(k_m16_build, dim3((unsigned)(ld16 / kM16R), (unsigned)((n + kM16C - 1) / kM16C)), dim3(256), 0,
+                     st, A, n, n, lda, (_Float16*)A16, ld16, scale);
+}
+
+// part[chunk * m + r] = sum over the chunk's columns of A16[r, j] 2^e x_j: a thread holds 8 consecutive
+// rows (one 16-byte load per column), FP32 sums within a 64-column tile, FP64 across tiles; the chunk
+// partial sums are added in a fixed order by k_gemv_done
+constexpr int kM16Chunks = 32;
+__global__ __launch_bounds__(256) void k_gemv16_part(long m, long n, const _Float16* __restrict__ A16, long ld16,
+                                                     const float* __restrict__ scale, long row_tiles,
+                                                     const double* __restrict__ x, double* __restrict__ part) {
+  const long r0 = ((long)blockIdx.x * 256 + threadIdx.x) * 8;
+  if (r0 >= m) return;
+  const long tiles = (n + kM16C - 1) / kM16C;
+  const long tpc = (tiles + kM16Chunks - 1) / kM16Chunks;  // column tiles per chunk
+  const long t0 = (long)blockIdx.y * tpc, t1 = min(tiles, t0 + tpc);
+  const long rt = r0 / kM16R;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (long ct = t0; ct < t1; ++ct) {
+    const long j0 = ct * kM16C, j1 = min(n, j0 + kM16C);
+    float at[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long j = j0;
+    for (; j + 8 <= j1; j += 8) {
+      half8 h[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) h[q] = *(const half8*)(A16 + (j + q) * ld16 + r0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const float xq = (float)x[j + q];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) at[k] = fmaf((float)h[q][k], xq, at[k]);
+      }
+    }
+    for (; j < j1; ++j) {
+      const half8 h = *(const half8*)(A16 + j * ld16 + r0);
+      const float xq = (float)x[j];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) at[k] = fmaf((float)h[k], xq, at[k]);
+    }
+    const double sc = (double)scale[ct * row_tiles + rt];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] += sc * (double)at[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (r0 + k < m) part[(long)blockIdx.y * m + r0 + k] = acc[k];
+}
+
+void gemv_m16(long n, double alpha, const unsigned short* A16, long ld16, const float* scale, const double* x,
+              double beta, double* y, double* ws, hipStream_t st) {
+  static_assert(kM16Chunks <= kGemvChunks, "the product's partial sums share the FP64 product's workspace");
+  hipLaunchKernelGGL(k_gemv16_part, dim3((unsigned)((n + 2047) / 2048), kM16Chunks), dim3(256), 0, st, n, n,
+                     (const _Float16*)A16, ld16, scale, ld16 / kM16R, x, ws);
+  hipLaunchKernelGGL(k_gemv_done_n, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, kM16Chunks, alpha, ws,
+                     beta, y);
 }
 
 }  // namespace ibem

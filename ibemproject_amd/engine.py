@@ -25,11 +25,14 @@ SOLVE_MIXED_FP16 = 2     # FP32 LU, trailing updates on three FP16 products of h
 SOLVE_MIXED_FP16_1 = 3   # FP32 LU, trailing updates on one FP16 product (power-of-two scaled operands)
 SOLVE_MIXED_BF16 = 4     # FP32 LU, trailing updates on one BF16 product
 SOLVE_BJ_GMRES = 5       # FP64 GMRES preconditioned by block Jacobi (no n x n factor), LU fallback
+SOLVE_BJ_GMRES16 = 6     # the same, its directions from inner block-Jacobi GMRES on a 16-bit copy of A
 SOLVE_GMRES = 0x100  # flag for the mixed modes: GMRES-IR from the start
 SOLVE_REUSE = 0x200  # flag: reuse the factors of the last factorisation in the same mode
 SOLVE_COMPACT = 0x400  # flag: the factor takes A's place (16-bit factor copy for the mixed modes)
 TUNE_BIG_MIN_TILES = 1
 TUNE_PANEL_CU_PERCENT = 2
+TUNE_INNER_ITERS = 3
+TUNE_INNER_TOL_EXP = 4
 
 
 def set_tuning(key: int, value: int) -> None:
@@ -68,7 +71,7 @@ EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size"
            "ibem3d_set_known_values", "ibem3d_average_eigenstrain", "ibem3d_residual",
            "ibem3d_update_known_values", "ibem3d_batch_create", "ibem3d_batch_destroy",
            "ibem3d_batch_last_error", "ibem3d_batch_stiffness", "ibem3d_set_tuning",
-           "ibem3d_get_tuning", "ibem3d_set_initial_guess", "ibem3d_solve_fell_back",
+           "ibem3d_get_tuning", "ibem3d_set_initial_guess", "ibem3d_solve_fell_back", "ibem3d_inner_iterations",
            "ibem3d_warm_start_info",
            "ibem3d_gnn_last_error", "ibem3d_gnn_param_count", "ibem3d_gnn_graph", "ibem3d_gnn_create",
            "ibem3d_gnn_destroy", "ibem3d_gnn_get_params", "ibem3d_gnn_set_params", "ibem3d_gnn_set_data",
@@ -124,6 +127,8 @@ def load(path: str = LIB_PATH):
     L.ibem3d_update_known_values.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.ibem3d_set_initial_guess.argtypes = [ctypes.c_void_p, _dp]
     L.ibem3d_solve_fell_back.argtypes = [ctypes.c_void_p]
+    L.ibem3d_inner_iterations.restype = ctypes.c_long
+    L.ibem3d_inner_iterations.argtypes = [ctypes.c_void_p]
     L.ibem3d_warm_start_info.argtypes = [ctypes.c_void_p, _dp, _dp]
     L.ibem3d_set_tuning.argtypes = [ctypes.c_int, ctypes.c_long]
     L.ibem3d_get_tuning.restype = ctypes.c_long
@@ -327,6 +332,10 @@ class Engine:
 
     def fell_back(self) -> bool:
         return bool(self.L.ibem3d_solve_fell_back(self.ctx))
+
+    def inner_iterations(self) -> int:
+        """Inner iterations on the 16-bit A of the last SOLVE_BJ_GMRES16 solve (-1: ran without it)."""
+        return int(self.L.ibem3d_inner_iterations(self.ctx))
 
     def residual(self) -> float:
         """Relative residual ||b - A x|| / ||b|| of the current solution (after a compact FP64 solve

@@ -77,6 +77,12 @@ enum {
                                    not reach tol in 300 iterations (three restarts of 100), the
                                    MIXED_FP16_1 LU solves the system instead
                                    (ibem3d_solve_fell_back reports it) */
+  IBEM3D_SOLVE_BJ_GMRES16 = 6,  /* BJ_GMRES whose outer FP64 GMRES (FP64 residuals and products with
+                                   A) is preconditioned by a few inner block-Jacobi GMRES iterations
+                                   on a 16-bit copy of A (FP16, every 256 x 64 tile scaled by a power
+                                   of two; 2 n^2 bytes beside A, read at a quarter of A's bytes per
+                                   product). Without room for the copy it runs as BJ_GMRES. Same
+                                   fallback as BJ_GMRES. */
   IBEM3D_SOLVE_GMRES = 0x100,   /* flag for the mixed modes: refine by GMRES-IR (FP64 GMRES, right-
                                    preconditioned by the LU factors) from the start. Without it the
                                    refinement is classic and switches to GMRES-IR when an iteration
@@ -108,8 +114,12 @@ const char* ibem3d_last_error(const ibem3d_ctx* ctx);
 enum {
   IBEM3D_TUNE_BIG_MIN_TILES = 1,     /* 256 x 256 tiles from which a trailing update takes the persistent
                                         kernel (default 512; below it the 128 x 128 kernel) */
-  IBEM3D_TUNE_PANEL_CU_PERCENT = 2   /* percent of the default schedule of CUs the bulk trailing update
+  IBEM3D_TUNE_PANEL_CU_PERCENT = 2,  /* percent of the default schedule of CUs the bulk trailing update
                                         leaves to the concurrent panel (default 100, 0..400) */
+  IBEM3D_TUNE_INNER_ITERS = 3,       /* IBEM3D_SOLVE_BJ_GMRES16: inner GMRES iterations on the 16-bit A
+                                        per outer direction (default 6, 1..16) */
+  IBEM3D_TUNE_INNER_TOL_EXP = 4      /* IBEM3D_SOLVE_BJ_GMRES16: the inner GMRES stops at relative
+                                        residual 10^-value (default 3, 1..8) */
 };
 int ibem3d_set_tuning(int key, long value);
 long ibem3d_get_tuning(int key);  /* -1 for an unknown key */
@@ -180,6 +190,10 @@ int ibem3d_get_system(ibem3d_ctx* ctx, double* A, double* b);
 int ibem3d_set_initial_guess(ibem3d_ctx* ctx, const double* x_eig);
 /* 1 if the last IBEM3D_SOLVE_BJ_GMRES solve fell back to the mixed LU, else 0. */
 int ibem3d_solve_fell_back(const ibem3d_ctx* ctx);
+/* Inner GMRES iterations (on the 16-bit A) of the last IBEM3D_SOLVE_BJ_GMRES16 solve, beside the outer
+ * iterations ibem3d_solve reports; -1 when that solve ran as BJ_GMRES (no room for the copy). No
+ * counterpart in the reference. */
+long ibem3d_inner_iterations(const ibem3d_ctx* ctx);
 /* The last block-Jacobi solve's start from the initial guess: relative residual of the scaled guess
  * alpha x0 (alpha in [0, 1] minimises ||b - alpha A x0||, so the start is never worse than zero);
  * rel0 = -1 when no guess was used. */

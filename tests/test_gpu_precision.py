@@ -173,3 +173,33 @@ def test_block_jacobi_gmres(mid):
     assert it_w < it
     eng.set_initial_guess(None)
     eng.close()
+
+
+def test_block_jacobi_gmres16(mid):
+    """Flexible FP64 GMRES whose directions come from inner block-Jacobi GMRES on the FP16 copy of A:
+    1e-10 residual, the LU solution to 1e-8, and far fewer outer iterations (FP64 products) than
+    block-Jacobi GMRES needs iterations (a wrong 16-bit product would leave the inner directions poor
+    and the outer count high)."""
+    case(f"block-Jacobi GMRES, 16-bit inner, n60={mid['prob'].n60}")
+    eng = E.Engine(mid["prob"])
+    eng.assemble()
+    rel_b, it_b = eng.solve(E.SOLVE_BJ_GMRES, 1e-10)
+    rel, it = eng.solve(E.SOLVE_BJ_GMRES16, 1e-10)
+    inner = eng.inner_iterations()
+    assert not eng.fell_back() and inner > 0
+    check_refined(eng, mid, rel, it, "bj-gmres16")
+    print("block-Jacobi GMRES iterations", it_b, "| 16-bit inner: outer", it, "inner", inner)
+    assert 2 * it <= it_b
+    # reuse with the copy in place, then a warm start from the solution perturbed by 0.1 %
+    rel_r, it_r = eng.solve(E.SOLVE_BJ_GMRES16 | E.SOLVE_REUSE, 1e-10)
+    assert rel_r <= 1e-10 and it_r == it
+    nb = 3 * mid["prob"].NN
+    rng = np.random.default_rng(4)
+    eng.set_initial_guess(mid["x64"][nb:] * (1 + 1e-3 * rng.standard_normal(mid["x64"].size - nb)))
+    rel_w, it_w = eng.solve(E.SOLVE_BJ_GMRES16 | E.SOLVE_REUSE, 1e-10)
+    check_refined(eng, mid, rel_w, it_w, "bj-gmres16 warm")
+    eng.set_initial_guess(None)
+    # a LU mode afterwards gets the copy's memory back and still solves
+    rel_l, _ = eng.solve(E.SOLVE_MIXED_FP16_1, 1e-10)
+    assert rel_l <= 1e-10
+    eng.close()
