@@ -168,7 +168,7 @@ def cpu_baseline(prob, workload: str, ns: int = 16, ms: int = 3):
     wall_asm = time.time() - t0
     rng = np.random.default_rng(0)
     lu_pts = []
-    for nlu in (1024, 1448, 2048):
+    for nlu in (2048, 2896, 4096):  # every size past the last-level cache of one core's share
         A = np.asfortranarray(rng.standard_normal((nlu, nlu)) + nlu * np.eye(nlu))
         b = rng.standard_normal(nlu)
         x = np.zeros(nlu)
@@ -178,6 +178,7 @@ def cpu_baseline(prob, workload: str, ns: int = 16, ms: int = 3):
     num, n78 = prob.num, prob.n78
     ne_s, nn_s = sample.NE, sample.NN
     t_lu = float(np.exp(lnc) * n78 ** expo)
+    t_lu3 = float(lu_pts[-1][1] * (n78 / lu_pts[-1][0]) ** 3)  # the operation count's n^3 from the largest
     est_asm = (ph["addBEM"] * (prob.NE * prob.NN) / (ne_s * nn_s)
                + ph["addFieldToBEM"] * (prob.NN * num) / (nn_s * ns)
                + (ph["incl0"] + ph["incl1"] + ph["incl2"]) * (num / ns) ** 2
@@ -195,7 +196,8 @@ def cpu_baseline(prob, workload: str, ns: int = 16, ms: int = 3):
                        f"extrapolated to the reference's n78 = {n78} unknowns: {t_lu:.3g} s; not run "
                        f"end to end"),
             "phases_s": ph, "assembly_s_extrapolated": est_asm,
-            "fullpiv_lu": {"points": lu_pts, "exponent": expo, "s_at_n78": t_lu, "n78": n78},
+            "fullpiv_lu": {"points": lu_pts, "exponent": expo, "s_at_n78": t_lu, "n78": n78,
+                           "s_at_n78_by_n3": t_lu3},
             "fair_lu": fair}
 
 
@@ -422,7 +424,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD)
-    ap.add_argument("--mode", default="bj_gmres", choices=sorted(MODES))
+    ap.add_argument("--mode", default="bj_gmres16", choices=sorted(MODES))
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")

@@ -1138,6 +1138,14 @@ static int solve_bj(ibem3d_ctx* ctx, double tol, bool reuse, int mode) {
       const size_t need = ctx->d_A16 ? 0
                                      : 2 * (size_t)ld16 * n + 4 * m16_scale_count(n) +
                                            8 * (size_t)(2 * kInnerItersMax + 2) * n;
+      if (need > 0 && !fits(need)) {
+        // the LU modes' factor storage is of no use once this mode is prepared: its memory first
+        for (void** q : {(void**)&ctx->d_Af, (void**)&ctx->d_A32, (void**)&ctx->d_F16, (void**)&ctx->d_F16s})
+          if (*q) {
+            hipFree(*q);
+            *q = nullptr;
+          }
+      }
       if (need == 0 || fits(need)) {
         if (!ctx->d_A16) {
           if (int rc = dalloc(ctx, &ctx->d_A16, (size_t)ld16 * n)) return rc;

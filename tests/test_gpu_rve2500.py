@@ -4,8 +4,9 @@ The FP64 system takes 205.8 GB, so the solves use compact storage (the factor in
 copy of the mixed factor beside A). Size-independent properties: the refined solve reaches a 1e-10
 relative FP64 residual at every particle/matrix contrast E1/E0 = 2, 10, 100; the loading fixes the
 average stress (<s_xx> = 1, <s_yy> = <s_zz> = 0) and puts the apparent modulus between the Reuss and
-Voigt bounds; at contrast 10 the single-product and split-product solutions agree, and both agree
-with the plain FP64 LU to 1e-8 in the solution and the stresses."""
+Voigt bounds; at contrast 10 the single-product and split-product LU solutions and the block-Jacobi
+GMRES solutions (FP64 products only, and with the 16-bit inner matrix) all agree with the plain FP64
+LU to 1e-8 in the solution and the stresses."""
 import numpy as np
 import pytest
 
@@ -47,6 +48,12 @@ def test_rve2500_contrast(contrast):
     assert rel <= 1e-10
     _, _, xb = eng.getUTVector()
     assert np.linalg.norm(xb - x1) / np.linalg.norm(x1) < 1e-8
+    rel, it = eng.solve(E.SOLVE_BJ_GMRES16, 1e-10)
+    print(f"  block-Jacobi GMRES, 16-bit inner: residual {rel:.3e} outer {it} inner {eng.inner_iterations()} "
+          f"fell back {eng.fell_back()}")
+    assert rel <= 1e-10 and eng.inner_iterations() > 0
+    _, _, x16 = eng.getUTVector()
+    assert np.linalg.norm(x16 - x1) / np.linalg.norm(x1) < 1e-8
     eng.close()
 
 
@@ -66,6 +73,11 @@ def test_rve2500_against_fp64_lu():
     sb = eng.getFlux()
     print(f"  block-Jacobi GMRES: residual {relb:.3e} iterations {itb} fell back {eng.fell_back()}")
     assert relb <= 1e-10
+    rel16, it16 = eng.solve(E.SOLVE_BJ_GMRES16, 1e-10)
+    _, _, x16 = eng.getUTVector()
+    s16 = eng.getFlux()
+    print(f"  block-Jacobi GMRES, 16-bit inner: residual {rel16:.3e} outer {it16} inner {eng.inner_iterations()}")
+    assert rel16 <= 1e-10 and eng.inner_iterations() > 0
     eng.reset()
     eng.assemble()
     rel3, it3 = eng.solve(E.SOLVE_MIXED_FP16, 1e-10)
@@ -82,7 +94,8 @@ def test_rve2500_against_fp64_lu():
     rel64 = eng.residual()
     print(f"  fp64 LU (in place): residual {rel64:.3e}")
     assert rel64 <= 1e-10
-    for name, x, s in (("fp16 single", x1, s1), ("fp16 split", x3, s3), ("block-Jacobi GMRES", xb, sb)):
+    for name, x, s in (("fp16 single", x1, s1), ("fp16 split", x3, s3), ("block-Jacobi GMRES", xb, sb),
+                       ("block-Jacobi GMRES 16-bit inner", x16, s16)):
         ex = np.linalg.norm(x - x64) / np.linalg.norm(x64)
         es = np.abs(s - s64).max() / np.abs(s64).max()
         print(f"  {name} vs fp64 LU: x {ex:.3e} stress {es:.3e}")
