@@ -519,10 +519,10 @@ def main():
         g64_bytes = 8.0 * n * n + 8.0 * 33 * n  # A once, x per column chunk, partial sums
         g16_ms = tm.get("solve.gemv_m16", 0.0) / args.steps / inner if inner > 0 else 0.0
         ld16 = (n + 255) // 256 * 256
-        g16_bytes = 2.0 * ld16 * n + 4.0 * (ld16 // 256) * ((n + 63) // 64) + 8.0 * 33 * n
+        g16_bytes = 2.0 * ld16 * n + 4.0 * (ld16 // 256) * ((n + 31) // 32) + 8.0 * 33 * n
         use16 = inner > 0 and g16_ms * inner >= g64_ms * g64_launches
         gms, gbytes, glaunch = (g16_ms, g16_bytes, inner) if use16 else (g64_ms, g64_bytes, g64_launches)
-        gemv = {"kernel": ("k_gemv16_part + k_gemv_done_n (FP16 copy of A, power-of-two scale per 256x64 "
+        gemv = {"kernel": ("k_gemv16_part + k_gemv_done_n (FP16 copy of A, power-of-two scale per 256x32 "
                            "tile: eight rows per thread, 16-byte loads, FP32 sums per tile, FP64 across)"
                            if use16 else
                            "k_gemv_part + k_gemv_done (FP64 A x: 32 column chunks, two rows per thread, "

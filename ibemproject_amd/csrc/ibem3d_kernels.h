@@ -108,7 +108,7 @@ void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStrea
 size_t gemv_workspace_doubles(long m);
 void gemv_f64(long m, long n, double alpha, const double* A, long lda, const double* x, double beta,
               double* y, double* ws, hipStream_t st);
-// 16-bit copy of the n x n FP64 A (per 256 x 64 tile power-of-two scaled FP16, leading dimension
+// 16-bit copy of the n x n FP64 A (per 256 x 32 tile power-of-two scaled FP16, leading dimension
 // m16_ld(n), m16_scale_count(n) float scales) and y = alpha A16 x + beta y with it (FP32 sums within a
 // tile, FP64 across; ws as for gemv_f64)
 long m16_ld(long n);

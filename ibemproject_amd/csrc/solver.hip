@@ -2508,12 +2508,14 @@ void fill_zero(void* p, size_t bytes, hipStream_t st) {
 // y = alpha A x + beta y. The columns are split into kGemvChunks ranges so that enough loads are in
 // flight to stream A at HBM speed (one row per thread over all n columns keeps too few); the range
 // partial sums are added in a fixed order by a second kernel.
-constexpr int kGemvChunks = 32;
+constexpr int kGemvChunks = 64;
 size_t gemv_workspace_doubles(long m) { return (size_t)kGemvChunks * m; }
 
 // thread = rows 2 u and 2 u + 1 (one 16-byte load per column when lda is even and A 16-byte aligned,
 // else one row per thread); each row keeps eight partial sums over its columns j = q (mod 8) of the
-// chunk, added in a fixed order
+// chunk, added in a fixed order. A is streamed with nontemporal loads (read once per product; keeping
+// it out of the caches measured 3-5 % faster).
+typedef double dvec2 __attribute__((ext_vector_type(2)));
 __global__ __launch_bounds__(256) void k_gemv_part(long m, long n, const double* __restrict__ A,
                                                    long lda, const double* __restrict__ x,
                                                    double* __restrict__ part) {
@@ -2527,12 +2529,14 @@ __global__ __launch_bounds__(256) void k_gemv_part(long m, long n, const double*
       double a0[8] = {0, 0, 0, 0, 0, 0, 0, 0}, a1[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       long j = j0;
       for (; j + 8 <= j1; j += 8) {
+        dvec2 v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = __builtin_nontemporal_load((const dvec2*)(A + (j + q) * lda + r));
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-          const double2 v = *(const double2*)(A + (j + q) * lda + r);
           const double xq = x[j + q];
-          a0[q] += v.x * xq;
-          a1[q] += v.y * xq;
+          a0[q] += v[q].x * xq;
+          a1[q] += v[q].y * xq;
         }
       }
       for (; j < j1; ++j) {
@@ -2604,27 +2608,43 @@ void convert_f64_to_f32(const double* A, long n, long lda, float* B, long ldb, h
 
 // ---- 16-bit copy of A for the inner matrix-vector products of block-Jacobi GMRES -------------------
 // A16[j * ld16 + i] = fp16(A[j * lda + i] 2^-e), one power-of-two exponent e per tile of kM16R rows x
-// kM16C columns, chosen so the tile's largest magnitude lands in [2^13, 2^14) (an FP16 significand of
+// kM16C columns (256 x 32: a thread of the build holds 32 values), chosen so the tile's largest magnitude lands in [2^13, 2^14) (an FP16 significand of
 // 11 bits for everything within 2^27 of it); scale[ct * row_tiles + rt] = 2^e. Rows m <= i < ld16 hold
 // zeros, so the product reads whole 16-byte row groups.
-constexpr int kM16R = 256, kM16C = 64;
+constexpr int kM16R = 256, kM16C = 32;
 long m16_ld(long n) { return (n + kM16R - 1) / kM16R * kM16R; }
 size_t m16_scale_count(long n) { return (size_t)(m16_ld(n) / kM16R) * (size_t)((n + kM16C - 1) / kM16C); }
 
+// one workgroup per tile: thread t holds rows 2 (t % 128) and 2 (t % 128) + 1 of the 32 columns
+// c = t / 128 (mod 2) (16-byte nontemporal loads of A, read once), the tile maximum by wavefront and LDS
+// reductions, then the FP16 pairs written as 4-byte stores
 __global__ __launch_bounds__(256) void k_m16_build(const double* __restrict__ A, long m, long n, long lda,
                                                    _Float16* __restrict__ A16, long ld16,
                                                    float* __restrict__ scale) {
   __shared__ double wmax[4];
   const long rt = blockIdx.x, ct = blockIdx.y;
-  const long i = rt * kM16R + threadIdx.x;
+  const long i = rt * kM16R + 2 * (threadIdx.x & 127);
+  const int c0 = threadIdx.x >> 7;  // this thread's columns: c0, c0 + 2, ...
   const long j0 = ct * kM16C;
   const int nc = (int)min((long)kM16C, n - j0);
-  double v[kM16C];
+  const bool pair = ((lda & 1) == 0) && ((((size_t)A) & 15) == 0) && i + 1 < m;
+  dvec2 v[kM16C / 2];
   double mx = 0.0;
 #pragma unroll
-  for (int c = 0; c < kM16C; ++c) {
-    v[c] = (c < nc && i < m) ? A[(j0 + c) * lda + i] : 0.0;
-    mx = fmax(mx, fabs(v[c]));
+  for (int u = 0; u < kM16C / 2; ++u) {
+    const int c = c0 + 2 * u;
+    dvec2 w = {0.0, 0.0};
+    if (c < nc) {
+      const double* p = A + (j0 + c) * lda + i;
+      if (pair) {
+        w = __builtin_nontemporal_load((const dvec2*)p);
+      } else {
+        if (i < m) w.x = p[0];
+        if (i + 1 < m) w.y = p[1];
+      }
+    }
+    v[u] = w;
+    mx = fmax(mx, fmax(fabs(w.x), fabs(w.y)));
   }
   for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
   if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
@@ -2635,9 +2655,17 @@ __global__ __launch_bounds__(256) void k_m16_build(const double* __restrict__ A,
   const int e = mx > 0 ? ex - 14 : 0;
   const double down = ldexp(1.0, -e);
   if (threadIdx.x == 0) scale[ct * gridDim.x + rt] = ldexpf(1.0f, e);
+  typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 #pragma unroll
-  for (int c = 0; c < kM16C; ++c)
-    if (c < nc) A16[(j0 + c) * ld16 + i] = (_Float16)(float)(v[c] * down);
+  for (int u = 0; u < kM16C / 2; ++u) {
+    const int c = c0 + 2 * u;
+    if (c < nc) {
+      half2v h;
+      h.x = (_Float16)(float)(v[u].x * down);
+      h.y = (_Float16)(float)(v[u].y * down);
+      *(half2v*)(A16 + (j0 + c) * ld16 + i) = h;
+    }
+  }
 }
 
 void m16_build(const double* A, long n, long lda, unsigned short* A16, long ld16, float* scale, hipStream_t st) {
@@ -2646,9 +2674,10 @@ void m16_build(const double* A, long n, long lda, unsigned short* A16, long ld16
 }
 
 // part[chunk * m + r] = sum over the chunk's columns of A16[r, j] 2^e x_j: a thread holds 8 consecutive
-// rows (one 16-byte load per column), FP32 sums within a 64-column tile, FP64 across tiles; the chunk
-// partial sums are added in a fixed order by k_gemv_done
-constexpr int kM16Chunks = 32;
+// rows (one 16-byte nontemporal load per column), FP32 sums within a 64-column tile, FP64 across tiles;
+// the chunk partial sums are added in a fixed order by k_gemv_done_n. (Measured at n = 160,374: 16
+// chunks with nontemporal loads 6.8 TB/s; 32 chunks with cached loads 6.1; 16 rows per thread 4.2.)
+constexpr int kM16Chunks = 16;
 __global__ __launch_bounds__(256) void k_gemv16_part(long m, long n, const _Float16* __restrict__ A16, long ld16,
                                                      const float* __restrict__ scale, long row_tiles,
                                                      const double* __restrict__ x, double* __restrict__ part) {
@@ -2666,7 +2695,7 @@ __global__ __launch_bounds__(256) void k_gemv16_part(long m, long n, const _Floa
     for (; j + 8 <= j1; j += 8) {
       half8 h[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) h[q] = *(const half8*)(A16 + (j + q) * ld16 + r0);
+      for (int q = 0; q < 8; ++q) h[q] = __builtin_nontemporal_load((const half8*)(A16 + (j + q) * ld16 + r0));
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const float xq = (float)x[j + q];

@@ -79,7 +79,7 @@ enum {
                                    (ibem3d_solve_fell_back reports it) */
   IBEM3D_SOLVE_BJ_GMRES16 = 6,  /* BJ_GMRES whose outer FP64 GMRES (FP64 residuals and products with
                                    A) is preconditioned by a few inner block-Jacobi GMRES iterations
-                                   on a 16-bit copy of A (FP16, every 256 x 64 tile scaled by a power
+                                   on a 16-bit copy of A (FP16, every 256 x 32 tile scaled by a power
                                    of two; 2 n^2 bytes beside A, read at a quarter of A's bytes per
                                    product). Without room for the copy it runs as BJ_GMRES. Same
                                    fallback as BJ_GMRES. */
