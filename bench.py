@@ -254,12 +254,13 @@ def contrasts(args, hip):
         eng = Engine(prob)
         (rel, it, s), t, tm = timed(eng, hip, lambda: run_step(eng, MODES[args.mode]))
         _, _, x = eng.getUTVector()
+        inner = eng.inner_iterations() if args.mode == "bj_gmres16" else None
         (rel64, _, s64), t64, _ = timed(eng, hip, lambda: run_step(eng, SOLVE_FP64_LU))
         rel64 = eng.residual()
         _, _, x64 = eng.getUTVector()
         eng.close()
         rec = {"E1_over_E0": ratio, "nu1": prob.nu1, "step_s": t, "rel_residual": rel,
-               "iterations": it, "fp64_step_s": t64, "fp64_rel_residual": rel64,
+               "iterations": it, "inner_iterations": inner, "fp64_step_s": t64, "fp64_rel_residual": rel64,
                "x_relerr_vs_fp64_lu": float(np.linalg.norm(x - x64) / np.linalg.norm(x64)),
                "stress_relerr_vs_fp64_lu": float(np.abs(s - s64).max() / np.abs(s64).max())}
         progress(f"contrast {ratio:g}: {rec}")

@@ -7,7 +7,7 @@
 // 41-45), postprocess (getFlux -> stress.txt: postProcessorElastic3D.cpp:525-587) -- on one GPU
 // through the C ABI.
 //
-//   ibem3d_run [dir] [--out DIR] [--mode fp64|mixed|mixed16] [--tol T] [--device D] [--full] [--timings]
+//   ibem3d_run [dir] [--out DIR] [--mode fp64|mixed|mixed16|fp16_1|bf16|bj|bj16] [--tol T] [--device D] [--full] [--timings]
 //
 // Writes DIR/stress.txt in the reference's format; --full also writes stress_full.txt (%.17g).
 // Exit status 0 on success, 1 on a usage or input error, 2 on an engine error.
@@ -24,7 +24,7 @@ namespace {
 
 int usage() {
   std::fprintf(stderr,
-               "usage: ibem3d_run [dir] [--out DIR] [--mode fp64|mixed|mixed16] [--tol T] [--device D] "
+               "usage: ibem3d_run [dir] [--out DIR] [--mode fp64|mixed|mixed16|fp16_1|bf16|bj|bj16] [--tol T] [--device D] "
                "[--full] [--timings]\n");
   return 1;
 }
@@ -63,6 +63,14 @@ int main(int argc, char** argv) {
         mode = IBEM3D_SOLVE_MIXED_FP32;
       else if (m == "mixed16")
         mode = IBEM3D_SOLVE_MIXED_FP16;
+      else if (m == "fp16_1")
+        mode = IBEM3D_SOLVE_MIXED_FP16_1;
+      else if (m == "bf16")
+        mode = IBEM3D_SOLVE_MIXED_BF16;
+      else if (m == "bj")
+        mode = IBEM3D_SOLVE_BJ_GMRES;
+      else if (m == "bj16")
+        mode = IBEM3D_SOLVE_BJ_GMRES16;
       else
         return usage();
     } else if (a == "--tol") {

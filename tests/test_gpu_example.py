@@ -6,8 +6,8 @@ import os
 import numpy as np
 import pytest
 
-from ibemproject_amd.engine import (SOLVE_FP64_LU, SOLVE_GMRES, SOLVE_MIXED_FP16, SOLVE_MIXED_FP32,
-                                    Engine, run_reference_phases)
+from ibemproject_amd.engine import (SOLVE_BJ_GMRES, SOLVE_BJ_GMRES16, SOLVE_FP64_LU, SOLVE_GMRES,
+                                    SOLVE_MIXED_FP16, SOLVE_MIXED_FP32, Engine, run_reference_phases)
 from ibemproject_amd.formats import read_problem, read_stress
 
 pytestmark = pytest.mark.gpu
@@ -18,7 +18,8 @@ def case(msg):
     print(f"[case] {msg}", flush=True)
 
 
-@pytest.mark.parametrize("mode", [SOLVE_FP64_LU, SOLVE_MIXED_FP32, SOLVE_MIXED_FP16])
+@pytest.mark.parametrize("mode", [SOLVE_FP64_LU, SOLVE_MIXED_FP32, SOLVE_MIXED_FP16, SOLVE_BJ_GMRES,
+                                  SOLVE_BJ_GMRES16])
 def test_reference_example(mode):
     """U, T and the stresses against the reference's own solution: to 1e-10 on the FP64 path, 1e-8
     on the refined mixed-precision paths (measured: U 6e-15, T 1.2e-13, stresses 1.1e-14 on every
@@ -34,6 +35,8 @@ def test_reference_example(mode):
     rel, it = eng.solve(mode, 1e-12)
     U, T, X = eng.getUTVector()
     s = eng.getFlux()
+    if mode in (SOLVE_BJ_GMRES, SOLVE_BJ_GMRES16):
+        assert not eng.fell_back()
     eng.close()
     eu = np.linalg.norm(U - ref["U"]) / np.linalg.norm(ref["U"])
     et = np.linalg.norm(T - ref["T"]) / np.linalg.norm(ref["T"])
@@ -91,16 +94,18 @@ def test_workload_properties():
     eng.close()
 
 
-def test_native_driver_on_reference_example(tmp_path):
+@pytest.mark.parametrize("mode", ["fp64", "bj16"])
+def test_native_driver_on_reference_example(tmp_path, mode):
     """ibem3d_run (the native driver: read txt inputs -> all phases -> solve -> stress.txt) on the
-    reference's example directory."""
+    reference's example directory, with the plain FP64 LU and with block-Jacobi GMRES on the 16-bit
+    inner matrix."""
     import shutil
     import subprocess
     for f in ("BEM.txt", "position.txt", "postprocess.txt", "calculationType.txt"):
         shutil.copy(os.path.join(GOLD, f), tmp_path / f)
     exe = os.path.join(os.path.dirname(GOLD), "..", "..", "ibemproject_amd", "ibem3d_run")
-    case("native driver on the reference example")
-    r = subprocess.run([exe, str(tmp_path), "--mode", "fp64", "--full", "--timings"],
+    case(f"native driver on the reference example, mode {mode}")
+    r = subprocess.run([exe, str(tmp_path), "--mode", mode, "--full", "--timings"],
                        capture_output=True, text=True, timeout=300)
     print(r.stderr)
     assert r.returncode == 0
