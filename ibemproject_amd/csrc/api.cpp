@@ -64,6 +64,13 @@ struct ibem3d_ctx {
   // (nb x nb, nb = 3 NN) with its LU workspace, and the inverse of every sphere's 60 x 60 diagonal block
   double *d_Abb = nullptr, *d_rb = nullptr, *d_cb = nullptr, *d_Binv = nullptr, *d_tb = nullptr;
   LuWork Wb;
+  // after a block-Jacobi solve the next assembly factors the boundary block and inverts the sphere
+  // blocks on a second stream, beside the assembly kernels that do not write them (st_aux, events
+  // in / out); pending: launched and not yet joined by the main stream; ready: the preconditioner
+  // matches the assembled A
+  hipStream_t st_aux = nullptr;
+  hipEvent_t ev_aux_in = nullptr, ev_aux_out = nullptr;
+  bool bj_pending = false, bj_ready = false;
   // IBEM3D_SOLVE_BJ_GMRES16: the 16-bit copy of A (leading dimension ld16) with its tile scales, and
   // the inner GMRES's basis, directions and product vector
   unsigned short* d_A16 = nullptr;
@@ -175,8 +182,9 @@ void fold_times(ibem3d_ctx* c) {
 struct Scope {
   ibem3d_ctx* c;
   const char* n;
-  Scope(ibem3d_ctx* c_, const char* n_) : c(c_), n(n_) { tick(c, n, 0, nullptr); }
-  ~Scope() { tick(c, n, 1, nullptr); }
+  hipStream_t s;
+  Scope(ibem3d_ctx* c_, const char* n_, hipStream_t s_ = nullptr) : c(c_), n(n_), s(s_) { tick(c, n, 0, s); }
+  ~Scope() { tick(c, n, 1, s); }
 };
 
 template <typename T>
@@ -333,6 +341,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
   if (!ctx) return;
   hipSetDevice(ctx->dev);
   if (ctx->st) hipStreamSynchronize(ctx->st);
+  if (ctx->st_aux) hipStreamSynchronize(ctx->st_aux);
   ibem3d_reset_timings(ctx);
   for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
   ctx->pool.clear();
@@ -351,6 +360,9 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->W.d_trinv, ctx->W.d_trsm};
   for (void* q : ptrs)
     if (q) hipFree(q);
+  if (ctx->st_aux) hipStreamDestroy(ctx->st_aux);
+  for (hipEvent_t e : {ctx->ev_aux_in, ctx->ev_aux_out})
+    if (e) hipEventDestroy(e);
   if (ctx->W.st2) hipStreamDestroy(ctx->W.st2);
   if (ctx->Wb.st2) hipStreamDestroy(ctx->Wb.st2);
   for (hipEvent_t e : {ctx->Wb.ev_a, ctx->Wb.ev_b, ctx->Wb.ev_r, ctx->Wb.ev_cols, ctx->W.ev_r, ctx->W.ev_cols})
@@ -453,20 +465,59 @@ int ibem3d_apply_bc(ibem3d_ctx* ctx) {
   return 0;
 }
 
+// the block-Jacobi preconditioner's setup (defined with the solver below)
+static int bj_alloc(ibem3d_ctx* ctx);
+static int bj_enqueue_boundary(ibem3d_ctx* ctx, hipStream_t st);
+static void bj_enqueue_blocks(ibem3d_ctx* ctx, hipStream_t st);
+static int join_aux(ibem3d_ctx* ctx);
+
 int ibem3d_assemble(ibem3d_ctx* ctx) {
+  CTX_CHECK();
   int rc;
+  // After a block-Jacobi solve the next one needs the same preconditioner of the new A: its boundary
+  // block (written only by the boundary phase) is factored, and its sphere blocks (written only by
+  // the inclusion phase) inverted, on a second stream while the other phases run.
+  const bool bj_next = ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES || ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES16;
+  if (bj_next) {
+    if ((rc = join_aux(ctx))) return rc;
+    if (!ctx->st_aux) {
+      HIPC(hipStreamCreateWithFlags(&ctx->st_aux, hipStreamNonBlocking));
+      HIPC(hipEventCreateWithFlags(&ctx->ev_aux_in, hipEventDisableTiming));
+      HIPC(hipEventCreateWithFlags(&ctx->ev_aux_out, hipEventDisableTiming));
+    }
+    if ((rc = bj_alloc(ctx))) return rc;
+  }
   if ((rc = ibem3d_add_bem(ctx))) return rc;
+  if (bj_next) {
+    HIPC(hipEventRecord(ctx->ev_aux_in, ctx->st));
+    HIPC(hipStreamWaitEvent(ctx->st_aux, ctx->ev_aux_in, 0));
+    Scope sc(ctx, "assemble.bj_boundary_aux", ctx->st_aux);
+    if ((rc = bj_enqueue_boundary(ctx, ctx->st_aux))) return rc;
+  }
   if ((rc = ibem3d_add_field(ctx))) return rc;
   // all three eigenstrain orders in one pass each: every row belongs to one order, so the entries
   // are the ones the per-order calls write, and the per-pair tables are built once instead of three
   // times
   if ((rc = ibem3d_add_incl(ctx, -1))) return rc;
+  if (bj_next) {
+    HIPC(hipEventRecord(ctx->ev_aux_in, ctx->st));
+    HIPC(hipStreamWaitEvent(ctx->st_aux, ctx->ev_aux_in, 0));
+    {
+      Scope sc(ctx, "assemble.bj_blocks_aux", ctx->st_aux);
+      bj_enqueue_blocks(ctx, ctx->st_aux);
+    }
+    HIPC(hipEventRecord(ctx->ev_aux_out, ctx->st_aux));
+    ctx->bj_pending = true;
+    if ((rc = after_launch(ctx, "block-Jacobi setup beside the assembly"))) return rc;
+  }
   if ((rc = ibem3d_add_bem_rows(ctx, -1))) return rc;
   return ibem3d_apply_bc(ctx);
 }
 
 int ibem3d_reset(ibem3d_ctx* ctx) {
   CTX_CHECK();
+  if (int rc = join_aux(ctx)) return rc;  // the second stream may still read A's boundary block
+  ctx->bj_ready = false;
   const long n = ctx->n;
   fill_zero(ctx->d_A, sizeof(double) * (size_t)n * 3 * ctx->NN, ctx->st);
   fill_zero(ctx->d_b, sizeof(double) * n, ctx->st);
@@ -524,7 +575,8 @@ static int ensure_solver_ws(ibem3d_ctx* ctx) {
   const long n = ctx->n;
   if (!ctx->W.d_ipiv) {
     if (int rc = alloc_luwork(ctx, ctx->W, n)) return rc;
-    if (int rc = dalloc(ctx, &ctx->d_eqws, equilibrate_workspace_doubles(n))) return rc;
+    if (!ctx->d_eqws)
+      if (int rc = dalloc(ctx, &ctx->d_eqws, equilibrate_workspace_doubles(n))) return rc;
     if (int rc = dalloc(ctx, &ctx->d_r, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_c, (size_t)n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_work, (size_t)n)) return rc;
@@ -699,8 +751,8 @@ static void bj_apply(ibem3d_ctx* ctx, const double* v, double* z) {
 
 // the preconditioner from the assembled A: boundary block copied, equilibrated and factored; every
 // sphere block inverted
-static int bj_setup(ibem3d_ctx* ctx) {
-  const long n = ctx->n, nb = 3L * ctx->NN;
+static int bj_alloc(ibem3d_ctx* ctx) {
+  const long nb = 3L * ctx->NN;
   if (!ctx->d_Abb) {
     if (int rc = dalloc(ctx, &ctx->d_Abb, (size_t)nb * nb)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_rb, (size_t)nb)) return rc;
@@ -708,22 +760,54 @@ static int bj_setup(ibem3d_ctx* ctx) {
     if (int rc = dalloc(ctx, &ctx->d_tb, (size_t)nb)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_Binv, (size_t)kBlk * kBlk * std::max(ctx->num, 1))) return rc;
     if (int rc = alloc_luwork(ctx, ctx->Wb, nb)) return rc;
+    if (!ctx->d_eqws)
+      if (int rc = dalloc(ctx, &ctx->d_eqws, equilibrate_workspace_doubles(ctx->n))) return rc;
   }
-  {
-    Scope s(ctx, "solve.bj_setup");
-    equilibrate_copy_f64(ctx->d_A, nb, n, ctx->d_rb, ctx->d_cb, ctx->d_Abb, nb, ctx->d_eqws, ctx->st);
-    if (lu_factor_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->st, nullptr, nullptr))
-      return fail(ctx, IBEM3D_EHIP, "boundary block LU launch failed");
-    if (lu_prepare_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->st))
-      return fail(ctx, IBEM3D_EHIP, "boundary block solve preparation failed");
-    if (ctx->num > 0)
-      hipLaunchKernelGGL(k_block_inv, dim3(ctx->num), dim3(256), 0, ctx->st, ctx->d_A, n, nb, ctx->d_Binv);
-  }
+  return 0;
+}
+
+// the boundary block's copy, equilibration and LU on stream st (reads A's first 3 NN rows and columns)
+static int bj_enqueue_boundary(ibem3d_ctx* ctx, hipStream_t st) {
+  const long n = ctx->n, nb = 3L * ctx->NN;
+  equilibrate_copy_f64(ctx->d_A, nb, n, ctx->d_rb, ctx->d_cb, ctx->d_Abb, nb, ctx->d_eqws, st);
+  if (lu_factor_f64(ctx->d_Abb, nb, nb, ctx->Wb, st, nullptr, nullptr))
+    return fail(ctx, IBEM3D_EHIP, "boundary block LU launch failed");
+  if (lu_prepare_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, st))
+    return fail(ctx, IBEM3D_EHIP, "boundary block solve preparation failed");
+  return 0;
+}
+
+// every sphere block's inverse on stream st (reads the sphere blocks of A's diagonal)
+static void bj_enqueue_blocks(ibem3d_ctx* ctx, hipStream_t st) {
+  if (ctx->num > 0)
+    hipLaunchKernelGGL(k_block_inv, dim3(ctx->num), dim3(256), 0, st, ctx->d_A, ctx->n, 3L * ctx->NN, ctx->d_Binv);
+}
+
+static int bj_check(ibem3d_ctx* ctx) {
   int info = 0;
   HIPC(hipMemcpyAsync(&info, ctx->Wb.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
   HIPC(hipStreamSynchronize(ctx->st));
   if (info) return fail(ctx, IBEM3D_ESOLVE, "boundary block LU: zero pivot");
+  ctx->bj_ready = true;
   return after_launch(ctx, "block-Jacobi setup");
+}
+
+// the main stream takes over the setup launched beside the assembly
+static int join_aux(ibem3d_ctx* ctx) {
+  if (!ctx->bj_pending) return 0;
+  ctx->bj_pending = false;
+  HIPC(hipStreamWaitEvent(ctx->st, ctx->ev_aux_out, 0));
+  return bj_check(ctx);
+}
+
+static int bj_setup(ibem3d_ctx* ctx) {
+  if (int rc = bj_alloc(ctx)) return rc;
+  {
+    Scope s(ctx, "solve.bj_setup");
+    if (int rc = bj_enqueue_boundary(ctx, ctx->st)) return rc;
+    bj_enqueue_blocks(ctx, ctx->st);
+  }
+  return bj_check(ctx);
 }
 
 // v <- (LU)^-1 v with the factors of the prepared mixed mode: the FP32 factor, or its 16-bit copy
@@ -1131,8 +1215,9 @@ static int solve_bj(ibem3d_ctx* ctx, double tol, bool reuse, int mode) {
   ctx->bj_fallback = false;
   if (!reuse || ctx->prepared_mode != mode) {
     ctx->prepared_mode = -1;
-    if (int rc = bj_setup(ctx)) return rc;
     ctx->m16_skipped = false;
+    // the FP16 copy reads A only: it is built while the second stream may still be finishing the
+    // block-Jacobi setup launched beside the assembly (which reads A too)
     if (mode == IBEM3D_SOLVE_BJ_GMRES16) {
       const long ld16 = m16_ld(n);
       const size_t need = ctx->d_A16 ? 0
@@ -1164,7 +1249,12 @@ static int solve_bj(ibem3d_ctx* ctx, double tol, bool reuse, int mode) {
         ctx->m16_skipped = true;
       }
     }
+    if (int rc = join_aux(ctx)) return rc;
+    if (!ctx->bj_ready)  // else built beside the assembly of this A
+      if (int rc = bj_setup(ctx)) return rc;
     ctx->prepared_mode = mode;
+  } else if (int rc = join_aux(ctx)) {
+    return rc;
   }
   int rc;
   if (ctx->has_x0 && ctx->num > 0) {
@@ -1236,6 +1326,8 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
     compact = need > 0 && !fits(need);
   }
   if (ctx->phases != 0xFFu) return fail(ctx, IBEM3D_ESTATE, "solve: system not assembled");
+  if (!bj)  // the block-Jacobi modes join the second stream themselves, after the FP16 copy's build
+    if (int rc = join_aux(ctx)) return rc;
   if (ctx->a_factored && !(reuse && mode == IBEM3D_SOLVE_FP64_LU))
     return fail(ctx, IBEM3D_ESTATE, "solve: A holds a factor (compact solve); assemble again");
   if (int rc = ensure_solver_ws(ctx)) return rc;

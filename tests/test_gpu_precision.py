@@ -203,3 +203,35 @@ def test_block_jacobi_gmres16(mid):
     rel_l, _ = eng.solve(E.SOLVE_MIXED_FP16_1, 1e-10)
     assert rel_l <= 1e-10
     eng.close()
+
+
+def test_block_jacobi_setup_beside_assembly(mid):
+    """After a block-Jacobi solve the next assembly builds the preconditioner on a second stream: the
+    next solves reproduce the first bit for bit, also after a reset that follows such an assembly
+    directly, and an LU mode after it still solves."""
+    case(f"block-Jacobi setup beside the assembly, n60={mid['prob'].n60}")
+    eng = E.Engine(mid["prob"])
+    xs = []
+    for step in range(3):
+        eng.reset()
+        eng.assemble()
+        if step == 1:  # a reset straight after the speculative assembly, then assemble again
+            eng.reset()
+            eng.assemble()
+        rel, it = eng.solve(E.SOLVE_BJ_GMRES16, 1e-10)
+        _, _, x = eng.getUTVector()
+        xs.append((rel, it, eng.inner_iterations(), x))
+        assert rel <= 1e-10 and not eng.fell_back()
+    tm = eng.timings()
+    print("steps", [(r, i, k) for r, i, k, _ in xs], "aux phases",
+          {k: round(v, 2) for k, v in tm.items() if k.endswith("_aux")})
+    assert "assemble.bj_boundary_aux" in tm and "assemble.bj_blocks_aux" in tm
+    for r, i, k, x in xs[1:]:
+        assert (r, i, k) == xs[0][:3] and np.array_equal(x, xs[0][3])
+    eng.reset()
+    eng.assemble()
+    rel, _ = eng.solve(E.SOLVE_MIXED_FP16_1, 1e-10)
+    assert rel <= 1e-10
+    _, _, x = eng.getUTVector()
+    assert np.linalg.norm(x - xs[0][3]) / np.linalg.norm(xs[0][3]) < 1e-8
+    eng.close()
