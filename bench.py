@@ -30,15 +30,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from ibemproject_amd.engine import (SOLVE_BJ_GMRES, SOLVE_BJ_GMRES16, SOLVE_FP64_LU,  # noqa: E402
-                                    SOLVE_MIXED_BF16, SOLVE_MIXED_FP16, SOLVE_MIXED_FP16_1, SOLVE_MIXED_FP32,
-                                    Engine)
+                                    SOLVE_MIXED_BF16, SOLVE_MIXED_BF16X3, SOLVE_MIXED_FP16, SOLVE_MIXED_FP16_1,
+                                    SOLVE_MIXED_FP32, Engine)
 from ibemproject_amd.generate import make_workload  # noqa: E402
 
 MI355X_PEAK = {"f16_mfma_tflops": 2500.0, "f32_mfma_tflops": 157.3, "f64_mfma_tflops": 78.6,
                "hbm_gbs": 8000.0}
 NB = 1024  # LU block size (solver.hip IBEM_LU_NB)
 DEFAULT_WORKLOAD = "rve2500_m24"
-MODES = {"bj_gmres16": SOLVE_BJ_GMRES16, "bj_gmres": SOLVE_BJ_GMRES, "fp16_1": SOLVE_MIXED_FP16_1, "fp16x3": SOLVE_MIXED_FP16, "bf16": SOLVE_MIXED_BF16,
+MODES = {"bj_gmres16": SOLVE_BJ_GMRES16, "bj_gmres": SOLVE_BJ_GMRES, "fp16_1": SOLVE_MIXED_FP16_1, "fp16x3": SOLVE_MIXED_FP16, "bf16": SOLVE_MIXED_BF16, "bf16x3": SOLVE_MIXED_BF16X3,
          "fp32": SOLVE_MIXED_FP32}
 # the trailing-update kernel of each mode (k_gemm_f16x3_256 / k_gemm_f16x3 instantiated for the
 # product type) and its MFMA products per algorithmic multiply-add
@@ -48,6 +48,8 @@ GEMM_KERNEL = {"fp16_1": ("k_gemm_f16x3_256<kProdF16> (LU trailing update: persi
                           "128x128 for updates under 512 tiles)", 1),
                "fp16x3": ("k_gemm_f16x3_256<kProdF16x3> (three FP16 products of hi/lo splits)", 3),
                "bf16": ("k_gemm_f16x3_256<kProdBF16> (one v_mfma_f32_16x16x32_bf16 product)", 1),
+               "bf16x3": ("k_gemm_f16x3_256<kProdBF16x3> (three v_mfma_f32_16x16x32_bf16 products of "
+                          "hi/lo BF16 splits)", 3),
                "fp32": ("k_gemm_sub<float> (FP32 MFMA)", 1)}
 # HBM bytes per trailing-update launch (FETCH_SIZE x f + WRITE_SIZE x 1.0 per dispatch, averaged over
 # one LU's launches, f the factor that makes the counter read 1 GiB for a 1 GiB copy in the same
@@ -545,7 +547,7 @@ def main():
     lu_gemm_ms = (lu_side["phase_ms"].get("lu.gemm", 0.0) if lu_side else gemm_ms)
     lu_achieved = flops / (lu_gemm_ms * 1e-3) / 1e12 if lu_gemm_ms else None
     kname, prods = GEMM_KERNEL[lu_mode]
-    op_bytes = 4 if lu_mode in ("fp16x3", "fp32") else 2
+    op_bytes = 4 if lu_mode in ("fp16x3", "bf16x3", "fp32") else 2
     peak = MI355X_PEAK["f32_mfma_tflops"] if lu_mode == "fp32" else MI355X_PEAK["f16_mfma_tflops"]
     lu_roof = {"kernel": kname, "bound": "mfma", "achieved": lu_achieved, "peak": peak,
                "unit": "TFLOP/s", "frac": lu_achieved / peak if lu_achieved else None,

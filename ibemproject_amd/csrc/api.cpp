@@ -1131,8 +1131,10 @@ static int solve_fp64(ibem3d_ctx* ctx, bool reuse, bool compact) {
 static int solve_mixed(ibem3d_ctx* ctx, int mode, double tol, bool gmres_first, bool reuse, bool compact) {
   const long n = ctx->n;
   if (!(tol > 0)) tol = 1e-10;
-  const int prod = mode == IBEM3D_SOLVE_MIXED_FP16_1 ? kProdF16
-                   : mode == IBEM3D_SOLVE_MIXED_BF16 ? kProdBF16 : kProdF16x3;
+  const int prod = mode == IBEM3D_SOLVE_MIXED_FP16_1   ? kProdF16
+                   : mode == IBEM3D_SOLVE_MIXED_BF16   ? kProdBF16
+                   : mode == IBEM3D_SOLVE_MIXED_BF16X3 ? kProdBF16x3
+                                                       : kProdF16x3;
   const bool split = mode != IBEM3D_SOLVE_MIXED_FP32;
   ctx->lda32 = (n + 63) / 64 * 64;
   if (!ctx->d_v32)
@@ -1311,7 +1313,7 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
   const bool reuse = (mode & IBEM3D_SOLVE_REUSE) != 0;
   bool compact = (mode & IBEM3D_SOLVE_COMPACT) != 0;
   mode &= ~(IBEM3D_SOLVE_GMRES | IBEM3D_SOLVE_REUSE | IBEM3D_SOLVE_COMPACT);
-  if (mode < IBEM3D_SOLVE_FP64_LU || mode > IBEM3D_SOLVE_BJ_GMRES16)
+  if (mode < IBEM3D_SOLVE_FP64_LU || mode > IBEM3D_SOLVE_MIXED_BF16X3)
     return fail(ctx, IBEM3D_EINVAL, "solve: unknown mode");
   const bool bj = mode == IBEM3D_SOLVE_BJ_GMRES || mode == IBEM3D_SOLVE_BJ_GMRES16;
   const long n = ctx->n;
@@ -1719,8 +1721,10 @@ int enqueue(ibem3d_ctx* ctx, int mode, int K, int nloads, const double* d_uvals,
             double* d_rel, double* d_part) {
   const long n = ctx->n, NN = ctx->NN;
   hipStream_t st = ctx->st;
-  const int prod = mode == IBEM3D_SOLVE_MIXED_FP16_1 ? kProdF16
-                   : mode == IBEM3D_SOLVE_MIXED_BF16 ? kProdBF16 : kProdF16x3;
+  const int prod = mode == IBEM3D_SOLVE_MIXED_FP16_1   ? kProdF16
+                   : mode == IBEM3D_SOLVE_MIXED_BF16   ? kProdBF16
+                   : mode == IBEM3D_SOLVE_MIXED_BF16X3 ? kProdBF16x3
+                                                       : kProdF16x3;
   const bool split = mode != IBEM3D_SOLVE_MIXED_FP32;
   HIPC(hipMemcpyAsync(ctx->d_uval, d_uvals, sizeof(double) * 3 * NN, hipMemcpyDeviceToDevice, st));
   // assembly (every phase, all orders at once)

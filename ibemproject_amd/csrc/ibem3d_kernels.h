@@ -47,9 +47,10 @@ void gemm_f64_sub(long M, long N, long K, const double* A, long lda, const doubl
 void gemm_f32_sub(long M, long N, long K, const float* A, long lda, const float* B, long ldb,
                   float* C, long ldc, hipStream_t st);
 // Product types of the split-operand GEMMs: three FP16 MFMA products of hi/lo splits (FP32-level
-// accuracy), one FP16 product, or one BF16 product, each of operands whose rows (A) and columns (B)
-// are scaled by powers of two that bring their largest magnitude into [2^14, 2^15).
-enum { kProdF16x3 = 0, kProdF16 = 1, kProdBF16 = 2 };
+// accuracy), one FP16 product, one BF16 product, or three BF16 products of hi/lo BF16 splits (16
+// significant bits per operand), each of operands whose rows (A) and columns (B) are scaled by
+// powers of two that bring their largest magnitude into [2^14, 2^15).
+enum { kProdF16x3 = 0, kProdF16 = 1, kProdBF16 = 2, kProdBF16x3 = 3 };
 // same in FP32 arithmetic from 16-bit MFMA products (K <= 1024); ws holds
 // f16x3_workspace_doubles(max(M, N)) doubles
 size_t f16x3_workspace_doubles(long n);

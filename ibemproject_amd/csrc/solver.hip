@@ -309,10 +309,17 @@ __device__ inline void split2(float x, _Float16& hi, _Float16& lo) {
 // Product types of the split-operand GEMMs (kProd*, ibem3d_kernels.h). BF16 operands are stored
 // as their bit patterns in the same 16-bit arrays.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+// product types with a lo part (three products per multiply-add: hi hi + hi lo + lo hi)
+template <int PROD>
+constexpr bool has_lo() { return PROD == kProdF16x3 || PROD == kProdBF16x3; }
 template <int PROD>
 __device__ inline void split_to(float x, _Float16& hi, _Float16& lo) {
   if constexpr (PROD == kProdBF16) {
     hi = __builtin_bit_cast(_Float16, (__bf16)x);
+  } else if constexpr (PROD == kProdBF16x3) {
+    const __bf16 h = (__bf16)x;
+    hi = __builtin_bit_cast(_Float16, h);
+    lo = __builtin_bit_cast(_Float16, (__bf16)(x - (float)h));
   } else if constexpr (PROD == kProdF16) {
     hi = (_Float16)x;
   } else {
@@ -321,7 +328,7 @@ __device__ inline void split_to(float x, _Float16& hi, _Float16& lo) {
 }
 template <int PROD>
 __device__ inline v4f mma16(half8 a, half8 b, v4f c) {
-  if constexpr (PROD == kProdBF16)
+  if constexpr (PROD == kProdBF16 || PROD == kProdBF16x3)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
                                                    __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
   else
@@ -390,7 +397,7 @@ __global__ __launch_bounds__(256) void k_split_rows(long M, int K, int kpad, con
         l[q] = y;
       }
       *(half8*)(hi + row * kSplitPadK + k0 + k8) = h;
-      if constexpr (PROD == kProdF16x3) *(half8*)(lo + row * kSplitPadK + k0 + k8) = l;
+      if constexpr (has_lo<PROD>()) *(half8*)(lo + row * kSplitPadK + k0 + k8) = l;
     }
   }
 }
@@ -433,7 +440,7 @@ __global__ __launch_bounds__(256) void k_split_cols(long N, int K, int kpad, con
       l[q] = b;
     }
     *(half4*)(hi + c * kSplitPadK + lane * VPL + q4) = h;
-    if constexpr (PROD == kProdF16x3) *(half4*)(lo + c * kSplitPadK + lane * VPL + q4) = l;
+    if constexpr (has_lo<PROD>()) *(half4*)(lo + c * kSplitPadK + lane * VPL + q4) = l;
   }
 }
 
@@ -500,7 +507,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
       const half8 z = {0, 0, 0, 0, 0, 0, 0, 0};
       rAh[sl][e] = ga < M ? *(const half8*)(Ah + ga * kSplitPadK + ko) : z;
       rBh[sl][e] = gb < N ? *(const half8*)(Bh + gb * kSplitPadK + ko) : z;
-      if constexpr (PROD == kProdF16x3) {
+      if constexpr (has_lo<PROD>()) {
         rAl[sl][e] = ga < M ? *(const half8*)(Al + ga * kSplitPadK + ko) : z;
         rBl[sl][e] = gb < N ? *(const half8*)(Bl + gb * kSplitPadK + ko) : z;
       }
@@ -512,7 +519,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
       const int v = tid + 256 * e, row = v / VPR, part = v % VPR;
       *(half8*)(sAh + row * LS + part * 8) = rAh[sl][e];
       *(half8*)(sBh + row * LS + part * 8) = rBh[sl][e];
-      if constexpr (PROD == kProdF16x3) {
+      if constexpr (has_lo<PROD>()) {
         *(half8*)(sAl + row * LS + part * 8) = rAl[sl][e];
         *(half8*)(sBl + row * LS + part * 8) = rBl[sl][e];
       }
@@ -533,25 +540,23 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f16x3(long M, long N, int K,
       for (int i = 0; i < 4; ++i) {
         const int off = (wm + i * 16 + lr) * LS + kk + lk * 8;
         ah[i] = *(const half8*)(sAh + off);
-        if constexpr (PROD == kProdF16x3) al[i] = *(const half8*)(sAl + off);
+        if constexpr (has_lo<PROD>()) al[i] = *(const half8*)(sAl + off);
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int off = (wn + j * 16 + lr) * LS + kk + lk * 8;
         bh[j] = *(const half8*)(sBh + off);
-        if constexpr (PROD == kProdF16x3) bl[j] = *(const half8*)(sBl + off);
+        if constexpr (has_lo<PROD>()) bl[j] = *(const half8*)(sBl + off);
       }
-      if constexpr (PROD == kProdF16x3) {
+      if constexpr (has_lo<PROD>()) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < 4; ++j) acc[i][j] = mma16<PROD>(al[i], bh[j], acc[i][j]);
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < 4; ++j) acc[i][j] = mma16<PROD>(ah[i], bl[j], acc[i][j]);
       }
 #pragma unroll
       for (int i = 0; i < 4; ++i)
@@ -668,12 +673,12 @@ __global__ __launch_bounds__(64 * kBigWaves, 1) void k_gemm_f16x3_256(
     for (int q = 0; q < kBigQ; ++q) {
       __builtin_amdgcn_global_load_lds((const void*)(ah0 + (offA[q] + kb)),
                                        (lds_void_t*)(base + 0 * kBigArr + q * 512), 16, 0, 0);
-      if constexpr (PROD == kProdF16x3)
+      if constexpr (has_lo<PROD>())
         __builtin_amdgcn_global_load_lds((const void*)(al0 + (offA[q] + kb)),
                                          (lds_void_t*)(base + 1 * kBigArr + q * 512), 16, 0, 0);
       __builtin_amdgcn_global_load_lds((const void*)(bh0 + (offB[q] + kb)),
                                        (lds_void_t*)(base + 2 * kBigArr + q * 512), 16, 0, 0);
-      if constexpr (PROD == kProdF16x3)
+      if constexpr (has_lo<PROD>())
         __builtin_amdgcn_global_load_lds((const void*)(bl0 + (offB[q] + kb)),
                                          (lds_void_t*)(base + 3 * kBigArr + q * 512), 16, 0, 0);
     }
@@ -701,17 +706,17 @@ __global__ __launch_bounds__(64 * kBigWaves, 1) void k_gemm_f16x3_256(
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         bh[j] = *(const half8*)(S + 2 * kBigArr + (wn + 16 * j) * kBigK + off);
-        if constexpr (PROD == kProdF16x3) bl[j] = *(const half8*)(S + 3 * kBigArr + (wn + 16 * j) * kBigK + off);
+        if constexpr (has_lo<PROD>()) bl[j] = *(const half8*)(S + 3 * kBigArr + (wn + 16 * j) * kBigK + off);
       }
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         const half8 ah = *(const half8*)(S + (wm + 16 * i) * kBigK + off);
-        if constexpr (PROD == kProdF16x3) {
+        if constexpr (has_lo<PROD>()) {
           const half8 al = *(const half8*)(S + kBigArr + (wm + 16 * i) * kBigK + off);
 #pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < 4; ++j) acc[i][j] = mma16<PROD>(al, bh[j], acc[i][j]);
 #pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < 4; ++j) acc[i][j] = mma16<PROD>(ah, bl[j], acc[i][j]);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = mma16<PROD>(ah, bh[j], acc[i][j]);
@@ -828,6 +833,8 @@ static void f16x3_split_rows(long M, int K, const float* A, long lda, const F16x
     hipLaunchKernelGGL(k_split_rows<kProdF16>, g, b, 0, st, M, K, kpad, A, lda, w.Ah, w.Al, w.ia);
   else if (prod == kProdBF16)
     hipLaunchKernelGGL(k_split_rows<kProdBF16>, g, b, 0, st, M, K, kpad, A, lda, w.Ah, w.Al, w.ia);
+  else if (prod == kProdBF16x3)
+    hipLaunchKernelGGL(k_split_rows<kProdBF16x3>, g, b, 0, st, M, K, kpad, A, lda, w.Ah, w.Al, w.ia);
   else
     hipLaunchKernelGGL(k_split_rows<kProdF16x3>, g, b, 0, st, M, K, kpad, A, lda, w.Ah, w.Al, w.ia);
 }
@@ -840,6 +847,8 @@ static void f16x3_split_cols(long N, int K, const float* B, long ldb, const F16x
     hipLaunchKernelGGL(k_split_cols<kProdF16>, g, b, 0, st, N, K, kpad, B, ldb, h, l, w.ib + c0);
   else if (prod == kProdBF16)
     hipLaunchKernelGGL(k_split_cols<kProdBF16>, g, b, 0, st, N, K, kpad, B, ldb, h, l, w.ib + c0);
+  else if (prod == kProdBF16x3)
+    hipLaunchKernelGGL(k_split_cols<kProdBF16x3>, g, b, 0, st, N, K, kpad, B, ldb, h, l, w.ib + c0);
   else
     hipLaunchKernelGGL(k_split_cols<kProdF16x3>, g, b, 0, st, N, K, kpad, B, ldb, h, l, w.ib + c0);
 }
@@ -929,6 +938,8 @@ static void f16x3_gemm(long M, long N, int K, const F16x3Ws& w, float* C, long l
     f16x3_gemm_t<kProdF16>(M, N, K, w, C, ldc, st, c0, cus);
   else if (prod == kProdBF16)
     f16x3_gemm_t<kProdBF16>(M, N, K, w, C, ldc, st, c0, cus);
+  else if (prod == kProdBF16x3)
+    f16x3_gemm_t<kProdBF16x3>(M, N, K, w, C, ldc, st, c0, cus);
   else
     f16x3_gemm_t<kProdF16x3>(M, N, K, w, C, ldc, st, c0, cus);
 }

@@ -26,6 +26,7 @@ SOLVE_MIXED_FP16_1 = 3   # FP32 LU, trailing updates on one FP16 product (power-
 SOLVE_MIXED_BF16 = 4     # FP32 LU, trailing updates on one BF16 product
 SOLVE_BJ_GMRES = 5       # FP64 GMRES preconditioned by block Jacobi (no n x n factor), LU fallback
 SOLVE_BJ_GMRES16 = 6     # the same, its directions from inner block-Jacobi GMRES on a 16-bit copy of A
+SOLVE_MIXED_BF16X3 = 7   # split BF16 (hi/lo) trailing-update products + FP64 refinement
 SOLVE_GMRES = 0x100  # flag for the mixed modes: GMRES-IR from the start
 SOLVE_REUSE = 0x200  # flag: reuse the factors of the last factorisation in the same mode
 SOLVE_COMPACT = 0x400  # flag: the factor takes A's place (16-bit factor copy for the mixed modes)

@@ -83,6 +83,8 @@ enum {
                                    of two; 2 n^2 bytes beside A, read at a quarter of A's bytes per
                                    product). Without room for the copy it runs as BJ_GMRES. Same
                                    fallback as BJ_GMRES. */
+  IBEM3D_SOLVE_MIXED_BF16X3 = 7,/* MIXED_FP16's three-product split with BF16 hi/lo operands (16
+                                   significant bits each) on BF16 MFMA + FP64 refinement */
   IBEM3D_SOLVE_GMRES = 0x100,   /* flag for the mixed modes: refine by GMRES-IR (FP64 GMRES, right-
                                    preconditioned by the LU factors) from the start. Without it the
                                    refinement is classic and switches to GMRES-IR when an iteration

@@ -1,6 +1,6 @@
 """Factor precisions and storage layouts of the solve (MI355X), against LAPACK on the same assembled
-system: single FP16 and BF16 trailing-update products (per-row / per-column power-of-two scaled
-operands), and compact storage, where the factor takes the place of A (FP64 in place; mixed modes:
+system: single FP16 and BF16 trailing-update products and split (hi/lo) BF16 products (per-row /
+per-column power-of-two scaled operands), and compact storage, where the factor takes the place of A (FP64 in place; mixed modes:
 FP32 in place, then a 16-bit per-tile-scaled copy and A assembled again). Every refined result must
 reach a 1e-10 relative residual and agree with LAPACK to 1e-8; the FP64 path to 1e-10. Each test
 prints its error against the FP64 LU solution."""
@@ -49,7 +49,7 @@ def check_refined(eng, d, rel, it, label):
     return X
 
 
-@pytest.mark.parametrize("mode_name", ["SOLVE_MIXED_FP16_1", "SOLVE_MIXED_BF16"])
+@pytest.mark.parametrize("mode_name", ["SOLVE_MIXED_FP16_1", "SOLVE_MIXED_BF16", "SOLVE_MIXED_BF16X3"])
 def test_single_product_modes(mid, mode_name):
     case(f"single-product LU: {mode_name} n60={mid['prob'].n60}")
     eng = E.Engine(mid["prob"])
@@ -60,7 +60,7 @@ def test_single_product_modes(mid, mode_name):
 
 
 @pytest.mark.parametrize("mode_name", ["SOLVE_MIXED_FP32", "SOLVE_MIXED_FP16", "SOLVE_MIXED_FP16_1",
-                                       "SOLVE_MIXED_BF16"])
+                                       "SOLVE_MIXED_BF16", "SOLVE_MIXED_BF16X3"])
 def test_compact_mixed_modes(mid, mode_name):
     """Compact storage: the FP32 factor overwrites A in place, its 16-bit copy preconditions FP64
     GMRES against the re-assembled A, which must equal the first assembly bit for bit."""

@@ -14,7 +14,8 @@ from ibemproject_amd.formats import read_problem
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = ["contrast2", "contrast100", "rve64_m8"]
-MIXED = ["SOLVE_MIXED_FP16_1", "SOLVE_MIXED_FP16", "SOLVE_MIXED_BF16", "SOLVE_MIXED_FP32", "SOLVE_BJ_GMRES"]
+MIXED = ["SOLVE_MIXED_FP16_1", "SOLVE_MIXED_FP16", "SOLVE_MIXED_BF16", "SOLVE_MIXED_BF16X3", "SOLVE_MIXED_FP32",
+         "SOLVE_BJ_GMRES", "SOLVE_BJ_GMRES16"]
 
 
 def case(msg):
@@ -63,7 +64,7 @@ def test_mixed_against_reference(fixture, mode_name):
     print(f"  residual {rel:.2e} iterations {it}")
     assert rel <= 1e-10
     compare(eng, ref, 1e-8, mode_name)
-    if name == "rve64_m8" and mode_name != "SOLVE_BJ_GMRES":  # the compact layout on the same system
+    if name == "rve64_m8" and not mode_name.startswith("SOLVE_BJ"):  # the compact layout on the same system
         eng.reset()
         eng.assemble()
         rel, it = eng.solve(getattr(E, mode_name) | E.SOLVE_COMPACT, 1e-10)
