@@ -596,6 +596,7 @@ def main():
                               "stress_rel": float(np.abs(s - s64).max() / np.abs(s64).max())}
                              if fp64 else None),
         "assembly_ms": asm_ms,
+        "rows_form": "moment" if eng.rows_moment_form() else "pointwise",
         "lu_ms": lu_ms or None,
         "lu_tflops_effective": 2.0 * n ** 3 / 3 / (lu_ms * 1e-3) / 1e12 if lu_ms else None,
         "phase_ms": {k: v / args.steps for k, v in tm.items()},

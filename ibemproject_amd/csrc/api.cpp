@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -88,6 +89,7 @@ struct ibem3d_ctx {
   LuWork W;
   unsigned phases = 0;
   bool solved = false;
+  double* d_cw = nullptr;      // every boundary element planar: k_rows' moment-form weights
   double rel_res = -1.0;
   int iters = 0;
   std::string err;
@@ -332,8 +334,29 @@ ibem3d_ctx* ibem3d_create(const ibem3d_problem* p, int device) {
   fill_zero(ctx->d_A, sizeof(double) * (size_t)n * 3 * p->NN, ctx->st);
   fill_zero(ctx->d_b, sizeof(double) * n, ctx->st);
   launch_qpoints(ctx->d_nodes, ctx->d_conn, p->NE, ctx->d_qp, ctx->st);
-  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->st) != hipSuccess)
-    return bail("ibem3d_create: quadrature setup failed");
+  {
+    // planar elements (one normal at every quadrature point) let k_rows integrate the Kelvin tables
+    // before the row contraction
+    int* d_flag = nullptr;
+    std::vector<int> flag(p->NE, 0);
+    if (hipMalloc((void**)&d_flag, sizeof(int) * flag.size()) != hipSuccess)
+      return bail("ibem3d_create: hipMalloc (element flags)");
+    if (dalloc(ctx, &ctx->d_cw, (size_t)p->NE * 64 * 4)) {
+      hipFree(d_flag);
+      return bail("");
+    }
+    launch_planar(ctx->d_qp, p->NE, d_flag, ctx->d_cw, ctx->st);
+    const bool ok = hipGetLastError() == hipSuccess &&
+                    hipMemcpyAsync(flag.data(), d_flag, sizeof(int) * p->NE, hipMemcpyDeviceToHost,
+                                   ctx->st) == hipSuccess &&
+                    hipStreamSynchronize(ctx->st) == hipSuccess;
+    hipFree(d_flag);
+    if (!ok) return bail("ibem3d_create: quadrature setup failed");
+    if (!std::all_of(flag.begin(), flag.end(), [](int f) { return f == 1; })) {
+      hipFree(ctx->d_cw);
+      ctx->d_cw = nullptr;
+    }
+  }
   return ctx;
 }
 
@@ -345,7 +368,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
   ibem3d_reset_timings(ctx);
   for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
   ctx->pool.clear();
-  void* ptrs[] = {ctx->d_stU,   ctx->d_stT,      ctx->d_sts,
+  void* ptrs[] = {ctx->d_stU,   ctx->d_stT,      ctx->d_sts,      ctx->d_cw,
                   ctx->d_nodes, ctx->d_cen,      ctx->d_rad,      ctx->d_pts,  ctx->d_qp,
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
@@ -381,6 +404,11 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
     }                                                                         \
     if (hipSetDevice(ctx->dev) != hipSuccess) return fail(ctx, IBEM3D_EHIP, "hipSetDevice"); \
   } while (0)
+
+// k_rows' moment-form weights when every element is planar and the pointwise form is not asked for
+static const double* rows_moment_form(const ibem3d_ctx* ctx) {
+  return get_tuning(kTuneRowsPointwise) == 0 ? ctx->d_cw : nullptr;
+}
 
 static int after_launch(ibem3d_ctx* ctx, const char* what) {
   hipError_t e = hipGetLastError();
@@ -449,7 +477,7 @@ int ibem3d_add_bem_rows(ibem3d_ctx* ctx, int order) {
     Scope s(ctx, "assemble.rows");
     launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, ctx->n, 3L * ctx->NN, ctx->M,
                 mask, ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b,
-                ctx->st);
+                ctx->st, true, rows_moment_form(ctx));
   }
   ctx->phases |= (unsigned)mask << 5;
   ctx->solved = false;
@@ -1366,6 +1394,8 @@ int ibem3d_set_initial_guess(ibem3d_ctx* ctx, const double* x_eig) {
 
 int ibem3d_solve_fell_back(const ibem3d_ctx* ctx) { return ctx && ctx->bj_fallback ? 1 : 0; }
 
+int ibem3d_rows_moment_form(const ibem3d_ctx* ctx) { return ctx && rows_moment_form(ctx) ? 1 : 0; }
+
 long ibem3d_inner_iterations(const ibem3d_ctx* ctx) {
   if (!ctx) return -1;
   return ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES16 && ctx->m16_skipped ? -1 : ctx->inner_iters;
@@ -1453,7 +1483,7 @@ int ibem3d_update_known_values(ibem3d_ctx* ctx, const double* bc_values, const d
     if (ctx->num > 0)
       launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, ctx->n, 3L * ctx->NN, ctx->M, 7,
                   ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b,
-                  ctx->st, false);
+                  ctx->st, false, rows_moment_form(ctx));
   }
   ctx->solved = false;
   return after_launch(ctx, "update_known_values");
@@ -1736,7 +1766,7 @@ int enqueue(ibem3d_ctx* ctx, int mode, int K, int nloads, const double* d_uvals,
     launch_field(ctx->d_nodes, ctx->NN, ctx->d_cen, ctx->d_rad, ctx->num, n, 3 * NN, ctx->M, ctx->d_A, st);
     launch_incl(ctx->d_cen, ctx->d_rad, ctx->num, n, 3 * NN, ctx->M, 7, ctx->d_A, st);
     launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, n, 3 * NN, ctx->M, 7, ctx->d_node_col,
-                ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, st);
+                ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, st, true, rows_moment_form(ctx));
   }
   // mixed LU of R A C
   if (equilibrate_copy_f32(ctx->d_A, n, n, ctx->d_r, ctx->d_c, ctx->d_A32, ctx->lda32, ctx->d_eqws, st, &ctx->W))
@@ -1754,7 +1784,8 @@ int enqueue(ibem3d_ctx* ctx, int mode, int K, int nloads, const double* d_uvals,
                  ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, ctx->d_bemws, st, false);
       if (ctx->num > 0)
         launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, n, 3 * NN, ctx->M, 7,
-                    ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, st, false);
+                    ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, st, false,
+                    rows_moment_form(ctx));
     }
     // K steps of classic refinement from x = 0: x <- x + C LU^-1 R (b - A x)
     fill_zero(ctx->d_x, sizeof(double) * n, st);

@@ -485,6 +485,188 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
   if (wv == 0 && active) b[row] += sb[0][lane] + sb[1][lane] + sb[2][lane] + sb[3][lane];
 }
 
+// ---- K2, moment form (every element planar) ------------------------------------------------------
+// Normals within kPlanarTol of each other count as one: replacing them by the first point's changes
+// an entry by at most ~kPlanarTol of its magnitude.
+constexpr double kPlanarTol = 1e-14;
+// On a planar element the unit normal is the same at every Gauss point, so a row's strain kernels
+// su_n, st_n are linear in the Kelvin table with coefficients that do not depend on the point, and
+//   sum_g w_g N_q(g) su_n(tab_g) = su_n(P_q),   P_q[slot] = sum_g w_g N_q(g) tab_g[slot].
+// Per element: (1) the 64 tables as in k_rows; (2) thread = (used slot, half of the Gauss points)
+// forms the four moments P_q of its slot over 32 points (w N_q from the read-only quadrature array,
+// wave-uniform addresses, so scalar loads); (3) lane = row, wavefront = shape function q: one
+// contraction of the moment table (the two halves summed as they are read) gives the row's H and
+// G entries of node q, folded through the column map as in k_rows. The contraction runs 4 times per
+// element and row instead of 64; the Gauss points, weights and tables are the same as k_rows',
+// only the order of the sums differs.
+__global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp,
+                                                const double* __restrict__ cw,
+                                                const int* __restrict__ conn,
+                                                const double* __restrict__ cen, int NE, long n,
+                                                long row0, Mat M, int rowmask,
+                                                const int* __restrict__ node_col,
+                                                const int* __restrict__ slot_col,
+                                                const double* __restrict__ uval,
+                                                const double* __restrict__ tval,
+                                                double* __restrict__ A, double* __restrict__ b,
+                                                int write_A) {
+  constexpr int QN = 64 * kQpStride;
+  constexpr int QPT = (QN + 255) / 256;
+  __shared__ double sq[2][QN];                    // current / next element
+  __shared__ double stab[64 * kKStride];          // the 64 Gauss points' tables
+  __shared__ double smom[2][4][kKStride];         // moments: [half of the points][q][slot]
+  __shared__ int scol[2][24];
+  __shared__ double sknown[2][24];
+  double* sb = stab;  // after the last element: per-wave right-hand-side sums (4 x 64)
+  const int s = blockIdx.x;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  // step (3): lane = row, wave = shape function
+  const Slot R = slot60(lane < 60 ? lane : 59);
+  const bool active = lane < 60 && ((rowmask >> R.order) & 1);
+  const RowTerms RT = row_terms(M, R);
+  const double fac = R.order == 2 ? 0.5 : 1.0;
+  const double c0 = cen[3 * s], c1 = cen[3 * s + 1], c2 = cen[3 * s + 2];
+  const long row = row0 + 60L * s + lane;
+  // step (2): used slot mB, Gauss points 32 half .. 32 half + 31
+  const int half = wv >> 1;
+  const int mB = (wv & 1) * 64 + lane;
+  const bool actB = mB < kKUsed;
+  const int slotB = kelvin_used_slot(actB ? mB : 0);
+  double bacc = 0.0;
+  auto load_cols = [&](int e) {
+    if (t < 24) {
+      const int par = e & 1;
+      const int k = t < 12 ? t : t - 12, q = k / 3, nc = k % 3;
+      if (t < 12) {
+        const int dof = 3 * (conn[4 * e + q] - 1) + nc;
+        const int col = node_col[dof];
+        scol[par][t] = col;
+        sknown[par][t] = col >= 0 ? 0.0 : uval[dof];
+      } else {
+        const int slot = 12 * e + 3 * q + nc;
+        const int kc = slot_col[slot];
+        scol[par][t] = kc;
+        sknown[par][t] = kc == -1 ? tval[slot] : 0.0;
+      }
+    }
+  };
+  if (NE > 0)
+    for (int u = t; u < QN; u += 256) sq[0][u] = qp[u];
+  double qn[QPT];
+  for (int e = 0; e < NE; ++e) {
+    const int cur = e & 1;
+    load_cols(e);
+    if (e + 1 < NE) {
+#pragma unroll
+      for (int k = 0; k < QPT; ++k) {
+        const int u = t + 256 * k;
+        qn[k] = u < QN ? qp[(size_t)(e + 1) * QN + u] : 0.0;
+      }
+    }
+    __syncthreads();  // sq[cur] complete; the previous element's contraction is done with smom
+    {
+      const double* Q = sq[cur] + lane * kQpStride;
+      const double x[3] = {Q[0] - c0, Q[1] - c1, Q[2] - c2};
+      double y0[7], y1[7], y2[7], PR[7], PT[5];
+      kelvin_point(x, y0, y1, y2, PR, PT);
+      double* tab = stab + lane * kKStride;
+      switch (wv) {
+        case 0: kelvin_group<0>(y0, y1, y2, PR, PT, tab); break;
+        case 1: kelvin_group<1>(y0, y1, y2, PR, PT, tab); break;
+        case 2: kelvin_group<2>(y0, y1, y2, PR, PT, tab); break;
+        default: kelvin_group<3>(y0, y1, y2, PR, PT, tab); break;
+      }
+    }
+    __syncthreads();
+    if (actB) {
+      // weights of four points per step (two 64-byte scalar loads, one wait)
+      const double* We = cw + ((size_t)e * 64 + 32 * half) * 4;
+      const double* Te = stab + 32 * half * kKStride + slotB;
+      double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+      for (int gi = 0; gi < 32; gi += 4) {
+        double w[16], v[4];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) w[k] = We[4 * gi + k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = Te[(gi + k) * kKStride];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          p0 += v[k] * w[4 * k];
+          p1 += v[k] * w[4 * k + 1];
+          p2 += v[k] * w[4 * k + 2];
+          p3 += v[k] * w[4 * k + 3];
+        }
+      }
+      smom[half][0][slotB] = p0;
+      smom[half][1][slotB] = p1;
+      smom[half][2][slotB] = p2;
+      smom[half][3][slotB] = p3;
+    }
+    if (e + 1 < NE) {
+#pragma unroll
+      for (int k = 0; k < QPT; ++k) {
+        const int u = t + 256 * k;
+        if (u < QN) sq[cur ^ 1][u] = qn[k];
+      }
+    }
+    __syncthreads();
+    if (active) {
+      const int q = wv;
+      const double* nv = sq[cur] + 3;  // the element's normal (equal at every point)
+      double su[3], st[3];
+      row_gp2_t(RT, TabSum2{smom[0][q], smom[1][q]}, nv, su, st);
+      double old[6];
+      int col[6];
+#pragma unroll
+      for (int nn = 0; nn < 3; ++nn) {
+        col[nn] = scol[cur][3 * q + nn];
+        col[3 + nn] = scol[cur][12 + 3 * q + nn];
+      }
+#pragma unroll
+      for (int o = 0; o < 6; ++o) old[o] = col[o] >= 0 && write_A ? A[(long)col[o] * n + row] : 0.0;
+#pragma unroll
+      for (int nn = 0; nn < 3; ++nn) {
+        const double hv = -fac * su[nn];
+        if (col[nn] >= 0) {
+          if (write_A) A[(long)col[nn] * n + row] = old[nn] + hv;
+        } else {
+          bacc -= hv * sknown[cur][3 * q + nn];
+        }
+        const double gv = fac * st[nn];
+        if (col[3 + nn] >= 0) {
+          if (write_A) A[(long)col[3 + nn] * n + row] = old[3 + nn] - gv;
+        } else {
+          bacc += gv * sknown[cur][12 + 3 * q + nn];  // zero unless the traction is known
+        }
+      }
+    }
+  }
+  // the tables were last read before the barrier that preceded the last contraction
+  sb[wv * 64 + lane] = active ? bacc : 0.0;
+  __syncthreads();
+  if (wv == 0 && active) b[row] += sb[lane] + sb[64 + lane] + sb[128 + lane] + sb[192 + lane];
+}
+
+// flag[e] = 1 when the element's 64 unit normals agree with the first point's to kPlanarTol in every
+// component (a planar element: its normals differ by rounding only), else 0;
+// cw[e][g][q] = w_g jac_g N_q(g), the moment form's weights.
+__global__ __launch_bounds__(64) void k_planar(const double* __restrict__ qp, int NE,
+                                               int* __restrict__ flag, double* __restrict__ cw) {
+  const int e = blockIdx.x;
+  const int g = threadIdx.x;
+  if (e >= NE) return;
+  const double* Q0 = qp + (size_t)e * 64 * kQpStride;
+  const double* Q = Q0 + g * kQpStride;
+  const bool same = fabs(Q[3] - Q0[3]) <= kPlanarTol && fabs(Q[4] - Q0[4]) <= kPlanarTol &&
+                    fabs(Q[5] - Q0[5]) <= kPlanarTol;
+  const bool all = __all(same);
+  if (g == 0) flag[e] = all ? 1 : 0;
+  double* o = cw + ((size_t)e * 64 + g) * 4;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) o[q] = Q[10] * Q[6 + q];
+}
+
 // ---- launchers ----------------------------------------------------------------------------------
 void launch_qpoints(const double* nodes, const int* conn, int NE, double* qp, hipStream_t st) {
   hipLaunchKernelGGL(k_qpoints, dim3(NE), dim3(64), 0, st, nodes, conn, NE, qp);
@@ -526,9 +708,17 @@ void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, 
 void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long n,
                  long row0, const Mat& M, int rowmask, const int* node_col, const int* slot_col,
                  const double* uval, const double* tval, double* A, double* b, hipStream_t st,
-                 bool write_A) {
-  hipLaunchKernelGGL(k_rows, dim3(num), dim3(256), 0, st, qp, conn, cen, NE, n, row0, M, rowmask,
-                     node_col, slot_col, uval, tval, A, b, (int)write_A);
+                 bool write_A, const double* cw) {
+  if (cw)
+    hipLaunchKernelGGL(k_rows_m, dim3(num), dim3(256), 0, st, qp, cw, conn, cen, NE, n, row0, M, rowmask,
+                       node_col, slot_col, uval, tval, A, b, (int)write_A);
+  else
+    hipLaunchKernelGGL(k_rows, dim3(num), dim3(256), 0, st, qp, conn, cen, NE, n, row0, M, rowmask,
+                       node_col, slot_col, uval, tval, A, b, (int)write_A);
+}
+
+void launch_planar(const double* qp, int NE, int* flag, double* cw, hipStream_t st) {
+  if (NE > 0) hipLaunchKernelGGL(k_planar, dim3(NE), dim3(64), 0, st, qp, NE, flag, cw);
 }
 
 }  // namespace ibem

@@ -28,7 +28,10 @@ void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, 
 void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long n,
                  long row0, const Mat& M, int rowmask, const int* node_col, const int* slot_col,
                  const double* uval, const double* tval, double* A, double* b, hipStream_t st,
-                 bool write_A = true);
+                 bool write_A = true, const double* cw = nullptr);
+// flag[e] = 1 when element e's 64 quadrature-point normals are bitwise equal (planar element);
+// cw[e][g][q] = w_g jac_g N_q(g). launch_rows with cw runs the moment form (every element planar).
+void launch_planar(const double* qp, int NE, int* flag, double* cw, hipStream_t st);
 // (write_A = false: b only, accumulated in the same order as with the matrix entries)
 
 // ---- postprocess (post.hip) ----
@@ -146,7 +149,10 @@ void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, co
 // to the concurrent panel. set_tuning returns nonzero for an unknown key or a value out of range.
 // kTuneInnerIters / kTuneInnerTolExp: the inner GMRES of IBEM3D_SOLVE_BJ_GMRES16 (iterations per outer
 // direction, at most kInnerItersMax; stopping tolerance 10^-exp).
-enum { kTuneBigMinTiles = 1, kTunePanelCuPercent = 2, kTuneInnerIters = 3, kTuneInnerTolExp = 4 };
+// kTuneRowsPointwise: 1 makes the boundary->inclusion rows use the per-Gauss-point contraction even
+// where every element is planar (default 0: the moment form there).
+enum { kTuneBigMinTiles = 1, kTunePanelCuPercent = 2, kTuneInnerIters = 3, kTuneInnerTolExp = 4,
+       kTuneRowsPointwise = 5 };
 constexpr int kInnerItersMax = 16;
 int set_tuning(int key, long value);
 long get_tuning(int key);

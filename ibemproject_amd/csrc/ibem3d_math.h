@@ -920,21 +920,23 @@ IBEM_HD RowTerms row_terms(const Mat& M, const Slot& R) {
 
 // row_gp with every table value read at a precomputed position (no selects): the k- and l-shifted
 // entries come from the table, and the d_nk, d_nl terms are multiplied in. Same values as row_gp.
-IBEM_HD void row_gp2(const RowTerms& R, const double* tab, const double* nv, double* su, double* st) {
+// tab(o) returns the table value at slot o (one table, or the sum of partial tables).
+template <class Tab>
+IBEM_HD void row_gp2_t(const RowTerms& R, const Tab& tab, const double* nv, double* su, double* st) {
   double T6[6], S6[6], Tnk[3], Tnl[3], Sn[3], Tn[3];
 #pragma unroll
   for (int u = 0; u < 6; ++u) {
-    T6[u] = tab[R.oT[u]];
-    S6[u] = tab[R.oS[u]];
+    T6[u] = tab(R.oT[u]);
+    S6[u] = tab(R.oS[u]);
   }
 #pragma unroll
   for (int n = 0; n < 3; ++n) {
-    Tnk[n] = tab[R.oTnk[n]];
-    Tnl[n] = tab[R.oTnl[n]];
-    Sn[n] = tab[R.oSn[n]];
-    Tn[n] = tab[R.oTn[n]];
+    Tnk[n] = tab(R.oTnk[n]);
+    Tnl[n] = tab(R.oTnl[n]);
+    Sn[n] = tab(R.oSn[n]);
+    Tn[n] = tab(R.oTn[n]);
   }
-  const double Tkl = tab[R.oTkl], Tk = tab[R.oTk], Tl = tab[R.oTl];
+  const double Tkl = tab(R.oTkl), Tk = tab(R.oTk), Tl = tab(R.oTl);
   const double V0 = nv[0] * T6[0] + nv[1] * T6[1] + nv[2] * T6[2];
   const double V1 = nv[0] * T6[1] + nv[1] * T6[3] + nv[2] * T6[4];
   const double V2 = nv[0] * T6[2] + nv[1] * T6[4] + nv[2] * T6[5];
@@ -954,6 +956,25 @@ IBEM_HD void row_gp2(const RowTerms& R, const double* tab, const double* nv, dou
     st[n] = -R.sg * (R.B1 * Sn[n] + R.B3 * Tn[n] + R.B2 * (R.dk[n] * Tl + R.dl[n] * Tk));
   }
 }
+
+struct TabOne {
+  const double* p;
+  IBEM_HD double operator()(int o) const { return p[o]; }
+};
+struct TabSum2 {
+  const double* p;
+  const double* q;
+  IBEM_HD double operator()(int o) const { return p[o] + q[o]; }
+};
+
+IBEM_HD void row_gp2(const RowTerms& R, const double* tab, const double* nv, double* su, double* st) {
+  row_gp2_t(R, TabOne{tab}, nv, su, st);
+}
+
+// Slots of the Kelvin table that the strain kernels read: d^beta r for 3 <= |beta| <= 6 at
+// 10 .. 83, d^beta r^-1 for 1 <= |beta| <= 4 at 85 .. 118.
+constexpr int kKUsed = 108;
+IBEM_HD int kelvin_used_slot(int m) { return m < 74 ? 10 + m : 11 + m; }
 
 // su[n], st[n] (n = 0..2) of one row at one Gauss point with unit normal nv and table tab
 IBEM_HD void row_gp(const RowTerms& R, const double* tab, const double* nv, double* su, double* st) {

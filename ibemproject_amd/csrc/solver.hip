@@ -886,6 +886,7 @@ static long g_big_min_tiles = IBEM_BIG_MIN_TILES;
 static long g_panel_cu_percent = 100;
 static long g_inner_iters = 6;     // inner GMRES iterations on the 16-bit A per outer direction
 static long g_inner_tol_exp = 3;   // inner GMRES stops at relative residual 10^-exp
+static long g_rows_pointwise = 0;  // 1: per-Gauss-point boundary->inclusion rows on planar elements too
 int set_tuning(int key, long value) {
   if (key == kTuneBigMinTiles && value >= 1) {
     g_big_min_tiles = value;
@@ -903,6 +904,10 @@ int set_tuning(int key, long value) {
     g_inner_tol_exp = value;
     return 0;
   }
+  if (key == kTuneRowsPointwise && (value == 0 || value == 1)) {
+    g_rows_pointwise = value;
+    return 0;
+  }
   return -1;
 }
 long get_tuning(int key) {
@@ -911,6 +916,7 @@ long get_tuning(int key) {
     case kTunePanelCuPercent: return g_panel_cu_percent;
     case kTuneInnerIters: return g_inner_iters;
     case kTuneInnerTolExp: return g_inner_tol_exp;
+    case kTuneRowsPointwise: return g_rows_pointwise;
     default: return -1;
   }
 }

@@ -120,8 +120,12 @@ enum {
                                         leaves to the concurrent panel (default 100, 0..400) */
   IBEM3D_TUNE_INNER_ITERS = 3,       /* IBEM3D_SOLVE_BJ_GMRES16: inner GMRES iterations on the 16-bit A
                                         per outer direction (default 6, 1..16) */
-  IBEM3D_TUNE_INNER_TOL_EXP = 4      /* IBEM3D_SOLVE_BJ_GMRES16: the inner GMRES stops at relative
+  IBEM3D_TUNE_INNER_TOL_EXP = 4,     /* IBEM3D_SOLVE_BJ_GMRES16: the inner GMRES stops at relative
                                         residual 10^-value (default 3, 1..8) */
+  IBEM3D_TUNE_ROWS_POINTWISE = 5     /* 1: the boundary->inclusion rows contract every Gauss point's
+                                        Kelvin table (the reference's order of sums) even where every
+                                        element is planar; 0 (default): there the tables are integrated
+                                        over the element first (4 contractions per element and row) */
 };
 int ibem3d_set_tuning(int key, long value);
 long ibem3d_get_tuning(int key);  /* -1 for an unknown key */
@@ -196,6 +200,13 @@ int ibem3d_solve_fell_back(const ibem3d_ctx* ctx);
  * iterations ibem3d_solve reports; -1 when that solve ran as BJ_GMRES (no room for the copy). No
  * counterpart in the reference. */
 long ibem3d_inner_iterations(const ibem3d_ctx* ctx);
+/* 1 when the boundary->inclusion rows (ibem3d_add_bem_rows, and the right-hand side of
+ * ibem3d_update_known_values) are integrated in moment form: every boundary element is planar (its
+ * Gauss-point normals agree to 1e-14), so the Kelvin tables are summed over the element's Gauss
+ * points before the row contraction, one contraction per shape function instead of one per point;
+ * 0 for the per-point contraction (a non-planar element, or IBEM3D_TUNE_ROWS_POINTWISE = 1). The
+ * entries agree to rounding. No counterpart in the reference. */
+int ibem3d_rows_moment_form(const ibem3d_ctx* ctx);
 /* The last block-Jacobi solve's start from the initial guess: relative residual of the scaled guess
  * alpha x0 (alpha in [0, 1] minimises ||b - alpha A x0||, so the start is never worse than zero);
  * rel0 = -1 when no guess was used. */

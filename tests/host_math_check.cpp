@@ -172,3 +172,69 @@ void hc_pot_coefs(double a, int inside, double* out) {
     for (int j = 0; j < 4; ++j) out[4 * i + j] = C[i][j];
 }
 }
+
+extern "C" {
+// One element (corner nodes el[12]) against one sphere centre c[3], the boundary->inclusion strain
+// rows integrated two ways: pointwise (k_rows: contraction at every Gauss point, then the weighted
+// sum) and in moment form (k_rows_m: the weighted sums of the Kelvin tables over the two halves of
+// the points, then one contraction per shape function, with the normal of the first point).
+// out_pw / out_mom: 60 x 24 row-major, su_n summed against N_q at n * 4 + q, st_n at 12 + n * 4 + q.
+void hc_rows_element(double E0, double nu0, double E1, double nu1, const double* c, const double* el,
+                     double* out_pw, double* out_mom) {
+  Mat M = make_mat(E0, nu0, E1, nu1);
+  std::vector<double> tabs(64 * kKStride, 0.0);
+  QPoint Q[64];
+  double w[64];
+  for (int g = 0; g < 64; ++g) {
+    const int a = g / 8, b = g % 8;
+    qpoint(el, gauss8_x(a), gauss8_x(b), Q[g]);
+    w[g] = gauss8_w(a) * gauss8_w(b) * Q[g].jac;
+    const double x[3] = {Q[g].g[0] - c[0], Q[g].g[1] - c[1], Q[g].g[2] - c[2]};
+    double y0[7], y1[7], y2[7], PR[7], PT[5];
+    kelvin_point(x, y0, y1, y2, PR, PT);
+    double* tab = tabs.data() + g * kKStride;
+    kelvin_group<0>(y0, y1, y2, PR, PT, tab);
+    kelvin_group<1>(y0, y1, y2, PR, PT, tab);
+    kelvin_group<2>(y0, y1, y2, PR, PT, tab);
+    kelvin_group<3>(y0, y1, y2, PR, PT, tab);
+  }
+  std::vector<double> mom(2 * 4 * kKStride, 0.0);
+  for (int half = 0; half < 2; ++half)
+    for (int m = 0; m < kKUsed; ++m) {
+      const int slot = kelvin_used_slot(m);
+      for (int q = 0; q < 4; ++q) {
+        double p = 0.0;
+        for (int gi = 0; gi < 32; ++gi) {
+          const int g = 32 * half + gi;
+          p += tabs[g * kKStride + slot] * (w[g] * Q[g].N[q]);
+        }
+        mom[(half * 4 + q) * kKStride + slot] = p;
+      }
+    }
+  for (int r = 0; r < 60; ++r) {
+    const RowTerms RT = row_terms(M, slot60(r));
+    double aH[12] = {0}, aG[12] = {0};
+    for (int g = 0; g < 64; ++g) {
+      double su[3], st[3];
+      row_gp2(RT, tabs.data() + g * kKStride, Q[g].nv, su, st);
+      for (int q = 0; q < 4; ++q)
+        for (int n = 0; n < 3; ++n) {
+          aH[n * 4 + q] += su[n] * (Q[g].N[q] * w[g]);
+          aG[n * 4 + q] += st[n] * (Q[g].N[q] * w[g]);
+        }
+    }
+    for (int o = 0; o < 12; ++o) {
+      out_pw[24 * r + o] = aH[o];
+      out_pw[24 * r + 12 + o] = aG[o];
+    }
+    for (int q = 0; q < 4; ++q) {
+      double su[3], st[3];
+      row_gp2_t(RT, TabSum2{mom.data() + q * kKStride, mom.data() + (4 + q) * kKStride}, Q[0].nv, su, st);
+      for (int n = 0; n < 3; ++n) {
+        out_mom[24 * r + n * 4 + q] = su[n];
+        out_mom[24 * r + 12 + n * 4 + q] = st[n];
+      }
+    }
+  }
+}
+}

@@ -174,3 +174,40 @@ def test_repeated_phase_is_refused():
     eng.reset()
     eng.assemble()
     eng.close()
+
+
+@pytest.mark.parametrize("kind", ["five", "many"])
+def test_rows_moment_form_matches_pointwise(kind):
+    """The boundary->inclusion rows assembled in moment form (every element of the cube's faces is
+    planar: the Kelvin tables are integrated over the element before the row contraction) equal the
+    per-Gauss-point form entry by entry to 1e-13 of the block's largest entry, A and b, and the
+    right-hand-side-only re-assembly agrees as well."""
+    from ibemproject_amd import engine as E
+    prob = small_problem(kind)
+    case(f"rows moment vs pointwise, {kind}: NE={prob.NE} num={prob.num} n60={prob.n60}")
+    out = {}
+    try:
+        for pointwise in (1, 0):
+            E.set_tuning(E.TUNE_ROWS_POINTWISE, pointwise)
+            eng = Engine(prob)
+            assert eng.rows_moment_form() == (pointwise == 0)
+            run_reference_phases(eng)
+            A, b = eng.system()
+            eng.update_known_values(prob.bc[:, 1].copy(), prob.tbc[:, 12:].copy())
+            out[pointwise] = (A, b, eng.rhs())
+            eng.close()
+    finally:
+        E.set_tuning(E.TUNE_ROWS_POINTWISE, 0)
+    nb = 3 * prob.NN
+    Ap, bp, rp = out[1]
+    Am, bm, rm = out[0]
+    blk_p, blk_m = Ap[nb:, :nb], Am[nb:, :nb]
+    assert np.abs(blk_p).max() > 0
+    err = np.abs(blk_m - blk_p).max() / np.abs(blk_p).max()
+    print(f"  rows block max diff / max entry {err:.2e}")
+    assert err <= 1e-13
+    # the rest of A is untouched by the rows kernel: bit-identical
+    assert np.array_equal(Am[:nb], Ap[:nb]) and np.array_equal(Am[nb:, nb:], Ap[nb:, nb:])
+    for x, y in ((bm, bp), (rm, rp)):
+        assert np.abs(x[nb:] - y[nb:]).max() <= 1e-13 * np.abs(y[nb:]).max()
+        assert np.array_equal(x[:nb], y[:nb])

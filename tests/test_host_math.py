@@ -175,3 +175,47 @@ def test_inclusion_block_weighted_tables(hostmath):
         hostmath.hc_incl_block(1.0, 0.3, 10.0, 0.25, x.ctypes.data_as(dp), a, self_, o1.ctypes.data_as(dp))
         hostmath.hc_incl_block_w(1.0, 0.3, 10.0, 0.25, x.ctypes.data_as(dp), a, self_, o2.ctypes.data_as(dp))
         assert np.abs(o1 - o2).max() <= 1e-12 * np.abs(o1).max()
+
+
+def test_rows_moment_form(hostmath):
+    """The boundary->inclusion rows integrated in moment form (the Kelvin tables summed against
+    w N_q over the Gauss points first, one contraction per shape function) equal the pointwise form
+    (a contraction at every Gauss point) on planar elements: face quads of the cube, a tilted planar
+    quad, spheres near and far. On a twisted (non-planar) quad the two differ, which is why the
+    moment form is used only when every element is planar."""
+    import ctypes
+    dp = ctypes.POINTER(ctypes.c_double)
+    f = hostmath.hc_rows_element
+    f.argtypes = [ctypes.c_double] * 4 + [dp] * 4
+
+    def run(c, el, mat=(1.0, 0.3, 10.0, 0.25)):
+        c = np.ascontiguousarray(c, dtype=float)
+        el = np.ascontiguousarray(el, dtype=float).ravel()
+        pw = np.zeros((60, 24))
+        mo = np.zeros((60, 24))
+        f(*mat, c.ctypes.data_as(dp), el.ctypes.data_as(dp), pw.ctypes.data_as(dp), mo.ctypes.data_as(dp))
+        return pw, mo
+
+    h = 1.0 / 24
+    face_x1 = [[1, 0, 0], [1, h, 0], [1, h, h], [1, 0, h]]            # on x = 1, normal +x
+    face_z0 = [[0.5, 0.5, 0], [0.5, 0.5 + h, 0], [0.5 + h, 0.5 + h, 0], [0.5 + h, 0.5, 0]]
+    rng = np.random.default_rng(5)
+    for el in (face_x1, face_z0):
+        for c in ([0.5, 0.5, 0.5], [0.97, 0.02, 0.03], [0.52, 0.51, 0.03]):
+            for mat in ((1.0, 0.3, 10.0, 0.25), (1.0, 0.3, 100.0, 0.25), (1.0, 0.3, 2.0, 0.25)):
+                pw, mo = run(c, el, mat)
+                assert np.abs(pw).max() > 0
+                assert np.abs(mo - pw).max() <= 1e-13 * np.abs(pw).max()
+    # a tilted planar quad (a parallelogram in the plane through p0 spanned by u, v)
+    p0 = np.array([0.2, 0.3, 0.1])
+    u = np.array([0.05, 0.01, 0.02])
+    v = np.array([-0.01, 0.04, 0.015])
+    tilted = [p0, p0 + u, p0 + u + v, p0 + v]
+    for _ in range(5):
+        c = rng.uniform(0.0, 1.0, size=3)
+        pw, mo = run(c, tilted)
+        assert np.abs(mo - pw).max() <= 1e-12 * np.abs(pw).max()
+    # a twisted quad: one corner lifted out of the plane
+    twisted = [p0, p0 + u, p0 + u + v + np.array([0.0, 0.0, 0.03]), p0 + v]
+    pw, mo = run([0.25, 0.35, 0.2], twisted)
+    assert np.abs(mo - pw).max() > 1e-6 * np.abs(pw).max()
