@@ -89,7 +89,12 @@ struct ibem3d_ctx {
   LuWork W;
   unsigned phases = 0;
   bool solved = false;
-  double* d_cw = nullptr;      // every boundary element planar: k_rows' moment-form weights
+  // every boundary element planar: the moment form of k_rows (weights, element columns, known values)
+  double* d_cw = nullptr;
+  int* d_ecol = nullptr;
+  double* d_eknown = nullptr;
+  double* d_sink = nullptr;
+  RowsMoment rows_mf{};
   double rel_res = -1.0;
   int iters = 0;
   std::string err;
@@ -355,6 +360,24 @@ ibem3d_ctx* ibem3d_create(const ibem3d_problem* p, int device) {
     if (!std::all_of(flag.begin(), flag.end(), [](int f) { return f == 1; })) {
       hipFree(ctx->d_cw);
       ctx->d_cw = nullptr;
+    } else {
+      // element columns in element order, the first toucher of every column marked
+      std::vector<int> ecol(24 * (size_t)p->NE);
+      std::vector<char> seen(3 * (size_t)p->NN, 0);
+      for (int e = 0; e < p->NE; ++e)
+        for (int k = 0; k < 24; ++k) {
+          const int kk = k < 12 ? k : k - 12, q = kk / 3, nc = kk % 3;
+          int col = k < 12 ? ctx->node_col[3 * (p->conn[4 * e + q] - 1) + nc] : ctx->slot_col[12 * e + 3 * q + nc];
+          if (col >= 0) {
+            if (!seen[col]) col |= kRowsColFirst;
+            seen[col & ~kRowsColFirst] = 1;
+          }
+          ecol[24 * (size_t)e + k] = col;
+        }
+      if (upload(ctx, &ctx->d_ecol, ecol.data(), ecol.size()) ||
+          dalloc(ctx, &ctx->d_eknown, 24 * (size_t)p->NE) || dalloc(ctx, &ctx->d_sink, 256))
+        return bail("");
+      ctx->rows_mf = RowsMoment{ctx->d_cw, ctx->d_ecol, ctx->d_eknown, ctx->d_sink};
     }
   }
   return ctx;
@@ -368,7 +391,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
   ibem3d_reset_timings(ctx);
   for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
   ctx->pool.clear();
-  void* ptrs[] = {ctx->d_stU,   ctx->d_stT,      ctx->d_sts,      ctx->d_cw,
+  void* ptrs[] = {ctx->d_stU,   ctx->d_stT,      ctx->d_sts,      ctx->d_cw,  ctx->d_ecol, ctx->d_eknown, ctx->d_sink,
                   ctx->d_nodes, ctx->d_cen,      ctx->d_rad,      ctx->d_pts,  ctx->d_qp,
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
@@ -405,9 +428,9 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
     if (hipSetDevice(ctx->dev) != hipSuccess) return fail(ctx, IBEM3D_EHIP, "hipSetDevice"); \
   } while (0)
 
-// k_rows' moment-form weights when every element is planar and the pointwise form is not asked for
-static const double* rows_moment_form(const ibem3d_ctx* ctx) {
-  return get_tuning(kTuneRowsPointwise) == 0 ? ctx->d_cw : nullptr;
+// k_rows' moment form when every element is planar and the pointwise form is not asked for
+static const RowsMoment* rows_moment_form(const ibem3d_ctx* ctx) {
+  return ctx->d_cw && get_tuning(kTuneRowsPointwise) == 0 ? &ctx->rows_mf : nullptr;
 }
 
 static int after_launch(ibem3d_ctx* ctx, const char* what) {

@@ -489,27 +489,60 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
 // Normals within kPlanarTol of each other count as one: replacing them by the first point's changes
 // an entry by at most ~kPlanarTol of its magnitude.
 constexpr double kPlanarTol = 1e-14;
+
+// Barrier that orders LDS only: global loads issued before it stay in flight.
+__device__ __forceinline__ void barrier_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// eknown[e][k]: the known displacement (k < 12, node dof 3 q + n) or traction (k >= 12, slot
+// 3 q + n) that column k of element e folds into b; zero where the value is an unknown.
+__global__ __launch_bounds__(256) void k_elem_known(const int* __restrict__ conn, int NE,
+                                                    const int* __restrict__ node_col,
+                                                    const int* __restrict__ slot_col,
+                                                    const double* __restrict__ uval,
+                                                    const double* __restrict__ tval,
+                                                    double* __restrict__ eknown) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= 24L * NE) return;
+  const int e = (int)(i / 24), k = (int)(i % 24);
+  const int kk = k < 12 ? k : k - 12, q = kk / 3, nc = kk % 3;
+  double v;
+  if (k < 12) {
+    const int dof = 3 * (conn[4 * e + q] - 1) + nc;
+    v = node_col[dof] >= 0 ? 0.0 : uval[dof];
+  } else {
+    const int slot = 12 * e + 3 * q + nc;
+    v = slot_col[slot] == -1 ? tval[slot] : 0.0;
+  }
+  eknown[i] = v;
+}
+
 // On a planar element the unit normal is the same at every Gauss point, so a row's strain kernels
 // su_n, st_n are linear in the Kelvin table with coefficients that do not depend on the point, and
 //   sum_g w_g N_q(g) su_n(tab_g) = su_n(P_q),   P_q[slot] = sum_g w_g N_q(g) tab_g[slot].
 // Per element: (1) the 64 tables as in k_rows; (2) thread = (used slot, half of the Gauss points)
-// forms the four moments P_q of its slot over 32 points (w N_q from the read-only quadrature array,
-// wave-uniform addresses, so scalar loads); (3) lane = row, wavefront = shape function q: one
-// contraction of the moment table (the two halves summed as they are read) gives the row's H and
-// G entries of node q, folded through the column map as in k_rows. The contraction runs 4 times per
-// element and row instead of 64; the Gauss points, weights and tables are the same as k_rows',
-// only the order of the sums differs.
+// forms the four moments P_q of its slot over 32 points (w N_q from cw, wave-uniform addresses, so
+// scalar loads); (3) lane = row, wavefront = shape function q: one contraction of the moment table
+// (the two halves summed as they are read) gives the row's H and G entries of node q, folded
+// through the column map as in k_rows. The contraction runs 4 times per element and row instead of
+// 64; the Gauss points, weights and tables are the same as k_rows', only the order of the sums
+// differs.
+// Memory ordering per element: the next element's quadrature data and column entries are loaded
+// into registers before (1); one full barrier after (1) makes the previous element's A stores
+// visible, so this element's A reads go out right after it and arrive during (2); the barrier
+// between (2) and (3) orders LDS only. The columns an element's rows touch are shared with the
+// elements around its nodes, so a read waits for the previous element's writes but no longer.
 __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp,
                                                 const double* __restrict__ cw,
-                                                const int* __restrict__ conn,
+                                                const int* __restrict__ ecol,
+                                                const double* __restrict__ eknown,
                                                 const double* __restrict__ cen, int NE, long n,
                                                 long row0, Mat M, int rowmask,
-                                                const int* __restrict__ node_col,
-                                                const int* __restrict__ slot_col,
-                                                const double* __restrict__ uval,
-                                                const double* __restrict__ tval,
                                                 double* __restrict__ A, double* __restrict__ b,
-                                                int write_A) {
+                                                int write_A, double* __restrict__ sink) {
   constexpr int QN = 64 * kQpStride;
   constexpr int QPT = (QN + 255) / 256;
   __shared__ double sq[2][QN];                    // current / next element
@@ -534,38 +567,30 @@ __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp
   const bool actB = mB < kKUsed;
   const int slotB = kelvin_used_slot(actB ? mB : 0);
   double bacc = 0.0;
-  auto load_cols = [&](int e) {
-    if (t < 24) {
-      const int par = e & 1;
-      const int k = t < 12 ? t : t - 12, q = k / 3, nc = k % 3;
-      if (t < 12) {
-        const int dof = 3 * (conn[4 * e + q] - 1) + nc;
-        const int col = node_col[dof];
-        scol[par][t] = col;
-        sknown[par][t] = col >= 0 ? 0.0 : uval[dof];
-      } else {
-        const int slot = 12 * e + 3 * q + nc;
-        const int kc = slot_col[slot];
-        scol[par][t] = kc;
-        sknown[par][t] = kc == -1 ? tval[slot] : 0.0;
-      }
-    }
-  };
-  if (NE > 0)
-    for (int u = t; u < QN; u += 256) sq[0][u] = qp[u];
+  if (NE <= 0) return;
+  for (int u = t; u < QN; u += 256) sq[0][u] = qp[u];
+  if (t < 24) {
+    scol[0][t] = ecol[t];
+    sknown[0][t] = eknown[t];
+  }
+  __syncthreads();
   double qn[QPT];
+  int cn = 0;
+  double kn = 0.0;
   for (int e = 0; e < NE; ++e) {
     const int cur = e & 1;
-    load_cols(e);
-    if (e + 1 < NE) {
+    if (e + 1 < NE) {  // element e + 1 into registers (stored to LDS in step 2), no branches
 #pragma unroll
       for (int k = 0; k < QPT; ++k) {
         const int u = t + 256 * k;
-        qn[k] = u < QN ? qp[(size_t)(e + 1) * QN + u] : 0.0;
+        qn[k] = qp[(size_t)(e + 1) * QN + (u < QN ? u : QN - 1)];
       }
+      // every thread loads (lanes past 23 repeat entry 23): no branch, so no early wait
+      const int tc = t < 24 ? t : 23;
+      cn = ecol[24 * (e + 1) + tc];
+      kn = eknown[24 * (e + 1) + tc];
     }
-    __syncthreads();  // sq[cur] complete; the previous element's contraction is done with smom
-    {
+    {  // (1) tables
       const double* Q = sq[cur] + lane * kQpStride;
       const double x[3] = {Q[0] - c0, Q[1] - c1, Q[2] - c2};
       double y0[7], y1[7], y2[7], PR[7], PT[5];
@@ -578,9 +603,28 @@ __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp
         default: kelvin_group<3>(y0, y1, y2, PR, PT, tab); break;
       }
     }
-    __syncthreads();
-    if (actB) {
-      // weights of four points per step (two 64-byte scalar loads, one wait)
+    __syncthreads();  // tables complete; the previous element's A stores visible
+    // this element's A entries (node q = wv): read now, used after step 2
+    const int q = wv;
+    int col[6];
+    double old[6];
+#pragma unroll
+    for (int nn = 0; nn < 3; ++nn) {
+      col[nn] = scol[cur][3 * q + nn];
+      col[3 + nn] = scol[cur][12 + 3 * q + nn];
+    }
+    // six loads without branches (a column not read points at an entry of this row block's first
+    // row), so that later waits can count them
+    bool rd[6];
+    double ld[6];
+#pragma unroll
+    for (int o = 0; o < 6; ++o) {
+      rd[o] = active && write_A && col[o] >= 0 && !(col[o] & kRowsColFirst);
+      ld[o] = A[rd[o] ? (long)col[o] * n + row : row0];
+    }
+#pragma unroll
+    for (int o = 0; o < 6; ++o) old[o] = rd[o] ? ld[o] : 0.0;
+    if (actB) {  // (2) moments; weights of four points per step (two 64-byte scalar loads, one wait)
       const double* We = cw + ((size_t)e * 64 + 32 * half) * 4;
       const double* Te = stab + 32 * half * kKStride + slotB;
       double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
@@ -609,40 +653,36 @@ __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp
         const int u = t + 256 * k;
         if (u < QN) sq[cur ^ 1][u] = qn[k];
       }
+      if (t < 24) {
+        scol[cur ^ 1][t] = cn;
+        sknown[cur ^ 1][t] = kn;
+      }
     }
-    __syncthreads();
-    if (active) {
-      const int q = wv;
+    barrier_lds();
+    {  // (3) contraction and fold: straight-line for every lane (lanes past row 59 and rows of an
+       // order not asked for compute row 59's values and store them to the sink)
       const double* nv = sq[cur] + 3;  // the element's normal (equal at every point)
       double su[3], st[3];
       row_gp2_t(RT, TabSum2{smom[0][q], smom[1][q]}, nv, su, st);
-      double old[6];
-      int col[6];
+      double val[6];
 #pragma unroll
       for (int nn = 0; nn < 3; ++nn) {
-        col[nn] = scol[cur][3 * q + nn];
-        col[3 + nn] = scol[cur][12 + 3 * q + nn];
+        const double hv = -fac * su[nn], gv = fac * st[nn];
+        val[nn] = old[nn] + hv;
+        val[3 + nn] = old[3 + nn] - gv;
+        const double kh = active && col[nn] < 0 ? sknown[cur][3 * q + nn] : 0.0;
+        const double kg = active && col[3 + nn] < 0 ? sknown[cur][12 + 3 * q + nn] : 0.0;
+        bacc += gv * kg - hv * kh;  // known values fold into b (as in k_rows)
       }
 #pragma unroll
-      for (int o = 0; o < 6; ++o) old[o] = col[o] >= 0 && write_A ? A[(long)col[o] * n + row] : 0.0;
-#pragma unroll
-      for (int nn = 0; nn < 3; ++nn) {
-        const double hv = -fac * su[nn];
-        if (col[nn] >= 0) {
-          if (write_A) A[(long)col[nn] * n + row] = old[nn] + hv;
-        } else {
-          bacc -= hv * sknown[cur][3 * q + nn];
-        }
-        const double gv = fac * st[nn];
-        if (col[3 + nn] >= 0) {
-          if (write_A) A[(long)col[3 + nn] * n + row] = old[3 + nn] - gv;
-        } else {
-          bacc += gv * sknown[cur][12 + 3 * q + nn];  // zero unless the traction is known
-        }
+      for (int o = 0; o < 6; ++o) {
+        const bool wr = active && write_A && col[o] >= 0;
+        double* dst = wr ? A + (long)(col[o] & ~kRowsColFirst) * n + row : sink + t;
+        *dst = val[o];
       }
     }
   }
-  // the tables were last read before the barrier that preceded the last contraction
+  __syncthreads();  // the last element's tables were read before the last contraction
   sb[wv * 64 + lane] = active ? bacc : 0.0;
   __syncthreads();
   if (wv == 0 && active) b[row] += sb[lane] + sb[64 + lane] + sb[128 + lane] + sb[192 + lane];
@@ -708,11 +748,13 @@ void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, 
 void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long n,
                  long row0, const Mat& M, int rowmask, const int* node_col, const int* slot_col,
                  const double* uval, const double* tval, double* A, double* b, hipStream_t st,
-                 bool write_A, const double* cw) {
-  if (cw)
-    hipLaunchKernelGGL(k_rows_m, dim3(num), dim3(256), 0, st, qp, cw, conn, cen, NE, n, row0, M, rowmask,
-                       node_col, slot_col, uval, tval, A, b, (int)write_A);
-  else
+                 bool write_A, const RowsMoment* mf) {
+  if (mf) {
+    hipLaunchKernelGGL(k_elem_known, dim3((unsigned)((24L * NE + 255) / 256)), dim3(256), 0, st, conn, NE,
+                       node_col, slot_col, uval, tval, mf->eknown);
+    hipLaunchKernelGGL(k_rows_m, dim3(num), dim3(256), 0, st, qp, mf->cw, mf->ecol, mf->eknown, cen, NE, n,
+                       row0, M, rowmask, A, b, (int)write_A, mf->sink);
+  } else
     hipLaunchKernelGGL(k_rows, dim3(num), dim3(256), 0, st, qp, conn, cen, NE, n, row0, M, rowmask,
                        node_col, slot_col, uval, tval, A, b, (int)write_A);
 }
