@@ -70,7 +70,7 @@ struct ibem3d_ctx {
   // in / out); pending: launched and not yet joined by the main stream; ready: the preconditioner
   // matches the assembled A
   hipStream_t st_aux = nullptr;
-  hipEvent_t ev_aux_in = nullptr, ev_aux_out = nullptr;
+  hipEvent_t ev_aux_in = nullptr, ev_aux_in2 = nullptr, ev_aux_out = nullptr;
   bool bj_pending = false, bj_ready = false;
   // IBEM3D_SOLVE_BJ_GMRES16: the 16-bit copy of A (leading dimension ld16) with its tile scales, and
   // the inner GMRES's basis, directions and product vector
@@ -357,22 +357,36 @@ ibem3d_ctx* ibem3d_create(const ibem3d_problem* p, int device) {
                     hipStreamSynchronize(ctx->st) == hipSuccess;
     hipFree(d_flag);
     if (!ok) return bail("ibem3d_create: quadrature setup failed");
-    if (!std::all_of(flag.begin(), flag.end(), [](int f) { return f == 1; })) {
+    if (!std::all_of(flag.begin(), flag.end(), [](int f) { return f == 1; }) ||
+        3L * p->NN > kRowsColMask) {
       hipFree(ctx->d_cw);
       ctx->d_cw = nullptr;
     } else {
-      // element columns in element order, the first toucher of every column marked
+      // element columns in element order: the first toucher of every column marked, and the
+      // columns the previous element touched too marked with their index there
       std::vector<int> ecol(24 * (size_t)p->NE);
       std::vector<char> seen(3 * (size_t)p->NN, 0);
+      auto raw = [&](int e, int k) {
+        const int kk = k < 12 ? k : k - 12, q = kk / 3, nc = kk % 3;
+        return k < 12 ? ctx->node_col[3 * (p->conn[4 * e + q] - 1) + nc] : ctx->slot_col[12 * e + 3 * q + nc];
+      };
       for (int e = 0; e < p->NE; ++e)
         for (int k = 0; k < 24; ++k) {
-          const int kk = k < 12 ? k : k - 12, q = kk / 3, nc = kk % 3;
-          int col = k < 12 ? ctx->node_col[3 * (p->conn[4 * e + q] - 1) + nc] : ctx->slot_col[12 * e + 3 * q + nc];
-          if (col >= 0) {
-            if (!seen[col]) col |= kRowsColFirst;
-            seen[col & ~kRowsColFirst] = 1;
+          const int c = raw(e, k);
+          int code = c;
+          if (c >= 0) {
+            if (!seen[c]) {
+              code |= kRowsColFirst;
+            } else if (e > 0) {
+              for (int k2 = 0; k2 < 24; ++k2)
+                if (raw(e - 1, k2) == c) {
+                  code |= kRowsColFwd | (k2 << kRowsColFwdShift);
+                  break;
+                }
+            }
+            seen[c] = 1;
           }
-          ecol[24 * (size_t)e + k] = col;
+          ecol[24 * (size_t)e + k] = code;
         }
       if (upload(ctx, &ctx->d_ecol, ecol.data(), ecol.size()) ||
           dalloc(ctx, &ctx->d_eknown, 24 * (size_t)p->NE) || dalloc(ctx, &ctx->d_sink, 256))
@@ -407,7 +421,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (ctx->st_aux) hipStreamDestroy(ctx->st_aux);
-  for (hipEvent_t e : {ctx->ev_aux_in, ctx->ev_aux_out})
+  for (hipEvent_t e : {ctx->ev_aux_in, ctx->ev_aux_in2, ctx->ev_aux_out})
     if (e) hipEventDestroy(e);
   if (ctx->W.st2) hipStreamDestroy(ctx->W.st2);
   if (ctx->Wb.st2) hipStreamDestroy(ctx->Wb.st2);
@@ -527,32 +541,36 @@ int ibem3d_assemble(ibem3d_ctx* ctx) {
   int rc;
   // After a block-Jacobi solve the next one needs the same preconditioner of the new A: its boundary
   // block (written only by the boundary phase) is factored, and its sphere blocks (written only by
-  // the inclusion phase) inverted, on a second stream while the other phases run.
+  // the inclusion phase) inverted, on a second stream while the other phases run. Every phase of the
+  // main stream is enqueued first: the setup is some two thousand small launches, and the host can
+  // wait in a launch while the second stream's queue drains, which must not hold back the assembly.
   const bool bj_next = ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES || ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES16;
   if (bj_next) {
     if ((rc = join_aux(ctx))) return rc;
     if (!ctx->st_aux) {
       HIPC(hipStreamCreateWithFlags(&ctx->st_aux, hipStreamNonBlocking));
       HIPC(hipEventCreateWithFlags(&ctx->ev_aux_in, hipEventDisableTiming));
+      HIPC(hipEventCreateWithFlags(&ctx->ev_aux_in2, hipEventDisableTiming));
       HIPC(hipEventCreateWithFlags(&ctx->ev_aux_out, hipEventDisableTiming));
     }
     if ((rc = bj_alloc(ctx))) return rc;
   }
   if ((rc = ibem3d_add_bem(ctx))) return rc;
-  if (bj_next) {
-    HIPC(hipEventRecord(ctx->ev_aux_in, ctx->st));
-    HIPC(hipStreamWaitEvent(ctx->st_aux, ctx->ev_aux_in, 0));
-    Scope sc(ctx, "assemble.bj_boundary_aux", ctx->st_aux);
-    if ((rc = bj_enqueue_boundary(ctx, ctx->st_aux))) return rc;
-  }
+  if (bj_next) HIPC(hipEventRecord(ctx->ev_aux_in, ctx->st));    // boundary block complete
   if ((rc = ibem3d_add_field(ctx))) return rc;
   // all three eigenstrain orders in one pass each: every row belongs to one order, so the entries
   // are the ones the per-order calls write, and the per-pair tables are built once instead of three
   // times
   if ((rc = ibem3d_add_incl(ctx, -1))) return rc;
+  if (bj_next) HIPC(hipEventRecord(ctx->ev_aux_in2, ctx->st));   // sphere blocks complete
+  if ((rc = ibem3d_add_bem_rows(ctx, -1))) return rc;
   if (bj_next) {
-    HIPC(hipEventRecord(ctx->ev_aux_in, ctx->st));
     HIPC(hipStreamWaitEvent(ctx->st_aux, ctx->ev_aux_in, 0));
+    {
+      Scope sc(ctx, "assemble.bj_boundary_aux", ctx->st_aux);
+      if ((rc = bj_enqueue_boundary(ctx, ctx->st_aux))) return rc;
+    }
+    HIPC(hipStreamWaitEvent(ctx->st_aux, ctx->ev_aux_in2, 0));
     {
       Scope sc(ctx, "assemble.bj_blocks_aux", ctx->st_aux);
       bj_enqueue_blocks(ctx, ctx->st_aux);
@@ -561,7 +579,6 @@ int ibem3d_assemble(ibem3d_ctx* ctx) {
     ctx->bj_pending = true;
     if ((rc = after_launch(ctx, "block-Jacobi setup beside the assembly"))) return rc;
   }
-  if ((rc = ibem3d_add_bem_rows(ctx, -1))) return rc;
   return ibem3d_apply_bc(ctx);
 }
 

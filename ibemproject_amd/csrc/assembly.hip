@@ -530,11 +530,15 @@ __global__ __launch_bounds__(256) void k_elem_known(const int* __restrict__ conn
 // through the column map as in k_rows. The contraction runs 4 times per element and row instead of
 // 64; the Gauss points, weights and tables are the same as k_rows', only the order of the sums
 // differs.
-// Memory ordering per element: the next element's quadrature data and column entries are loaded
-// into registers before (1); one full barrier after (1) makes the previous element's A stores
-// visible, so this element's A reads go out right after it and arrive during (2); the barrier
-// between (2) and (3) orders LDS only. The columns an element's rows touch are shared with the
-// elements around its nodes, so a read waits for the previous element's writes but no longer.
+// The fold adds into columns that the elements around a node share. A column the previous element
+// wrote takes that element's value from LDS (sfwd); every other column was last written two or
+// more elements ago, so its A entries are read at the top of the element, before (1), and arrive
+// while (1) and (2) run; a column's first element writes without reading. The next element's
+// quadrature points and column entries are loaded into registers before (1) as well. Barriers: one
+// full barrier after (1), one ordering LDS only after (2).
+constexpr int kTabStride = 109;  // per Gauss point: slots 10 .. 118 of the Kelvin table
+constexpr int kTabBase = 10;     // table slot of the first stored value
+constexpr int kSqN = 64 * 3 + 3; // staged per element: the 64 points, then the element normal
 __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp,
                                                 const double* __restrict__ cw,
                                                 const int* __restrict__ ecol,
@@ -543,11 +547,10 @@ __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp
                                                 long row0, Mat M, int rowmask,
                                                 double* __restrict__ A, double* __restrict__ b,
                                                 int write_A, double* __restrict__ sink) {
-  constexpr int QN = 64 * kQpStride;
-  constexpr int QPT = (QN + 255) / 256;
-  __shared__ double sq[2][QN];                    // current / next element
-  __shared__ double stab[64 * kKStride];          // the 64 Gauss points' tables
-  __shared__ double smom[2][4][kKStride];         // moments: [half of the points][q][slot]
+  __shared__ double sq[2][kSqN + 1];                // current / next element
+  __shared__ double stab[64 * kTabStride];          // the 64 Gauss points' tables
+  __shared__ double smom[2][4][kTabStride];         // moments: [half of the points][q][slot]
+  __shared__ double sfwd[24][64];                   // the previous element's new values [column][row]
   __shared__ int scol[2][24];
   __shared__ double sknown[2][24];
   double* sb = stab;  // after the last element: per-wave right-hand-side sums (4 x 64)
@@ -557,7 +560,8 @@ __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp
   // step (3): lane = row, wave = shape function
   const Slot R = slot60(lane < 60 ? lane : 59);
   const bool active = lane < 60 && ((rowmask >> R.order) & 1);
-  const RowTerms RT = row_terms(M, R);
+  // the row's table positions packed (8 registers across the loop), unpacked in step (3)
+  RowTermsP RP = row_terms_pack(row_terms(M, R), R.order);
   const double fac = R.order == 2 ? 0.5 : 1.0;
   const double c0 = cen[3 * s], c1 = cen[3 * s + 1], c2 = cen[3 * s + 2];
   const long row = row0 + 60L * s + lane;
@@ -565,37 +569,50 @@ __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp
   const int half = wv >> 1;
   const int mB = (wv & 1) * 64 + lane;
   const bool actB = mB < kKUsed;
-  const int slotB = kelvin_used_slot(actB ? mB : 0);
+  const int slotB = kelvin_used_slot(actB ? mB : 0) - kTabBase;
+  // staged quadrature value u of element e: the points' coordinates, then the normal
+  const int tq = t < kSqN ? t : kSqN - 1;
+  const long qoff = tq < 192 ? (long)(tq / 3) * kQpStride + tq % 3 : 3 + (tq - 192);
+  const int tc = t < 24 ? t : 23;
   double bacc = 0.0;
   if (NE <= 0) return;
-  for (int u = t; u < QN; u += 256) sq[0][u] = qp[u];
+  if (t < kSqN) sq[0][t] = qp[qoff];
   if (t < 24) {
     scol[0][t] = ecol[t];
     sknown[0][t] = eknown[t];
   }
   __syncthreads();
-  double qn[QPT];
+  const int q = wv;
+  double qn = 0.0, kn = 0.0;
   int cn = 0;
-  double kn = 0.0;
   for (int e = 0; e < NE; ++e) {
     const int cur = e & 1;
     if (e + 1 < NE) {  // element e + 1 into registers (stored to LDS in step 2), no branches
-#pragma unroll
-      for (int k = 0; k < QPT; ++k) {
-        const int u = t + 256 * k;
-        qn[k] = qp[(size_t)(e + 1) * QN + (u < QN ? u : QN - 1)];
-      }
-      // every thread loads (lanes past 23 repeat entry 23): no branch, so no early wait
-      const int tc = t < 24 ? t : 23;
+      qn = qp[(size_t)(e + 1) * 64 * kQpStride + qoff];
       cn = ecol[24 * (e + 1) + tc];
       kn = eknown[24 * (e + 1) + tc];
     }
+    // (0) this element's columns of node q; reads of the columns written two or more elements ago
+    // (issued after the prefetch, which step 2 waits for: the counter is in order)
+    int col[6];
+    bool rd[6];
+    double ld[6];
+#pragma unroll
+    for (int nn = 0; nn < 3; ++nn) {
+      col[nn] = scol[cur][3 * q + nn];
+      col[3 + nn] = scol[cur][12 + 3 * q + nn];
+    }
+#pragma unroll
+    for (int o = 0; o < 6; ++o) {
+      rd[o] = active && write_A && col[o] >= 0 && !(col[o] & (kRowsColFirst | kRowsColFwd));
+      ld[o] = A[rd[o] ? (long)(col[o] & kRowsColMask) * n + row : row0];
+    }
     {  // (1) tables
-      const double* Q = sq[cur] + lane * kQpStride;
-      const double x[3] = {Q[0] - c0, Q[1] - c1, Q[2] - c2};
+      const double* G = sq[cur] + 3 * lane;
+      const double x[3] = {G[0] - c0, G[1] - c1, G[2] - c2};
       double y0[7], y1[7], y2[7], PR[7], PT[5];
       kelvin_point(x, y0, y1, y2, PR, PT);
-      double* tab = stab + lane * kKStride;
+      double* tab = stab + lane * kTabStride - kTabBase;
       switch (wv) {
         case 0: kelvin_group<0>(y0, y1, y2, PR, PT, tab); break;
         case 1: kelvin_group<1>(y0, y1, y2, PR, PT, tab); break;
@@ -603,37 +620,21 @@ __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp
         default: kelvin_group<3>(y0, y1, y2, PR, PT, tab); break;
       }
     }
-    __syncthreads();  // tables complete; the previous element's A stores visible
-    // this element's A entries (node q = wv): read now, used after step 2
-    const int q = wv;
-    int col[6];
-    double old[6];
+    __syncthreads();  // tables complete; the previous element's forwarded values visible
+    double fw[6];
 #pragma unroll
-    for (int nn = 0; nn < 3; ++nn) {
-      col[nn] = scol[cur][3 * q + nn];
-      col[3 + nn] = scol[cur][12 + 3 * q + nn];
-    }
-    // six loads without branches (a column not read points at an entry of this row block's first
-    // row), so that later waits can count them
-    bool rd[6];
-    double ld[6];
-#pragma unroll
-    for (int o = 0; o < 6; ++o) {
-      rd[o] = active && write_A && col[o] >= 0 && !(col[o] & kRowsColFirst);
-      ld[o] = A[rd[o] ? (long)col[o] * n + row : row0];
-    }
-#pragma unroll
-    for (int o = 0; o < 6; ++o) old[o] = rd[o] ? ld[o] : 0.0;
+    for (int o = 0; o < 6; ++o)
+      fw[o] = col[o] >= 0 && (col[o] & kRowsColFwd) ? sfwd[(col[o] >> kRowsColFwdShift) & 31][lane] : 0.0;
     if (actB) {  // (2) moments; weights of four points per step (two 64-byte scalar loads, one wait)
       const double* We = cw + ((size_t)e * 64 + 32 * half) * 4;
-      const double* Te = stab + 32 * half * kKStride + slotB;
+      const double* Te = stab + 32 * half * kTabStride + slotB;
       double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
       for (int gi = 0; gi < 32; gi += 4) {
         double w[16], v[4];
 #pragma unroll
         for (int k = 0; k < 16; ++k) w[k] = We[4 * gi + k];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) v[k] = Te[(gi + k) * kKStride];
+        for (int k = 0; k < 4; ++k) v[k] = Te[(gi + k) * kTabStride];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           p0 += v[k] * w[4 * k];
@@ -648,11 +649,7 @@ __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp
       smom[half][3][slotB] = p3;
     }
     if (e + 1 < NE) {
-#pragma unroll
-      for (int k = 0; k < QPT; ++k) {
-        const int u = t + 256 * k;
-        if (u < QN) sq[cur ^ 1][u] = qn[k];
-      }
+      if (t < kSqN) sq[cur ^ 1][t] = qn;
       if (t < 24) {
         scol[cur ^ 1][t] = cn;
         sknown[cur ^ 1][t] = kn;
@@ -661,23 +658,29 @@ __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp
     barrier_lds();
     {  // (3) contraction and fold: straight-line for every lane (lanes past row 59 and rows of an
        // order not asked for compute row 59's values and store them to the sink)
-      const double* nv = sq[cur] + 3;  // the element's normal (equal at every point)
+      const double* nv = sq[cur] + 192;  // the element's normal (equal at every point)
+#pragma unroll
+      for (int i = 0; i < 7; ++i) asm volatile("" : "+v"(RP.po[i]));  // unpack here, not before the loop
+      asm volatile("" : "+v"(RP.kl));
+      const RowTerms RT = row_terms_unpack(RP, M);
       double su[3], st[3];
-      row_gp2_t(RT, TabSum2{smom[0][q], smom[1][q]}, nv, su, st);
+      row_gp2_t(RT, TabSum2{smom[0][q] - kTabBase, smom[1][q] - kTabBase}, nv, su, st);
       double val[6];
 #pragma unroll
       for (int nn = 0; nn < 3; ++nn) {
         const double hv = -fac * su[nn], gv = fac * st[nn];
-        val[nn] = old[nn] + hv;
-        val[3 + nn] = old[3 + nn] - gv;
+        val[nn] = (rd[nn] ? ld[nn] : fw[nn]) + hv;
+        val[3 + nn] = (rd[3 + nn] ? ld[3 + nn] : fw[3 + nn]) - gv;
         const double kh = active && col[nn] < 0 ? sknown[cur][3 * q + nn] : 0.0;
         const double kg = active && col[3 + nn] < 0 ? sknown[cur][12 + 3 * q + nn] : 0.0;
         bacc += gv * kg - hv * kh;  // known values fold into b (as in k_rows)
+        sfwd[3 * q + nn][lane] = val[nn];
+        sfwd[12 + 3 * q + nn][lane] = val[3 + nn];
       }
 #pragma unroll
       for (int o = 0; o < 6; ++o) {
         const bool wr = active && write_A && col[o] >= 0;
-        double* dst = wr ? A + (long)(col[o] & ~kRowsColFirst) * n + row : sink + t;
+        double* dst = wr ? A + (long)(col[o] & kRowsColMask) * n + row : sink + t;
         *dst = val[o];
       }
     }

@@ -30,10 +30,12 @@ void launch_rows(const double* qp, const int* conn, const double* cen, int num, 
                  const double* uval, const double* tval, double* A, double* b, hipStream_t st,
                  bool write_A = true, const struct RowsMoment* mf = nullptr);
 // The moment form of the boundary->inclusion rows (every element planar): cw[e][g][q] =
-// w_g jac_g N_q(g); ecol[e][k] = the BC-folded column of element e's H (k < 12, node dof 3 q + n) or
-// G (k >= 12, slot 3 q + n) entries, -1 / -2 where the value is known, with kRowsColFirst set on the
-// first element (in element order) that touches the column; eknown: NE x 24 workspace; sink: 256
-// doubles that absorb the stores of entries not written (never read).
+// w_g jac_g N_q(g); ecol[e][k] = the BC-folded column (bits 0..23) of element e's H (k < 12, node
+// dof 3 q + n) or G (k >= 12, slot 3 q + n) entries, -1 / -2 where the value is known, with
+// kRowsColFirst set on the first element (in element order) that touches the column, and
+// kRowsColFwd with the column's index k' in element e - 1 (bits 24..28) where element e - 1 touches
+// it too; eknown: NE x 24 workspace; sink: 256 doubles that absorb the stores of entries not
+// written (never read).
 struct RowsMoment {
   const double* cw;
   const int* ecol;
@@ -41,6 +43,9 @@ struct RowsMoment {
   double* sink;
 };
 constexpr int kRowsColFirst = 1 << 30;
+constexpr int kRowsColFwd = 1 << 29;
+constexpr int kRowsColFwdShift = 24;
+constexpr int kRowsColMask = (1 << 24) - 1;
 // flag[e] = 1 when element e's 64 quadrature-point normals agree to rounding (planar element).
 void launch_planar(const double* qp, int NE, int* flag, double* cw, hipStream_t st);
 // (write_A = false: b only, accumulated in the same order as with the matrix entries)

@@ -918,6 +918,74 @@ IBEM_HD RowTerms row_terms(const Mat& M, const Slot& R) {
   return t;
 }
 
+// RowTerms with its 27 table positions (each < 128) packed four to an int and the row's constants
+// reduced to k, l and the order: 8 ints, unpacked to the same RowTerms where they are used.
+struct RowTermsP {
+  int po[7];
+  int kl;  // k | l << 2 | order << 4
+};
+
+IBEM_HD RowTermsP row_terms_pack(const RowTerms& t, int order) {
+  int o[28] = {0};
+  for (int u = 0; u < 6; ++u) {
+    o[u] = t.oT[u];
+    o[6 + u] = t.oS[u];
+  }
+  for (int n = 0; n < 3; ++n) {
+    o[12 + n] = t.oSn[n];
+    o[15 + n] = t.oTn[n];
+    o[18 + n] = t.oTnk[n];
+    o[21 + n] = t.oTnl[n];
+  }
+  o[24] = t.oTkl;
+  o[25] = t.oTk;
+  o[26] = t.oTl;
+  RowTermsP p;
+  for (int i = 0; i < 7; ++i)
+    p.po[i] = o[4 * i] | o[4 * i + 1] << 8 | o[4 * i + 2] << 16 | o[4 * i + 3] << 24;
+  p.kl = t.k | t.l << 2 | order << 4;
+  return p;
+}
+
+IBEM_HD RowTerms row_terms_unpack(const RowTermsP& p, const Mat& M) {
+  RowTerms t;
+  auto at = [&](int i) { return (p.po[i >> 2] >> (8 * (i & 3))) & 255; };
+#pragma unroll
+  for (int u = 0; u < 6; ++u) {
+    t.oT[u] = at(u);
+    t.oS[u] = at(6 + u);
+  }
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    t.oSn[n] = at(12 + n);
+    t.oTn[n] = at(15 + n);
+    t.oTnk[n] = at(18 + n);
+    t.oTnl[n] = at(21 + n);
+  }
+  t.oTkl = at(24);
+  t.oTk = at(25);
+  t.oTl = at(26);
+  t.k = p.kl & 3;
+  t.l = (p.kl >> 2) & 3;
+  const int order = p.kl >> 4;
+  const double kap = M.cg1 - 2.0 * M.cg2;
+  const bool kl = t.k == t.l;
+  t.A1 = kl ? 2.0 * M.dlam * M.mu0 * kap : 0.0;
+  t.A2 = 2.0 * M.dmu * M.lam0 * kap;
+  t.A3 = -4.0 * M.dmu * M.mu0 * M.cg2;
+  t.A4 = M.dmu * M.mu0 * M.cg1;
+  t.B1 = -2.0 * M.dmu * M.cg2;
+  t.B2 = M.dmu * M.cg1;
+  t.B3 = kl ? M.dlam * kap : 0.0;
+  t.sg = order == 1 ? -1.0 : 1.0;
+#pragma unroll
+  for (int n = 0; n < 3; ++n) {
+    t.dk[n] = n == t.k ? 1.0 : 0.0;
+    t.dl[n] = n == t.l ? 1.0 : 0.0;
+  }
+  return t;
+}
+
 // row_gp with every table value read at a precomputed position (no selects): the k- and l-shifted
 // entries come from the table, and the d_nk, d_nl terms are multiplied in. Same values as row_gp.
 // tab(o) returns the table value at slot o (one table, or the sum of partial tables).
