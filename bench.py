@@ -106,6 +106,7 @@ def calibrate_copy(gib: int = 1, reps: int = 3) -> None:
 
 
 def run_step(eng: Engine, mode: int, tol: float = 1e-10):
+    """reset, every assembly phase, the solve to tol, the stresses at the postprocess points."""
     eng.reset()
     eng.assemble()
     rel, it = eng.solve(mode, tol)

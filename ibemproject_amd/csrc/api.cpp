@@ -1670,35 +1670,32 @@ int ibem3d_get_ut(ibem3d_ctx* ctx, double* U, double* T, double* X60) {
   return 0;
 }
 
+// ---- stresses at the postprocess points on the device -----------------------------------------
+static int stress_alloc(ibem3d_ctx* ctx) {
+  if (ctx->d_stU) return 0;
+  if (int rc = dalloc(ctx, &ctx->d_stU, 3L * ctx->NN)) return rc;
+  if (int rc = dalloc(ctx, &ctx->d_stT, 12L * ctx->NE)) return rc;
+  return dalloc(ctx, &ctx->d_sts, 6L * ctx->nump);
+}
+
+// the stresses of the current solution d_x into d_sts, on stream st
+static void enqueue_stress(ibem3d_ctx* ctx, hipStream_t st) {
+  const int NN = ctx->NN, NE = ctx->NE;
+  launch_stress_inputs(ctx->d_x, ctx->d_node_col, ctx->d_uval, ctx->d_slot_col, ctx->d_tval, NN, NE,
+                       ctx->d_stU, ctx->d_stT, st);
+  launch_stress(ctx->d_qp, ctx->d_conn, ctx->d_nodes, NN, NE, ctx->d_cen, ctx->d_rad, ctx->num, ctx->d_pts,
+                ctx->nump, ctx->M, ctx->d_stU, ctx->d_stT, ctx->d_x + 3L * NN, ctx->d_sts, st);
+}
+
 int ibem3d_stress(ibem3d_ctx* ctx, double* s) {
   CTX_CHECK();
   if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "stress: no solution");
   if (ctx->nump == 0) return 0;
-  const int NN = ctx->NN, NE = ctx->NE;
-  std::vector<double> U(3L * NN + 78L * ctx->num), T(12L * NE);
-  if (int rc = ibem3d_get_ut(ctx, U.data(), T.data(), nullptr)) return rc;
-  // T in element-slot order for the kernel: slots whose flag is neither 0 nor 1 carry 0
-  std::vector<double> Tslot(12L * NE, 0.0);
-  {
-    long m = 0;
-    for (int e = 0; e < NE; ++e)
-      for (int i = 0; i < 12; ++i)
-        if (ctx->slot_col[12 * e + i] != -2) Tslot[12 * e + i] = T[m++];
-  }
-  // device copies of U's boundary part, T and the output: allocated on the first call, kept
-  if (!ctx->d_stU) {
-    if (int rc = dalloc(ctx, &ctx->d_stU, 3L * NN)) return rc;
-    if (int rc = dalloc(ctx, &ctx->d_stT, Tslot.size())) return rc;
-    if (int rc = dalloc(ctx, &ctx->d_sts, 6L * ctx->nump)) return rc;
-  }
-  HIPC(hipMemcpyAsync(ctx->d_stU, U.data(), sizeof(double) * 3L * NN, hipMemcpyHostToDevice, ctx->st));
-  HIPC(hipMemcpyAsync(ctx->d_stT, Tslot.data(), sizeof(double) * Tslot.size(), hipMemcpyHostToDevice,
-                      ctx->st));
+  // U and T from the device solution (no host round trip), then the kernel
+  if (int rc = stress_alloc(ctx)) return rc;
   {
     Scope sc(ctx, "post.stress");
-    launch_stress(ctx->d_qp, ctx->d_conn, ctx->d_nodes, NN, NE, ctx->d_cen, ctx->d_rad, ctx->num,
-                  ctx->d_pts, ctx->nump, ctx->M, ctx->d_stU, ctx->d_stT, ctx->d_x + 3L * NN,
-                  ctx->d_sts, ctx->st);
+    enqueue_stress(ctx, ctx->st);
   }
   if (int rc = after_launch(ctx, "k_stress")) return rc;
   HIPC(hipMemcpyAsync(s, ctx->d_sts, sizeof(double) * 6 * ctx->nump, hipMemcpyDeviceToHost, ctx->st));

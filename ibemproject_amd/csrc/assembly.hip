@@ -50,9 +50,10 @@ __global__ __launch_bounds__(64) void k_qpoints(const double* __restrict__ nodes
 // Workgroup = kInclTile consecutive target spheres (rows) x one source sphere h (columns). Per pair:
 // the four radial basis tables (d^beta of r, r^-1 outside; of rho .. rho^4 inside, |beta| <= 8) and
 // the potential coefficients, then the weighted tables Psi^w,g / Phi^w,g for the ten weights, all in
-// LDS; then thread = row of the tile: its table positions once (incl_row), and the 60 columns with
-// the column decode resolved at compile time (incl_entry_w), one coalesced column segment per
-// column (rows of consecutive targets are contiguous). The weighted tables replace the basis tables
+// LDS; then thread = row of the tile: its table positions once (incl_row), and the 60 columns
+// unrolled with the column decode resolved at compile time and the row's conditions as selects
+// (incl_row_entries), one coalesced column segment per column (rows of consecutive targets are
+// contiguous). The weighted tables replace the basis tables
 // in the same LDS (each thread holds its kInclPer weighted values in registers across a barrier), so
 // that 38 KB per workgroup lets four workgroups share a CU.
 constexpr int kInclThreads = 256;
@@ -61,7 +62,7 @@ constexpr int kInclPer = (kInclU + kInclThreads - 1) / kInclThreads;  // per thr
 static_assert(4 * kTabN <= kInclW, "a pair's basis tables fit in its weighted-table slot");
 static_assert(kInclTile * 64 <= kInclThreads, "one wave per target sphere in the entry phase");
 
-__global__ __launch_bounds__(kInclThreads) void k_incl(const double* __restrict__ cen,
+__global__ __launch_bounds__(kInclThreads, 4) void k_incl(const double* __restrict__ cen,
                                                        const double* __restrict__ rad, int num, long n,
                                                        long row0, Mat M, int rowmask,
                                                        double* __restrict__ A) {
@@ -122,15 +123,8 @@ __global__ __launch_bounds__(kInclThreads) void k_incl(const double* __restrict_
   if (!((rowmask >> R.order) & 1)) return;
   const long row = row0 + 60L * s + r;
   double* Acol = A + (row0 + 60L * h) * n + row;
-  const bool self = (s == h);
   const double* PS = sU + ps * kInclW;
-  const double* PH = PS + kWgt * kPS;
-  for (int mn = 0; mn < 6; ++mn) {
-    const InclCol K = incl_col(R, mn);
-#pragma unroll
-    for (int w = 0; w < kWgt; ++w)
-      Acol[(long)incl_colidx(mn, w) * n] = incl_entry_mw(R, K, PS, PH, M, self, w);
-  }
+  incl_row_entries(incl_row_pack(R), PS, PS + kWgt * kPS, M, s == h, Acol, n);
 }
 
 // ---- K3: boundary rows x eigenstrain columns ----------------------------------------------------

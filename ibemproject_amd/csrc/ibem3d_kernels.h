@@ -51,6 +51,9 @@ void launch_planar(const double* qp, int NE, int* flag, double* cw, hipStream_t 
 // (write_A = false: b only, accumulated in the same order as with the matrix entries)
 
 // ---- postprocess (post.hip) ----
+// U (3 NN node dofs) and T (12 NE element slots) of the stress kernel from the device solution x
+void launch_stress_inputs(const double* x, const int* node_col, const double* uval, const int* slot_col,
+                          const double* tval, int NN, int NE, double* U, double* T, hipStream_t st);
 void launch_stress(const double* qp, const int* conn, const double* nodes, int NN, int NE,
                    const double* cen, const double* rad, int num, const double* pts, int nump,
                    const Mat& M, const double* U, const double* T, const double* E60, double* s,

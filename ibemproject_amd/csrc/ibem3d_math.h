@@ -37,6 +37,13 @@
 #else
 #define IBEM_HD inline
 #endif
+// Instruction-scheduling fence in device code (keeps the scheduler from interleaving unrolled
+// blocks, which bounds their live ranges); nothing in the host build of these functions.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define IBEM_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define IBEM_SCHED_FENCE() ((void)0)
+#endif
 
 namespace ibem {
 
@@ -744,6 +751,82 @@ IBEM_HD double incl_entry_mw(const InclRow& R, const InclCol& K, const double* P
   double v = -fac * sum;
   if (self && R.order == corder && R.p == cp && R.q == cq) v += C0(M, i, j, m, n);
   return mapw(m, n) * v;
+}
+
+// InclRow's 40 table positions (each < 128) packed four to an int, and (i, j, order, p, q).
+struct InclRowP {
+  int po[10];  // iS[6] iT[18] iXi[3] iXj[3] imn[6] i2k[3] iij
+  int meta;    // i | j << 2 | order << 4 | (p + 1) << 6 | (q + 1) << 8
+};
+
+IBEM_HD InclRowP incl_row_pack(const InclRow& R) {
+  int o[40];
+  for (int u = 0; u < 6; ++u) {
+    o[u] = R.iS[u];
+    o[30 + u] = R.imn[u];
+  }
+  for (int u = 0; u < 18; ++u) o[6 + u] = R.iT[u];
+  for (int x = 0; x < 3; ++x) {
+    o[24 + x] = R.iXi[x];
+    o[27 + x] = R.iXj[x];
+    o[36 + x] = R.i2k[x];
+  }
+  o[39] = R.iij;
+  InclRowP p;
+  for (int k = 0; k < 10; ++k) p.po[k] = o[4 * k] | o[4 * k + 1] << 8 | o[4 * k + 2] << 16 | o[4 * k + 3] << 24;
+  p.meta = R.i | R.j << 2 | R.order << 4 | (R.p + 1) << 6 | (R.q + 1) << 8;
+  return p;
+}
+
+// All 60 entries of row P of an inclusion block, written to Acol[c * lda] (c = the 60-layout
+// column): the same values and the same order of operations as incl_entry_mw, with the column loop
+// unrolled so that (m, n, w) are constants, and every term read from the tables and then kept or
+// dropped by a select, so that the row-dependent conditions (i == n, ...) cost no branches.
+IBEM_HD void incl_row_entries(const InclRowP& P, const double* PS, const double* PH, const Mat& M,
+                              bool self, double* Acol, long lda) {
+  auto at = [&](int k) { return (P.po[k >> 2] >> (8 * (k & 3))) & 255; };
+  const int i = P.meta & 3, j = (P.meta >> 2) & 3, order = (P.meta >> 4) & 3;
+  const int rp = ((P.meta >> 6) & 3) - 1, rq = ((P.meta >> 8) & 3) - 1;
+  const double rfac = order == 2 ? 0.5 : 1.0;
+  const double nu = M.nu0;
+  const bool dij = i == j;
+#pragma unroll
+  for (int mn = 0; mn < 6; ++mn) {
+    const int m = pair_i(mn), n = pair_j(mn);
+    const bool din = i == n, djn = j == n, dim = i == m, djm = j == m;
+    const double c0 = C0(M, i, j, m, n);
+    const int oS = at(mn), oT0 = at(6 + 3 * mn), oT1 = at(7 + 3 * mn), oT2 = at(8 + 3 * mn);
+    const int oXjm = at(27 + m), oXim = at(24 + m), oXjn = at(27 + n), oXin = at(24 + n);
+    const int omn = at(30 + mn), oij = at(39);
+#pragma unroll
+    for (int w = 0; w < kWgt; ++w) {
+      const double* ps = PS + w * kPS;
+      const double* ph = PH + w * kPH;
+      double D = ps[oS];
+      if (m == n) D -= 2.0 * nu * ph[oij];
+      const double a0 = ph[oXjm], a1 = ph[oXim], a2 = ph[oXjn], a3 = ph[oXin];
+      double t = 0.0;
+      t += din ? a0 : 0.0;
+      t += djn ? a1 : 0.0;
+      t += dim ? a2 : 0.0;
+      t += djm ? a3 : 0.0;
+      D -= (1.0 - nu) * t;
+      double sum = 2.0 * M.dmu * M.cD * D;
+      double tr = ps[oT0] + ps[oT1] + ps[oT2];
+      if (m == n) tr -= 2.0 * nu * (ph[at(36)] + ph[at(37)] + ph[at(38)]);
+      tr -= 4.0 * (1.0 - nu) * ph[omn];
+      sum += dij ? M.dlam * M.cD * tr : 0.0;
+      // the column's order and (p, q): p = w - 1 for first order, the pair w - 4 for second
+      const int corder = w == 0 ? 0 : (w < 4 ? 1 : 2);
+      const int cp = w == 0 ? -1 : (w < 4 ? w - 1 : (w - 4 < 3 ? 0 : (w - 4 < 5 ? 1 : 2)));
+      const int cq = w < 4 ? -1 : (w - 4 < 3 ? w - 4 : (w - 4 < 5 ? w - 6 : 2));
+      const double fac = rfac * ((corder == 2 && cp != cq) ? 2.0 : 1.0);
+      double v = -fac * sum;
+      v += (self && order == corder && rp == cp && rq == cq) ? c0 : 0.0;
+      Acol[(long)incl_colidx(mn, w) * lda] = mapw(m, n) * v;
+    }
+    IBEM_SCHED_FENCE();  // one pair's ten entries at a time: bounded live ranges
+  }
 }
 
 // Entry (row R, column c); equals incl_entry.

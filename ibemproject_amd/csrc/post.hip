@@ -151,6 +151,33 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
   }
 }
 
+// The stress kernel's inputs from the device solution x: U[i] for the 3 NN node dofs (unknown:
+// x[node_col[i]], known: uval[i]) and T per element slot (unknown: x[slot_col], known: tval, -2:
+// 0), the values getUTVector assembles (BEMBuilderElastic3D.cpp:2053-2117).
+__global__ __launch_bounds__(256) void k_stress_inputs(const double* __restrict__ x,
+                                                       const int* __restrict__ node_col,
+                                                       const double* __restrict__ uval,
+                                                       const int* __restrict__ slot_col,
+                                                       const double* __restrict__ tval, int NN, int NE,
+                                                       double* __restrict__ U, double* __restrict__ T) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < 3L * NN) {
+    const int c = node_col[i];
+    U[i] = c >= 0 ? x[c] : uval[i];
+  } else if (i < 3L * NN + 12L * NE) {
+    const long k = i - 3L * NN;
+    const int c = slot_col[k];
+    T[k] = c == -1 ? tval[k] : (c >= 0 ? x[c] : 0.0);
+  }
+}
+
+void launch_stress_inputs(const double* x, const int* node_col, const double* uval, const int* slot_col,
+                          const double* tval, int NN, int NE, double* U, double* T, hipStream_t st) {
+  const long m = 3L * NN + 12L * NE;
+  hipLaunchKernelGGL(k_stress_inputs, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, x, node_col,
+                     uval, slot_col, tval, NN, NE, U, T);
+}
+
 void launch_stress(const double* qp, const int* conn, const double* nodes, int NN, int NE,
                    const double* cen, const double* rad, int num, const double* pts, int nump,
                    const Mat& M, const double* U, const double* T, const double* E60, double* s,

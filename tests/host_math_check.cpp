@@ -158,6 +158,10 @@ void hc_incl_block_w(double E0, double nu0, double E1, double nu1, const double*
   for (int u = 0; u < kInclW; ++u) incl_weighted(basis.data(), C, u, PS.data(), PH.data());
   for (int r = 0; r < 60; ++r) {
     const InclRow R = incl_row(r);
+    if (self >= 2) {  // the unrolled, branch-free row (what k_incl runs); self = 3 for the self block
+      incl_row_entries(incl_row_pack(R), PS.data(), PH.data(), M, self == 3, out + r, 60);
+      continue;
+    }
     for (int c = 0; c < 60; ++c) out[c * 60 + r] = incl_entry_w(R, PS.data(), PH.data(), M, self != 0, c);
   }
 }

@@ -175,6 +175,10 @@ def test_inclusion_block_weighted_tables(hostmath):
         hostmath.hc_incl_block(1.0, 0.3, 10.0, 0.25, x.ctypes.data_as(dp), a, self_, o1.ctypes.data_as(dp))
         hostmath.hc_incl_block_w(1.0, 0.3, 10.0, 0.25, x.ctypes.data_as(dp), a, self_, o2.ctypes.data_as(dp))
         assert np.abs(o1 - o2).max() <= 1e-12 * np.abs(o1).max()
+        # the unrolled branch-free row form gives the per-entry form's values bit for bit
+        o3 = np.zeros(3600)
+        hostmath.hc_incl_block_w(1.0, 0.3, 10.0, 0.25, x.ctypes.data_as(dp), a, 2 + self_, o3.ctypes.data_as(dp))
+        assert np.array_equal(o3, o2)
 
 
 def test_rows_moment_form(hostmath):
