@@ -92,6 +92,7 @@ struct ibem3d_ctx {
   // every boundary element planar: the moment form of k_rows (weights, element columns, known values)
   double* d_cw = nullptr;
   int* d_ecol = nullptr;
+  int* d_wdesc = nullptr;      // k_incl's weighted-table descriptors (kInclW)
   double* d_eknown = nullptr;
   double* d_sink = nullptr;
   RowsMoment rows_mf{};
@@ -339,6 +340,8 @@ ibem3d_ctx* ibem3d_create(const ibem3d_problem* p, int device) {
   fill_zero(ctx->d_A, sizeof(double) * (size_t)n * 3 * p->NN, ctx->st);
   fill_zero(ctx->d_b, sizeof(double) * n, ctx->st);
   launch_qpoints(ctx->d_nodes, ctx->d_conn, p->NE, ctx->d_qp, ctx->st);
+  if (dalloc(ctx, &ctx->d_wdesc, kInclW)) return bail("");
+  launch_incl_desc(ctx->d_wdesc, ctx->st);
   {
     // planar elements (one normal at every quadrature point) let k_rows integrate the Kelvin tables
     // before the row contraction
@@ -405,7 +408,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
   ibem3d_reset_timings(ctx);
   for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
   ctx->pool.clear();
-  void* ptrs[] = {ctx->d_stU,   ctx->d_stT,      ctx->d_sts,      ctx->d_cw,  ctx->d_ecol, ctx->d_eknown, ctx->d_sink,
+  void* ptrs[] = {ctx->d_stU,   ctx->d_stT,      ctx->d_sts,      ctx->d_cw,  ctx->d_ecol, ctx->d_eknown, ctx->d_sink, ctx->d_wdesc,
                   ctx->d_nodes, ctx->d_cen,      ctx->d_rad,      ctx->d_pts,  ctx->d_qp,
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
@@ -497,7 +500,7 @@ int ibem3d_add_incl(ibem3d_ctx* ctx, int order) {
   if (int rc = phase_guard(ctx, (unsigned)mask << 2, "add_incl")) return rc;
   if (ctx->num > 0) {
     Scope s(ctx, "assemble.incl");
-    launch_incl(ctx->d_cen, ctx->d_rad, ctx->num, ctx->n, 3L * ctx->NN, ctx->M, mask, ctx->d_A,
+    launch_incl(ctx->d_cen, ctx->d_rad, ctx->num, ctx->n, 3L * ctx->NN, ctx->M, mask, ctx->d_wdesc, ctx->d_A,
                 ctx->st);
   }
   ctx->phases |= (unsigned)mask << 2;
@@ -1801,7 +1804,7 @@ int enqueue(ibem3d_ctx* ctx, int mode, int K, int nloads, const double* d_uvals,
              ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, ctx->d_bemws, st);
   if (ctx->num > 0) {
     launch_field(ctx->d_nodes, ctx->NN, ctx->d_cen, ctx->d_rad, ctx->num, n, 3 * NN, ctx->M, ctx->d_A, st);
-    launch_incl(ctx->d_cen, ctx->d_rad, ctx->num, n, 3 * NN, ctx->M, 7, ctx->d_A, st);
+    launch_incl(ctx->d_cen, ctx->d_rad, ctx->num, n, 3 * NN, ctx->M, 7, ctx->d_wdesc, ctx->d_A, st);
     launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, n, 3 * NN, ctx->M, 7, ctx->d_node_col,
                 ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, st, true, rows_moment_form(ctx));
   }

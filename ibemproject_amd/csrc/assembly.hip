@@ -65,6 +65,7 @@ static_assert(kInclTile * 64 <= kInclThreads, "one wave per target sphere in the
 __global__ __launch_bounds__(kInclThreads, 4) void k_incl(const double* __restrict__ cen,
                                                        const double* __restrict__ rad, int num, long n,
                                                        long row0, Mat M, int rowmask,
+                                                       const int* __restrict__ wdesc,
                                                        double* __restrict__ A) {
   // pair ps: basis tables [4][kTabN] at sU + ps kInclW, later its weighted tables [PS | PH] there
   __shared__ double sU[kInclU];
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(kInclThreads, 4) void k_incl(const double* __restri
   for (int k = 0; k < kInclPer; ++k) {
     const int u = t + kInclThreads * k;
     const int ps = u / kInclW, v = u % kInclW;
-    wv[k] = (u < kInclU && s0 + ps < num) ? incl_weighted_val(sU + ps * kInclW, sC[ps], v) : 0.0;
+    wv[k] = (u < kInclU && s0 + ps < num) ? incl_weighted_val_d(sU + ps * kInclW, sC[ps], wdesc[v]) : 0.0;
   }
   __syncthreads();
 #pragma unroll
@@ -709,10 +710,20 @@ void launch_qpoints(const double* nodes, const int* conn, int NE, double* qp, hi
   hipLaunchKernelGGL(k_qpoints, dim3(NE), dim3(64), 0, st, nodes, conn, NE, qp);
 }
 
+// descriptors of the kInclW weighted values of a pair (incl_weighted_desc), computed once per context
+__global__ void k_incl_desc(int* __restrict__ desc) {
+  const int u = blockIdx.x * 256 + threadIdx.x;
+  if (u < kInclW) desc[u] = incl_weighted_desc(u);
+}
+
 void launch_incl(const double* cen, const double* rad, int num, long n, long row0, const Mat& M,
-                 int rowmask, double* A, hipStream_t st) {
+                 int rowmask, const int* wdesc, double* A, hipStream_t st) {
   dim3 grid((num + kInclTile - 1) / kInclTile, num);
-  hipLaunchKernelGGL(k_incl, grid, dim3(kInclThreads), 0, st, cen, rad, num, n, row0, M, rowmask, A);
+  hipLaunchKernelGGL(k_incl, grid, dim3(kInclThreads), 0, st, cen, rad, num, n, row0, M, rowmask, wdesc, A);
+}
+
+void launch_incl_desc(int* desc, hipStream_t st) {
+  hipLaunchKernelGGL(k_incl_desc, dim3((kInclW + 255) / 256), dim3(256), 0, st, desc);
 }
 
 void launch_field(const double* nodes, int NN, const double* cen, const double* rad, int num, long n,

@@ -14,8 +14,10 @@ constexpr int kInclTile = 4;   // target spheres per k_incl workgroup
 
 // ---- assembly (assembly.hip) ----
 void launch_qpoints(const double* nodes, const int* conn, int NE, double* qp, hipStream_t st);
+// wdesc: the kInclW pair-independent descriptors of the weighted tables (launch_incl_desc)
 void launch_incl(const double* cen, const double* rad, int num, long n, long row0, const Mat& M,
-                 int rowmask, double* A, hipStream_t st);
+                 int rowmask, const int* wdesc, double* A, hipStream_t st);
+void launch_incl_desc(int* desc, hipStream_t st);  // kInclW descriptors
 void launch_field(const double* nodes, int NN, const double* cen, const double* rad, int num, long n,
                   long row0, const Mat& M, double* A, hipStream_t st);
 // k_bem's workspace: kBemChunk elements x 24 values x 3 NN rows, plus 6 x 3 NN carried sums

@@ -609,10 +609,12 @@ constexpr int kPS = 84, kPH = 35, kWgt = 10;
 constexpr int kInclW = kWgt * (kPS + kPH);  // weighted entries per pair
 
 // Basis table value of radial function f at multi-index (b0,b1,b2): basis is [4][kTabN].
-IBEM_HD double fbasis(const double* basis, const double (*C)[4], int f, int b0, int b1, int b2) {
-  const int t = tidx(b0, b1, b2);
+IBEM_HD double fbasis_t(const double* basis, const double (*C)[4], int f, int t) {
   return C[f][0] * basis[t] + C[f][1] * basis[kTabN + t] + C[f][2] * basis[2 * kTabN + t] +
          C[f][3] * basis[3 * kTabN + t];
+}
+IBEM_HD double fbasis(const double* basis, const double (*C)[4], int f, int b0, int b1, int b2) {
+  return fbasis_t(basis, C, f, tidx(b0, b1, b2));
 }
 
 // One weighted entry u (0 <= u < kInclW): PS[u] for u < kWgt kPS, else PH[u - kWgt kPS].
@@ -638,6 +640,32 @@ IBEM_HD double incl_weighted_val(const double* basis, const double (*C)[4], int 
   }
   return val;
 }
+// The same value through a descriptor of u that does not depend on the pair (incl_weighted_desc):
+// basis position t1 of function f1, and for p == q second-order weights the second term t2, f2.
+IBEM_HD int incl_weighted_desc(int u) {
+  const bool psi = u < kWgt * kPS;
+  const int v = psi ? u : u - kWgt * kPS;
+  const int per = psi ? kPS : kPH;
+  const int w = v / per, g = v % per;
+  int c0, c1, c2;
+  tdecode(g, c0, c1, c2);
+  const int base = psi ? 4 : 0;
+  if (w == 0) return tidx(c0, c1, c2) | base << 8;
+  if (w < 4) {
+    const int p = w - 1;
+    return tidx(c0 + (p == 0), c1 + (p == 1), c2 + (p == 2)) | (base + 1) << 8;
+  }
+  const int p = pair_i(w - 4), q = pair_j(w - 4);
+  const int d = tidx(c0 + (p == 0) + (q == 0), c1 + (p == 1) + (q == 1), c2 + (p == 2) + (q == 2)) | (base + 3) << 8;
+  return p == q ? d | 1 << 11 | tidx(c0, c1, c2) << 12 | (base + 2) << 20 : d;
+}
+IBEM_HD double incl_weighted_val_d(const double* basis, const double (*C)[4], int d) {
+  const double val = fbasis_t(basis, C, (d >> 8) & 7, d & 255);
+  const bool two = (d >> 11) & 1;
+  const double val2 = fbasis_t(basis, C, two ? (d >> 20) & 7 : 0, two ? (d >> 12) & 255 : 0);
+  return two ? val + val2 : val;
+}
+
 IBEM_HD void incl_weighted(const double* basis, const double (*C)[4], int u, double* PS, double* PH) {
   const double val = incl_weighted_val(basis, C, u);
   if (u < kWgt * kPS)

@@ -156,6 +156,14 @@ void hc_incl_block_w(double E0, double nu0, double E1, double nu1, const double*
     basis[3 * kTabN + t] = B3;
   }
   for (int u = 0; u < kInclW; ++u) incl_weighted(basis.data(), C, u, PS.data(), PH.data());
+  if (self >= 2) {  // the weighted tables through the pair-independent descriptors (k_incl)
+    for (int u = 0; u < kInclW; ++u) {
+      const double v = incl_weighted_val_d(basis.data(), C, incl_weighted_desc(u));
+      const double ref = u < kWgt * kPS ? PS[u] : PH[u - kWgt * kPS];
+      if (!(v == ref)) out[0] = 1e300;  // descriptor path disagrees
+      if (u < kWgt * kPS) PS[u] = v; else PH[u - kWgt * kPS] = v;
+    }
+  }
   for (int r = 0; r < 60; ++r) {
     const InclRow R = incl_row(r);
     if (self >= 2) {  // the unrolled, branch-free row (what k_incl runs); self = 3 for the self block
