@@ -64,6 +64,11 @@ struct ibem3d_ctx {
   // block-Jacobi preconditioner (IBEM3D_SOLVE_BJ_GMRES): the equilibrated boundary block R A_bb C
   // (nb x nb, nb = 3 NN) with its LU workspace, and the inverse of every sphere's 60 x 60 diagonal block
   double *d_Abb = nullptr, *d_rb = nullptr, *d_cb = nullptr, *d_Binv = nullptr, *d_tb = nullptr;
+  // the boundary block's natural column order (column of row k = node dof k's own unknown), when
+  // the column map makes it a permutation; the block is then factored without interchanges
+  int* d_bperm = nullptr;
+  std::vector<int> h_bperm;
+  bool bj_nopiv = false;       // the current boundary factor is of the permuted block
   LuWork Wb;
   // after a block-Jacobi solve the next assembly factors the boundary block and inverts the sphere
   // blocks on a second stream, beside the assembly kernels that do not write them (st_aux, events
@@ -241,6 +246,17 @@ int build_column_map(ibem3d_ctx* ctx, const ibem3d_problem* p) {
     return fail(ctx, IBEM3D_EINVAL,
                 "boundary conditions: displacement unknowns + traction keys = " + std::to_string(t) +
                     " != 3 NN = " + std::to_string(3 * NN));
+  // the natural column of row k (node dof k): its displacement unknown, else its traction key; kept
+  // when that is a permutation of the boundary columns
+  ctx->h_bperm.assign(3 * NN, -1);
+  std::vector<char> hit(3 * NN, 0);
+  bool perm_ok = true;
+  for (int k = 0; k < 3 * NN && perm_ok; ++k) {
+    const int c = ctx->node_col[k] >= 0 ? ctx->node_col[k] : key[k];
+    if (c < 0 || hit[c]) perm_ok = false;
+    else hit[c] = 1, ctx->h_bperm[k] = c;
+  }
+  if (!perm_ok) ctx->h_bperm.clear();
   return 0;
 }
 
@@ -332,6 +348,7 @@ ibem3d_ctx* ibem3d_create(const ibem3d_problem* p, int device) {
       upload(ctx, &ctx->d_tval, tval.data(), tval.size()) ||
       upload(ctx, &ctx->d_node_col, ctx->node_col.data(), ctx->node_col.size()) ||
       upload(ctx, &ctx->d_slot_col, ctx->slot_col.data(), ctx->slot_col.size()) ||
+      (!ctx->h_bperm.empty() && upload(ctx, &ctx->d_bperm, ctx->h_bperm.data(), ctx->h_bperm.size())) ||
       dalloc(ctx, &ctx->d_qp, (size_t)p->NE * 64 * kQpStride) ||
       dalloc(ctx, &ctx->d_A, (size_t)n * n) || dalloc(ctx, &ctx->d_b, (size_t)n) ||
       dalloc(ctx, &ctx->d_x, (size_t)n))
@@ -408,7 +425,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
   ibem3d_reset_timings(ctx);
   for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
   ctx->pool.clear();
-  void* ptrs[] = {ctx->d_stU,   ctx->d_stT,      ctx->d_sts,      ctx->d_cw,  ctx->d_ecol, ctx->d_eknown, ctx->d_sink, ctx->d_wdesc,
+  void* ptrs[] = {ctx->d_stU,   ctx->d_stT,      ctx->d_sts,      ctx->d_cw,  ctx->d_ecol, ctx->d_eknown, ctx->d_sink, ctx->d_wdesc, ctx->d_bperm,
                   ctx->d_nodes, ctx->d_cen,      ctx->d_rad,      ctx->d_pts,  ctx->d_qp,
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
@@ -535,7 +552,7 @@ int ibem3d_apply_bc(ibem3d_ctx* ctx) {
 
 // the block-Jacobi preconditioner's setup (defined with the solver below)
 static int bj_alloc(ibem3d_ctx* ctx);
-static int bj_enqueue_boundary(ibem3d_ctx* ctx, hipStream_t st);
+static int bj_enqueue_boundary(ibem3d_ctx* ctx, hipStream_t st, bool allow_nopiv = true);
 static void bj_enqueue_blocks(ibem3d_ctx* ctx, hipStream_t st);
 static int join_aux(ibem3d_ctx* ctx);
 
@@ -660,6 +677,11 @@ static int ensure_solver_ws(ibem3d_ctx* ctx) {
 __global__ void k_scale_vec(double* y, const double* x, const double* s, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = x[i] * s[i];
+}
+// y[perm[i]] = x[i] s[i]: the solution of the column-permuted boundary block back in column order
+__global__ void k_scale_scatter(double* y, const double* x, const double* s, const int* perm, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[perm[i]] = x[i] * s[i];
 }
 __global__ void k_scale_to_f32(float* y, const double* x, const double* s, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -811,11 +833,20 @@ __global__ __launch_bounds__(64) void k_block_mv(const double* __restrict__ inv,
 
 // z = M^-1 v with M = blockdiag(A_bb, A_11, ..., A_num,num): the boundary block through its
 // equilibrated LU, C_b (R_b A_bb C_b)^-1 R_b, the sphere blocks through their inverses
+// x_b = C t for the boundary block's solution t (scattered back when its columns were permuted)
+static void bj_unscale(ibem3d_ctx* ctx, double* xb, const double* t, hipStream_t st) {
+  const long nb = 3L * ctx->NN;
+  if (ctx->bj_nopiv)
+    hipLaunchKernelGGL(k_scale_scatter, vgrid(nb), dim3(256), 0, st, xb, t, ctx->d_cb, ctx->d_bperm, nb);
+  else
+    hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, st, xb, t, ctx->d_cb, nb);
+}
+
 static void bj_apply(ibem3d_ctx* ctx, const double* v, double* z) {
   const long nb = 3L * ctx->NN;
   hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, ctx->d_tb, v, ctx->d_rb, nb);
   lu_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->d_tb, ctx->st);
-  hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, z, ctx->d_tb, ctx->d_cb, nb);
+  bj_unscale(ctx, z, ctx->d_tb, ctx->st);
   if (ctx->num > 0)
     hipLaunchKernelGGL(k_block_mv, dim3(ctx->num), dim3(64), 0, ctx->st, ctx->d_Binv, v, z, nb);
 }
@@ -838,11 +869,23 @@ static int bj_alloc(ibem3d_ctx* ctx) {
 }
 
 // the boundary block's copy, equilibration and LU on stream st (reads A's first 3 NN rows and columns)
-static int bj_enqueue_boundary(ibem3d_ctx* ctx, hipStream_t st) {
+// Pivots of the equilibrated, column-permuted boundary block below this fall back to partial pivoting
+constexpr double kBjPivTol = 1e-6;
+
+static int bj_enqueue_boundary(ibem3d_ctx* ctx, hipStream_t st, bool allow_nopiv) {
   const long n = ctx->n, nb = 3L * ctx->NN;
-  equilibrate_copy_f64(ctx->d_A, nb, n, ctx->d_rb, ctx->d_cb, ctx->d_Abb, nb, ctx->d_eqws, st);
-  if (lu_factor_f64(ctx->d_Abb, nb, nb, ctx->Wb, st, nullptr, nullptr))
-    return fail(ctx, IBEM3D_EHIP, "boundary block LU launch failed");
+  const long mode = get_tuning(kTuneBjNopiv);
+  ctx->bj_nopiv = allow_nopiv && ctx->d_bperm && mode != 0;
+  if (ctx->bj_nopiv) {
+    // rows and their own unknowns on the diagonal: no interchanges (info reports a small pivot)
+    equilibrate_copy_f64(ctx->d_A, nb, n, ctx->d_rb, ctx->d_cb, ctx->d_Abb, nb, ctx->d_eqws, st, ctx->d_bperm);
+    if (lu_factor_f64_nopiv(ctx->d_Abb, nb, nb, ctx->Wb, st, mode == 2 ? 1e300 : kBjPivTol))
+      return fail(ctx, IBEM3D_EHIP, "boundary block LU launch failed");
+  } else {
+    equilibrate_copy_f64(ctx->d_A, nb, n, ctx->d_rb, ctx->d_cb, ctx->d_Abb, nb, ctx->d_eqws, st);
+    if (lu_factor_f64(ctx->d_Abb, nb, nb, ctx->Wb, st, nullptr, nullptr))
+      return fail(ctx, IBEM3D_EHIP, "boundary block LU launch failed");
+  }
   if (lu_prepare_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, st))
     return fail(ctx, IBEM3D_EHIP, "boundary block solve preparation failed");
   return 0;
@@ -858,6 +901,12 @@ static int bj_check(ibem3d_ctx* ctx) {
   int info = 0;
   HIPC(hipMemcpyAsync(&info, ctx->Wb.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
   HIPC(hipStreamSynchronize(ctx->st));
+  if (info && ctx->bj_nopiv) {
+    // a small pivot without interchanges: the boundary block again, with partial pivoting
+    if (int rc = bj_enqueue_boundary(ctx, ctx->st, false)) return rc;
+    HIPC(hipMemcpyAsync(&info, ctx->Wb.d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->st));
+    HIPC(hipStreamSynchronize(ctx->st));
+  }
   if (info) return fail(ctx, IBEM3D_ESOLVE, "boundary block LU: zero pivot");
   ctx->bj_ready = true;
   return after_launch(ctx, "block-Jacobi setup");
@@ -1339,7 +1388,7 @@ static int solve_bj(ibem3d_ctx* ctx, double tol, bool reuse, int mode) {
     gemv_f64(n, n, -1.0, ctx->d_A, n, ctx->d_x, 1.0, ctx->d_res, ctx->d_gemvws, ctx->st);
     hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, ctx->d_tb, ctx->d_res, ctx->d_rb, nb);
     lu_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->d_tb, ctx->st);
-    hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, ctx->d_x, ctx->d_tb, ctx->d_cb, nb);
+    bj_unscale(ctx, ctx->d_x, ctx->d_tb, ctx->st);
     double rel0 = 0.0;
     if ((rc = residual(ctx, &rel0))) return rc;
     // x0 <- alpha x0 with alpha minimising ||b - alpha A x0|| (A x0 = b - r0): the start is never
@@ -1436,6 +1485,8 @@ int ibem3d_set_initial_guess(ibem3d_ctx* ctx, const double* x_eig) {
 }
 
 int ibem3d_solve_fell_back(const ibem3d_ctx* ctx) { return ctx && ctx->bj_fallback ? 1 : 0; }
+
+int ibem3d_bj_nopiv(const ibem3d_ctx* ctx) { return ctx && ctx->bj_nopiv ? 1 : 0; }
 
 int ibem3d_rows_moment_form(const ibem3d_ctx* ctx) { return ctx && rows_moment_form(ctx) ? 1 : 0; }
 

@@ -109,6 +109,9 @@ size_t lu_moves_ints(long n);
 size_t lu_trinv_doubles();
 size_t lu_trsm_doubles(long n);
 int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx);
+// LU without row interchanges (identity ipiv, so the solves below apply unchanged); W.d_info = column
+// + 1 of the first pivot with magnitude below piv_tol
+int lu_factor_f64_nopiv(double* A, long n, long lda, LuWork& W, hipStream_t st, double piv_tol);
 // f16ws != nullptr: the trailing updates run on 16-bit MFMA products of type prod (kProd*) and the
 // panels' deep updates on split FP16 products (workspace of lu_f16_workspace_doubles(n) doubles);
 // otherwise both run on FP32 MFMA
@@ -146,8 +149,9 @@ void gemv_m16(long n, double alpha, const unsigned short* A16, long ld16, const 
 // on st and the rest on W's second stream, recording W.ev_cols: the next lu_factor_f32 on W must
 // then be called with cols_pending. Returns nonzero on a HIP error.
 size_t equilibrate_workspace_doubles(long n);
+// perm (optional): output column j is A's column perm[j] (c then belongs to the output's columns)
 void equilibrate_copy_f64(const double* A, long n, long lda, double* r, double* c, double* out, long ldo,
-                          double* ws, hipStream_t st);
+                          double* ws, hipStream_t st, const int* perm = nullptr);
 int equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
                          double* ws, hipStream_t st, LuWork* W = nullptr);
 // Compact storage (the factor takes the place of the FP64 system matrix): A <- R A C in place, or
@@ -172,8 +176,11 @@ void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, co
 // direction, at most kInnerItersMax; stopping tolerance 10^-exp).
 // kTuneRowsPointwise: 1 makes the boundary->inclusion rows use the per-Gauss-point contraction even
 // where every element is planar (default 0: the moment form there).
+// kTuneBjNopiv: the block-Jacobi boundary block's factorisation: 1 (default) without interchanges
+// in its natural column order where the column map allows, 0 always with partial pivoting, 2 with a
+// pivot threshold no pivot meets (the partial-pivoting fallback runs every time).
 enum { kTuneBigMinTiles = 1, kTunePanelCuPercent = 2, kTuneInnerIters = 3, kTuneInnerTolExp = 4,
-       kTuneRowsPointwise = 5 };
+       kTuneRowsPointwise = 5, kTuneBjNopiv = 6 };
 constexpr int kInnerItersMax = 16;
 int set_tuning(int key, long value);
 long get_tuning(int key);

@@ -887,6 +887,7 @@ static long g_panel_cu_percent = 100;
 static long g_inner_iters = 6;     // inner GMRES iterations on the 16-bit A per outer direction
 static long g_inner_tol_exp = 3;   // inner GMRES stops at relative residual 10^-exp
 static long g_rows_pointwise = 0;  // 1: per-Gauss-point boundary->inclusion rows on planar elements too
+static long g_bj_nopiv = 1;        // block-Jacobi boundary block: 1 no interchanges, 0 pivoting, 2 fallback
 int set_tuning(int key, long value) {
   if (key == kTuneBigMinTiles && value >= 1) {
     g_big_min_tiles = value;
@@ -908,6 +909,10 @@ int set_tuning(int key, long value) {
     g_rows_pointwise = value;
     return 0;
   }
+  if (key == kTuneBjNopiv && value >= 0 && value <= 2) {
+    g_bj_nopiv = value;
+    return 0;
+  }
   return -1;
 }
 long get_tuning(int key) {
@@ -917,6 +922,7 @@ long get_tuning(int key) {
     case kTuneInnerIters: return g_inner_iters;
     case kTuneInnerTolExp: return g_inner_tol_exp;
     case kTuneRowsPointwise: return g_rows_pointwise;
+    case kTuneBjNopiv: return g_bj_nopiv;
     default: return -1;
   }
 }
@@ -1733,6 +1739,80 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
   laswp<T>(c.A, lda, j0, w1, c.W->d_ipiv, j0 + w1, j0 + w, c.st);
 }
 
+// ================================================================ LU without interchanges =====
+// For a matrix whose equilibrated diagonal dominates (the block-Jacobi preconditioner's boundary
+// block in its natural column order: every row's own unknown on the diagonal), the LU needs no
+// pivoting: the panel's leaves become one launch each, with no candidate tree and no interchanges.
+// Leaf of w <= 32 columns at j0: every workgroup factors the w x w diagonal block in LDS (row r by
+// thread r, one barrier per step); each thread below the block then solves its row of L21 against
+// U11 in registers. Workgroup 0 writes the block back and the identity interchanges, and sets
+// info = column + 1 of the first pivot whose magnitude is below piv_tol (or not a number).
+template <typename T>
+__global__ __launch_bounds__(256) void k_leaf_nopiv(T* __restrict__ A, long lda, long n, long j0, int w,
+                                                   int* __restrict__ ipiv, int* __restrict__ info,
+                                                   double piv_tol) {
+  __shared__ T D[kPanelW][kPanelW + 1];
+  const int t = threadIdx.x;
+  for (int u = t; u < w * w; u += 256) {
+    const int r = u % w, c = u / w;
+    D[r][c] = A[(j0 + c) * lda + j0 + r];
+  }
+  __syncthreads();
+  for (int k = 0; k < w; ++k) {
+    if (t > k && t < w) {
+      const T l = D[t][k] / D[k][k];
+      D[t][k] = l;
+      for (int c = k + 1; c < w; ++c) D[t][c] -= l * D[k][c];
+    }
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && t < w) {
+    for (int c = 0; c < w; ++c) A[(j0 + c) * lda + j0 + t] = D[t][c];
+    ipiv[j0 + t] = (int)(j0 + t);
+    if (!(fabs((double)D[t][t]) >= piv_tol)) atomicCAS(info, 0, (int)(j0 + t + 1));
+  }
+  const long i = j0 + w + (long)blockIdx.x * 256 + t;
+  if (i >= n) return;
+  T x[kPanelW];
+#pragma unroll
+  for (int c = 0; c < kPanelW; ++c) x[c] = c < w ? A[(j0 + c) * lda + i] : T(0);
+#pragma unroll
+  for (int c = 0; c < kPanelW; ++c) {
+    if (c < w) {
+      T v = x[c];
+#pragma unroll
+      for (int k = 0; k < c; ++k) v -= x[k] * D[k][c];
+      x[c] = v / D[c][c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kPanelW; ++c)
+    if (c < w) A[(j0 + c) * lda + i] = x[c];
+}
+
+template <typename T>
+static void panel_rec_nopiv(LuCtx<T>& c, long j0, long w, double piv_tol) {
+  const long lda = c.lda;
+  if (w <= kPanelW) {
+    const long below = c.n - j0 - w;
+    const unsigned P = (unsigned)std::max<long>(1, (below + 255) / 256);
+    hipLaunchKernelGGL(k_leaf_nopiv<T>, dim3(P), dim3(256), 0, c.st, c.A, lda, c.n, j0, (int)w, c.W->d_ipiv,
+                       c.W->d_info, piv_tol);
+    return;
+  }
+  const long w1 = (w / 2 + kPanelW - 1) / kPanelW * kPanelW;
+  panel_rec_nopiv(c, j0, w1, piv_tol);
+  trsm_lower_unit<T>(c.A + j0 * lda + j0, lda, w1, c.A + (j0 + w1) * lda + j0, lda, w - w1, c.st);
+  const long mr = c.n - j0 - w1;
+  if ((w1 == 32 || w1 == 64) && w - w1 <= kSkinnyMax)
+    gemm_skinny<T>(mr, w - w1, w1, c.A + j0 * lda + j0 + w1, lda, c.A + (j0 + w1) * lda + j0, lda,
+                   c.A + (j0 + w1) * lda + j0 + w1, lda, c.st);
+  else
+    gemm_sub<T>(mr, w - w1, w1, c.A + j0 * lda + j0 + w1, lda, c.A + (j0 + w1) * lda + j0, lda,
+                c.A + (j0 + w1) * lda + j0 + w1, lda, c.st);
+  panel_rec_nopiv(c, j0 + w1, w - w1, piv_tol);
+}
+
 constexpr long kLuNB = IBEM_LU_NB;
 // CUs the bulk trailing update leaves idle for the concurrent panel, by trailing size m. The panel
 // (latency-bound, cost ~ a + b m) and the bulk update (~ m^2) overlap; under a full-GPU update the
@@ -1955,6 +2035,36 @@ static int lu_factor(T* A, long n, long lda, LuWork& W, hipStream_t st, LuTick t
 
 int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx) {
   return lu_factor<double>(A, n, lda, W, st, tick, tctx);
+}
+
+// The blocked right-looking LU of lu_factor with its look-ahead, without interchanges (panels by
+// panel_rec_nopiv); W.d_info reports a pivot below piv_tol.
+int lu_factor_f64_nopiv(double* A, long n, long lda, LuWork& W, hipStream_t st, double piv_tol) {
+  if (lu_streams(W)) return -1;
+  LuCtx<double> c{A, n, lda, &W, st, nullptr, nullptr, 0, nullptr};
+  LuCtx<double> c2{A, n, lda, &W, W.st2, nullptr, nullptr, 0, nullptr};
+  fill_zero(W.d_info, sizeof(int), st);
+  panel_rec_nopiv(c, 0, std::min(kLuNB, n), piv_tol);
+  for (long k = 0; k < n; k += kLuNB) {
+    const long kb = std::min(kLuNB, n - k);
+    if (k > 0) hipStreamWaitEvent(st, W.ev_b, 0);  // panel k, factored on the second stream
+    if (k + kb >= n) break;
+    const long m = n - k - kb;
+    const long kb2 = std::min(kLuNB, m);
+    const double* L11 = A + k * lda + k;
+    trsm_lower_unit<double>(L11, lda, kb, A + (k + kb) * lda + k, lda, kb2, st);
+    trailing_update<double>(A, n, lda, k, kb, 0, kb2, nullptr, st);
+    hipEventRecord(W.ev_a, st);
+    const long nr = m - kb2;
+    if (nr > 0) {
+      trsm_lower_unit<double>(L11, lda, kb, A + (k + kb + kb2) * lda + k, lda, nr, st);
+      trailing_update<double>(A, n, lda, k, kb, kb2, nr, nullptr, st);
+    }
+    hipStreamWaitEvent(W.st2, W.ev_a, 0);
+    panel_rec_nopiv(c2, k + kb, kb2, piv_tol);
+    hipEventRecord(W.ev_b, W.st2);
+  }
+  return 0;
 }
 int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
                   double* f16ws, bool cols_pending, int prod) {
@@ -2384,14 +2494,15 @@ __global__ __launch_bounds__(256) void k_rowmax_done(const double* __restrict__ 
 #define IBEM_COL_THREADS 1024
 #endif
 constexpr int kColThreads = IBEM_COL_THREADS;
+// perm (optional): output column j is A's column perm[j].
 template <typename TO>
 __global__ __launch_bounds__(kColThreads) void k_colscale_copy(const double* __restrict__ A, long n, long lda,
                                                                const double* __restrict__ r,
                                                                double* __restrict__ c, TO* __restrict__ out,
-                                                               long ldo, long j0) {
+                                                               long ldo, long j0, const int* __restrict__ perm) {
   __shared__ double red[kColThreads / 64];
   const long j = j0 + blockIdx.x;
-  const double* col = A + j * lda;
+  const double* col = A + (perm ? (long)perm[j] : j) * lda;
   double m = 0.0;
   for (long i = threadIdx.x; i < n; i += kColThreads) m = fmax(m, fabs(col[i] * r[i]));
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
@@ -2414,25 +2525,26 @@ size_t equilibrate_workspace_doubles(long n) { return (size_t)kRowChunks * n; }
 template <typename TO>
 static int equilibrate_copy(const double* A, long n, long lda, double* r, double* c, TO* out, long ldo,
                             double* ws, hipStream_t st, hipStream_t st_rest = nullptr,
-                            hipEvent_t ev_r = nullptr, hipEvent_t ev_rest = nullptr, long cols_first = 0) {
+                            hipEvent_t ev_r = nullptr, hipEvent_t ev_rest = nullptr, long cols_first = 0,
+                            const int* perm = nullptr) {
   hipLaunchKernelGGL(k_rowmax_part, dim3((unsigned)((n + 255) / 256), kRowChunks), dim3(256), 0, st, A,
                      n, lda, ws);
   hipLaunchKernelGGL(k_rowmax_done, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ws, n, r);
   const long first = st_rest && cols_first > 0 && cols_first < n ? cols_first : n;
   hipLaunchKernelGGL(k_colscale_copy<TO>, dim3((unsigned)first), dim3(kColThreads), 0, st, A, n, lda, r, c,
-                     out, ldo, 0L);
+                     out, ldo, 0L, perm);
   if (first < n) {
     if (hipEventRecord(ev_r, st) != hipSuccess) return -1;  // the row scales
     if (hipStreamWaitEvent(st_rest, ev_r, 0) != hipSuccess) return -1;
     hipLaunchKernelGGL(k_colscale_copy<TO>, dim3((unsigned)(n - first)), dim3(kColThreads), 0, st_rest, A, n,
-                       lda, r, c, out, ldo, first);
+                       lda, r, c, out, ldo, first, perm);
     if (hipEventRecord(ev_rest, st_rest) != hipSuccess) return -1;
   }
   return 0;
 }
 void equilibrate_copy_f64(const double* A, long n, long lda, double* r, double* c, double* out, long ldo,
-                          double* ws, hipStream_t st) {
-  (void)equilibrate_copy<double>(A, n, lda, r, c, out, ldo, ws, st);
+                          double* ws, hipStream_t st, const int* perm) {
+  (void)equilibrate_copy<double>(A, n, lda, r, c, out, ldo, ws, st, nullptr, nullptr, nullptr, 0, perm);
 }
 int equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
                          double* ws, hipStream_t st, LuWork* W) {
@@ -2486,13 +2598,13 @@ int equilibrate_to_f32_in_place(double* A, long n, double* r, double* c, long ld
                      n, n, ws);
   hipLaunchKernelGGL(k_rowmax_done, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ws, n, r);
   hipLaunchKernelGGL(k_colscale_copy<float>, dim3((unsigned)S), dim3(kColThreads), 0, st, A, n, n, r, c,
-                     tmp, lda32, 0L);
+                     tmp, lda32, 0L, nullptr);
   float* out = (float*)A;
   for (long j0 = S; j0 < n;) {
     const long j1 = std::min<long>(n, (2 * n * j0) / lda32);  // 4 lda32 j1 <= 8 n j0
     if (j1 <= j0) return -1;
     hipLaunchKernelGGL(k_colscale_copy<float>, dim3((unsigned)(j1 - j0)), dim3(kColThreads), 0, st, A, n,
-                       n, r, c, out, lda32, j0);
+                       n, r, c, out, lda32, j0, nullptr);
     j0 = j1;
   }
   if (hipMemcpyAsync(out, tmp, sizeof(float) * (size_t)S * lda32, hipMemcpyDeviceToDevice, st) != hipSuccess)

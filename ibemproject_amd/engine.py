@@ -35,6 +35,7 @@ TUNE_PANEL_CU_PERCENT = 2
 TUNE_INNER_ITERS = 3
 TUNE_INNER_TOL_EXP = 4
 TUNE_ROWS_POINTWISE = 5
+TUNE_BJ_NOPIV = 6
 
 
 def set_tuning(key: int, value: int) -> None:
@@ -74,7 +75,7 @@ EXPORTS = ["ibem3d_create", "ibem3d_destroy", "ibem3d_last_error", "ibem3d_size"
            "ibem3d_update_known_values", "ibem3d_batch_create", "ibem3d_batch_destroy",
            "ibem3d_batch_last_error", "ibem3d_batch_stiffness", "ibem3d_set_tuning",
            "ibem3d_get_tuning", "ibem3d_set_initial_guess", "ibem3d_solve_fell_back", "ibem3d_inner_iterations",
-           "ibem3d_rows_moment_form",
+           "ibem3d_rows_moment_form", "ibem3d_bj_nopiv",
            "ibem3d_warm_start_info",
            "ibem3d_gnn_last_error", "ibem3d_gnn_param_count", "ibem3d_gnn_graph", "ibem3d_gnn_create",
            "ibem3d_gnn_destroy", "ibem3d_gnn_get_params", "ibem3d_gnn_set_params", "ibem3d_gnn_set_data",
@@ -131,6 +132,7 @@ def load(path: str = LIB_PATH):
     L.ibem3d_set_initial_guess.argtypes = [ctypes.c_void_p, _dp]
     L.ibem3d_solve_fell_back.argtypes = [ctypes.c_void_p]
     L.ibem3d_rows_moment_form.argtypes = [ctypes.c_void_p]
+    L.ibem3d_bj_nopiv.argtypes = [ctypes.c_void_p]
     L.ibem3d_inner_iterations.restype = ctypes.c_long
     L.ibem3d_inner_iterations.argtypes = [ctypes.c_void_p]
     L.ibem3d_warm_start_info.argtypes = [ctypes.c_void_p, _dp, _dp]
@@ -336,6 +338,10 @@ class Engine:
 
     def fell_back(self) -> bool:
         return bool(self.L.ibem3d_solve_fell_back(self.ctx))
+
+    def bj_nopiv(self) -> bool:
+        """True when the block-Jacobi boundary factor has no row interchanges (natural column order)."""
+        return bool(self.L.ibem3d_bj_nopiv(self.ctx))
 
     def rows_moment_form(self) -> bool:
         """True when the boundary->inclusion rows are integrated in moment form (planar elements)."""

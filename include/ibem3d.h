@@ -122,10 +122,16 @@ enum {
                                         per outer direction (default 6, 1..16) */
   IBEM3D_TUNE_INNER_TOL_EXP = 4,     /* IBEM3D_SOLVE_BJ_GMRES16: the inner GMRES stops at relative
                                         residual 10^-value (default 3, 1..8) */
-  IBEM3D_TUNE_ROWS_POINTWISE = 5     /* 1: the boundary->inclusion rows contract every Gauss point's
+  IBEM3D_TUNE_ROWS_POINTWISE = 5,    /* 1: the boundary->inclusion rows contract every Gauss point's
                                         Kelvin table (the reference's order of sums) even where every
                                         element is planar; 0 (default): there the tables are integrated
                                         over the element first (4 contractions per element and row) */
+  IBEM3D_TUNE_BJ_NOPIV = 6           /* the block-Jacobi boundary block: 1 (default) factored without
+                                        row interchanges in its natural column order (each boundary
+                                        dof's own unknown on the diagonal) when the column map gives
+                                        one, falling back to partial pivoting on a pivot below 1e-6;
+                                        0 always partial pivoting; 2 a threshold no pivot meets, so
+                                        the fallback runs every time */
 };
 int ibem3d_set_tuning(int key, long value);
 long ibem3d_get_tuning(int key);  /* -1 for an unknown key */
@@ -207,6 +213,10 @@ long ibem3d_inner_iterations(const ibem3d_ctx* ctx);
  * 0 for the per-point contraction (a non-planar element, or IBEM3D_TUNE_ROWS_POINTWISE = 1). The
  * entries agree to rounding. No counterpart in the reference. */
 int ibem3d_rows_moment_form(const ibem3d_ctx* ctx);
+/* 1 when the block-Jacobi preconditioner's current boundary factor was computed without row
+ * interchanges (IBEM3D_TUNE_BJ_NOPIV), 0 with partial pivoting or before any block-Jacobi setup.
+ * No counterpart in the reference. */
+int ibem3d_bj_nopiv(const ibem3d_ctx* ctx);
 /* The last block-Jacobi solve's start from the initial guess: relative residual of the scaled guess
  * alpha x0 (alpha in [0, 1] minimises ||b - alpha A x0||, so the start is never worse than zero);
  * rel0 = -1 when no guess was used. */

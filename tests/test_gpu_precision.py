@@ -235,3 +235,39 @@ def test_block_jacobi_setup_beside_assembly(mid):
     _, _, x = eng.getUTVector()
     assert np.linalg.norm(x - xs[0][3]) / np.linalg.norm(xs[0][3]) < 1e-8
     eng.close()
+
+
+def test_bj_boundary_without_interchanges(mid):
+    """The block-Jacobi boundary block factored without row interchanges in its natural column order
+    (the default), with partial pivoting, and through the pivoting fallback (a threshold no pivot
+    meets): the three preconditioners are the same matrix inverse, so each solve reaches 1e-10 with
+    the same iteration counts (+-1) and the solutions agree to 1e-8 (the bar of the refined modes);
+    the second assembly's setup on the second stream takes the same path."""
+    case(f"block-Jacobi boundary block without interchanges, n60={mid['prob'].n60}")
+    out = {}
+    try:
+        for setting in (1, 0, 2):
+            E.set_tuning(E.TUNE_BJ_NOPIV, setting)
+            eng = E.Engine(mid["prob"])
+            res = []
+            for step in range(2):  # the second assembly builds the preconditioner beside it
+                eng.reset()
+                eng.assemble()
+                rel, it = eng.solve(E.SOLVE_BJ_GMRES16, 1e-10)
+                assert rel <= 1e-10 and not eng.fell_back()
+                _, _, X = eng.getUTVector()
+                res.append((it, eng.inner_iterations(), X, eng.bj_nopiv()))
+            rel_b, it_b = eng.solve(E.SOLVE_BJ_GMRES, 1e-10)
+            assert rel_b <= 1e-10
+            res.append((it_b, 0, eng.getUTVector()[2], eng.bj_nopiv()))
+            out[setting] = res
+            print(f"  setting {setting}: " + ", ".join(f"its {r[0]}/{r[1]} nopiv {r[3]}" for r in res))
+            eng.close()
+    finally:
+        E.set_tuning(E.TUNE_BJ_NOPIV, 1)
+    assert all(r[3] for r in out[1]) and not any(r[3] for r in out[0]) and not any(r[3] for r in out[2])
+    x0 = out[0][0][2]
+    for setting in (1, 2):
+        for a, b in zip(out[setting], out[0]):
+            assert abs(a[0] - b[0]) <= 1 and abs(a[1] - b[1]) <= 2
+            assert np.linalg.norm(a[2] - b[2]) / np.linalg.norm(x0) < 1e-8
