@@ -650,7 +650,7 @@ static int alloc_luwork(ibem3d_ctx* ctx, LuWork& W, long n) {
   if (int rc = dalloc(ctx, &W.d_ipiv, (size_t)n)) return rc;
   if (int rc = dalloc(ctx, &W.d_ws, lu_workspace_doubles(n))) return rc;
   fill_zero(W.d_ws, sizeof(double) * lu_workspace_doubles(n), ctx->st);
-  if (int rc = dalloc(ctx, &W.d_info, 1)) return rc;
+  if (int rc = dalloc(ctx, &W.d_info, 2)) return rc;  // info, and the no-pivot leaves' arrival count
   if (int rc = dalloc(ctx, &W.d_perm, lu_moves_ints(n))) return rc;
   if (int rc = dalloc(ctx, &W.d_inv, lu_inverse_doubles(n))) return rc;
   if (int rc = dalloc(ctx, &W.d_tmp, (size_t)n)) return rc;

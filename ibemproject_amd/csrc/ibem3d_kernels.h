@@ -95,7 +95,7 @@ struct LuWork {
   hipEvent_t ev_r = nullptr, ev_cols = nullptr;  // equilibration: row scales ready / last columns copied
   int* d_ipiv = nullptr;   // n
   double* d_ws = nullptr;  // lu_workspace_doubles(n), zero-filled once at allocation
-  int* d_info = nullptr;
+  int* d_info = nullptr;   // 2 ints: info, and the no-pivot LU leaves' arrival count
   int* d_perm = nullptr;   // lu_moves_ints(n): each step's interchanges as (dst, src) moves + counts
   double* d_inv = nullptr; // lu_inverse_doubles(n): inverses of the diagonal blocks of L and U
   double* d_tmp = nullptr; // n: solve scratch

@@ -1742,52 +1742,125 @@ static void panel_rec(LuCtx<T>& c, long j0, long w) {
 // ================================================================ LU without interchanges =====
 // For a matrix whose equilibrated diagonal dominates (the block-Jacobi preconditioner's boundary
 // block in its natural column order: every row's own unknown on the diagonal), the LU needs no
-// pivoting: the panel's leaves become one launch each, with no candidate tree and no interchanges.
-// Leaf of w <= 32 columns at j0: every workgroup factors the w x w diagonal block in LDS (row r by
-// thread r, one barrier per step); each thread below the block then solves its row of L21 against
-// U11 in registers. Workgroup 0 writes the block back and the identity interchanges, and sets
-// info = column + 1 of the first pivot whose magnitude is below piv_tol (or not a number).
+// pivoting: a panel's leaf is one launch, with no candidate tree and no interchanges.
+
+// lane k's value of v to every lane (readlane: a scalar broadcast, no LDS round trip)
+__device__ __forceinline__ double lane_bcast(double v, int k) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, k), hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), k);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float lane_bcast(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// Leaf of 32 columns at j0. Every workgroup: wavefront 0 loads the 32 x 32 diagonal block (row r in lane r's registers) and factors it, the pivot row
+// broadcast by readlane; the factored block L11\U11 and U11's reciprocal diagonal go to LDS; then
+// each thread below the block solves its row of L21 against U11, unrolled. The original block must
+// stay in A until every workgroup has read it: each counts its arrival (cnt[0]) once its wavefront 0
+// holds the block, and the last to arrive writes the factored block back from LDS, the identity
+// interchanges, info = column + 1 of the first pivot whose magnitude is below piv_tol (or not a
+// number), and resets the count for the next leaf.
 template <typename T>
-__global__ __launch_bounds__(256) void k_leaf_nopiv(T* __restrict__ A, long lda, long n, long j0, int w,
+__global__ __launch_bounds__(256) void k_leaf_nopiv(T* __restrict__ A, long lda, long n, long j0,
                                                    int* __restrict__ ipiv, int* __restrict__ info,
-                                                   double piv_tol) {
-  __shared__ T D[kPanelW][kPanelW + 1];
+                                                   int* __restrict__ cnt, double piv_tol) {
+  constexpr int W = kPanelW, w = kPanelW;
+  __shared__ T F[W][W + 1];
+  __shared__ T rdiag[W];
+  __shared__ int last;
   const int t = threadIdx.x;
-  for (int u = t; u < w * w; u += 256) {
-    const int r = u % w, c = u / w;
-    D[r][c] = A[(j0 + c) * lda + j0 + r];
+  if (t < 64) {
+    T a[W];
+    const int r = t < W ? t : W - 1;
+#pragma unroll
+    for (int c = 0; c < W; ++c) a[c] = A[(j0 + c) * lda + j0 + r];
+    if (t == 0) {
+      __atomic_thread_fence(__ATOMIC_SEQ_CST);  // the block's loads are done (their values are used)
+      last = (__hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+              (int)gridDim.x - 1);
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      {
+        const T piv = lane_bcast(a[k], k);
+        const bool below = t > k && t < w;
+        const T l = a[k] / piv;
+        if (below) a[k] = l;
+#pragma unroll
+        for (int c = k + 1; c < W; ++c) {
+          const T ukc = lane_bcast(a[c], k);
+          if (below) a[c] -= l * ukc;
+        }
+      }
+    }
+    if (t < W) {
+#pragma unroll
+      for (int c = 0; c < W; ++c) F[t][c] = a[c];
+      rdiag[t] = T(1) / a[t];
+    }
   }
   __syncthreads();
-  for (int k = 0; k < w; ++k) {
-    if (t > k && t < w) {
-      const T l = D[t][k] / D[k][k];
-      D[t][k] = l;
-      for (int c = k + 1; c < w; ++c) D[t][c] -= l * D[k][c];
-    }
-    __syncthreads();
-  }
-  if (blockIdx.x == 0 && t < w) {
-    for (int c = 0; c < w; ++c) A[(j0 + c) * lda + j0 + t] = D[t][c];
+  if (last && t < w) {
+    for (int c = 0; c < w; ++c) A[(j0 + c) * lda + j0 + t] = F[t][c];
     ipiv[j0 + t] = (int)(j0 + t);
-    if (!(fabs((double)D[t][t]) >= piv_tol)) atomicCAS(info, 0, (int)(j0 + t + 1));
+    if (!(fabs((double)F[t][t]) >= piv_tol)) atomicCAS(info, 0, (int)(j0 + t + 1));
+    if (t == 0) *cnt = 0;
   }
   const long i = j0 + w + (long)blockIdx.x * 256 + t;
   if (i >= n) return;
-  T x[kPanelW];
+  T x[W];
 #pragma unroll
-  for (int c = 0; c < kPanelW; ++c) x[c] = c < w ? A[(j0 + c) * lda + i] : T(0);
+  for (int c = 0; c < W; ++c) x[c] = A[(j0 + c) * lda + i];
 #pragma unroll
-  for (int c = 0; c < kPanelW; ++c) {
-    if (c < w) {
-      T v = x[c];
+  for (int c = 0; c < W; ++c) {
+    T v = x[c];
 #pragma unroll
-      for (int k = 0; k < c; ++k) v -= x[k] * D[k][c];
-      x[c] = v / D[c][c];
-    }
+    for (int k = 0; k < c; ++k) v -= x[k] * F[k][c];
+    x[c] = v * rdiag[c];
   }
 #pragma unroll
-  for (int c = 0; c < kPanelW; ++c)
-    if (c < w) A[(j0 + c) * lda + i] = x[c];
+  for (int c = 0; c < W; ++c)
+    A[(j0 + c) * lda + i] = x[c];
+}
+
+// The last, partial leaf (w < 32; once per factorisation): the diagonal block factored in LDS, row r
+// by thread r with a barrier per step, the same arrival count and write-back as k_leaf_nopiv.
+template <typename T>
+__global__ __launch_bounds__(256) void k_leaf_nopiv_part(T* __restrict__ A, long lda, long n, long j0, int w,
+                                                        int* __restrict__ ipiv, int* __restrict__ info,
+                                                        int* __restrict__ cnt, double piv_tol) {
+  constexpr int W = kPanelW;
+  __shared__ T F[W][W + 1];
+  __shared__ int last;
+  const int t = threadIdx.x;
+  for (int u = t; u < w * w; u += 256) F[u % w][u / w] = A[(j0 + u / w) * lda + j0 + u % w];
+  __syncthreads();
+  if (t == 0) {
+    __atomic_thread_fence(__ATOMIC_SEQ_CST);  // every thread's loads of the block are done
+    last = (__hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == (int)gridDim.x - 1);
+  }
+  for (int k = 0; k < w; ++k) {
+    if (t > k && t < w) {
+      const T l = F[t][k] / F[k][k];
+      F[t][k] = l;
+      for (int c = k + 1; c < w; ++c) F[t][c] -= l * F[k][c];
+    }
+    __syncthreads();
+  }
+  if (last && t < w) {
+    for (int c = 0; c < w; ++c) A[(j0 + c) * lda + j0 + t] = F[t][c];
+    ipiv[j0 + t] = (int)(j0 + t);
+    if (!(fabs((double)F[t][t]) >= piv_tol)) atomicCAS(info, 0, (int)(j0 + t + 1));
+    if (t == 0) *cnt = 0;
+  }
+  const long i = j0 + w + (long)blockIdx.x * 256 + t;
+  if (i >= n) return;
+  for (int c = 0; c < w; ++c) {  // x U11 = a, column by column in place
+    T v = A[(j0 + c) * lda + i];
+    for (int k = 0; k < c; ++k) v -= A[(j0 + k) * lda + i] * F[k][c];
+    A[(j0 + c) * lda + i] = v / F[c][c];
+  }
 }
 
 template <typename T>
@@ -1796,8 +1869,12 @@ static void panel_rec_nopiv(LuCtx<T>& c, long j0, long w, double piv_tol) {
   if (w <= kPanelW) {
     const long below = c.n - j0 - w;
     const unsigned P = (unsigned)std::max<long>(1, (below + 255) / 256);
-    hipLaunchKernelGGL(k_leaf_nopiv<T>, dim3(P), dim3(256), 0, c.st, c.A, lda, c.n, j0, (int)w, c.W->d_ipiv,
-                       c.W->d_info, piv_tol);
+    if (w == kPanelW)
+      hipLaunchKernelGGL(k_leaf_nopiv<T>, dim3(P), dim3(256), 0, c.st, c.A, lda, c.n, j0, c.W->d_ipiv,
+                         c.W->d_info, c.W->d_info + 1, piv_tol);
+    else
+      hipLaunchKernelGGL(k_leaf_nopiv_part<T>, dim3(P), dim3(256), 0, c.st, c.A, lda, c.n, j0, (int)w,
+                         c.W->d_ipiv, c.W->d_info, c.W->d_info + 1, piv_tol);
     return;
   }
   const long w1 = (w / 2 + kPanelW - 1) / kPanelW * kPanelW;
@@ -2039,18 +2116,22 @@ int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick
 
 // The blocked right-looking LU of lu_factor with its look-ahead, without interchanges (panels by
 // panel_rec_nopiv); W.d_info reports a pivot below piv_tol.
+#ifndef IBEM_NOPIV_NB
+#define IBEM_NOPIV_NB 256
+#endif
+constexpr long kNopivNB = IBEM_NOPIV_NB;  // panel width: the panel's own updates scale with it
 int lu_factor_f64_nopiv(double* A, long n, long lda, LuWork& W, hipStream_t st, double piv_tol) {
   if (lu_streams(W)) return -1;
   LuCtx<double> c{A, n, lda, &W, st, nullptr, nullptr, 0, nullptr};
   LuCtx<double> c2{A, n, lda, &W, W.st2, nullptr, nullptr, 0, nullptr};
-  fill_zero(W.d_info, sizeof(int), st);
-  panel_rec_nopiv(c, 0, std::min(kLuNB, n), piv_tol);
-  for (long k = 0; k < n; k += kLuNB) {
-    const long kb = std::min(kLuNB, n - k);
+  fill_zero(W.d_info, 2 * sizeof(int), st);  // info and the leaves' arrival count
+  panel_rec_nopiv(c, 0, std::min(kNopivNB, n), piv_tol);
+  for (long k = 0; k < n; k += kNopivNB) {
+    const long kb = std::min(kNopivNB, n - k);
     if (k > 0) hipStreamWaitEvent(st, W.ev_b, 0);  // panel k, factored on the second stream
     if (k + kb >= n) break;
     const long m = n - k - kb;
-    const long kb2 = std::min(kLuNB, m);
+    const long kb2 = std::min(kNopivNB, m);
     const double* L11 = A + k * lda + k;
     trsm_lower_unit<double>(L11, lda, kb, A + (k + kb) * lda + k, lda, kb2, st);
     trailing_update<double>(A, n, lda, k, kb, 0, kb2, nullptr, st);

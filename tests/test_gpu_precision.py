@@ -266,6 +266,8 @@ def test_bj_boundary_without_interchanges(mid):
     finally:
         E.set_tuning(E.TUNE_BJ_NOPIV, 1)
     assert all(r[3] for r in out[1]) and not any(r[3] for r in out[0]) and not any(r[3] for r in out[2])
+    for setting in (1, 0, 2):  # the two assemblies and solves of each setting agree bit for bit
+        assert out[setting][0][0] == out[setting][1][0] and np.array_equal(out[setting][0][2], out[setting][1][2])
     x0 = out[0][0][2]
     for setting in (1, 2):
         for a, b in zip(out[setting], out[0]):
