@@ -63,7 +63,7 @@ static_assert(4 * kTabN <= kInclW, "a pair's basis tables fit in its weighted-ta
 static_assert(kInclTile * 64 <= kInclThreads, "one wave per target sphere in the entry phase");
 
 __global__ __launch_bounds__(kInclThreads, 4) void k_incl(const double* __restrict__ cen,
-                                                       const double* __restrict__ rad, int num, long n,
+                                                       const double* __restrict__ rad, int num, long lda,
                                                        long row0, Mat M, int rowmask,
                                                        const int* __restrict__ wdesc,
                                                        double* __restrict__ A) {
@@ -123,9 +123,9 @@ __global__ __launch_bounds__(kInclThreads, 4) void k_incl(const double* __restri
   const InclRow R = incl_row(r);
   if (!((rowmask >> R.order) & 1)) return;
   const long row = row0 + 60L * s + r;
-  double* Acol = A + (row0 + 60L * h) * n + row;
+  double* Acol = A + (row0 + 60L * h) * lda + row;
   const double* PS = sU + ps * kInclW;
-  incl_row_entries(incl_row_pack(R), PS, PS + kWgt * kPS, M, s == h, Acol, n);
+  incl_row_entries(incl_row_pack(R), PS, PS + kWgt * kPS, M, s == h, Acol, lda);
 }
 
 // ---- K3: boundary rows x eigenstrain columns ----------------------------------------------------
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kInclThreads, 4) void k_incl(const double* __restri
 // each writes its row's 60 columns from the table.
 __global__ __launch_bounds__(192) void k_field(const double* __restrict__ nodes, int NN,
                                                const double* __restrict__ cen,
-                                               const double* __restrict__ rad, long n, long row0,
+                                               const double* __restrict__ rad, long lda, long row0,
                                                Mat M, double* __restrict__ A) {
   __shared__ double stab[64][kFieldTab];
   const int K = blockIdx.y;
@@ -153,8 +153,8 @@ __global__ __launch_bounds__(192) void k_field(const double* __restrict__ nodes,
   __syncthreads();
   if (j >= NN) return;
   const long row = 3L * j + i;
-  double* Acol = A + (row0 + 60L * K) * n + row;
-  for (int c = 0; c < 60; ++c) Acol[(long)c * n] = field_entry_tab(stab[jl], M, i, c);
+  double* Acol = A + (row0 + 60L * K) * lda + row;
+  for (int c = 0; c < 60; ++c) Acol[(long)c * lda] = field_entry_tab(stab[jl], M, i, c);
 }
 
 // ---- K4: boundary rows x boundary columns, b, rigid-body diagonal -------------------------------
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(192) void k_bem_quad(const double* __restrict__ qp,
 // row sum of H and the known-value terms of (row, c) across rounds ([c][R] each); the last round
 // adds the rigid-body diagonal and leaves the three known-value sums for k_bem_finish.
 __global__ __launch_bounds__(256) void k_bem_fold(const int* __restrict__ conn, int NN, int e0, int ne,
-                                                  long n, const int* __restrict__ node_col,
+                                                  long lda, const int* __restrict__ node_col,
                                                   const int* __restrict__ slot_col,
                                                   const double* __restrict__ uval,
                                                   const double* __restrict__ tval,
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void k_bem_fold(const int* __restrict__ conn, 
       rs += hv;
       const int col = node_col[dof];
       if (col >= 0) {
-        if (write_A) A[(long)col * n + row] += hv;
+        if (write_A) A[(long)col * lda + row] += hv;
       } else {
         bacc -= hv * uval[dof];
       }
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(256) void k_bem_fold(const int* __restrict__ conn, 
       const double gv = in[(3 * q + c) * R];
       const int kc = slot_col[slot];
       if (kc >= 0) {
-        if (write_A) A[(long)kc * n + row] -= gv;
+        if (write_A) A[(long)kc * lda + row] -= gv;
       } else if (kc == -1) {
         bacc += gv * tval[slot];
       }
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(256) void k_bem_fold(const int* __restrict__ conn, 
     const int dof = 3 * (int)(row / 3) + c;
     const int col = node_col[dof];
     if (col >= 0) {
-      if (write_A) A[(long)col * n + row] -= rs;
+      if (write_A) A[(long)col * lda + row] -= rs;
     } else {
       bacc += rs * uval[dof];
     }
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_bem_finish(int NN, const double* __rest
 // in element order: deterministic, no atomics.
 __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
                                               const int* __restrict__ conn,
-                                              const double* __restrict__ cen, int NE, long n,
+                                              const double* __restrict__ cen, int NE, long lda,
                                               long row0, Mat M, int rowmask,
                                               const int* __restrict__ node_col,
                                               const int* __restrict__ slot_col,
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
         const int og = o < 12 ? o : o - 12;
         const int k = 3 * (og % 4) + og / 4;  // 3 q + n
         col[oo] = scol[cur][(o < 12 ? 0 : 12) + k];
-        old[oo] = col[oo] >= 0 && write_A ? A[(long)col[oo] * n + row] : 0.0;
+        old[oo] = col[oo] >= 0 && write_A ? A[(long)col[oo] * lda + row] : 0.0;
       }
 #pragma unroll
       for (int oo = 0; oo < 6; ++oo) {
@@ -460,14 +460,14 @@ __global__ __launch_bounds__(256) void k_rows(const double* __restrict__ qp,
         if (o < 12) {
           const double hv = -fac * v[oo];
           if (col[oo] >= 0) {
-            if (write_A) A[(long)col[oo] * n + row] = old[oo] + hv;
+            if (write_A) A[(long)col[oo] * lda + row] = old[oo] + hv;
           } else {
             bacc -= hv * sknown[cur][k];
           }
         } else {
           const double gv = fac * v[oo];
           if (col[oo] >= 0) {
-            if (write_A) A[(long)col[oo] * n + row] = old[oo] - gv;
+            if (write_A) A[(long)col[oo] * lda + row] = old[oo] - gv;
           } else {
             bacc += gv * sknown[cur][12 + k];  // zero unless the traction is known
           }
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp
                                                 const double* __restrict__ cw,
                                                 const int* __restrict__ ecol,
                                                 const double* __restrict__ eknown,
-                                                const double* __restrict__ cen, int NE, long n,
+                                                const double* __restrict__ cen, int NE, long lda,
                                                 long row0, Mat M, int rowmask,
                                                 double* __restrict__ A, double* __restrict__ b,
                                                 int write_A, double* __restrict__ sink) {
@@ -600,7 +600,7 @@ __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp
 #pragma unroll
     for (int o = 0; o < 6; ++o) {
       rd[o] = active && write_A && col[o] >= 0 && !(col[o] & (kRowsColFirst | kRowsColFwd));
-      ld[o] = A[rd[o] ? (long)(col[o] & kRowsColMask) * n + row : row0];
+      ld[o] = A[rd[o] ? (long)(col[o] & kRowsColMask) * lda + row : row0];
     }
     {  // (1) tables
       const double* G = sq[cur] + 3 * lane;
@@ -675,7 +675,7 @@ __global__ __launch_bounds__(256, 2) void k_rows_m(const double* __restrict__ qp
 #pragma unroll
       for (int o = 0; o < 6; ++o) {
         const bool wr = active && write_A && col[o] >= 0;
-        double* dst = wr ? A + (long)(col[o] & kRowsColMask) * n + row : sink + t;
+        double* dst = wr ? A + (long)(col[o] & kRowsColMask) * lda + row : sink + t;
         *dst = val[o];
       }
     }
@@ -716,27 +716,27 @@ __global__ void k_incl_desc(int* __restrict__ desc) {
   if (u < kInclW) desc[u] = incl_weighted_desc(u);
 }
 
-void launch_incl(const double* cen, const double* rad, int num, long n, long row0, const Mat& M,
+void launch_incl(const double* cen, const double* rad, int num, long lda, long row0, const Mat& M,
                  int rowmask, const int* wdesc, double* A, hipStream_t st) {
   dim3 grid((num + kInclTile - 1) / kInclTile, num);
-  hipLaunchKernelGGL(k_incl, grid, dim3(kInclThreads), 0, st, cen, rad, num, n, row0, M, rowmask, wdesc, A);
+  hipLaunchKernelGGL(k_incl, grid, dim3(kInclThreads), 0, st, cen, rad, num, lda, row0, M, rowmask, wdesc, A);
 }
 
 void launch_incl_desc(int* desc, hipStream_t st) {
   hipLaunchKernelGGL(k_incl_desc, dim3((kInclW + 255) / 256), dim3(256), 0, st, desc);
 }
 
-void launch_field(const double* nodes, int NN, const double* cen, const double* rad, int num, long n,
+void launch_field(const double* nodes, int NN, const double* cen, const double* rad, int num, long lda,
                   long row0, const Mat& M, double* A, hipStream_t st) {
   dim3 grid((NN + 63) / 64, num);
-  hipLaunchKernelGGL(k_field, grid, dim3(192), 0, st, nodes, NN, cen, rad, n, row0, M, A);
+  hipLaunchKernelGGL(k_field, grid, dim3(192), 0, st, nodes, NN, cen, rad, lda, row0, M, A);
 }
 
 size_t bem_workspace_doubles(int NN, int NE) {
   return ((size_t)std::min(NE, kBemChunk) * 24 + 6) * 3 * (size_t)NN;
 }
 
-void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, int NE, long n,
+void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, int NE, long lda,
                 double E0, double nu0, const int* node_col, const int* slot_col, const double* uval,
                 const double* tval, double* A, double* b, double* ws, hipStream_t st, bool write_A) {
   const long R = 3L * NN;
@@ -747,23 +747,23 @@ void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, 
     hipLaunchKernelGGL(k_bem_quad, dim3((NN + 63) / 64, (ne + kBemEPB - 1) / kBemEPB), dim3(192), 0, st,
                        qp, conn, nodes, NN, e0, ne, E0, nu0, W);
     hipLaunchKernelGGL(k_bem_fold, dim3((unsigned)((R + 255) / 256), 3), dim3(256), 0, st, conn, NN, e0,
-                       ne, n, node_col, slot_col, uval, tval, W, carry, (int)(e0 == 0),
+                       ne, lda, node_col, slot_col, uval, tval, W, carry, (int)(e0 == 0),
                        (int)(e0 + ne >= NE), A, (int)write_A);
   }
   hipLaunchKernelGGL(k_bem_finish, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, st, NN, carry, b);
 }
 
-void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long n,
+void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long lda,
                  long row0, const Mat& M, int rowmask, const int* node_col, const int* slot_col,
                  const double* uval, const double* tval, double* A, double* b, hipStream_t st,
                  bool write_A, const RowsMoment* mf) {
   if (mf) {
     hipLaunchKernelGGL(k_elem_known, dim3((unsigned)((24L * NE + 255) / 256)), dim3(256), 0, st, conn, NE,
                        node_col, slot_col, uval, tval, mf->eknown);
-    hipLaunchKernelGGL(k_rows_m, dim3(num), dim3(256), 0, st, qp, mf->cw, mf->ecol, mf->eknown, cen, NE, n,
+    hipLaunchKernelGGL(k_rows_m, dim3(num), dim3(256), 0, st, qp, mf->cw, mf->ecol, mf->eknown, cen, NE, lda,
                        row0, M, rowmask, A, b, (int)write_A, mf->sink);
   } else
-    hipLaunchKernelGGL(k_rows, dim3(num), dim3(256), 0, st, qp, conn, cen, NE, n, row0, M, rowmask,
+    hipLaunchKernelGGL(k_rows, dim3(num), dim3(256), 0, st, qp, conn, cen, NE, lda, row0, M, rowmask,
                        node_col, slot_col, uval, tval, A, b, (int)write_A);
 }
 

@@ -61,8 +61,8 @@ void isotropic_stress(double E, double nu, const double* e, double* s) {
 // device bytes a context of n unknowns holds for a batched mixed solve (A, the FP32 factor, the split
 // workspace, solver workspaces, the six solutions), used to size the memory waves
 double rve_bytes(long n, int mode) {
-  const double lda32 = (double)((n + 63) / 64 * 64);
-  double b = 8.0 * n * n + 4.0 * lda32 * n;
+  const double lda = (double)((n + 63) / 64 * 64);  // A and the FP32 factor: 64-aligned columns
+  double b = 8.0 * lda * n + 4.0 * lda * n;
   if (mode != IBEM3D_SOLVE_MIXED_FP32) b += 2.0 * (4.0 * n * 1024 * 2 + 8.0 * n);
   b += 8.0 * 1024 * n + 2.0 * 8 * 128 * n + 8.0 * 32 * n * 2 + 8.0 * 6 * n + 64.0 * n;
   return b + 64.0 * (1 << 20);  // boundary quadrature workspace and small buffers

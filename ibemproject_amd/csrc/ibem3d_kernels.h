@@ -13,21 +13,22 @@ constexpr int kQpStride = 11;  // per Gauss point: g[3], nv[3], N[4], w*jac
 constexpr int kInclTile = 4;   // target spheres per k_incl workgroup
 
 // ---- assembly (assembly.hip) ----
+// A is column-major with leading dimension lda (>= n): entry (row, col) at A[col * lda + row].
 void launch_qpoints(const double* nodes, const int* conn, int NE, double* qp, hipStream_t st);
 // wdesc: the kInclW pair-independent descriptors of the weighted tables (launch_incl_desc)
-void launch_incl(const double* cen, const double* rad, int num, long n, long row0, const Mat& M,
+void launch_incl(const double* cen, const double* rad, int num, long lda, long row0, const Mat& M,
                  int rowmask, const int* wdesc, double* A, hipStream_t st);
 void launch_incl_desc(int* desc, hipStream_t st);  // kInclW descriptors
-void launch_field(const double* nodes, int NN, const double* cen, const double* rad, int num, long n,
+void launch_field(const double* nodes, int NN, const double* cen, const double* rad, int num, long lda,
                   long row0, const Mat& M, double* A, hipStream_t st);
 // k_bem's workspace: kBemChunk elements x 24 values x 3 NN rows, plus 6 x 3 NN carried sums
 constexpr int kBemChunk = 256;
 size_t bem_workspace_doubles(int NN, int NE);
-void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, int NE, long n,
+void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, int NE, long lda,
                 double E0, double nu0, const int* node_col, const int* slot_col, const double* uval,
                 const double* tval, double* A, double* b, double* ws, hipStream_t st,
                 bool write_A = true);
-void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long n,
+void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long lda,
                  long row0, const Mat& M, int rowmask, const int* node_col, const int* slot_col,
                  const double* uval, const double* tval, double* A, double* b, hipStream_t st,
                  bool write_A = true, const struct RowsMoment* mf = nullptr);
@@ -159,7 +160,7 @@ int equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c
 // (n <= lda32 < 2n), the first columns staged through tmp (at least two columns of lda32 floats).
 // Return nonzero on a HIP error or too small a tmp.
 void equilibrate_in_place_f64(double* A, long n, long lda, double* r, double* c, double* ws, hipStream_t st);
-int equilibrate_to_f32_in_place(double* A, long n, double* r, double* c, long lda32, float* tmp,
+int equilibrate_to_f32_in_place(double* A, long n, long lda, double* r, double* c, long lda32, float* tmp,
                                 size_t tmp_floats, double* ws, hipStream_t st);
 // FP16 copy of the FP32 factor: each 128 x 128 tile scaled by a power of two (largest magnitude in
 // [2^14, 2^15)), column-major with leading dimension ldf; isc holds factor16_scales(n) inverse

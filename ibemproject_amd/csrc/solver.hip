@@ -2664,28 +2664,28 @@ void equilibrate_in_place_f64(double* A, long n, long lda, double* r, double* c,
   hipLaunchKernelGGL(k_colscale_inplace, dim3((unsigned)n), dim3(kColThreads), 0, st, A, n, lda, r, c);
 }
 
-// FP64 A (n x n, leading dimension n) -> FP32 R A C with leading dimension lda32 (>= n) in the same
-// memory. FP32 column j occupies bytes [4 lda32 j, 4 lda32 (j + 1)), which lie inside FP64 columns
-// < j: columns are converted in increasing blocks [j0, j1) with 4 lda32 j1 <= 8 n j0, so that a
-// launch writes only over FP64 columns that earlier launches have read, and never over the columns
-// it reads itself. The first columns (whose FP32 place overlaps themselves) go through tmp first.
-// Returns nonzero when tmp holds fewer than two columns.
-int equilibrate_to_f32_in_place(double* A, long n, double* r, double* c, long lda32, float* tmp,
+// FP64 A (n x n, leading dimension lda >= n) -> FP32 R A C with leading dimension lda32 (>= n) in the
+// same memory. FP32 column j occupies bytes [4 lda32 j, 4 lda32 (j + 1)), which lie inside FP64
+// columns < j: columns are converted in increasing blocks [j0, j1) with 4 lda32 j1 <= 8 lda j0, so
+// that a launch writes only over FP64 columns that earlier launches have read, and never over the
+// columns it reads itself. The first columns (whose FP32 place overlaps themselves) go through tmp
+// first. Returns nonzero when tmp holds fewer than two columns.
+int equilibrate_to_f32_in_place(double* A, long n, long lda, double* r, double* c, long lda32, float* tmp,
                                 size_t tmp_floats, double* ws, hipStream_t st) {
-  if (lda32 < n || lda32 >= 2 * n) return -1;
+  if (lda < n || lda32 < n || lda32 >= 2 * lda) return -1;
   const long S = std::min<long>(n, (long)(tmp_floats / (size_t)lda32));
   if (S < 2 && S < n) return -1;
   hipLaunchKernelGGL(k_rowmax_part, dim3((unsigned)((n + 255) / 256), kRowChunks), dim3(256), 0, st, A,
-                     n, n, ws);
+                     n, lda, ws);
   hipLaunchKernelGGL(k_rowmax_done, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ws, n, r);
-  hipLaunchKernelGGL(k_colscale_copy<float>, dim3((unsigned)S), dim3(kColThreads), 0, st, A, n, n, r, c,
+  hipLaunchKernelGGL(k_colscale_copy<float>, dim3((unsigned)S), dim3(kColThreads), 0, st, A, n, lda, r, c,
                      tmp, lda32, 0L, nullptr);
   float* out = (float*)A;
   for (long j0 = S; j0 < n;) {
-    const long j1 = std::min<long>(n, (2 * n * j0) / lda32);  // 4 lda32 j1 <= 8 n j0
+    const long j1 = std::min<long>(n, (2 * lda * j0) / lda32);  // 4 lda32 j1 <= 8 lda j0
     if (j1 <= j0) return -1;
     hipLaunchKernelGGL(k_colscale_copy<float>, dim3((unsigned)(j1 - j0)), dim3(kColThreads), 0, st, A, n,
-                       n, r, c, out, lda32, j0, nullptr);
+                       lda, r, c, out, lda32, j0, nullptr);
     j0 = j1;
   }
   if (hipMemcpyAsync(out, tmp, sizeof(float) * (size_t)S * lda32, hipMemcpyDeviceToDevice, st) != hipSuccess)
