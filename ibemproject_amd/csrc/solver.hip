@@ -2827,17 +2827,20 @@ size_t m16_scale_count(long n) { return (size_t)(m16_ld(n) / kM16R) * (size_t)((
 
 // one workgroup per tile: thread t holds rows 2 (t % 128) and 2 (t % 128) + 1 of the 32 columns
 // c = t / 128 (mod 2) (16-byte nontemporal loads of A, read once), the tile maximum by wavefront and LDS
-// reductions, then the FP16 pairs written as 4-byte stores
+// reductions, then the FP16 values: a full tile goes through LDS, eight columns at a time, so that
+// every thread stores eight rows of one column (16 bytes); an edge tile stores its pairs directly
 __global__ __launch_bounds__(256) void k_m16_build(const double* __restrict__ A, long m, long n, long lda,
                                                    _Float16* __restrict__ A16, long ld16,
                                                    float* __restrict__ scale) {
   __shared__ double wmax[4];
+  __shared__ unsigned tp[4][kM16R];
   const long rt = blockIdx.x, ct = blockIdx.y;
   const long i = rt * kM16R + 2 * (threadIdx.x & 127);
   const int c0 = threadIdx.x >> 7;  // this thread's columns: c0, c0 + 2, ...
   const long j0 = ct * kM16C;
   const int nc = (int)min((long)kM16C, n - j0);
   const bool pair = ((lda & 1) == 0) && ((((size_t)A) & 15) == 0) && i + 1 < m;
+  const bool full = nc == kM16C && rt * kM16R + kM16R <= m && pair;  // uniform: lda, A, whole tile
   dvec2 v[kM16C / 2];
   double mx = 0.0;
 #pragma unroll
@@ -2866,14 +2869,33 @@ __global__ __launch_bounds__(256) void k_m16_build(const double* __restrict__ A,
   const double down = ldexp(1.0, -e);
   if (threadIdx.x == 0) scale[ct * gridDim.x + rt] = ldexpf(1.0f, e);
   typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+  unsigned pk[kM16C / 2];  // the two FP16 values of column c0 + 2 u
 #pragma unroll
   for (int u = 0; u < kM16C / 2; ++u) {
-    const int c = c0 + 2 * u;
-    if (c < nc) {
-      half2v h;
-      h.x = (_Float16)(float)(v[u].x * down);
-      h.y = (_Float16)(float)(v[u].y * down);
-      *(half2v*)(A16 + (j0 + c) * ld16 + i) = h;
+    half2v h;
+    h.x = (_Float16)(float)(v[u].x * down);
+    h.y = (_Float16)(float)(v[u].y * down);
+    pk[u] = __builtin_bit_cast(unsigned, h);
+  }
+  if (full) {
+    // group g: columns c0 + 2 (4 g + q), q < 4, of both halves; tp[q][128 c0 + t % 128] holds the pair
+    // of rows 2 (t % 128) .. + 1; thread (q, half cc, octet o) then stores rows 8 o .. 8 o + 7
+    const int q = threadIdx.x >> 6, cc = (threadIdx.x >> 5) & 1, oct = threadIdx.x & 31;
+#pragma unroll
+    for (int g = 0; g < kM16C / 8; ++g) {
+      __syncthreads();  // the previous group's reads of tp are done
+#pragma unroll
+      for (int k = 0; k < 4; ++k) tp[k][(threadIdx.x & 127) + 128 * c0] = pk[4 * g + k];
+      __syncthreads();
+      const unsigned* src = &tp[q][128 * cc + 4 * oct];
+      const uint4 w4 = make_uint4(src[0], src[1], src[2], src[3]);
+      *(uint4*)(A16 + (j0 + cc + 2 * (4 * g + q)) * ld16 + rt * kM16R + 8 * oct) = w4;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < kM16C / 2; ++u) {
+      const int c = c0 + 2 * u;
+      if (c < nc) *(unsigned*)(A16 + (j0 + c) * ld16 + i) = pk[u];
     }
   }
 }
