@@ -78,6 +78,7 @@ struct ibem3d_ctx {
   // matches the assembled A
   hipStream_t st_aux = nullptr;
   hipEvent_t ev_aux_in = nullptr, ev_aux_in2 = nullptr, ev_aux_out = nullptr;
+  hipEvent_t ev_ov_in = nullptr, ev_ov_out = nullptr;  // inner iterations: boundary solve beside the product
   bool bj_pending = false, bj_ready = false;
   // IBEM3D_SOLVE_BJ_GMRES16: the 16-bit copy of A (leading dimension ld16) with its tile scales, and
   // the inner GMRES's basis, directions and product vector
@@ -444,7 +445,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (ctx->st_aux) hipStreamDestroy(ctx->st_aux);
-  for (hipEvent_t e : {ctx->ev_aux_in, ctx->ev_aux_in2, ctx->ev_aux_out})
+  for (hipEvent_t e : {ctx->ev_aux_in, ctx->ev_aux_in2, ctx->ev_aux_out, ctx->ev_ov_in, ctx->ev_ov_out})
     if (e) hipEventDestroy(e);
   if (ctx->W.st2) hipStreamDestroy(ctx->W.st2);
   if (ctx->Wb.st2) hipStreamDestroy(ctx->Wb.st2);
@@ -559,6 +560,15 @@ static int bj_enqueue_boundary(ibem3d_ctx* ctx, hipStream_t st, bool allow_nopiv
 static void bj_enqueue_blocks(ibem3d_ctx* ctx, hipStream_t st);
 static int join_aux(ibem3d_ctx* ctx);
 
+// the second stream and the events that order it against the main stream
+static int ensure_aux(ibem3d_ctx* ctx) {
+  if (ctx->st_aux) return 0;
+  HIPC(hipStreamCreateWithFlags(&ctx->st_aux, hipStreamNonBlocking));
+  for (hipEvent_t* e : {&ctx->ev_aux_in, &ctx->ev_aux_in2, &ctx->ev_aux_out, &ctx->ev_ov_in, &ctx->ev_ov_out})
+    HIPC(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  return 0;
+}
+
 int ibem3d_assemble(ibem3d_ctx* ctx) {
   CTX_CHECK();
   int rc;
@@ -570,12 +580,7 @@ int ibem3d_assemble(ibem3d_ctx* ctx) {
   const bool bj_next = ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES || ctx->prepared_mode == IBEM3D_SOLVE_BJ_GMRES16;
   if (bj_next) {
     if ((rc = join_aux(ctx))) return rc;
-    if (!ctx->st_aux) {
-      HIPC(hipStreamCreateWithFlags(&ctx->st_aux, hipStreamNonBlocking));
-      HIPC(hipEventCreateWithFlags(&ctx->ev_aux_in, hipEventDisableTiming));
-      HIPC(hipEventCreateWithFlags(&ctx->ev_aux_in2, hipEventDisableTiming));
-      HIPC(hipEventCreateWithFlags(&ctx->ev_aux_out, hipEventDisableTiming));
-    }
+    if ((rc = ensure_aux(ctx))) return rc;
     if ((rc = bj_alloc(ctx))) return rc;
   }
   if ((rc = ibem3d_add_bem(ctx))) return rc;
@@ -852,10 +857,16 @@ static void bj_unscale(ibem3d_ctx* ctx, double* xb, const double* t, hipStream_t
     hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, st, xb, t, ctx->d_cb, nb);
 }
 
+// d_tb <- (R A_bb C)^-1 d_tb on stream st
+static void bj_boundary_solve(ibem3d_ctx* ctx, hipStream_t st) {
+  const long nb = 3L * ctx->NN;
+  lu_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->d_tb, st);
+}
+
 static void bj_apply(ibem3d_ctx* ctx, const double* v, double* z) {
   const long nb = 3L * ctx->NN;
   hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, ctx->d_tb, v, ctx->d_rb, nb);
-  lu_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->d_tb, ctx->st);
+  bj_boundary_solve(ctx, ctx->st);
   bj_unscale(ctx, z, ctx->d_tb, ctx->st);
   if (ctx->num > 0)
     hipLaunchKernelGGL(k_block_mv, dim3(ctx->num), dim3(64), 0, ctx->st, ctx->d_Binv, v, z, nb);
@@ -955,6 +966,43 @@ static void precond_solve(ibem3d_ctx* ctx, float* v) {
 // preconditioned by block Jacobi, CGS2 Arnoldi, at most kTuneInnerIters iterations, stopping at the
 // relative residual 10^-kTuneInnerTolExp (the 16-bit matrix holds A to about 2^-12, so a tighter
 // inner solve buys the outer iteration nothing)
+// z = M^-1 v, then w = A16 z. With IBEM3D_TUNE_BJ_OVERLAP the boundary block's triangular solves
+// (a latency-bound chain of small launches) run on the second stream while the product covers the
+// column tiles past the boundary block (chunks 4..15 of its partial sums, from the sphere blocks'
+// part of z); the tiles holding boundary columns (chunks 0..3) follow once that part of z is solved.
+// The chunk sums are added in the same fixed order either way.
+static int bj_apply_m16(ibem3d_ctx* ctx, const double* v, double* z, double* w) {
+  const long n = ctx->n, nb = 3L * ctx->NN;
+  const long tiles = m16_tiles(n), tb = (nb + 31) / 32;  // tiles [0, tb) hold the boundary columns
+  if (!get_tuning(kTuneBjOverlap) || tb >= tiles) {
+    bj_apply(ctx, v, z);
+    Scope s(ctx, "solve.gemv_m16");
+    gemv_m16(n, 1.0, ctx->d_A16, ctx->ld16, ctx->d_A16s, z, 0.0, w, ctx->d_gemvws, ctx->st);
+    return 0;
+  }
+  if (int rc = ensure_aux(ctx)) return rc;
+  hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, ctx->d_tb, v, ctx->d_rb, nb);
+  HIPC(hipEventRecord(ctx->ev_ov_in, ctx->st));
+  HIPC(hipStreamWaitEvent(ctx->st_aux, ctx->ev_ov_in, 0));
+  bj_boundary_solve(ctx, ctx->st_aux);
+  bj_unscale(ctx, z, ctx->d_tb, ctx->st_aux);
+  HIPC(hipEventRecord(ctx->ev_ov_out, ctx->st_aux));
+  if (ctx->num > 0)
+    hipLaunchKernelGGL(k_block_mv, dim3(ctx->num), dim3(64), 0, ctx->st, ctx->d_Binv, v, z, nb);
+  constexpr int kFirst = 4;  // chunks of the boundary tiles
+  {
+    Scope s(ctx, "solve.gemv_m16");
+    gemv_m16_part(n, ctx->d_A16, ctx->ld16, ctx->d_A16s, z, ctx->d_gemvws, tb, tiles, kFirst, 16 - kFirst, ctx->st);
+  }
+  HIPC(hipStreamWaitEvent(ctx->st, ctx->ev_ov_out, 0));
+  {
+    Scope s(ctx, "solve.gemv_m16");
+    gemv_m16_part(n, ctx->d_A16, ctx->ld16, ctx->d_A16s, z, ctx->d_gemvws, 0, tb, 0, kFirst, ctx->st);
+    gemv_m16_finish(n, 1.0, ctx->d_gemvws, 0.0, w, ctx->st);
+  }
+  return 0;
+}
+
 static int inner_m16(ibem3d_ctx* ctx, const double* v, double* z) {
   const long n = ctx->n;
   const int kmax = (int)get_tuning(kTuneInnerIters);
@@ -974,11 +1022,7 @@ static int inner_m16(ibem3d_ctx* ctx, const double* v, double* z) {
   for (int j = 0; j < kmax; ++j) {
     double* vj = V + (size_t)j * n;
     double* zj = Z + (size_t)j * n;
-    bj_apply(ctx, vj, zj);
-    {
-      Scope s(ctx, "solve.gemv_m16");
-      gemv_m16(n, 1.0, ctx->d_A16, ctx->ld16, ctx->d_A16s, zj, 0.0, w, ctx->d_gemvws, ctx->st);
-    }
+    if (int rc = bj_apply_m16(ctx, vj, zj, w)) return rc;
     double* Hj = H.data() + (size_t)j * (kmax + 1);
     std::vector<double> h(j + 1);
     for (int pass = 0; pass < 2; ++pass) {

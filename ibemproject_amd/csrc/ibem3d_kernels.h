@@ -145,6 +145,13 @@ size_t m16_scale_count(long n);
 void m16_build(const double* A, long n, long lda, unsigned short* A16, long ld16, float* scale, hipStream_t st);
 void gemv_m16(long n, double alpha, const unsigned short* A16, long ld16, const float* scale, const double* x,
               double beta, double* y, double* ws, hipStream_t st);
+// The same product in parts: the partial sums of column tiles [t0, t1) (32 columns each, m16_tiles(n)
+// in all) into chunks [chunk0, chunk0 + chunks) of ws; once all 16 chunks hold partial sums,
+// gemv_m16_finish adds them in a fixed order, y = alpha sum + beta y.
+long m16_tiles(long n);
+void gemv_m16_part(long n, const unsigned short* A16, long ld16, const float* scale, const double* x, double* ws,
+                   long t0, long t1, int chunk0, int chunks, hipStream_t st);
+void gemv_m16_finish(long n, double alpha, const double* ws, double beta, double* y, hipStream_t st);
 // out = R A C with the same scaling, A left unchanged (FP64 or FP32 output); ws holds
 // equilibrate_workspace_doubles(n) doubles. With W, the FP32 copy writes its first LU panel's columns
 // on st and the rest on W's second stream, recording W.ev_cols: the next lu_factor_f32 on W must
@@ -180,8 +187,10 @@ void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, co
 // kTuneBjNopiv: the block-Jacobi boundary block's factorisation: 1 (default) without interchanges
 // in its natural column order where the column map allows, 0 always with partial pivoting, 2 with a
 // pivot threshold no pivot meets (the partial-pivoting fallback runs every time).
+// kTuneBjOverlap: 1 (default) runs the boundary block's triangular solves of each inner iteration on
+// the second stream beside the FP16 product's sphere columns; 0 runs them before the product.
 enum { kTuneBigMinTiles = 1, kTunePanelCuPercent = 2, kTuneInnerIters = 3, kTuneInnerTolExp = 4,
-       kTuneRowsPointwise = 5, kTuneBjNopiv = 6 };
+       kTuneRowsPointwise = 5, kTuneBjNopiv = 6, kTuneBjOverlap = 7 };
 constexpr int kInnerItersMax = 16;
 int set_tuning(int key, long value);
 long get_tuning(int key);

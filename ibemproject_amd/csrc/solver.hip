@@ -888,6 +888,7 @@ static long g_inner_iters = 6;     // inner GMRES iterations on the 16-bit A per
 static long g_inner_tol_exp = 3;   // inner GMRES stops at relative residual 10^-exp
 static long g_rows_pointwise = 0;  // 1: per-Gauss-point boundary->inclusion rows on planar elements too
 static long g_bj_nopiv = 1;        // block-Jacobi boundary block: 1 no interchanges, 0 pivoting, 2 fallback
+static long g_bj_overlap = 1;      // 1: its triangular solves beside the FP16 product
 int set_tuning(int key, long value) {
   if (key == kTuneBigMinTiles && value >= 1) {
     g_big_min_tiles = value;
@@ -913,6 +914,10 @@ int set_tuning(int key, long value) {
     g_bj_nopiv = value;
     return 0;
   }
+  if (key == kTuneBjOverlap && (value == 0 || value == 1)) {
+    g_bj_overlap = value;
+    return 0;
+  }
   return -1;
 }
 long get_tuning(int key) {
@@ -923,6 +928,7 @@ long get_tuning(int key) {
     case kTuneInnerTolExp: return g_inner_tol_exp;
     case kTuneRowsPointwise: return g_rows_pointwise;
     case kTuneBjNopiv: return g_bj_nopiv;
+    case kTuneBjOverlap: return g_bj_overlap;
     default: return -1;
   }
 }
@@ -2905,22 +2911,23 @@ void m16_build(const double* A, long n, long lda, unsigned short* A16, long ld16
                      st, A, n, n, lda, (_Float16*)A16, ld16, scale);
 }
 
-// part[chunk * m + r] = sum over the chunk's columns of A16[r, j] 2^e x_j: a thread holds 8 consecutive
-// rows (one 16-byte nontemporal load per column), FP32 sums within a 64-column tile, FP64 across tiles;
-// the chunk partial sums are added in a fixed order by k_gemv_done_n. (Measured at n = 160,374: 16
-// chunks with nontemporal loads 6.8 TB/s; 32 chunks with cached loads 6.1; 16 rows per thread 4.2.)
+// part[(chunk0 + c) * m + r] = sum over chunk c's columns of A16[r, j] 2^e x_j, the column tiles
+// [t0, t1) split into gridDim.y chunks: a thread holds 8 consecutive rows (one 16-byte nontemporal
+// load per column), FP32 sums within a 32-column tile, FP64 across tiles; the partial sums of all
+// kM16Chunks chunks are added in a fixed order by k_gemv_done_n. (Measured at n = 160,374: 16 chunks
+// with nontemporal loads 6.8 TB/s; 32 chunks with cached loads 6.1; 16 rows per thread 4.2.)
 constexpr int kM16Chunks = 16;
 __global__ __launch_bounds__(256) void k_gemv16_part(long m, long n, const _Float16* __restrict__ A16, long ld16,
                                                      const float* __restrict__ scale, long row_tiles,
-                                                     const double* __restrict__ x, double* __restrict__ part) {
+                                                     const double* __restrict__ x, double* __restrict__ part,
+                                                     long t0, long t1, int chunk0) {
   const long r0 = ((long)blockIdx.x * 256 + threadIdx.x) * 8;
   if (r0 >= m) return;
-  const long tiles = (n + kM16C - 1) / kM16C;
-  const long tpc = (tiles + kM16Chunks - 1) / kM16Chunks;  // column tiles per chunk
-  const long t0 = (long)blockIdx.y * tpc, t1 = min(tiles, t0 + tpc);
+  const long tpc = (t1 - t0 + gridDim.y - 1) / gridDim.y;  // column tiles per chunk
+  const long ta = t0 + (long)blockIdx.y * tpc, tz = min(t1, ta + tpc);
   const long rt = r0 / kM16R;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (long ct = t0; ct < t1; ++ct) {
+  for (long ct = ta; ct < tz; ++ct) {
     const long j0 = ct * kM16C, j1 = min(n, j0 + kM16C);
     float at[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     long j = j0;
@@ -2947,16 +2954,27 @@ __global__ __launch_bounds__(256) void k_gemv16_part(long m, long n, const _Floa
   }
 #pragma unroll
   for (int k = 0; k < 8; ++k)
-    if (r0 + k < m) part[(long)blockIdx.y * m + r0 + k] = acc[k];
+    if (r0 + k < m) part[(long)(chunk0 + blockIdx.y) * m + r0 + k] = acc[k];
+}
+
+long m16_tiles(long n) { return (n + kM16C - 1) / kM16C; }
+
+void gemv_m16_part(long n, const unsigned short* A16, long ld16, const float* scale, const double* x, double* ws,
+                   long t0, long t1, int chunk0, int chunks, hipStream_t st) {
+  static_assert(kM16Chunks <= kGemvChunks, "the product's partial sums share the FP64 product's workspace");
+  hipLaunchKernelGGL(k_gemv16_part, dim3((unsigned)((n + 2047) / 2048), (unsigned)chunks), dim3(256), 0, st, n, n,
+                     (const _Float16*)A16, ld16, scale, ld16 / kM16R, x, ws, t0, t1, chunk0);
+}
+
+void gemv_m16_finish(long n, double alpha, const double* ws, double beta, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_gemv_done_n, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, kM16Chunks, alpha, ws,
+                     beta, y);
 }
 
 void gemv_m16(long n, double alpha, const unsigned short* A16, long ld16, const float* scale, const double* x,
               double beta, double* y, double* ws, hipStream_t st) {
-  static_assert(kM16Chunks <= kGemvChunks, "the product's partial sums share the FP64 product's workspace");
-  hipLaunchKernelGGL(k_gemv16_part, dim3((unsigned)((n + 2047) / 2048), kM16Chunks), dim3(256), 0, st, n, n,
-                     (const _Float16*)A16, ld16, scale, ld16 / kM16R, x, ws);
-  hipLaunchKernelGGL(k_gemv_done_n, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, kM16Chunks, alpha, ws,
-                     beta, y);
+  gemv_m16_part(n, A16, ld16, scale, x, ws, 0, m16_tiles(n), 0, kM16Chunks, st);
+  gemv_m16_finish(n, alpha, ws, beta, y, st);
 }
 
 }  // namespace ibem

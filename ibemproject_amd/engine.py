@@ -36,6 +36,7 @@ TUNE_INNER_ITERS = 3
 TUNE_INNER_TOL_EXP = 4
 TUNE_ROWS_POINTWISE = 5
 TUNE_BJ_NOPIV = 6
+TUNE_BJ_OVERLAP = 7
 
 
 def set_tuning(key: int, value: int) -> None:

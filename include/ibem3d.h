@@ -126,12 +126,16 @@ enum {
                                         Kelvin table (the reference's order of sums) even where every
                                         element is planar; 0 (default): there the tables are integrated
                                         over the element first (4 contractions per element and row) */
-  IBEM3D_TUNE_BJ_NOPIV = 6           /* the block-Jacobi boundary block: 1 (default) factored without
+  IBEM3D_TUNE_BJ_NOPIV = 6,          /* the block-Jacobi boundary block: 1 (default) factored without
                                         row interchanges in its natural column order (each boundary
                                         dof's own unknown on the diagonal) when the column map gives
                                         one, falling back to partial pivoting on a pivot below 1e-6;
                                         0 always partial pivoting; 2 a threshold no pivot meets, so
                                         the fallback runs every time */
+  IBEM3D_TUNE_BJ_OVERLAP = 7         /* IBEM3D_SOLVE_BJ_GMRES16: 1 (default) the boundary block's
+                                        triangular solves of each inner iteration run on a second
+                                        stream while the FP16 product covers the columns past the
+                                        boundary block; 0: before the product */
 };
 int ibem3d_set_tuning(int key, long value);
 long ibem3d_get_tuning(int key);  /* -1 for an unknown key */
