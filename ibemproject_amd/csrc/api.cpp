@@ -115,6 +115,11 @@ struct ibem3d_ctx {
   long timing_errors = 0;
   // stress inputs and output (kept between calls)
   double *d_stU = nullptr, *d_stT = nullptr, *d_sts = nullptr;
+  // stresses computed on the second stream beside a solve's final residual check (ev_st_out marks
+  // their end): stress_pending while the main stream has not waited for them, stress_spec while they
+  // belong to the current solution
+  bool stress_pending = false, stress_spec = false;
+  hipEvent_t ev_st_in = nullptr, ev_st_out = nullptr;
   // host copies for the volume averages
   std::vector<double> h_qp, h_rad;
   std::vector<int> h_conn;
@@ -445,7 +450,8 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
   for (void* q : ptrs)
     if (q) hipFree(q);
   if (ctx->st_aux) hipStreamDestroy(ctx->st_aux);
-  for (hipEvent_t e : {ctx->ev_aux_in, ctx->ev_aux_in2, ctx->ev_aux_out, ctx->ev_ov_in, ctx->ev_ov_out})
+  for (hipEvent_t e : {ctx->ev_aux_in, ctx->ev_aux_in2, ctx->ev_aux_out, ctx->ev_ov_in, ctx->ev_ov_out,
+                       ctx->ev_st_in, ctx->ev_st_out})
     if (e) hipEventDestroy(e);
   if (ctx->W.st2) hipStreamDestroy(ctx->W.st2);
   if (ctx->Wb.st2) hipStreamDestroy(ctx->Wb.st2);
@@ -564,7 +570,8 @@ static int join_aux(ibem3d_ctx* ctx);
 static int ensure_aux(ibem3d_ctx* ctx) {
   if (ctx->st_aux) return 0;
   HIPC(hipStreamCreateWithFlags(&ctx->st_aux, hipStreamNonBlocking));
-  for (hipEvent_t* e : {&ctx->ev_aux_in, &ctx->ev_aux_in2, &ctx->ev_aux_out, &ctx->ev_ov_in, &ctx->ev_ov_out})
+  for (hipEvent_t* e : {&ctx->ev_aux_in, &ctx->ev_aux_in2, &ctx->ev_aux_out, &ctx->ev_ov_in, &ctx->ev_ov_out,
+                        &ctx->ev_st_in, &ctx->ev_st_out})
     HIPC(hipEventCreateWithFlags(e, hipEventDisableTiming));
   return 0;
 }
@@ -615,8 +622,20 @@ int ibem3d_assemble(ibem3d_ctx* ctx) {
   return ibem3d_apply_bc(ctx);
 }
 
+// the main stream waits for stresses still being computed on the second stream (they read the
+// solution and the known values); with invalidate, they no longer count as the current solution's
+static int join_stress(ibem3d_ctx* ctx, bool invalidate) {
+  if (ctx->stress_pending) {
+    HIPC(hipStreamWaitEvent(ctx->st, ctx->ev_st_out, 0));
+    ctx->stress_pending = false;
+  }
+  if (invalidate) ctx->stress_spec = false;
+  return 0;
+}
+
 int ibem3d_reset(ibem3d_ctx* ctx) {
   CTX_CHECK();
+  if (int rc = join_stress(ctx, true)) return rc;
   if (int rc = join_aux(ctx)) return rc;  // the second stream may still read A's boundary block
   ctx->bj_ready = false;
   const long n = ctx->n;
@@ -1090,6 +1109,26 @@ static int residual(ibem3d_ctx* ctx, double* rel) {
   return 0;
 }
 
+// The stresses of the x just formed, on the second stream while the main stream computes its true
+// residual (the product is HBM-bound, the stress kernel FP64-bound): kept for ibem3d_stress when that
+// residual ends the solve, dropped when another cycle changes x (IBEM3D_TUNE_STRESS_BESIDE).
+static int stress_alloc(ibem3d_ctx* ctx);
+static void enqueue_stress(ibem3d_ctx* ctx, hipStream_t st);
+static int stress_beside(ibem3d_ctx* ctx) {
+  if (ctx->nump == 0 || !get_tuning(kTuneStressBeside)) return 0;
+  if (int rc = ensure_aux(ctx)) return rc;
+  if (int rc = stress_alloc(ctx)) return rc;
+  HIPC(hipEventRecord(ctx->ev_st_in, ctx->st));
+  HIPC(hipStreamWaitEvent(ctx->st_aux, ctx->ev_st_in, 0));
+  {
+    Scope sc(ctx, "post.stress", ctx->st_aux);
+    enqueue_stress(ctx, ctx->st_aux);
+  }
+  HIPC(hipEventRecord(ctx->ev_st_out, ctx->st_aux));
+  ctx->stress_pending = ctx->stress_spec = true;
+  return after_launch(ctx, "k_stress beside the residual");
+}
+
 // GMRES-IR (Carson and Higham): FP64 GMRES on A d = r, right-preconditioned by the FP32 factors of
 // the equilibrated system, M^-1 v = C LU32^-1 (R v); restarted every kGmres vectors, each cycle ending
 // with the true FP64 residual b - A x. Arnoldi by classical Gram-Schmidt applied twice; the small
@@ -1191,7 +1230,9 @@ static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0, int max
       y[i] = d != 0 ? acc / d : 0.0;
     }
     HIPC(hipMemcpyAsync(ctx->d_gh, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, ctx->st));
+    if (int rc = join_stress(ctx, true)) return rc;  // the previous cycle's stresses read x
     hipLaunchKernelGGL(k_combo, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, Z, n, k, ctx->d_gh, 1.0, n);
+    if (int rc = stress_beside(ctx)) return rc;
     if (int rc = residual(ctx, &rel)) return rc;  // r = b - A x in d_res
     ctx->rel_res = rel;
     if (!(rel == rel)) return fail(ctx, IBEM3D_ESOLVE, "refinement diverged (NaN)");
@@ -1473,6 +1514,7 @@ static int solve_bj(ibem3d_ctx* ctx, double tol, bool reuse, int mode) {
   if (rc == IBEM3D_ESOLVE) {
     // the fallback: the mixed LU (it reports its own refinement steps)
     ctx->bj_fallback = true;
+    if (int rc2 = join_stress(ctx, true)) return rc2;
     const int bj_iters = ctx->iters;
     ctx->iters = 0;
     ctx->solved = false;
@@ -1487,6 +1529,7 @@ static int solve_bj(ibem3d_ctx* ctx, double tol, bool reuse, int mode) {
 
 int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
   CTX_CHECK();
+  if (int rc = join_stress(ctx, true)) return rc;
   const bool gmres_first = (mode & IBEM3D_SOLVE_GMRES) != 0;
   const bool reuse = (mode & IBEM3D_SOLVE_REUSE) != 0;
   bool compact = (mode & IBEM3D_SOLVE_COMPACT) != 0;
@@ -1528,6 +1571,7 @@ int ibem3d_solve(ibem3d_ctx* ctx, int mode, double tol) {
 // Jacobi GMRES solves; NULL clears it.
 int ibem3d_set_initial_guess(ibem3d_ctx* ctx, const double* x_eig) {
   CTX_CHECK();
+  if (int rc = join_stress(ctx, true)) return rc;
   if (!x_eig) {
     ctx->has_x0 = false;
     return 0;
@@ -1612,6 +1656,7 @@ static int upload_known_values(ibem3d_ctx* ctx, const double* bc_values, const d
 
 int ibem3d_set_known_values(ibem3d_ctx* ctx, const double* bc_values, const double* tbc_values) {
   CTX_CHECK();
+  if (int rc = join_stress(ctx, true)) return rc;
   if (int rc = upload_known_values(ctx, bc_values, tbc_values)) return rc;
   return ibem3d_reset(ctx);
 }
@@ -1621,6 +1666,7 @@ int ibem3d_set_known_values(ibem3d_ctx* ctx, const double* bc_values, const doub
 // that occupies its memory, is not touched.
 int ibem3d_update_known_values(ibem3d_ctx* ctx, const double* bc_values, const double* tbc_values) {
   CTX_CHECK();
+  if (int rc = join_stress(ctx, true)) return rc;
   if (ctx->phases != 0xFFu)
     return fail(ctx, IBEM3D_ESTATE, "update_known_values: the system must have been assembled");
   if (int rc = upload_known_values(ctx, bc_values, tbc_values)) return rc;
@@ -1803,13 +1849,16 @@ int ibem3d_stress(ibem3d_ctx* ctx, double* s) {
   CTX_CHECK();
   if (!ctx->solved) return fail(ctx, IBEM3D_ESTATE, "stress: no solution");
   if (ctx->nump == 0) return 0;
-  // U and T from the device solution (no host round trip), then the kernel
-  if (int rc = stress_alloc(ctx)) return rc;
-  {
-    Scope sc(ctx, "post.stress");
-    enqueue_stress(ctx, ctx->st);
+  if (int rc = join_stress(ctx, false)) return rc;
+  if (!ctx->stress_spec) {
+    // U and T from the device solution (no host round trip), then the kernel
+    if (int rc = stress_alloc(ctx)) return rc;
+    {
+      Scope sc(ctx, "post.stress");
+      enqueue_stress(ctx, ctx->st);
+    }
+    if (int rc = after_launch(ctx, "k_stress")) return rc;
   }
-  if (int rc = after_launch(ctx, "k_stress")) return rc;
   HIPC(hipMemcpyAsync(s, ctx->d_sts, sizeof(double) * 6 * ctx->nump, hipMemcpyDeviceToHost, ctx->st));
   HIPC(hipStreamSynchronize(ctx->st));
   return 0;

@@ -189,8 +189,10 @@ void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, co
 // pivot threshold no pivot meets (the partial-pivoting fallback runs every time).
 // kTuneBjOverlap: 1 (default) runs the boundary block's triangular solves of each inner iteration on
 // the second stream beside the FP16 product's sphere columns; 0 runs them before the product.
+// kTuneStressBeside: 1 (default) computes the stresses of each GMRES cycle's new x on the second
+// stream beside its true residual, kept when that residual ends the solve; 0 only on request.
 enum { kTuneBigMinTiles = 1, kTunePanelCuPercent = 2, kTuneInnerIters = 3, kTuneInnerTolExp = 4,
-       kTuneRowsPointwise = 5, kTuneBjNopiv = 6, kTuneBjOverlap = 7 };
+       kTuneRowsPointwise = 5, kTuneBjNopiv = 6, kTuneBjOverlap = 7, kTuneStressBeside = 8 };
 constexpr int kInnerItersMax = 16;
 int set_tuning(int key, long value);
 long get_tuning(int key);

@@ -889,6 +889,7 @@ static long g_inner_tol_exp = 3;   // inner GMRES stops at relative residual 10^
 static long g_rows_pointwise = 0;  // 1: per-Gauss-point boundary->inclusion rows on planar elements too
 static long g_bj_nopiv = 1;        // block-Jacobi boundary block: 1 no interchanges, 0 pivoting, 2 fallback
 static long g_bj_overlap = 1;      // 1: its triangular solves beside the FP16 product
+static long g_stress_beside = 1;   // 1: stresses beside each GMRES cycle's true residual
 int set_tuning(int key, long value) {
   if (key == kTuneBigMinTiles && value >= 1) {
     g_big_min_tiles = value;
@@ -918,6 +919,10 @@ int set_tuning(int key, long value) {
     g_bj_overlap = value;
     return 0;
   }
+  if (key == kTuneStressBeside && (value == 0 || value == 1)) {
+    g_stress_beside = value;
+    return 0;
+  }
   return -1;
 }
 long get_tuning(int key) {
@@ -929,6 +934,7 @@ long get_tuning(int key) {
     case kTuneRowsPointwise: return g_rows_pointwise;
     case kTuneBjNopiv: return g_bj_nopiv;
     case kTuneBjOverlap: return g_bj_overlap;
+    case kTuneStressBeside: return g_stress_beside;
     default: return -1;
   }
 }

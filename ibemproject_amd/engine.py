@@ -37,6 +37,7 @@ TUNE_INNER_TOL_EXP = 4
 TUNE_ROWS_POINTWISE = 5
 TUNE_BJ_NOPIV = 6
 TUNE_BJ_OVERLAP = 7
+TUNE_STRESS_BESIDE = 8
 
 
 def set_tuning(key: int, value: int) -> None:

@@ -132,10 +132,15 @@ enum {
                                         one, falling back to partial pivoting on a pivot below 1e-6;
                                         0 always partial pivoting; 2 a threshold no pivot meets, so
                                         the fallback runs every time */
-  IBEM3D_TUNE_BJ_OVERLAP = 7         /* IBEM3D_SOLVE_BJ_GMRES16: 1 (default) the boundary block's
+  IBEM3D_TUNE_BJ_OVERLAP = 7,        /* IBEM3D_SOLVE_BJ_GMRES16: 1 (default) the boundary block's
                                         triangular solves of each inner iteration run on a second
                                         stream while the FP16 product covers the columns past the
                                         boundary block; 0: before the product */
+  IBEM3D_TUNE_STRESS_BESIDE = 8      /* 1 (default): each GMRES cycle computes the stresses of its new
+                                        solution on a second stream beside its true FP64 residual;
+                                        ibem3d_stress returns them when that residual ended the solve
+                                        (bit-identical to computing them afterwards); 0: ibem3d_stress
+                                        computes them */
 };
 int ibem3d_set_tuning(int key, long value);
 long ibem3d_get_tuning(int key);  /* -1 for an unknown key */

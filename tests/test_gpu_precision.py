@@ -278,3 +278,37 @@ def test_bj_boundary_without_interchanges(mid):
         for a, b in zip(out[cfg], out[(0, 1)]):
             assert abs(a[0] - b[0]) <= 1 and abs(a[1] - b[1]) <= 2
             assert np.linalg.norm(a[2] - b[2]) / np.linalg.norm(x0) < 1e-8
+
+
+def test_stress_beside_final_residual(mid):
+    """Stresses computed on the second stream beside a solve's final true residual (the default)
+    equal the ones computed after the solve bit for bit, for the 16-bit-inner block-Jacobi mode and
+    a GMRES-IR LU mode, and after the known values change the solve's new stresses are returned,
+    not the previous ones."""
+    p = mid["prob"]
+    bc = np.asarray(p.bc).reshape(-1, 2)
+    tbc = np.asarray(p.tbc).reshape(-1, 24)
+    case(f"stresses beside the final residual, n60={p.n60}, points={len(np.asarray(p.points).reshape(-1, 3))}")
+    out = {}
+    try:
+        for flag in (1, 0):
+            E.set_tuning(E.TUNE_STRESS_BESIDE, flag)
+            eng = E.Engine(p)
+            res = []
+            for load, mode in ((1.0, E.SOLVE_BJ_GMRES16), (2.0, E.SOLVE_BJ_GMRES16),
+                               (1.0, E.SOLVE_MIXED_FP16_1 | E.SOLVE_GMRES)):
+                eng.set_known_values(load * bc[:, 1], load * tbc[:, 12:])
+                eng.assemble()
+                rel, _ = eng.solve(mode, 1e-10)
+                assert rel <= 1e-10
+                res.append(np.array(eng.getFlux(), copy=True))
+                res.append(np.array(eng.getFlux(), copy=True))  # a second request: the same values
+            out[flag] = res
+            eng.close()
+    finally:
+        E.set_tuning(E.TUNE_STRESS_BESIDE, 1)
+    for a, b in zip(out[1], out[0]):
+        assert np.array_equal(a, b)
+    s1, s2 = out[1][0], out[1][2]
+    assert not np.array_equal(s1, s2)
+    assert np.linalg.norm(s2 - 2.0 * s1) <= 1e-8 * np.linalg.norm(s2)
