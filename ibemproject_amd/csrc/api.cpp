@@ -1000,12 +1000,10 @@ static int bj_apply_m16(ibem3d_ctx* ctx, const double* v, double* z, double* w) 
     return 0;
   }
   if (int rc = ensure_aux(ctx)) return rc;
+  // the main stream's launches first: the host then spends the boundary solve's ~90 launches while
+  // the product runs, not before it
   hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, ctx->d_tb, v, ctx->d_rb, nb);
   HIPC(hipEventRecord(ctx->ev_ov_in, ctx->st));
-  HIPC(hipStreamWaitEvent(ctx->st_aux, ctx->ev_ov_in, 0));
-  bj_boundary_solve(ctx, ctx->st_aux);
-  bj_unscale(ctx, z, ctx->d_tb, ctx->st_aux);
-  HIPC(hipEventRecord(ctx->ev_ov_out, ctx->st_aux));
   if (ctx->num > 0)
     hipLaunchKernelGGL(k_block_mv, dim3(ctx->num), dim3(64), 0, ctx->st, ctx->d_Binv, v, z, nb);
   constexpr int kFirst = 4;  // chunks of the boundary tiles
@@ -1013,6 +1011,10 @@ static int bj_apply_m16(ibem3d_ctx* ctx, const double* v, double* z, double* w) 
     Scope s(ctx, "solve.gemv_m16");
     gemv_m16_part(n, ctx->d_A16, ctx->ld16, ctx->d_A16s, z, ctx->d_gemvws, tb, tiles, kFirst, 16 - kFirst, ctx->st);
   }
+  HIPC(hipStreamWaitEvent(ctx->st_aux, ctx->ev_ov_in, 0));
+  bj_boundary_solve(ctx, ctx->st_aux);
+  bj_unscale(ctx, z, ctx->d_tb, ctx->st_aux);
+  HIPC(hipEventRecord(ctx->ev_ov_out, ctx->st_aux));
   HIPC(hipStreamWaitEvent(ctx->st, ctx->ev_ov_out, 0));
   {
     Scope s(ctx, "solve.gemv_m16");
