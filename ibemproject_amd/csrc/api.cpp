@@ -987,9 +987,8 @@ static void precond_solve(ibem3d_ctx* ctx, float* v) {
 // inner solve buys the outer iteration nothing)
 // z = M^-1 v, then w = A16 z. With IBEM3D_TUNE_BJ_OVERLAP the boundary block's triangular solves
 // (a latency-bound chain of small launches) run on the second stream while the product covers the
-// column tiles past the boundary block (chunks 4..15 of its partial sums, from the sphere blocks'
-// part of z); the tiles holding boundary columns (chunks 0..3) follow once that part of z is solved.
-// The chunk sums are added in the same fixed order either way.
+// column tiles past the boundary block (from the sphere blocks' part of z); the tiles holding
+// boundary columns follow once that part of z is solved. The chunk sums are added in a fixed order.
 static int bj_apply_m16(ibem3d_ctx* ctx, const double* v, double* z, double* w) {
   const long n = ctx->n, nb = 3L * ctx->NN;
   const long tiles = m16_tiles(n), tb = (nb + 31) / 32;  // tiles [0, tb) hold the boundary columns
@@ -1006,10 +1005,14 @@ static int bj_apply_m16(ibem3d_ctx* ctx, const double* v, double* z, double* w) 
   HIPC(hipEventRecord(ctx->ev_ov_in, ctx->st));
   if (ctx->num > 0)
     hipLaunchKernelGGL(k_block_mv, dim3(ctx->num), dim3(64), 0, ctx->st, ctx->d_Binv, v, z, nb);
-  constexpr int kFirst = 4;  // chunks of the boundary tiles
+  // chunks of the boundary tiles and of the rest: 6 + 3 instead of the plain product's 16, so that
+  // the boundary solve's 1024-thread workgroups find room beside the product's (measured on the
+  // 2,500-sphere RVE: 12 + 4 0.716 s per step, 6 + 3 0.701, 5 + 3 0.697 with a slower product, 8 + 2
+  // 0.724, 7 + 3 0.728)
+  constexpr int kFirst = 3, kRest = 6;
   {
     Scope s(ctx, "solve.gemv_m16");
-    gemv_m16_part(n, ctx->d_A16, ctx->ld16, ctx->d_A16s, z, ctx->d_gemvws, tb, tiles, kFirst, 16 - kFirst, ctx->st);
+    gemv_m16_part(n, ctx->d_A16, ctx->ld16, ctx->d_A16s, z, ctx->d_gemvws, tb, tiles, kFirst, kRest, ctx->st);
   }
   HIPC(hipStreamWaitEvent(ctx->st_aux, ctx->ev_ov_in, 0));
   bj_boundary_solve(ctx, ctx->st_aux);
@@ -1019,7 +1022,7 @@ static int bj_apply_m16(ibem3d_ctx* ctx, const double* v, double* z, double* w) 
   {
     Scope s(ctx, "solve.gemv_m16");
     gemv_m16_part(n, ctx->d_A16, ctx->ld16, ctx->d_A16s, z, ctx->d_gemvws, 0, tb, 0, kFirst, ctx->st);
-    gemv_m16_finish(n, 1.0, ctx->d_gemvws, 0.0, w, ctx->st);
+    gemv_m16_finish(n, 1.0, ctx->d_gemvws, 0.0, w, ctx->st, kFirst + kRest);
   }
   return 0;
 }

@@ -146,12 +146,12 @@ void m16_build(const double* A, long n, long lda, unsigned short* A16, long ld16
 void gemv_m16(long n, double alpha, const unsigned short* A16, long ld16, const float* scale, const double* x,
               double beta, double* y, double* ws, hipStream_t st);
 // The same product in parts: the partial sums of column tiles [t0, t1) (32 columns each, m16_tiles(n)
-// in all) into chunks [chunk0, chunk0 + chunks) of ws; once all 16 chunks hold partial sums,
-// gemv_m16_finish adds them in a fixed order, y = alpha sum + beta y.
+// in all) into chunks [chunk0, chunk0 + chunks) of ws (at most 16); once chunks [0, total) hold
+// partial sums, gemv_m16_finish adds them in a fixed order, y = alpha sum + beta y.
 long m16_tiles(long n);
 void gemv_m16_part(long n, const unsigned short* A16, long ld16, const float* scale, const double* x, double* ws,
                    long t0, long t1, int chunk0, int chunks, hipStream_t st);
-void gemv_m16_finish(long n, double alpha, const double* ws, double beta, double* y, hipStream_t st);
+void gemv_m16_finish(long n, double alpha, const double* ws, double beta, double* y, hipStream_t st, int total);
 // out = R A C with the same scaling, A left unchanged (FP64 or FP32 output); ws holds
 // equilibrate_workspace_doubles(n) doubles. With W, the FP32 copy writes its first LU panel's columns
 // on st and the rest on W's second stream, recording W.ev_cols: the next lu_factor_f32 on W must

@@ -2972,15 +2972,15 @@ void gemv_m16_part(long n, const unsigned short* A16, long ld16, const float* sc
                      (const _Float16*)A16, ld16, scale, ld16 / kM16R, x, ws, t0, t1, chunk0);
 }
 
-void gemv_m16_finish(long n, double alpha, const double* ws, double beta, double* y, hipStream_t st) {
-  hipLaunchKernelGGL(k_gemv_done_n, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, kM16Chunks, alpha, ws,
+void gemv_m16_finish(long n, double alpha, const double* ws, double beta, double* y, hipStream_t st, int chunks) {
+  hipLaunchKernelGGL(k_gemv_done_n, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, chunks, alpha, ws,
                      beta, y);
 }
 
 void gemv_m16(long n, double alpha, const unsigned short* A16, long ld16, const float* scale, const double* x,
               double beta, double* y, double* ws, hipStream_t st) {
   gemv_m16_part(n, A16, ld16, scale, x, ws, 0, m16_tiles(n), 0, kM16Chunks, st);
-  gemv_m16_finish(n, alpha, ws, beta, y, st);
+  gemv_m16_finish(n, alpha, ws, beta, y, st, kM16Chunks);
 }
 
 }  // namespace ibem
