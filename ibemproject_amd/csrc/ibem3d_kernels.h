@@ -133,10 +133,11 @@ void lu_solve_f32(const float* LU, long n, long lda, const LuWork& W, float* x, 
 
 // row / column equilibration: r_i = 1 / max_j |A_ij|, c_j = 1 / max_i |r_i A_ij|; A <- R A C
 void equilibrate_f64(double* A, long n, long lda, double* r, double* c, hipStream_t st);
-// y = alpha A x + beta y (FP64, column-major); ws holds gemv_workspace_doubles(m) doubles
+// y = alpha A x + beta y (FP64, column-major); ws holds gemv_workspace_doubles(m) doubles; the
+// columns are split into chunks (at most 64) whose partial sums are added in a fixed order
 size_t gemv_workspace_doubles(long m);
 void gemv_f64(long m, long n, double alpha, const double* A, long lda, const double* x, double beta,
-              double* y, double* ws, hipStream_t st);
+              double* y, double* ws, hipStream_t st, int chunks = 64);
 // 16-bit copy of the n x n FP64 A (per 256 x 32 tile power-of-two scaled FP16, leading dimension
 // m16_ld(n), m16_scale_count(n) float scales) and y = alpha A16 x + beta y with it (FP32 sums within a
 // tile, FP64 across; ws as for gemv_f64)

@@ -2741,7 +2741,7 @@ typedef double dvec2 __attribute__((ext_vector_type(2)));
 __global__ __launch_bounds__(256) void k_gemv_part(long m, long n, const double* __restrict__ A,
                                                    long lda, const double* __restrict__ x,
                                                    double* __restrict__ part) {
-  const long cw = (n + kGemvChunks - 1) / kGemvChunks;
+  const long cw = (n + gridDim.y - 1) / gridDim.y;  // columns per chunk
   const long j0 = (long)blockIdx.y * cw, j1 = min(n, j0 + cw);
   const bool pairs = (lda & 1) == 0 && (((size_t)A) & 15) == 0;
   if (pairs) {
@@ -2785,12 +2785,13 @@ __global__ __launch_bounds__(256) void k_gemv_part(long m, long n, const double*
   part[(long)blockIdx.y * m + r] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
-__global__ __launch_bounds__(256) void k_gemv_done(long m, double alpha, const double* __restrict__ part,
-                                                   double beta, double* __restrict__ y) {
+__global__ __launch_bounds__(256) void k_gemv_done(long m, int chunks, double alpha,
+                                                   const double* __restrict__ part, double beta,
+                                                   double* __restrict__ y) {
   const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= m) return;
   double v = 0.0;
-  for (int c = 0; c < kGemvChunks; ++c) v += part[(long)c * m + r];
+  for (int c = 0; c < chunks; ++c) v += part[(long)c * m + r];
   v *= alpha;
   y[r] = beta == 0.0 ? v : v + beta * y[r];
 }
@@ -2807,12 +2808,13 @@ __global__ __launch_bounds__(256) void k_gemv_done_n(long m, int chunks, double 
 }
 
 void gemv_f64(long m, long n, double alpha, const double* A, long lda, const double* x, double beta,
-              double* y, double* ws, hipStream_t st) {
+              double* y, double* ws, hipStream_t st, int chunks) {
   const bool pairs = (lda & 1) == 0 && (((size_t)A) & 15) == 0;
   const long rows_per_wg = pairs ? 512 : 256;
-  hipLaunchKernelGGL(k_gemv_part, dim3((unsigned)((m + rows_per_wg - 1) / rows_per_wg), kGemvChunks), dim3(256),
-                     0, st, m, n, A, lda, x, ws);
-  hipLaunchKernelGGL(k_gemv_done, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, m, alpha, ws,
+  chunks = std::max(1, std::min(chunks, kGemvChunks));
+  hipLaunchKernelGGL(k_gemv_part, dim3((unsigned)((m + rows_per_wg - 1) / rows_per_wg), (unsigned)chunks),
+                     dim3(256), 0, st, m, n, A, lda, x, ws);
+  hipLaunchKernelGGL(k_gemv_done, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, m, chunks, alpha, ws,
                      beta, y);
 }
 
