@@ -186,6 +186,7 @@ __global__ __launch_bounds__(192) void k_bem_quad(const double* __restrict__ qp,
   const int ic = active ? i : 0;
   const double xi0 = nodes[3 * ic], xi1 = nodes[3 * ic + 1], xi2 = nodes[3 * ic + 2];
   const long row = 3L * i + ii;
+  const KelvinC K = kelvin_consts(E0, nu0);
   const int el0 = blockIdx.y * kBemEPB, el1 = min(el0 + kBemEPB, ne);
   for (int el = el0; el < el1; ++el) {
     const int e = e0 + el;
@@ -207,10 +208,10 @@ __global__ __launch_bounds__(192) void k_bem_quad(const double* __restrict__ qp,
       for (int g = 0; g < 64; ++g) {
         const double* Q = sq + g * kQpStride;
         const double d0 = Q[0] - xi0, d1 = Q[1] - xi1, d2 = Q[2] - xi2;
-        const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-        const double dxr[3] = {d0 / r, d1 / r, d2 / r};
+        const double ir = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        const double dxr[3] = {d0 * ir, d1 * ir, d2 * ir};
         double up[3], tp[3];
-        kelvin_ut_row(ii, dxr, r, E0, nu0, Q + 3, up, tp);
+        kelvin_ut_row(ii, dxr, ir, K, Q + 3, up, tp);
         const double w = Q[10];
         for (int q = 0; q < 4; ++q) {
           const double nw = Q[6 + q] * w;
@@ -232,10 +233,10 @@ __global__ __launch_bounds__(192) void k_bem_quad(const double* __restrict__ qp,
             QPoint Q;
             qpoint(elx, xs, es, Q);
             const double d0 = Q.g[0] - xi0, d1 = Q.g[1] - xi1, d2 = Q.g[2] - xi2;
-            const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-            const double dxr[3] = {d0 / r, d1 / r, d2 / r};
+            const double ir = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            const double dxr[3] = {d0 * ir, d1 * ir, d2 * ir};
             double up[3], tp[3];
-            kelvin_ut_row(ii, dxr, r, E0, nu0, Q.nv, up, tp);
+            kelvin_ut_row(ii, dxr, ir, K, Q.nv, up, tp);
             const double w = gauss8_w(ga) * gauss8_w(gb) * Q.jac * jb;
             for (int q = 0; q < 4; ++q) {
               const double nw = Q.N[q] * w;

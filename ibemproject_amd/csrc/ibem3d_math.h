@@ -923,18 +923,29 @@ IBEM_HD void strain_kernel(const double* tabS, const double* tabT, const Mat& M,
   for (int n = 0; n < 3; ++n) strain_kernel_n(tabS, tabT, M, dlam, dmu, nv, k, l, e, sg, n, su3[n], st3[n]);
 }
 
-// Kelvin kernel row ii of U* and T* at unit direction dxr, distance r (UK/TK, BEMElasticHelper3D.cpp:276-321)
-IBEM_HD void kelvin_ut_row(int ii, const double* dxr, double r, double E, double nu, const double* nv,
-                           double* up, double* tp) {
+// Material constants of the Kelvin kernels: cu = 1 / (16 pi G (1 - nu)), ct = 1 / (8 pi (1 - nu)),
+// c1 = 3 - 4 nu, c3 = 1 - 2 nu
+struct KelvinC {
+  double cu, ct, c1, c3;
+};
+IBEM_HD KelvinC kelvin_consts(double E, double nu) {
   const double G = E / (2 * (1 + nu));
-  const double conr = 1.0 / (16 * kPi * G * (1 - nu)) / r;
-  const double c1 = 3 - 4 * nu, c3 = 1 - 2 * nu;
-  const double Conr = 1.0 / (8 * kPi * (1 - nu)) / (r * r);
+  return KelvinC{1.0 / (16 * kPi * G * (1 - nu)), 1.0 / (8 * kPi * (1 - nu)), 3 - 4 * nu, 1 - 2 * nu};
+}
+
+// Kelvin kernel row ii of U* and T* at unit direction dxr and inverse distance ir (UK/TK,
+// BEMElasticHelper3D.cpp:276-321): U*_ij = cu / r (c1 d_ij + r_i r_j),
+// T*_ij = -ct / r^2 ( (c3 d_ij + 3 r_i r_j) dr/dn - c3 (n_j r_i - n_i r_j) )
+IBEM_HD void kelvin_ut_row(int ii, const double* dxr, double ir, const KelvinC& K, const double* nv,
+                           double* up, double* tp) {
+  const double conr = K.cu * ir, Conr = K.ct * ir * ir;
+  const double di = ii == 0 ? dxr[0] : (ii == 1 ? dxr[1] : dxr[2]);
+  const double ni = ii == 0 ? nv[0] : (ii == 1 ? nv[1] : nv[2]);
   const double cth = nv[0] * dxr[0] + nv[1] * dxr[1] + nv[2] * dxr[2];
   for (int j = 0; j < 3; ++j) {
-    up[j] = conr * (c1 * (ii == j) + dxr[ii] * dxr[j]);
-    tp[j] = (ii == j) ? -Conr * (c3 + 3 * dxr[ii] * dxr[ii]) * cth
-                      : -Conr * (3 * dxr[ii] * dxr[j] * cth - c3 * (nv[j] * dxr[ii] - nv[ii] * dxr[j]));
+    up[j] = conr * (K.c1 * (ii == j) + di * dxr[j]);
+    tp[j] = (ii == j) ? -Conr * (K.c3 + 3 * di * di) * cth
+                      : -Conr * (3 * di * dxr[j] * cth - K.c3 * (nv[j] * di - ni * dxr[j]));
   }
 }
 
