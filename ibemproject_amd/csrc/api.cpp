@@ -55,7 +55,11 @@ struct ibem3d_ctx {
   // GMRES-IR: Krylov basis V (kGmres + 1 columns), preconditioned directions Z (kGmres columns),
   // dot-product partial sums, small coefficient vector
   double *d_gV = nullptr, *d_gZ = nullptr, *d_gpart = nullptr, *d_gh = nullptr;
-  double* d_bemws = nullptr;  // boundary-boundary quadrature values of one round of elements
+  double* d_bemws = nullptr;  // boundary-boundary quadrature values (whole mesh, or one round of elements)
+  // the boundary phase as a gather (launch_bem_gather) when its whole-mesh workspace fits: every
+  // boundary column's contributions in element order (CSR), each column's node dof
+  int *d_bg_ptr = nullptr, *d_bg_ent = nullptr, *d_bg_dof = nullptr;
+  bool bem_gather = false;
   double* d_gemvws = nullptr; // residual GEMV partial sums
   int prepared_mode = -1;      // mode (with IBEM3D_SOLVE_COMPACT) whose factors are in place (-1: none)
   // compact storage: 16-bit copy of the FP32 factor (leading dimension ldf) and its tile scales
@@ -299,6 +303,9 @@ int ibem3d_set_tuning(int key, long value) {
 }
 long ibem3d_get_tuning(int key) { return get_tuning(key); }
 
+static bool fits(size_t bytes);
+static void zero_boundary_columns(ibem3d_ctx* ctx, hipStream_t st);
+
 ibem3d_ctx* ibem3d_create(const ibem3d_problem* p, int device) {
   if (!p || p->NN <= 0 || p->NE <= 0 || p->num < 0 || p->nump < 0 || !p->nodes || !p->conn ||
       !p->bc || !p->tbc || (p->num > 0 && (!p->centres || !p->radii)) ||
@@ -362,10 +369,39 @@ ibem3d_ctx* ibem3d_create(const ibem3d_problem* p, int device) {
       dalloc(ctx, &ctx->d_A, (size_t)ctx->lda * n) || dalloc(ctx, &ctx->d_b, (size_t)n) ||
       dalloc(ctx, &ctx->d_x, (size_t)n))
     return bail("");
-  // the boundary-column block and b are accumulated by the assembly kernels
-  fill_zero(ctx->d_A, sizeof(double) * (size_t)ctx->lda * 3 * p->NN, ctx->st);
-  fill_zero(ctx->d_b, sizeof(double) * n, ctx->st);
+  fill_zero(ctx->d_b, sizeof(double) * n, ctx->st);  // b is accumulated by the assembly kernels
   launch_qpoints(ctx->d_nodes, ctx->d_conn, p->NE, ctx->d_qp, ctx->st);
+  {
+    // boundary columns' contributions in element order: H of displacement-unknown node dofs (added),
+    // G of traction-unknown slots (subtracted), as ApplyBC folds them (BEMBuilderElastic3D.cpp:
+    // 1960-2016); the gather form needs the whole mesh's quadrature values at once
+    const long R = 3L * p->NN;
+    std::vector<int> cnt(R + 1, 0), dof(R, -1);
+    for (int e = 0; e < p->NE; ++e)
+      for (int k = 0; k < 12; ++k) {
+        const int d = 3 * (p->conn[4 * e + k / 3] - 1) + k % 3;
+        if (ctx->node_col[d] >= 0) ++cnt[ctx->node_col[d] + 1];
+        if (ctx->slot_col[12 * e + k] >= 0) ++cnt[ctx->slot_col[12 * e + k] + 1];
+      }
+    for (long j = 0; j < R; ++j) cnt[j + 1] += cnt[j];
+    std::vector<int> ent(cnt[R]), pos(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < p->NE; ++e)
+      for (int k = 0; k < 12; ++k) {
+        const int d = 3 * (p->conn[4 * e + k / 3] - 1) + k % 3;
+        if (ctx->node_col[d] >= 0) ent[pos[ctx->node_col[d]]++] = 24 * e + 12 + k;
+        if (ctx->slot_col[12 * e + k] >= 0)
+          ent[pos[ctx->slot_col[12 * e + k]]++] = (int)((24u * e + k) | 0x80000000u);
+      }
+    for (long d = 0; d < R; ++d)
+      if (ctx->node_col[d] >= 0) dof[ctx->node_col[d]] = (int)d;
+    ctx->bem_gather = 24L * p->NE < (1L << 31) &&
+                      fits(sizeof(double) * bem_gather_workspace_doubles(p->NN, p->NE));
+    if (ctx->bem_gather &&
+        (upload(ctx, &ctx->d_bg_ptr, cnt.data(), cnt.size()) || upload(ctx, &ctx->d_bg_ent, ent.data(), ent.size()) ||
+         upload(ctx, &ctx->d_bg_dof, dof.data(), dof.size()) ||
+         dalloc(ctx, &ctx->d_bemws, bem_gather_workspace_doubles(p->NN, p->NE))))
+      return bail("");
+  }
   if (dalloc(ctx, &ctx->d_wdesc, kInclW)) return bail("");
   launch_incl_desc(ctx->d_wdesc, ctx->st);
   {
@@ -423,6 +459,7 @@ ibem3d_ctx* ibem3d_create(const ibem3d_problem* p, int device) {
       ctx->rows_mf = RowsMoment{ctx->d_cw, ctx->d_ecol, ctx->d_eknown, ctx->d_sink};
     }
   }
+  zero_boundary_columns(ctx, ctx->st);
   return ctx;
 }
 
@@ -443,7 +480,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->d_Binv,  ctx->d_tb,       ctx->d_x0s,      ctx->Wb.d_ipiv, ctx->Wb.d_ws,
                   ctx->d_A16,   ctx->d_A16s,     ctx->d_iV,       ctx->d_iZ,   ctx->d_iw,
                   ctx->Wb.d_info, ctx->Wb.d_perm, ctx->Wb.d_inv,  ctx->Wb.d_tmp, ctx->Wb.d_trinv,
-                  ctx->Wb.d_trsm,
+                  ctx->Wb.d_trsm, ctx->d_bg_ptr, ctx->d_bg_ent, ctx->d_bg_dof,
                   ctx->d_gV,    ctx->d_gZ,       ctx->d_gpart,    ctx->d_gh,   ctx->d_bemws, ctx->d_gemvws,
                   ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info, ctx->W.d_perm, ctx->W.d_inv, ctx->W.d_tmp,
                   ctx->W.d_trinv, ctx->W.d_trsm};
@@ -477,6 +514,38 @@ static const RowsMoment* rows_moment_form(const ibem3d_ctx* ctx) {
   return ctx->d_cw && get_tuning(kTuneRowsPointwise) == 0 ? &ctx->rows_mf : nullptr;
 }
 
+// A's boundary columns before an assembly. The gather form of the boundary phase writes every entry
+// of the boundary rows and the moment form of the rows phase every entry of the sphere rows (a
+// column's first element writes it), so nothing needs clearing then; the per-point rows form
+// accumulates into its entries, which are then cleared before it runs (k_zero_rows_mask).
+static void zero_boundary_columns(ibem3d_ctx* ctx, hipStream_t st) {
+  if (!ctx->bem_gather) fill_zero(ctx->d_A, sizeof(double) * (size_t)ctx->lda * 3 * ctx->NN, st);
+}
+
+// the boundary phase (A's boundary block and b's known-value terms; b only without write_A)
+static int bem_enqueue(ibem3d_ctx* ctx, hipStream_t st, bool write_A) {
+  if (ctx->bem_gather) {
+    launch_bem_gather(ctx->d_qp, ctx->d_conn, ctx->d_nodes, ctx->NN, ctx->NE, ctx->lda, ctx->E0, ctx->nu0,
+                      ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, ctx->d_bemws,
+                      BemGather{ctx->d_bg_ptr, ctx->d_bg_ent, ctx->d_bg_dof}, st, write_A);
+    return 0;
+  }
+  if (!ctx->d_bemws)
+    if (int rc = dalloc(ctx, &ctx->d_bemws, bem_workspace_doubles(ctx->NN, ctx->NE))) return rc;
+  launch_bem(ctx->d_qp, ctx->d_conn, ctx->d_nodes, ctx->NN, ctx->NE, ctx->lda, ctx->E0, ctx->nu0, ctx->d_node_col,
+             ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, ctx->d_bemws, st, write_A);
+  return 0;
+}
+
+// A[row, j] = 0 for the sphere rows (row0 + 60 h + r) of the orders in rowmask and columns j < ncols
+__global__ __launch_bounds__(256) void k_zero_rows_mask(double* __restrict__ A, long lda, long row0, long nrows,
+                                                        int rowmask) {
+  const long r = (long)blockIdx.x * 256 + threadIdx.x;
+  if (r >= nrows) return;
+  const int loc = (int)(r % 60), order = loc < 6 ? 0 : (loc < 24 ? 1 : 2);
+  if ((rowmask >> order) & 1) A[(long)blockIdx.y * lda + row0 + r] = 0.0;
+}
+
 static int after_launch(ibem3d_ctx* ctx, const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(ctx, IBEM3D_EHIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -496,11 +565,7 @@ int ibem3d_add_bem(ibem3d_ctx* ctx) {
   if (int rc = phase_guard(ctx, 1u, "add_bem")) return rc;
   {
     Scope s(ctx, "assemble.bem");
-    if (!ctx->d_bemws)
-      if (int rc = dalloc(ctx, &ctx->d_bemws, bem_workspace_doubles(ctx->NN, ctx->NE))) return rc;
-    launch_bem(ctx->d_qp, ctx->d_conn, ctx->d_nodes, ctx->NN, ctx->NE, ctx->lda, ctx->E0, ctx->nu0,
-               ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b,
-               ctx->d_bemws, ctx->st);
+    if (int rc = bem_enqueue(ctx, ctx->st, true)) return rc;
   }
   ctx->phases |= 1u;
   ctx->solved = false;
@@ -542,6 +607,11 @@ int ibem3d_add_bem_rows(ibem3d_ctx* ctx, int order) {
   if (int rc = phase_guard(ctx, (unsigned)mask << 5, "add_bem_rows")) return rc;
   if (ctx->num > 0) {
     Scope s(ctx, "assemble.rows");
+    if (!rows_moment_form(ctx) && ctx->bem_gather) {  // the per-point form accumulates: clear first
+      const long nr = 60L * ctx->num;
+      hipLaunchKernelGGL(k_zero_rows_mask, dim3((unsigned)((nr + 255) / 256), (unsigned)(3 * ctx->NN)), dim3(256), 0,
+                         ctx->st, ctx->d_A, ctx->lda, 3L * ctx->NN, nr, mask);
+    }
     launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, ctx->lda, 3L * ctx->NN, ctx->M,
                 mask, ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b,
                 ctx->st, true, rows_moment_form(ctx));
@@ -639,7 +709,7 @@ int ibem3d_reset(ibem3d_ctx* ctx) {
   if (int rc = join_aux(ctx)) return rc;  // the second stream may still read A's boundary block
   ctx->bj_ready = false;
   const long n = ctx->n;
-  fill_zero(ctx->d_A, sizeof(double) * (size_t)ctx->lda * 3 * ctx->NN, ctx->st);
+  zero_boundary_columns(ctx, ctx->st);
   fill_zero(ctx->d_b, sizeof(double) * n, ctx->st);
   if (int rc = after_launch(ctx, "reset")) return rc;
   ctx->phases = 0;
@@ -668,6 +738,9 @@ int ibem3d_get_system(ibem3d_ctx* ctx, double* A, double* b) {
   CTX_CHECK();
   if (ctx->a_factored)
     return fail(ctx, IBEM3D_ESTATE, "get_system: A holds a factor (compact solve); assemble again");
+  if (ctx->phases != 0xFFu)
+    return fail(ctx, IBEM3D_ESTATE, "get_system: every assembly phase must run first (mask " +
+                                        std::to_string(ctx->phases) + ")");
   const long n = ctx->n;
   if (A)
     HIPC(hipMemcpy2DAsync(A, sizeof(double) * n, ctx->d_A, sizeof(double) * ctx->lda, sizeof(double) * n, n,
@@ -1678,11 +1751,7 @@ int ibem3d_update_known_values(ibem3d_ctx* ctx, const double* bc_values, const d
   fill_zero(ctx->d_b, sizeof(double) * ctx->n, ctx->st);
   {
     Scope s(ctx, "assemble.rhs");
-    if (!ctx->d_bemws)
-      if (int rc = dalloc(ctx, &ctx->d_bemws, bem_workspace_doubles(ctx->NN, ctx->NE))) return rc;
-    launch_bem(ctx->d_qp, ctx->d_conn, ctx->d_nodes, ctx->NN, ctx->NE, ctx->lda, ctx->E0, ctx->nu0,
-               ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b,
-               ctx->d_bemws, ctx->st, false);
+    if (int rc = bem_enqueue(ctx, ctx->st, false)) return rc;
     if (ctx->num > 0)
       launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, ctx->lda, 3L * ctx->NN, ctx->M, 7,
                   ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b,
@@ -1961,10 +2030,14 @@ int enqueue(ibem3d_ctx* ctx, int mode, int K, int nloads, const double* d_uvals,
   const bool split = mode != IBEM3D_SOLVE_MIXED_FP32;
   HIPC(hipMemcpyAsync(ctx->d_uval, d_uvals, sizeof(double) * 3 * NN, hipMemcpyDeviceToDevice, st));
   // assembly (every phase, all orders at once)
-  fill_zero(ctx->d_A, sizeof(double) * (size_t)ctx->lda * 3 * NN, st);
+  zero_boundary_columns(ctx, st);
   fill_zero(ctx->d_b, sizeof(double) * n, st);
-  launch_bem(ctx->d_qp, ctx->d_conn, ctx->d_nodes, ctx->NN, ctx->NE, ctx->lda, ctx->E0, ctx->nu0, ctx->d_node_col,
-             ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, ctx->d_bemws, st);
+  if (int rc = bem_enqueue(ctx, st, true)) return rc;
+  if (ctx->num > 0 && !rows_moment_form(ctx) && ctx->bem_gather) {  // the per-point rows accumulate
+    const long nr = 60L * ctx->num;
+    hipLaunchKernelGGL(k_zero_rows_mask, dim3((unsigned)((nr + 255) / 256), (unsigned)(3 * NN)), dim3(256), 0, st,
+                       ctx->d_A, ctx->lda, 3L * NN, nr, 7);
+  }
   if (ctx->num > 0) {
     launch_field(ctx->d_nodes, ctx->NN, ctx->d_cen, ctx->d_rad, ctx->num, ctx->lda, 3 * NN, ctx->M, ctx->d_A, st);
     launch_incl(ctx->d_cen, ctx->d_rad, ctx->num, ctx->lda, 3 * NN, ctx->M, 7, ctx->d_wdesc, ctx->d_A, st);
@@ -1983,8 +2056,7 @@ int enqueue(ibem3d_ctx* ctx, int mode, int K, int nloads, const double* d_uvals,
       HIPC(hipMemcpyAsync(ctx->d_uval, d_uvals + (size_t)t * 3 * NN, sizeof(double) * 3 * NN,
                           hipMemcpyDeviceToDevice, st));
       fill_zero(ctx->d_b, sizeof(double) * n, st);
-      launch_bem(ctx->d_qp, ctx->d_conn, ctx->d_nodes, ctx->NN, ctx->NE, ctx->lda, ctx->E0, ctx->nu0, ctx->d_node_col,
-                 ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, ctx->d_bemws, st, false);
+      if (int rc = bem_enqueue(ctx, st, false)) return rc;
       if (ctx->num > 0)
         launch_rows(ctx->d_qp, ctx->d_conn, ctx->d_cen, ctx->num, ctx->NE, ctx->lda, 3 * NN, ctx->M, 7,
                     ctx->d_node_col, ctx->d_slot_col, ctx->d_uval, ctx->d_tval, ctx->d_A, ctx->d_b, st, false,

@@ -312,6 +312,115 @@ __global__ __launch_bounds__(256) void k_bem_fold(const int* __restrict__ conn, 
   carry[(3 + c) * R + row] = bacc;
 }
 
+// ---- the boundary phase as a gather (whole-mesh workspace) --------------------------------------
+// After k_bem_quad over every element (W[e][v][row]):
+//   k_bem_rowsum  thread = (row, component c, element part p): over the part's elements in order,
+//                 rs = the sum of H over every node dof of component c (the rigid-body diagonal,
+//                 BEMBuilderElastic3D.cpp:1478-1501) and bk = the known-value terms of b (ApplyBC
+//                 :1960-2016): -H u for known displacements, +G t for known tractions
+//   k_bem_rowfin  thread = row: the parts added in order; the own dof's diagonal term goes to b when
+//                 its displacement is known; b[row] += the three components' sums in order; rs kept
+//   k_bem_gather  thread = (row, column): the column's H (+) and G (-) values in element order,
+//                 minus rs_c of the row when the column is the row's own node dof c; written once.
+constexpr int kBemParts = 8;
+size_t bem_gather_workspace_doubles(int NN, int NE) {
+  const size_t R = 3 * (size_t)NN;
+  return (size_t)NE * 24 * R + (size_t)kBemParts * 6 * R + 3 * R;
+}
+
+__global__ __launch_bounds__(256) void k_bem_rowsum(const int* __restrict__ conn, int NN, int NE,
+                                                    const int* __restrict__ node_col,
+                                                    const int* __restrict__ slot_col,
+                                                    const double* __restrict__ uval,
+                                                    const double* __restrict__ tval,
+                                                    const double* __restrict__ W, double* __restrict__ part) {
+  const long R = 3L * NN;
+  const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y, p = blockIdx.z;
+  if (row >= R) return;
+  const int e0 = (int)((long)NE * p / kBemParts), e1 = (int)((long)NE * (p + 1) / kBemParts);
+  double rs = 0.0, bk = 0.0;
+  for (int e = e0; e < e1; ++e) {
+    const double* in = W + (size_t)e * 24 * R + row;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int dof = 3 * (conn[4 * e + q] - 1) + c;
+      const double hv = in[(12 + 3 * q + c) * R];
+      rs += hv;
+      if (node_col[dof] < 0) bk -= hv * uval[dof];
+      const int slot = 12 * e + 3 * q + c;
+      if (slot_col[slot] == -1) bk += in[(3 * q + c) * R] * tval[slot];
+    }
+  }
+  part[((size_t)p * 6 + c) * R + row] = rs;
+  part[((size_t)p * 6 + 3 + c) * R + row] = bk;
+}
+
+__global__ __launch_bounds__(256) void k_bem_rowfin(int NN, const int* __restrict__ node_col,
+                                                    const double* __restrict__ uval,
+                                                    const double* __restrict__ part, double* __restrict__ rs,
+                                                    double* __restrict__ b) {
+  const long R = 3L * NN;
+  const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= R) return;
+  double bsum[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double r = 0.0, k = 0.0;
+    for (int p = 0; p < kBemParts; ++p) {
+      r += part[((size_t)p * 6 + c) * R + row];
+      k += part[((size_t)p * 6 + 3 + c) * R + row];
+    }
+    const int dof = 3 * (int)(row / 3) + c;
+    if (node_col[dof] < 0) k += r * uval[dof];
+    rs[c * R + row] = r;
+    bsum[c] = k;
+  }
+  b[row] += (bsum[0] + bsum[1]) + bsum[2];
+}
+
+constexpr int kBemGatherCols = 8;  // columns per workgroup of the gather
+__global__ __launch_bounds__(256) void k_bem_gather(int NN, long lda, const int* __restrict__ cptr,
+                                                    const int* __restrict__ cent, const int* __restrict__ cdof,
+                                                    const double* __restrict__ W, const double* __restrict__ rs,
+                                                    double* __restrict__ A) {
+  const long R = 3L * NN;
+  const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= R) return;
+  const int i = (int)(row / 3);
+  const long j0 = (long)blockIdx.y * kBemGatherCols, j1 = min(R, j0 + kBemGatherCols);
+  for (long j = j0; j < j1; ++j) {
+    double v = 0.0;
+    for (int k = cptr[j]; k < cptr[j + 1]; ++k) {
+      const int x = cent[k];
+      const double w = W[(size_t)(x & 0x7fffffff) * R + row];
+      v = x < 0 ? v - w : v + w;
+    }
+    const int d = cdof[j];
+    if (d >= 0 && d / 3 == i) v -= rs[(d % 3) * R + row];
+    A[j * lda + row] = v;
+  }
+}
+
+void launch_bem_gather(const double* qp, const int* conn, const double* nodes, int NN, int NE, long lda,
+                       double E0, double nu0, const int* node_col, const int* slot_col, const double* uval,
+                       const double* tval, double* A, double* b, double* ws, const BemGather& G,
+                       hipStream_t st, bool write_A) {
+  const long R = 3L * NN;
+  double* W = ws;
+  double* part = ws + (size_t)NE * 24 * R;
+  double* rs = part + (size_t)kBemParts * 6 * R;
+  hipLaunchKernelGGL(k_bem_quad, dim3((NN + 63) / 64, (NE + kBemEPB - 1) / kBemEPB), dim3(192), 0, st, qp, conn,
+                     nodes, NN, 0, NE, E0, nu0, W);
+  const unsigned rb = (unsigned)((R + 255) / 256);
+  hipLaunchKernelGGL(k_bem_rowsum, dim3(rb, 3, kBemParts), dim3(256), 0, st, conn, NN, NE, node_col, slot_col, uval,
+                     tval, W, part);
+  hipLaunchKernelGGL(k_bem_rowfin, dim3(rb), dim3(256), 0, st, NN, node_col, uval, part, rs, b);
+  if (write_A)
+    hipLaunchKernelGGL(k_bem_gather, dim3(rb, (unsigned)((R + kBemGatherCols - 1) / kBemGatherCols)), dim3(256), 0,
+                       st, NN, lda, G.cptr, G.cent, G.cdof, W, rs, A);
+}
+
 // b[row] += the three components' known-value sums, in component order
 __global__ __launch_bounds__(256) void k_bem_finish(int NN, const double* __restrict__ carry,
                                                     double* __restrict__ b) {

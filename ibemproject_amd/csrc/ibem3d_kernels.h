@@ -28,6 +28,22 @@ void launch_bem(const double* qp, const int* conn, const double* nodes, int NN, 
                 double E0, double nu0, const int* node_col, const int* slot_col, const double* uval,
                 const double* tval, double* A, double* b, double* ws, hipStream_t st,
                 bool write_A = true);
+// The boundary phase as a gather over a whole-mesh workspace (24 NE 3NN doubles of quadrature values
+// plus row sums, bem_gather_workspace_doubles): every boundary column's contributions listed in
+// element order (cptr: 3NN + 1 offsets into cent; an entry is e * 24 + value index, with bit 31 set
+// for the G values that are subtracted), cdof[j] = the node dof of displacement column j (-1 for a
+// traction key). Each entry of A's boundary block is written once (no accumulation into A), and b
+// gets the known-value terms. write_A = false: b only.
+struct BemGather {
+  const int* cptr;
+  const int* cent;
+  const int* cdof;
+};
+size_t bem_gather_workspace_doubles(int NN, int NE);
+void launch_bem_gather(const double* qp, const int* conn, const double* nodes, int NN, int NE, long lda,
+                       double E0, double nu0, const int* node_col, const int* slot_col, const double* uval,
+                       const double* tval, double* A, double* b, double* ws, const BemGather& G,
+                       hipStream_t st, bool write_A = true);
 void launch_rows(const double* qp, const int* conn, const double* cen, int num, int NE, long lda,
                  long row0, const Mat& M, int rowmask, const int* node_col, const int* slot_col,
                  const double* uval, const double* tval, double* A, double* b, hipStream_t st,
