@@ -201,7 +201,9 @@ int ibem3d_average_eigenstrain(ibem3d_ctx* ctx, double* eps_star);
  * each in the order 00 01 02 11 12 22; any pointer may be NULL. Host arithmetic in a fixed order. */
 int ibem3d_average_fields(ibem3d_ctx* ctx, double* sigma, double* eps, double* volume);
 
-/* Copy the assembled system (A column-major n x n, b n) to the host; either may be NULL. */
+/* Copy the assembled system (A column-major n x n, b n) to the host; either may be NULL. Every
+ * assembly phase must have run since the last reset (IBEM3D_ESTATE otherwise: the phases write
+ * their blocks of A rather than accumulating into a cleared matrix). */
 int ibem3d_get_system(ibem3d_ctx* ctx, double* A, double* b);
 
 /* Initial guess of the eigenstrain unknowns (60 num, the device's 60 layout: 6 uniform, 18 linear,
