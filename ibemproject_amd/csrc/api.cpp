@@ -661,19 +661,14 @@ int ibem3d_assemble(ibem3d_ctx* ctx) {
     if ((rc = bj_alloc(ctx))) return rc;
   }
   if ((rc = ibem3d_add_bem(ctx))) return rc;
-#ifndef IBEM_AUX_AT
-#define IBEM_AUX_AT 0
-#endif
-  if (bj_next && IBEM_AUX_AT == 0) HIPC(hipEventRecord(ctx->ev_aux_in, ctx->st));    // boundary block complete
+  if (bj_next) HIPC(hipEventRecord(ctx->ev_aux_in, ctx->st));  // boundary block complete
   if ((rc = ibem3d_add_field(ctx))) return rc;
   // all three eigenstrain orders in one pass each: every row belongs to one order, so the entries
   // are the ones the per-order calls write, and the per-pair tables are built once instead of three
   // times
   if ((rc = ibem3d_add_incl(ctx, -1))) return rc;
-  if (bj_next) HIPC(hipEventRecord(ctx->ev_aux_in2, ctx->st));   // sphere blocks complete
-  if (bj_next && IBEM_AUX_AT == 1) HIPC(hipEventRecord(ctx->ev_aux_in, ctx->st));
+  if (bj_next) HIPC(hipEventRecord(ctx->ev_aux_in2, ctx->st));  // sphere blocks complete
   if ((rc = ibem3d_add_bem_rows(ctx, -1))) return rc;
-  if (bj_next && IBEM_AUX_AT == 2) HIPC(hipEventRecord(ctx->ev_aux_in, ctx->st));
   if (bj_next) {
     HIPC(hipStreamWaitEvent(ctx->st_aux, ctx->ev_aux_in, 0));
     {
