@@ -70,6 +70,8 @@ struct ibem3d_ctx {
   // block-Jacobi preconditioner (IBEM3D_SOLVE_BJ_GMRES): the equilibrated boundary block R A_bb C
   // (nb x nb, nb = 3 NN) with its LU workspace, and the inverse of every sphere's 60 x 60 diagonal block
   double *d_Abb = nullptr, *d_rb = nullptr, *d_cb = nullptr, *d_Binv = nullptr, *d_tb = nullptr;
+  double* d_tb32 = nullptr;  // an FP32 boundary factor's right-hand side (nb floats)
+  bool bj_f32 = false;       // the boundary factor in d_Abb is FP32 (leading dimension nb)
   // the boundary block's natural column order (column of row k = node dof k's own unknown), when
   // the column map makes it a permutation; the block is then factored without interchanges
   int* d_bperm = nullptr;
@@ -477,7 +479,7 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
                   ctx->d_work,  ctx->d_res,      ctx->d_Af,       ctx->d_A32,  ctx->d_v32, ctx->d_f16ws, ctx->d_eqws,
                   ctx->d_F16,   ctx->d_F16s,     ctx->d_Abb,      ctx->d_rb,   ctx->d_cb,
-                  ctx->d_Binv,  ctx->d_tb,       ctx->d_x0s,      ctx->Wb.d_ipiv, ctx->Wb.d_ws,
+                  ctx->d_Binv,  ctx->d_tb,       ctx->d_tb32,     ctx->d_x0s,      ctx->Wb.d_ipiv, ctx->Wb.d_ws,
                   ctx->d_A16,   ctx->d_A16s,     ctx->d_iV,       ctx->d_iZ,   ctx->d_iw,
                   ctx->Wb.d_info, ctx->Wb.d_perm, ctx->Wb.d_inv,  ctx->Wb.d_tmp, ctx->Wb.d_trinv,
                   ctx->Wb.d_trsm, ctx->d_bg_ptr, ctx->d_bg_ent, ctx->d_bg_dof,
@@ -944,10 +946,26 @@ static void bj_unscale(ibem3d_ctx* ctx, double* xb, const double* t, hipStream_t
     hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, st, xb, t, ctx->d_cb, nb);
 }
 
-// d_tb <- (R A_bb C)^-1 d_tb on stream st
+__global__ void k_to_f32(float* __restrict__ y, const double* __restrict__ x, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+__global__ void k_from_f32(double* __restrict__ y, const float* __restrict__ x, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (double)x[i];
+}
+
+// d_tb <- (R A_bb C)^-1 d_tb on stream st (an FP32 factor solves with FP32 vectors)
 static void bj_boundary_solve(ibem3d_ctx* ctx, hipStream_t st) {
   const long nb = 3L * ctx->NN;
-  lu_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->d_tb, st);
+  if (ctx->bj_f32) {
+    float* t = reinterpret_cast<float*>(ctx->d_tb32);
+    hipLaunchKernelGGL(k_to_f32, vgrid(nb), dim3(256), 0, st, t, ctx->d_tb, nb);
+    lu_solve_f32(reinterpret_cast<const float*>(ctx->d_Abb), nb, nb, ctx->Wb, t, st);
+    hipLaunchKernelGGL(k_from_f32, vgrid(nb), dim3(256), 0, st, ctx->d_tb, t, nb);
+  } else {
+    lu_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->d_tb, st);
+  }
 }
 
 static void bj_apply(ibem3d_ctx* ctx, const double* v, double* z) {
@@ -968,6 +986,7 @@ static int bj_alloc(ibem3d_ctx* ctx) {
     if (int rc = dalloc(ctx, &ctx->d_rb, (size_t)nb)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_cb, (size_t)nb)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_tb, (size_t)nb)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_tb32, (size_t)nb)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_Binv, (size_t)kBlk * kBlk * std::max(ctx->num, 1))) return rc;
     if (int rc = alloc_luwork(ctx, ctx->Wb, nb)) return rc;
     if (!ctx->d_eqws)
@@ -984,6 +1003,17 @@ static int bj_enqueue_boundary(ibem3d_ctx* ctx, hipStream_t st, bool allow_nopiv
   const long n = ctx->n, nb = 3L * ctx->NN;
   const long mode = get_tuning(kTuneBjNopiv);
   ctx->bj_nopiv = allow_nopiv && ctx->d_bperm && mode != 0;
+  ctx->bj_f32 = ctx->bj_nopiv && get_tuning(kTuneBjFp32) != 0;
+  if (ctx->bj_f32) {
+    float* F = reinterpret_cast<float*>(ctx->d_Abb);
+    if (equilibrate_copy_f32(ctx->d_A, nb, ctx->lda, ctx->d_rb, ctx->d_cb, F, nb, ctx->d_eqws, st, nullptr,
+                             ctx->d_bperm) ||
+        lu_factor_f32_nopiv(F, nb, nb, ctx->Wb, st, mode == 2 ? 1e300 : kBjPivTol))
+      return fail(ctx, IBEM3D_EHIP, "boundary block LU launch failed");
+    if (lu_prepare_solve_f32(F, nb, nb, ctx->Wb, st))
+      return fail(ctx, IBEM3D_EHIP, "boundary block solve preparation failed");
+    return 0;
+  }
   if (ctx->bj_nopiv) {
     // rows and their own unknowns on the diagonal: no interchanges (info reports a small pivot)
     equilibrate_copy_f64(ctx->d_A, nb, ctx->lda, ctx->d_rb, ctx->d_cb, ctx->d_Abb, nb, ctx->d_eqws, st,
@@ -1559,7 +1589,7 @@ static int solve_bj(ibem3d_ctx* ctx, double tol, bool reuse, int mode) {
     HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
     gemv_f64(n, n, -1.0, ctx->d_A, ctx->lda, ctx->d_x, 1.0, ctx->d_res, ctx->d_gemvws, ctx->st);
     hipLaunchKernelGGL(k_scale_vec, vgrid(nb), dim3(256), 0, ctx->st, ctx->d_tb, ctx->d_res, ctx->d_rb, nb);
-    lu_solve_f64(ctx->d_Abb, nb, nb, ctx->Wb, ctx->d_tb, ctx->st);
+    bj_boundary_solve(ctx, ctx->st);
     bj_unscale(ctx, ctx->d_x, ctx->d_tb, ctx->st);
     double rel0 = 0.0;
     if ((rc = residual(ctx, &rel0))) return rc;
@@ -1661,7 +1691,7 @@ int ibem3d_set_initial_guess(ibem3d_ctx* ctx, const double* x_eig) {
 
 int ibem3d_solve_fell_back(const ibem3d_ctx* ctx) { return ctx && ctx->bj_fallback ? 1 : 0; }
 
-int ibem3d_bj_nopiv(const ibem3d_ctx* ctx) { return ctx && ctx->bj_nopiv ? 1 : 0; }
+int ibem3d_bj_nopiv(const ibem3d_ctx* ctx) { return ctx && ctx->bj_nopiv ? (ctx->bj_f32 ? 2 : 1) : 0; }
 
 int ibem3d_rows_moment_form(const ibem3d_ctx* ctx) { return ctx && rows_moment_form(ctx) ? 1 : 0; }
 

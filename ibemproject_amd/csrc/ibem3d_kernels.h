@@ -129,6 +129,7 @@ int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick
 // LU without row interchanges (identity ipiv, so the solves below apply unchanged); W.d_info = column
 // + 1 of the first pivot with magnitude below piv_tol
 int lu_factor_f64_nopiv(double* A, long n, long lda, LuWork& W, hipStream_t st, double piv_tol);
+int lu_factor_f32_nopiv(float* A, long n, long lda, LuWork& W, hipStream_t st, double piv_tol);
 // f16ws != nullptr: the trailing updates run on 16-bit MFMA products of type prod (kProd*) and the
 // panels' deep updates on split FP16 products (workspace of lu_f16_workspace_doubles(n) doubles);
 // otherwise both run on FP32 MFMA
@@ -178,7 +179,7 @@ size_t equilibrate_workspace_doubles(long n);
 void equilibrate_copy_f64(const double* A, long n, long lda, double* r, double* c, double* out, long ldo,
                           double* ws, hipStream_t st, const int* perm = nullptr);
 int equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
-                         double* ws, hipStream_t st, LuWork* W = nullptr);
+                         double* ws, hipStream_t st, LuWork* W = nullptr, const int* perm = nullptr);
 // Compact storage (the factor takes the place of the FP64 system matrix): A <- R A C in place, or
 // FP64 A (leading dimension n) -> FP32 R A C in the same memory with leading dimension lda32
 // (n <= lda32 < 2n), the first columns staged through tmp (at least two columns of lda32 floats).
@@ -208,8 +209,11 @@ void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, co
 // the second stream beside the FP16 product's sphere columns; 0 runs them before the product.
 // kTuneStressBeside: 1 (default) computes the stresses of each GMRES cycle's new x on the second
 // stream beside its true residual, kept when that residual ends the solve; 0 only on request.
+// kTuneBjFp32: 1 (default) factors the block-Jacobi boundary block in FP32 when it needs no row
+// interchanges (the partial-pivoting fallback stays FP64); 0 in FP64.
 enum { kTuneBigMinTiles = 1, kTunePanelCuPercent = 2, kTuneInnerIters = 3, kTuneInnerTolExp = 4,
-       kTuneRowsPointwise = 5, kTuneBjNopiv = 6, kTuneBjOverlap = 7, kTuneStressBeside = 8 };
+       kTuneRowsPointwise = 5, kTuneBjNopiv = 6, kTuneBjOverlap = 7, kTuneStressBeside = 8,
+       kTuneBjFp32 = 9 };
 constexpr int kInnerItersMax = 16;
 int set_tuning(int key, long value);
 long get_tuning(int key);

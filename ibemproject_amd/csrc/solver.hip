@@ -890,6 +890,7 @@ static long g_rows_pointwise = 0;  // 1: per-Gauss-point boundary->inclusion row
 static long g_bj_nopiv = 1;        // block-Jacobi boundary block: 1 no interchanges, 0 pivoting, 2 fallback
 static long g_bj_overlap = 1;      // 1: its triangular solves beside the FP16 product
 static long g_stress_beside = 1;   // 1: stresses beside each GMRES cycle's true residual
+static long g_bj_fp32 = 1;         // 1: the interchange-free boundary factor in FP32
 int set_tuning(int key, long value) {
   if (key == kTuneBigMinTiles && value >= 1) {
     g_big_min_tiles = value;
@@ -923,6 +924,10 @@ int set_tuning(int key, long value) {
     g_stress_beside = value;
     return 0;
   }
+  if (key == kTuneBjFp32 && (value == 0 || value == 1)) {
+    g_bj_fp32 = value;
+    return 0;
+  }
   return -1;
 }
 long get_tuning(int key) {
@@ -935,6 +940,7 @@ long get_tuning(int key) {
     case kTuneBjNopiv: return g_bj_nopiv;
     case kTuneBjOverlap: return g_bj_overlap;
     case kTuneStressBeside: return g_stress_beside;
+    case kTuneBjFp32: return g_bj_fp32;
     default: return -1;
   }
 }
@@ -2132,10 +2138,11 @@ int lu_factor_f64(double* A, long n, long lda, LuWork& W, hipStream_t st, LuTick
 #define IBEM_NOPIV_NB 256
 #endif
 constexpr long kNopivNB = IBEM_NOPIV_NB;  // panel width: the panel's own updates scale with it
-int lu_factor_f64_nopiv(double* A, long n, long lda, LuWork& W, hipStream_t st, double piv_tol) {
+template <typename T>
+static int lu_factor_nopiv(T* A, long n, long lda, LuWork& W, hipStream_t st, double piv_tol) {
   if (lu_streams(W)) return -1;
-  LuCtx<double> c{A, n, lda, &W, st, nullptr, nullptr, 0, nullptr};
-  LuCtx<double> c2{A, n, lda, &W, W.st2, nullptr, nullptr, 0, nullptr};
+  LuCtx<T> c{A, n, lda, &W, st, nullptr, nullptr, 0, nullptr};
+  LuCtx<T> c2{A, n, lda, &W, W.st2, nullptr, nullptr, 0, nullptr};
   fill_zero(W.d_info, 2 * sizeof(int), st);  // info and the leaves' arrival count
   panel_rec_nopiv(c, 0, std::min(kNopivNB, n), piv_tol);
   for (long k = 0; k < n; k += kNopivNB) {
@@ -2144,20 +2151,26 @@ int lu_factor_f64_nopiv(double* A, long n, long lda, LuWork& W, hipStream_t st, 
     if (k + kb >= n) break;
     const long m = n - k - kb;
     const long kb2 = std::min(kNopivNB, m);
-    const double* L11 = A + k * lda + k;
-    trsm_lower_unit<double>(L11, lda, kb, A + (k + kb) * lda + k, lda, kb2, st);
-    trailing_update<double>(A, n, lda, k, kb, 0, kb2, nullptr, st);
+    const T* L11 = A + k * lda + k;
+    trsm_lower_unit<T>(L11, lda, kb, A + (k + kb) * lda + k, lda, kb2, st);
+    trailing_update<T>(A, n, lda, k, kb, 0, kb2, nullptr, st);
     hipEventRecord(W.ev_a, st);
     const long nr = m - kb2;
     if (nr > 0) {
-      trsm_lower_unit<double>(L11, lda, kb, A + (k + kb + kb2) * lda + k, lda, nr, st);
-      trailing_update<double>(A, n, lda, k, kb, kb2, nr, nullptr, st);
+      trsm_lower_unit<T>(L11, lda, kb, A + (k + kb + kb2) * lda + k, lda, nr, st);
+      trailing_update<T>(A, n, lda, k, kb, kb2, nr, nullptr, st);
     }
     hipStreamWaitEvent(W.st2, W.ev_a, 0);
     panel_rec_nopiv(c2, k + kb, kb2, piv_tol);
     hipEventRecord(W.ev_b, W.st2);
   }
   return 0;
+}
+int lu_factor_f64_nopiv(double* A, long n, long lda, LuWork& W, hipStream_t st, double piv_tol) {
+  return lu_factor_nopiv<double>(A, n, lda, W, st, piv_tol);
+}
+int lu_factor_f32_nopiv(float* A, long n, long lda, LuWork& W, hipStream_t st, double piv_tol) {
+  return lu_factor_nopiv<float>(A, n, lda, W, st, piv_tol);
 }
 int lu_factor_f32(float* A, long n, long lda, LuWork& W, hipStream_t st, LuTick tick, void* tctx,
                   double* f16ws, bool cols_pending, int prod) {
@@ -2640,8 +2653,9 @@ void equilibrate_copy_f64(const double* A, long n, long lda, double* r, double* 
   (void)equilibrate_copy<double>(A, n, lda, r, c, out, ldo, ws, st, nullptr, nullptr, nullptr, 0, perm);
 }
 int equilibrate_copy_f32(const double* A, long n, long lda, double* r, double* c, float* out, long ldo,
-                         double* ws, hipStream_t st, LuWork* W) {
-  if (!W) return equilibrate_copy<float>(A, n, lda, r, c, out, ldo, ws, st);
+                         double* ws, hipStream_t st, LuWork* W, const int* perm) {
+  if (!W) return equilibrate_copy<float>(A, n, lda, r, c, out, ldo, ws, st, nullptr, nullptr, nullptr, 0, perm);
+  if (perm) return -1;
   if (lu_streams(*W)) return -1;
   // the columns past the first panel are copied beside it
   return equilibrate_copy<float>(A, n, lda, r, c, out, ldo, ws, st, W->st2, W->ev_r, W->ev_cols, kLuNB);

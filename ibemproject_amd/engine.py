@@ -38,6 +38,7 @@ TUNE_ROWS_POINTWISE = 5
 TUNE_BJ_NOPIV = 6
 TUNE_BJ_OVERLAP = 7
 TUNE_STRESS_BESIDE = 8
+TUNE_BJ_FP32 = 9
 
 
 def set_tuning(key: int, value: int) -> None:
@@ -341,9 +342,10 @@ class Engine:
     def fell_back(self) -> bool:
         return bool(self.L.ibem3d_solve_fell_back(self.ctx))
 
-    def bj_nopiv(self) -> bool:
-        """True when the block-Jacobi boundary factor has no row interchanges (natural column order)."""
-        return bool(self.L.ibem3d_bj_nopiv(self.ctx))
+    def bj_nopiv(self) -> int:
+        """The block-Jacobi boundary factor: 2 without row interchanges (natural column order) in
+        FP32, 1 without interchanges in FP64, 0 with partial pivoting or none yet."""
+        return int(self.L.ibem3d_bj_nopiv(self.ctx))
 
     def rows_moment_form(self) -> bool:
         """True when the boundary->inclusion rows are integrated in moment form (planar elements)."""

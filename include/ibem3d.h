@@ -136,11 +136,14 @@ enum {
                                         triangular solves of each inner iteration run on a second
                                         stream while the FP16 product covers the columns past the
                                         boundary block; 0: before the product */
-  IBEM3D_TUNE_STRESS_BESIDE = 8      /* 1 (default): each GMRES cycle computes the stresses of its new
+  IBEM3D_TUNE_STRESS_BESIDE = 8,     /* 1 (default): each GMRES cycle computes the stresses of its new
                                         solution on a second stream beside its true FP64 residual;
                                         ibem3d_stress returns them when that residual ended the solve
                                         (bit-identical to computing them afterwards); 0: ibem3d_stress
                                         computes them */
+  IBEM3D_TUNE_BJ_FP32 = 9            /* 1 (default): the block-Jacobi boundary block, when factored
+                                        without row interchanges, is factored and solved in FP32 (it
+                                        only shapes search directions; residuals stay FP64); 0: FP64 */
 };
 int ibem3d_set_tuning(int key, long value);
 long ibem3d_get_tuning(int key);  /* -1 for an unknown key */
@@ -224,9 +227,9 @@ long ibem3d_inner_iterations(const ibem3d_ctx* ctx);
  * 0 for the per-point contraction (a non-planar element, or IBEM3D_TUNE_ROWS_POINTWISE = 1). The
  * entries agree to rounding. No counterpart in the reference. */
 int ibem3d_rows_moment_form(const ibem3d_ctx* ctx);
-/* 1 when the block-Jacobi preconditioner's current boundary factor was computed without row
- * interchanges (IBEM3D_TUNE_BJ_NOPIV), 0 with partial pivoting or before any block-Jacobi setup.
- * No counterpart in the reference. */
+/* The block-Jacobi preconditioner's current boundary factor: 2 computed without row interchanges in
+ * FP32 (IBEM3D_TUNE_BJ_NOPIV, IBEM3D_TUNE_BJ_FP32), 1 without interchanges in FP64, 0 with partial
+ * pivoting (FP64) or before any block-Jacobi setup. No counterpart in the reference. */
 int ibem3d_bj_nopiv(const ibem3d_ctx* ctx);
 /* The last block-Jacobi solve's start from the initial guess: relative residual of the scaled guess
  * alpha x0 (alpha in [0, 1] minimises ||b - alpha A x0||, so the start is never worse than zero);

@@ -239,18 +239,21 @@ def test_block_jacobi_setup_beside_assembly(mid):
 
 def test_bj_boundary_without_interchanges(mid):
     """The block-Jacobi boundary block factored without row interchanges in its natural column order
-    (the default), with partial pivoting, and through the pivoting fallback (a threshold no pivot
-    meets), and its triangular solves beside the FP16 product (the default) or before it: the
-    preconditioners are the same matrix inverse, so each solve reaches 1e-10 with the same iteration
-    counts (+-1) and the solutions agree to 1e-8 (the bar of the refined modes); the second
-    assembly's setup on the second stream takes the same path, and repeats the first bit for bit."""
+    in FP32 (the default) or FP64, with partial pivoting, and through the pivoting fallback (a
+    threshold no pivot meets), and its triangular solves beside the FP16 product (the default) or
+    before it: the preconditioners are the same matrix inverse to FP32 accuracy or better, so each
+    solve reaches 1e-10 with the same iteration counts (+-1) and the solutions agree to 1e-8 (the bar
+    of the refined modes); the second assembly's setup on the second stream takes the same path,
+    and repeats the first bit for bit."""
     case(f"block-Jacobi boundary block without interchanges, n60={mid['prob'].n60}")
     out = {}
-    configs = [(1, 1), (0, 1), (2, 1), (1, 0)]  # (IBEM3D_TUNE_BJ_NOPIV, IBEM3D_TUNE_BJ_OVERLAP)
+    # (IBEM3D_TUNE_BJ_NOPIV, IBEM3D_TUNE_BJ_OVERLAP, IBEM3D_TUNE_BJ_FP32) and the factor expected
+    configs = [((1, 1, 1), 2), ((1, 1, 0), 1), ((0, 1, 1), 0), ((2, 1, 1), 0), ((1, 0, 1), 2)]
     try:
-        for cfg in configs:
+        for cfg, want in configs:
             E.set_tuning(E.TUNE_BJ_NOPIV, cfg[0])
             E.set_tuning(E.TUNE_BJ_OVERLAP, cfg[1])
+            E.set_tuning(E.TUNE_BJ_FP32, cfg[2])
             eng = E.Engine(mid["prob"])
             res = []
             for step in range(2):  # the second assembly builds the preconditioner beside it
@@ -264,18 +267,19 @@ def test_bj_boundary_without_interchanges(mid):
             assert rel_b <= 1e-10
             res.append((it_b, 0, eng.getUTVector()[2], eng.bj_nopiv()))
             out[cfg] = res
-            print(f"  setting {cfg}: " + ", ".join(f"its {r[0]}/{r[1]} nopiv {r[3]}" for r in res))
+            print(f"  setting {cfg}: " + ", ".join(f"its {r[0]}/{r[1]} factor {r[3]}" for r in res))
+            assert all(r[3] == want for r in res)
             eng.close()
     finally:
         E.set_tuning(E.TUNE_BJ_NOPIV, 1)
         E.set_tuning(E.TUNE_BJ_OVERLAP, 1)
-    assert all(r[3] for r in out[(1, 1)]) and all(r[3] for r in out[(1, 0)])
-    assert not any(r[3] for r in out[(0, 1)]) and not any(r[3] for r in out[(2, 1)])
-    for cfg in configs:  # the two assemblies and solves of each setting agree bit for bit
+        E.set_tuning(E.TUNE_BJ_FP32, 1)
+    for cfg, _ in configs:  # the two assemblies and solves of each setting agree bit for bit
         assert out[cfg][0][0] == out[cfg][1][0] and np.array_equal(out[cfg][0][2], out[cfg][1][2])
-    x0 = out[(0, 1)][0][2]
-    for cfg in ((1, 1), (2, 1), (1, 0)):
-        for a, b in zip(out[cfg], out[(0, 1)]):
+    base = out[(0, 1, 1)]
+    x0 = base[0][2]
+    for cfg, _ in configs:
+        for a, b in zip(out[cfg], base):
             assert abs(a[0] - b[0]) <= 1 and abs(a[1] - b[1]) <= 2
             assert np.linalg.norm(a[2] - b[2]) / np.linalg.norm(x0) < 1e-8
 
