@@ -55,6 +55,7 @@ struct ibem3d_ctx {
   // GMRES-IR: Krylov basis V (kGmres + 1 columns), preconditioned directions Z (kGmres columns),
   // dot-product partial sums, small coefficient vector
   double *d_gV = nullptr, *d_gZ = nullptr, *d_gpart = nullptr, *d_gh = nullptr;
+  double* d_gW = nullptr;  // the GMRES cycle's FP64 products A z_j, before orthogonalisation
   double* d_bemws = nullptr;  // boundary-boundary quadrature values (whole mesh, or one round of elements)
   // the boundary phase as a gather (launch_bem_gather) when its whole-mesh workspace fits: every
   // boundary column's contributions in element order (CSR), each column's node dof
@@ -120,7 +121,7 @@ struct ibem3d_ctx {
   std::map<std::string, double> folded_ms;
   long timing_errors = 0;
   // stress inputs and output (kept between calls)
-  double *d_stU = nullptr, *d_stT = nullptr, *d_sts = nullptr;
+  double *d_stU = nullptr, *d_stT = nullptr, *d_sts = nullptr, *d_stws = nullptr;
   // stresses computed on the second stream beside a solve's final residual check (ev_st_out marks
   // their end): stress_pending while the main stream has not waited for them, stress_spec while they
   // belong to the current solution
@@ -473,7 +474,8 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
   ibem3d_reset_timings(ctx);
   for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
   ctx->pool.clear();
-  void* ptrs[] = {ctx->d_stU,   ctx->d_stT,      ctx->d_sts,      ctx->d_cw,  ctx->d_ecol, ctx->d_eknown, ctx->d_sink, ctx->d_wdesc, ctx->d_bperm,
+  void* ptrs[] = {ctx->d_stU,   ctx->d_stT,      ctx->d_sts,      ctx->d_stws,     ctx->d_cw,   ctx->d_ecol, ctx->d_eknown,
+                  ctx->d_sink,  ctx->d_wdesc,    ctx->d_bperm,
                   ctx->d_nodes, ctx->d_cen,      ctx->d_rad,      ctx->d_pts,  ctx->d_qp,
                   ctx->d_uval,  ctx->d_tval,     ctx->d_conn,     ctx->d_node_col, ctx->d_slot_col,
                   ctx->d_A,     ctx->d_b,        ctx->d_x,        ctx->d_r,    ctx->d_c,
@@ -483,7 +485,8 @@ void ibem3d_destroy(ibem3d_ctx* ctx) {
                   ctx->d_A16,   ctx->d_A16s,     ctx->d_iV,       ctx->d_iZ,   ctx->d_iw,
                   ctx->Wb.d_info, ctx->Wb.d_perm, ctx->Wb.d_inv,  ctx->Wb.d_tmp, ctx->Wb.d_trinv,
                   ctx->Wb.d_trsm, ctx->d_bg_ptr, ctx->d_bg_ent, ctx->d_bg_dof,
-                  ctx->d_gV,    ctx->d_gZ,       ctx->d_gpart,    ctx->d_gh,   ctx->d_bemws, ctx->d_gemvws,
+                  ctx->d_gV,    ctx->d_gZ,       ctx->d_gW,       ctx->d_gpart, ctx->d_gh,  ctx->d_bemws,
+                  ctx->d_gemvws,
                   ctx->W.d_ipiv, ctx->W.d_ws,  ctx->W.d_info, ctx->W.d_perm, ctx->W.d_inv, ctx->W.d_tmp,
                   ctx->W.d_trinv, ctx->W.d_trsm};
   for (void* q : ptrs)
@@ -1195,14 +1198,9 @@ static void free_m16(ibem3d_ctx* ctx) {
   ctx->d_iV = ctx->d_iZ = ctx->d_iw = nullptr;
 }
 
-// relative residual ||b - A x|| / ||b|| with the FP64 system matrix
-static int residual(ibem3d_ctx* ctx, double* rel) {
+// ||d_res|| / ||b||
+static int residual_norm(ibem3d_ctx* ctx, double* rel) {
   const long n = ctx->n;
-  HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
-  {
-    Scope s(ctx, "solve.residual");
-    gemv_f64(n, n, -1.0, ctx->d_A, ctx->lda, ctx->d_x, 1.0, ctx->d_res, ctx->d_gemvws, ctx->st);
-  }
   std::vector<double> r(n), b(n);
   HIPC(hipMemcpyAsync(r.data(), ctx->d_res, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
   HIPC(hipMemcpyAsync(b.data(), ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->st));
@@ -1210,6 +1208,17 @@ static int residual(ibem3d_ctx* ctx, double* rel) {
   const double nb = dnrm2_host(b);
   *rel = nb > 0 ? dnrm2_host(r) / nb : dnrm2_host(r);
   return 0;
+}
+
+// relative residual ||b - A x|| / ||b|| with the FP64 system matrix (b - A x left in d_res)
+static int residual(ibem3d_ctx* ctx, double* rel) {
+  const long n = ctx->n;
+  HIPC(hipMemcpyAsync(ctx->d_res, ctx->d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->st));
+  {
+    Scope s(ctx, "solve.residual");
+    gemv_f64(n, n, -1.0, ctx->d_A, ctx->lda, ctx->d_x, 1.0, ctx->d_res, ctx->d_gemvws, ctx->st);
+  }
+  return residual_norm(ctx, rel);
 }
 
 // The stresses of the x just formed, on the second stream while the main stream computes its true
@@ -1244,12 +1253,14 @@ static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0, int max
   if (!ctx->d_gV) {
     if (int rc = dalloc(ctx, &ctx->d_gV, (size_t)(kGmres + 1) * n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_gZ, (size_t)kGmres * n)) return rc;
+    if (int rc = dalloc(ctx, &ctx->d_gW, (size_t)kGmres * n)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_gpart, (size_t)(kGmres + 1) * kDotBlocks)) return rc;
     if (int rc = dalloc(ctx, &ctx->d_gh, (size_t)kGmres + 1)) return rc;
   }
   double* V = ctx->d_gV;
   double* Z = ctx->d_gZ;
   double* w = ctx->d_work;
+  const bool upd = get_tuning(kTuneResidualUpdate) != 0;
   double bnorm = 0.0;
   if (int rc = dots(ctx, ctx->d_b, 1, ctx->d_b, &bnorm)) return rc;
   bnorm = std::sqrt(bnorm);
@@ -1287,6 +1298,9 @@ static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0, int max
         Scope s(ctx, "solve.residual");
         gemv_f64(n, n, 1.0, ctx->d_A, ctx->lda, zj, 0.0, w, ctx->d_gemvws, ctx->st);
       }
+      if (upd)
+        HIPC(hipMemcpyAsync(ctx->d_gW + (size_t)j * n, w, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                            ctx->st));
       if (int rc = after_launch(ctx, "gmres")) return rc;
       double* Hj = H.data() + (size_t)j * (kGmres + 1);  // column j
       {
@@ -1336,7 +1350,17 @@ static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0, int max
     if (int rc = join_stress(ctx, true)) return rc;  // the previous cycle's stresses read x
     hipLaunchKernelGGL(k_combo, vgrid(n), dim3(256), 0, ctx->st, ctx->d_x, Z, n, k, ctx->d_gh, 1.0, n);
     if (int rc = stress_beside(ctx)) return rc;
-    if (int rc = residual(ctx, &rel)) return rc;  // r = b - A x in d_res
+    bool met = false;
+    if (upd) {
+      // b - A x = r - (A Z) y with the cycle's own FP64 products: the new x's residual without
+      // another pass over A (it differs from b - A x computed by a product by rounding alone)
+      hipLaunchKernelGGL(k_combo, vgrid(n), dim3(256), 0, ctx->st, ctx->d_res, ctx->d_gW, n, k, ctx->d_gh,
+                         -1.0, n);
+      if (int rc = residual_norm(ctx, &rel)) return rc;
+      met = rel <= tol;
+    }
+    if (!met)
+      if (int rc = residual(ctx, &rel)) return rc;  // r = b - A x in d_res
     ctx->rel_res = rel;
     if (!(rel == rel)) return fail(ctx, IBEM3D_ESOLVE, "refinement diverged (NaN)");
     beta = rel * bnorm;
@@ -1932,6 +1956,7 @@ static int stress_alloc(ibem3d_ctx* ctx) {
   if (ctx->d_stU) return 0;
   if (int rc = dalloc(ctx, &ctx->d_stU, 3L * ctx->NN)) return rc;
   if (int rc = dalloc(ctx, &ctx->d_stT, 12L * ctx->NE)) return rc;
+  if (int rc = dalloc(ctx, &ctx->d_stws, (long)stress_workspace_doubles(ctx->num, ctx->nump))) return rc;
   return dalloc(ctx, &ctx->d_sts, 6L * ctx->nump);
 }
 
@@ -1941,7 +1966,7 @@ static void enqueue_stress(ibem3d_ctx* ctx, hipStream_t st) {
   launch_stress_inputs(ctx->d_x, ctx->d_node_col, ctx->d_uval, ctx->d_slot_col, ctx->d_tval, NN, NE,
                        ctx->d_stU, ctx->d_stT, st);
   launch_stress(ctx->d_qp, ctx->d_conn, ctx->d_nodes, NN, NE, ctx->d_cen, ctx->d_rad, ctx->num, ctx->d_pts,
-                ctx->nump, ctx->M, ctx->d_stU, ctx->d_stT, ctx->d_x + 3L * NN, ctx->d_sts, st);
+                ctx->nump, ctx->M, ctx->d_stU, ctx->d_stT, ctx->d_x + 3L * NN, ctx->d_stws, ctx->d_sts, st);
 }
 
 int ibem3d_stress(ibem3d_ctx* ctx, double* s) {

@@ -111,6 +111,26 @@ def main():
             code = code.replace(f"tab[{tidx(*b) + 84}]", f"tab[{tidx(*b) + 35}]")
         lines.append(code)
     lines.append("}")
+    # the stress kernel's inclusion term at points outside a sphere: every d^beta r (4 <= |beta| <= 6)
+    # and d^beta r^-1 (2 <= |beta| <= 6) at x = point - centre, each contracted at once with the
+    # sphere's six rows of moments (V[ij * 144 + s], s = tidx - 20 for r, 64 + tidx - 4 for r^-1)
+    mom = [("S", b, tidx(*b) - 20) for d in range(4, 7) for b in multi(d)] + \
+          [("T", b, 64 + tidx(*b) - 4) for d in range(2, 7) for b in multi(d)]
+    assert sorted(s for _, _, s in mom) == list(range(144))
+    lines += [
+        "// incl_stress_moments(y0, y1, y2, PR, PT, V, e): e[ij] += sum_beta d^beta r V[144 ij + tidx(beta) - 20]",
+        "// (4 <= |beta| <= 6) + sum_beta d^beta r^-1 V[144 ij + 60 + tidx(beta)] (2 <= |beta| <= 6);",
+        "// PR[m], PT[m] as above with m <= 6 for both.",
+        "IBEM_HD void incl_stress_moments(const double* y0, const double* y1, const double* y2,",
+        "                                 const double* PR, const double* PT,",
+        "                                 const double* __restrict__ V, double* e) {",
+        "  double d;",
+    ]
+    for t, b, s in mom:
+        code = entry_code(t, b)[0]
+        lines.append("  d = " + code.split(" = ", 1)[1])
+        lines.append("  " + " ".join(f"e[{ij}] += d * V[{144 * ij + s}];" for ij in range(6)))
+    lines.append("}")
     with open(os.path.join(HERE, "kelvin_tables.inc"), "w") as f:
         f.write("\n".join(lines) + "\n")
     print("groups:", load)

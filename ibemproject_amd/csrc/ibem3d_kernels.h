@@ -73,10 +73,12 @@ void launch_planar(const double* qp, int NE, int* flag, double* cw, hipStream_t 
 // U (3 NN node dofs) and T (12 NE element slots) of the stress kernel from the device solution x
 void launch_stress_inputs(const double* x, const int* node_col, const double* uval, const int* slot_col,
                           const double* tval, int NN, int NE, double* U, double* T, hipStream_t st);
+// workspace of launch_stress (doubles): per-sphere moments, per-chunk inclusion strains, masks
+size_t stress_workspace_doubles(int num, int nump);
 void launch_stress(const double* qp, const int* conn, const double* nodes, int NN, int NE,
                    const double* cen, const double* rad, int num, const double* pts, int nump,
-                   const Mat& M, const double* U, const double* T, const double* E60, double* s,
-                   hipStream_t st);
+                   const Mat& M, const double* U, const double* T, const double* E60, double* ws,
+                   double* s, hipStream_t st);
 
 // ---- dense solver (solver.hip) ----
 struct Timer;  // optional per-launch timing hook (api.cpp)
@@ -211,9 +213,12 @@ void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, co
 // stream beside its true residual, kept when that residual ends the solve; 0 only on request.
 // kTuneBjFp32: 1 (default) factors the block-Jacobi boundary block in FP32 when it needs no row
 // interchanges (the partial-pivoting fallback stays FP64); 0 in FP64.
+// kTuneResidualUpdate: 1 (default) checks each GMRES cycle's new x with r - (A Z) y, from the FP64
+// products of the cycle (b - A x by a product only when that misses the tolerance); 0 always by a
+// product.
 enum { kTuneBigMinTiles = 1, kTunePanelCuPercent = 2, kTuneInnerIters = 3, kTuneInnerTolExp = 4,
        kTuneRowsPointwise = 5, kTuneBjNopiv = 6, kTuneBjOverlap = 7, kTuneStressBeside = 8,
-       kTuneBjFp32 = 9 };
+       kTuneBjFp32 = 9, kTuneResidualUpdate = 10 };
 constexpr int kInnerItersMax = 16;
 int set_tuning(int key, long value);
 long get_tuning(int key);

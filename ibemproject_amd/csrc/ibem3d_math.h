@@ -325,6 +325,38 @@ IBEM_HD double Dtrace(const double* tab, int stride, const Mat& M, int w, int m,
   return v;
 }
 
+// Transpose of pot(): the table entry pot() reads (table f = 0..7, index t) and its weight, passed
+// to sink(f, t, s) -- so that sum over calls of sink's s T_f[t] equals the forward value.
+template <class Sink>
+IBEM_HD void pot_adj(Sink& sink, int psi, int w, int c0, int c1, int c2, double s) {
+  const int f0 = psi ? 4 : 0;
+  if (w == 0) {
+    sink(f0, tidx(c0, c1, c2), s);
+  } else if (w < 4) {
+    const int p = w - 1;
+    sink(f0 + 1, tidx(c0 + (p == 0), c1 + (p == 1), c2 + (p == 2)), s);
+  } else {
+    const int p = pair_i(w - 4), q = pair_j(w - 4);
+    sink(f0 + 3, tidx(c0 + (p == 0) + (q == 0), c1 + (p == 1) + (q == 1), c2 + (p == 2) + (q == 2)), s);
+    if (p == q) sink(f0 + 2, tidx(c0, c1, c2), s);
+  }
+}
+
+// Transpose of Dten() with e = 0: s times each table entry's weight in D(k,l | m,n | w) to sink
+template <class Sink>
+IBEM_HD void Dten_adj(Sink& sink, const Mat& M, int w, int k, int l, int m, int n, double s) {
+#define IBEM_C3(a, b) ((a) == 0) + ((b) == 0), ((a) == 1) + ((b) == 1), ((a) == 2) + ((b) == 2)
+  const double t = M.cD * s, u = -(1.0 - M.nu0) * t;
+  pot_adj(sink, 1, w, (m == 0) + (n == 0) + (k == 0) + (l == 0), (m == 1) + (n == 1) + (k == 1) + (l == 1),
+          (m == 2) + (n == 2) + (k == 2) + (l == 2), t);
+  if (m == n) pot_adj(sink, 0, w, IBEM_C3(k, l), -2.0 * M.nu0 * t);
+  if (k == n) pot_adj(sink, 0, w, IBEM_C3(m, l), u);
+  if (l == n) pot_adj(sink, 0, w, IBEM_C3(m, k), u);
+  if (k == m) pot_adj(sink, 0, w, IBEM_C3(n, l), u);
+  if (l == m) pot_adj(sink, 0, w, IBEM_C3(n, k), u);
+#undef IBEM_C3
+}
+
 // Row / column decode of the 60-unknown sphere block:
 //   0..5   E_ij            (t = pair)
 //   6..23  E_ij,k          6 + 3 t + k
@@ -956,8 +988,9 @@ constexpr int kKStride = 119;
 
 #include "kelvin_tables.inc"
 
-// Inputs of kelvin_group at x: y_c[p] = (2 x_c)^p, PR[m] = d^m/drho^m rho^(1/2),
-// PT[m] = d^m/drho^m rho^(-1/2).
+// Inputs of kelvin_group at x: y_c[p] = (2 x_c)^p, PR[m] = d^m/drho^m rho^(1/2) (m <= 6),
+// PT[m] = d^m/drho^m rho^(-1/2) (m <= MT).
+template <int MT = 4>
 IBEM_HD void kelvin_point(const double* x, double* y0, double* y1, double* y2, double* PR,
                           double* PT) {
   const double rho = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
@@ -974,7 +1007,7 @@ IBEM_HD void kelvin_point(const double* x, double* y0, double* y1, double* y2, d
   for (int m = 1; m <= 6; ++m) PR[m] = PR[m - 1] * (0.5 - (m - 1)) * inv;
   PT[0] = 1.0 / r;
 #pragma unroll
-  for (int m = 1; m <= 4; ++m) PT[m] = PT[m - 1] * (-0.5 - (m - 1)) * inv;
+  for (int m = 1; m <= MT; ++m) PT[m] = PT[m - 1] * (-0.5 - (m - 1)) * inv;
 }
 
 // select t[i] (i = 0..2 / 0..5) without an indexed private array

@@ -891,6 +891,7 @@ static long g_bj_nopiv = 1;        // block-Jacobi boundary block: 1 no intercha
 static long g_bj_overlap = 1;      // 1: its triangular solves beside the FP16 product
 static long g_stress_beside = 1;   // 1: stresses beside each GMRES cycle's true residual
 static long g_bj_fp32 = 1;         // 1: the interchange-free boundary factor in FP32
+static long g_res_update = 1;      // 1: a cycle's residual from its own products when it converges
 int set_tuning(int key, long value) {
   if (key == kTuneBigMinTiles && value >= 1) {
     g_big_min_tiles = value;
@@ -928,6 +929,10 @@ int set_tuning(int key, long value) {
     g_bj_fp32 = value;
     return 0;
   }
+  if (key == kTuneResidualUpdate && (value == 0 || value == 1)) {
+    g_res_update = value;
+    return 0;
+  }
   return -1;
 }
 long get_tuning(int key) {
@@ -941,6 +946,7 @@ long get_tuning(int key) {
     case kTuneBjOverlap: return g_bj_overlap;
     case kTuneStressBeside: return g_stress_beside;
     case kTuneBjFp32: return g_bj_fp32;
+    case kTuneResidualUpdate: return g_res_update;
     default: return -1;
   }
 }

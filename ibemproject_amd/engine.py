@@ -39,6 +39,7 @@ TUNE_BJ_NOPIV = 6
 TUNE_BJ_OVERLAP = 7
 TUNE_STRESS_BESIDE = 8
 TUNE_BJ_FP32 = 9
+TUNE_RESIDUAL_UPDATE = 10
 
 
 def set_tuning(key: int, value: int) -> None:

@@ -141,9 +141,14 @@ enum {
                                         ibem3d_stress returns them when that residual ended the solve
                                         (bit-identical to computing them afterwards); 0: ibem3d_stress
                                         computes them */
-  IBEM3D_TUNE_BJ_FP32 = 9            /* 1 (default): the block-Jacobi boundary block, when factored
+  IBEM3D_TUNE_BJ_FP32 = 9,           /* 1 (default): the block-Jacobi boundary block, when factored
                                         without row interchanges, is factored and solved in FP32 (it
                                         only shapes search directions; residuals stay FP64); 0: FP64 */
+  IBEM3D_TUNE_RESIDUAL_UPDATE = 10   /* 1 (default): a GMRES cycle checks its new x with
+                                        r - (A Z) y, r the cycle's starting residual and A Z its FP64
+                                        products (the same b - A x without another pass over A), and
+                                        computes b - A x by a product only when that misses the
+                                        tolerance; 0: always by a product */
 };
 int ibem3d_set_tuning(int key, long value);
 long ibem3d_get_tuning(int key);  /* -1 for an unknown key */

@@ -145,6 +145,38 @@ def test_stress(solved):
     assert e < 1e-10
 
 
+def test_stress_sphere_chunks(oracle_lib):
+    """The inclusion term runs over chunks of spheres sized from the point count (about 4096
+    wavefronts of 64 points, at most 64 spheres a chunk). 12,800 points put two of the 24 spheres in
+    each chunk, their first 40 alone one: the two agree to summation order, and the 40 match the
+    oracle. Points at two sphere centres make the containing sphere the second of its chunk and the
+    first of another."""
+    import copy
+    big = make_rve(13, 24, 0.05, 0.08, 7, nump=12800, E1=30.0, nu1=0.3)
+    pts = big.points.copy()
+    pts[0] = big.centres[3]
+    pts[1] = big.centres[0]
+    big.points = pts
+    small = copy.deepcopy(big)
+    small.points = pts[:40].copy()
+    out = {}
+    for name, prob in (("12800 points", big), ("40 points", small)):
+        case(f"stress sphere chunks: {name}, num={prob.num}")
+        eng = Engine(prob)
+        eng.assemble()
+        eng.solve(SOLVE_FP64_LU)
+        out[name] = (eng.getFlux(), eng.getUTVector())
+        eng.close()
+    s_big, s_small = out["12800 points"][0], out["40 points"][0]
+    scale = np.abs(s_small).max()
+    assert np.abs(s_big[:40] - s_small).max() / scale < 1e-13
+    U, T, _ = out["40 points"][1]
+    ref = oracle_lib.OracleProblem(small).stress(U, T)
+    e = np.abs(s_small - ref).max() / np.abs(ref).max()
+    print("stress relerr", e)
+    assert e < 1e-10
+
+
 @pytest.mark.parametrize("mode", [SOLVE_MIXED_FP32, SOLVE_MIXED_FP16, SOLVE_MIXED_FP16 | SOLVE_GMRES])
 def test_mixed_refined(solved, mode):
     eng, ref = solved["eng"], solved["ref"]

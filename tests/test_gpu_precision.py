@@ -316,3 +316,34 @@ def test_stress_beside_final_residual(mid):
     s1, s2 = out[1][0], out[1][2]
     assert not np.array_equal(s1, s2)
     assert np.linalg.norm(s2 - 2.0 * s1) <= 1e-8 * np.linalg.norm(s2)
+
+
+def test_residual_from_cycle_products(mid):
+    """The GMRES cycle that meets the tolerance checks its x with r - (A Z) y from its own FP64
+    products (the default) instead of another product b - A x: the same x bit for bit, the same
+    iteration counts, and a reported residual that matches b - A x computed by a product to rounding
+    (for the 16-bit-inner block-Jacobi mode and a GMRES-IR LU mode)."""
+    p = mid["prob"]
+    case(f"residual from the cycle's products, n60={p.n60}")
+    out = {}
+    try:
+        for flag in (1, 0):
+            E.set_tuning(E.TUNE_RESIDUAL_UPDATE, flag)
+            eng = E.Engine(p)
+            eng.assemble()
+            res = []
+            for mode in (E.SOLVE_BJ_GMRES16, E.SOLVE_MIXED_FP16_1 | E.SOLVE_GMRES):
+                rel, it = eng.solve(mode, 1e-10)
+                _, _, x = eng.getUTVector()
+                res.append((rel, it, np.array(x, copy=True), eng.residual()))
+            out[flag] = res
+            eng.close()
+    finally:
+        E.set_tuning(E.TUNE_RESIDUAL_UPDATE, 1)
+    for (r1, i1, x1, t1), (r0, i0, x0, t0) in zip(out[1], out[0]):
+        print(f"reported {r1:.3e} (update) {r0:.3e} (product); by a product afterwards {t1:.3e}")
+        assert i1 == i0
+        assert np.array_equal(x1, x0)
+        assert r1 <= 1e-10 and t1 <= 1e-10
+        assert r0 == t0
+        assert abs(r1 - t1) <= 1e-2 * t1 + 1e-15
