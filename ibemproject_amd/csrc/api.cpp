@@ -1221,9 +1221,9 @@ static int residual(ibem3d_ctx* ctx, double* rel) {
   return residual_norm(ctx, rel);
 }
 
-// The stresses of the x just formed, on the second stream while the main stream computes its true
-// residual (the product is HBM-bound, the stress kernel FP64-bound): kept for ibem3d_stress when that
-// residual ends the solve, dropped when another cycle changes x (IBEM3D_TUNE_STRESS_BESIDE).
+// The stresses of the x just formed, on the second stream while the main stream checks its residual
+// (from the cycle's products, or by a product): kept for ibem3d_stress when that residual ends the
+// solve, dropped when another cycle changes x (IBEM3D_TUNE_STRESS_BESIDE).
 static int stress_alloc(ibem3d_ctx* ctx);
 static void enqueue_stress(ibem3d_ctx* ctx, hipStream_t st);
 static int stress_beside(ibem3d_ctx* ctx) {
@@ -1243,10 +1243,12 @@ static int stress_beside(ibem3d_ctx* ctx) {
 
 // GMRES-IR (Carson and Higham): FP64 GMRES on A d = r, right-preconditioned by the FP32 factors of
 // the equilibrated system, M^-1 v = C LU32^-1 (R v); restarted every kGmres vectors, each cycle ending
-// with the true FP64 residual b - A x. Arnoldi by classical Gram-Schmidt applied twice; the small
-// Hessenberg least-squares problem by Givens rotations on the host. iters counts preconditioner
-// applications (LU32 solves). Each cycle costs one FP64 matrix-vector product more than the same
-// number of classic refinement steps, so the solve uses it when classic refinement stalls.
+// with the FP64 residual b - A x of its new x: r - (A Z) y from the cycle's own products when that
+// meets tol (kTuneResidualUpdate), else by a product. Arnoldi by classical Gram-Schmidt applied twice;
+// the small Hessenberg least-squares problem by Givens rotations on the host. iters counts
+// preconditioner applications (LU32 solves). A cycle that does not converge costs one FP64
+// matrix-vector product more than the same number of classic refinement steps, so the solve uses it
+// when classic refinement stalls.
 // warm: start from the current x, whose residual b - A x is in d_res with relative norm rel0.
 static int gmres_ir(ibem3d_ctx* ctx, double tol, bool warm, double rel0, int max_cycles = 10) {
   const long n = ctx->n;

@@ -210,7 +210,7 @@ void lu_solve_16(const unsigned short* F, long n, long ldf, const float* isc, co
 // kTuneBjOverlap: 1 (default) runs the boundary block's triangular solves of each inner iteration on
 // the second stream beside the FP16 product's sphere columns; 0 runs them before the product.
 // kTuneStressBeside: 1 (default) computes the stresses of each GMRES cycle's new x on the second
-// stream beside its true residual, kept when that residual ends the solve; 0 only on request.
+// stream beside its residual check, kept when that residual ends the solve; 0 only on request.
 // kTuneBjFp32: 1 (default) factors the block-Jacobi boundary block in FP32 when it needs no row
 // interchanges (the partial-pivoting fallback stays FP64); 0 in FP64.
 // kTuneResidualUpdate: 1 (default) checks each GMRES cycle's new x with r - (A Z) y, from the FP64

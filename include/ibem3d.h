@@ -137,7 +137,7 @@ enum {
                                         stream while the FP16 product covers the columns past the
                                         boundary block; 0: before the product */
   IBEM3D_TUNE_STRESS_BESIDE = 8,     /* 1 (default): each GMRES cycle computes the stresses of its new
-                                        solution on a second stream beside its true FP64 residual;
+                                        solution on a second stream beside its FP64 residual check;
                                         ibem3d_stress returns them when that residual ended the solve
                                         (bit-identical to computing them afterwards); 0: ibem3d_stress
                                         computes them */
