@@ -2868,7 +2868,9 @@ __global__ __launch_bounds__(256) void k_m16_build(const double* __restrict__ A,
                                                    float* __restrict__ scale) {
   __shared__ double wmax[4];
   __shared__ unsigned tp[4][kM16R];
-  const long rt = blockIdx.x, ct = blockIdx.y;
+  // grid: x = column tile, y = row tile (consecutive workgroups take neighbouring 32-column blocks of
+  // the same rows: measured 50.2 ms against 54.7 with x = row tile at 160,374 unknowns)
+  const long rt = blockIdx.y, ct = blockIdx.x;
   const long i = rt * kM16R + 2 * (threadIdx.x & 127);
   const int c0 = threadIdx.x >> 7;  // this thread's columns: c0, c0 + 2, ...
   const long j0 = ct * kM16C;
@@ -2901,7 +2903,7 @@ __global__ __launch_bounds__(256) void k_m16_build(const double* __restrict__ A,
   if (mx > 0) frexp(mx, &ex);  // mx = f 2^ex, f in [0.5, 1)
   const int e = mx > 0 ? ex - 14 : 0;
   const double down = ldexp(1.0, -e);
-  if (threadIdx.x == 0) scale[ct * gridDim.x + rt] = ldexpf(1.0f, e);
+  if (threadIdx.x == 0) scale[ct * gridDim.y + rt] = ldexpf(1.0f, e);
   typedef _Float16 half2v __attribute__((ext_vector_type(2)));
   unsigned pk[kM16C / 2];  // the two FP16 values of column c0 + 2 u
 #pragma unroll
@@ -2922,8 +2924,11 @@ __global__ __launch_bounds__(256) void k_m16_build(const double* __restrict__ A,
       for (int k = 0; k < 4; ++k) tp[k][(threadIdx.x & 127) + 128 * c0] = pk[4 * g + k];
       __syncthreads();
       const unsigned* src = &tp[q][128 * cc + 4 * oct];
-      const uint4 w4 = make_uint4(src[0], src[1], src[2], src[3]);
-      *(uint4*)(A16 + (j0 + cc + 2 * (4 * g + q)) * ld16 + rt * kM16R + 8 * oct) = w4;
+      // nontemporal: the copy is read back only after all of it is written (46.4 ms with these
+      // stores and the grid above, 50.2 with ordinary stores)
+      typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+      const u32x4 w4 = {src[0], src[1], src[2], src[3]};
+      __builtin_nontemporal_store(w4, (u32x4*)(A16 + (j0 + cc + 2 * (4 * g + q)) * ld16 + rt * kM16R + 8 * oct));
     }
   } else {
 #pragma unroll
@@ -2935,7 +2940,7 @@ __global__ __launch_bounds__(256) void k_m16_build(const double* __restrict__ A,
 }
 
 void m16_build(const double* A, long n, long lda, unsigned short* A16, long ld16, float* scale, hipStream_t st) {
-  hipLaunchKernelGGL(k_m16_build, dim3((unsigned)(ld16 / kM16R), (unsigned)((n + kM16C - 1) / kM16C)), dim3(256), 0,
+  hipLaunchKernelGGL(k_m16_build, dim3((unsigned)((n + kM16C - 1) / kM16C), (unsigned)(ld16 / kM16R)), dim3(256), 0,
                      st, A, n, n, lda, (_Float16*)A16, ld16, scale);
 }
 
