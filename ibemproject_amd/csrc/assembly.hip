@@ -59,7 +59,7 @@ __global__ __launch_bounds__(64) void k_qpoints(const double* __restrict__ nodes
 constexpr int kInclThreads = 256;
 constexpr int kInclU = kInclTile * kInclW;                              // weighted values per workgroup
 constexpr int kInclPer = (kInclU + kInclThreads - 1) / kInclThreads;  // per thread
-static_assert(4 * kTabN <= kInclW, "a pair's basis tables fit in its weighted-table slot");
+static_assert(4 * kTabN + kRadF <= kInclW, "a pair's basis tables and radial factors fit in its weighted-table slot");
 static_assert(kInclTile * 64 <= kInclThreads, "one wave per target sphere in the entry phase");
 
 __global__ __launch_bounds__(kInclThreads, 4) void k_incl(const double* __restrict__ cen,
@@ -84,18 +84,25 @@ __global__ __launch_bounds__(kInclThreads, 4) void k_incl(const double* __restri
     for (int f = 0; f < 8; ++f)
 #pragma unroll
       for (int b = 0; b < 4; ++b) sC[t][f][b] = C[f][b];
+    // the pair's radial factors and coordinate powers, after its basis tables (outside pairs)
+    rad_factors(x, sU + t * kInclW + 4 * kTabN);
   }
+  __syncthreads();
   for (int u = t; u < kInclTile * kTabN; u += kInclThreads) {
     const int ps = u / kTabN, tt = u % kTabN;
     const int s = s0 + ps;
     if (s >= num) continue;
     const double x[3] = {cen[3 * s] - ch0, cen[3 * s + 1] - ch1, cen[3 * s + 2] - ch2};
-    RadPoint P;
-    rad_point(P, x);
     int b0, b1, b2;
     tdecode(tt, b0, b1, b2);
-    double B0, B1, B2, B3;
-    rad_basis(P, is_inside(x, a), b0, b1, b2, B0, B1, B2, B3);
+    double B0, B1, B2 = 0.0, B3 = 0.0;
+    if (is_inside(x, a)) {
+      RadPoint P;
+      rad_point(P, x);
+      rad_basis(P, true, b0, b1, b2, B0, B1, B2, B3);
+    } else {
+      rad_basis_out(sU + ps * kInclW + 4 * kTabN, b0, b1, b2, B0, B1);
+    }
     double* basis = sU + ps * kInclW;
     basis[tt] = B0;
     basis[kTabN + tt] = B1;

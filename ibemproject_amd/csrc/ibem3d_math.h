@@ -191,6 +191,43 @@ IBEM_HD void rad_basis(const RadPoint& P, bool inside, int b0, int b1, int b2, d
   }
 }
 
+// The outside basis (d^beta r, d^beta r^-1) from per-point factors F (kRadF doubles): F[m] =
+// 2^m f^(m)(rho) for f = rho^(1/2) and F[9 + m] for f = rho^(-1/2) (m <= 8), F[18 + 9 c + p] = x_c^p
+// (p <= 8); the same pair sums as rad_basis with each term's radial factor applied as it is summed.
+constexpr int kRadF = 45;
+IBEM_HD void rad_factors(const double* x, double* F) {
+  const double rho = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+  const double r = sqrt(rho), inv2 = 2.0 / rho;
+  F[0] = r;
+  F[9] = 1.0 / r;
+  for (int m = 1; m <= 8; ++m) {
+    F[m] = F[m - 1] * (0.5 - (m - 1)) * inv2;
+    F[9 + m] = F[9 + m - 1] * (-0.5 - (m - 1)) * inv2;
+  }
+  for (int c = 0; c < 3; ++c) {
+    F[18 + 9 * c] = 1.0;
+    for (int p = 1; p <= 8; ++p) F[18 + 9 * c + p] = F[18 + 9 * c + p - 1] * x[c];
+  }
+}
+IBEM_HD void rad_basis_out(const double* F, int b0, int b1, int b2, double& B0, double& B1) {
+  const int k = b0 + b1 + b2;
+  double sp = 0.0, sm = 0.0;
+  for (int j0 = 0; 2 * j0 <= b0; ++j0) {
+    const double c0 = pair_count(b0, j0) * F[18 + b0 - 2 * j0];
+    for (int j1 = 0; 2 * j1 <= b1; ++j1) {
+      const double c1 = c0 * pair_count(b1, j1) * F[27 + b1 - 2 * j1];
+      for (int j2 = 0; 2 * j2 <= b2; ++j2) {
+        const double v = c1 * pair_count(b2, j2) * F[36 + b2 - 2 * j2];
+        const int m = k - (j0 + j1 + j2);
+        sp += v * F[m];
+        sm += v * F[9 + m];
+      }
+    }
+  }
+  B0 = sp;
+  B1 = sm;
+}
+
 // Potential table slots
 enum { kF0 = 0, kF1 = 1, kF2a = 2, kF2b = 3, kG0 = 4, kG1 = 5, kG2a = 6, kG2b = 7 };
 
