@@ -156,7 +156,22 @@ __global__ __launch_bounds__(192) void k_field(const double* __restrict__ nodes,
   const bool inside = is_inside(x, a);
   double C[8][4];
   pot_coefs(a, inside, C);
-  for (int u = i; u < kFieldMi; u += 3) field_tab_fill(P, inside, C, u, stab[jl]);
+  if (inside) {
+    for (int u = i; u < kFieldMi; u += 3) field_tab_fill(P, inside, C, u, stab[jl]);
+  } else if (i == 0) {
+    // outside (every node of a valid RVE): one thread per node forms all 55 multi-indices' d r and
+    // d r^-1 in straight-line code and writes the tables' entries (no lane-dependent loops)
+    double y0[7], y1[7], y2[7], PR[7], PT[6];
+    kelvin_point<5>(x, y0, y1, y2, PR, PT);
+    double* tab = stab[jl];
+    auto out = [&](int b0, int b1, int b2, double dr, double dri) {
+      const int o = b0 + b1 + b2, idx = mi_index(b0, b1, b2);
+#pragma unroll
+      for (int f = 0; f < 8; ++f)
+        if (field_tab_order(f) == o) tab[field_tab_off(f) + idx] = C[f][0] * dr + C[f][1] * dri;
+    };
+    field_basis(y0, y1, y2, PR, PT, out);
+  }
   __syncthreads();
   if (j >= NN) return;
   const long row = 3L * j + i;

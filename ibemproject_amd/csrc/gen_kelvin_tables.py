@@ -131,6 +131,22 @@ def main():
         lines.append("  d = " + code.split(" = ", 1)[1])
         lines.append("  " + " ".join(f"e[{ij}] += d * V[{144 * ij + s}];" for ij in range(6)))
     lines.append("}")
+    # the field kernel's outside basis: d^beta r and d^beta r^-1 for 1 <= |beta| <= 5, each pair passed
+    # to out(b0, b1, b2, d r, d r^-1) as soon as it is formed (out resolves the table slots)
+    lines += [
+        "// field_basis(y0, y1, y2, PR, PT, out): out(b0, b1, b2, d^beta r, d^beta r^-1) for every",
+        "// 1 <= |beta| <= 5; PR[m], PT[m] as above with m <= 5.",
+        "template <class Out>",
+        "IBEM_HD void field_basis(const double* y0, const double* y1, const double* y2, const double* PR,",
+        "                         const double* PT, Out& out) {",
+    ]
+    for d in range(1, 6):
+        for b in multi(d):
+            er = entry_code("S", b)[0].split(" = ", 1)[1].rstrip(";")
+            et = entry_code("T", b)[0].split(" = ", 1)[1].rstrip(";")
+            lines.append(f"  out({b[0]}, {b[1]}, {b[2]}, {er},")
+            lines.append(f"      {et});")
+    lines.append("}")
     with open(os.path.join(HERE, "kelvin_tables.inc"), "w") as f:
         f.write("\n".join(lines) + "\n")
     print("groups:", load)
