@@ -9,7 +9,7 @@
 // sum with lane = point over a chunk of spheres (the moments are wave-uniform, read through the
 // scalar cache) and marks the spheres that contain the point; those few go through the tables in
 // k_stress as before.
-// k_stress: one workgroup per point (6 waves). Boundary term: wave v takes elements v, v+6, ...;
+// k_stress: one workgroup per point (8 waves). Boundary term: wave v takes elements v, v+8, ...;
 // lane = Gauss point; each lane builds the Kelvin tables it needs (d r of orders 3 and 4, d r^-1 of
 // orders 1 and 2) in registers with generated straight-line code. Inclusion term: the chunk sums of
 // k_stress_incl, plus the containing spheres four at a time, their eight potential tables built in
@@ -23,9 +23,9 @@
 
 namespace ibem {
 
-constexpr int kPostThreads = 384;
+constexpr int kPostThreads = 512;  // 8 waves: the boundary term's element loop; waves 0-5 = strain pairs
 constexpr int kPostTab = 84;  // |beta| <= 6 for the D_4x/5x/6x tables
-constexpr int kPostSph = kPostThreads / kPostTab;  // spheres per round of the inclusion term (4)
+constexpr int kPostSph = kPostThreads / kPostTab;  // containing spheres per round of the inclusion term (6)
 constexpr int kMomR = 64, kMomT = 80, kMom = kMomR + kMomT;  // moments per strain pair
 constexpr int kInclWaves = 4096;  // target waves of k_stress_incl (points x sphere chunks)
 
@@ -173,7 +173,8 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
   const int ij = tid / 64, c = tid % 64;  // thread -> (strain pair, unknown)
   const int slot = tid / kPostTab, u = tid % kPostTab;
   double ein = 0.0;
-  for (int k = c; k < chunks; k += 64) ein += part[((size_t)p * 6 + ij) * chunks + k];
+  if (ij < 6)
+    for (int k = c; k < chunks; k += 64) ein += part[((size_t)p * 6 + ij) * chunks + k];
   __shared__ int shs[kPostSph];
   __shared__ int snh;
   int inside_h = -1;
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
       for (int f = 0; f < 8; ++f) ptab[(slot * 8 + f) * kPostTab + u] = v[f];
     }
     __syncthreads();
-    if (c < 60) {
+    if (c < 60 && ij < 6) {
       // strain e_ij += D(i,j | m,n | w) map_mn (x2 for merged quadratic unknowns) X_c
       const Slot Cc = slot60(c);
       const double f = mapw(Cc.i, Cc.j) * ((Cc.order == 2 && Cc.p != Cc.q) ? 2.0 : 1.0);
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(kPostThreads) void k_stress(
     __syncthreads();
   }
   __shared__ double estr[6];
-  if (lane == 0) estr[wave] = red[tid][0];
+  if (lane == 0 && wave < 6) estr[wave] = red[tid][0];
   __syncthreads();
   // boundary term: reduce each wave's 64 lanes, then thread 0 adds the six wave sums
   for (int l = 0; l < 6; ++l) red[tid][l] = acc[l];
